@@ -1,0 +1,347 @@
+"""Seeded synthetic workloads: the five benchmark scenarios as engine configs.
+
+Each scenario is expressed in the engine's own config format (the same JSON keys the
+reference engine reads, ``engine/src/io/Config.cpp:324-346``) with the terrain as an integer
+CSV heightmap (``terrain.type = "file"``, loader ``engine/src/terrain/HeightmapLoader.cpp:37-85``)
+and explicit crater columns (``vents.placement = "positions"``,
+``engine/src/terrain/VentPlacer.cpp:60-81``).  The reference's own cone generator
+(``engine/src/terrain/TerrainGenerator.cpp:53-84``) cannot express a summit bowl, a caldera
+or 4/6-octave noise, so the shapes are produced here, deterministically from a seed, and
+written out as files.  Heights are integers, so the loader's ``ceil`` is the identity.
+
+Nothing here runs a simulation; it only builds inputs.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+_M64 = (1 << 64) - 1
+
+
+def splitmix64(x: int) -> int:
+    """SplitMix64 finaliser (Steele, Lea, Flood 2014) on a Python int."""
+    x = (x + 0x9E3779B97F4A7C15) & _M64
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & _M64
+    return x ^ (x >> 31)
+
+
+def _splitmix64_np(x: np.ndarray) -> np.ndarray:
+    x = x.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+class Stream:
+    """Small deterministic generator (SplitMix64 counter stream) for workload draws."""
+
+    def __init__(self, seed: int):
+        self.state = splitmix64(seed ^ 0x5752_4B4C_4F41_4453)
+
+    def u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _M64
+        return splitmix64(self.state)
+
+    def uniform(self) -> float:
+        return (self.u64() >> 11) * (1.0 / 9007199254740992.0)
+
+    def below(self, n: int) -> int:
+        return self.u64() % n
+
+    def log_uniform_int(self, lo: int, hi: int) -> int:
+        v = math.exp(math.log(lo) + self.uniform() * (math.log(hi) - math.log(lo)))
+        return int(min(hi, max(lo, round(v))))
+
+
+def value_noise(nx: int, ny: int, seed: int, octaves: int, base_spacing: int) -> np.ndarray:
+    """Multi-octave lattice value noise in [-1, 1] on an (ny, nx) grid.
+
+    Octave ``o`` has lattice spacing ``base_spacing / 2**o`` (at least 2) and weight ``2**-o``;
+    lattice values come from a SplitMix64 hash of (seed, octave, i, j); bilinear interpolation
+    with smoothstep weights.  The sum is normalised by the total weight.
+    """
+    xs = np.arange(nx, dtype=np.int64)
+    ys = np.arange(ny, dtype=np.int64)
+    total = np.zeros((ny, nx), dtype=np.float64)
+    weight_sum = 0.0
+    for o in range(octaves):
+        sp = max(2, base_spacing >> o)
+        w = 0.5 ** o
+        ix, fx = xs // sp, (xs % sp) / sp
+        iy, fy = ys // sp, (ys % sp) / sp
+        sx = fx * fx * (3.0 - 2.0 * fx)
+        sy = fy * fy * (3.0 - 2.0 * fy)
+        gi = np.arange(ix.max() + 2, dtype=np.uint64)
+        gj = np.arange(iy.max() + 2, dtype=np.uint64)
+        key = (gi[None, :] | (gj[:, None] << np.uint64(32))) ^ np.uint64(
+            splitmix64((seed * 0x100000001B3 + o) & _M64))
+        lat = (_splitmix64_np(key) >> np.uint64(11)).astype(np.float64) * (2.0 / 9007199254740992.0) - 1.0
+        v00 = lat[iy][:, ix]
+        v10 = lat[iy][:, ix + 1]
+        v01 = lat[iy + 1][:, ix]
+        v11 = lat[iy + 1][:, ix + 1]
+        a = v00 + (v10 - v00) * sx[None, :]
+        b = v01 + (v11 - v01) * sx[None, :]
+        total += w * (a + (b - a) * sy[:, None])
+        weight_sum += w
+    return total / weight_sum
+
+
+@dataclass
+class Workload:
+    """One engine run: config dict (reference keys) plus its integer heightmap (ny, nx)."""
+
+    name: str
+    config: dict
+    heights: np.ndarray  # int32, shape (ny, nx)
+
+    @property
+    def dims(self) -> Tuple[int, int, int]:
+        s = self.config["space"]
+        return s["nx"], s["ny"], s["nz"]
+
+    @property
+    def cells(self) -> int:
+        nx, ny, nz = self.dims
+        return nx * ny * nz
+
+    def write(self, folder: str, csv_name: Optional[str] = None) -> str:
+        """Writes ``<name>.json`` and its CSV heightmap into ``folder``; returns the JSON path."""
+        os.makedirs(folder, exist_ok=True)
+        csv_name = csv_name or f"{self.name}.csv"
+        write_heightmap_csv(os.path.join(folder, csv_name), self.heights)
+        cfg = json.loads(json.dumps(self.config))
+        cfg["terrain"]["file"] = csv_name
+        path = os.path.join(folder, f"{self.name}.json")
+        with open(path, "w") as f:
+            json.dump(cfg, f, indent=2)
+        return path
+
+
+def write_heightmap_csv(path: str, heights: np.ndarray) -> None:
+    """One CSV row per y, one integer column per x (the loader's layout)."""
+    h = np.asarray(heights, dtype=np.int64)
+    with open(path, "w") as f:
+        for row in h:
+            f.write(",".join(map(str, row.tolist())))
+            f.write("\n")
+
+
+def _base_config(nx: int, ny: int, nz: int, *, neighborhood: str, boundary: str, seed: int,
+                 max_steps: int, vents: List[Tuple[int, int]], q: int, mcs: int, kt: float,
+                 cooling_ambient: float = 1.0) -> dict:
+    return {
+        "space": {"nx": nx, "ny": ny, "nz": nz},
+        "boundary": boundary,
+        "neighborhood": neighborhood,
+        "rngSeed": seed,
+        "terrain": {"type": "file", "file": None},
+        "vents": {"count": len(vents), "placement": "positions", "positions": [list(v) for v in vents],
+                  "eruptionSteps": int(round(0.6 * max_steps)), "eruptTemp": 1200},
+        "ca": {"mode": "lava", "maxSteps": max_steps, "coolingAmbient": cooling_ambient,
+               "coolingLateral": 1.5, "coolingFall": 0.1, "solidusTemp": 800, "ponding": True,
+               "sweep": "front"},
+        "mc": {"enabled": True, "domain": "basalt", "Q": q, "mcs": mcs, "J": 1.0, "kT": kt,
+               "mobilityFromCooling": True, "mobilityMin": 0.05, "statsEvery": 10, "update": "checkerboard"},
+        "output": {"format": "bin"},
+    }
+
+
+def _clamp_heights(h: np.ndarray, nz: int) -> np.ndarray:
+    return np.clip(np.rint(h), 0, nz).astype(np.int32)
+
+
+# ---------------------------------------------------------------------------------------------
+# The five scenarios
+# ---------------------------------------------------------------------------------------------
+
+def cross_section_2d(seed: int = 42, nx: int = 1024, nz: int = 512, steps: int = 400, mcs: int = 100,
+                     name: str = "cross_section_2d") -> Workload:
+    """2D cone cross-section (x-z plane): h(x) = 400 - 0.6|x - nx/2|, a flat summit bowl
+    60 cells wide and 40 deep, uniform integer noise in [-2, 2] per column; one crater at
+    the bowl floor; von Neumann, absorbing, Q 64, kT 0.5."""
+    cx = nx // 2
+    x = np.arange(nx, dtype=np.float64)
+    peak = 400.0 * nz / 512.0
+    slope = 0.6 * peak / 400.0
+    h = peak - slope * np.abs(x - cx)
+    rim = peak - slope * 30
+    h = np.where(np.abs(x - cx) < 30, rim - 40.0 * nz / 512.0, h)
+    noise = (_splitmix64_np(np.arange(nx, dtype=np.uint64) ^ np.uint64(splitmix64(seed))) % np.uint64(5)
+             ).astype(np.int64) - 2
+    heights = _clamp_heights(h + noise, nz - 1)[None, :]
+    cfg = _base_config(nx, 1, nz, neighborhood="von_neumann", boundary="absorbing", seed=seed,
+                       max_steps=steps, vents=[(cx, 0)], q=64, mcs=mcs, kt=0.5)
+    return Workload(name, cfg, heights)
+
+
+def cone_3d(seed: int = 42, n: int = 512, nz: int = 256, steps: int = 1000, mcs: int = 200,
+            neighborhood: str = "moore", name: str = "cone_3d", craters: int = 3) -> Workload:
+    """3D radial cone: height 220 at the centre falling linearly to 20 at radius 250 (flat
+    beyond), plus 4-octave value noise of amplitude 12; three craters at seeded columns within
+    radius 30 of the summit; Moore, absorbing, Q 64, kT 0.5."""
+    s = n / 512.0
+    zs = nz / 256.0
+    yy, xx = np.mgrid[0:n, 0:n].astype(np.float64)
+    c = (n - 1) / 2.0
+    r = np.hypot(xx - c, yy - c)
+    h = 20.0 * zs + (200.0 * zs) * np.clip(1.0 - r / (250.0 * s), 0.0, 1.0)
+    h += 12.0 * zs * value_noise(n, n, seed, 4, max(4, int(32 * s)))
+    heights = _clamp_heights(h, nz - 1)
+    st = Stream(seed ^ 0xC0DE)
+    vents: List[Tuple[int, int]] = []
+    while len(vents) < craters:
+        ang = 2 * math.pi * st.uniform()
+        rad = 30.0 * s * math.sqrt(st.uniform())
+        v = (int(round(c + rad * math.cos(ang))), int(round(c + rad * math.sin(ang))))
+        if v not in vents and 0 <= v[0] < n and 0 <= v[1] < n:
+            vents.append(v)
+    cfg = _base_config(n, n, nz, neighborhood=neighborhood, boundary="absorbing", seed=seed,
+                       max_steps=steps, vents=vents, q=64, mcs=mcs, kt=0.5)
+    return Workload(name, cfg, heights)
+
+
+def caldera_3d(seed: int = 42, n: int = 1024, nz: int = 384, steps: int = 2000, mcs: int = 100,
+               name: str = "caldera_3d", craters: int = 8, neighborhood: str = "pentagonal",
+               boundary: str = "periodic", q: int = 128) -> Workload:
+    """3D caldera: a double cone (outer flank up to the rim, inner ring depression of depth 60
+    at radius 150) plus 6-octave value noise of amplitude 20; eight craters on the ring;
+    random pentagonal neighbourhood, periodic x/y (z is never periodic in lava mode), Q 128."""
+    s = n / 1024.0
+    zs = nz / 384.0
+    yy, xx = np.mgrid[0:n, 0:n].astype(np.float64)
+    c = (n - 1) / 2.0
+    r = np.hypot(xx - c, yy - c)
+    outer = 20.0 * zs + 280.0 * zs * np.clip(1.0 - r / (500.0 * s), 0.0, 1.0)
+    ring = 60.0 * zs * np.clip(1.0 - np.abs(r - 150.0 * s) / (40.0 * s), 0.0, 1.0)
+    h = outer - ring + 20.0 * zs * value_noise(n, n, seed, 6, max(4, int(64 * s)))
+    heights = _clamp_heights(h, nz - 1)
+    st = Stream(seed ^ 0xCA1D)
+    vents: List[Tuple[int, int]] = []
+    k = 0
+    while len(vents) < craters:
+        ang = 2 * math.pi * (k + 0.5 * st.uniform()) / craters
+        k += 1
+        v = (int(round(c + 150.0 * s * math.cos(ang))), int(round(c + 150.0 * s * math.sin(ang))))
+        if v not in vents:
+            vents.append(v)
+    cfg = _base_config(n, n, nz, neighborhood=neighborhood, boundary=boundary, seed=seed,
+                       max_steps=steps, vents=vents, q=q, mcs=mcs, kt=0.5)
+    return Workload(name, cfg, heights)
+
+
+_NEIGHBORHOODS = ["von_neumann", "moore", "pentagonal", "hexagonal"]
+
+
+def random_job(st: Stream, job_seed: int, name: str, *, dims: Tuple[int, int, int],
+               steps: int, mcs: int, q: int = 64, craters: Optional[int] = None,
+               neighborhood: Optional[str] = None) -> Workload:
+    """One job of the batched / serving mixes: cone or caldera plus noise, 1-4 craters,
+    neighbourhood uniform over von Neumann / Moore / pentagonal / hexagonal (hexagonal only in 2D)."""
+    nx, ny, nz = dims
+    is2d = ny == 1
+    if neighborhood is None:
+        choices = _NEIGHBORHOODS if is2d else _NEIGHBORHOODS[:3]
+        neighborhood = choices[st.below(len(choices))]
+    craters = craters if craters is not None else 1 + st.below(4)
+    caldera = st.below(2) == 1
+    c_x = (nx - 1) / 2.0
+    c_y = (ny - 1) / 2.0
+    if is2d:
+        x = np.arange(nx, dtype=np.float64)
+        r = np.abs(x - c_x)[None, :]
+        noise = value_noise(nx, 1, job_seed, 3, 16)
+    else:
+        yy, xx = np.mgrid[0:ny, 0:nx].astype(np.float64)
+        r = np.hypot(xx - c_x, yy - c_y)
+        noise = value_noise(nx, ny, job_seed, 4, 16)
+    R = 0.5 * min(nx, ny if not is2d else nx)
+    peak = 0.75 * nz
+    h = 0.05 * nz + peak * np.clip(1.0 - r / R, 0.0, 1.0)
+    if caldera:
+        h -= 0.15 * nz * np.clip(1.0 - np.abs(r - 0.3 * R) / (0.1 * R + 1.0), 0.0, 1.0)
+    h += 0.04 * nz * noise
+    heights = _clamp_heights(h, nz - 1)
+    vents: List[Tuple[int, int]] = []
+    rad_max = max(1.0, (0.3 if caldera else 0.08) * R)
+    guard = 0
+    while len(vents) < craters and guard < 1000:
+        guard += 1
+        ang = 2 * math.pi * st.uniform()
+        rad = rad_max if caldera else rad_max * math.sqrt(st.uniform())
+        vx = int(round(c_x + rad * math.cos(ang)))
+        vy = 0 if is2d else int(round(c_y + rad * math.sin(ang)))
+        v = (min(nx - 1, max(0, vx)), min(ny - 1, max(0, vy)))
+        if v not in vents:
+            vents.append(v)
+    cfg = _base_config(nx, ny, nz, neighborhood=neighborhood, boundary="absorbing", seed=job_seed,
+                       max_steps=steps, vents=vents, q=q, mcs=mcs, kt=0.5)
+    return Workload(name, cfg, heights)
+
+
+def batched_mix(seed: int = 42, jobs: int = 256) -> List[Workload]:
+    """256 uneven independent jobs from one seed: half 2D (edges log-uniform 128-2048), half 3D
+    (edges log-uniform 64-256), 200-800 CA steps, then 50 Potts sweeps (Q 64)."""
+    st = Stream(seed ^ 0xBA7C)
+    out = []
+    for j in range(jobs):
+        js = st.u64() & 0x7FFFFFFFFFFFFFFF
+        if j % 2 == 0:
+            nx = st.log_uniform_int(128, 2048)
+            nz = st.log_uniform_int(128, 2048)
+            dims = (nx, 1, max(32, nz // 2))
+        else:
+            e = st.log_uniform_int(64, 256)
+            dims = (e, st.log_uniform_int(64, 256), st.log_uniform_int(64, 256))
+        steps = 200 + st.below(601)
+        out.append(random_job(st, js, f"job{j:03d}", dims=dims, steps=steps, mcs=50))
+    return out
+
+
+@dataclass
+class Arrival:
+    t: float
+    workload: Workload
+
+
+def serving_stream(seed: int = 42, rate: float = 20.0, duration: float = 600.0,
+                   max_jobs: Optional[int] = None, scale: float = 1.0) -> List[Arrival]:
+    """Poisson arrivals at ``rate`` jobs/s over ``duration`` seconds from a seeded mix:
+    70% 2D 512x256 (300 steps), 25% 3D 256^3 (600 steps), 5% 3D 768x768x256 (1200 steps);
+    each with a Potts phase of 50-200 sweeps.  ``scale`` shrinks the grids for quick runs."""
+    st = Stream(seed ^ 0x5E7F)
+    t = 0.0
+    out: List[Arrival] = []
+    while True:
+        t += -math.log(1.0 - st.uniform()) / rate
+        if t >= duration or (max_jobs is not None and len(out) >= max_jobs):
+            break
+        u = st.uniform()
+        js = st.u64() & 0x7FFFFFFFFFFFFFFF
+        mcs = 50 + st.below(151)
+        if u < 0.70:
+            dims, steps = (max(16, int(512 * scale)), 1, max(16, int(256 * scale))), 300
+        elif u < 0.95:
+            e = max(16, int(256 * scale))
+            dims, steps = (e, e, e), 600
+        else:
+            e = max(16, int(768 * scale))
+            dims, steps = (e, e, max(16, int(256 * scale))), 1200
+        out.append(Arrival(t, random_job(st, js, f"req{len(out):05d}", dims=dims, steps=steps, mcs=mcs)))
+    return out
+
+
+SCENARIOS = {
+    "cross_section_2d": cross_section_2d,
+    "cone_3d": cone_3d,
+    "caldera_3d": caldera_3d,
+}
