@@ -1,0 +1,174 @@
+/* vca.h — C ABI of the MI355X lava-CA / Potts-MC engine (libvca.so).
+ *
+ * The reference engine has no FFI layer: its hot paths sit behind two C++ class templates,
+ * LavaCA<Storage> (engine/src/ca/LavaCA.hpp:80-296) and PottsMC (engine/src/mc/PottsMC.hpp:52-268),
+ * called by Simulation::runWith (engine/src/app/Simulation.cpp:158-292) and McStage::run
+ * (engine/src/app/McStage.cpp:93-301).  This header replaces those two classes with opaque handles
+ * whose state lives in GPU memory; each entry point names the member function it stands in for.
+ *
+ * Conventions
+ *   - Every function returns VCA_OK (0) or a negative VCA_ERR_* code; vca_last_error() gives the
+ *     message of the last failure on the calling thread.  No C++ exception crosses this boundary.
+ *   - Host arrays are owned by the caller; device memory is owned by the handle.
+ *   - Arrays are unframed and in global-index order i = x + nx*(y + ny*z) (engine/src/core/Types.hpp:95-100),
+ *     the order of ca_result.bin / mc_result.bin.
+ *   - One handle is driven by one host thread at a time.  Calls on one context are stream ordered.
+ */
+#ifndef VCA_H
+#define VCA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VCA_OK 0
+#define VCA_ERR_ARG (-1)      /* invalid argument (reference: std::invalid_argument / ConfigError) */
+#define VCA_ERR_HIP (-2)      /* HIP runtime failure */
+#define VCA_ERR_NOMEM (-3)    /* device allocation failed */
+#define VCA_ERR_STATE (-4)    /* call not valid in the handle's state */
+#define VCA_ERR_DEVICE (-5)   /* no usable gfx950 device / library built without it */
+
+/* Cell types (engine/src/core/Types.hpp:13-19); values are part of the file format. */
+#define VCA_AIR 0
+#define VCA_ROCK 1
+#define VCA_LAVA 2
+#define VCA_BASALT 3
+
+/* Neighbourhoods (engine/src/core/Types.hpp:22) */
+#define VCA_VON_NEUMANN 0
+#define VCA_MOORE 1
+#define VCA_PENTAGONAL 2
+#define VCA_HEXAGONAL 3
+
+/* Stop reasons (engine/src/ca/CaRules.hpp:108) */
+#define VCA_RUNNING 0
+#define VCA_NO_LAVA 1
+#define VCA_NO_AIR 2
+#define VCA_NO_CHANGE 3
+#define VCA_MAX_STEPS 4
+
+typedef struct vca_ctx vca_ctx;
+typedef struct vca_ca vca_ca;
+typedef struct vca_mc vca_mc;
+
+/* ---- library / context ------------------------------------------------------------------ */
+int vca_version(void);
+/* Copies the last error message of this thread (NUL-terminated, truncated to cap). */
+int vca_last_error(char* buf, size_t cap);
+/* Context = one device + one HIP stream.  stream == NULL creates a private default-priority stream;
+ * otherwise the caller's hipStream_t is used (not destroyed by vca_ctx_destroy). */
+int vca_ctx_create(int device, void* stream, vca_ctx** out);
+int vca_ctx_destroy(vca_ctx* ctx);
+int vca_ctx_synchronize(vca_ctx* ctx);
+/* Free / total device memory in bytes (admission control of the serving loop). */
+int vca_device_memory(vca_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+
+/* ---- lava / classic CA: replaces LavaCA<Storage> ---------------------------------------- */
+typedef struct {
+    int32_t nx, ny, nz;          /* Dims (Types.hpp:38-72); 2D lava is the x-z plane (ny = 1) */
+    int32_t classic;             /* CaParams::mode: 0 lava, 1 classic grain growth (CaRules.hpp:93-105) */
+    int32_t neighborhood;        /* VCA_VON_NEUMANN .. VCA_HEXAGONAL */
+    int32_t periodic;            /* BoundaryKind: 0 absorbing, 1 periodic (Boundary.cpp:5-11) */
+    uint64_t seed;               /* CaParams::seed */
+    int32_t max_steps;           /* CaParams::maxSteps (<= 65535) */
+    int32_t eruption_steps;      /* CaParams::eruptionSteps */
+    float erupt_temp, solidus_temp, cooling_ambient, cooling_lateral, cooling_fall;
+    int32_t ponding;             /* CaParams::ponding */
+    int32_t dense;               /* 0: visit only tiles near lava (ca.sweep = front); 1: every tile (full) */
+} vca_ca_desc;
+
+typedef struct {
+    int32_t x, y, z, flow_id;    /* Vent (engine/src/terrain/VentPlacer.hpp:18-23) */
+} vca_vent;
+
+/* LavaCA ctor + applyTerrain + placeSeeds (LavaCA.hpp:84-227).  height / spill: nx*ny column
+ * arrays (height = rock cells per column, spill = priority-flood spill level; both NULL in classic
+ * mode).  Counts the initial state (countInitialState, LavaCA.hpp:1252-1280). */
+int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* desc, const int32_t* height, const int32_t* spill,
+                  const vca_vent* vents, int32_t nvents, vca_ca** out);
+int vca_ca_destroy(vca_ca* ca);
+/* LavaCA::step x n (LavaCA.hpp:230-256): enqueues up to max_new_steps synchronous steps; steps after a
+ * stop rule fired are no-ops on the device.  Blocks until done, returns the step count and stop reason. */
+int vca_ca_run(vca_ca* ca, int32_t max_new_steps, int32_t* steps_done, int32_t* stop_reason);
+/* Enqueue only (no host wait); pair with vca_ca_status. */
+int vca_ca_enqueue(vca_ca* ca, int32_t new_steps);
+/* LavaCA::steps / stopReason (LavaCA.hpp:262-264); synchronises the context stream. */
+int vca_ca_status(vca_ca* ca, int32_t* steps_done, int32_t* stop_reason);
+/* activeHistory / changedHistory (LavaCA.hpp:266-270); returns the number of entries in *n. */
+int vca_ca_histories(vca_ca* ca, uint64_t* active, uint64_t* changed, int32_t cap, int32_t* n);
+/* LavaCA::result (LavaCA.hpp:276): any pointer may be NULL. */
+int vca_ca_download(vca_ca* ca, uint8_t* type, int32_t* flow, float* temp, uint16_t* solid, uint16_t* arrival);
+/* Work statistics: tiles visited and changes applied over all steps, tile size in cells. */
+int vca_ca_work(vca_ca* ca, uint64_t* tiles_visited, uint64_t* changes, int32_t* tile_cells);
+
+/* ---- Potts MC: replaces PottsMC ------------------------------------------------------------ */
+typedef struct {
+    int32_t nx, ny, nz;
+    int32_t neighborhood;
+    int32_t periodic[3];         /* effective periodicity per axis (PottsMC.cpp:18) */
+    uint64_t seed;               /* McParams (McRules.hpp:80-85) */
+    int32_t Q;
+    double J, kT;
+    int32_t checkerboard;        /* mc.update: 1 checkerboard (colour passes), 0 random (sequential) */
+} vca_mc_desc;
+
+typedef struct {
+    uint64_t attempts, accepted, uphill;   /* PottsMC::attempts/accepted/acceptedUphill (PottsMC.hpp:104-106) */
+    int32_t sweeps;
+} vca_mc_counts;
+
+typedef struct {
+    int32_t mcs;
+    uint64_t grains, cells, largest;        /* Grains::label (Grains.cpp:33-150) */
+    double mean_grain_size, energy;         /* PottsMC::measure / totalEnergy (PottsMC.cpp:535-664) */
+} vca_mc_stats;
+
+/* Grain counts by mobility class (McStage.cpp:131-150, threshold 0.5). */
+typedef struct {
+    uint64_t grains, grain_cells, largest;
+} vca_mc_class;
+
+/* PottsMC ctor + setDomain/setDomainAll + setMobility + setCheckerboard + initialize
+ * (PottsMC.cpp:14-143).  domain: N bytes (NULL = whole space).  mobility: N floats (NULL = none); it
+ * feeds the by-mobility grain statistics, and the trial skip as well when mobility_dynamics != 0
+ * (McStage.cpp:205-223: mobilityFromCooling).  Mobility and Metropolis probabilities become exact
+ * 53-bit integer thresholds on the host, so the device never evaluates exp. */
+int vca_mc_create(vca_ctx* ctx, const vca_mc_desc* desc, const uint8_t* domain, const float* mobility,
+                  int32_t mobility_dynamics, vca_mc** out);
+/* Same, but the basalt domain and the cooling mobility (CoolingMobility.cpp:8-129) are derived from the
+ * final CA state held by `ca` (same dims), without the caller downloading it. */
+int vca_mc_create_from_ca(vca_ctx* ctx, const vca_mc_desc* desc, vca_ca* ca, int32_t use_mobility,
+                          double mobility_min, vca_mc** out);
+int vca_mc_destroy(vca_mc* mc);
+/* PottsMC::sweep x n (PottsMC.cpp:516-533); counts are cumulative since creation. */
+int vca_mc_sweep(vca_mc* mc, int32_t n, vca_mc_counts* counts);
+int vca_mc_counts_get(vca_mc* mc, vca_mc_counts* counts);
+/* PottsMC::measure (PottsMC.cpp:535-553) on the device.  classes (2 entries: high, low) are filled
+ * when the handle has mobility data and classes != NULL; high_cells gets the number of domain cells
+ * with mobility >= 0.5. */
+int vca_mc_measure(vca_mc* mc, vca_mc_stats* stats, vca_mc_class* classes, uint64_t* high_cells);
+/* PottsMC::crystals (PottsMC.cpp:666-697): orientation per cell, 0 outside the domain. */
+int vca_mc_download(vca_mc* mc, int32_t* crystal);
+/* Grain representative per cell (smallest global index of the grain, 0xFFFFFFFF outside). */
+int vca_mc_download_labels(vca_mc* mc, uint32_t* root);
+int vca_mc_info(vca_mc* mc, int32_t* colors, uint64_t* domain_cells);
+/* Device-side mobility of the handle (NULL when off), for host statistics; N floats. */
+int vca_mc_download_mobility(vca_mc* mc, float* mobility);
+
+/* ---- timing (benchmark support) ---------------------------------------------------------- */
+typedef struct {
+    double ms;          /* summed device time of the kernel */
+    uint64_t launches;  /* launches timed */
+} vca_kernel_time;
+/* Enables per-launch HIP-event timing of the dominant kernels (adds a host sync per step). */
+int vca_ctx_set_timing(vca_ctx* ctx, int32_t on);
+/* name: "ca_rules", "ca_descent", "ca_apply", "ca_front", "mc_sweep" */
+int vca_ctx_kernel_time(vca_ctx* ctx, const char* name, vca_kernel_time* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VCA_H */

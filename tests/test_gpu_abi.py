@@ -1,0 +1,171 @@
+"""GPU parity through the C ABI (libvca.so): the HIP CA step and MC sweep against the oracle.
+
+Each golden case's inputs (terrain, spill levels, seeds) are fed to both the HIP path and the CPU
+oracle; every state array, step count, stop reason and counter history must be identical (bit-exact,
+integer / byte / float32 bits).  The CA result is also compared with the reference engine's own
+ca_result.bin, so the HIP path is checked against the reference directly.
+"""
+import gzip
+import json
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    with open(os.path.join(GOLD, "manifest.json")) as f:
+        return sorted(json.load(f))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from volcano_ca_mc_amd import native
+    c = native.Context(0)
+    yield c
+    c.close()
+
+
+def _inputs(name):
+    d = os.path.join(GOLD, "cases", name)
+    with open(os.path.join(d, "config.json")) as f:
+        cfg = O.merged(json.load(f))
+    return d, cfg
+
+
+def _gpu_ca(ctx, cfg, hm, spill, vents, dense=False):
+    from volcano_ca_mc_amd import native as N
+    p = O.ca_params(cfg)
+    params = N.CaParams(seed=p.seed, classic=bool(p.classic), max_steps=p.max_steps, eruption_steps=p.eruption_steps,
+                        erupt_temp=p.erupt_temp, solidus_temp=p.solidus_temp, cooling_ambient=p.cooling_ambient,
+                        cooling_lateral=p.cooling_lateral, cooling_fall=p.cooling_fall, ponding=bool(p.ponding),
+                        dense=dense)
+    s = cfg["space"]
+    return N.LavaCA(ctx, (s["nx"], s["ny"], s["nz"]), params, cfg["neighborhood"], cfg["boundary"] == "periodic",
+                    hm, spill, vents.tolist())
+
+
+def _ca_cases():
+    return [c for c in _cases() if not c.startswith("mcfull")]
+
+
+@pytest.mark.parametrize("name", _ca_cases())
+def test_ca_matches_oracle_and_reference(ctx, name):
+    d, cfg = _inputs(name)
+    hm = spill = None
+    if cfg["ca"]["mode"] == "lava":
+        hm = O.terrain_for(cfg, d)
+        per = cfg["boundary"] == "periodic"
+        spill = O.fill_depressions(hm, per, per)
+    vents = O.vents_for(cfg, hm)
+    print(f"case {name}: dims {cfg['space']}", flush=True)
+    orc = O.OracleCA(O.ca_params(cfg), hm, spill, vents)
+    orc.run()
+    ref = orc.state()
+    for dense in (False, True):
+        gpu = _gpu_ca(ctx, cfg, hm, spill, vents, dense=dense)
+        steps, stop = gpu.run()
+        assert (steps, stop) == (orc.steps, orc.stop_reason)
+        got = gpu.result()
+        for k in ("type", "flow", "solid", "arrival"):
+            assert np.array_equal(got[k], ref[k]), k
+        assert np.array_equal(got["temp"].view(np.uint32), ref["temp"].view(np.uint32))
+        assert gpu.histories() == orc.histories()
+        gpu.close()
+    # and directly against the reference engine's file
+    blob = gzip.decompress(open(os.path.join(d, "ca_result.bin.gz"), "rb").read())
+    assert O.ca_result_bin(tuple(cfg["space"][k] for k in ("nx", "ny", "nz")), got) == blob
+
+
+def _mc_cases():
+    return [c for c in _cases() if "classic" not in c]
+
+
+@pytest.mark.parametrize("name", _mc_cases())
+def test_mc_matches_oracle(ctx, name):
+    import ctypes as Cc
+    from volcano_ca_mc_amd import native as N
+    d, cfg = _inputs(name)
+    if not cfg["mc"]["enabled"]:
+        pytest.skip("no MC phase")
+    s = cfg["space"]
+    dims = (s["nx"], s["ny"], s["nz"])
+    mc = cfg["mc"]
+    full = mc["domain"] == "full"
+    periodic = cfg["boundary"] == "periodic"
+    per = [periodic and dims[0] > 1, periodic and dims[1] > 1, periodic and full and dims[2] > 1]
+    domain = mobility = None
+    if not full:
+        blob = gzip.decompress(open(os.path.join(d, "ca_result.bin.gz"), "rb").read())
+        n = dims[0] * dims[1] * dims[2]
+        off = 20
+        t = np.frombuffer(blob, np.uint8, n, off); off += n
+        off += 8 * n
+        sol = np.frombuffer(blob, np.uint16, n, off); off += 2 * n
+        arr = np.frombuffer(blob, np.uint16, n, off)
+        domain = np.empty(n, np.uint8)
+        mobility = np.empty(n, np.float32)
+        O.lib().orc_mobility_from_cooling(np.ascontiguousarray(t).ctypes.data, np.ascontiguousarray(sol).ctypes.data,
+                                          np.ascontiguousarray(arr).ctypes.data, n, float(mc["mobilityMin"]),
+                                          domain.ctypes.data, mobility.ctypes.data)
+    dyn = bool(mc["mobilityFromCooling"]) and not full
+    print(f"case {name}: dims {dims} Q {mc['Q']} update {mc['update']}", flush=True)
+    params = O.McParams(*dims, O.NB_KINDS[cfg["neighborhood"]], (Cc.c_int * 3)(*map(int, per)), int(cfg["rngSeed"]),
+                        int(mc["Q"]), float(mc["J"]), float(mc["kT"]), int(mc["update"] == "checkerboard"))
+    orc = O.OracleMC(params, domain, mobility if dyn else None)
+    gpu = N.PottsMC(ctx, dims, cfg["neighborhood"], per, int(cfg["rngSeed"]), int(mc["Q"]), float(mc["J"]),
+                    float(mc["kT"]), mc["update"] == "checkerboard", domain, mobility, mobility_dynamics=dyn)
+    assert gpu.info() == (orc.colors, orc.domain_cells)
+    for k in range(int(mc["mcs"]) + 1):
+        if k > 0:
+            orc.sweep()
+            gpu.sweep(1)
+            assert gpu.counters() == orc.counters(), k
+        if k % 3 == 0 or k == int(mc["mcs"]):
+            so, root = orc.measure(want_root=True)
+            sg = gpu.measure()
+            assert (sg["grains"], sg["cells"], sg["largest"]) == (so.grains, so.cells, so.largest), k
+            assert sg["energy"] == so.energy
+            assert sg["mean_grain_size"] == so.mean_grain_size
+            assert np.array_equal(gpu.labels(), root)
+    assert np.array_equal(gpu.crystals(), orc.spins())
+    # and the reference's mc_result.bin
+    p = os.path.join(d, "mc_result.bin.gz")
+    if os.path.exists(p):
+        blob = gzip.decompress(open(p, "rb").read())
+        assert O.mc_result_bin(dims, gpu.crystals()) == blob
+
+
+def test_mc_from_ca_matches_host_domain(ctx):
+    """vca_mc_create_from_ca derives the basalt domain and cooling mobility from the device CA state."""
+    from volcano_ca_mc_amd import native as N
+    d, cfg = _inputs("lava3d_moore_periodic")
+    hm = O.terrain_for(cfg, d)
+    spill = O.fill_depressions(hm, True, True)
+    vents = O.vents_for(cfg, hm)
+    gpu_ca = _gpu_ca(ctx, cfg, hm, spill, vents)
+    gpu_ca.run()
+    st = gpu_ca.result()
+    dims = gpu_ca.dims
+    mc = cfg["mc"]
+    per = [True, True, False]
+    a = N.PottsMC.from_ca(ctx, gpu_ca, cfg["neighborhood"], per, int(cfg["rngSeed"]), mc["Q"], mc["J"], mc["kT"],
+                          True, True, mc["mobilityMin"])
+    n = st["type"].size
+    dom = np.empty(n, np.uint8)
+    mob = np.empty(n, np.float32)
+    O.lib().orc_mobility_from_cooling(st["type"].ctypes.data, st["solid"].ctypes.data, st["arrival"].ctypes.data, n,
+                                      float(mc["mobilityMin"]), dom.ctypes.data, mob.ctypes.data)
+    b = N.PottsMC(ctx, dims, cfg["neighborhood"], per, int(cfg["rngSeed"]), mc["Q"], mc["J"], mc["kT"], True, dom,
+                  mob, True)
+    a.sweep(5)
+    b.sweep(5)
+    assert np.array_equal(a.crystals(), b.crystals())
+    assert a.counters() == b.counters()
+    assert a.measure() == b.measure()

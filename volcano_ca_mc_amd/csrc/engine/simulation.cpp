@@ -1,0 +1,800 @@
+#include "simulation.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <filesystem>
+#include <fstream>
+#include <limits>
+#include <sstream>
+
+namespace eng {
+
+namespace fs = std::filesystem;
+
+namespace {
+
+double round3(double v) { return std::round(v * 1000.0) / 1000.0; }
+double rounded(double v, int decimals) {
+    const double f = std::pow(10.0, decimals);
+    return std::round(v * f) / f;
+}
+std::string fixed(double v, int decimals) {
+    char b[64];
+    std::snprintf(b, sizeof(b), "%.*f", decimals, v);
+    return b;
+}
+
+std::string cpu_name() {
+    std::ifstream in("/proc/cpuinfo");
+    std::string line;
+    while (std::getline(in, line))
+        if (line.rfind("model name", 0) == 0) {
+            const auto c = line.find(':');
+            if (c != std::string::npos) return line.substr(c + 2);
+        }
+    return "nieznany";
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// Timing / Progress / Gpu
+// ------------------------------------------------------------------------------------------------
+void Timing::add(const std::string& section, double ms) {
+    for (auto& s : sections_)
+        if (s.first == section) {
+            s.second += ms;
+            return;
+        }
+    sections_.emplace_back(section, ms);
+}
+double Timing::get(const std::string& section) const {
+    for (const auto& s : sections_)
+        if (s.first == section) return s.second;
+    return 0.0;
+}
+void Timing::add_file(const std::string& path, double ms) {
+    for (auto& f : files_)
+        if (f.first == path) {
+            f.second += ms;
+            return;
+        }
+    files_.emplace_back(path, ms);
+}
+void Timing::record_files() {
+    Json files = Json::object();
+    for (const auto& f : files_) {
+        std::error_code ec;
+        uint64_t bytes = 0;
+        if (fs::is_directory(f.first, ec)) {
+            for (const auto& e : fs::directory_iterator(f.first, ec))
+                if (e.is_regular_file(ec)) bytes += e.file_size(ec);
+        } else {
+            bytes = fs::file_size(f.first, ec);
+            if (ec) bytes = 0;
+        }
+        Json e = Json::object();
+        e["bytes"] = Json(static_cast<unsigned long long>(bytes));
+        e["writeMs"] = Json(round3(f.second));
+        files[fs::path(f.first).filename().string()] = e;
+    }
+    set_info("files", files);
+}
+Json Timing::to_json() const {
+    Json j = Json::object();
+    j["unit"] = "ms";
+    for (const auto& s : sections_) j[s.first] = Json(round3(s.second));
+    for (const auto& kv : info_.items()) j[kv.first] = kv.second;
+    return j;
+}
+
+void Progress::report(const char* stage, long long done, long long total, bool force, const char* key, uint64_t value) {
+    if (!on_ || (!force && !first_ && since_.ms() < 500.0)) return;
+    first_ = false;
+    std::cout << "PROGRESS " << stage << ' ' << done << '/' << total;
+    if (key) std::cout << ' ' << key << '=' << value;
+    std::cout << '\n' << std::flush;
+    since_.reset();
+}
+
+void vca_check(int rc, const char* what) {
+    if (rc != VCA_OK) {
+        char buf[1024];
+        vca_last_error(buf, sizeof(buf));
+        throw std::runtime_error(std::string(what) + ": " + buf);
+    }
+}
+
+Gpu::Gpu() {
+    vca_check(vca_ctx_create(0, nullptr, &ctx_), "GPU (libvca)");
+    name_ = "gfx950";
+}
+Gpu::~Gpu() {
+    if (ctx_) vca_ctx_destroy(ctx_);
+}
+
+const char* stop_text(int stop) {  // engine/src/ca/CaRules.hpp:110-119
+    switch (stop) {
+        case 0: return "trwa";
+        case 1: return "brak płynnej lawy";
+        case 2: return "brak wolnych komórek (AIR)";
+        case 3: return "brak zmian w kroku";
+        case 4: return "osiągnięto ca.maxSteps";
+    }
+    return "?";
+}
+
+// ------------------------------------------------------------------------------------------------
+// CA stage
+// ------------------------------------------------------------------------------------------------
+namespace {
+
+struct CaHandle {
+    vca_ca* h = nullptr;
+    ~CaHandle() {
+        if (h) vca_ca_destroy(h);
+    }
+};
+
+void download(vca_ca* ca, CaState& s) {
+    vca_check(vca_ca_download(ca, s.type.data(), s.flow.data(), s.temp.data(), s.solid.data(), s.arrival.data()),
+              "pobranie wyniku CA");
+}
+
+}  // namespace
+
+// flowSums + statsFromSums (engine/src/app/Simulation.cpp:408-524)
+Json ca_stats(const Config& c, const RunSummary& s) {
+    const size_t flows = s.vents.size();
+    std::vector<uint64_t> lava(flows + 1, 0), basalt(flows + 1, 0);
+    std::vector<double> reach(flows + 1, 0.0);
+    const Dims& d = s.result.dims;
+    size_t i = 0;
+    for (int z = 0; z < d.nz; ++z)
+        for (int y = 0; y < d.ny; ++y)
+            for (int x = 0; x < d.nx; ++x, ++i) {
+                const uint8_t t = s.result.type[i];
+                const int32_t f = s.result.flow[i];
+                if ((t != VCA_LAVA && t != VCA_BASALT) || f <= 0 || static_cast<size_t>(f) > flows) continue;
+                const size_t fi = static_cast<size_t>(f);
+                (t == VCA_LAVA ? lava : basalt)[fi] += 1;
+                const Vent& v = s.vents[fi - 1];
+                reach[fi] = std::max(reach[fi], std::hypot(double(x - v.x), double(y - v.y)));
+            }
+    Json j = Json::object();
+    j["mode"] = c.ca.classic ? "classic" : "lava";
+    j["steps"] = s.steps;
+    j["stopReason"] = stop_text(s.stop);
+    uint64_t total = 0, total_lava = 0, min_size = std::numeric_limits<uint64_t>::max(), max_size = 0;
+    double max_reach = 0.0;
+    Json list = Json::array();
+    for (size_t f = 1; f <= flows; ++f) {
+        const uint64_t cells = lava[f] + basalt[f];
+        total += cells;
+        total_lava += lava[f];
+        max_reach = std::max(max_reach, reach[f]);
+        min_size = std::min(min_size, cells);
+        max_size = std::max(max_size, cells);
+        if (f <= 10000) {
+            const Vent& v = s.vents[f - 1];
+            Json e = Json::object();
+            e["flowId"] = Json(static_cast<unsigned long long>(f));
+            e["vent"] = Json::array({Json(v.x), Json(v.y), Json(v.z)});
+            e["cells"] = Json(static_cast<unsigned long long>(cells));
+            if (!c.ca.classic) {
+                e["lava"] = Json(static_cast<unsigned long long>(lava[f]));
+                e["basalt"] = Json(static_cast<unsigned long long>(basalt[f]));
+                e["maxReach"] = Json(std::round(reach[f] * 100.0) / 100.0);
+            }
+            list.push_back(e);
+        }
+    }
+    if (!c.ca.classic) {
+        j["totalLavaVolume"] = Json(static_cast<unsigned long long>(total));
+        j["liquidLava"] = Json(static_cast<unsigned long long>(total_lava));
+        j["maxReach"] = Json(std::round(max_reach * 100.0) / 100.0);
+    } else {
+        j["grains"] = Json(static_cast<unsigned long long>(flows));
+        j["meanGrainSize"] = Json(flows ? double(total) / double(flows) : 0.0);
+        j["minGrainSize"] = Json(static_cast<unsigned long long>(flows ? min_size : 0));
+        j["maxGrainSize"] = Json(static_cast<unsigned long long>(max_size));
+    }
+    j["flows"] = list;
+    if (flows > 10000) j["flowsTruncated"] = true;
+    Json a = Json::array(), ch = Json::array();
+    for (uint64_t v : s.active) a.push_back(Json(static_cast<unsigned long long>(v)));
+    for (uint64_t v : s.changed) ch.push_back(Json(static_cast<unsigned long long>(v)));
+    j["activeCells"] = a;
+    j["changedCells"] = ch;
+    return j;
+}
+
+RunSummary run_simulation(const Config& cfg, const std::string& out_dir, Timing& timing, Progress& progress, Gpu& gpu) {
+    RunSummary s;
+    const fs::path out(out_dir);
+    {
+        Clock t;
+        write_json((out / "config_used.json").string(), cfg.effective);
+        timing.add_file((out / "config_used.json").string(), t.ms());
+    }
+    if (cfg.mc.full_only()) {
+        s.ca_ran = false;
+    } else {
+        // terrain + spill levels, craters (host, once per run)
+        Clock t;
+        Heightmap hm = build_terrain(cfg);
+        Heightmap spill;
+        if (!hm.empty()) spill = fill_depressions(hm, cfg.periodic, cfg.periodic);
+        const double terrain_ms = t.ms();
+        t.reset();
+        s.vents = build_vents(cfg, hm);
+        const double vents_ms = t.ms();
+        // GPU state: allocation, rock columns, seeds, initial counts
+        t.reset();
+        vca_ca_desc d{};
+        d.nx = cfg.space.nx;
+        d.ny = cfg.space.ny;
+        d.nz = cfg.space.nz;
+        d.classic = cfg.ca.classic;
+        d.neighborhood = static_cast<int32_t>(cfg.neighborhood);
+        d.periodic = cfg.periodic;
+        d.seed = cfg.seed;
+        d.max_steps = cfg.ca.max_steps;
+        d.eruption_steps = cfg.vents.eruption_steps;
+        // CaParams narrows the double config values to float (Simulation.cpp:122-136)
+        d.erupt_temp = static_cast<float>(cfg.vents.erupt_temp);
+        d.solidus_temp = static_cast<float>(cfg.ca.solidus);
+        d.cooling_ambient = static_cast<float>(cfg.ca.cooling_ambient);
+        d.cooling_lateral = static_cast<float>(cfg.ca.cooling_lateral);
+        d.cooling_fall = static_cast<float>(cfg.ca.cooling_fall);
+        d.ponding = cfg.ca.ponding;
+        d.dense = cfg.ca.front ? 0 : 1;
+        std::vector<vca_vent> vv;
+        for (const Vent& v : s.vents) vv.push_back(vca_vent{v.x, v.y, v.z, v.flow_id});
+        CaHandle ca;
+        vca_check(vca_ca_create(gpu.ctx(), &d, hm.empty() ? nullptr : hm.height.data(),
+                                spill.empty() ? nullptr : spill.height.data(), vv.data(), static_cast<int32_t>(vv.size()),
+                                &ca.h),
+                  "CA");
+        timing.add("allocate", t.ms());
+        timing.add("terrain", terrain_ms);
+        timing.add("vents", vents_ms);
+
+        // steps; snapshots (output.snapshotsEvery) and progress between chunks, outside caSim
+        const int every = cfg.output.snapshots_every;
+        int32_t steps = 0, stop = 0;
+        vca_check(vca_ca_status(ca.h, &steps, &stop), "CA");
+        double ca_ms = 0.0, snap_ms = 0.0;
+        const fs::path snap_dir = out / "snapshots";
+        CaState snap;
+        auto write_snapshot = [&](int k) {
+            Clock ts;
+            if (snap.type.empty()) snap.resize(cfg.space);
+            download(ca.h, snap);
+            char name[32];
+            std::snprintf(name, sizeof(name), "step_%06d.bin", k);
+            write_ca_bin((snap_dir / name).string(), snap);
+            snap_ms += ts.ms();
+        };
+        auto report = [&](bool force) {
+            if (!progress.enabled()) return;
+            uint64_t active = 0;
+            if (steps > 0) {
+                std::vector<uint64_t> a(static_cast<size_t>(steps)), c(static_cast<size_t>(steps));
+                int32_t n = 0;
+                vca_check(vca_ca_histories(ca.h, a.data(), c.data(), steps, &n), "CA");
+                active = a.back();
+            }
+            progress.report("ca", steps, cfg.ca.max_steps, force, "active", active);
+        };
+        if (every > 0) {
+            fs::create_directories(snap_dir);
+            write_snapshot(0);
+        }
+        report(true);
+        while (stop == VCA_RUNNING) {
+            int chunk = progress.enabled() ? 64 : cfg.ca.max_steps;
+            if (every > 0) chunk = std::min(chunk, every - steps % every);
+            Clock tc;
+            vca_check(vca_ca_run(ca.h, chunk, &steps, &stop), "CA");
+            ca_ms += tc.ms();
+            report(stop != VCA_RUNNING);
+            if (every > 0 && (steps % every == 0 || stop != VCA_RUNNING)) write_snapshot(steps);
+        }
+        timing.add("caSim", ca_ms);
+        if (every > 0) {
+            timing.add("snapshots", snap_ms);
+            timing.add_file(snap_dir.string(), snap_ms);
+        }
+        if (progress.enabled()) progress.report("save", 0, 1, true);
+
+        t.reset();
+        s.steps = steps;
+        s.stop = stop;
+        s.active.assign(static_cast<size_t>(steps), 0);
+        s.changed.assign(static_cast<size_t>(steps), 0);
+        int32_t n = 0;
+        vca_check(vca_ca_histories(ca.h, s.active.data(), s.changed.data(), steps, &n), "CA");
+        s.result.resize(cfg.space);
+        download(ca.h, s.result);
+        Clock tf;
+        if (cfg.output.format == OutputFormat::Txt || cfg.output.format == OutputFormat::Both) {
+            tf.reset();
+            write_ca_txt((out / "ca_result.txt").string(), s.result);
+            timing.add_file((out / "ca_result.txt").string(), tf.ms());
+        }
+        if (cfg.output.format == OutputFormat::Bin || cfg.output.format == OutputFormat::Both) {
+            tf.reset();
+            write_ca_bin((out / "ca_result.bin").string(), s.result);
+            timing.add_file((out / "ca_result.bin").string(), tf.ms());
+        }
+        timing.add("save", t.ms());
+    }
+    Clock ts;
+    Json stats = s.ca_ran ? ca_stats(cfg, s) : Json::object();
+    if (s.ca_ran) timing.add("caStats", ts.ms());
+    Json par = Json::object();
+    par["threads"] = cfg.parallel.threads;
+    par["schedule"] = cfg.parallel.schedule;
+    par["chunk"] = cfg.parallel.chunk;
+    par["affinity"] = cfg.parallel.affinity;
+    stats["parallel"] = par;
+    if (cfg.mc.enabled) {
+        s.mc = run_mc(cfg, s.ca_ran ? &s.result : nullptr, Cooling::Full, timing, progress, out_dir, gpu);
+        s.mc_ran = true;
+        if (progress.enabled()) progress.report("save", 0, 1, true);
+        save_mc(cfg, s.mc, out_dir, timing);
+        stats["mc"] = s.mc.stats;
+    }
+    Clock tw;
+    write_json((out / "stats.json").string(), stats);
+    timing.add_file((out / "stats.json").string(), tw.ms());
+    timing.add(s.mc_ran ? "mcSave" : "save", tw.ms());
+    return s;
+}
+
+// ------------------------------------------------------------------------------------------------
+// MC stage
+// ------------------------------------------------------------------------------------------------
+namespace {
+
+struct McHandle {
+    vca_mc* h = nullptr;
+    ~McHandle() {
+        if (h) vca_mc_destroy(h);
+    }
+};
+
+struct ClassStat {
+    uint64_t grains = 0, cells = 0, largest = 0;
+    double mean() const { return grains > 0 ? double(cells) / double(grains) : 0.0; }
+};
+
+Json class_json(uint64_t cells, const ClassStat& a, const ClassStat& b) {
+    Json j = Json::object();
+    j["cells"] = Json(static_cast<unsigned long long>(cells));
+    j["grainsStart"] = Json(static_cast<unsigned long long>(a.grains));
+    j["grainsEnd"] = Json(static_cast<unsigned long long>(b.grains));
+    j["meanGrainSizeStart"] = Json(rounded(a.mean(), 3));
+    j["meanGrainSizeEnd"] = Json(rounded(b.mean(), 3));
+    j["largestGrainEnd"] = Json(static_cast<unsigned long long>(b.largest));
+    return j;
+}
+
+}  // namespace
+
+McSummary run_mc(const Config& cfg, const CaState* ca, Cooling cooling, Timing& timing, Progress& progress,
+                 const std::string& out_dir, Gpu& gpu) {
+    McSummary s;
+    s.full = ca == nullptr;
+    s.mcs = cfg.mc.mcs;
+    s.Q = cfg.mc.Q;
+    Clock t;
+    const Dims dims = s.full ? cfg.space : ca->dims;
+    s.dims = dims;
+    vca_mc_desc d{};
+    d.nx = dims.nx;
+    d.ny = dims.ny;
+    d.nz = dims.nz;
+    d.neighborhood = static_cast<int32_t>(cfg.neighborhood);
+    // full domain wraps every axis, basalt only x and y (McStage.cpp:102-106; PottsMC.cpp:18)
+    d.periodic[0] = cfg.periodic && dims.nx > 1;
+    d.periodic[1] = cfg.periodic && dims.ny > 1;
+    d.periodic[2] = cfg.periodic && s.full && dims.nz > 1;
+    d.seed = cfg.seed;
+    d.Q = cfg.mc.Q;
+    d.J = cfg.mc.J;
+    d.kT = cfg.mc.kT;
+    d.checkerboard = cfg.mc.checkerboard;
+    std::vector<uint8_t> domain;
+    std::vector<float> mobility;
+    bool dyn = false;
+    if (!s.full) {
+        const size_t n = ca->type.size();
+        domain.resize(n);
+        for (size_t i = 0; i < n; ++i) domain[i] = (ca->type[i] == VCA_BASALT || ca->type[i] == VCA_LAVA) ? 1 : 0;
+        if (cooling != Cooling::None) {
+            // CoolingMobility::fromCooling (CoolingMobility.cpp:8-129)
+            uint32_t end = 0;
+            for (size_t i = 0; i < n; ++i)
+                if (domain[i]) end = std::max<uint32_t>(end, std::max(ca->solid[i], ca->arrival[i]));
+            std::vector<uint32_t> time(n, 0);
+            uint32_t tmin = 0xFFFFFFFFu, tmax = 0;
+            for (size_t i = 0; i < n; ++i) {
+                if (!domain[i]) continue;
+                const uint32_t sd = ca->type[i] == VCA_BASALT ? ca->solid[i] : end;
+                time[i] = sd > ca->arrival[i] ? sd - ca->arrival[i] : 0;
+                tmin = std::min(tmin, time[i]);
+                tmax = std::max(tmax, time[i]);
+            }
+            mobility.assign(n, 0.0f);
+            for (size_t i = 0; i < n; ++i) {
+                if (!domain[i]) continue;
+                if (tmax == tmin) {
+                    mobility[i] = 1.0f;
+                    continue;
+                }
+                const double norm = double(time[i] - tmin) / double(tmax - tmin);
+                mobility[i] = static_cast<float>(cfg.mc.mobility_min + (1.0 - cfg.mc.mobility_min) * norm);
+            }
+            if (cfg.mc.mobility_from_cooling) {
+                dyn = true;
+                s.mobility_used = true;
+            }
+        }
+        if (cooling == Cooling::None && cfg.mc.mobility_from_cooling)
+            s.warnings.push_back("wynik CA bez solidStep/arrivalStep (plik txt): mobilność ze stygnięcia wyłączona, "
+                                 "MC jak dla mobilityFromCooling = false");
+        if (cooling == Cooling::SolidOnly)
+            s.warnings.push_back("ca_result.bin w wersji 1 nie zawiera arrivalStep, czas w stanie płynnym liczony od "
+                                 "kroku 0 (= solidStep)");
+    }
+    McHandle mc;
+    vca_check(vca_mc_create(gpu.ctx(), &d, s.full ? nullptr : domain.data(), mobility.empty() ? nullptr : mobility.data(),
+                            dyn ? 1 : 0, &mc.h),
+              "MC");
+    int32_t colors = 0;
+    uint64_t dom_cells = 0;
+    vca_check(vca_mc_info(mc.h, &colors, &dom_cells), "MC");
+    s.domain_cells = dom_cells;
+    if (dom_cells == 0)
+        s.warnings.push_back(s.full ? "domena MC jest pusta" : "domena MC jest pusta: wynik CA nie zawiera bazaltu");
+    timing.add("mcInit", t.ms());
+
+    const bool has_classes = !mobility.empty();
+    auto measure = [&](McRow& row, ClassStat* cls, uint64_t* largest, uint64_t* high) {
+        vca_mc_stats st{};
+        vca_mc_class c[2]{};
+        uint64_t hi = 0;
+        vca_check(vca_mc_measure(mc.h, &st, has_classes ? c : nullptr, &hi), "MC");
+        row.mcs = st.mcs;
+        row.grains = st.grains;
+        row.mean_grain_size = st.mean_grain_size;
+        row.energy = st.energy;
+        if (cls) {
+            cls[0] = ClassStat{c[0].grains, c[0].grain_cells, c[0].largest};
+            cls[1] = ClassStat{c[1].grains, c[1].grain_cells, c[1].largest};
+        }
+        if (largest) *largest = st.largest;
+        if (high) *high = hi;
+    };
+    const int snap_every = out_dir.empty() ? 0 : cfg.mc.snapshots_every;
+    const fs::path snap_dir = fs::path(out_dir) / "mc_snapshots";
+    double snap_ms = 0.0;
+    std::vector<int32_t> snap;
+    auto write_snapshot = [&](int k) {
+        Clock ts;
+        snap.resize(dims.cells());
+        vca_check(vca_mc_download(mc.h, snap.data()), "MC");
+        char name[32];
+        std::snprintf(name, sizeof(name), "mcs_%06d.bin", k);
+        write_crystal_bin((snap_dir / name).string(), dims, snap);
+        snap_ms += ts.ms();
+    };
+    if (snap_every > 0) {
+        fs::create_directories(snap_dir);
+        write_snapshot(0);
+    }
+    double sim_ms = 0.0, stats_ms = 0.0;
+    ClassStat start_cls[2], end_cls[2];
+    uint64_t largest_end = 0, high_cells = 0;
+    t.reset();
+    McRow row0;
+    measure(row0, start_cls, &largest_end, &high_cells);
+    s.history.push_back(row0);
+    for (int c = 0; c < 2; ++c) end_cls[c] = start_cls[c];
+    stats_ms += t.ms();
+    progress.report("mc", 0, cfg.mc.mcs, true);
+    uint64_t att_before = 0, acc_before = 0;
+    int k = 0;
+    while (k < cfg.mc.mcs) {
+        // sweep up to the next measurement / snapshot point
+        int next = std::min(cfg.mc.mcs, (k / cfg.mc.stats_every + 1) * cfg.mc.stats_every);
+        if (snap_every > 0) next = std::min(next, (k / snap_every + 1) * snap_every);
+        if (progress.enabled()) next = std::min(next, k + 16);
+        vca_mc_counts counts{};
+        Clock tc;
+        vca_check(vca_mc_sweep(mc.h, next - k, &counts), "MC");
+        sim_ms += tc.ms();
+        k = next;
+        if (k % cfg.mc.stats_every == 0 || k == cfg.mc.mcs) {
+            Clock tm;
+            McRow row;
+            const bool last = k == cfg.mc.mcs;
+            measure(row, last ? end_cls : nullptr, last ? &largest_end : nullptr, nullptr);
+            const uint64_t att = counts.attempts - att_before, acc = counts.accepted - acc_before;
+            row.accept_rate = att > 0 ? double(acc) / double(att) : 0.0;
+            att_before = counts.attempts;
+            acc_before = counts.accepted;
+            s.history.push_back(row);
+            stats_ms += tm.ms();
+        }
+        if (snap_every > 0 && (k % snap_every == 0 || k == cfg.mc.mcs)) write_snapshot(k);
+        progress.report("mc", k, cfg.mc.mcs, k == cfg.mc.mcs);
+    }
+    timing.add("mcSim", sim_ms);
+    if (snap_every > 0) {
+        timing.add("mcSnapshots", snap_ms);
+        timing.add_file(snap_dir.string(), snap_ms);
+    }
+
+    // stats.json "mc" section (McStage::statsJson, McStage.cpp:326-383)
+    t.reset();
+    vca_mc_counts total{};
+    vca_check(vca_mc_counts_get(mc.h, &total), "MC");
+    const McRow& first = s.history.front();
+    const McRow& last = s.history.back();
+    Json j = Json::object();
+    j["domain"] = s.full ? "full" : "basalt";
+    j["domainCells"] = Json(static_cast<unsigned long long>(s.domain_cells));
+    j["space"] = Json::array({Json(dims.nx), Json(dims.ny), Json(dims.nz)});
+    j["Q"] = cfg.mc.Q;
+    j["mcs"] = cfg.mc.mcs;
+    j["J"] = Json(cfg.mc.J);
+    j["kT"] = Json(cfg.mc.kT);
+    j["update"] = cfg.mc.checkerboard ? "checkerboard" : "random";
+    if (cfg.mc.checkerboard) j["colors"] = colors;
+    j["neighborhood"] = neighborhood_name(cfg.neighborhood);
+    j["boundary"] = cfg.periodic ? "periodic" : "absorbing";
+    j["grainConnectivity"] = cfg.neighborhood != Neighborhood::VonNeumann ? "moore" : "von_neumann";
+    j["mobility"] = s.mobility_used ? "cooling" : "off";
+    if (s.mobility_used) j["mobilityMin"] = Json(cfg.mc.mobility_min);
+    j["grainsStart"] = Json(static_cast<unsigned long long>(first.grains));
+    j["grainsEnd"] = Json(static_cast<unsigned long long>(last.grains));
+    j["meanGrainSizeStart"] = Json(rounded(first.mean_grain_size, 3));
+    j["meanGrainSizeEnd"] = Json(rounded(last.mean_grain_size, 3));
+    j["largestGrainEnd"] = Json(static_cast<unsigned long long>(largest_end));
+    j["energyStart"] = Json(first.energy);
+    j["energyEnd"] = Json(last.energy);
+    j["attempts"] = Json(static_cast<unsigned long long>(total.attempts));
+    j["accepted"] = Json(static_cast<unsigned long long>(total.accepted));
+    j["acceptedUphill"] = Json(static_cast<unsigned long long>(total.uphill));
+    if (has_classes) {
+        Json bm = Json::object();
+        bm["threshold"] = Json(0.5);
+        bm["mobilityMin"] = Json(cfg.mc.mobility_min);
+        bm["rule"] = "ziarno należy do klasy, w której jest większość jego komórek";
+        bm["high"] = class_json(high_cells, start_cls[0], end_cls[0]);
+        bm["low"] = class_json(s.domain_cells - high_cells, start_cls[1], end_cls[1]);
+        j["byMobility"] = bm;
+    }
+    s.stats = j;
+    s.crystals.resize(dims.cells());
+    vca_check(vca_mc_download(mc.h, s.crystals.data()), "MC");
+    timing.add("mcStats", stats_ms + t.ms());
+    return s;
+}
+
+void save_mc(const Config& cfg, const McSummary& s, const std::string& out_dir, Timing& timing) {
+    Clock t;
+    const fs::path out(out_dir);
+    Clock tf;
+    if (cfg.output.format == OutputFormat::Txt || cfg.output.format == OutputFormat::Both) {
+        tf.reset();
+        write_crystal_txt((out / "mc_result.txt").string(), s.dims, s.crystals);
+        timing.add_file((out / "mc_result.txt").string(), tf.ms());
+    }
+    if (cfg.output.format == OutputFormat::Bin || cfg.output.format == OutputFormat::Both) {
+        tf.reset();
+        write_crystal_bin((out / "mc_result.bin").string(), s.dims, s.crystals);
+        timing.add_file((out / "mc_result.bin").string(), tf.ms());
+    }
+    tf.reset();
+    write_mc_stats_csv((out / "mc_stats.csv").string(), s.history);
+    timing.add_file((out / "mc_stats.csv").string(), tf.ms());
+    timing.add("mcSave", t.ms());
+}
+
+// ------------------------------------------------------------------------------------------------
+// CLI driver (engine/src/app/Simulation.cpp:526-683)
+// ------------------------------------------------------------------------------------------------
+namespace {
+
+void common_info(Timing& timing, const Config& c, const Dims& d) {
+    timing.set_info("cells", Json(static_cast<unsigned long long>(d.cells())));
+    timing.set_info("space", Json::array({Json(d.nx), Json(d.ny), Json(d.nz)}));
+    timing.set_info("neighborhood", neighborhood_name(c.neighborhood));
+    timing.set_info("boundary", c.periodic ? "periodic" : "absorbing");
+}
+void mc_info(Timing& timing, const McSummary& mc) {
+    timing.set_info("mcDomain", mc.full ? "full" : "basalt");
+    timing.set_info("mcs", mc.mcs);
+    timing.set_info("domainCells", Json(static_cast<unsigned long long>(mc.domain_cells)));
+    timing.set_info("Q", mc.Q);
+    timing.set_info("mcMobility", mc.mobility_used);
+}
+void env_info(Timing& timing, const Gpu& gpu) {
+    timing.set_info("cpu", cpu_name());
+    timing.set_info("device", gpu.device_name());
+    timing.set_info("compiler", std::string("GCC ") + __VERSION__ + " + hipcc (gfx950)");
+    timing.set_info("build", "Release");
+}
+std::string times_line(const Timing& timing) {
+    std::string line = "Czasy [ms]     ";
+    const char* sep = "";
+    for (const auto& s : timing.sections()) {
+        line += sep + s.first + ' ' + fixed(s.second, 2);
+        sep = " | ";
+    }
+    return line + '\n';
+}
+
+}  // namespace
+
+int run_from_cli(const CliOptions& opt, const Clock& since_start) {
+    Timing timing;
+    Progress progress(opt.progress);
+    Clock t;
+    Config cfg = Config::load(opt.config_path, opt.overrides);
+    timing.add("readConfig", t.ms());
+    timing.set_info("threads", cfg.parallel.threads);
+    timing.set_info("schedule", cfg.parallel.schedule);
+    timing.set_info("chunk", cfg.parallel.chunk);
+    timing.set_info("affinity", cfg.parallel.affinity);
+    for (const std::string& w : cfg.warnings) std::cerr << "Ostrzeżenie: " << w << '\n';
+    std::string out_dir = opt.out_dir;
+    if (out_dir.empty()) {
+        const std::string stem = opt.config_path.empty() ? "default" : fs::path(opt.config_path).stem().string();
+        out_dir = (fs::path("results") / stem).string();
+    }
+    fs::create_directories(out_dir);
+    Gpu gpu;
+
+    if (!opt.mc_from.empty()) {
+        // engine --mc-from: MC on a saved CA result (Simulation.cpp:334-402)
+        if (cfg.mc.full)
+            throw ConfigError("--mc-from krystalizuje wynik CA z pliku, więc wymaga mc.domain = \"basalt\" (podano \"full\")");
+        std::vector<std::string> warnings;
+        if (!cfg.mc.enabled) {
+            warnings.push_back("mc.enabled = false, a w trybie --mc-from Monte Carlo jest uruchamiane zawsze");
+            cfg.mc.enabled = true;
+        }
+        Clock tr;
+        CaSource src;
+        const CaState ca = read_ca_folder(opt.mc_from, &src);
+        timing.add("readInput", tr.ms());
+        if (ca.dims != cfg.space && cfg.space_from_user)
+            warnings.push_back("wymiary z pliku " + std::to_string(ca.dims.nx) + " x " + std::to_string(ca.dims.ny) + " x " +
+                               std::to_string(ca.dims.nz) + " różnią się od space z configu; obowiązują wymiary z pliku");
+        cfg.space = ca.dims;
+        if (cfg.neighborhood == Neighborhood::Hexagonal && !cfg.space.is2d())
+            throw ConfigError("Sąsiedztwo „hexagonal” jest zdefiniowane tylko w 2D, a wynik CA z pliku jest 3D");
+        Json sp = Json::object();
+        sp["nx"] = ca.dims.nx;
+        sp["ny"] = ca.dims.ny;
+        sp["nz"] = ca.dims.nz;
+        cfg.effective["space"] = sp;
+        cfg.effective["mc"]["enabled"] = true;
+        cfg.effective["_mcFrom"] = fs::absolute(fs::path(opt.mc_from)).lexically_normal().string();
+        const fs::path out(out_dir);
+        tr.reset();
+        write_json((out / "config_used.json").string(), cfg.effective);
+        timing.add_file((out / "config_used.json").string(), tr.ms());
+        const Cooling cooling = src.has_arrival ? Cooling::Full : src.has_solid ? Cooling::SolidOnly : Cooling::None;
+        McSummary mc = run_mc(cfg, &ca, cooling, timing, progress, out_dir, gpu);
+        mc.warnings.insert(mc.warnings.begin(), warnings.begin(), warnings.end());
+        progress.report("save", 0, 1, true);
+        save_mc(cfg, mc, out_dir, timing);
+        tr.reset();
+        Json stats = Json::object();
+        Json source = Json::object();
+        source["folder"] = cfg.effective["_mcFrom"];
+        source["file"] = src.path;
+        source["format"] = src.format;
+        source["version"] = src.version;
+        source["coolingData"] = cooling == Cooling::Full ? "solidStep+arrivalStep" : cooling == Cooling::SolidOnly ? "solidStep" : "brak";
+        stats["source"] = source;
+        Json par = Json::object();
+        par["threads"] = cfg.parallel.threads;
+        par["schedule"] = cfg.parallel.schedule;
+        par["chunk"] = cfg.parallel.chunk;
+        par["affinity"] = cfg.parallel.affinity;
+        stats["parallel"] = par;
+        stats["mc"] = mc.stats;
+        Clock tw;
+        write_json((out / "stats.json").string(), stats);
+        timing.add_file((out / "stats.json").string(), tw.ms());
+        timing.add("mcSave", tr.ms());
+        timing.add("total", since_start.ms());
+        common_info(timing, cfg, mc.dims);
+        timing.set_info("mode", "mc_from");
+        timing.set_info("mcFrom", opt.mc_from);
+        mc_info(timing, mc);
+        env_info(timing, gpu);
+        timing.record_files();
+        write_json((out / "times.json").string(), timing.to_json());
+        for (const std::string& w : mc.warnings) std::cerr << "Ostrzeżenie: " << w << '\n';
+        std::cout << "=== Monte Carlo na wyniku CA z pliku zakończone ===\n"
+                  << "Wejście        " << opt.mc_from << '\n'
+                  << "Przestrzeń     " << mc.dims.nx << " x " << mc.dims.ny << " x " << mc.dims.nz << '\n'
+                  << times_line(timing) << "Wyniki         " << out_dir << '\n';
+        return 0;
+    }
+
+    const RunSummary s = run_simulation(cfg, out_dir, timing, progress, gpu);
+    timing.add("total", since_start.ms());
+    const Dims& d = cfg.space;
+    common_info(timing, cfg, d);
+    if (s.ca_ran) {
+        timing.set_info("steps", s.steps);
+        timing.set_info("grid", cfg.grid_matrix ? "matrix" : "vector");
+        timing.set_info("mode", cfg.ca.classic ? "classic" : "lava");
+        timing.set_info("stopReason", stop_text(s.stop));
+    } else {
+        timing.set_info("mode", "mc_full");
+    }
+    if (s.mc_ran) mc_info(timing, s.mc);
+    env_info(timing, gpu);
+    timing.record_files();
+    write_json((fs::path(out_dir) / "times.json").string(), timing.to_json());
+    for (const std::string& w : s.mc.warnings) std::cerr << "Ostrzeżenie: " << w << '\n';
+    std::ostringstream os;
+    os << "=== Symulacja zakończona ===\n";
+    os << "Przestrzeń     " << d.nx << " x " << d.ny << " x " << d.nz << " (" << (d.is2d() ? "2D" : "3D") << ", " << d.cells()
+       << " komórek)\n";
+    if (s.ca_ran) {
+        os << "Tryb / siatka  " << (cfg.ca.classic ? "classic" : "lava") << (s.mc_ran ? " + MC" : "") << " / GPU\n";
+        os << "Kroki          " << s.steps << " (koniec: " << stop_text(s.stop) << ")\n";
+    } else {
+        os << "Tryb           Monte Carlo na całej przestrzeni (bez terenu i CA)\n";
+    }
+    if (s.mc_ran)
+        os << "Monte Carlo    domena " << (s.mc.full ? "full" : "basalt") << ": " << s.mc.domain_cells << " komórek, Q "
+           << s.mc.Q << ", " << s.mc.mcs << " MCS\n";
+    os << times_line(timing) << "Wyniki         " << out_dir << '\n';
+    std::cout << os.str();
+    return 0;
+}
+
+// engine --validate (engine/src/app/ConfigCheck.cpp:43-72)
+int run_validate(const CliOptions& opt, std::ostream& out) {
+    Json errors = Json::array(), warnings = Json::array();
+    try {
+        const Config cfg = Config::load(opt.config_path, opt.overrides);
+        for (const std::string& w : cfg.warnings) warnings.push_back(Json(w));
+        if (!opt.mc_from.empty()) {
+            if (cfg.mc.full)
+                errors.push_back(Json("--mc-from krystalizuje wynik CA z pliku, więc wymaga mc.domain = \"basalt\""));
+            if (!cfg.mc.enabled)
+                warnings.push_back(Json("mc.enabled = false, a w trybie --mc-from Monte Carlo jest uruchamiane zawsze"));
+            const fs::path dir(opt.mc_from);
+            if (!fs::is_directory(dir))
+                errors.push_back(Json("Folder wyniku CA (--mc-from) nie istnieje: " + opt.mc_from));
+            else if (!fs::exists(dir / "ca_result.bin") && !fs::exists(dir / "ca_result.txt"))
+                errors.push_back(Json("W folderze " + opt.mc_from + " nie ma ca_result.bin ani ca_result.txt"));
+        } else if (!cfg.mc.full_only() && !cfg.ca.classic && cfg.terrain.type == TerrainType::File) {
+            (void)build_terrain(cfg);
+        }
+    } catch (const std::exception& e) {
+        errors.push_back(Json(std::string(e.what())));
+    }
+    Json r = Json::object();
+    r["ok"] = errors.size() == 0;
+    r["errors"] = errors;
+    r["warnings"] = warnings;
+    out << r.dump(2) << '\n';
+    return errors.size() == 0 ? 0 : 1;
+}
+
+}  // namespace eng

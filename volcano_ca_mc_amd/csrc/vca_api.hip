@@ -1,0 +1,902 @@
+// extern "C" layer of libvca.so: handles, device memory, host-built tables, launch orchestration.
+// Declarations and the reference interfaces each entry point replaces: include/vca.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "vca.h"
+#include "vca_internal.h"
+
+using namespace vca;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct VcaError : std::runtime_error {
+    int code;
+    VcaError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw VcaError(VCA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return VCA_OK;
+    } catch (const VcaError& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return VCA_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return VCA_ERR_ARG;
+    }
+}
+
+void require(bool ok, const std::string& msg) {
+    if (!ok) throw VcaError(VCA_ERR_ARG, msg);
+}
+
+// Device allocation that remembers its size; freed with the owner.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void alloc(size_t n) {
+        release();
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            throw VcaError(VCA_ERR_NOMEM, "hipMalloc of " + std::to_string(n) + " bytes failed: " + hipGetErrorString(e));
+        }
+        bytes = n;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ~DevBuf() { release(); }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+struct KernelTime {
+    double ms = 0.0;
+    uint64_t launches = 0;
+};
+
+// Neighbourhood tables from the reference rules (engine/src/core/Neighborhood.cpp:55-119).
+int pos(int dx, int dy, int dz) { return (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1); }
+
+bool keep2d(int kind, int var, int u, int v) {
+    switch (kind) {
+        case VCA_VON_NEUMANN: return std::abs(u) + std::abs(v) == 1;
+        case VCA_MOORE: return true;
+        case VCA_PENTAGONAL: return var == 0 ? u != -1 : var == 1 ? u != 1 : var == 2 ? v != -1 : v != 1;
+        case VCA_HEXAGONAL:
+            return var == 0 ? !((u == -1 && v == -1) || (u == 1 && v == 1)) : !((u == -1 && v == 1) || (u == 1 && v == -1));
+    }
+    return false;
+}
+bool keep3d(int kind, int var, int dx, int dy, int dz) {
+    switch (kind) {
+        case VCA_VON_NEUMANN: return std::abs(dx) + std::abs(dy) + std::abs(dz) == 1;
+        case VCA_MOORE: return true;
+        case VCA_PENTAGONAL: {
+            const bool a[6] = {dx != -1, dx != 1, dy != -1, dy != 1, dz != -1, dz != 1};
+            return a[var];
+        }
+    }
+    return false;
+}
+
+// Returns the variant masks of kind on dims; throws for the unsupported combinations.
+std::vector<uint32_t> variant_masks(int kind, int nx, int ny, int nz) {
+    const int sizes[3] = {nx, ny, nz};
+    const int axes = (nx > 1) + (ny > 1) + (nz > 1);
+    require(axes >= 2, "sąsiedztwo wymaga przestrzeni 2D lub 3D");
+    require(kind >= 0 && kind <= 3, "nieznane sąsiedztwo");
+    std::vector<uint32_t> out;
+    if (axes == 2) {
+        int plane[2], k = 0;
+        for (int a = 0; a < 3; ++a)
+            if (sizes[a] > 1) plane[k++] = a;
+        const int count = kind == VCA_PENTAGONAL ? 4 : kind == VCA_HEXAGONAL ? 2 : 1;
+        for (int var = 0; var < count; ++var) {
+            uint32_t m = 0;
+            for (int v = -1; v <= 1; ++v)
+                for (int u = -1; u <= 1; ++u) {
+                    if ((u == 0 && v == 0) || !keep2d(kind, var, u, v)) continue;
+                    int d[3] = {0, 0, 0};
+                    d[plane[0]] = u;
+                    d[plane[1]] = v;
+                    m |= 1u << pos(d[0], d[1], d[2]);
+                }
+            out.push_back(m);
+        }
+        return out;
+    }
+    require(kind != VCA_HEXAGONAL, "sąsiedztwo heksagonalne jest zdefiniowane tylko w 2D");
+    const int count = kind == VCA_PENTAGONAL ? 6 : 1;
+    for (int var = 0; var < count; ++var) {
+        uint32_t m = 0;
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx)
+                    if (!(dx == 0 && dy == 0 && dz == 0) && keep3d(kind, var, dx, dy, dz)) m |= 1u << pos(dx, dy, dz);
+        out.push_back(m);
+    }
+    return out;
+}
+
+NbTable make_nb(int kind, int nx, int ny, int nz) {
+    NbTable t;
+    const std::vector<uint32_t> m = variant_masks(kind, nx, ny, nz);
+    t.variants = static_cast<int32_t>(m.size());
+    for (size_t v = 0; v < m.size(); ++v) {
+        t.mask[v] = m[v];
+        t.any_mask |= m[v];
+        t.max_size = std::max(t.max_size, __builtin_popcount(m[v]));
+    }
+    for (int k = 9; k < 18; ++k)
+        if ((t.any_mask >> k) & 1u) t.side_mask |= 1u << k;
+    for (int k = 0; k < 9; ++k)
+        if (k != 4 && ((t.any_mask >> k) & 1u)) t.diagonal_falls = 1;  // dz == -1 with dx|dy != 0
+    t.conn_mask = variant_masks(kind == VCA_VON_NEUMANN ? VCA_VON_NEUMANN : VCA_MOORE, nx, ny, nz)[0];
+    return t;
+}
+
+}  // namespace
+
+struct vca_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool timing = false;
+    std::map<std::string, KernelTime> times;
+    hipEvent_t ev[8] = {};
+
+    void set_device() const { check(hipSetDevice(device), "hipSetDevice"); }
+    void add_time(const char* name, hipEvent_t a, hipEvent_t b) {
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+        KernelTime& k = times[name];
+        k.ms += ms;
+        k.launches += 1;
+    }
+};
+
+struct vca_ca {
+    vca_ctx* ctx = nullptr;
+    CaParams P{};
+    CaDev D{};
+    DevBuf type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, active, changes, ctrl, hist_a, hist_c;
+    int grid_tiles = 1;
+    uint64_t n = 0;
+};
+
+struct vca_mc {
+    vca_ctx* ctx = nullptr;
+    McParams P{};
+    int spin_bytes = 4;
+    bool checkerboard = true;
+    double J = 1.0, kT = 0.0;
+    uint64_t n = 0;
+    uint32_t ndomain = 0;
+    int colors = 0;
+    std::vector<uint32_t> color_start;
+    int sweeps = 0;
+    bool has_mob_dyn = false;
+    bool has_mob_cells = false;
+    DevBuf spin, list, mob_thr, mob_cells, accept, counters, energy, parent, size, hi, gout;
+};
+
+// ===================================================================================================
+// library / context
+// ===================================================================================================
+extern "C" int vca_version(void) { return 1; }
+
+extern "C" int vca_last_error(char* buf, size_t cap) {
+    if (buf && cap > 0) {
+        std::strncpy(buf, g_last_error.c_str(), cap - 1);
+        buf[cap - 1] = '\0';
+    }
+    return VCA_OK;
+}
+
+extern "C" int vca_ctx_create(int device, void* stream, vca_ctx** out) {
+    return guarded([&] {
+        require(out != nullptr, "out == NULL");
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+            throw VcaError(VCA_ERR_DEVICE, "no HIP device available");
+        require(device >= 0 && device < count, "device index out of range");
+        hipDeviceProp_t prop;
+        check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            throw VcaError(VCA_ERR_DEVICE, std::string("libvca is built for gfx950, device is ") + prop.gcnArchName);
+        auto* c = new vca_ctx();
+        c->device = device;
+        c->set_device();
+        if (stream) {
+            c->stream = static_cast<hipStream_t>(stream);
+        } else {
+            check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+            c->own_stream = true;
+        }
+        for (auto& e : c->ev) check(hipEventCreate(&e), "hipEventCreate");
+        *out = c;
+    });
+}
+
+extern "C" int vca_ctx_destroy(vca_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        ctx->set_device();
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto& e : ctx->ev) (void)hipEventDestroy(e);
+        if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+extern "C" int vca_ctx_synchronize(vca_ctx* ctx) {
+    return guarded([&] {
+        require(ctx, "ctx == NULL");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+extern "C" int vca_device_memory(vca_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
+    return guarded([&] {
+        require(ctx, "ctx == NULL");
+        ctx->set_device();
+        size_t f = 0, t = 0;
+        check(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+        if (free_bytes) *free_bytes = f;
+        if (total_bytes) *total_bytes = t;
+    });
+}
+
+extern "C" int vca_ctx_set_timing(vca_ctx* ctx, int32_t on) {
+    return guarded([&] {
+        require(ctx, "ctx == NULL");
+        ctx->timing = on != 0;
+    });
+}
+
+extern "C" int vca_ctx_kernel_time(vca_ctx* ctx, const char* name, vca_kernel_time* out) {
+    return guarded([&] {
+        require(ctx && name && out, "NULL argument");
+        auto it = ctx->times.find(name);
+        out->ms = it == ctx->times.end() ? 0.0 : it->second.ms;
+        out->launches = it == ctx->times.end() ? 0 : it->second.launches;
+    });
+}
+
+// ===================================================================================================
+// CA
+// ===================================================================================================
+extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* height, const int32_t* spill,
+                             const vca_vent* vents, int32_t nvents, vca_ca** out) {
+    return guarded([&] {
+        require(ctx && d && out, "NULL argument");
+        require(d->nx >= 1 && d->ny >= 1 && d->nz >= 1, "wymiary muszą być >= 1");
+        const uint64_t n = static_cast<uint64_t>(d->nx) * d->ny * d->nz;
+        require(n <= 2000000000ULL, "przestrzeń ma za dużo komórek (maksimum 2 000 000 000)");
+        require(d->max_steps >= 0 && d->max_steps <= 65535, "ca.maxSteps poza zakresem 0..65535");
+        require(nvents >= 0 && (nvents == 0 || vents != nullptr), "vents == NULL");
+        const bool classic = d->classic != 0;
+        if (!classic) {
+            require(height != nullptr && spill != nullptr, "tryb lava wymaga mapy wysokości i poziomów przelewu");
+            require(!(((d->nx > 1) + (d->ny > 1) + (d->nz > 1)) == 2 && d->ny != 1),
+                    "w trybie lava przestrzeń 2D to przekrój x-z (ny = 1)");
+        }
+        ctx->set_device();
+        auto ca = std::make_unique<vca_ca>();
+        ca->ctx = ctx;
+        ca->n = n;
+        CaParams& P = ca->P;
+        P.nx = d->nx;
+        P.ny = d->ny;
+        P.nz = d->nz;
+        P.classic = classic;
+        P.per[0] = d->periodic && d->nx > 1;
+        P.per[1] = d->periodic && d->ny > 1;
+        P.per[2] = d->periodic && classic && d->nz > 1;  // Boundary.cpp:5-11
+        P.seed = d->seed;
+        P.max_steps = d->max_steps;
+        P.eruption_steps = d->eruption_steps;
+        P.erupt_temp = d->erupt_temp;
+        P.solidus = d->solidus_temp;
+        P.cool_amb = d->cooling_ambient;
+        P.cool_lat = d->cooling_lateral;
+        P.cool_fall = d->cooling_fall;
+        P.ponding = d->ponding != 0;
+        P.dense = d->dense != 0;
+        P.nb = make_nb(d->neighborhood, d->nx, d->ny, d->nz);
+        if (d->ny == 1) {
+            P.ty = 1;
+            P.tz = 16;
+        } else if (d->nz == 1) {
+            P.ty = 16;
+            P.tz = 1;
+        } else {
+            P.ty = 4;
+            P.tz = 4;
+        }
+        P.ntx = (d->nx + kTileX - 1) / kTileX;
+        P.nty = (d->ny + P.ty - 1) / P.ty;
+        P.ntz = (d->nz + P.tz - 1) / P.tz;
+        P.ntiles = static_cast<uint32_t>(static_cast<uint64_t>(P.ntx) * P.nty * P.ntz);
+        ca->grid_tiles = static_cast<int>(std::min<uint32_t>(P.ntiles, 8192u));
+
+        // vents: flowId -> cell (isVent, LavaCA.hpp:335-341)
+        int32_t maxf = 0;
+        std::vector<uint32_t> vcell(static_cast<size_t>(nvents));
+        std::vector<int32_t> vflow(static_cast<size_t>(nvents));
+        std::vector<float> vtemp(static_cast<size_t>(nvents), classic ? 0.0f : d->erupt_temp);
+        for (int k = 0; k < nvents; ++k) {
+            const vca_vent& v = vents[k];
+            require(v.x >= 0 && v.x < d->nx && v.y >= 0 && v.y < d->ny && v.z >= 0 && v.z < d->nz,
+                    "krater poza siatką");
+            vcell[k] = static_cast<uint32_t>(v.x + static_cast<uint64_t>(d->nx) * (v.y + static_cast<uint64_t>(d->ny) * v.z));
+            vflow[k] = v.flow_id;
+            maxf = std::max(maxf, v.flow_id);
+        }
+        std::vector<uint32_t> vent_table(static_cast<size_t>(maxf) + 1, 0xFFFFFFFFu);
+        if (!classic)
+            for (int k = 0; k < nvents; ++k)
+                if (vflow[k] >= 0) vent_table[static_cast<size_t>(vflow[k])] = vcell[k];
+        P.nvent_slots = maxf + 1;
+
+        hipStream_t s = ctx->stream;
+        ca->type.alloc(n);
+        ca->flow.alloc(n * 4);
+        ca->temp.alloc(n * 4);
+        ca->solid.alloc(n * 2);
+        ca->arrival.alloc(n * 2);
+        ca->descent.alloc(n);
+        ca->vent_cell.alloc(vent_table.size() * 4);
+        ca->tile_flags.alloc(P.ntiles);
+        ca->active.alloc(static_cast<size_t>(P.ntiles) * 4);
+        ca->changes.alloc(n * sizeof(Change));
+        ca->ctrl.alloc(sizeof(CaCtrl));
+        const size_t hcap = static_cast<size_t>(std::max(1, d->max_steps));
+        ca->hist_a.alloc(hcap * 8);
+        ca->hist_c.alloc(hcap * 8);
+        check(hipMemsetAsync(ca->type.p, 0, n, s), "memset");
+        check(hipMemsetAsync(ca->flow.p, 0, n * 4, s), "memset");
+        check(hipMemsetAsync(ca->temp.p, 0, n * 4, s), "memset");
+        check(hipMemsetAsync(ca->solid.p, 0, n * 2, s), "memset");
+        check(hipMemsetAsync(ca->arrival.p, 0, n * 2, s), "memset");
+        check(hipMemsetAsync(ca->descent.p, 0xFF, n, s), "memset");
+        check(hipMemsetAsync(ca->ctrl.p, 0, sizeof(CaCtrl), s), "memset");
+        check(hipMemsetAsync(ca->hist_a.p, 0, hcap * 8, s), "memset");
+        check(hipMemsetAsync(ca->hist_c.p, 0, hcap * 8, s), "memset");
+        check(hipMemcpyAsync(ca->vent_cell.p, vent_table.data(), vent_table.size() * 4, hipMemcpyHostToDevice, s),
+              "copy vents");
+
+        CaDev& D = ca->D;
+        D.type = ca->type.as<uint8_t>();
+        D.flow = ca->flow.as<int32_t>();
+        D.temp = ca->temp.as<float>();
+        D.solid = ca->solid.as<uint16_t>();
+        D.arrival = ca->arrival.as<uint16_t>();
+        D.descent = ca->descent.as<int8_t>();
+        D.vent_cell = ca->vent_cell.as<uint32_t>();
+        D.tile_flags = ca->tile_flags.as<uint8_t>();
+        D.active = ca->active.as<uint32_t>();
+        D.changes = ca->changes.as<Change>();
+        D.ctrl = ca->ctrl.as<CaCtrl>();
+        D.hist_active = ca->hist_a.as<unsigned long long>();
+        D.hist_changed = ca->hist_c.as<unsigned long long>();
+        D.spill = nullptr;
+        const size_t cols = static_cast<size_t>(d->nx) * d->ny;
+        if (!classic) {
+            ca->spill.alloc(cols * 4);
+            D.spill = ca->spill.as<int32_t>();
+            check(hipMemcpyAsync(ca->spill.p, spill, cols * 4, hipMemcpyHostToDevice, s), "copy spill");
+            DevBuf h;
+            h.alloc(cols * 4);
+            check(hipMemcpyAsync(h.p, height, cols * 4, hipMemcpyHostToDevice, s), "copy height");
+            ca_launch_terrain(P, D, h.as<int32_t>(), s);
+            check(hipStreamSynchronize(s), "terrain");
+        }
+        if (nvents > 0) {
+            DevBuf c, f, t;
+            c.alloc(vcell.size() * 4);
+            f.alloc(vflow.size() * 4);
+            t.alloc(vtemp.size() * 4);
+            check(hipMemcpyAsync(c.p, vcell.data(), vcell.size() * 4, hipMemcpyHostToDevice, s), "copy");
+            check(hipMemcpyAsync(f.p, vflow.data(), vflow.size() * 4, hipMemcpyHostToDevice, s), "copy");
+            check(hipMemcpyAsync(t.p, vtemp.data(), vtemp.size() * 4, hipMemcpyHostToDevice, s), "copy");
+            ca_launch_seeds(P, D, c.as<uint32_t>(), f.as<int32_t>(), t.as<float>(), nvents, s);
+            check(hipStreamSynchronize(s), "seeds");
+        }
+        ca_launch_init_flags(P, D, s);
+        CaCtrl c0;
+        check(hipMemcpyAsync(&c0, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, s), "read ctrl");
+        check(hipStreamSynchronize(s), "init");
+        // countInitialState (LavaCA.hpp:1252-1280)
+        int32_t stop = VCA_RUNNING;
+        if (!classic && c0.lava_total == 0)
+            stop = VCA_NO_LAVA;
+        else if (classic && c0.air_total == 0)
+            stop = VCA_NO_AIR;
+        else if (d->max_steps <= 0)
+            stop = VCA_MAX_STEPS;
+        c0.stop = stop;
+        check(hipMemcpyAsync(ca->ctrl.p, &c0, sizeof(CaCtrl), hipMemcpyHostToDevice, s), "write ctrl");
+        check(hipStreamSynchronize(s), "init");
+        *out = ca.release();
+    });
+}
+
+extern "C" int vca_ca_destroy(vca_ca* ca) {
+    return guarded([&] {
+        if (!ca) return;
+        ca->ctx->set_device();
+        (void)hipStreamSynchronize(ca->ctx->stream);
+        delete ca;
+    });
+}
+
+static void ca_status_sync(vca_ca* ca, int32_t* steps, int32_t* stop) {
+    CaCtrl c;
+    check(hipMemcpyAsync(&c, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, ca->ctx->stream), "read ctrl");
+    check(hipStreamSynchronize(ca->ctx->stream), "ca step");
+    if (steps) *steps = c.steps;
+    if (stop) *stop = c.stop;
+}
+
+extern "C" int vca_ca_enqueue(vca_ca* ca, int32_t new_steps) {
+    return guarded([&] {
+        require(ca, "ca == NULL");
+        ca->ctx->set_device();
+        vca_ctx* c = ca->ctx;
+        for (int k = 0; k < new_steps; ++k) {
+            ca_launch_step(ca->P, ca->D, ca->grid_tiles, c->stream, c->timing ? c->ev : nullptr);
+            if (c->timing) {
+                check(hipEventSynchronize(c->ev[5]), "timing");
+                c->add_time("ca_front", c->ev[0], c->ev[1]);
+                if (!ca->P.classic) c->add_time("ca_descent", c->ev[1], c->ev[2]);
+                c->add_time("ca_rules", c->ev[2], c->ev[3]);
+                c->add_time("ca_apply", c->ev[3], c->ev[4]);
+                c->add_time("ca_finish", c->ev[4], c->ev[5]);
+                c->add_time("ca_step", c->ev[0], c->ev[5]);
+            }
+        }
+        check(hipGetLastError(), "ca launch");
+    });
+}
+
+extern "C" int vca_ca_run(vca_ca* ca, int32_t max_new_steps, int32_t* steps_done, int32_t* stop_reason) {
+    return guarded([&] {
+        require(ca, "ca == NULL");
+        ca->ctx->set_device();
+        int32_t steps = 0, stop = 0;
+        ca_status_sync(ca, &steps, &stop);
+        int remaining = max_new_steps;
+        // Chunks of device-side steps with one host check in between: steps after a stop are no-ops.
+        while (remaining > 0 && stop == VCA_RUNNING) {
+            const int chunk = std::min(remaining, 128);
+            if (vca_ca_enqueue(ca, chunk) != VCA_OK) throw VcaError(VCA_ERR_HIP, g_last_error);
+            remaining -= chunk;
+            ca_status_sync(ca, &steps, &stop);
+        }
+        if (steps_done) *steps_done = steps;
+        if (stop_reason) *stop_reason = stop;
+    });
+}
+
+extern "C" int vca_ca_status(vca_ca* ca, int32_t* steps_done, int32_t* stop_reason) {
+    return guarded([&] {
+        require(ca, "ca == NULL");
+        ca->ctx->set_device();
+        ca_status_sync(ca, steps_done, stop_reason);
+    });
+}
+
+extern "C" int vca_ca_histories(vca_ca* ca, uint64_t* active, uint64_t* changed, int32_t cap, int32_t* n) {
+    return guarded([&] {
+        require(ca, "ca == NULL");
+        ca->ctx->set_device();
+        int32_t steps = 0;
+        ca_status_sync(ca, &steps, nullptr);
+        const int32_t m = std::min(steps, std::max(0, cap));
+        if (m > 0 && active)
+            check(hipMemcpy(active, ca->hist_a.p, static_cast<size_t>(m) * 8, hipMemcpyDeviceToHost), "hist");
+        if (m > 0 && changed)
+            check(hipMemcpy(changed, ca->hist_c.p, static_cast<size_t>(m) * 8, hipMemcpyDeviceToHost), "hist");
+        if (n) *n = steps;
+    });
+}
+
+extern "C" int vca_ca_download(vca_ca* ca, uint8_t* type, int32_t* flow, float* temp, uint16_t* solid,
+                               uint16_t* arrival) {
+    return guarded([&] {
+        require(ca, "ca == NULL");
+        ca->ctx->set_device();
+        check(hipStreamSynchronize(ca->ctx->stream), "sync");
+        const uint64_t n = ca->n;
+        if (type) check(hipMemcpy(type, ca->type.p, n, hipMemcpyDeviceToHost), "download");
+        if (flow) check(hipMemcpy(flow, ca->flow.p, n * 4, hipMemcpyDeviceToHost), "download");
+        if (temp) check(hipMemcpy(temp, ca->temp.p, n * 4, hipMemcpyDeviceToHost), "download");
+        if (solid) check(hipMemcpy(solid, ca->solid.p, n * 2, hipMemcpyDeviceToHost), "download");
+        if (arrival) check(hipMemcpy(arrival, ca->arrival.p, n * 2, hipMemcpyDeviceToHost), "download");
+    });
+}
+
+extern "C" int vca_ca_work(vca_ca* ca, uint64_t* tiles_visited, uint64_t* changes, int32_t* tile_cells) {
+    return guarded([&] {
+        require(ca, "ca == NULL");
+        ca->ctx->set_device();
+        CaCtrl c;
+        check(hipMemcpyAsync(&c, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, ca->ctx->stream), "read ctrl");
+        check(hipStreamSynchronize(ca->ctx->stream), "sync");
+        if (tiles_visited) *tiles_visited = c.tiles_visited;
+        if (changes) *changes = c.changes_total;
+        if (tile_cells) *tile_cells = kTileCells;
+    });
+}
+
+// ===================================================================================================
+// MC
+// ===================================================================================================
+namespace {
+
+// Colour of (x, y, z): buildColors / colorOf (PottsMC.cpp:145-224).
+struct Colouring {
+    bool parity = false;
+    int index[27];
+    int count = 0;
+    int dims[3];
+    int per[3];
+    int axis_class(int c, int a) const {
+        if (per[a] && (dims[a] % 2) == 1 && c == dims[a] - 1) return 2;
+        return c & 1;
+    }
+    int of(int x, int y, int z) const {
+        if (parity) return (x + y + z) & 1;
+        return index[axis_class(x, 0) + 3 * axis_class(y, 1) + 9 * axis_class(z, 2)];
+    }
+};
+
+Colouring make_colouring(int kind, const int dims[3], const int per[3]) {
+    Colouring c;
+    bool odd = false;
+    for (int a = 0; a < 3; ++a) {
+        c.dims[a] = dims[a];
+        c.per[a] = per[a];
+        odd = odd || (per[a] && dims[a] % 2 == 1);
+    }
+    c.parity = kind == VCA_VON_NEUMANN && !odd;
+    std::fill(std::begin(c.index), std::end(c.index), -1);
+    if (c.parity) {
+        c.count = 2;
+        return c;
+    }
+    int classes[3];
+    for (int a = 0; a < 3; ++a) classes[a] = dims[a] == 1 ? 1 : (per[a] && dims[a] % 2 == 1 ? 3 : 2);
+    for (int cz = 0; cz < classes[2]; ++cz)
+        for (int cy = 0; cy < classes[1]; ++cy)
+            for (int cx = 0; cx < classes[0]; ++cx) c.index[cx + 3 * cy + 9 * cz] = c.count++;
+    return c;
+}
+
+// Exact integer form of `toUniform(r) >= m` (skip) for a float mobility m: skip iff (r >> 11) >= thr.
+uint64_t mobility_threshold(float m) {
+    const double two53 = 9007199254740992.0;
+    if (std::isnan(m)) return 1ULL << 53;  // u >= NaN is false: never skipped
+    if (!(m > 0.0f)) return 0;             // u >= m for every u in [0, 1)
+    const double v = std::ceil(static_cast<double>(m) * two53);
+    return v >= two53 ? (1ULL << 53) : static_cast<uint64_t>(v);
+}
+
+}  // namespace
+
+extern "C" int vca_mc_create(vca_ctx* ctx, const vca_mc_desc* d, const uint8_t* domain, const float* mobility,
+                             int32_t mobility_dynamics, vca_mc** out) {
+    return guarded([&] {
+        require(ctx && d && out, "NULL argument");
+        require(d->Q >= 2, "MC wymaga co najmniej 2 orientacji (Q >= 2)");
+        require(d->nx >= 1 && d->ny >= 1 && d->nz >= 1, "wymiary muszą być >= 1");
+        const uint64_t n = static_cast<uint64_t>(d->nx) * d->ny * d->nz;
+        require(n <= 2000000000ULL, "przestrzeń ma za dużo komórek");
+        ctx->set_device();
+        auto mc = std::make_unique<vca_mc>();
+        mc->ctx = ctx;
+        mc->n = n;
+        McParams& P = mc->P;
+        P.nx = d->nx;
+        P.ny = d->ny;
+        P.nz = d->nz;
+        const int dims[3] = {d->nx, d->ny, d->nz};
+        for (int a = 0; a < 3; ++a) P.per[a] = d->periodic[a] && dims[a] > 1;
+        P.seed = d->seed;
+        P.Q = d->Q;
+        P.nb = make_nb(d->neighborhood, d->nx, d->ny, d->nz);
+        mc->checkerboard = d->checkerboard != 0;
+        mc->J = d->J;
+        mc->kT = d->kT;
+        mc->spin_bytes = d->Q <= 255 ? 1 : (d->Q <= 65535 ? 2 : 4);
+
+        // Domain lists: colour-major, global index ascending within a colour (buildColors), or index order
+        // for the sequential random mode (initialize, PottsMC.cpp:104-143).
+        std::vector<uint32_t> list;
+        const Colouring col = make_colouring(d->neighborhood, dims, P.per);
+        if (mc->checkerboard) {
+            mc->colors = col.count;
+            std::vector<std::vector<uint32_t>> per_col(static_cast<size_t>(col.count));
+            uint64_t g = 0;
+            for (int z = 0; z < d->nz; ++z)
+                for (int y = 0; y < d->ny; ++y)
+                    for (int x = 0; x < d->nx; ++x, ++g)
+                        if (!domain || domain[g]) per_col[static_cast<size_t>(col.of(x, y, z))].push_back(static_cast<uint32_t>(g));
+            mc->color_start.push_back(0);
+            for (auto& v : per_col) {
+                list.insert(list.end(), v.begin(), v.end());
+                mc->color_start.push_back(static_cast<uint32_t>(list.size()));
+            }
+        } else {
+            mc->colors = 0;
+            for (uint64_t g = 0; g < n; ++g)
+                if (!domain || domain[g]) list.push_back(static_cast<uint32_t>(g));
+            mc->color_start = {0, static_cast<uint32_t>(list.size())};
+        }
+        mc->ndomain = static_cast<uint32_t>(list.size());
+        hipStream_t s = ctx->stream;
+        mc->spin.alloc(n * static_cast<uint64_t>(mc->spin_bytes));
+        check(hipMemsetAsync(mc->spin.p, 0, n * static_cast<uint64_t>(mc->spin_bytes), s), "memset");
+        mc->list.alloc(list.size() * 4);
+        check(hipMemcpyAsync(mc->list.p, list.data(), list.size() * 4, hipMemcpyHostToDevice, s), "copy list");
+        std::vector<uint64_t> thr;
+        if (mobility) {
+            mc->has_mob_cells = true;
+            mc->mob_cells.alloc(n * 4);
+            check(hipMemcpyAsync(mc->mob_cells.p, mobility, n * 4, hipMemcpyHostToDevice, s), "copy mobility");
+            if (mobility_dynamics) {
+                thr.resize(list.size());
+                for (size_t j = 0; j < list.size(); ++j) thr[j] = mobility_threshold(mobility[list[j]]);
+                mc->mob_thr.alloc(thr.size() * 8);
+                check(hipMemcpyAsync(mc->mob_thr.p, thr.data(), thr.size() * 8, hipMemcpyHostToDevice, s), "copy thr");
+                mc->has_mob_dyn = true;
+            }
+        }
+        // Metropolis table T[before][after] (PottsMC.cpp:431-436): dE = J*after - J*before in double;
+        // accept-always for dE <= 0, else (r >> 11) < ceil(exp(-dE/kT) * 2^53) (exp from the host libm).
+        std::vector<uint64_t> acc(27 * 27, 0);
+        for (int b = 0; b < 27; ++b)
+            for (int a = 0; a < 27; ++a) {
+                const double dE = d->J * a - d->J * b;
+                uint64_t t = 0;
+                if (dE <= 0.0) {
+                    t = kDownhill;
+                } else if (d->kT > 0.0) {
+                    const double v = std::ceil(std::exp(-dE / d->kT) * 9007199254740992.0);
+                    t = v >= 9007199254740992.0 ? (1ULL << 53) : static_cast<uint64_t>(v);
+                }
+                acc[static_cast<size_t>(b) * 27 + a] = t;
+            }
+        mc->accept.alloc(acc.size() * 8);
+        check(hipMemcpyAsync(mc->accept.p, acc.data(), acc.size() * 8, hipMemcpyHostToDevice, s), "copy table");
+        mc->counters.alloc(3 * 8);
+        check(hipMemsetAsync(mc->counters.p, 0, 3 * 8, s), "memset");
+        mc->energy.alloc(8);
+        mc->gout.alloc(16 * 8);
+        mc_launch_init_spins(P, mc->spin.p, mc->spin_bytes, mc->list.as<uint32_t>(), mc->ndomain, s);
+        check(hipStreamSynchronize(s), "mc init");
+        *out = mc.release();
+    });
+}
+
+extern "C" int vca_mc_create_from_ca(vca_ctx* ctx, const vca_mc_desc* d, vca_ca* ca, int32_t use_mobility,
+                                     double mobility_min, vca_mc** out) {
+    return guarded([&] {
+        require(ctx && d && ca && out, "NULL argument");
+        require(ca->P.nx == d->nx && ca->P.ny == d->ny && ca->P.nz == d->nz, "wymiary MC różnią się od CA");
+        const uint64_t n = ca->n;
+        std::vector<uint8_t> type(n);
+        std::vector<uint16_t> solid(n), arrival(n);
+        if (vca_ca_download(ca, type.data(), nullptr, nullptr, solid.data(), arrival.data()) != VCA_OK)
+            throw VcaError(VCA_ERR_HIP, g_last_error);
+        // CoolingMobility (CoolingMobility.cpp:8-129)
+        std::vector<uint8_t> dom(n);
+        uint32_t end = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            dom[i] = (type[i] == VCA_BASALT || type[i] == VCA_LAVA) ? 1 : 0;
+            if (dom[i]) end = std::max<uint32_t>(end, std::max(solid[i], arrival[i]));
+        }
+        std::vector<uint32_t> time(n, 0);
+        uint32_t tmin = 0xFFFFFFFFu, tmax = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            if (!dom[i]) continue;
+            const uint32_t sd = type[i] == VCA_BASALT ? solid[i] : end;
+            time[i] = sd > arrival[i] ? sd - arrival[i] : 0;
+            tmin = std::min(tmin, time[i]);
+            tmax = std::max(tmax, time[i]);
+        }
+        std::vector<float> mob(n, 0.0f);
+        for (uint64_t i = 0; i < n; ++i) {
+            if (!dom[i]) continue;
+            if (tmax == tmin) {
+                mob[i] = 1.0f;
+                continue;
+            }
+            const double norm = double(time[i] - tmin) / double(tmax - tmin);
+            mob[i] = static_cast<float>(mobility_min + (1.0 - mobility_min) * norm);
+        }
+        vca_mc* mc = nullptr;
+        if (vca_mc_create(ctx, d, dom.data(), mob.data(), use_mobility, &mc) != VCA_OK)
+            throw VcaError(VCA_ERR_ARG, g_last_error);
+        *out = mc;
+    });
+}
+
+extern "C" int vca_mc_destroy(vca_mc* mc) {
+    return guarded([&] {
+        if (!mc) return;
+        mc->ctx->set_device();
+        (void)hipStreamSynchronize(mc->ctx->stream);
+        delete mc;
+    });
+}
+
+extern "C" int vca_mc_counts_get(vca_mc* mc, vca_mc_counts* out) {
+    return guarded([&] {
+        require(mc && out, "NULL argument");
+        mc->ctx->set_device();
+        unsigned long long c[3];
+        check(hipMemcpyAsync(c, mc->counters.p, sizeof(c), hipMemcpyDeviceToHost, mc->ctx->stream), "counters");
+        check(hipStreamSynchronize(mc->ctx->stream), "sync");
+        out->attempts = c[0];
+        out->accepted = c[1];
+        out->uphill = c[2];
+        out->sweeps = mc->sweeps;
+    });
+}
+
+extern "C" int vca_mc_sweep(vca_mc* mc, int32_t nsweeps, vca_mc_counts* counts) {
+    return guarded([&] {
+        require(mc && nsweeps >= 0, "invalid argument");
+        vca_ctx* c = mc->ctx;
+        c->set_device();
+        hipStream_t s = c->stream;
+        const uint64_t* thr = mc->has_mob_dyn ? mc->mob_thr.as<uint64_t>() : nullptr;
+        for (int k = 0; k < nsweeps; ++k) {
+            const uint64_t prefix = hash_prefix(mc->P.seed, static_cast<uint64_t>(mc->sweeps));
+            if (c->timing) check(hipEventRecord(c->ev[6], s), "event");
+            if (mc->checkerboard) {
+                for (int col = 0; col < mc->colors; ++col) {
+                    const uint32_t b = mc->color_start[static_cast<size_t>(col)];
+                    const uint32_t e = mc->color_start[static_cast<size_t>(col) + 1];
+                    mc_launch_color(mc->P, mc->spin.p, mc->spin_bytes, mc->list.as<uint32_t>() + b, thr ? thr + b : nullptr,
+                                    e - b, prefix, mc->accept.as<uint64_t>(),
+                                    mc->counters.as<unsigned long long>(), s);
+                }
+            } else {
+                mc_launch_random(mc->P, mc->spin.p, mc->spin_bytes, mc->list.as<uint32_t>(), thr, mc->ndomain, prefix,
+                                 mc->accept.as<uint64_t>(), mc->counters.as<unsigned long long>(), s);
+            }
+            if (c->timing) {
+                check(hipEventRecord(c->ev[7], s), "event");
+                check(hipEventSynchronize(c->ev[7]), "event");
+                c->add_time("mc_sweep", c->ev[6], c->ev[7]);
+            }
+            ++mc->sweeps;
+        }
+        check(hipGetLastError(), "mc launch");
+        if (counts) {
+            if (vca_mc_counts_get(mc, counts) != VCA_OK) throw VcaError(VCA_ERR_HIP, g_last_error);
+        }
+    });
+}
+
+extern "C" int vca_mc_measure(vca_mc* mc, vca_mc_stats* st, vca_mc_class* classes, uint64_t* high_cells) {
+    return guarded([&] {
+        require(mc && st, "NULL argument");
+        mc->ctx->set_device();
+        hipStream_t s = mc->ctx->stream;
+        const uint64_t n = mc->n;
+        if (!mc->parent.p) {
+            mc->parent.alloc(n * 4);
+            mc->size.alloc(n * 4);
+            if (mc->has_mob_cells) mc->hi.alloc(n * 4);
+        }
+        check(hipMemsetAsync(mc->energy.p, 0, 8, s), "memset");
+        check(hipMemsetAsync(mc->gout.p, 0, 16 * 8, s), "memset");
+        const uint64_t prefix = hash_prefix(mc->P.seed, static_cast<uint64_t>(mc->sweeps));
+        mc_launch_energy(mc->P, mc->spin.p, mc->spin_bytes, mc->list.as<uint32_t>(), mc->ndomain, prefix,
+                         mc->energy.as<unsigned long long>(), s);
+        mc_launch_grains(mc->P, mc->spin.p, mc->spin_bytes, mc->list.as<uint32_t>(), mc->ndomain,
+                         mc->parent.as<uint32_t>(), mc->size.as<uint32_t>(),
+                         mc->has_mob_cells ? mc->hi.as<uint32_t>() : nullptr, mc->mob_cells.as<float>(),
+                         mc->gout.as<unsigned long long>(), s);
+        unsigned long long unlike = 0, g[16];
+        check(hipMemcpyAsync(&unlike, mc->energy.p, 8, hipMemcpyDeviceToHost, s), "energy");
+        check(hipMemcpyAsync(g, mc->gout.p, sizeof(g), hipMemcpyDeviceToHost, s), "grains");
+        check(hipStreamSynchronize(s), "measure");
+        st->mcs = mc->sweeps;
+        st->grains = g[0];
+        st->cells = mc->ndomain;
+        st->largest = g[1];
+        st->mean_grain_size = g[0] > 0 ? double(mc->ndomain) / double(g[0]) : 0.0;
+        // totalEnergy: J * P for symmetric neighbourhoods, 0.5 * J * sum for random ones (PottsMC.cpp:642, 663)
+        st->energy = mc->P.nb.variants == 1 ? mc->J * static_cast<double>(unlike) : 0.5 * mc->J * static_cast<double>(unlike);
+        if (classes && mc->has_mob_cells) {
+            classes[0] = vca_mc_class{g[2], g[3], g[4]};
+            classes[1] = vca_mc_class{g[5], g[6], g[7]};
+        }
+        if (high_cells) *high_cells = g[8];
+    });
+}
+
+extern "C" int vca_mc_download(vca_mc* mc, int32_t* crystal) {
+    return guarded([&] {
+        require(mc && crystal, "NULL argument");
+        mc->ctx->set_device();
+        check(hipStreamSynchronize(mc->ctx->stream), "sync");
+        const uint64_t n = mc->n;
+        if (mc->spin_bytes == 4) {
+            check(hipMemcpy(crystal, mc->spin.p, n * 4, hipMemcpyDeviceToHost), "download");
+            return;
+        }
+        std::vector<uint8_t> raw(n * static_cast<uint64_t>(mc->spin_bytes));
+        check(hipMemcpy(raw.data(), mc->spin.p, raw.size(), hipMemcpyDeviceToHost), "download");
+        if (mc->spin_bytes == 1)
+            for (uint64_t i = 0; i < n; ++i) crystal[i] = raw[i];
+        else
+            for (uint64_t i = 0; i < n; ++i) crystal[i] = reinterpret_cast<const uint16_t*>(raw.data())[i];
+    });
+}
+
+extern "C" int vca_mc_download_labels(vca_mc* mc, uint32_t* root) {
+    return guarded([&] {
+        require(mc && root, "NULL argument");
+        require(mc->parent.p != nullptr, "vca_mc_measure musi poprzedzać pobranie etykiet");
+        mc->ctx->set_device();
+        DevBuf tmp;
+        tmp.alloc(mc->n * 4);
+        mc_launch_labels(mc->P, mc->spin.p, mc->spin_bytes, mc->parent.as<uint32_t>(), tmp.as<uint32_t>(),
+                         mc->ctx->stream);
+        check(hipMemcpyAsync(root, tmp.p, mc->n * 4, hipMemcpyDeviceToHost, mc->ctx->stream), "labels");
+        check(hipStreamSynchronize(mc->ctx->stream), "labels");
+    });
+}
+
+extern "C" int vca_mc_info(vca_mc* mc, int32_t* colors, uint64_t* domain_cells) {
+    return guarded([&] {
+        require(mc, "mc == NULL");
+        if (colors) *colors = mc->colors;
+        if (domain_cells) *domain_cells = mc->ndomain;
+    });
+}
+
+extern "C" int vca_mc_download_mobility(vca_mc* mc, float* mobility) {
+    return guarded([&] {
+        require(mc && mobility, "NULL argument");
+        require(mc->has_mob_cells, "brak mobilności");
+        mc->ctx->set_device();
+        check(hipMemcpy(mobility, mc->mob_cells.p, mc->n * 4, hipMemcpyDeviceToHost), "mobility");
+    });
+}

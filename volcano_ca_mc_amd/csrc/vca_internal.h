@@ -1,0 +1,151 @@
+// Internal types shared by the HIP kernels (vca_device.hip) and the C-ABI layer (vca_api.hip).
+//
+// Grid layout in HBM: structure of arrays, unframed, global-index order i = x + nx*(y + ny*z).
+// Boundaries are evaluated by index wrap (periodic axis) or as VOID (absorbing axis) instead of a
+// copied frame (the reference's frame: engine/src/core/Boundary.hpp:66-131).
+//
+// Neighbourhoods are bitmasks over the 27 positions of a 3x3x3 block, position
+//   k = (dx+1) + 3*(dy+1) + 9*(dz+1)      (k = 13 is the cell itself),
+// one mask per variant, variants in the reference order (engine/src/core/Neighborhood.cpp:55-119).
+// The order of offsets inside a variant does not matter on the device: the flowId vote, the maximum
+// temperature and the Potts unlike-counts are all order independent.  k > 13 is exactly the
+// reference's "forward" half (positive linear offset, engine/src/mc/PottsMC.cpp:599-604).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace vca {
+
+constexpr uint8_t kAir = 0, kRock = 1, kLava = 2, kBasalt = 3;
+constexpr int kMaxDrop = 16;  // engine/src/ca/LavaCA.hpp:1293
+
+// Salts (engine/src/core/Rng.hpp:22-31)
+constexpr uint64_t kSaltNeighborhood = 0x4E45494748424F52ULL;
+constexpr uint64_t kSaltTieBreak = 0x5449455F42524B31ULL;
+constexpr uint64_t kSaltMcInit = 0x4D43494E49544F52ULL;
+constexpr uint64_t kSaltMcPick = 0x4D435049434B4345ULL;
+constexpr uint64_t kSaltMcMobility = 0x4D434D4F42494C49ULL;
+constexpr uint64_t kSaltMcOrient = 0x4D434F5249454E54ULL;
+constexpr uint64_t kSaltMcAccept = 0x4D43414343455054ULL;
+
+__host__ __device__ inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+// hashPrefix / hashIndex / hashSalt (engine/src/core/Rng.hpp:66-72)
+__host__ __device__ inline uint64_t hash_prefix(uint64_t seed, uint64_t step) {
+    return splitmix64(splitmix64(seed ^ 0xA0761D6478BD642FULL) ^ step);
+}
+
+struct NbTable {
+    int32_t variants = 1;
+    uint32_t mask[6] = {0, 0, 0, 0, 0, 0};
+    int32_t max_size = 0;
+    uint32_t any_mask = 0;        // union over variants
+    uint32_t side_mask = 0;       // union over variants of dz == 0 positions (D(n), LavaCA.hpp:102-112)
+    int32_t diagonal_falls = 0;   // some variant has dz == -1 with dx|dy != 0
+    uint32_t conn_mask = 0;       // grain connectivity: Moore, or von Neumann when the kind is von Neumann
+};
+
+// Tile = 64 x TY x TZ cells (TY*TZ = 16): one 256-thread workgroup, 4 cells per thread.
+constexpr int kTileX = 64;
+constexpr int kTileCells = 1024;
+constexpr int kBlock = 256;
+
+struct CaParams {
+    int32_t nx, ny, nz;
+    int32_t classic;
+    int32_t per[3];
+    uint64_t seed;
+    int32_t max_steps, eruption_steps;
+    float erupt_temp, solidus, cool_amb, cool_lat, cool_fall;
+    int32_t ponding;
+    int32_t dense;
+    NbTable nb;
+    int32_t ty, tz;               // tile extent in y and z
+    int32_t ntx, nty, ntz;        // tiles per axis
+    uint32_t ntiles;
+    int32_t nvent_slots;
+};
+
+// One change of one cell in one step (applied after every rule of the step was evaluated).
+struct Change {
+    uint32_t idx;
+    int32_t flow;
+    float temp;
+    uint16_t stamp;
+    uint8_t type;
+    uint8_t kind;                 // kTemp, kSolidify, kBorn
+};
+constexpr uint8_t kKindTemp = 0, kKindSolidify = 1, kKindBorn = 2;
+
+struct CaCtrl {
+    int32_t steps;
+    int32_t stop;
+    uint32_t nactive;
+    uint32_t nchanges;
+    unsigned long long lava_total, air_total;
+    unsigned long long lava_after, born, changed;
+    unsigned long long tiles_visited, changes_total;
+};
+
+struct CaDev {
+    uint8_t* type;
+    int32_t* flow;
+    float* temp;
+    uint16_t* solid;
+    uint16_t* arrival;
+    int8_t* descent;
+    int32_t* spill;               // nx*ny, nullptr in classic mode
+    uint32_t* vent_cell;          // by flowId, 0xFFFFFFFF = none
+    uint8_t* tile_flags;          // bit0: has lava, bit1: has air
+    uint32_t* active;             // active tile list
+    Change* changes;
+    CaCtrl* ctrl;
+    unsigned long long* hist_active;
+    unsigned long long* hist_changed;
+};
+constexpr uint8_t kFlagLava = 1, kFlagAir = 2;
+
+struct McParams {
+    int32_t nx, ny, nz;
+    int32_t per[3];
+    uint64_t seed;
+    int32_t Q;
+    NbTable nb;
+};
+
+constexpr uint64_t kDownhill = ~0ULL;  // accept-table entry for dE <= 0
+
+// ---- launchers (vca_device.hip) ----
+void ca_launch_terrain(const CaParams& P, const CaDev& D, const int32_t* height, hipStream_t s);
+void ca_launch_seeds(const CaParams& P, const CaDev& D, const uint32_t* cells, const int32_t* flows,
+                     const float* temps, int n, hipStream_t s);
+void ca_launch_init_flags(const CaParams& P, const CaDev& D, hipStream_t s);
+// One step: front list, D(n), rules, apply, finish.  ev != nullptr records 8 events around the kernels.
+void ca_launch_step(const CaParams& P, const CaDev& D, int grid_tiles, hipStream_t s, hipEvent_t* ev);
+
+// Colour pass of the checkerboard sweep.  spin_bytes: 1, 2 or 4.
+void mc_launch_color(const McParams& P, void* spin, int spin_bytes, const uint32_t* list, const uint64_t* mob_thr,
+                     uint32_t count, uint64_t prefix, const uint64_t* accept, unsigned long long* counters,
+                     hipStream_t s);
+void mc_launch_random(const McParams& P, void* spin, int spin_bytes, const uint32_t* list, const uint64_t* mob_thr,
+                      uint32_t count, uint64_t prefix, const uint64_t* accept, unsigned long long* counters,
+                      hipStream_t s);
+void mc_launch_init_spins(const McParams& P, void* spin, int spin_bytes, const uint32_t* list, uint32_t count,
+                          hipStream_t s);
+void mc_launch_energy(const McParams& P, const void* spin, int spin_bytes, const uint32_t* list, uint32_t count,
+                      uint64_t prefix, unsigned long long* unlike, hipStream_t s);
+// Grain labels: parent (N), size (N), hi (N, nullable); out[0..] = grains, largest, class counters.
+void mc_launch_grains(const McParams& P, const void* spin, int spin_bytes, const uint32_t* list, uint32_t count,
+                      uint32_t* parent, uint32_t* size, uint32_t* hi, const float* mobility,
+                      unsigned long long* out, hipStream_t s);
+void mc_launch_download(const McParams& P, const void* spin, int spin_bytes, int32_t* out, hipStream_t s);
+void mc_launch_labels(const McParams& P, const void* spin, int spin_bytes, const uint32_t* parent, uint32_t* out,
+                      hipStream_t s);
+
+}  // namespace vca
