@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark of the GPU engine on the 3D cone workload (one MI355X).
+
+Workload: 512 x 512 x 256 voxels, radial cone 220 -> 20 at r = 250 plus 4-octave
+value noise (amplitude 12), three craters within r = 30 of the summit, Moore neighbourhood, absorbing
+boundaries, up to 1000 lava-CA steps, then 200 checkerboard Potts sweeps (Q 64, kT 0.5) on the basalt.
+Synthetic, seeded terrain (volcano_ca_mc_amd/workloads.py).
+
+One bench "step" = one complete job of that workload through the drop-in `engine` binary (C++ host +
+HIP kernels): the CA runs to its stop rule, then the MC sweeps.  The timed regions are the engine's
+caSim (all CA steps, state resident in HBM) and mcSim (all sweeps); terrain, allocation, MC set-up,
+statistics and file output are outside them (output.format = none).
+
+Printed: one JSON line.  value = CA cell-updates/s = cells x executed steps / caSim (the reference's
+"ns per cell and step" unit); mc_value = MC spin-attempts/s = domain cells x sweeps / mcSim.
+roofline: the dominant kernel's algorithmic bytes per launch / its mean launch time from HIP events
+recorded on the engine's stream during the timed runs.  cpu_baseline: the reference engine
+(oracle/_ref/engine, built from the reference sources) on the host cores, on a bounded sample
+(first 300 CA steps, 50 sweeps) of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+ENGINE = os.path.join(ROOT, "volcano_ca_mc_amd", "_lib", "engine")
+REF_ENGINE = os.path.join(ROOT, "oracle", "_ref", "engine")
+HBM_PEAK_GBS = 8000.0
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_engine(exe: str, cfg_path: str, out: str, extra, timeout: int = 1800) -> dict:
+    t0 = time.time()
+    r = subprocess.run([exe, "--config", cfg_path, "--out", out, *extra], capture_output=True, text=True,
+                       timeout=timeout, cwd=os.path.dirname(cfg_path))
+    wall = time.time() - t0
+    if r.returncode != 0:
+        raise RuntimeError(f"{exe} failed ({r.returncode}): {r.stderr[-2000:]}")
+    with open(os.path.join(out, "times.json")) as f:
+        times = json.load(f)
+    times["_wall_s"] = wall
+    return times
+
+
+# Algorithmic bytes of one launch of each kernel (what the launch must move at minimum), from the
+# work counters the engine reports.  Per visited cell the rules read its type byte; per lava cell they
+# also read flowId and temperature (4 + 4 B); each change writes a 16 B record.  The apply kernel reads
+# every record (16 B) and writes the changed fields (type 1 + temp 4 + one of flow/stamp ~4 B ~ 9 B).
+def kernel_bytes(times: dict) -> dict:
+    d = times.get("detail", {})
+    w = d.get("caWork", {})
+    k = d.get("kernels", {})
+    out = {}
+    steps = max(1, times.get("steps", 1))
+    visited_cells = w.get("tilesVisited", 0) * w.get("tileCells", 0)
+    lava_steps = w.get("lavaCellSteps", 0)
+    changes = w.get("changes", 0)
+    if "ca_rules" in k:
+        out["ca_rules"] = (1 * visited_cells + 8 * lava_steps + 16 * changes) / steps
+    if "ca_apply" in k:
+        out["ca_apply"] = (16 + 9) * changes / steps
+    if "ca_descent" in k:
+        out["ca_descent"] = (1 * visited_cells + 1 * lava_steps) / steps
+    if "mc_color" in k:
+        # per trial: list entry 4 B + mobility threshold 8 B + own spin 1 B + spin write 1 B (upper bound)
+        dom = times.get("domainCells", 0)
+        out["mc_color"] = dom * (4 + 8 + 1 + 1) * times.get("mcs", 0) / max(1, k["mc_color"]["launches"])
+    return out
+
+
+def cpu_baseline(cfg_path: str, cells: int, tmp: str) -> dict | None:
+    if not os.path.exists(REF_ENGINE):
+        return None
+    threads = min(16, os.cpu_count() or 1)
+    out = os.path.join(tmp, "cpu")
+    log(f"cpu baseline: reference engine, {threads} threads, first 300 CA steps + 50 sweeps")
+    t = run_engine(REF_ENGINE, cfg_path, out, ["--threads", str(threads), "--set", "ca.maxSteps=300",
+                                               "--set", "mc.mcs=50", "--set", "output.format=none"])
+    ca_rate = cells * t["steps"] / (t["caSim"] / 1000.0)
+    mc_rate = t["domainCells"] * t["mcs"] / (t["mcSim"] / 1000.0) if t.get("mcSim") else None
+    return {"value": ca_rate, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
+            "mc_value": mc_rate, "mc_unit": "spin-attempts/s",
+            "sample": f"reference engine on the same workload, first {t['steps']} CA steps and {t['mcs']} Potts "
+                      f"sweeps ({t['domainCells']} domain cells); caSim {t['caSim']:.0f} ms, mcSim {t['mcSim']:.0f} ms"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed jobs")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed jobs")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1.0 = the benchmark workload)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": a.gpus}))
+        return 2
+    if not os.path.exists(ENGINE):
+        raise SystemExit(f"{ENGINE} missing: run `python -m volcano_ca_mc_amd.build` first")
+    from volcano_ca_mc_amd import workloads as W
+    n = max(16, int(round(512 * a.scale)))
+    wl = W.cone_3d(n=n, nz=max(16, int(round(256 * a.scale))))
+    nx, ny, nz = wl.dims
+    cells = nx * ny * nz
+    with tempfile.TemporaryDirectory(prefix="vca_bench_") as tmp:
+        cfg_path = wl.write(tmp)
+        extra = ["--timing-detail", "--set", "output.format=none"]
+        runs = []
+        for i in range(a.warmup + a.steps):
+            t = run_engine(ENGINE, cfg_path, os.path.join(tmp, f"run{i}"), extra)
+            tag = "warmup" if i < a.warmup else "timed"
+            log(f"{tag} job {i}: steps {t['steps']} caSim {t['caSim']:.1f} ms mcSim {t['mcSim']:.1f} ms "
+                f"total {t['total']:.1f} ms")
+            if i >= a.warmup:
+                runs.append(t)
+        ca_ms = [t["caSim"] for t in runs]
+        mc_ms = [t["mcSim"] for t in runs]
+        steps = runs[0]["steps"]
+        dom = runs[0]["domainCells"]
+        mcs = runs[0]["mcs"]
+        ca_rate = cells * steps / (statistics.median(ca_ms) / 1000.0)
+        mc_rate = dom * mcs / (statistics.median(mc_ms) / 1000.0)
+        lat = sorted(t["total"] for t in runs)
+        p50 = statistics.median(lat)
+        p99 = lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))]
+        # roofline of the dominant kernel (largest summed device time across the timed jobs)
+        ktot = {}
+        for t in runs:
+            for name, v in t.get("detail", {}).get("kernels", {}).items():
+                if name in ("ca_step", "mc_sweep"):
+                    continue
+                e = ktot.setdefault(name, [0.0, 0])
+                e[0] += v["ms"]
+                e[1] += v["launches"]
+        bytes_per_launch = kernel_bytes(runs[0])
+        dom_k = max((k for k in ktot if k in bytes_per_launch), key=lambda k: ktot[k][0])
+        mean_ms = ktot[dom_k][0] / ktot[dom_k][1]
+        achieved = bytes_per_launch[dom_k] / (mean_ms / 1000.0) / 1e9
+        traffic = None
+        if os.path.exists(PROFILE_TRAFFIC):
+            with open(PROFILE_TRAFFIC) as f:
+                traffic = json.load(f).get(dom_k)
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_k,
+                    "kernel_mean_ms": round(mean_ms, 5), "algorithmic_bytes_per_launch": round(bytes_per_launch[dom_k]),
+                    "kernel_share": {k: round(v[0] / sum(x[0] for x in ktot.values()), 3) for k, v in ktot.items()}}
+        cpu = None if a.no_cpu else cpu_baseline(cfg_path, cells, tmp)
+    result = {
+        "metric": "CA cell-updates/s + MC spin-attempts/s (bit-exact); p50/p99 job latency",
+        "value": ca_rate,
+        "unit": "cell-updates/s",
+        "mc_value": mc_rate,
+        "mc_unit": "spin-attempts/s",
+        "p50_job_ms": p50,
+        "p99_job_ms": p99,
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": statistics.median(ca_ms) + statistics.median(mc_ms),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8/i32/f32 (integer and float32 rules, bit-exact)",
+        "data": "synthetic (seeded terrain)",
+        "config": {"workload": f"cone_3d {nx}x{ny}x{nz} moore absorbing, 3 craters, maxSteps 1000, "
+                               f"Potts Q64 kT0.5 200 sweeps (checkerboard)", "cells": cells, "ca_steps": steps,
+                   "mc_domain_cells": dom, "mc_sweeps": mcs, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

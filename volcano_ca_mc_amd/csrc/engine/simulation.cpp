@@ -316,6 +316,7 @@ RunSummary run_simulation(const Config& cfg, const std::string& out_dir, Timing&
         s.changed.assign(static_cast<size_t>(steps), 0);
         int32_t n = 0;
         vca_check(vca_ca_histories(ca.h, s.active.data(), s.changed.data(), steps, &n), "CA");
+        vca_check(vca_ca_work(ca.h, &s.tiles_visited, &s.changes, &s.tile_cells), "CA");
         s.result.resize(cfg.space);
         download(ca.h, s.result);
         Clock tf;
@@ -660,6 +661,35 @@ int run_from_cli(const CliOptions& opt, const Clock& since_start) {
     }
     fs::create_directories(out_dir);
     Gpu gpu;
+    if (opt.timing_detail) vca_check(vca_ctx_set_timing(gpu.ctx(), 1), "GPU");
+    // --timing-detail: device time of every kernel (HIP events on the engine's stream) and CA work counters
+    auto detail = [&](const RunSummary* s) {
+        if (!opt.timing_detail) return;
+        Json d = Json::object();
+        d["unit"] = "ms";
+        Json k = Json::object();
+        for (const char* name : {"ca_front", "ca_descent", "ca_rules", "ca_apply", "ca_finish", "ca_step", "mc_color", "mc_sweep"}) {
+            vca_kernel_time kt{};
+            vca_check(vca_ctx_kernel_time(gpu.ctx(), name, &kt), "GPU");
+            if (kt.launches == 0) continue;
+            Json e = Json::object();
+            e["ms"] = Json(kt.ms);
+            e["launches"] = Json(static_cast<unsigned long long>(kt.launches));
+            k[name] = e;
+        }
+        d["kernels"] = k;
+        if (s && s->ca_ran) {
+            Json w = Json::object();
+            w["tilesVisited"] = Json(static_cast<unsigned long long>(s->tiles_visited));
+            w["tileCells"] = s->tile_cells;
+            w["changes"] = Json(static_cast<unsigned long long>(s->changes));
+            uint64_t lava = 0;
+            for (uint64_t a : s->active) lava += a;
+            w["lavaCellSteps"] = Json(static_cast<unsigned long long>(lava));
+            d["caWork"] = w;
+        }
+        timing.set_info("detail", d);
+    };
 
     if (!opt.mc_from.empty()) {
         // engine --mc-from: MC on a saved CA result (Simulation.cpp:334-402)
@@ -722,6 +752,7 @@ int run_from_cli(const CliOptions& opt, const Clock& since_start) {
         timing.set_info("mcFrom", opt.mc_from);
         mc_info(timing, mc);
         env_info(timing, gpu);
+        detail(nullptr);
         timing.record_files();
         write_json((out / "times.json").string(), timing.to_json());
         for (const std::string& w : mc.warnings) std::cerr << "Ostrzeżenie: " << w << '\n';
@@ -746,6 +777,7 @@ int run_from_cli(const CliOptions& opt, const Clock& since_start) {
     }
     if (s.mc_ran) mc_info(timing, s.mc);
     env_info(timing, gpu);
+    detail(&s);
     timing.record_files();
     write_json((fs::path(out_dir) / "times.json").string(), timing.to_json());
     for (const std::string& w : s.mc.warnings) std::cerr << "Ostrzeżenie: " << w << '\n';

@@ -107,6 +107,8 @@ struct RunSummary {
     int stop = 0;
     std::vector<Vent> vents;
     std::vector<uint64_t> active, changed;
+    uint64_t tiles_visited = 0, changes = 0;
+    int32_t tile_cells = 0;
     CaState result;
     bool mc_ran = false;
     McSummary mc;

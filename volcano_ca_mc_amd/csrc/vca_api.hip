@@ -173,15 +173,46 @@ struct vca_ctx {
     bool own_stream = false;
     bool timing = false;
     std::map<std::string, KernelTime> times;
-    hipEvent_t ev[8] = {};
+    // Event pool for per-launch timing: events are recorded around launches without any host wait and
+    // drained (elapsed times summed per kernel name) at the next point where the host synchronises anyway.
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    struct Pending {
+        const char* name;
+        size_t a, b;
+    };
+    std::vector<Pending> pending;
 
     void set_device() const { check(hipSetDevice(device), "hipSetDevice"); }
-    void add_time(const char* name, hipEvent_t a, hipEvent_t b) {
-        float ms = 0.0f;
-        check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
-        KernelTime& k = times[name];
-        k.ms += ms;
-        k.launches += 1;
+    hipEvent_t take() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            check(hipEventCreate(&e), "hipEventCreate");
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    size_t record() {
+        hipEvent_t e = take();
+        check(hipEventRecord(e, stream), "hipEventRecord");
+        return used - 1;
+    }
+    void mark(const char* name, size_t a, size_t b) { pending.push_back(Pending{name, a, b}); }
+    void drain() {
+        if (pending.empty()) {
+            used = 0;
+            return;
+        }
+        check(hipEventSynchronize(pool[used - 1]), "hipEventSynchronize");
+        for (const Pending& p : pending) {
+            float ms = 0.0f;
+            check(hipEventElapsedTime(&ms, pool[p.a], pool[p.b]), "hipEventElapsedTime");
+            KernelTime& k = times[p.name];
+            k.ms += ms;
+            k.launches += 1;
+        }
+        pending.clear();
+        used = 0;
     }
 };
 
@@ -243,7 +274,6 @@ extern "C" int vca_ctx_create(int device, void* stream, vca_ctx** out) {
             check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
             c->own_stream = true;
         }
-        for (auto& e : c->ev) check(hipEventCreate(&e), "hipEventCreate");
         *out = c;
     });
 }
@@ -253,7 +283,7 @@ extern "C" int vca_ctx_destroy(vca_ctx* ctx) {
         if (!ctx) return;
         ctx->set_device();
         (void)hipStreamSynchronize(ctx->stream);
-        for (auto& e : ctx->ev) (void)hipEventDestroy(e);
+        for (auto& e : ctx->pool) (void)hipEventDestroy(e);
         if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
     });
@@ -280,13 +310,18 @@ extern "C" int vca_device_memory(vca_ctx* ctx, uint64_t* free_bytes, uint64_t* t
 extern "C" int vca_ctx_set_timing(vca_ctx* ctx, int32_t on) {
     return guarded([&] {
         require(ctx, "ctx == NULL");
+        ctx->set_device();
+        ctx->drain();
         ctx->timing = on != 0;
+        if (ctx->timing) ctx->times.clear();
     });
 }
 
 extern "C" int vca_ctx_kernel_time(vca_ctx* ctx, const char* name, vca_kernel_time* out) {
     return guarded([&] {
         require(ctx && name && out, "NULL argument");
+        ctx->set_device();
+        ctx->drain();
         auto it = ctx->times.find(name);
         out->ms = it == ctx->times.end() ? 0.0 : it->second.ms;
         out->launches = it == ctx->times.end() ? 0 : it->second.launches;
@@ -465,6 +500,7 @@ static void ca_status_sync(vca_ca* ca, int32_t* steps, int32_t* stop) {
     CaCtrl c;
     check(hipMemcpyAsync(&c, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, ca->ctx->stream), "read ctrl");
     check(hipStreamSynchronize(ca->ctx->stream), "ca step");
+    ca->ctx->drain();
     if (steps) *steps = c.steps;
     if (stop) *stop = c.stop;
 }
@@ -475,16 +511,24 @@ extern "C" int vca_ca_enqueue(vca_ca* ca, int32_t new_steps) {
         ca->ctx->set_device();
         vca_ctx* c = ca->ctx;
         for (int k = 0; k < new_steps; ++k) {
-            ca_launch_step(ca->P, ca->D, ca->grid_tiles, c->stream, c->timing ? c->ev : nullptr);
-            if (c->timing) {
-                check(hipEventSynchronize(c->ev[5]), "timing");
-                c->add_time("ca_front", c->ev[0], c->ev[1]);
-                if (!ca->P.classic) c->add_time("ca_descent", c->ev[1], c->ev[2]);
-                c->add_time("ca_rules", c->ev[2], c->ev[3]);
-                c->add_time("ca_apply", c->ev[3], c->ev[4]);
-                c->add_time("ca_finish", c->ev[4], c->ev[5]);
-                c->add_time("ca_step", c->ev[0], c->ev[5]);
+            if (!c->timing) {
+                ca_launch_step(ca->P, ca->D, ca->grid_tiles, c->stream, nullptr);
+                continue;
             }
+            hipEvent_t ev[6];
+            size_t idx[6];
+            for (int e = 0; e < 6; ++e) {
+                ev[e] = c->take();
+                idx[e] = c->used - 1;
+            }
+            ca_launch_step(ca->P, ca->D, ca->grid_tiles, c->stream, ev);
+            c->mark("ca_front", idx[0], idx[1]);
+            if (!ca->P.classic) c->mark("ca_descent", idx[1], idx[2]);
+            c->mark("ca_rules", idx[2], idx[3]);
+            c->mark("ca_apply", idx[3], idx[4]);
+            c->mark("ca_finish", idx[4], idx[5]);
+            c->mark("ca_step", idx[0], idx[5]);
+            if (c->used > 6 * 256) c->drain();  // bound the pool on long enqueues
         }
         check(hipGetLastError(), "ca launch");
     });
@@ -769,6 +813,7 @@ extern "C" int vca_mc_counts_get(vca_mc* mc, vca_mc_counts* out) {
         unsigned long long c[3];
         check(hipMemcpyAsync(c, mc->counters.p, sizeof(c), hipMemcpyDeviceToHost, mc->ctx->stream), "counters");
         check(hipStreamSynchronize(mc->ctx->stream), "sync");
+        mc->ctx->drain();
         out->attempts = c[0];
         out->accepted = c[1];
         out->uphill = c[2];
@@ -785,23 +830,24 @@ extern "C" int vca_mc_sweep(vca_mc* mc, int32_t nsweeps, vca_mc_counts* counts) 
         const uint64_t* thr = mc->has_mob_dyn ? mc->mob_thr.as<uint64_t>() : nullptr;
         for (int k = 0; k < nsweeps; ++k) {
             const uint64_t prefix = hash_prefix(mc->P.seed, static_cast<uint64_t>(mc->sweeps));
-            if (c->timing) check(hipEventRecord(c->ev[6], s), "event");
+            const size_t t0 = c->timing ? c->record() : 0;
             if (mc->checkerboard) {
                 for (int col = 0; col < mc->colors; ++col) {
                     const uint32_t b = mc->color_start[static_cast<size_t>(col)];
                     const uint32_t e = mc->color_start[static_cast<size_t>(col) + 1];
+                    const size_t k0 = c->timing ? c->record() : 0;
                     mc_launch_color(mc->P, mc->spin.p, mc->spin_bytes, mc->list.as<uint32_t>() + b, thr ? thr + b : nullptr,
                                     e - b, prefix, mc->accept.as<uint64_t>(),
                                     mc->counters.as<unsigned long long>(), s);
+                    if (c->timing && e > b) c->mark("mc_color", k0, c->record());
                 }
             } else {
                 mc_launch_random(mc->P, mc->spin.p, mc->spin_bytes, mc->list.as<uint32_t>(), thr, mc->ndomain, prefix,
                                  mc->accept.as<uint64_t>(), mc->counters.as<unsigned long long>(), s);
             }
             if (c->timing) {
-                check(hipEventRecord(c->ev[7], s), "event");
-                check(hipEventSynchronize(c->ev[7]), "event");
-                c->add_time("mc_sweep", c->ev[6], c->ev[7]);
+                c->mark("mc_sweep", t0, c->record());
+                if (c->used > 4096) c->drain();
             }
             ++mc->sweeps;
         }

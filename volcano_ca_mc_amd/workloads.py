@@ -345,3 +345,15 @@ SCENARIOS = {
     "cone_3d": cone_3d,
     "caldera_3d": caldera_3d,
 }
+
+
+def _main(argv):
+    """python -m volcano_ca_mc_amd.workloads write <scenario> <folder>: writes config JSON + CSV."""
+    if len(argv) != 3 or argv[0] != "write" or argv[1] not in SCENARIOS:
+        raise SystemExit("usage: python -m volcano_ca_mc_amd.workloads write {%s} <folder>" % "|".join(SCENARIOS))
+    print(SCENARIOS[argv[1]]().write(argv[2]))
+
+
+if __name__ == "__main__":
+    import sys
+    _main(sys.argv[1:])
