@@ -163,9 +163,11 @@ typedef struct {
     double ms;          /* summed device time of the kernel */
     uint64_t launches;  /* launches timed */
 } vca_kernel_time;
-/* Enables per-launch HIP-event timing of the dominant kernels (adds a host sync per step). */
+/* Enables per-launch HIP-event timing of every kernel of the CA step and the MC sweep.  Events come from a
+ * pool and are read back at the synchronisation points the calls already have (no extra host sync). */
 int vca_ctx_set_timing(vca_ctx* ctx, int32_t on);
-/* name: "ca_rules", "ca_descent", "ca_apply", "ca_front", "mc_sweep" */
+/* name: "ca_front", "ca_descent", "ca_rules", "ca_wide", "ca_apply", "ca_finish", "ca_step" (whole step),
+ * "mc_color" (one colour launch), "mc_sweep" (whole sweep) */
 int vca_ctx_kernel_time(vca_ctx* ctx, const char* name, vca_kernel_time* out);
 
 #ifdef __cplusplus

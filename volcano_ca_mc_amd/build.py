@@ -19,6 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIB_DIR, "libvca.so")
 ENGINE = os.path.join(LIB_DIR, "engine")
+CALIB = os.path.join(LIB_DIR, "calib_copy")
 INCLUDE = os.path.join(ROOT, "include")
 ARCH = "gfx950"
 
@@ -82,10 +83,19 @@ def build_engine() -> str:
     return ENGINE
 
 
+def build_calib() -> str:
+    """Byte-counter calibration program (a plain copy of known size) for rocprofv3 --pmc runs."""
+    tmp = CALIB + ".tmp"
+    _run([hipcc(), *HIP_FLAGS, os.path.join(CSRC, "calib_copy.hip"), "-o", tmp])
+    os.replace(tmp, CALIB)
+    return CALIB
+
+
 def build(engine: bool = True) -> None:
     build_lib()
     if engine:
         build_engine()
+    build_calib()
 
 
 if __name__ == "__main__":

@@ -668,7 +668,8 @@ int run_from_cli(const CliOptions& opt, const Clock& since_start) {
         Json d = Json::object();
         d["unit"] = "ms";
         Json k = Json::object();
-        for (const char* name : {"ca_front", "ca_descent", "ca_rules", "ca_apply", "ca_finish", "ca_step", "mc_color", "mc_sweep"}) {
+        for (const char* name : {"ca_front", "ca_descent", "ca_rules", "ca_wide", "ca_apply", "ca_finish", "ca_step",
+                                  "mc_color", "mc_sweep"}) {
             vca_kernel_time kt{};
             vca_check(vca_ctx_kernel_time(gpu.ctx(), name, &kt), "GPU");
             if (kt.launches == 0) continue;

@@ -220,7 +220,8 @@ struct vca_ca {
     vca_ctx* ctx = nullptr;
     CaParams P{};
     CaDev D{};
-    DevBuf type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, active, changes, ctrl, hist_a, hist_c;
+    DevBuf type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, active, changes, tile_nchg, overflow,
+        part, ctrl, hist_a, hist_c;
     int grid_tiles = 1;
     uint64_t n = 0;
 };
@@ -414,7 +415,10 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         ca->vent_cell.alloc(vent_table.size() * 4);
         ca->tile_flags.alloc(P.ntiles);
         ca->active.alloc(static_cast<size_t>(P.ntiles) * 4);
-        ca->changes.alloc(n * sizeof(Change));
+        ca->changes.alloc(static_cast<size_t>(P.ntiles) * kTileCells * sizeof(Change));
+        ca->tile_nchg.alloc(static_cast<size_t>(P.ntiles) * 4);
+        ca->part.alloc(static_cast<size_t>(ca->grid_tiles + kWideBlocks) * sizeof(CaPart));
+        ca->overflow.alloc(n * sizeof(uint32_t));
         ca->ctrl.alloc(sizeof(CaCtrl));
         const size_t hcap = static_cast<size_t>(std::max(1, d->max_steps));
         ca->hist_a.alloc(hcap * 8);
@@ -442,6 +446,9 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         D.tile_flags = ca->tile_flags.as<uint8_t>();
         D.active = ca->active.as<uint32_t>();
         D.changes = ca->changes.as<Change>();
+        D.overflow = ca->overflow.as<uint32_t>();
+        D.tile_nchg = ca->tile_nchg.as<uint32_t>();
+        D.part = ca->part.as<CaPart>();
         D.ctrl = ca->ctrl.as<CaCtrl>();
         D.hist_active = ca->hist_a.as<unsigned long long>();
         D.hist_changed = ca->hist_c.as<unsigned long long>();
@@ -515,9 +522,9 @@ extern "C" int vca_ca_enqueue(vca_ca* ca, int32_t new_steps) {
                 ca_launch_step(ca->P, ca->D, ca->grid_tiles, c->stream, nullptr);
                 continue;
             }
-            hipEvent_t ev[6];
-            size_t idx[6];
-            for (int e = 0; e < 6; ++e) {
+            hipEvent_t ev[7];
+            size_t idx[7];
+            for (int e = 0; e < 7; ++e) {
                 ev[e] = c->take();
                 idx[e] = c->used - 1;
             }
@@ -525,10 +532,11 @@ extern "C" int vca_ca_enqueue(vca_ca* ca, int32_t new_steps) {
             c->mark("ca_front", idx[0], idx[1]);
             if (!ca->P.classic) c->mark("ca_descent", idx[1], idx[2]);
             c->mark("ca_rules", idx[2], idx[3]);
-            c->mark("ca_apply", idx[3], idx[4]);
-            c->mark("ca_finish", idx[4], idx[5]);
-            c->mark("ca_step", idx[0], idx[5]);
-            if (c->used > 6 * 256) c->drain();  // bound the pool on long enqueues
+            c->mark("ca_wide", idx[3], idx[4]);
+            c->mark("ca_apply", idx[4], idx[5]);
+            c->mark("ca_finish", idx[5], idx[6]);
+            c->mark("ca_step", idx[0], idx[6]);
+            if (c->used > 7 * 256) c->drain();  // bound the pool on long enqueues
         }
         check(hipGetLastError(), "ca launch");
     });

@@ -1,6 +1,6 @@
 // HIP kernels of the lava CA step and the Potts/Metropolis sweep, written for gfx950 (CDNA4).
 //
-// CA step (one synchronous step of engine/src/ca/LavaCA.hpp) = five stream-ordered launches:
+// CA step (one synchronous step of engine/src/ca/LavaCA.hpp) = six stream-ordered launches:
 //   ca_front   : active tile list — a 64x4x4 (3D) / 64x1x16 (x-z) tile is visited when it holds lava,
 //                or holds AIR next to a tile with lava (the reference's "may change" set,
 //                engine/src/ca/CaRules.hpp:69-79, at tile granularity).  Wave ballot + one atomic per
@@ -9,6 +9,7 @@
 //   ca_rules   : the rules of every visited cell read the start-of-step state only and append the cell's
 //                new state to a change list (wave-aggregated append), count lava/born/changed, and set the
 //                tile's lava/air flags from the new state (LavaCA.hpp:963-1048, 1109-1202).
+//   ca_wide    : the few AIR cells whose vote has more than four distinct flowIds (full vote).
 //   ca_apply   : scatters the change list into the state (LavaCA.hpp:1052-1066).
 //   ca_finish  : one lane: histories, stop rule, counter reset (LavaCA.hpp:1206-1229).
 // Every kernel first reads the device stop flag and returns when a stop rule already fired, so the host
@@ -51,6 +52,30 @@ __device__ inline uint32_t wave_append(bool pred, uint32_t* counter) {
         base = __shfl(base, leader);
     }
     return base + static_cast<uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL)));
+}
+
+// Sums three per-thread counters over the block and writes them to one CaPart row (every thread calls).
+__device__ inline void block_part(CaPart* row, unsigned long long a, unsigned long long b, unsigned long long c) {
+    __shared__ unsigned long long red[3][kBlock / 64];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    const int w = static_cast<int>(threadIdx.x >> 6);
+    if ((threadIdx.x & 63u) == 0) {
+        red[0][w] = a;
+        red[1][w] = b;
+        red[2][w] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        CaPart r{0, 0, 0, 0};
+        for (int k = 0; k < kBlock / 64; ++k) {
+            r.lava_after += red[0][k];
+            r.born += red[1][k];
+            r.changed += red[2][k];
+        }
+        *row = r;
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -267,18 +292,6 @@ __device__ void gather_classic(const CaParams& P, const CaDev& D, int x, int y, 
     }
 }
 
-__device__ __noinline__ bool wide_vote(const CaParams& P, const CaDev& D, int x, int y, int z, uint32_t mask,
-                                        uint64_t h_index, int32_t& flow, float& temp) {
-    VoteWide v;
-    if (P.classic)
-        gather_classic(P, D, x, y, z, mask, v);
-    else
-        gather_lava(P, D, x, y, z, mask, v);
-    if (v.n == 0) return false;
-    v.winner(h_index, flow, temp);
-    return true;
-}
-
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------
@@ -423,7 +436,103 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(CaParams P, CaDev D)
     }
 }
 
+// Tile of the rules kernel staged in LDS: the 64 x TY x TZ cells plus a 1-cell halo.  type holds 0xFF for
+// positions outside the grid (absorbing edges, and past the far side of a partial tile); flow, temperature
+// and D(n) are loaded for LAVA positions only.
+template <int TY, int TZ>
+struct RulesTile {
+    static constexpr int LX = kTileX + 2, LY = TY + 2, LZ = TZ + 2, N = LX * LY * LZ;
+    uint8_t type[N];
+    int8_t desc[N];
+    int32_t flow[N];
+    float temp[N];
+    int32_t spill[kTileX * TY];  // spill height of the tile's own columns
+};
+constexpr uint8_t kVoidLds = 0xFF;
+
+// dropDepth (LavaCA.hpp:404-414) of the AIR cell at LDS position c (layer lz + 1 of its column): the layers
+// held in LDS are read there, deeper ones from global memory.
+template <int TY, int TZ>
+__device__ int drop_depth_lds(const CaDev& D, const RulesTile<TY, TZ>& T, int c, int lz, int z, uint32_t col,
+                              uint32_t plane, int spill_c, bool has_spill) {
+    constexpr int LXY = RulesTile<TY, TZ>::LX * RulesTile<TY, TZ>::LY;
+    int zc = z;
+    for (int d = 0; d <= kMaxDrop; ++d, --zc) {
+        // type(zc) is AIR here: the cell itself when d == 0, checked one iteration earlier otherwise
+        if (zc == 0) return d;  // hasSupport: the floor
+        const int l = lz - d;   // LDS layer of zc - 1 is l (the cell sits in layer lz + 1)
+        const uint8_t b = l >= 0 ? T.type[c - (d + 1) * LXY] : D.type[col + plane * static_cast<uint32_t>(zc - 1)];
+        if (b == kRock || b == kBasalt) return d;
+        if (b == kLava && has_spill && zc - 1 < spill_c) return d;
+        if (b != kAir) return d + 1 <= kMaxDrop ? -1 : kMaxDrop;
+    }
+    return kMaxDrop;
+}
+
+// gatherLavaCandidates (LavaCA.hpp:465-519) / the classic vote (LavaCA.hpp:1165-1202) for the AIR cell at LDS
+// position c; bits = the cell's neighbour positions (k as in vca_internal.h) that hold LAVA and that the rules
+// read.  Only ponding below the staged layers and deep drop depths touch global memory.
+template <int TY, int TZ, class V>
+__device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY, TZ>& T, int c, int lz, int z,
+                           uint32_t col, uint32_t plane, int spill_c, bool has_spill, uint32_t bits, V& v) {
+    constexpr int LX = RulesTile<TY, TZ>::LX, LXY = LX * RulesTile<TY, TZ>::LY;
+    int cdrop = -100;
+    while (bits) {
+        const int k = __ffs(bits) - 1;
+        bits &= bits - 1;
+        const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
+        const int n = c + dx + LX * dy + LXY * dz;
+        if (P.classic) {
+            v.add(T.flow[n], 0.0f);
+            continue;
+        }
+        float cooling;
+        if (dz < 0) {
+            // tryPonding (LavaCA.hpp:443-461); only k == 4 (directly below) is in the mask in lava mode
+            if (z > spill_c || z == 0) continue;
+            bool ok = true;
+            if (z - 1 > 0) {  // hasSupport(z - 1)
+                const uint8_t b2 = lz >= 1 ? T.type[c - 2 * LXY] : D.type[col + plane * static_cast<uint32_t>(z - 2)];
+                ok = b2 == kRock || b2 == kBasalt || (b2 == kLava && has_spill && z - 2 < spill_c);
+            }
+            if (!ok) continue;
+            const uint8_t h0 = T.type[n - 1], h1 = T.type[n + 1];
+            bool closed = h0 != kAir && h0 != kVoidLds && h1 != kAir && h1 != kVoidLds;
+            if (P.ny > 1) {
+                const uint8_t h2 = T.type[n - LX], h3 = T.type[n + LX];
+                closed = closed && h2 != kAir && h2 != kVoidLds && h3 != kAir && h3 != kVoidLds;
+            }
+            if (!closed) continue;
+            cooling = P.cool_lat;
+        } else if (dz > 0 && dx == 0 && dy == 0) {
+            cooling = P.cool_fall;
+        } else {
+            const int d = T.desc[n];
+            if (d < 0) continue;
+            if (cdrop == -100) cdrop = drop_depth_lds(D, T, c, lz, z, col, plane, spill_c, has_spill);
+            if (dz > 0) {
+                if (cdrop != 0 || d != 1) continue;
+                cooling = P.cool_fall;
+            } else {
+                if (cdrop != d || (d == 1 && P.nb.diagonal_falls)) continue;
+                cooling = P.cool_lat;
+            }
+        }
+        v.add(T.flow[n], T.temp[n] - cooling);
+    }
+}
+
+// Rules of every cell of the visited tiles (LavaCA.hpp:963-1048, 1109-1202).  The tile and a 1-cell halo
+// are staged in LDS first (types for every position, then flowId / temperature / D(n) for LAVA positions,
+// all independent coalesced loads), so an AIR cell folds its 26 neighbour types into a bitmask without
+// branches and gathers its candidates from LDS.  Cells whose vote overflows the four register slots are
+// handed to ca_wide.
+template <int TY, int TZ>
 __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
+    using Tile = RulesTile<TY, TZ>;
+    constexpr int LX = Tile::LX, LY = Tile::LY, LN = Tile::N;
+    __shared__ Tile T;
+    __shared__ uint32_t s_nchg;
     if (D.ctrl->stop) return;
     const uint32_t nact = D.ctrl->nactive;
     const int32_t t = D.ctrl->steps;
@@ -431,28 +540,73 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
     const bool erupting = t < P.eruption_steps;
     const uint16_t stamp = static_cast<uint16_t>(t < 65535 ? t : 65535);
     const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
+    const bool has_spill = !P.classic && D.spill != nullptr;
+    // positions the rules read as lava sources: dz >= 0, plus directly below for ponding (lava mode)
+    const uint32_t rules_mask =
+        P.classic ? P.nb.any_mask : ((P.nb.any_mask & ~0x1FFu) | (P.ponding ? (P.nb.any_mask & (1u << 4)) : 0u));
     unsigned long long lava_after = 0, born = 0, changed = 0;
     for (uint32_t s = blockIdx.x; s < nact; s += gridDim.x) {
         const uint32_t tile = D.active[s];
         const TileCoord o = tile_origin(P, tile);
+        __syncthreads();  // the previous tile's LDS reads are done
+        for (int q = static_cast<int>(threadIdx.x); q < LN; q += kBlock) {
+            const int lx = q % LX, r = q / LX;
+            int gx = o.x0 - 1 + lx, gy = o.y0 - 1 + r % LY, gz = o.z0 - 1 + r / LY;
+            uint8_t v = kVoidLds;
+            if (gx <= P.nx && gy <= P.ny && gz <= P.nz && wrap_axis(gx, P.nx, P.per[0]) &&
+                wrap_axis(gy, P.ny, P.per[1]) && wrap_axis(gz, P.nz, P.per[2]))
+                v = D.type[static_cast<uint32_t>(gx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(gy) +
+                           plane * static_cast<uint32_t>(gz)];
+            T.type[q] = v;
+        }
+        if (threadIdx.x == 0) s_nchg = 0;
+        if (has_spill) {
+            for (int q = static_cast<int>(threadIdx.x); q < kTileX * TY; q += kBlock) {
+                const int x = o.x0 + q % kTileX, y = o.y0 + q / kTileX;
+                T.spill[q] = (x < P.nx && y < P.ny)
+                                 ? D.spill[static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y)]
+                                 : 0;
+            }
+        }
+        __syncthreads();
+        for (int q = static_cast<int>(threadIdx.x); q < LN; q += kBlock) {
+            if (T.type[q] != kLava) continue;
+            const int lx = q % LX, r = q / LX;
+            int gx = o.x0 - 1 + lx, gy = o.y0 - 1 + r % LY, gz = o.z0 - 1 + r / LY;
+            wrap_axis(gx, P.nx, P.per[0]);  // in range: the position holds a cell
+            wrap_axis(gy, P.ny, P.per[1]);
+            wrap_axis(gz, P.nz, P.per[2]);
+            const uint32_t g = static_cast<uint32_t>(gx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(gy) +
+                               plane * static_cast<uint32_t>(gz);
+            T.flow[q] = D.flow[g];
+            if (!P.classic) {
+                T.temp[q] = D.temp[g];
+                T.desc[q] = D.descent[g];
+            }
+        }
+        __syncthreads();
         int tile_lava = 0, tile_air = 0;
 #pragma unroll 1
         for (int k = 0; k < 4; ++k) {
-            int x, y, z;
-            tile_cell(P, o, k, x, y, z);
-            bool want = false;
-            Change c;
+            const int lx = static_cast<int>(threadIdx.x & 63u);
+            const int row = static_cast<int>(threadIdx.x >> 6) + 4 * k;
+            const int ly = row % TY, lz = row / TY;
+            const int x = o.x0 + lx, y = o.y0 + ly, z = o.z0 + lz;
+            const int c = (lx + 1) + LX * ((ly + 1) + LY * (lz + 1));
+            bool want = false, wide = false;
+            Change ch;
+            uint32_t i = 0;
             if (x < P.nx && y < P.ny && z < P.nz) {
-                const uint32_t i = static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y) +
-                                   plane * static_cast<uint32_t>(z);
-                const uint8_t ty = D.type[i];
+                const uint32_t col = static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y);
+                i = col + plane * static_cast<uint32_t>(z);
+                const uint8_t ty = T.type[c];
                 if (ty == kLava) {
                     if (P.classic) {
                         tile_lava = 1;
                     } else {
                         // rule 2 (LavaCA.hpp:1129-1147, 973-994)
-                        const int32_t f = D.flow[i];
-                        float tc = D.temp[i];
+                        const int32_t f = T.flow[c];
+                        float tc = T.temp[c];
                         const float before = tc;
                         if (erupting && f > 0 && f < P.nvent_slots && D.vent_cell[f] == i)
                             tc = P.erupt_temp;
@@ -460,45 +614,56 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
                             tc = tc - P.cool_amb;
                         if (tc < P.solidus) {
                             want = true;
-                            c = Change{i, f, tc, stamp, kBasalt, kKindSolidify};
+                            ch = Change{i, f, tc, stamp, kBasalt, kKindSolidify};
                             ++changed;
                         } else {
                             ++lava_after;
                             tile_lava = 1;
                             if (tc != before) {
                                 want = true;
-                                c = Change{i, f, tc, 0, kLava, kKindTemp};
+                                ch = Change{i, f, tc, 0, kLava, kKindTemp};
                                 ++changed;
                             }
                         }
                     }
                 } else if (ty == kAir) {
-                    // rule 1 (LavaCA.hpp:1115-1128) / classic (LavaCA.hpp:1175-1190)
-                    uint32_t mask = P.nb.mask[0];
-                    uint64_t h_index = 0;
-                    const bool random_nb = P.nb.variants > 1;
-                    if (random_nb) {
-                        h_index = splitmix64(prefix ^ i);
-                        mask = P.nb.mask[splitmix64(h_index ^ kSaltNeighborhood) % static_cast<uint64_t>(P.nb.variants)];
+                    uint32_t lava_bits = 0;
+#pragma unroll
+                    for (int q = 0; q < 27; ++q) {
+                        if (q == 13) continue;
+                        const int off = (q % 3 - 1) + LX * (((q / 3) % 3 - 1) + LY * (q / 9 - 1));
+                        lava_bits |= (T.type[c + off] == kLava ? 1u : 0u) << q;
                     }
-                    Vote4 v;
-                    if (P.classic)
-                        gather_classic(P, D, x, y, z, mask, v);
-                    else
-                        gather_lava(P, D, x, y, z, mask, v);
-                    bool has = v.n > 0;
+                    lava_bits &= rules_mask;
                     int32_t wf = 0;
                     float wt = 0.0f;
-                    if (has) {
-                        if (!random_nb) h_index = splitmix64(prefix ^ i);
-                        if (v.overflow)
-                            has = wide_vote(P, D, x, y, z, mask, h_index, wf, wt);
-                        else
-                            v.winner(h_index, wf, wt);
+                    bool has = false;
+                    if (lava_bits) {
+                        // rule 1 (LavaCA.hpp:1115-1128) / classic (LavaCA.hpp:1175-1190)
+                        const uint64_t h_index = splitmix64(prefix ^ i);
+                        uint32_t mask = P.nb.mask[0];
+                        if (P.nb.variants > 1)
+                            mask = P.nb.mask[splitmix64(h_index ^ kSaltNeighborhood) % static_cast<uint64_t>(P.nb.variants)];
+                        const uint32_t bits = lava_bits & mask;
+                        if (bits) {
+                            const int spill_c = has_spill ? T.spill[lx + kTileX * ly] : 0;
+                            Vote4 v;
+                            gather_lds(P, D, T, c, lz, z, col, plane, spill_c, has_spill, bits, v);
+                            has = v.n > 0;
+                            if (v.overflow) {
+                                wide = true;  // ca_wide decides this cell
+                                has = false;
+                            } else if (has) {
+                                v.winner(h_index, wf, wt);
+                            }
+                        }
                     }
-                    if (has && (P.classic || wt >= P.solidus)) {
+                    if (wide) {
+                        tile_lava = 1;  // either outcome: both flags keep the next front a superset
+                        tile_air = 1;
+                    } else if (has && (P.classic || wt >= P.solidus)) {
                         want = true;
-                        c = Change{i, wf, P.classic ? 0.0f : wt, stamp, kLava, kKindBorn};
+                        ch = Change{i, wf, P.classic ? 0.0f : wt, stamp, kLava, kKindBorn};
                         ++lava_after;
                         ++born;
                         ++changed;
@@ -508,50 +673,122 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
                     }
                 }
             }
-            const uint32_t slot = wave_append(want, &D.ctrl->nchanges);
-            if (want) D.changes[slot] = c;
+            const uint32_t slot = wave_append(want, &s_nchg);
+            if (want) D.changes[static_cast<size_t>(tile) * kTileCells + slot] = ch;
+            if (__ballot(wide)) {
+                const uint32_t oslot = wave_append(wide, &D.ctrl->noverflow);
+                if (wide) D.overflow[oslot] = i;
+            }
         }
         const int any_lava = __syncthreads_or(tile_lava);
         const int any_air = __syncthreads_or(tile_air);
-        if (threadIdx.x == 0)
+        if (threadIdx.x == 0) {
             D.tile_flags[tile] = static_cast<uint8_t>((any_lava ? kFlagLava : 0) | (any_air ? kFlagAir : 0));
+            D.tile_nchg[tile] = s_nchg;
+        }
     }
-    lava_after = wave_sum(lava_after);
-    born = wave_sum(born);
-    changed = wave_sum(changed);
-    if ((threadIdx.x & 63u) == 0) {
-        if (lava_after) atomicAdd(&D.ctrl->lava_after, lava_after);
-        if (born) atomicAdd(&D.ctrl->born, born);
-        if (changed) atomicAdd(&D.ctrl->changed, changed);
-    }
+    block_part(&D.part[blockIdx.x], lava_after, born, changed);
 }
 
+// AIR cells whose candidates carry more than four distinct flowIds (rare): the full vote over private
+// memory, then the change and counters the rules kernel leaves to this kernel.  Reads the start-of-step
+// state only, so it runs between the rules and the apply launch.
+__global__ void __launch_bounds__(kBlock) ca_wide_kernel(CaParams P, CaDev D, int part_row0) {
+    if (D.ctrl->stop) return;
+    const uint32_t n = D.ctrl->noverflow;
+    const int32_t t = D.ctrl->steps;
+    const uint64_t prefix = hash_prefix(P.seed, static_cast<uint64_t>(t));
+    const uint16_t stamp = static_cast<uint16_t>(t < 65535 ? t : 65535);
+    unsigned long long lava_after = 0, born = 0, changed = 0;
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        const uint32_t i = D.overflow[j];
+        const int x = static_cast<int>(i % static_cast<uint32_t>(P.nx));
+        const uint32_t r = i / static_cast<uint32_t>(P.nx);
+        const int y = static_cast<int>(r % static_cast<uint32_t>(P.ny));
+        const int z = static_cast<int>(r / static_cast<uint32_t>(P.ny));
+        const uint64_t h_index = splitmix64(prefix ^ i);
+        uint32_t mask = P.nb.mask[0];
+        if (P.nb.variants > 1)
+            mask = P.nb.mask[splitmix64(h_index ^ kSaltNeighborhood) % static_cast<uint64_t>(P.nb.variants)];
+        VoteWide v;
+        if (P.classic)
+            gather_classic(P, D, x, y, z, mask, v);
+        else
+            gather_lava(P, D, x, y, z, mask, v);
+        if (v.n == 0) continue;
+        int32_t wf = 0;
+        float wt = 0.0f;
+        v.winner(h_index, wf, wt);
+        if (!P.classic && wt < P.solidus) continue;
+        // the cell's tile: records of its rules pass are already counted in tile_nchg
+        const uint32_t tile = static_cast<uint32_t>(x / kTileX) +
+                              static_cast<uint32_t>(P.ntx) * (static_cast<uint32_t>(y / P.ty) +
+                                                               static_cast<uint32_t>(P.nty) * static_cast<uint32_t>(z / P.tz));
+        const uint32_t slot = atomicAdd(&D.tile_nchg[tile], 1u);
+        D.changes[static_cast<size_t>(tile) * kTileCells + slot] =
+            Change{i, wf, P.classic ? 0.0f : wt, stamp, kLava, kKindBorn};
+        ++lava_after;
+        ++born;
+        ++changed;
+    }
+    block_part(&D.part[part_row0 + static_cast<int>(blockIdx.x)], lava_after, born, changed);
+}
+
+// Scatters each visited tile's change records into the state (LavaCA.hpp:1052-1066).
 __global__ void __launch_bounds__(kBlock) ca_apply_kernel(CaDev D) {
     if (D.ctrl->stop) return;
-    const uint32_t n = D.ctrl->nchanges;
-    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
-        const Change c = D.changes[j];
-        D.type[c.idx] = c.type;
-        D.temp[c.idx] = c.temp;
-        if (c.kind == kKindBorn) {
-            D.flow[c.idx] = c.flow;
-            D.arrival[c.idx] = c.stamp;
-        } else if (c.kind == kKindSolidify) {
-            D.solid[c.idx] = c.stamp;
+    const uint32_t nact = D.ctrl->nactive;
+    for (uint32_t s = blockIdx.x; s < nact; s += gridDim.x) {
+        const uint32_t tile = D.active[s];
+        const uint32_t n = D.tile_nchg[tile];
+        const Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
+        for (uint32_t j = threadIdx.x; j < n; j += kBlock) {
+            const Change c = rec[j];
+            D.type[c.idx] = c.type;
+            D.temp[c.idx] = c.temp;
+            if (c.kind == kKindBorn) {
+                D.flow[c.idx] = c.flow;
+                D.arrival[c.idx] = c.stamp;
+            } else if (c.kind == kKindSolidify) {
+                D.solid[c.idx] = c.stamp;
+            }
         }
     }
 }
 
 // finishStep (LavaCA.hpp:1206-1229) + frontCounts (LavaCA.hpp:865-871)
-__global__ void ca_finish_kernel(CaParams P, CaDev D) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__global__ void __launch_bounds__(kBlock) ca_finish_kernel(CaParams P, CaDev D, int nrows) {
+    __shared__ unsigned long long red[3][kBlock / 64];
     CaCtrl* c = D.ctrl;
     if (c->stop) return;
+    unsigned long long la = 0, bo = 0, ch = 0;
+    for (int r = static_cast<int>(threadIdx.x); r < nrows; r += kBlock) {
+        la += D.part[r].lava_after;
+        bo += D.part[r].born;
+        ch += D.part[r].changed;
+    }
+    la = wave_sum(la);
+    bo = wave_sum(bo);
+    ch = wave_sum(ch);
+    const int w = static_cast<int>(threadIdx.x >> 6);
+    if ((threadIdx.x & 63u) == 0) {
+        red[0][w] = la;
+        red[1][w] = bo;
+        red[2][w] = ch;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    unsigned long long lava_after = 0, born = 0, changed = 0;
+    for (int k = 0; k < kBlock / 64; ++k) {
+        lava_after += red[0][k];
+        born += red[1][k];
+        changed += red[2][k];
+    }
     const int32_t t = c->steps;
-    const unsigned long long active = P.classic ? c->lava_total + c->born : c->lava_after;
-    const unsigned long long air = c->air_total - c->born;
+    const unsigned long long active = P.classic ? c->lava_total + born : lava_after;
+    const unsigned long long air = c->air_total - born;
     D.hist_active[t] = active;
-    D.hist_changed[t] = c->changed;
+    D.hist_changed[t] = changed;
     c->steps = t + 1;
     c->lava_total = active;
     c->air_total = air;
@@ -560,18 +797,15 @@ __global__ void ca_finish_kernel(CaParams P, CaDev D) {
         stop = 1;  // NoLava
     else if (P.classic && air == 0)
         stop = 2;  // NoAir
-    else if (c->changed == 0 && P.nb.variants == 1)
+    else if (changed == 0 && P.nb.variants == 1)
         stop = 3;  // NoChange
     else if (t + 1 >= P.max_steps)
         stop = 4;  // MaxSteps
     c->stop = stop;
     c->tiles_visited += c->nactive;
-    c->changes_total += c->nchanges;
-    c->lava_after = 0;
-    c->born = 0;
-    c->changed = 0;
+    c->changes_total += changed;  // every change is one record
     c->nactive = 0;
-    c->nchanges = 0;
+    c->noverflow = 0;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -873,12 +1107,19 @@ void ca_launch_step(const CaParams& P, const CaDev& D, int grid_tiles, hipStream
     if (ev) hipEventRecord(ev[1], s);
     if (!P.classic) hipLaunchKernelGGL(ca_descent_kernel, dim3(gt), dim3(kBlock), 0, s, P, D);
     if (ev) hipEventRecord(ev[2], s);
-    hipLaunchKernelGGL(ca_rules_kernel, dim3(gt), dim3(kBlock), 0, s, P, D);
+    if (P.ty == 4)
+        hipLaunchKernelGGL((ca_rules_kernel<4, 4>), dim3(gt), dim3(kBlock), 0, s, P, D);
+    else if (P.ty == 1)
+        hipLaunchKernelGGL((ca_rules_kernel<1, 16>), dim3(gt), dim3(kBlock), 0, s, P, D);
+    else
+        hipLaunchKernelGGL((ca_rules_kernel<16, 1>), dim3(gt), dim3(kBlock), 0, s, P, D);
     if (ev) hipEventRecord(ev[3], s);
-    hipLaunchKernelGGL(ca_apply_kernel, dim3(gt), dim3(kBlock), 0, s, D);
+    hipLaunchKernelGGL(ca_wide_kernel, dim3(kWideBlocks), dim3(kBlock), 0, s, P, D, grid_tiles);
     if (ev) hipEventRecord(ev[4], s);
-    hipLaunchKernelGGL(ca_finish_kernel, dim3(1), dim3(64), 0, s, P, D);
+    hipLaunchKernelGGL(ca_apply_kernel, dim3(gt), dim3(kBlock), 0, s, D);
     if (ev) hipEventRecord(ev[5], s);
+    hipLaunchKernelGGL(ca_finish_kernel, dim3(1), dim3(kBlock), 0, s, P, D, grid_tiles + kWideBlocks);
+    if (ev) hipEventRecord(ev[6], s);
 }
 
 #define VCA_SPIN_DISPATCH(bytes, CALL)                 \

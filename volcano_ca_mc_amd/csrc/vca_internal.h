@@ -55,6 +55,7 @@ struct NbTable {
 constexpr int kTileX = 64;
 constexpr int kTileCells = 1024;
 constexpr int kBlock = 256;
+constexpr int kWideBlocks = 64;  // grid of ca_wide
 
 struct CaParams {
     int32_t nx, ny, nz;
@@ -87,10 +88,15 @@ struct CaCtrl {
     int32_t steps;
     int32_t stop;
     uint32_t nactive;
-    uint32_t nchanges;
+    uint32_t noverflow;           // AIR cells whose vote needs more than four flowId slots
     unsigned long long lava_total, air_total;
-    unsigned long long lava_after, born, changed;
     unsigned long long tiles_visited, changes_total;
+};
+
+// Counters of one step, one row per block of the rules and wide kernels (summed by ca_finish), so that no
+// counter is a single address every block updates.
+struct CaPart {
+    unsigned long long lava_after, born, changed, reserved;
 };
 
 struct CaDev {
@@ -104,7 +110,10 @@ struct CaDev {
     uint32_t* vent_cell;          // by flowId, 0xFFFFFFFF = none
     uint8_t* tile_flags;          // bit0: has lava, bit1: has air
     uint32_t* active;             // active tile list
-    Change* changes;
+    Change* changes;              // kTileCells records per tile: the step's changes of tile t at t * kTileCells
+    uint32_t* tile_nchg;          // records of each visited tile this step
+    uint32_t* overflow;           // cell indices, capacity nx*ny*nz
+    CaPart* part;                 // grid_tiles + kWideBlocks rows
     CaCtrl* ctrl;
     unsigned long long* hist_active;
     unsigned long long* hist_changed;
