@@ -220,8 +220,8 @@ struct vca_ca {
     vca_ctx* ctx = nullptr;
     CaParams P{};
     CaDev D{};
-    DevBuf type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, active, changes, tile_nchg, overflow,
-        part, ctrl, hist_a, hist_c;
+    DevBuf type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, active, lava_tiles, changes, tile_nchg,
+        overflow, part, ctrl, hist_a, hist_c;
     int grid_tiles = 1;
     uint64_t n = 0;
 };
@@ -417,7 +417,8 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         ca->active.alloc(static_cast<size_t>(P.ntiles) * 4);
         ca->changes.alloc(static_cast<size_t>(P.ntiles) * kTileCells * sizeof(Change));
         ca->tile_nchg.alloc(static_cast<size_t>(P.ntiles) * 4);
-        ca->part.alloc(static_cast<size_t>(ca->grid_tiles + kWideBlocks) * sizeof(CaPart));
+        ca->part.alloc(static_cast<size_t>(kPartRows) * sizeof(CaPart));
+        ca->lava_tiles.alloc(static_cast<size_t>(P.ntiles) * 4);
         ca->overflow.alloc(n * sizeof(uint32_t));
         ca->ctrl.alloc(sizeof(CaCtrl));
         const size_t hcap = static_cast<size_t>(std::max(1, d->max_steps));
@@ -430,6 +431,7 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         check(hipMemsetAsync(ca->arrival.p, 0, n * 2, s), "memset");
         check(hipMemsetAsync(ca->descent.p, 0xFF, n, s), "memset");
         check(hipMemsetAsync(ca->ctrl.p, 0, sizeof(CaCtrl), s), "memset");
+        check(hipMemsetAsync(ca->part.p, 0, static_cast<size_t>(kPartRows) * sizeof(CaPart), s), "memset");
         check(hipMemsetAsync(ca->hist_a.p, 0, hcap * 8, s), "memset");
         check(hipMemsetAsync(ca->hist_c.p, 0, hcap * 8, s), "memset");
         check(hipMemcpyAsync(ca->vent_cell.p, vent_table.data(), vent_table.size() * 4, hipMemcpyHostToDevice, s),
@@ -449,6 +451,7 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         D.overflow = ca->overflow.as<uint32_t>();
         D.tile_nchg = ca->tile_nchg.as<uint32_t>();
         D.part = ca->part.as<CaPart>();
+        D.lava_tiles = ca->lava_tiles.as<uint32_t>();
         D.ctrl = ca->ctrl.as<CaCtrl>();
         D.hist_active = ca->hist_a.as<unsigned long long>();
         D.hist_changed = ca->hist_c.as<unsigned long long>();

@@ -1,17 +1,18 @@
 // HIP kernels of the lava CA step and the Potts/Metropolis sweep, written for gfx950 (CDNA4).
 //
 // CA step (one synchronous step of engine/src/ca/LavaCA.hpp) = six stream-ordered launches:
-//   ca_front   : active tile list — a 64x4x4 (3D) / 64x1x16 (x-z) tile is visited when it holds lava,
-//                or holds AIR next to a tile with lava (the reference's "may change" set,
-//                engine/src/ca/CaRules.hpp:69-79, at tile granularity).  Wave ballot + one atomic per
-//                wave appends the tile ids.
-//   ca_descent : D(n) of every lava cell of the visited tiles (LavaCA.hpp:421-429, 877-888).
-//   ca_rules   : the rules of every visited cell read the start-of-step state only and append the cell's
-//                new state to a change list (wave-aggregated append), count lava/born/changed, and set the
-//                tile's lava/air flags from the new state (LavaCA.hpp:963-1048, 1109-1202).
+//   ca_front   : active tile list — a 64x4x4 (3D) / 64x1x16 (x-z) / 64x16x1 (x-y) tile is visited when it
+//                holds lava, or holds AIR next to a tile with lava (the reference's "may change" set,
+//                engine/src/ca/CaRules.hpp:69-79, at tile granularity) — plus the list of tiles holding lava.
+//                Wave ballot + one atomic per wave appends the tile ids.
+//   ca_descent : D(n) of every lava cell (LavaCA.hpp:421-429, 877-888), one block per lava tile staged in LDS.
+//   ca_rules   : one block per visited tile staged in LDS with its halo; the rules of every cell read the
+//                start-of-step state only and write the cell's new state as a record into the tile's own
+//                slot range of the change buffer (LDS counter, no global atomics), count lava/born/changed,
+//                and set the tile's lava/air flags from the new state (LavaCA.hpp:963-1048, 1109-1202).
 //   ca_wide    : the few AIR cells whose vote has more than four distinct flowIds (full vote).
-//   ca_apply   : scatters the change list into the state (LavaCA.hpp:1052-1066).
-//   ca_finish  : one lane: histories, stop rule, counter reset (LavaCA.hpp:1206-1229).
+//   ca_apply   : scatters the records of every visited tile into the state (LavaCA.hpp:1052-1066).
+//   ca_finish  : counter rows -> histories, stop rule (LavaCA.hpp:1206-1229).
 // Every kernel first reads the device stop flag and returns when a stop rule already fired, so the host
 // can enqueue many steps without waiting.
 //
@@ -54,7 +55,7 @@ __device__ inline uint32_t wave_append(bool pred, uint32_t* counter) {
     return base + static_cast<uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL)));
 }
 
-// Sums three per-thread counters over the block and writes them to one CaPart row (every thread calls).
+// Sums three per-thread counters over the block and adds them to one CaPart row (every thread calls).
 __device__ inline void block_part(CaPart* row, unsigned long long a, unsigned long long b, unsigned long long c) {
     __shared__ unsigned long long red[3][kBlock / 64];
     a = wave_sum(a);
@@ -68,13 +69,15 @@ __device__ inline void block_part(CaPart* row, unsigned long long a, unsigned lo
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        CaPart r{0, 0, 0, 0};
+        unsigned long long r0 = 0, r1 = 0, r2 = 0;
         for (int k = 0; k < kBlock / 64; ++k) {
-            r.lava_after += red[0][k];
-            r.born += red[1][k];
-            r.changed += red[2][k];
+            r0 += red[0][k];
+            r1 += red[1][k];
+            r2 += red[2][k];
         }
-        *row = r;
+        if (r0) atomicAdd(&row->lava_after, r0);
+        if (r1) atomicAdd(&row->born, r1);
+        if (r2) atomicAdd(&row->changed, r2);
     }
 }
 
@@ -99,6 +102,10 @@ __device__ inline void tile_cell(const CaParams& P, const TileCoord& o, int k, i
     x = o.x0 + lx;
     y = o.y0 + row % P.ty;
     z = o.z0 + row / P.ty;
+}
+
+__device__ inline uint32_t col_index(const CaParams& P, int x, int y) {
+    return static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y);
 }
 
 // hasSupport (LavaCA.hpp:390-399); col = x + nx*y of an interior column.
@@ -357,111 +364,168 @@ __global__ void __launch_bounds__(kBlock) ca_init_flags_kernel(CaParams P, CaDev
 // CA step kernels
 // ------------------------------------------------------------------------------------------------
 
+// Visited-tile lists of the step.  A tile is visited when it holds lava (lava cools) or holds AIR next to a
+// tile with lava: the reference's "may change" set (CaRules.hpp:69-79, LavaCA.hpp:844-871) at tile
+// granularity.  In lava mode the tiles holding lava also go to a second list, the cells whose D(n) is needed.
 __global__ void __launch_bounds__(kBlock) ca_front_kernel(CaParams P, CaDev D) {
     if (D.ctrl->stop) return;
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    bool act = false;
+    bool act = false, lava = false;
     if (t < P.ntiles) {
         const uint8_t f = D.tile_flags[t];
-        if (P.dense) {
+        lava = (f & kFlagLava) != 0;
+        if (P.dense || (!P.classic && lava) || (P.classic && lava && (f & kFlagAir))) {
             act = true;
-        } else if (!P.classic && (f & kFlagLava)) {
-            act = true;  // lava cools (rule 2)
         } else if (f & kFlagAir) {
-            if (P.classic && (f & kFlagLava)) {
-                act = true;
-            } else {
-                const int tx = static_cast<int>(t % static_cast<uint32_t>(P.ntx));
-                const uint32_t r = t / static_cast<uint32_t>(P.ntx);
-                const int ty = static_cast<int>(r % static_cast<uint32_t>(P.nty));
-                const int tz = static_cast<int>(r / static_cast<uint32_t>(P.nty));
-                for (int dz = -1; dz <= 1 && !act; ++dz) {
-                    int qz = tz + dz;
-                    if (!wrap_axis(qz, P.ntz, P.per[2])) continue;
-                    for (int dy = -1; dy <= 1 && !act; ++dy) {
-                        int qy = ty + dy;
-                        if (!wrap_axis(qy, P.nty, P.per[1])) continue;
-                        for (int dx = -1; dx <= 1; ++dx) {
-                            int qx = tx + dx;
-                            if (!wrap_axis(qx, P.ntx, P.per[0])) continue;
-                            const uint32_t q = static_cast<uint32_t>(qx) +
-                                               static_cast<uint32_t>(P.ntx) *
-                                                   (static_cast<uint32_t>(qy) + static_cast<uint32_t>(P.nty) * qz);
-                            if (D.tile_flags[q] & kFlagLava) {
-                                act = true;
-                                break;
-                            }
-                        }
-                    }
-                }
+            const int tx = static_cast<int>(t % static_cast<uint32_t>(P.ntx));
+            const uint32_t r = t / static_cast<uint32_t>(P.ntx);
+            const int ty = static_cast<int>(r % static_cast<uint32_t>(P.nty));
+            const int tz = static_cast<int>(r / static_cast<uint32_t>(P.nty));
+            uint8_t any = 0;  // all 26 neighbour flags are loaded together (no early exit)
+#pragma unroll
+            for (int q = 0; q < 27; ++q) {
+                if (q == 13) continue;
+                int qx = tx + q % 3 - 1, qy = ty + (q / 3) % 3 - 1, qz = tz + q / 9 - 1;
+                if (!wrap_axis(qx, P.ntx, P.per[0]) || !wrap_axis(qy, P.nty, P.per[1]) || !wrap_axis(qz, P.ntz, P.per[2]))
+                    continue;
+                any |= D.tile_flags[static_cast<uint32_t>(qx) +
+                                    static_cast<uint32_t>(P.ntx) *
+                                        (static_cast<uint32_t>(qy) + static_cast<uint32_t>(P.nty) * static_cast<uint32_t>(qz))];
             }
+            act = (any & kFlagLava) != 0;
         }
     }
     const uint32_t slot = wave_append(act, &D.ctrl->nactive);
     if (act) D.active[slot] = t;
-}
-
-__global__ void __launch_bounds__(kBlock) ca_descent_kernel(CaParams P, CaDev D) {
-    if (D.ctrl->stop || P.classic) return;
-    const uint32_t nact = D.ctrl->nactive;
-    const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
-    for (uint32_t s = blockIdx.x; s < nact; s += gridDim.x) {
-        const TileCoord o = tile_origin(P, D.active[s]);
-#pragma unroll 1
-        for (int k = 0; k < 4; ++k) {
-            int x, y, z;
-            tile_cell(P, o, k, x, y, z);
-            if (x >= P.nx || y >= P.ny || z >= P.nz) continue;
-            const uint32_t col = static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y);
-            const uint32_t i = col + plane * static_cast<uint32_t>(z);
-            if (D.type[i] != kLava) continue;
-            int d = -1;
-            if (has_support(D, col, plane, z)) {
-                d = 0;
-                const uint32_t side = P.nb.side_mask;
-#pragma unroll
-                for (int q = 9; q < 18; ++q) {  // dz == 0 positions
-                    if (q == 13 || !((side >> q) & 1u)) continue;
-                    int qx = x + (q % 3 - 1), qy = y + ((q / 3) % 3 - 1);
-                    if (!wrap_axis(qx, P.nx, P.per[0]) || !wrap_axis(qy, P.ny, P.per[1])) continue;
-                    const uint32_t qc = static_cast<uint32_t>(qx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(qy);
-                    if (D.type[qc + plane * static_cast<uint32_t>(z)] == kAir) {
-                        const int dd = drop_depth(D, qc, plane, z);
-                        d = dd > d ? dd : d;
-                    }
-                }
-            }
-            D.descent[i] = static_cast<int8_t>(d);
-        }
+    if (!P.classic) {
+        const uint32_t lslot = wave_append(lava, &D.ctrl->nlava);
+        if (lava) D.lava_tiles[lslot] = t;
     }
 }
 
-// Tile of the rules kernel staged in LDS: the 64 x TY x TZ cells plus a 1-cell halo.  type holds 0xFF for
-// positions outside the grid (absorbing edges, and past the far side of a partial tile); flow, temperature
-// and D(n) are loaded for LAVA positions only.
+// A tile of 64 x TY x TZ cells and its 1-cell halo, as staged in LDS by the descent and rules kernels:
+// R = LY * LZ halo rows of LX = 66 positions.  Wave w stages rows w, w + 4, ... with lane lx at halo column
+// lx + 1 (x = x0 + lx); threads 2r and 2r + 1 stage the two edge columns of row r.
 template <int TY, int TZ>
-struct RulesTile {
-    static constexpr int LX = kTileX + 2, LY = TY + 2, LZ = TZ + 2, N = LX * LY * LZ;
-    uint8_t type[N];
-    int8_t desc[N];
-    int32_t flow[N];
-    float temp[N];
-    int32_t spill[kTileX * TY];  // spill height of the tile's own columns
+struct Halo {
+    static constexpr int LX = kTileX + 2, LY = TY + 2, LZ = TZ + 2, LXY = LX * LY, N = LXY * LZ;
+    static constexpr int R = LY * LZ;                                   // halo rows
+    static constexpr int KR = (R + kBlock / 64 - 1) / (kBlock / 64);    // rows per wave
+    static constexpr int KC = (LXY + kBlock - 1) / kBlock;              // spill column rounds per thread
+    static_assert(2 * R <= kBlock, "edge columns: one thread each");
+    static_assert(TY * TZ == 16, "four waves x four rows per tile");
 };
-constexpr uint8_t kVoidLds = 0xFF;
+constexpr uint8_t kVoidLds = 0xFF;  // halo position with no cell (absorbing edge, or past a partial tile)
+constexpr uint32_t kNoCell = 0xFFFFFFFFu;
 
-// dropDepth (LavaCA.hpp:404-414) of the AIR cell at LDS position c (layer lz + 1 of its column): the layers
-// held in LDS are read there, deeper ones from global memory.
+// Global index of x = 0 in halo row r of the tile at o, or kNoCell when the row holds no cells.  Rows past the
+// far side of a partial tile (y > ny, ...) neighbour no cell and hold none either.
 template <int TY, int TZ>
-__device__ int drop_depth_lds(const CaDev& D, const RulesTile<TY, TZ>& T, int c, int lz, int z, uint32_t col,
+__device__ inline uint32_t halo_row(const CaParams& P, const TileCoord& o, int r) {
+    int gy = o.y0 - 1 + r % Halo<TY, TZ>::LY, gz = o.z0 - 1 + r / Halo<TY, TZ>::LY;
+    if (gy > P.ny || gz > P.nz || !wrap_axis(gy, P.ny, P.per[1]) || !wrap_axis(gz, P.nz, P.per[2])) return kNoCell;
+    return static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(gy) +
+           static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny) * static_cast<uint32_t>(gz);
+}
+// Global x of halo x position gx in [-1, nx], or -1 when it holds no cell.
+__device__ inline int halo_x(const CaParams& P, int gx) {
+    if (gx > P.nx || !wrap_axis(gx, P.nx, P.per[0])) return -1;
+    return gx;
+}
+
+// Halo of one tile in LDS with per-row lava masks: bit lx of lava[r] = halo column lx + 1 of row r holds
+// LAVA; edge[2r] / edge[2r + 1] = the left / right edge column of row r holds LAVA.
+template <int TY, int TZ>
+struct HaloTypes {
+    uint8_t type[Halo<TY, TZ>::N];
+    uint64_t lava[Halo<TY, TZ>::R];
+    uint8_t edge[2 * Halo<TY, TZ>::R];
+};
+
+// Stages the types of the tile at o and its halo, row by row, with the row lava masks.  All loads of a
+// thread are issued before its first LDS store, so a tile costs one memory latency.  Returns whether this
+// thread saw lava; own_air |= this thread saw AIR among the tile's own cells.
+template <int TY, int TZ>
+__device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileCoord& o, HaloTypes<TY, TZ>& S,
+                                  int& own_air) {
+    using H = Halo<TY, TZ>;
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int lx = static_cast<int>(threadIdx.x & 63u);
+    const int t = static_cast<int>(threadIdx.x);
+    const int gx = halo_x(P, o.x0 + lx);  // x == nx is the wrapped x = 0 of a periodic partial tile
+    uint8_t v[H::KR];
+#pragma unroll
+    for (int k = 0; k < H::KR; ++k) {
+        const int r = w + 4 * k;
+        v[k] = kVoidLds;
+        if (r < H::R) {
+            const uint32_t base = halo_row<TY, TZ>(P, o, r);
+            if (base != kNoCell && gx >= 0) v[k] = D.type[base + static_cast<uint32_t>(gx)];
+        }
+    }
+    uint8_t e = kVoidLds;
+    if (t < 2 * H::R) {
+        const uint32_t base = halo_row<TY, TZ>(P, o, t >> 1);
+        const int ex = halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1);
+        if (base != kNoCell && ex >= 0) e = D.type[base + static_cast<uint32_t>(ex)];
+    }
+    bool lava = false;
+#pragma unroll
+    for (int k = 0; k < H::KR; ++k) {
+        const int r = w + 4 * k;
+        if (r < H::R) {
+            S.type[lx + 1 + H::LX * r] = v[k];
+            const uint64_t m = __ballot(v[k] == kLava);
+            if (lx == 0) S.lava[r] = m;
+            lava = lava || m != 0;
+            const int ly = r % H::LY, lz = r / H::LY;
+            if (ly >= 1 && ly <= TY && lz >= 1 && lz <= TZ && o.x0 + lx < P.nx && v[k] == kAir) own_air = 1;
+        }
+    }
+    if (t < 2 * H::R) {
+        S.type[((t & 1) ? H::LX - 1 : 0) + H::LX * (t >> 1)] = e;
+        S.edge[t] = e == kLava ? 1 : 0;
+        lava = lava || e == kLava;
+    }
+    return lava;
+}
+
+// Spill heights of the LX x LY halo columns into LDS (0 where the column holds no cells).
+template <int TY, int TZ>
+__device__ inline void stage_spill(const CaParams& P, const CaDev& D, const TileCoord& o, int32_t* s_spill) {
+    using H = Halo<TY, TZ>;
+    int32_t v[H::KC];
+#pragma unroll
+    for (int k = 0; k < H::KC; ++k) {
+        const int q = static_cast<int>(threadIdx.x) + k * kBlock;
+        v[k] = 0;
+        if (q < H::LXY) {
+            int gx = o.x0 - 1 + q % H::LX, gy = o.y0 - 1 + q / H::LX;
+            if (gx <= P.nx && gy <= P.ny && wrap_axis(gx, P.nx, P.per[0]) && wrap_axis(gy, P.ny, P.per[1]))
+                v[k] = D.spill[static_cast<uint32_t>(gx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(gy)];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < H::KC; ++k) {
+        const int q = static_cast<int>(threadIdx.x) + k * kBlock;
+        if (q < H::LXY) s_spill[q] = v[k];
+    }
+}
+
+// dropDepth (LavaCA.hpp:404-414) of the AIR cell at halo position c, which sits in halo layer lz + 1 of the
+// column col (global) with spill height spill_c.  Layers held in LDS are read there, deeper ones from global
+// memory.
+template <int TY, int TZ>
+__device__ int drop_depth_lds(const CaDev& D, const uint8_t* s_type, int c, int lz, int z, uint32_t col,
                               uint32_t plane, int spill_c, bool has_spill) {
-    constexpr int LXY = RulesTile<TY, TZ>::LX * RulesTile<TY, TZ>::LY;
+    constexpr int LXY = Halo<TY, TZ>::LXY;
     int zc = z;
+    #pragma clang loop unroll(disable)
     for (int d = 0; d <= kMaxDrop; ++d, --zc) {
         // type(zc) is AIR here: the cell itself when d == 0, checked one iteration earlier otherwise
         if (zc == 0) return d;  // hasSupport: the floor
-        const int l = lz - d;   // LDS layer of zc - 1 is l (the cell sits in layer lz + 1)
-        const uint8_t b = l >= 0 ? T.type[c - (d + 1) * LXY] : D.type[col + plane * static_cast<uint32_t>(zc - 1)];
+        const int l = lz - d;   // halo layer of zc - 1
+        const uint8_t b = l >= 0 ? s_type[c - (d + 1) * LXY] : D.type[col + plane * static_cast<uint32_t>(zc - 1)];
         if (b == kRock || b == kBasalt) return d;
         if (b == kLava && has_spill && zc - 1 < spill_c) return d;
         if (b != kAir) return d + 1 <= kMaxDrop ? -1 : kMaxDrop;
@@ -469,14 +533,103 @@ __device__ int drop_depth_lds(const CaDev& D, const RulesTile<TY, TZ>& T, int c,
     return kMaxDrop;
 }
 
+// D(n) of every lava cell of the tiles holding lava (LavaCA.hpp:421-429, 877-888), from the LDS-staged tile:
+// hasSupport of the cell, then the deepest dropDepth over its AIR side neighbours (union of the variants).
+template <int TY, int TZ>
+__global__ void __launch_bounds__(kBlock) ca_descent_kernel(CaParams P, CaDev D) {
+    using H = Halo<TY, TZ>;
+    __shared__ HaloTypes<TY, TZ> S;
+    __shared__ int32_t s_spill[H::LXY];
+    __shared__ uint16_t s_list[kTileCells];
+    __shared__ uint32_t s_n;
+    if (D.ctrl->stop || P.classic) return;
+    const uint8_t* s_type = S.type;
+    const uint32_t n = D.ctrl->nlava;
+    const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
+    const bool has_spill = D.spill != nullptr;
+    const uint32_t side = P.nb.side_mask;
+    for (uint32_t s = blockIdx.x; s < n; s += gridDim.x) {
+        const TileCoord o = tile_origin(P, D.lava_tiles[s]);
+        __syncthreads();  // the previous tile's LDS reads are done
+        int own_air = 0;
+        stage_rows<TY, TZ>(P, D, o, S, own_air);
+        if (has_spill) stage_spill<TY, TZ>(P, D, o, s_spill);
+        if (threadIdx.x == 0) s_n = 0;
+        __syncthreads();
+        {
+            // the tile's lava cells into a list (tile-local index lx + 64 * row)
+            const int lx = static_cast<int>(threadIdx.x & 63u);
+            const int w = static_cast<int>(threadIdx.x >> 6);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int row = w + 4 * k;
+                const int rc = (row % TY + 1) + H::LY * (row / TY + 1);
+                // positions past the grid's far side hold wrapped halo cells of a periodic partial tile
+                const bool inside = o.x0 + lx < P.nx && o.y0 + row % TY < P.ny && o.z0 + row / TY < P.nz;
+                const bool lava = inside && ((S.lava[rc] >> lx) & 1u);
+                const uint32_t slot = wave_append(lava, &s_n);
+                if (lava) s_list[slot] = static_cast<uint16_t>(lx + 64 * row);
+            }
+        }
+        __syncthreads();
+        const uint32_t nl = s_n;
+        for (uint32_t j = threadIdx.x; j < nl; j += kBlock) {
+            const int e = s_list[j];
+            const int lx = e & 63, row = e >> 6;
+            const int ly = row % TY, lz = row / TY;
+            const int x = o.x0 + lx, y = o.y0 + ly, z = o.z0 + lz;
+            const int c = (lx + 1) + H::LX * ((ly + 1) + H::LY * (lz + 1));
+            const int sc = (lx + 1) + H::LX * (ly + 1);
+            bool sup = z == 0;  // hasSupport (LavaCA.hpp:390-399)
+            if (!sup) {
+                const uint8_t b = s_type[c - H::LXY];
+                sup = b == kRock || b == kBasalt || (b == kLava && has_spill && z - 1 < s_spill[sc]);
+            }
+            int d = -1;
+            if (sup) {
+                d = 0;
+#pragma unroll
+                for (int q = 9; q < 18; ++q) {  // dz == 0 positions
+                    if (q == 13 || !((side >> q) & 1u)) continue;
+                    const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1;
+                    const int nb = c + dx + H::LX * dy;
+                    if (s_type[nb] != kAir) continue;  // VOID (no cell) is not AIR
+                    int qx = x + dx, qy = y + dy;
+                    wrap_axis(qx, P.nx, P.per[0]);
+                    wrap_axis(qy, P.ny, P.per[1]);
+                    const uint32_t qc = static_cast<uint32_t>(qx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(qy);
+                    const int dd = drop_depth_lds<TY, TZ>(D, s_type, nb, lz, z, qc, plane,
+                                                          has_spill ? s_spill[sc + dx + H::LX * dy] : 0, has_spill);
+                    d = dd > d ? dd : d;
+                }
+            }
+            D.descent[col_index(P, x, y) + plane * static_cast<uint32_t>(z)] = static_cast<int8_t>(d);
+        }
+    }
+}
+
+// Tile of the rules kernel in LDS: types of every halo position, flowId / temperature / D(n) of the LAVA
+// positions, spill heights of the halo columns.
+template <int TY, int TZ>
+struct RulesTile {
+    using H = Halo<TY, TZ>;
+    HaloTypes<TY, TZ> S;
+    uint16_t list[kTileCells];  // candidate cells of the tile (tile-local index lx + 64 * row)
+    int8_t desc[H::N];
+    int32_t flow[H::N];
+    float temp[H::N];
+    int32_t spill[H::LXY];
+};
+
 // gatherLavaCandidates (LavaCA.hpp:465-519) / the classic vote (LavaCA.hpp:1165-1202) for the AIR cell at LDS
 // position c; bits = the cell's neighbour positions (k as in vca_internal.h) that hold LAVA and that the rules
 // read.  Only ponding below the staged layers and deep drop depths touch global memory.
 template <int TY, int TZ, class V>
 __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY, TZ>& T, int c, int lz, int z,
                            uint32_t col, uint32_t plane, int spill_c, bool has_spill, uint32_t bits, V& v) {
-    constexpr int LX = RulesTile<TY, TZ>::LX, LXY = LX * RulesTile<TY, TZ>::LY;
+    constexpr int LX = Halo<TY, TZ>::LX, LXY = Halo<TY, TZ>::LXY;
     int cdrop = -100;
+    #pragma clang loop unroll(disable)
     while (bits) {
         const int k = __ffs(bits) - 1;
         bits &= bits - 1;
@@ -492,14 +645,14 @@ __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY
             if (z > spill_c || z == 0) continue;
             bool ok = true;
             if (z - 1 > 0) {  // hasSupport(z - 1)
-                const uint8_t b2 = lz >= 1 ? T.type[c - 2 * LXY] : D.type[col + plane * static_cast<uint32_t>(z - 2)];
+                const uint8_t b2 = lz >= 1 ? T.S.type[c - 2 * LXY] : D.type[col + plane * static_cast<uint32_t>(z - 2)];
                 ok = b2 == kRock || b2 == kBasalt || (b2 == kLava && has_spill && z - 2 < spill_c);
             }
             if (!ok) continue;
-            const uint8_t h0 = T.type[n - 1], h1 = T.type[n + 1];
+            const uint8_t h0 = T.S.type[n - 1], h1 = T.S.type[n + 1];
             bool closed = h0 != kAir && h0 != kVoidLds && h1 != kAir && h1 != kVoidLds;
             if (P.ny > 1) {
-                const uint8_t h2 = T.type[n - LX], h3 = T.type[n + LX];
+                const uint8_t h2 = T.S.type[n - LX], h3 = T.S.type[n + LX];
                 closed = closed && h2 != kAir && h2 != kVoidLds && h3 != kAir && h3 != kVoidLds;
             }
             if (!closed) continue;
@@ -509,7 +662,7 @@ __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY
         } else {
             const int d = T.desc[n];
             if (d < 0) continue;
-            if (cdrop == -100) cdrop = drop_depth_lds(D, T, c, lz, z, col, plane, spill_c, has_spill);
+            if (cdrop == -100) cdrop = drop_depth_lds<TY, TZ>(D, T.S.type, c, lz, z, col, plane, spill_c, has_spill);
             if (dz > 0) {
                 if (cdrop != 0 || d != 1) continue;
                 cooling = P.cool_fall;
@@ -529,10 +682,10 @@ __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY
 // handed to ca_wide.
 template <int TY, int TZ>
 __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
-    using Tile = RulesTile<TY, TZ>;
-    constexpr int LX = Tile::LX, LY = Tile::LY, LN = Tile::N;
-    __shared__ Tile T;
-    __shared__ uint32_t s_nchg;
+    using H = Halo<TY, TZ>;
+    constexpr int LX = H::LX, LY = H::LY;
+    __shared__ RulesTile<TY, TZ> T;
+    __shared__ uint32_t s_nchg, s_nlist;
     if (D.ctrl->stop) return;
     const uint32_t nact = D.ctrl->nactive;
     const int32_t t = D.ctrl->steps;
@@ -549,57 +702,124 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
         const uint32_t tile = D.active[s];
         const TileCoord o = tile_origin(P, tile);
         __syncthreads();  // the previous tile's LDS reads are done
-        for (int q = static_cast<int>(threadIdx.x); q < LN; q += kBlock) {
-            const int lx = q % LX, r = q / LX;
-            int gx = o.x0 - 1 + lx, gy = o.y0 - 1 + r % LY, gz = o.z0 - 1 + r / LY;
-            uint8_t v = kVoidLds;
-            if (gx <= P.nx && gy <= P.ny && gz <= P.nz && wrap_axis(gx, P.nx, P.per[0]) &&
-                wrap_axis(gy, P.ny, P.per[1]) && wrap_axis(gz, P.nz, P.per[2]))
-                v = D.type[static_cast<uint32_t>(gx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(gy) +
-                           plane * static_cast<uint32_t>(gz)];
-            T.type[q] = v;
+        int own_air = 0;
+        const bool saw_lava = stage_rows<TY, TZ>(P, D, o, T.S, own_air);
+        if (has_spill) stage_spill<TY, TZ>(P, D, o, T.spill);
+        if (threadIdx.x == 0) {
+            s_nchg = 0;
+            s_nlist = 0;
         }
-        if (threadIdx.x == 0) s_nchg = 0;
-        if (has_spill) {
-            for (int q = static_cast<int>(threadIdx.x); q < kTileX * TY; q += kBlock) {
-                const int x = o.x0 + q % kTileX, y = o.y0 + q / kTileX;
-                T.spill[q] = (x < P.nx && y < P.ny)
-                                 ? D.spill[static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y)]
-                                 : 0;
+        if (!__syncthreads_or(saw_lava)) {
+            // no lava in the tile or its halo: no cell changes; the tile keeps AIR iff it has AIR
+            const int any_air = __syncthreads_or(own_air);
+            if (threadIdx.x == 0) {
+                D.tile_flags[tile] = static_cast<uint8_t>(any_air ? kFlagAir : 0);
+                D.tile_nchg[tile] = 0;
             }
+            continue;
         }
-        __syncthreads();
-        for (int q = static_cast<int>(threadIdx.x); q < LN; q += kBlock) {
-            if (T.type[q] != kLava) continue;
-            const int lx = q % LX, r = q / LX;
-            int gx = o.x0 - 1 + lx, gy = o.y0 - 1 + r % LY, gz = o.z0 - 1 + r / LY;
-            wrap_axis(gx, P.nx, P.per[0]);  // in range: the position holds a cell
-            wrap_axis(gy, P.ny, P.per[1]);
-            wrap_axis(gz, P.nz, P.per[2]);
-            const uint32_t g = static_cast<uint32_t>(gx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(gy) +
-                               plane * static_cast<uint32_t>(gz);
-            T.flow[q] = D.flow[g];
-            if (!P.classic) {
-                T.temp[q] = D.temp[g];
-                T.desc[q] = D.descent[g];
-            }
-        }
-        __syncthreads();
-        int tile_lava = 0, tile_air = 0;
-#pragma unroll 1
-        for (int k = 0; k < 4; ++k) {
+        {
+            // flowId / temperature / D(n) of the LAVA positions, all loads in flight before the stores
+            const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
             const int lx = static_cast<int>(threadIdx.x & 63u);
-            const int row = static_cast<int>(threadIdx.x >> 6) + 4 * k;
+            const int t = static_cast<int>(threadIdx.x);
+            int32_t fl[H::KR];
+            float tp[H::KR];
+            int8_t ds[H::KR];
+            uint32_t lava = 0;
+#pragma unroll
+            for (int k = 0; k < H::KR; ++k) {
+                const int r = w + 4 * k;
+                fl[k] = 0;
+                tp[k] = 0.0f;
+                ds[k] = -1;
+                if (r < H::R && ((T.S.lava[r] >> lx) & 1u)) {
+                    lava |= 1u << k;
+                    const uint32_t g = halo_row<TY, TZ>(P, o, r) + static_cast<uint32_t>(halo_x(P, o.x0 + lx));
+                    fl[k] = D.flow[g];
+                    if (!P.classic) {
+                        tp[k] = D.temp[g];
+                        ds[k] = D.descent[g];
+                    }
+                }
+            }
+            int32_t efl = 0;
+            float etp = 0.0f;
+            int8_t eds = -1;
+            const bool elava = t < 2 * H::R && T.S.edge[t];
+            if (elava) {
+                const uint32_t g = halo_row<TY, TZ>(P, o, t >> 1) +
+                                   static_cast<uint32_t>(halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1));
+                efl = D.flow[g];
+                if (!P.classic) {
+                    etp = D.temp[g];
+                    eds = D.descent[g];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < H::KR; ++k) {
+                if ((lava >> k) & 1u) {
+                    const int q = lx + 1 + H::LX * (w + 4 * k);
+                    T.flow[q] = fl[k];
+                    T.temp[q] = tp[k];
+                    T.desc[q] = ds[k];
+                }
+            }
+            if (elava) {
+                const int q = ((t & 1) ? H::LX - 1 : 0) + H::LX * (t >> 1);
+                T.flow[q] = efl;
+                T.temp[q] = etp;
+                T.desc[q] = eds;
+            }
+        }
+        int tile_lava = 0, tile_air = 0;
+        {
+            // candidate cells: LAVA, or AIR within one cell of a LAVA position (row masks of the 3 x 3 halo
+            // rows around the cell's row, dilated by one in x); every other cell keeps its state
+            const int lx = static_cast<int>(threadIdx.x & 63u);
+            const int w = static_cast<int>(threadIdx.x >> 6);
+#pragma unroll 1
+            for (int k = 0; k < 4; ++k) {
+                const int row = w + 4 * k;
+                const int rc = (row % TY + 1) + LY * (row / TY + 1);
+                uint64_t u = 0;
+                uint32_t el = 0, er = 0;
+#pragma unroll
+                for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        const int rr = rc + dy + LY * dz;
+                        u |= T.S.lava[rr];
+                        el |= T.S.edge[2 * rr];
+                        er |= T.S.edge[2 * rr + 1];
+                    }
+                const uint64_t near = u | (u << 1) | (u >> 1) | static_cast<uint64_t>(el) |
+                                      (static_cast<uint64_t>(er) << 63);
+                // positions past the grid's far side hold wrapped halo cells of a periodic partial tile
+                const bool inside = o.x0 + lx < P.nx && o.y0 + row % TY < P.ny && o.z0 + row / TY < P.nz;
+                const uint8_t ty = inside ? T.S.type[(lx + 1) + LX * rc] : kVoidLds;
+                const bool cand = ty == kLava || (ty == kAir && ((near >> lx) & 1u));
+                if (ty == kAir && !cand) tile_air = 1;
+                const uint32_t slot = wave_append(cand, &s_nlist);
+                if (cand) T.list[slot] = static_cast<uint16_t>(lx + 64 * row);
+            }
+        }
+        __syncthreads();  // list and lava data of the tile are in LDS
+        const uint32_t nlist = s_nlist;
+        for (uint32_t j0 = 0; j0 < nlist; j0 += kBlock) {
+            const uint32_t j = j0 + threadIdx.x;
+            const int e = j < nlist ? T.list[j] : 0;
+            const int lx = e & 63, row = e >> 6;
             const int ly = row % TY, lz = row / TY;
             const int x = o.x0 + lx, y = o.y0 + ly, z = o.z0 + lz;
             const int c = (lx + 1) + LX * ((ly + 1) + LY * (lz + 1));
             bool want = false, wide = false;
             Change ch;
             uint32_t i = 0;
-            if (x < P.nx && y < P.ny && z < P.nz) {
+            if (j < nlist) {
                 const uint32_t col = static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y);
                 i = col + plane * static_cast<uint32_t>(z);
-                const uint8_t ty = T.type[c];
+                const uint8_t ty = T.S.type[c];
                 if (ty == kLava) {
                     if (P.classic) {
                         tile_lava = 1;
@@ -627,14 +847,21 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
                         }
                     }
                 } else if (ty == kAir) {
+                    // LAVA among the 26 neighbours, three x positions at a time from the row masks
                     uint32_t lava_bits = 0;
+                    const int rc = (ly + 1) + LY * (lz + 1);
 #pragma unroll
-                    for (int q = 0; q < 27; ++q) {
-                        if (q == 13) continue;
-                        const int off = (q % 3 - 1) + LX * (((q / 3) % 3 - 1) + LY * (q / 9 - 1));
-                        lava_bits |= (T.type[c + off] == kLava ? 1u : 0u) << q;
-                    }
-                    lava_bits &= rules_mask;
+                    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+                        for (int dy = -1; dy <= 1; ++dy) {
+                            const int rr = rc + dy + LY * dz;
+                            const uint64_t m = T.S.lava[rr];
+                            const uint64_t sh = lx == 0 ? ((m << 1) | T.S.edge[2 * rr]) : (m >> (lx - 1));
+                            uint32_t b3 = static_cast<uint32_t>(sh) & 7u;
+                            if (lx == 63) b3 = (b3 & 3u) | (static_cast<uint32_t>(T.S.edge[2 * rr + 1]) << 2);
+                            lava_bits |= b3 << (3 * (dy + 1) + 9 * (dz + 1));
+                        }
+                    lava_bits &= rules_mask & ~(1u << 13);
                     int32_t wf = 0;
                     float wt = 0.0f;
                     bool has = false;
@@ -646,7 +873,7 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
                             mask = P.nb.mask[splitmix64(h_index ^ kSaltNeighborhood) % static_cast<uint64_t>(P.nb.variants)];
                         const uint32_t bits = lava_bits & mask;
                         if (bits) {
-                            const int spill_c = has_spill ? T.spill[lx + kTileX * ly] : 0;
+                            const int spill_c = has_spill ? T.spill[(lx + 1) + LX * (ly + 1)] : 0;
                             Vote4 v;
                             gather_lds(P, D, T, c, lz, z, col, plane, spill_c, has_spill, bits, v);
                             has = v.n > 0;
@@ -687,13 +914,13 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
             D.tile_nchg[tile] = s_nchg;
         }
     }
-    block_part(&D.part[blockIdx.x], lava_after, born, changed);
+    block_part(&D.part[blockIdx.x % kPartRows], lava_after, born, changed);
 }
 
 // AIR cells whose candidates carry more than four distinct flowIds (rare): the full vote over private
 // memory, then the change and counters the rules kernel leaves to this kernel.  Reads the start-of-step
 // state only, so it runs between the rules and the apply launch.
-__global__ void __launch_bounds__(kBlock) ca_wide_kernel(CaParams P, CaDev D, int part_row0) {
+__global__ void __launch_bounds__(kBlock) ca_wide_kernel(CaParams P, CaDev D) {
     if (D.ctrl->stop) return;
     const uint32_t n = D.ctrl->noverflow;
     const int32_t t = D.ctrl->steps;
@@ -731,7 +958,7 @@ __global__ void __launch_bounds__(kBlock) ca_wide_kernel(CaParams P, CaDev D, in
         ++born;
         ++changed;
     }
-    block_part(&D.part[part_row0 + static_cast<int>(blockIdx.x)], lava_after, born, changed);
+    block_part(&D.part[(blockIdx.x * 61u + 17u) % kPartRows], lava_after, born, changed);
 }
 
 // Scatters each visited tile's change records into the state (LavaCA.hpp:1052-1066).
@@ -757,15 +984,17 @@ __global__ void __launch_bounds__(kBlock) ca_apply_kernel(CaDev D) {
 }
 
 // finishStep (LavaCA.hpp:1206-1229) + frontCounts (LavaCA.hpp:865-871)
-__global__ void __launch_bounds__(kBlock) ca_finish_kernel(CaParams P, CaDev D, int nrows) {
+__global__ void __launch_bounds__(kBlock) ca_finish_kernel(CaParams P, CaDev D) {
     __shared__ unsigned long long red[3][kBlock / 64];
     CaCtrl* c = D.ctrl;
     if (c->stop) return;
     unsigned long long la = 0, bo = 0, ch = 0;
-    for (int r = static_cast<int>(threadIdx.x); r < nrows; r += kBlock) {
-        la += D.part[r].lava_after;
-        bo += D.part[r].born;
-        ch += D.part[r].changed;
+    for (int r = static_cast<int>(threadIdx.x); r < kPartRows; r += kBlock) {
+        CaPart* row = &D.part[r];
+        la += row->lava_after;
+        bo += row->born;
+        ch += row->changed;
+        *row = CaPart{0, 0, 0, 0};  // cleared for the next step
     }
     la = wave_sum(la);
     bo = wave_sum(bo);
@@ -806,6 +1035,7 @@ __global__ void __launch_bounds__(kBlock) ca_finish_kernel(CaParams P, CaDev D, 
     c->changes_total += changed;  // every change is one record
     c->nactive = 0;
     c->noverflow = 0;
+    c->nlava = 0;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1105,7 +1335,14 @@ void ca_launch_step(const CaParams& P, const CaDev& D, int grid_tiles, hipStream
     if (ev) hipEventRecord(ev[0], s);
     hipLaunchKernelGGL(ca_front_kernel, dim3(grid_for(P.ntiles)), dim3(kBlock), 0, s, P, D);
     if (ev) hipEventRecord(ev[1], s);
-    if (!P.classic) hipLaunchKernelGGL(ca_descent_kernel, dim3(gt), dim3(kBlock), 0, s, P, D);
+    if (!P.classic) {
+        if (P.ty == 4)
+            hipLaunchKernelGGL((ca_descent_kernel<4, 4>), dim3(gt), dim3(kBlock), 0, s, P, D);
+        else if (P.ty == 1)
+            hipLaunchKernelGGL((ca_descent_kernel<1, 16>), dim3(gt), dim3(kBlock), 0, s, P, D);
+        else
+            hipLaunchKernelGGL((ca_descent_kernel<16, 1>), dim3(gt), dim3(kBlock), 0, s, P, D);
+    }
     if (ev) hipEventRecord(ev[2], s);
     if (P.ty == 4)
         hipLaunchKernelGGL((ca_rules_kernel<4, 4>), dim3(gt), dim3(kBlock), 0, s, P, D);
@@ -1114,11 +1351,11 @@ void ca_launch_step(const CaParams& P, const CaDev& D, int grid_tiles, hipStream
     else
         hipLaunchKernelGGL((ca_rules_kernel<16, 1>), dim3(gt), dim3(kBlock), 0, s, P, D);
     if (ev) hipEventRecord(ev[3], s);
-    hipLaunchKernelGGL(ca_wide_kernel, dim3(kWideBlocks), dim3(kBlock), 0, s, P, D, grid_tiles);
+    hipLaunchKernelGGL(ca_wide_kernel, dim3(kWideBlocks), dim3(kBlock), 0, s, P, D);
     if (ev) hipEventRecord(ev[4], s);
     hipLaunchKernelGGL(ca_apply_kernel, dim3(gt), dim3(kBlock), 0, s, D);
     if (ev) hipEventRecord(ev[5], s);
-    hipLaunchKernelGGL(ca_finish_kernel, dim3(1), dim3(kBlock), 0, s, P, D, grid_tiles + kWideBlocks);
+    hipLaunchKernelGGL(ca_finish_kernel, dim3(1), dim3(kBlock), 0, s, P, D);
     if (ev) hipEventRecord(ev[6], s);
 }
 

@@ -89,15 +89,18 @@ struct CaCtrl {
     int32_t stop;
     uint32_t nactive;
     uint32_t noverflow;           // AIR cells whose vote needs more than four flowId slots
+    uint32_t nlava;               // tiles of the step holding lava (lava mode)
+    uint32_t reserved;
     unsigned long long lava_total, air_total;
     unsigned long long tiles_visited, changes_total;
 };
 
-// Counters of one step, one row per block of the rules and wide kernels (summed by ca_finish), so that no
-// counter is a single address every block updates.
+// Counters of one step spread over kPartRows rows (block b of the rules and wide kernels adds to row
+// b % kPartRows; ca_finish sums and clears them), so that no counter is one address every block updates.
 struct CaPart {
     unsigned long long lava_after, born, changed, reserved;
 };
+constexpr int kPartRows = 256;
 
 struct CaDev {
     uint8_t* type;
@@ -110,10 +113,11 @@ struct CaDev {
     uint32_t* vent_cell;          // by flowId, 0xFFFFFFFF = none
     uint8_t* tile_flags;          // bit0: has lava, bit1: has air
     uint32_t* active;             // active tile list
+    uint32_t* lava_tiles;         // tiles holding lava (lava mode)
     Change* changes;              // kTileCells records per tile: the step's changes of tile t at t * kTileCells
     uint32_t* tile_nchg;          // records of each visited tile this step
     uint32_t* overflow;           // cell indices, capacity nx*ny*nz
-    CaPart* part;                 // grid_tiles + kWideBlocks rows
+    CaPart* part;                 // kPartRows rows
     CaCtrl* ctrl;
     unsigned long long* hist_active;
     unsigned long long* hist_changed;
