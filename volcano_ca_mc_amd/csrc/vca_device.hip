@@ -453,7 +453,7 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
     const int lx = static_cast<int>(threadIdx.x & 63u);
     const int t = static_cast<int>(threadIdx.x);
     const int gx = halo_x(P, o.x0 + lx);  // x == nx is the wrapped x = 0 of a periodic partial tile
-    uint8_t v[H::KR];
+    uint32_t v[H::KR];  // 32-bit so that no load result is packed (and waited for) before the stores
 #pragma unroll
     for (int k = 0; k < H::KR; ++k) {
         const int r = w + 4 * k;
@@ -463,7 +463,7 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
             if (base != kNoCell && gx >= 0) v[k] = D.type[base + static_cast<uint32_t>(gx)];
         }
     }
-    uint8_t e = kVoidLds;
+    uint32_t e = kVoidLds;
     if (t < 2 * H::R) {
         const uint32_t base = halo_row<TY, TZ>(P, o, t >> 1);
         const int ex = halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1);
@@ -474,7 +474,7 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
     for (int k = 0; k < H::KR; ++k) {
         const int r = w + 4 * k;
         if (r < H::R) {
-            S.type[lx + 1 + H::LX * r] = v[k];
+            S.type[lx + 1 + H::LX * r] = static_cast<uint8_t>(v[k]);
             const uint64_t m = __ballot(v[k] == kLava);
             if (lx == 0) S.lava[r] = m;
             lava = lava || m != 0;
@@ -483,7 +483,7 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
         }
     }
     if (t < 2 * H::R) {
-        S.type[((t & 1) ? H::LX - 1 : 0) + H::LX * (t >> 1)] = e;
+        S.type[((t & 1) ? H::LX - 1 : 0) + H::LX * (t >> 1)] = static_cast<uint8_t>(e);
         S.edge[t] = e == kLava ? 1 : 0;
         lava = lava || e == kLava;
     }
@@ -725,7 +725,7 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
             const int t = static_cast<int>(threadIdx.x);
             int32_t fl[H::KR];
             float tp[H::KR];
-            int8_t ds[H::KR];
+            int32_t ds[H::KR];  // 32-bit: see stage_rows
             uint32_t lava = 0;
 #pragma unroll
             for (int k = 0; k < H::KR; ++k) {
@@ -745,7 +745,7 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
             }
             int32_t efl = 0;
             float etp = 0.0f;
-            int8_t eds = -1;
+            int32_t eds = -1;
             const bool elava = t < 2 * H::R && T.S.edge[t];
             if (elava) {
                 const uint32_t g = halo_row<TY, TZ>(P, o, t >> 1) +
@@ -762,14 +762,14 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
                     const int q = lx + 1 + H::LX * (w + 4 * k);
                     T.flow[q] = fl[k];
                     T.temp[q] = tp[k];
-                    T.desc[q] = ds[k];
+                    T.desc[q] = static_cast<int8_t>(ds[k]);
                 }
             }
             if (elava) {
                 const int q = ((t & 1) ? H::LX - 1 : 0) + H::LX * (t >> 1);
                 T.flow[q] = efl;
                 T.temp[q] = etp;
-                T.desc[q] = eds;
+                T.desc[q] = static_cast<int8_t>(eds);
             }
         }
         int tile_lava = 0, tile_air = 0;
