@@ -239,7 +239,7 @@ struct vca_mc {
     int sweeps = 0;
     bool has_mob_dyn = false;
     bool has_mob_cells = false;
-    DevBuf spin, list, mob_thr, mob_cells, accept, counters, energy, parent, size, hi, gout;
+    DevBuf spin, list, mob_thr, mob_cells, accept, counters, parent, size, hi, gout;
 };
 
 // ===================================================================================================
@@ -755,10 +755,9 @@ extern "C" int vca_mc_create(vca_ctx* ctx, const vca_mc_desc* d, const uint8_t* 
             }
         mc->accept.alloc(acc.size() * 8);
         check(hipMemcpyAsync(mc->accept.p, acc.data(), acc.size() * 8, hipMemcpyHostToDevice, s), "copy table");
-        mc->counters.alloc(3 * 8);
-        check(hipMemsetAsync(mc->counters.p, 0, 3 * 8, s), "memset");
-        mc->energy.alloc(8);
-        mc->gout.alloc(16 * 8);
+        mc->counters.alloc(kMcRows * kMcCounters * 8);
+        check(hipMemsetAsync(mc->counters.p, 0, kMcRows * kMcCounters * 8, s), "memset");
+        mc->gout.alloc(kMcRows * kMcSlots * 8);
         mc_launch_init_spins(P, mc->spin.p, mc->spin_bytes, mc->list.as<uint32_t>(), mc->ndomain, s);
         check(hipStreamSynchronize(s), "mc init");
         *out = mc.release();
@@ -821,13 +820,17 @@ extern "C" int vca_mc_counts_get(vca_mc* mc, vca_mc_counts* out) {
     return guarded([&] {
         require(mc && out, "NULL argument");
         mc->ctx->set_device();
-        unsigned long long c[3];
-        check(hipMemcpyAsync(c, mc->counters.p, sizeof(c), hipMemcpyDeviceToHost, mc->ctx->stream), "counters");
+        std::vector<unsigned long long> c(kMcRows * kMcCounters);
+        check(hipMemcpyAsync(c.data(), mc->counters.p, c.size() * 8, hipMemcpyDeviceToHost, mc->ctx->stream),
+              "counters");
         check(hipStreamSynchronize(mc->ctx->stream), "sync");
         mc->ctx->drain();
-        out->attempts = c[0];
-        out->accepted = c[1];
-        out->uphill = c[2];
+        out->attempts = out->accepted = out->uphill = 0;
+        for (int r = 0; r < kMcRows; ++r) {
+            out->attempts += c[r * kMcCounters + 0];
+            out->accepted += c[r * kMcCounters + 1];
+            out->uphill += c[r * kMcCounters + 2];
+        }
         out->sweeps = mc->sweeps;
     });
 }
@@ -880,19 +883,24 @@ extern "C" int vca_mc_measure(vca_mc* mc, vca_mc_stats* st, vca_mc_class* classe
             mc->size.alloc(n * 4);
             if (mc->has_mob_cells) mc->hi.alloc(n * 4);
         }
-        check(hipMemsetAsync(mc->energy.p, 0, 8, s), "memset");
-        check(hipMemsetAsync(mc->gout.p, 0, 16 * 8, s), "memset");
+        check(hipMemsetAsync(mc->gout.p, 0, kMcRows * kMcSlots * 8, s), "memset");
         const uint64_t prefix = hash_prefix(mc->P.seed, static_cast<uint64_t>(mc->sweeps));
         mc_launch_energy(mc->P, mc->spin.p, mc->spin_bytes, mc->list.as<uint32_t>(), mc->ndomain, prefix,
-                         mc->energy.as<unsigned long long>(), s);
+                         mc->gout.as<unsigned long long>(), s);
         mc_launch_grains(mc->P, mc->spin.p, mc->spin_bytes, mc->list.as<uint32_t>(), mc->ndomain,
                          mc->parent.as<uint32_t>(), mc->size.as<uint32_t>(),
                          mc->has_mob_cells ? mc->hi.as<uint32_t>() : nullptr, mc->mob_cells.as<float>(),
                          mc->gout.as<unsigned long long>(), s);
-        unsigned long long unlike = 0, g[16];
-        check(hipMemcpyAsync(&unlike, mc->energy.p, 8, hipMemcpyDeviceToHost, s), "energy");
-        check(hipMemcpyAsync(g, mc->gout.p, sizeof(g), hipMemcpyDeviceToHost, s), "grains");
+        std::vector<unsigned long long> rows(kMcRows * kMcSlots);
+        check(hipMemcpyAsync(rows.data(), mc->gout.p, rows.size() * 8, hipMemcpyDeviceToHost, s), "grains");
         check(hipStreamSynchronize(s), "measure");
+        unsigned long long g[kMcSlots] = {};
+        for (int r = 0; r < kMcRows; ++r)
+            for (int i = 0; i < kMcSlots; ++i) {
+                const unsigned long long x = rows[static_cast<size_t>(r) * kMcSlots + i];
+                g[i] = (i == 1 || i == 4 || i == 7) ? std::max(g[i], x) : g[i] + x;
+            }
+        const unsigned long long unlike = g[9];
         st->mcs = mc->sweeps;
         st->grains = g[0];
         st->cells = mc->ndomain;

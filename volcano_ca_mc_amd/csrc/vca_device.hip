@@ -81,6 +81,41 @@ __device__ inline void block_part(CaPart* row, unsigned long long a, unsigned lo
     }
 }
 
+// Reduces N per-thread values over the block (a sum, or a max where bit i of max_mask is set) and folds the
+// block's results into row[0..N) with one atomic each (every thread of the block calls).
+template <int N>
+__device__ inline void block_fold(unsigned long long (&v)[N], uint32_t max_mask, unsigned long long* row) {
+    __shared__ unsigned long long red[N][kBlock / 64];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const bool mx = (max_mask >> i) & 1u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long y = __shfl_xor(v[i], o);
+            v[i] = mx ? (y > v[i] ? y : v[i]) : v[i] + y;
+        }
+    }
+    const int w = static_cast<int>(threadIdx.x >> 6);
+    if ((threadIdx.x & 63u) == 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) red[i][w] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const bool mx = (max_mask >> i) & 1u;
+            unsigned long long r = red[i][0];
+            for (int k = 1; k < kBlock / 64; ++k) r = mx ? (red[i][k] > r ? red[i][k] : r) : r + red[i][k];
+            if (r == 0) continue;
+            if (mx)
+                atomicMax(&row[i], r);
+            else
+                atomicAdd(&row[i], r);
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // CA helpers
 // ------------------------------------------------------------------------------------------------
@@ -1102,26 +1137,18 @@ template <typename S>
 __global__ void __launch_bounds__(kBlock) mc_color_kernel(McParams P, S* spin, const uint32_t* list,
                                                           const uint64_t* mob_thr, uint32_t count, uint64_t prefix,
                                                           const uint64_t* accept, unsigned long long* counters) {
-    unsigned long long att = 0, acc = 0, up = 0;
-    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    if (j < count) {
+    unsigned long long c[3] = {0, 0, 0};  // attempts, accepted, uphill
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < count; j += gridDim.x * kBlock) {
         const uint32_t g = list[j];
         const uint64_t h = splitmix64(prefix ^ g);
         if (mob_thr == nullptr || (splitmix64(h ^ kSaltMcMobility) >> 11) < mob_thr[j]) {
             const int var = P.nb.variants > 1
                                 ? static_cast<int>(splitmix64(h ^ kSaltNeighborhood) % static_cast<uint64_t>(P.nb.variants))
                                 : 0;
-            mc_trial(P, spin, g, h, var, accept, att, acc, up);
+            mc_trial(P, spin, g, h, var, accept, c[0], c[1], c[2]);
         }
     }
-    att = wave_sum(att);
-    acc = wave_sum(acc);
-    up = wave_sum(up);
-    if ((threadIdx.x & 63u) == 0) {
-        if (att) atomicAdd(&counters[0], att);
-        if (acc) atomicAdd(&counters[1], acc);
-        if (up) atomicAdd(&counters[2], up);
-    }
+    block_fold(c, 0u, counters + (blockIdx.x % kMcRows) * kMcCounters);
 }
 
 // mc.update = random (sweepTrials, PottsMC.cpp:312-391): inherently sequential, one lane.
@@ -1161,7 +1188,7 @@ __global__ void __launch_bounds__(kBlock) mc_init_kernel(McParams P, S* spin, co
 template <typename S>
 __global__ void __launch_bounds__(kBlock) mc_energy_kernel(McParams P, const S* spin, const uint32_t* list,
                                                            uint32_t count, uint64_t prefix, unsigned long long* out) {
-    unsigned long long u = 0;
+    unsigned long long u = 0;  // unlike neighbour pairs counted from this cell
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j < count) {
         const uint32_t g = list[j];
@@ -1184,8 +1211,8 @@ __global__ void __launch_bounds__(kBlock) mc_energy_kernel(McParams P, const S* 
         }
         u = static_cast<unsigned long long>(in_dom - same);
     }
-    u = wave_sum(u);
-    if ((threadIdx.x & 63u) == 0 && u) atomicAdd(out, u);
+    unsigned long long v[1] = {u};
+    block_fold(v, 0u, out + (blockIdx.x % kMcRows) * kMcSlots + 9);
 }
 
 // ---- grain labelling (Grains.cpp:33-150) by lock-free union-find: the larger root is always hooked under
@@ -1265,28 +1292,33 @@ __global__ void __launch_bounds__(kBlock) mc_grain_flatten_kernel(const uint32_t
             high = 1;
         }
     }
-    high = wave_sum(high);
-    if ((threadIdx.x & 63u) == 0 && high) atomicAdd(&out[8], high);
+    unsigned long long v[1] = {high};
+    block_fold(v, 0u, out + (blockIdx.x % kMcRows) * kMcSlots + 8);
 }
 
 __global__ void __launch_bounds__(kBlock) mc_grain_roots_kernel(const uint32_t* list, uint32_t count,
                                                                  uint32_t* parent, const uint32_t* size,
                                                                  const uint32_t* hi, unsigned long long* out) {
+    // grains, largest, then count / cells / largest of the high and the low mobility class
+    unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    if (j >= count) return;
-    const uint32_t g = list[j];
-    const uint32_t r = uf_find(parent, g);
-    parent[g] = r;  // flatten for label download (same value any concurrent reader could see)
-    if (r != g) return;
-    const unsigned long long sz = size[g];
-    atomicAdd(&out[0], 1ULL);
-    atomicMax(&out[1], sz);
-    if (hi != nullptr) {
-        const int c = (2ULL * hi[g] >= sz) ? 0 : 1;  // McStage.cpp:145
-        atomicAdd(&out[2 + 3 * c], 1ULL);
-        atomicAdd(&out[3 + 3 * c], sz);
-        atomicMax(&out[4 + 3 * c], sz);
+    if (j < count) {
+        const uint32_t g = list[j];
+        const uint32_t r = uf_find(parent, g);
+        parent[g] = r;  // flatten for label download (same value any concurrent reader could see)
+        if (r == g) {
+            const unsigned long long sz = size[g];
+            v[0] = 1;
+            v[1] = sz;
+            if (hi != nullptr) {
+                const int c = (2ULL * hi[g] >= sz) ? 0 : 1;  // McStage.cpp:145
+                v[2 + 3 * c] = 1;
+                v[3 + 3 * c] = sz;
+                v[4 + 3 * c] = sz;
+            }
+        }
     }
+    block_fold(v, (1u << 1) | (1u << 4) | (1u << 7), out + (blockIdx.x % kMcRows) * kMcSlots);
 }
 
 template <typename S>
@@ -1377,7 +1409,7 @@ void mc_launch_color(const McParams& P, void* spin, int bytes, const uint32_t* l
                      uint32_t count, uint64_t prefix, const uint64_t* accept, unsigned long long* counters,
                      hipStream_t s) {
     if (count == 0) return;
-    VCA_SPIN_DISPATCH(bytes, hipLaunchKernelGGL(mc_color_kernel<S>, dim3(grid_for(count, 1u << 30)), dim3(kBlock), 0, s,
+    VCA_SPIN_DISPATCH(bytes, hipLaunchKernelGGL(mc_color_kernel<S>, dim3(grid_for(count, 4096u)), dim3(kBlock), 0, s,
                                                 P, static_cast<S*>(spin), list, mob_thr, count, prefix, accept,
                                                 counters));
 }

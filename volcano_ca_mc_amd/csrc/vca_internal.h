@@ -102,6 +102,13 @@ struct CaPart {
 };
 constexpr int kPartRows = 256;
 
+// MC counters spread the same way: kMcRows rows; a sweep's counter row holds attempts, accepted, uphill;
+// a measurement row holds kMcSlots values: 0 grains, 1 largest (max), 2..4 / 5..7 count, cells, largest (max)
+// of the high / low mobility class, 8 high-mobility cells, 9 unlike neighbour pairs.
+constexpr int kMcRows = 64;
+constexpr int kMcCounters = 4;
+constexpr int kMcSlots = 16;
+
 struct CaDev {
     uint8_t* type;
     int32_t* flow;
