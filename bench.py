@@ -13,10 +13,11 @@ statistics and file output are outside them (output.format = none).
 
 Printed: one JSON line.  value = CA cell-updates/s = cells x executed steps / caSim (the reference's
 "ns per cell and step" unit); mc_value = MC spin-attempts/s = domain cells x sweeps / mcSim.
-roofline: the dominant kernel's algorithmic bytes per launch / its mean launch time from HIP events
-recorded on the engine's stream during the timed runs.  cpu_baseline: the reference engine
-(oracle/_ref/engine, built from the reference sources) on the host cores, on a bounded sample
-(first 300 CA steps, 50 sweeps) of the same workload.
+roofline: the dominant kernel's algorithmic bytes per launch (26 B per processed cell-step for the CA
+kernels) / its mean launch time from HIP events recorded on the engine's stream during the timed runs;
+traffic = measured HBM bytes per launch of that kernel from the committed rocprofv3 counter summary.
+cpu_baseline: the reference engine (oracle/_ref/engine, built from the reference sources) on the host
+cores, on a bounded sample (the full CA phase, the first 20 sweeps) of the same workload.
 """
 from __future__ import annotations
 
@@ -55,10 +56,13 @@ def run_engine(exe: str, cfg_path: str, out: str, extra, timeout: int = 1800) ->
     return times
 
 
-# Algorithmic bytes of one launch of each kernel (what the launch must move at minimum), from the
-# work counters the engine reports.  Per visited cell the rules read its type byte; per lava cell they
-# also read flowId and temperature (4 + 4 B); each change writes a 16 B record.  The apply kernel reads
-# every record (16 B) and writes the changed fields (type 1 + temp 4 + one of flow/stamp ~4 B ~ 9 B).
+# Algorithmic bytes of one launch of each kernel, from the engine's work counters.  The CA figure is 26 B per
+# processed cell-step (13 B of state read + 13 B written: u8 type, i32 flowId, f32 temperature, u16 solid and
+# arrival steps), processed = visited tiles x 1024 cells; neighbour reads count once.  The MC figure is
+# 2 * colours + 4 B per trial (spin read once per colour pass, class read, spin written).
+CA_BYTES_PER_CELL = 26
+
+
 def kernel_bytes(times: dict) -> dict:
     d = times.get("detail", {})
     w = d.get("caWork", {})
@@ -66,19 +70,22 @@ def kernel_bytes(times: dict) -> dict:
     out = {}
     steps = max(1, times.get("steps", 1))
     visited_cells = w.get("tilesVisited", 0) * w.get("tileCells", 0)
-    lava_steps = w.get("lavaCellSteps", 0)
-    changes = w.get("changes", 0)
     if "ca_rules" in k:
-        out["ca_rules"] = (1 * visited_cells + 8 * lava_steps + 16 * changes) / steps
-    if "ca_apply" in k:
-        out["ca_apply"] = (16 + 9) * changes / steps
-    if "ca_descent" in k:
-        out["ca_descent"] = (1 * visited_cells + 1 * lava_steps) / steps
+        out["ca_rules"] = CA_BYTES_PER_CELL * visited_cells / steps
     if "mc_color" in k:
-        # per trial: list entry 4 B + mobility threshold 8 B + own spin 1 B + spin write 1 B (upper bound)
+        colours = max(1, k["mc_color"]["launches"] // max(1, times.get("mcs", 1)))
         dom = times.get("domainCells", 0)
-        out["mc_color"] = dom * (4 + 8 + 1 + 1) * times.get("mcs", 0) / max(1, k["mc_color"]["launches"])
+        out["mc_color"] = dom * (2 * colours + 4) * times.get("mcs", 0) / max(1, k["mc_color"]["launches"])
     return out
+
+
+def minimal_bytes(times: dict) -> float:
+    """Lower bound of what one ca_rules launch must move in this in-place design: the type byte of every
+    visited cell, flowId + temperature of every lava cell, one 16 B record per change."""
+    w = times.get("detail", {}).get("caWork", {})
+    steps = max(1, times.get("steps", 1))
+    return (w.get("tilesVisited", 0) * w.get("tileCells", 0) + 8 * w.get("lavaCellSteps", 0)
+            + 16 * w.get("changes", 0)) / steps
 
 
 def cpu_baseline(cfg_path: str, cells: int, tmp: str) -> dict | None:
@@ -86,15 +93,16 @@ def cpu_baseline(cfg_path: str, cells: int, tmp: str) -> dict | None:
         return None
     threads = min(16, os.cpu_count() or 1)
     out = os.path.join(tmp, "cpu")
-    log(f"cpu baseline: reference engine, {threads} threads, first 300 CA steps + 50 sweeps")
-    t = run_engine(REF_ENGINE, cfg_path, out, ["--threads", str(threads), "--set", "ca.maxSteps=300",
-                                               "--set", "mc.mcs=50", "--set", "output.format=none"])
+    log(f"cpu baseline: reference engine, {threads} threads, full CA phase + first 20 sweeps")
+    t = run_engine(REF_ENGINE, cfg_path, out, ["--threads", str(threads), "--set", "mc.mcs=20",
+                                               "--set", "output.format=none"])
     ca_rate = cells * t["steps"] / (t["caSim"] / 1000.0)
     mc_rate = t["domainCells"] * t["mcs"] / (t["mcSim"] / 1000.0) if t.get("mcSim") else None
     return {"value": ca_rate, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
             "mc_value": mc_rate, "mc_unit": "spin-attempts/s",
-            "sample": f"reference engine on the same workload, first {t['steps']} CA steps and {t['mcs']} Potts "
-                      f"sweeps ({t['domainCells']} domain cells); caSim {t['caSim']:.0f} ms, mcSim {t['mcSim']:.0f} ms"}
+            "sample": f"reference engine (oracle/_ref, OpenMP) on the same workload: all {t['steps']} CA steps and the "
+                      f"first {t['mcs']} Potts sweeps ({t['domainCells']} domain cells); caSim {t['caSim']:.0f} ms, "
+                      f"mcSim {t['mcSim']:.0f} ms"}
 
 
 def main() -> int:
@@ -154,9 +162,14 @@ def main() -> int:
         if os.path.exists(PROFILE_TRAFFIC):
             with open(PROFILE_TRAFFIC) as f:
                 traffic = json.load(f).get(dom_k)
+        w = runs[0].get("detail", {}).get("caWork", {})
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_k,
                     "kernel_mean_ms": round(mean_ms, 5), "algorithmic_bytes_per_launch": round(bytes_per_launch[dom_k]),
+                    "bytes_per_unit": f"{CA_BYTES_PER_CELL} B per processed cell-step" if dom_k == "ca_rules" else None,
+                    "processed_cells_per_launch": round(w.get("tilesVisited", 0) * w.get("tileCells", 0) /
+                                                        max(1, steps)),
+                    "minimal_bytes_per_launch": round(minimal_bytes(runs[0])),
                     "kernel_share": {k: round(v[0] / sum(x[0] for x in ktot.values()), 3) for k, v in ktot.items()}}
         cpu = None if a.no_cpu else cpu_baseline(cfg_path, cells, tmp)
     result = {
@@ -174,7 +187,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u8/i32/f32 (integer and float32 rules, bit-exact)",
+        "dtype": "u8+i32+f32",
         "data": "synthetic (seeded terrain)",
         "config": {"workload": f"cone_3d {nx}x{ny}x{nz} moore absorbing, 3 craters, maxSteps 1000, "
                                f"Potts Q64 kT0.5 200 sweeps (checkerboard)", "cells": cells, "ca_steps": steps,
