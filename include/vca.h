@@ -78,6 +78,8 @@ typedef struct {
     float erupt_temp, solidus_temp, cooling_ambient, cooling_lateral, cooling_fall;
     int32_t ponding;             /* CaParams::ponding */
     int32_t dense;               /* 0: visit only tiles near lava (ca.sweep = front); 1: every tile (full) */
+    int32_t tile_shape;          /* 0: default for the dims; 1..5: 64x4x4, 64x2x8, 64x8x2, 64x1x16, 64x16x1
+                                    (results do not depend on it; see vca_autotune_shape) */
 } vca_ca_desc;
 
 typedef struct {
@@ -103,6 +105,31 @@ int vca_ca_histories(vca_ca* ca, uint64_t* active, uint64_t* changed, int32_t ca
 int vca_ca_download(vca_ca* ca, uint8_t* type, int32_t* flow, float* temp, uint16_t* solid, uint16_t* arrival);
 /* Work statistics: tiles visited and changes applied over all steps, tile size in cells. */
 int vca_ca_work(vca_ca* ca, uint64_t* tiles_visited, uint64_t* changes, int32_t* tile_cells);
+
+/* Device memory a simulation of these dims takes (state, change slots, tile lists), for admission control. */
+int vca_ca_footprint(const vca_ca_desc* desc, uint64_t* bytes);
+/* Tiles of the simulation (its share of a batch's tile range). */
+int vca_ca_tiles(vca_ca* ca, uint64_t* tiles);
+
+/* ---- batches of CA simulations -------------------------------------------------------------
+ * No counterpart in the reference (it runs one LavaCA per process; engine/src/app/Batch.cpp:40-120 spawns
+ * one engine per config).  Every step launch serves all running members: the work lists hold
+ * (simulation, tile) entries of every member, so many small grids fill the GPU together.  Members keep
+ * their own state and stop rules; results are identical to running each alone.  A simulation is in at
+ * most one batch; while it is, vca_ca_run / vca_ca_enqueue on it fail and vca_batch_* advance it. */
+typedef struct vca_batch vca_batch;
+int vca_batch_create(vca_ctx* ctx, int32_t max_sims, uint64_t max_tiles, vca_batch** out);
+int vca_batch_destroy(vca_batch* b);           /* detaches the remaining members */
+int vca_batch_add(vca_batch* b, vca_ca* ca);    /* VCA_ERR_NOMEM when no slot or tile range is free */
+int vca_batch_remove(vca_batch* b, vca_ca* ca);
+/* steps_per_graph > 0: step launches are captured once per membership as a hipGraph of that many steps
+ * and replayed (steps past a member's stop are no-ops); 0: direct launches. */
+int vca_batch_set_graph(vca_batch* b, int32_t steps_per_graph);
+int vca_batch_enqueue(vca_batch* b, int32_t steps);
+/* Enqueues up to max_steps steps in chunks and returns when no member is running or the steps are done;
+ * *running = members still running. */
+int vca_batch_run(vca_batch* b, int32_t max_steps, int32_t* running);
+int vca_batch_info(vca_batch* b, int32_t* members, uint64_t* tiles_used, int32_t* running);
 
 /* ---- Potts MC: replaces PottsMC ------------------------------------------------------------ */
 typedef struct {

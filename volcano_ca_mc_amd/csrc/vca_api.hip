@@ -4,10 +4,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -149,7 +151,7 @@ std::vector<uint32_t> variant_masks(int kind, int nx, int ny, int nz) {
 }
 
 NbTable make_nb(int kind, int nx, int ny, int nz) {
-    NbTable t;
+    NbTable t{};
     const std::vector<uint32_t> m = variant_masks(kind, nx, ny, nz);
     t.variants = static_cast<int32_t>(m.size());
     for (size_t v = 0; v < m.size(); ++v) {
@@ -163,6 +165,13 @@ NbTable make_nb(int kind, int nx, int ny, int nz) {
         if (k != 4 && ((t.any_mask >> k) & 1u)) t.diagonal_falls = 1;  // dz == -1 with dx|dy != 0
     t.conn_mask = variant_masks(kind == VCA_VON_NEUMANN ? VCA_VON_NEUMANN : VCA_MOORE, nx, ny, nz)[0];
     return t;
+}
+
+int default_shape(int nx, int ny, int nz) {
+    (void)nx;
+    if (ny == 1) return 3;  // x-z plane: 64 x 1 x 16
+    if (nz == 1) return 4;  // x-y plane: 64 x 16 x 1
+    return 0;               // 3D: 64 x 4 x 4
 }
 
 }  // namespace
@@ -216,14 +225,273 @@ struct vca_ctx {
     }
 };
 
+struct Batch;
+
 struct vca_ca {
     vca_ctx* ctx = nullptr;
     CaParams P{};
     CaDev D{};
-    DevBuf type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, active, lava_tiles, changes, tile_nchg,
-        overflow, part, ctrl, hist_a, hist_c;
-    int grid_tiles = 1;
+    int shape = 0;
+    DevBuf type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, changes, tile_nchg, tile_nwide,
+        part, ctrl, hist_a, hist_c;
     uint64_t n = 0;
+    std::unique_ptr<Batch> own;   // batch of one: the solo path
+    Batch* member = nullptr;      // the shared batch this simulation is in
+    int slot = -1;
+};
+
+// Entries of the per-device simulation table (kSimTable) handed out to batches as contiguous ranges.
+class SimTable {
+public:
+    static uint32_t take(int device, int n) {
+        std::lock_guard<std::mutex> lock(mu());
+        std::vector<char>& used = map()[device];
+        used.resize(kSimTable, 0);
+        for (int b = 0; b + n <= kSimTable; ++b) {
+            bool ok = true;
+            for (int k = 0; k < n && ok; ++k) ok = !used[static_cast<size_t>(b + k)];
+            if (!ok) continue;
+            for (int k = 0; k < n; ++k) used[static_cast<size_t>(b + k)] = 1;
+            return static_cast<uint32_t>(b);
+        }
+        throw VcaError(VCA_ERR_NOMEM, "simulation table full (" + std::to_string(kSimTable) + " entries per device)");
+    }
+    static void give(int device, uint32_t base, int n) {
+        std::lock_guard<std::mutex> lock(mu());
+        std::vector<char>& used = map()[device];
+        for (int k = 0; k < n && base + k < used.size(); ++k) used[base + k] = 0;
+    }
+
+private:
+    static std::mutex& mu() {
+        static std::mutex m;
+        return m;
+    }
+    static std::map<int, std::vector<char>>& map() {
+        static std::map<int, std::vector<char>> m;
+        return m;
+    }
+};
+
+// Simulations advanced together (see vca_batch_* in vca.h).  The solo path of a vca_ca is a batch of one.
+struct Batch {
+    vca_ctx* ctx = nullptr;
+    int max_slots = 0;
+    uint32_t cap = 0;                             // tiles
+    DevBuf owner, tile_base, active, lava, wide, ctrl;
+    uint32_t sim_base = 0;
+    bool has_table = false;
+    std::vector<CaSim> host_sims;                 // what the table entries hold (the copies read from here)
+    std::vector<vca_ca*> members;                 // slot -> simulation
+    std::vector<std::pair<uint32_t, uint32_t>> free_ranges;  // (first tile, count), sorted
+    uint32_t hwm = 0;
+    BatchDev B{};
+    int grid_tiles = 1;
+    int steps_per_graph = 0;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+
+    void init(vca_ctx* c, int slots, uint32_t tiles) {
+        ctx = c;
+        max_slots = slots;
+        cap = std::max<uint32_t>(tiles, 1u);
+        hipStream_t s = ctx->stream;
+        sim_base = SimTable::take(ctx->device, slots);
+        has_table = true;
+        host_sims.assign(static_cast<size_t>(slots), CaSim{});
+        for (int k = 0; k < slots; ++k) ca_write_sim(sim_base + static_cast<uint32_t>(k), &host_sims[static_cast<size_t>(k)], s);
+        check(hipGetLastError(), "simulation table");
+        owner.alloc(static_cast<size_t>(cap) * 4);
+        tile_base.alloc(static_cast<size_t>(slots) * 4);
+        active.alloc(static_cast<size_t>(cap) * kShapes * 4);
+        lava.alloc(static_cast<size_t>(cap) * kShapes * 4);
+        wide.alloc(static_cast<size_t>(cap) * 4);
+        ctrl.alloc(sizeof(BatchCtrl));
+        check(hipMemsetAsync(owner.p, 0xFF, owner.bytes, s), "memset");  // kNoSlot
+        check(hipMemsetAsync(ctrl.p, 0, sizeof(BatchCtrl), s), "memset");
+        members.assign(static_cast<size_t>(slots), nullptr);
+        free_ranges.clear();
+        hwm = 0;
+        B.sim_base = sim_base;
+        B.owner = owner.as<uint32_t>();
+        B.tile_base = tile_base.as<uint32_t>();
+        B.ntiles = 0;
+        B.nslots = 0;
+        B.cap = cap;
+        B.active = active.as<uint32_t>();
+        B.lava = lava.as<uint32_t>();
+        B.wide = wide.as<uint32_t>();
+        B.ctrl = ctrl.as<BatchCtrl>();
+        grid_tiles = static_cast<int>(std::min<uint32_t>(cap, 8192u));
+    }
+    ~Batch() {
+        drop_graph();
+        if (has_table) {
+            (void)hipStreamSynchronize(ctx->stream);
+            SimTable::give(ctx->device, sim_base, max_slots);
+        }
+    }
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_steps = 0;
+    }
+    uint32_t shapes() const {
+        uint32_t m = 0;
+        for (const vca_ca* c : members)
+            if (c) m |= 1u << c->shape;
+        return m;
+    }
+    bool any_lava_mode() const {
+        for (const vca_ca* c : members)
+            if (c && !c->P.classic) return true;
+        return false;
+    }
+    int count() const {
+        int k = 0;
+        for (const vca_ca* c : members) k += c != nullptr;
+        return k;
+    }
+    // First-fit tile range; false when none is free.
+    bool take_range(uint32_t n, uint32_t& base) {
+        for (size_t k = 0; k < free_ranges.size(); ++k) {
+            if (free_ranges[k].second < n) continue;
+            base = free_ranges[k].first;
+            free_ranges[k].first += n;
+            free_ranges[k].second -= n;
+            if (free_ranges[k].second == 0) free_ranges.erase(free_ranges.begin() + static_cast<long>(k));
+            return true;
+        }
+        if (static_cast<uint64_t>(hwm) + n > cap) return false;
+        base = hwm;
+        hwm += n;
+        return true;
+    }
+    void give_range(uint32_t base, uint32_t n) {
+        free_ranges.emplace_back(base, n);
+        std::sort(free_ranges.begin(), free_ranges.end());
+        std::vector<std::pair<uint32_t, uint32_t>> merged;
+        for (auto& r : free_ranges) {
+            if (!merged.empty() && merged.back().first + merged.back().second == r.first)
+                merged.back().second += r.second;
+            else
+                merged.push_back(r);
+        }
+        free_ranges.swap(merged);
+        while (!free_ranges.empty() && free_ranges.back().first + free_ranges.back().second == hwm) {
+            hwm = free_ranges.back().first;
+            free_ranges.pop_back();
+        }
+    }
+    void refresh() {
+        int top = 0;
+        for (int k = 0; k < max_slots; ++k)
+            if (members[static_cast<size_t>(k)]) top = k + 1;
+        B.nslots = static_cast<uint32_t>(std::max(top, 1));
+        B.ntiles = hwm;
+        drop_graph();
+    }
+    int attach(vca_ca* ca) {
+        int slot = -1;
+        for (int k = 0; k < max_slots; ++k)
+            if (!members[static_cast<size_t>(k)]) {
+                slot = k;
+                break;
+            }
+        if (slot < 0) throw VcaError(VCA_ERR_NOMEM, "batch: no free simulation slot");
+        uint32_t base = 0;
+        if (!take_range(ca->P.ntiles, base)) throw VcaError(VCA_ERR_NOMEM, "batch: no free tile range");
+        hipStream_t s = ctx->stream;
+        CaSim& sim = host_sims[static_cast<size_t>(slot)];
+        sim = CaSim{};
+        sim.P = ca->P;
+        sim.D = ca->D;
+        sim.running = 1;
+        sim.shape = ca->shape;
+        ca_write_sim(sim_base + static_cast<uint32_t>(slot), &sim, s);
+        check(hipMemcpyAsync(tile_base.as<uint32_t>() + slot, &base, 4, hipMemcpyHostToDevice, s), "batch add");
+        ca_launch_owner(owner.as<uint32_t>(), base, ca->P.ntiles, static_cast<uint32_t>(slot), s);
+        check(hipGetLastError(), "batch add");
+        check(hipStreamSynchronize(s), "batch add");
+        members[static_cast<size_t>(slot)] = ca;
+        bases.resize(static_cast<size_t>(max_slots));
+        bases[static_cast<size_t>(slot)] = base;
+        refresh();
+        return slot;
+    }
+    void detach(vca_ca* ca, int slot) {
+        hipStream_t s = ctx->stream;
+        host_sims[static_cast<size_t>(slot)].running = 0;
+        ca_write_sim(sim_base + static_cast<uint32_t>(slot), &host_sims[static_cast<size_t>(slot)], s);
+        const uint32_t base = bases[static_cast<size_t>(slot)];
+        ca_launch_owner(owner.as<uint32_t>(), base, ca->P.ntiles, kNoSlot, s);
+        check(hipGetLastError(), "batch remove");
+        check(hipStreamSynchronize(s), "batch remove");
+        give_range(base, ca->P.ntiles);
+        members[static_cast<size_t>(slot)] = nullptr;
+        refresh();
+    }
+    std::vector<uint32_t> bases;
+
+    void launch_direct(int steps) {
+        vca_ctx* c = ctx;
+        const uint32_t sh = shapes();
+        const bool lm = any_lava_mode();
+        for (int k = 0; k < steps; ++k) {
+            if (!c->timing) {
+                ca_launch_step(B, sh, lm, grid_tiles, c->stream, nullptr);
+                continue;
+            }
+            hipEvent_t ev[7];
+            size_t idx[7];
+            for (int e = 0; e < 7; ++e) {
+                ev[e] = c->take();
+                idx[e] = c->used - 1;
+            }
+            ca_launch_step(B, sh, lm, grid_tiles, c->stream, ev);
+            c->mark("ca_front", idx[0], idx[1]);
+            if (lm) c->mark("ca_descent", idx[1], idx[2]);
+            c->mark("ca_rules", idx[2], idx[3]);
+            c->mark("ca_wide", idx[3], idx[4]);
+            c->mark("ca_apply", idx[4], idx[5]);
+            c->mark("ca_finish", idx[5], idx[6]);
+            c->mark("ca_step", idx[0], idx[6]);
+            if (c->used > 7 * 256) c->drain();  // bound the pool on long enqueues
+        }
+        check(hipGetLastError(), "ca launch");
+    }
+    // Captures steps_per_graph steps of the current membership (no memset or host work inside).
+    bool capture() {
+        hipStream_t s = ctx->stream;
+        if (s == nullptr) return false;  // the legacy stream cannot be captured
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        const uint32_t sh = shapes();
+        const bool lm = any_lava_mode();
+        for (int k = 0; k < steps_per_graph; ++k) ca_launch_step(B, sh, lm, grid_tiles, s, nullptr);
+        check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        check(e, "hipGraphInstantiate");
+        graph = exec;
+        graph_steps = steps_per_graph;
+        return true;
+    }
+    void enqueue(int steps) {
+        if (steps <= 0 || count() == 0) return;
+        if (steps_per_graph > 0 && !ctx->timing) {
+            if (!graph && !capture()) steps_per_graph = 0;
+            while (graph && steps >= graph_steps) {
+                check(hipGraphLaunch(graph, ctx->stream), "hipGraphLaunch");
+                steps -= graph_steps;
+            }
+        }
+        launch_direct(steps);
+    }
 };
 
 struct vca_mc {
@@ -370,21 +638,16 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         P.ponding = d->ponding != 0;
         P.dense = d->dense != 0;
         P.nb = make_nb(d->neighborhood, d->nx, d->ny, d->nz);
-        if (d->ny == 1) {
-            P.ty = 1;
-            P.tz = 16;
-        } else if (d->nz == 1) {
-            P.ty = 16;
-            P.tz = 1;
-        } else {
-            P.ty = 4;
-            P.tz = 4;
-        }
+        require(d->tile_shape >= 0 && d->tile_shape <= kShapes, "tile_shape poza zakresem 0..5");
+        ca->shape = d->tile_shape > 0 ? d->tile_shape - 1 : default_shape(d->nx, d->ny, d->nz);
+        P.ty = kShapeTY[ca->shape];
+        P.tz = kShapeTZ[ca->shape];
         P.ntx = (d->nx + kTileX - 1) / kTileX;
         P.nty = (d->ny + P.ty - 1) / P.ty;
         P.ntz = (d->nz + P.tz - 1) / P.tz;
-        P.ntiles = static_cast<uint32_t>(static_cast<uint64_t>(P.ntx) * P.nty * P.ntz);
-        ca->grid_tiles = static_cast<int>(std::min<uint32_t>(P.ntiles, 8192u));
+        const uint64_t ntiles = static_cast<uint64_t>(P.ntx) * P.nty * P.ntz;
+        require(ntiles <= kTileMask, "za dużo kafli dla tej siatki");
+        P.ntiles = static_cast<uint32_t>(ntiles);
 
         // vents: flowId -> cell (isVent, LavaCA.hpp:335-341)
         int32_t maxf = 0;
@@ -414,12 +677,10 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         ca->descent.alloc(n);
         ca->vent_cell.alloc(vent_table.size() * 4);
         ca->tile_flags.alloc(P.ntiles);
-        ca->active.alloc(static_cast<size_t>(P.ntiles) * 4);
         ca->changes.alloc(static_cast<size_t>(P.ntiles) * kTileCells * sizeof(Change));
         ca->tile_nchg.alloc(static_cast<size_t>(P.ntiles) * 4);
+        ca->tile_nwide.alloc(static_cast<size_t>(P.ntiles) * 4);
         ca->part.alloc(static_cast<size_t>(kPartRows) * sizeof(CaPart));
-        ca->lava_tiles.alloc(static_cast<size_t>(P.ntiles) * 4);
-        ca->overflow.alloc(n * sizeof(uint32_t));
         ca->ctrl.alloc(sizeof(CaCtrl));
         const size_t hcap = static_cast<size_t>(std::max(1, d->max_steps));
         ca->hist_a.alloc(hcap * 8);
@@ -446,12 +707,10 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         D.descent = ca->descent.as<int8_t>();
         D.vent_cell = ca->vent_cell.as<uint32_t>();
         D.tile_flags = ca->tile_flags.as<uint8_t>();
-        D.active = ca->active.as<uint32_t>();
         D.changes = ca->changes.as<Change>();
-        D.overflow = ca->overflow.as<uint32_t>();
         D.tile_nchg = ca->tile_nchg.as<uint32_t>();
+        D.tile_nwide = ca->tile_nwide.as<uint32_t>();
         D.part = ca->part.as<CaPart>();
-        D.lava_tiles = ca->lava_tiles.as<uint32_t>();
         D.ctrl = ca->ctrl.as<CaCtrl>();
         D.hist_active = ca->hist_a.as<unsigned long long>();
         D.hist_changed = ca->hist_c.as<unsigned long long>();
@@ -493,6 +752,9 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         c0.stop = stop;
         check(hipMemcpyAsync(ca->ctrl.p, &c0, sizeof(CaCtrl), hipMemcpyHostToDevice, s), "write ctrl");
         check(hipStreamSynchronize(s), "init");
+        ca->own = std::make_unique<Batch>();
+        ca->own->init(ctx, 1, P.ntiles);
+        ca->own->attach(ca.get());
         *out = ca.release();
     });
 }
@@ -502,6 +764,8 @@ extern "C" int vca_ca_destroy(vca_ca* ca) {
         if (!ca) return;
         ca->ctx->set_device();
         (void)hipStreamSynchronize(ca->ctx->stream);
+        if (ca->member) ca->member->detach(ca, ca->slot);
+        ca->member = nullptr;
         delete ca;
     });
 }
@@ -518,36 +782,17 @@ static void ca_status_sync(vca_ca* ca, int32_t* steps, int32_t* stop) {
 extern "C" int vca_ca_enqueue(vca_ca* ca, int32_t new_steps) {
     return guarded([&] {
         require(ca, "ca == NULL");
+        if (ca->member) throw VcaError(VCA_ERR_STATE, "the simulation is in a batch: advance it with vca_batch_*");
         ca->ctx->set_device();
-        vca_ctx* c = ca->ctx;
-        for (int k = 0; k < new_steps; ++k) {
-            if (!c->timing) {
-                ca_launch_step(ca->P, ca->D, ca->grid_tiles, c->stream, nullptr);
-                continue;
-            }
-            hipEvent_t ev[7];
-            size_t idx[7];
-            for (int e = 0; e < 7; ++e) {
-                ev[e] = c->take();
-                idx[e] = c->used - 1;
-            }
-            ca_launch_step(ca->P, ca->D, ca->grid_tiles, c->stream, ev);
-            c->mark("ca_front", idx[0], idx[1]);
-            if (!ca->P.classic) c->mark("ca_descent", idx[1], idx[2]);
-            c->mark("ca_rules", idx[2], idx[3]);
-            c->mark("ca_wide", idx[3], idx[4]);
-            c->mark("ca_apply", idx[4], idx[5]);
-            c->mark("ca_finish", idx[5], idx[6]);
-            c->mark("ca_step", idx[0], idx[6]);
-            if (c->used > 7 * 256) c->drain();  // bound the pool on long enqueues
-        }
-        check(hipGetLastError(), "ca launch");
+        // the private batch's members[0] is this simulation; its slot there is 0
+        ca->own->enqueue(new_steps);
     });
 }
 
 extern "C" int vca_ca_run(vca_ca* ca, int32_t max_new_steps, int32_t* steps_done, int32_t* stop_reason) {
     return guarded([&] {
         require(ca, "ca == NULL");
+        if (ca->member) throw VcaError(VCA_ERR_STATE, "the simulation is in a batch: advance it with vca_batch_*");
         ca->ctx->set_device();
         int32_t steps = 0, stop = 0;
         ca_status_sync(ca, &steps, &stop);
@@ -612,6 +857,149 @@ extern "C" int vca_ca_work(vca_ca* ca, uint64_t* tiles_visited, uint64_t* change
         if (tiles_visited) *tiles_visited = c.tiles_visited;
         if (changes) *changes = c.changes_total;
         if (tile_cells) *tile_cells = kTileCells;
+    });
+}
+
+extern "C" int vca_ca_footprint(const vca_ca_desc* d, uint64_t* bytes) {
+    return guarded([&] {
+        require(d && bytes, "NULL argument");
+        require(d->nx >= 1 && d->ny >= 1 && d->nz >= 1, "wymiary muszą być >= 1");
+        const int shape = d->tile_shape > 0 ? d->tile_shape - 1 : default_shape(d->nx, d->ny, d->nz);
+        const uint64_t n = static_cast<uint64_t>(d->nx) * d->ny * d->nz;
+        const uint64_t tiles = static_cast<uint64_t>((d->nx + kTileX - 1) / kTileX) *
+                               static_cast<uint64_t>((d->ny + kShapeTY[shape] - 1) / kShapeTY[shape]) *
+                               static_cast<uint64_t>((d->nz + kShapeTZ[shape] - 1) / kShapeTZ[shape]);
+        const uint64_t cols = static_cast<uint64_t>(d->nx) * d->ny;
+        // state 14 B/cell, record slots 16 B per tile cell, per-tile counts and flags, the batch-of-one lists,
+        // spill levels, histories
+        *bytes = n * 14 + tiles * (kTileCells * sizeof(Change) + 9 + (2 * kShapes + 2) * 4) + cols * 4 +
+                 static_cast<uint64_t>(std::max(1, d->max_steps)) * 16 + kPartRows * sizeof(CaPart) + 4096;
+    });
+}
+
+extern "C" int vca_ca_tiles(vca_ca* ca, uint64_t* tiles) {
+    return guarded([&] {
+        require(ca && tiles, "NULL argument");
+        *tiles = ca->P.ntiles;
+    });
+}
+
+struct vca_batch {
+    Batch b;
+};
+
+extern "C" int vca_batch_create(vca_ctx* ctx, int32_t max_sims, uint64_t max_tiles, vca_batch** out) {
+    return guarded([&] {
+        require(ctx && out, "NULL argument");
+        require(max_sims >= 1 && max_sims <= kMaxSlots, "max_sims poza zakresem 1..1024");
+        require(max_tiles >= 1 && max_tiles <= kTileMask, "max_tiles poza zakresem");
+        ctx->set_device();
+        auto b = std::make_unique<vca_batch>();
+        b->b.init(ctx, max_sims, static_cast<uint32_t>(max_tiles));
+        check(hipStreamSynchronize(ctx->stream), "batch create");
+        *out = b.release();
+    });
+}
+
+extern "C" int vca_batch_destroy(vca_batch* b) {
+    return guarded([&] {
+        if (!b) return;
+        b->b.ctx->set_device();
+        (void)hipStreamSynchronize(b->b.ctx->stream);
+        for (vca_ca* c : b->b.members)
+            if (c) {
+                c->member = nullptr;
+                c->slot = -1;
+            }
+        delete b;
+    });
+}
+
+extern "C" int vca_batch_add(vca_batch* b, vca_ca* ca) {
+    return guarded([&] {
+        require(b && ca, "NULL argument");
+        require(ca->ctx == b->b.ctx, "the simulation and the batch use different contexts");
+        if (ca->member) throw VcaError(VCA_ERR_STATE, "the simulation is already in a batch");
+        b->b.ctx->set_device();
+        ca->slot = b->b.attach(ca);
+        ca->member = &b->b;
+    });
+}
+
+extern "C" int vca_batch_remove(vca_batch* b, vca_ca* ca) {
+    return guarded([&] {
+        require(b && ca, "NULL argument");
+        if (ca->member != &b->b) throw VcaError(VCA_ERR_STATE, "the simulation is not in this batch");
+        b->b.ctx->set_device();
+        b->b.detach(ca, ca->slot);
+        ca->member = nullptr;
+        ca->slot = -1;
+    });
+}
+
+extern "C" int vca_batch_set_graph(vca_batch* b, int32_t steps_per_graph) {
+    return guarded([&] {
+        require(b && steps_per_graph >= 0 && steps_per_graph <= 4096, "invalid argument");
+        b->b.ctx->set_device();
+        b->b.drop_graph();
+        b->b.steps_per_graph = steps_per_graph;
+    });
+}
+
+extern "C" int vca_batch_enqueue(vca_batch* b, int32_t steps) {
+    return guarded([&] {
+        require(b && steps >= 0, "invalid argument");
+        b->b.ctx->set_device();
+        b->b.enqueue(steps);
+    });
+}
+
+namespace {
+// Running members: reads every member's step counter and stop flag (one copy each).
+int batch_running(Batch& b) {
+    int running = 0;
+    for (vca_ca* c : b.members) {
+        if (!c) continue;
+        int32_t stop = 0;
+        check(hipMemcpyAsync(&stop, reinterpret_cast<char*>(c->ctrl.p) + offsetof(CaCtrl, stop), 4,
+                             hipMemcpyDeviceToHost, b.ctx->stream),
+              "read ctrl");
+        check(hipStreamSynchronize(b.ctx->stream), "batch status");
+        running += stop == VCA_RUNNING;
+    }
+    b.ctx->drain();
+    return running;
+}
+}  // namespace
+
+extern "C" int vca_batch_run(vca_batch* b, int32_t max_steps, int32_t* running) {
+    return guarded([&] {
+        require(b && max_steps >= 0, "invalid argument");
+        b->b.ctx->set_device();
+        int live = batch_running(b->b);
+        int remaining = max_steps;
+        while (remaining > 0 && live > 0) {
+            const int chunk = std::min(remaining, 128);
+            b->b.enqueue(chunk);
+            remaining -= chunk;
+            live = batch_running(b->b);
+        }
+        if (running) *running = live;
+    });
+}
+
+extern "C" int vca_batch_info(vca_batch* b, int32_t* members, uint64_t* tiles_used, int32_t* running) {
+    return guarded([&] {
+        require(b, "b == NULL");
+        b->b.ctx->set_device();
+        if (members) *members = b->b.count();
+        if (tiles_used) {
+            uint64_t t = 0;
+            for (vca_ca* c : b->b.members)
+                if (c) t += c->P.ntiles;
+            *tiles_used = t;
+        }
+        if (running) *running = batch_running(b->b);
     });
 }
 

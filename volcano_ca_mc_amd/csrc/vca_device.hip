@@ -27,6 +27,10 @@
 
 namespace vca {
 
+// The simulation table (see BatchDev).  In the constant address space, uniform reads of an entry are scalar
+// loads kept in SGPRs, like kernel arguments.
+__constant__ CaSim g_sims[kSimTable];
+
 namespace {
 
 __device__ inline bool wrap_axis(int& v, int n, int per) {
@@ -53,32 +57,6 @@ __device__ inline uint32_t wave_append(bool pred, uint32_t* counter) {
         base = __shfl(base, leader);
     }
     return base + static_cast<uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL)));
-}
-
-// Sums three per-thread counters over the block and adds them to one CaPart row (every thread calls).
-__device__ inline void block_part(CaPart* row, unsigned long long a, unsigned long long b, unsigned long long c) {
-    __shared__ unsigned long long red[3][kBlock / 64];
-    a = wave_sum(a);
-    b = wave_sum(b);
-    c = wave_sum(c);
-    const int w = static_cast<int>(threadIdx.x >> 6);
-    if ((threadIdx.x & 63u) == 0) {
-        red[0][w] = a;
-        red[1][w] = b;
-        red[2][w] = c;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long r0 = 0, r1 = 0, r2 = 0;
-        for (int k = 0; k < kBlock / 64; ++k) {
-            r0 += red[0][k];
-            r1 += red[1][k];
-            r2 += red[2][k];
-        }
-        if (r0) atomicAdd(&row->lava_after, r0);
-        if (r1) atomicAdd(&row->born, r1);
-        if (r2) atomicAdd(&row->changed, r2);
-    }
 }
 
 // Reduces N per-thread values over the block (a sum, or a max where bit i of max_mask is set) and folds the
@@ -114,6 +92,18 @@ __device__ inline void block_fold(unsigned long long (&v)[N], uint32_t max_mask,
                 atomicAdd(&row[i], r);
         }
     }
+}
+
+// Variant v's mask by selects, not by a dynamic index into the table (which would force a register copy of
+// the table into private memory).
+__device__ inline uint32_t nb_mask(const NbTable& nb, uint32_t v) {
+    uint32_t m = nb.mask[0];
+    m = v == 1 ? nb.mask[1] : m;
+    m = v == 2 ? nb.mask[2] : m;
+    m = v == 3 ? nb.mask[3] : m;
+    m = v == 4 ? nb.mask[4] : m;
+    m = v == 5 ? nb.mask[5] : m;
+    return m;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -399,42 +389,61 @@ __global__ void __launch_bounds__(kBlock) ca_init_flags_kernel(CaParams P, CaDev
 // CA step kernels
 // ------------------------------------------------------------------------------------------------
 
-// Visited-tile lists of the step.  A tile is visited when it holds lava (lava cools) or holds AIR next to a
-// tile with lava: the reference's "may change" set (CaRules.hpp:69-79, LavaCA.hpp:844-871) at tile
-// granularity.  In lava mode the tiles holding lava also go to a second list, the cells whose D(n) is needed.
-__global__ void __launch_bounds__(kBlock) ca_front_kernel(CaParams P, CaDev D) {
-    if (D.ctrl->stop) return;
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+// Visited-tile lists of one step of every running simulation of the batch (one lane per global tile).  A tile
+// is visited when it holds lava (lava cools) or holds AIR next to a tile with lava: the reference's "may
+// change" set (CaRules.hpp:69-79, LavaCA.hpp:844-871) at tile granularity.  In lava mode the tiles holding
+// lava also go to a second list: the cells whose D(n) is needed.  One list per tile shape; wave ballot + one
+// atomic per wave appends the entries.
+__global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B) {
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
     bool act = false, lava = false;
-    if (t < P.ntiles) {
-        const uint8_t f = D.tile_flags[t];
-        lava = (f & kFlagLava) != 0;
-        if (P.dense || (!P.classic && lava) || (P.classic && lava && (f & kFlagAir))) {
-            act = true;
-        } else if (f & kFlagAir) {
-            const int tx = static_cast<int>(t % static_cast<uint32_t>(P.ntx));
-            const uint32_t r = t / static_cast<uint32_t>(P.ntx);
-            const int ty = static_cast<int>(r % static_cast<uint32_t>(P.nty));
-            const int tz = static_cast<int>(r / static_cast<uint32_t>(P.nty));
-            uint8_t any = 0;  // all 26 neighbour flags are loaded together (no early exit)
+    int shape = -1;
+    uint32_t entry = 0;
+    if (g < B.ntiles) {
+        const uint32_t slot = B.owner[g];
+        if (slot != kNoSlot && g_sims[B.sim_base + slot].running && !g_sims[B.sim_base + slot].D.ctrl->stop) {
+            const CaParams& P = g_sims[B.sim_base + slot].P;
+            const uint32_t t = g - B.tile_base[slot];
+            const uint8_t* flags = g_sims[B.sim_base + slot].D.tile_flags;
+            const uint8_t f = flags[t];
+            lava = !P.classic && (f & kFlagLava);
+            shape = g_sims[B.sim_base + slot].shape;
+            entry = (slot << kTileBits) | t;
+            if (P.dense || lava || (P.classic && (f & kFlagLava) && (f & kFlagAir))) {
+                act = true;
+            } else if (f & kFlagAir) {
+                const int tx = static_cast<int>(t % static_cast<uint32_t>(P.ntx));
+                const uint32_t r = t / static_cast<uint32_t>(P.ntx);
+                const int ty = static_cast<int>(r % static_cast<uint32_t>(P.nty));
+                const int tz = static_cast<int>(r / static_cast<uint32_t>(P.nty));
+                uint8_t any = 0;  // all 26 neighbour flags are loaded together (no early exit)
 #pragma unroll
-            for (int q = 0; q < 27; ++q) {
-                if (q == 13) continue;
-                int qx = tx + q % 3 - 1, qy = ty + (q / 3) % 3 - 1, qz = tz + q / 9 - 1;
-                if (!wrap_axis(qx, P.ntx, P.per[0]) || !wrap_axis(qy, P.nty, P.per[1]) || !wrap_axis(qz, P.ntz, P.per[2]))
-                    continue;
-                any |= D.tile_flags[static_cast<uint32_t>(qx) +
-                                    static_cast<uint32_t>(P.ntx) *
-                                        (static_cast<uint32_t>(qy) + static_cast<uint32_t>(P.nty) * static_cast<uint32_t>(qz))];
+                for (int q = 0; q < 27; ++q) {
+                    if (q == 13) continue;
+                    int qx = tx + q % 3 - 1, qy = ty + (q / 3) % 3 - 1, qz = tz + q / 9 - 1;
+                    if (!wrap_axis(qx, P.ntx, P.per[0]) || !wrap_axis(qy, P.nty, P.per[1]) ||
+                        !wrap_axis(qz, P.ntz, P.per[2]))
+                        continue;
+                    any |= flags[static_cast<uint32_t>(qx) +
+                                 static_cast<uint32_t>(P.ntx) *
+                                     (static_cast<uint32_t>(qy) + static_cast<uint32_t>(P.nty) * static_cast<uint32_t>(qz))];
+                }
+                act = (any & kFlagLava) != 0;
             }
-            act = (any & kFlagLava) != 0;
         }
     }
-    const uint32_t slot = wave_append(act, &D.ctrl->nactive);
-    if (act) D.active[slot] = t;
-    if (!P.classic) {
-        const uint32_t lslot = wave_append(lava, &D.ctrl->nlava);
-        if (lava) D.lava_tiles[lslot] = t;
+#pragma unroll
+    for (int k = 0; k < kShapes; ++k) {
+        const bool a = act && shape == k;
+        if (__ballot(a)) {
+            const uint32_t slot = wave_append(a, &B.ctrl->nactive[k]);
+            if (a) B.active[k * B.cap + slot] = entry;
+        }
+        const bool l = lava && shape == k;
+        if (__ballot(l)) {
+            const uint32_t slot = wave_append(l, &B.ctrl->nlava[k]);
+            if (l) B.lava[k * B.cap + slot] = entry;
+        }
     }
 }
 
@@ -571,20 +580,24 @@ __device__ int drop_depth_lds(const CaDev& D, const uint8_t* s_type, int c, int 
 // D(n) of every lava cell of the tiles holding lava (LavaCA.hpp:421-429, 877-888), from the LDS-staged tile:
 // hasSupport of the cell, then the deepest dropDepth over its AIR side neighbours (union of the variants).
 template <int TY, int TZ>
-__global__ void __launch_bounds__(kBlock) ca_descent_kernel(CaParams P, CaDev D) {
+__global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shape) {
     using H = Halo<TY, TZ>;
     __shared__ HaloTypes<TY, TZ> S;
     __shared__ int32_t s_spill[H::LXY];
     __shared__ uint16_t s_list[kTileCells];
     __shared__ uint32_t s_n;
-    if (D.ctrl->stop || P.classic) return;
     const uint8_t* s_type = S.type;
-    const uint32_t n = D.ctrl->nlava;
-    const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
-    const bool has_spill = D.spill != nullptr;
-    const uint32_t side = P.nb.side_mask;
+    const uint32_t n = B.ctrl->nlava[shape];
+    const uint32_t* tiles = B.lava + shape * B.cap;
     for (uint32_t s = blockIdx.x; s < n; s += gridDim.x) {
-        const TileCoord o = tile_origin(P, D.lava_tiles[s]);
+        const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[s]);  // uniform: one entry per block
+        const CaSim& sim = g_sims[B.sim_base + (e >> kTileBits)];
+        const CaParams& P = sim.P;
+        const CaDev& D = sim.D;
+        const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
+        const bool has_spill = D.spill != nullptr;
+        const uint32_t side = P.nb.side_mask;
+        const TileCoord o = tile_origin(P, e & kTileMask);
         __syncthreads();  // the previous tile's LDS reads are done
         int own_air = 0;
         stage_rows<TY, TZ>(P, D, o, S, own_air);
@@ -716,25 +729,31 @@ __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY
 // branches and gathers its candidates from LDS.  Cells whose vote overflows the four register slots are
 // handed to ca_wide.
 template <int TY, int TZ>
-__global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
+__global__ void __launch_bounds__(kBlock) ca_rules_kernel(BatchDev B, int shape) {
     using H = Halo<TY, TZ>;
     constexpr int LX = H::LX, LY = H::LY;
     __shared__ RulesTile<TY, TZ> T;
-    __shared__ uint32_t s_nchg, s_nlist;
-    if (D.ctrl->stop) return;
-    const uint32_t nact = D.ctrl->nactive;
-    const int32_t t = D.ctrl->steps;
-    const uint64_t prefix = hash_prefix(P.seed, static_cast<uint64_t>(t));
-    const bool erupting = t < P.eruption_steps;
-    const uint16_t stamp = static_cast<uint16_t>(t < 65535 ? t : 65535);
-    const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
-    const bool has_spill = !P.classic && D.spill != nullptr;
-    // positions the rules read as lava sources: dz >= 0, plus directly below for ponding (lava mode)
-    const uint32_t rules_mask =
-        P.classic ? P.nb.any_mask : ((P.nb.any_mask & ~0x1FFu) | (P.ponding ? (P.nb.any_mask & (1u << 4)) : 0u));
-    unsigned long long lava_after = 0, born = 0, changed = 0;
+    __shared__ uint32_t s_nchg, s_nlist, s_nwide;
+    __shared__ unsigned long long s_red[3][kBlock / 64];
+    const uint32_t nact = B.ctrl->nactive[shape];
+    const uint32_t* tiles = B.active + shape * B.cap;
     for (uint32_t s = blockIdx.x; s < nact; s += gridDim.x) {
-        const uint32_t tile = D.active[s];
+        const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[s]);  // uniform: one entry per block
+        const CaSim& sim = g_sims[B.sim_base + (e >> kTileBits)];
+        const CaParams& P = sim.P;
+        const CaDev& D = sim.D;
+        const uint32_t tile = e & kTileMask;
+        const int32_t t = D.ctrl->steps;
+        const uint64_t prefix = hash_prefix(P.seed, static_cast<uint64_t>(t));
+        const bool erupting = t < P.eruption_steps;
+        const uint16_t stamp = static_cast<uint16_t>(t < 65535 ? t : 65535);
+        const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
+        const bool has_spill = !P.classic && D.spill != nullptr;
+        // positions the rules read as lava sources: dz >= 0, plus directly below for ponding (lava mode)
+        const uint32_t rules_mask =
+            P.classic ? P.nb.any_mask
+                      : ((P.nb.any_mask & ~0x1FFu) | (P.ponding ? (P.nb.any_mask & (1u << 4)) : 0u));
+        unsigned long long lava_after = 0, born = 0, changed = 0;
         const TileCoord o = tile_origin(P, tile);
         __syncthreads();  // the previous tile's LDS reads are done
         int own_air = 0;
@@ -743,6 +762,7 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
         if (threadIdx.x == 0) {
             s_nchg = 0;
             s_nlist = 0;
+            s_nwide = 0;
         }
         if (!__syncthreads_or(saw_lava)) {
             // no lava in the tile or its halo: no cell changes; the tile keeps AIR iff it has AIR
@@ -750,6 +770,8 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
             if (threadIdx.x == 0) {
                 D.tile_flags[tile] = static_cast<uint8_t>(any_air ? kFlagAir : 0);
                 D.tile_nchg[tile] = 0;
+                D.tile_nwide[tile] = 0;
+                atomicAdd(&D.part[tile % kPartRows].tiles, 1ULL);
             }
             continue;
         }
@@ -905,7 +927,8 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
                         const uint64_t h_index = splitmix64(prefix ^ i);
                         uint32_t mask = P.nb.mask[0];
                         if (P.nb.variants > 1)
-                            mask = P.nb.mask[splitmix64(h_index ^ kSaltNeighborhood) % static_cast<uint64_t>(P.nb.variants)];
+                            mask = nb_mask(P.nb, static_cast<uint32_t>(splitmix64(h_index ^ kSaltNeighborhood) %
+                                                              static_cast<uint64_t>(P.nb.variants)));
                         const uint32_t bits = lava_bits & mask;
                         if (bits) {
                             const int spill_c = has_spill ? T.spill[(lx + 1) + LX * (ly + 1)] : 0;
@@ -935,118 +958,174 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(CaParams P, CaDev D) {
                     }
                 }
             }
+            Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
             const uint32_t slot = wave_append(want, &s_nchg);
-            if (want) D.changes[static_cast<size_t>(tile) * kTileCells + slot] = ch;
+            if (want) rec[slot] = ch;
             if (__ballot(wide)) {
-                const uint32_t oslot = wave_append(wide, &D.ctrl->noverflow);
-                if (wide) D.overflow[oslot] = i;
+                // records fill the tile's slots from 0 up, cells for ca_wide from kTileCells - 1 down; both
+                // together are at most the tile's candidates, so they never meet
+                const uint32_t wslot = wave_append(wide, &s_nwide);
+                if (wide) rec[kTileCells - 1 - wslot] = Change{i, 0, 0.0f, 0, 0, kKindWide};
             }
+        }
+        // tile flags, record counts and the tile's counters (row tile % kPartRows of its simulation)
+        lava_after = wave_sum(lava_after);
+        born = wave_sum(born);
+        changed = wave_sum(changed);
+        if ((threadIdx.x & 63u) == 0) {
+            s_red[0][threadIdx.x >> 6] = lava_after;
+            s_red[1][threadIdx.x >> 6] = born;
+            s_red[2][threadIdx.x >> 6] = changed;
         }
         const int any_lava = __syncthreads_or(tile_lava);
         const int any_air = __syncthreads_or(tile_air);
         if (threadIdx.x == 0) {
             D.tile_flags[tile] = static_cast<uint8_t>((any_lava ? kFlagLava : 0) | (any_air ? kFlagAir : 0));
             D.tile_nchg[tile] = s_nchg;
+            D.tile_nwide[tile] = s_nwide;
+            if (s_nwide) B.wide[atomicAdd(&B.ctrl->nwide, 1u)] = e;
+            unsigned long long r0 = 0, r1 = 0, r2 = 0;
+            for (int w = 0; w < kBlock / 64; ++w) {
+                r0 += s_red[0][w];
+                r1 += s_red[1][w];
+                r2 += s_red[2][w];
+            }
+            CaPart* row = &D.part[tile % kPartRows];
+            if (r0) atomicAdd(&row->lava_after, r0);
+            if (r1) atomicAdd(&row->born, r1);
+            if (r2) atomicAdd(&row->changed, r2);
+            atomicAdd(&row->tiles, 1ULL);
         }
     }
-    block_part(&D.part[blockIdx.x % kPartRows], lava_after, born, changed);
 }
 
-// AIR cells whose candidates carry more than four distinct flowIds (rare): the full vote over private
-// memory, then the change and counters the rules kernel leaves to this kernel.  Reads the start-of-step
-// state only, so it runs between the rules and the apply launch.
-__global__ void __launch_bounds__(kBlock) ca_wide_kernel(CaParams P, CaDev D) {
-    if (D.ctrl->stop) return;
-    const uint32_t n = D.ctrl->noverflow;
-    const int32_t t = D.ctrl->steps;
-    const uint64_t prefix = hash_prefix(P.seed, static_cast<uint64_t>(t));
-    const uint16_t stamp = static_cast<uint16_t>(t < 65535 ? t : 65535);
-    unsigned long long lava_after = 0, born = 0, changed = 0;
-    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
-        const uint32_t i = D.overflow[j];
-        const int x = static_cast<int>(i % static_cast<uint32_t>(P.nx));
-        const uint32_t r = i / static_cast<uint32_t>(P.nx);
-        const int y = static_cast<int>(r % static_cast<uint32_t>(P.ny));
-        const int z = static_cast<int>(r / static_cast<uint32_t>(P.ny));
-        const uint64_t h_index = splitmix64(prefix ^ i);
-        uint32_t mask = P.nb.mask[0];
-        if (P.nb.variants > 1)
-            mask = P.nb.mask[splitmix64(h_index ^ kSaltNeighborhood) % static_cast<uint64_t>(P.nb.variants)];
-        VoteWide v;
-        if (P.classic)
-            gather_classic(P, D, x, y, z, mask, v);
-        else
-            gather_lava(P, D, x, y, z, mask, v);
-        if (v.n == 0) continue;
-        int32_t wf = 0;
-        float wt = 0.0f;
-        v.winner(h_index, wf, wt);
-        if (!P.classic && wt < P.solidus) continue;
-        // the cell's tile: records of its rules pass are already counted in tile_nchg
-        const uint32_t tile = static_cast<uint32_t>(x / kTileX) +
-                              static_cast<uint32_t>(P.ntx) * (static_cast<uint32_t>(y / P.ty) +
-                                                               static_cast<uint32_t>(P.nty) * static_cast<uint32_t>(z / P.tz));
-        const uint32_t slot = atomicAdd(&D.tile_nchg[tile], 1u);
-        D.changes[static_cast<size_t>(tile) * kTileCells + slot] =
-            Change{i, wf, P.classic ? 0.0f : wt, stamp, kLava, kKindBorn};
-        ++lava_after;
-        ++born;
-        ++changed;
+// AIR cells whose candidates carry more than four distinct flowIds (rare): one block per tile that has such
+// cells; the full vote over private memory, then the record and counters the rules kernel left to this
+// kernel.  Reads the start-of-step state only, so it runs between the rules and the apply launches.
+__global__ void __launch_bounds__(kBlock) ca_wide_kernel(BatchDev B) {
+    __shared__ uint32_t s_cell[kTileCells];
+    const uint32_t n = B.ctrl->nwide;
+    for (uint32_t s = blockIdx.x; s < n; s += gridDim.x) {
+        const uint32_t e = __builtin_amdgcn_readfirstlane(B.wide[s]);
+        const CaSim& sim = g_sims[B.sim_base + (e >> kTileBits)];
+        const CaParams& P = sim.P;
+        const CaDev& D = sim.D;
+        const uint32_t tile = e & kTileMask;
+        Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
+        const uint32_t nw = D.tile_nwide[tile];
+        __syncthreads();  // the previous tile's cells are read
+        for (uint32_t j = threadIdx.x; j < nw; j += kBlock) s_cell[j] = rec[kTileCells - 1 - j].idx;
+        __syncthreads();  // born records may now reuse those slots
+        const int32_t t = D.ctrl->steps;
+        const uint64_t prefix = hash_prefix(P.seed, static_cast<uint64_t>(t));
+        const uint16_t stamp = static_cast<uint16_t>(t < 65535 ? t : 65535);
+        unsigned long long born = 0;
+        for (uint32_t j = threadIdx.x; j < nw; j += kBlock) {
+            const uint32_t i = s_cell[j];
+            const int x = static_cast<int>(i % static_cast<uint32_t>(P.nx));
+            const uint32_t r = i / static_cast<uint32_t>(P.nx);
+            const int y = static_cast<int>(r % static_cast<uint32_t>(P.ny));
+            const int z = static_cast<int>(r / static_cast<uint32_t>(P.ny));
+            const uint64_t h_index = splitmix64(prefix ^ i);
+            uint32_t mask = P.nb.mask[0];
+            if (P.nb.variants > 1)
+                mask = nb_mask(P.nb, static_cast<uint32_t>(splitmix64(h_index ^ kSaltNeighborhood) %
+                                                              static_cast<uint64_t>(P.nb.variants)));
+            VoteWide v;
+            if (P.classic)
+                gather_classic(P, D, x, y, z, mask, v);
+            else
+                gather_lava(P, D, x, y, z, mask, v);
+            if (v.n == 0) continue;
+            int32_t wf = 0;
+            float wt = 0.0f;
+            v.winner(h_index, wf, wt);
+            if (!P.classic && wt < P.solidus) continue;
+            // slots below the tile's other records: records + born <= candidates <= kTileCells
+            rec[atomicAdd(&D.tile_nchg[tile], 1u)] = Change{i, wf, P.classic ? 0.0f : wt, stamp, kLava, kKindBorn};
+            ++born;
+        }
+        if (born) {
+            CaPart* row = &D.part[tile % kPartRows];
+            atomicAdd(&row->lava_after, born);
+            atomicAdd(&row->born, born);
+            atomicAdd(&row->changed, born);
+        }
     }
-    block_part(&D.part[(blockIdx.x * 61u + 17u) % kPartRows], lava_after, born, changed);
 }
 
 // Scatters each visited tile's change records into the state (LavaCA.hpp:1052-1066).
-__global__ void __launch_bounds__(kBlock) ca_apply_kernel(CaDev D) {
-    if (D.ctrl->stop) return;
-    const uint32_t nact = D.ctrl->nactive;
-    for (uint32_t s = blockIdx.x; s < nact; s += gridDim.x) {
-        const uint32_t tile = D.active[s];
-        const uint32_t n = D.tile_nchg[tile];
-        const Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
-        for (uint32_t j = threadIdx.x; j < n; j += kBlock) {
-            const Change c = rec[j];
-            D.type[c.idx] = c.type;
-            D.temp[c.idx] = c.temp;
-            if (c.kind == kKindBorn) {
-                D.flow[c.idx] = c.flow;
-                D.arrival[c.idx] = c.stamp;
-            } else if (c.kind == kKindSolidify) {
-                D.solid[c.idx] = c.stamp;
+__global__ void __launch_bounds__(kBlock) ca_apply_kernel(BatchDev B) {
+#pragma unroll 1
+    for (int k = 0; k < kShapes; ++k) {
+        const uint32_t nact = B.ctrl->nactive[k];
+        const uint32_t* tiles = B.active + k * B.cap;
+        for (uint32_t s = blockIdx.x; s < nact; s += gridDim.x) {
+            const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[s]);  // uniform: one entry per block
+            const CaDev& D = g_sims[B.sim_base + (e >> kTileBits)].D;
+            const uint32_t tile = e & kTileMask;
+            const uint32_t n = D.tile_nchg[tile];
+            const Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
+            for (uint32_t j = threadIdx.x; j < n; j += kBlock) {
+                const Change c = rec[j];
+                D.type[c.idx] = c.type;
+                D.temp[c.idx] = c.temp;
+                if (c.kind == kKindBorn) {
+                    D.flow[c.idx] = c.flow;
+                    D.arrival[c.idx] = c.stamp;
+                } else if (c.kind == kKindSolidify) {
+                    D.solid[c.idx] = c.stamp;
+                }
             }
         }
     }
 }
 
-// finishStep (LavaCA.hpp:1206-1229) + frontCounts (LavaCA.hpp:865-871)
-__global__ void __launch_bounds__(kBlock) ca_finish_kernel(CaParams P, CaDev D) {
-    __shared__ unsigned long long red[3][kBlock / 64];
+// finishStep (LavaCA.hpp:1206-1229) + frontCounts (LavaCA.hpp:865-871), one block per slot.  Block 0 also
+// clears the batch's list counters for the next step (every kernel of this step has finished with them).
+__global__ void __launch_bounds__(kBlock) ca_finish_kernel(BatchDev B) {
+    __shared__ unsigned long long red[4][kBlock / 64];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int k = 0; k < kShapes; ++k) {
+            B.ctrl->nactive[k] = 0;
+            B.ctrl->nlava[k] = 0;
+        }
+        B.ctrl->nwide = 0;
+    }
+    const CaSim& sim = g_sims[B.sim_base + blockIdx.x];
+    if (!sim.running) return;
+    const CaParams& P = sim.P;
+    const CaDev& D = sim.D;
     CaCtrl* c = D.ctrl;
     if (c->stop) return;
-    unsigned long long la = 0, bo = 0, ch = 0;
+    unsigned long long la = 0, bo = 0, ch = 0, tl = 0;
     for (int r = static_cast<int>(threadIdx.x); r < kPartRows; r += kBlock) {
         CaPart* row = &D.part[r];
         la += row->lava_after;
         bo += row->born;
         ch += row->changed;
+        tl += row->tiles;
         *row = CaPart{0, 0, 0, 0};  // cleared for the next step
     }
     la = wave_sum(la);
     bo = wave_sum(bo);
     ch = wave_sum(ch);
+    tl = wave_sum(tl);
     const int w = static_cast<int>(threadIdx.x >> 6);
     if ((threadIdx.x & 63u) == 0) {
         red[0][w] = la;
         red[1][w] = bo;
         red[2][w] = ch;
+        red[3][w] = tl;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    unsigned long long lava_after = 0, born = 0, changed = 0;
+    unsigned long long lava_after = 0, born = 0, changed = 0, tiles = 0;
     for (int k = 0; k < kBlock / 64; ++k) {
         lava_after += red[0][k];
         born += red[1][k];
         changed += red[2][k];
+        tiles += red[3][k];
     }
     const int32_t t = c->steps;
     const unsigned long long active = P.classic ? c->lava_total + born : lava_after;
@@ -1066,11 +1145,12 @@ __global__ void __launch_bounds__(kBlock) ca_finish_kernel(CaParams P, CaDev D) 
     else if (t + 1 >= P.max_steps)
         stop = 4;  // MaxSteps
     c->stop = stop;
-    c->tiles_visited += c->nactive;
+    c->tiles_visited += tiles;
     c->changes_total += changed;  // every change is one record
-    c->nactive = 0;
-    c->noverflow = 0;
-    c->nlava = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) ca_owner_kernel(uint32_t* owner, uint32_t base, uint32_t n, uint32_t slot) {
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) owner[base + j] = slot;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1362,33 +1442,46 @@ void ca_launch_init_flags(const CaParams& P, const CaDev& D, hipStream_t s) {
     hipLaunchKernelGGL(ca_init_flags_kernel, dim3(g ? g : 1), dim3(kBlock), 0, s, P, D);
 }
 
-void ca_launch_step(const CaParams& P, const CaDev& D, int grid_tiles, hipStream_t s, hipEvent_t* ev) {
+#define VCA_SHAPE_DISPATCH(k, KERNEL, ...)                                                              \
+    do {                                                                                               \
+        switch (k) {                                                                                   \
+            case 0: hipLaunchKernelGGL((KERNEL<4, 4>), __VA_ARGS__, 0); break;                         \
+            case 1: hipLaunchKernelGGL((KERNEL<2, 8>), __VA_ARGS__, 1); break;                         \
+            case 2: hipLaunchKernelGGL((KERNEL<8, 2>), __VA_ARGS__, 2); break;                         \
+            case 3: hipLaunchKernelGGL((KERNEL<1, 16>), __VA_ARGS__, 3); break;                        \
+            default: hipLaunchKernelGGL((KERNEL<16, 1>), __VA_ARGS__, 4); break;                       \
+        }                                                                                              \
+    } while (0)
+
+void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, int grid_tiles, hipStream_t s,
+                    hipEvent_t* ev) {
     const unsigned gt = static_cast<unsigned>(grid_tiles);
     if (ev) hipEventRecord(ev[0], s);
-    hipLaunchKernelGGL(ca_front_kernel, dim3(grid_for(P.ntiles)), dim3(kBlock), 0, s, P, D);
+    hipLaunchKernelGGL(ca_front_kernel, dim3(grid_for(B.ntiles)), dim3(kBlock), 0, s, B);
     if (ev) hipEventRecord(ev[1], s);
-    if (!P.classic) {
-        if (P.ty == 4)
-            hipLaunchKernelGGL((ca_descent_kernel<4, 4>), dim3(gt), dim3(kBlock), 0, s, P, D);
-        else if (P.ty == 1)
-            hipLaunchKernelGGL((ca_descent_kernel<1, 16>), dim3(gt), dim3(kBlock), 0, s, P, D);
-        else
-            hipLaunchKernelGGL((ca_descent_kernel<16, 1>), dim3(gt), dim3(kBlock), 0, s, P, D);
-    }
+    if (any_lava_mode)
+        for (int k = 0; k < kShapes; ++k)
+            if ((shapes >> k) & 1u) VCA_SHAPE_DISPATCH(k, ca_descent_kernel, dim3(gt), dim3(kBlock), 0, s, B);
     if (ev) hipEventRecord(ev[2], s);
-    if (P.ty == 4)
-        hipLaunchKernelGGL((ca_rules_kernel<4, 4>), dim3(gt), dim3(kBlock), 0, s, P, D);
-    else if (P.ty == 1)
-        hipLaunchKernelGGL((ca_rules_kernel<1, 16>), dim3(gt), dim3(kBlock), 0, s, P, D);
-    else
-        hipLaunchKernelGGL((ca_rules_kernel<16, 1>), dim3(gt), dim3(kBlock), 0, s, P, D);
+    for (int k = 0; k < kShapes; ++k)
+        if ((shapes >> k) & 1u) VCA_SHAPE_DISPATCH(k, ca_rules_kernel, dim3(gt), dim3(kBlock), 0, s, B);
     if (ev) hipEventRecord(ev[3], s);
-    hipLaunchKernelGGL(ca_wide_kernel, dim3(kWideBlocks), dim3(kBlock), 0, s, P, D);
+    hipLaunchKernelGGL(ca_wide_kernel, dim3(kWideBlocks), dim3(kBlock), 0, s, B);
     if (ev) hipEventRecord(ev[4], s);
-    hipLaunchKernelGGL(ca_apply_kernel, dim3(gt), dim3(kBlock), 0, s, D);
+    hipLaunchKernelGGL(ca_apply_kernel, dim3(gt), dim3(kBlock), 0, s, B);
     if (ev) hipEventRecord(ev[5], s);
-    hipLaunchKernelGGL(ca_finish_kernel, dim3(1), dim3(kBlock), 0, s, P, D);
+    hipLaunchKernelGGL(ca_finish_kernel, dim3(B.nslots), dim3(kBlock), 0, s, B);
     if (ev) hipEventRecord(ev[6], s);
+}
+
+void ca_write_sim(uint32_t index, const CaSim* sim, hipStream_t s) {
+    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_sims), sim, sizeof(CaSim), static_cast<size_t>(index) * sizeof(CaSim),
+                                 hipMemcpyHostToDevice, s);
+}
+
+void ca_launch_owner(uint32_t* owner, uint32_t base, uint32_t n, uint32_t slot, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(ca_owner_kernel, dim3(grid_for(n, 4096)), dim3(kBlock), 0, s, owner, base, n, slot);
 }
 
 #define VCA_SPIN_DISPATCH(bytes, CALL)                 \

@@ -41,14 +41,15 @@ __host__ __device__ inline uint64_t hash_prefix(uint64_t seed, uint64_t step) {
     return splitmix64(splitmix64(seed ^ 0xA0761D6478BD642FULL) ^ step);
 }
 
+// Plain data (no member initialisers): it lives in the constant-memory simulation table.
 struct NbTable {
-    int32_t variants = 1;
-    uint32_t mask[6] = {0, 0, 0, 0, 0, 0};
-    int32_t max_size = 0;
-    uint32_t any_mask = 0;        // union over variants
-    uint32_t side_mask = 0;       // union over variants of dz == 0 positions (D(n), LavaCA.hpp:102-112)
-    int32_t diagonal_falls = 0;   // some variant has dz == -1 with dx|dy != 0
-    uint32_t conn_mask = 0;       // grain connectivity: Moore, or von Neumann when the kind is von Neumann
+    int32_t variants;
+    uint32_t mask[6];
+    int32_t max_size;
+    uint32_t any_mask;            // union over variants
+    uint32_t side_mask;           // union over variants of dz == 0 positions (D(n), LavaCA.hpp:102-112)
+    int32_t diagonal_falls;       // some variant has dz == -1 with dx|dy != 0
+    uint32_t conn_mask;           // grain connectivity: Moore, or von Neumann when the kind is von Neumann
 };
 
 // Tile = 64 x TY x TZ cells (TY*TZ = 16): one 256-thread workgroup, 4 cells per thread.
@@ -82,23 +83,19 @@ struct Change {
     uint8_t type;
     uint8_t kind;                 // kTemp, kSolidify, kBorn
 };
-constexpr uint8_t kKindTemp = 0, kKindSolidify = 1, kKindBorn = 2;
+constexpr uint8_t kKindTemp = 0, kKindSolidify = 1, kKindBorn = 2, kKindWide = 3;
 
 struct CaCtrl {
     int32_t steps;
     int32_t stop;
-    uint32_t nactive;
-    uint32_t noverflow;           // AIR cells whose vote needs more than four flowId slots
-    uint32_t nlava;               // tiles of the step holding lava (lava mode)
-    uint32_t reserved;
     unsigned long long lava_total, air_total;
     unsigned long long tiles_visited, changes_total;
 };
 
-// Counters of one step spread over kPartRows rows (block b of the rules and wide kernels adds to row
-// b % kPartRows; ca_finish sums and clears them), so that no counter is one address every block updates.
+// Counters of one step spread over kPartRows rows (the rules kernel adds a visited tile's counts to row
+// tile % kPartRows; ca_finish sums and clears them), so that no counter is one address every block updates.
 struct CaPart {
-    unsigned long long lava_after, born, changed, reserved;
+    unsigned long long lava_after, born, changed, tiles;
 };
 constexpr int kPartRows = 256;
 
@@ -119,17 +116,59 @@ struct CaDev {
     int32_t* spill;               // nx*ny, nullptr in classic mode
     uint32_t* vent_cell;          // by flowId, 0xFFFFFFFF = none
     uint8_t* tile_flags;          // bit0: has lava, bit1: has air
-    uint32_t* active;             // active tile list
-    uint32_t* lava_tiles;         // tiles holding lava (lava mode)
-    Change* changes;              // kTileCells records per tile: the step's changes of tile t at t * kTileCells
-    uint32_t* tile_nchg;          // records of each visited tile this step
-    uint32_t* overflow;           // cell indices, capacity nx*ny*nz
+    Change* changes;              // kTileCells record slots per tile: tile t owns slots t*kTileCells ..
+    uint32_t* tile_nchg;          // records of each visited tile this step (from slot 0 up)
+    uint32_t* tile_nwide;         // cells left to ca_wide (kKindWide records from slot kTileCells-1 down)
     CaPart* part;                 // kPartRows rows
     CaCtrl* ctrl;
     unsigned long long* hist_active;
     unsigned long long* hist_changed;
 };
 constexpr uint8_t kFlagLava = 1, kFlagAir = 2;
+
+// Tile shapes (64 x TY x TZ): 0 = 4x4, 1 = 2x8, 2 = 8x2 (3D), 3 = 1x16 (x-z plane, ny = 1), 4 = 16x1 (nz = 1).
+constexpr int kShapes = 5;
+constexpr int kShapeTY[kShapes] = {4, 2, 8, 1, 16};
+constexpr int kShapeTZ[kShapes] = {4, 8, 2, 16, 1};
+
+// One simulation of a batch, as the step kernels see it.
+struct CaSim {
+    CaParams P;
+    CaDev D;
+    int32_t running;              // 1 while the simulation takes part in the batch's steps
+    int32_t shape;
+};
+
+// A batch: simulations in slots, their tiles in one global tile range.  Work-list entries are
+// slot << kTileBits | tile (tile local to the simulation).
+constexpr int kTileBits = 22;
+constexpr uint32_t kTileMask = (1u << kTileBits) - 1u;
+constexpr int kMaxSlots = 1024;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+
+struct BatchCtrl {
+    uint32_t nactive[kShapes];    // visited tiles per shape
+    uint32_t nlava[kShapes];      // tiles holding lava per shape (lava mode)
+    uint32_t nwide;               // tiles with cells for ca_wide
+    uint32_t reserved[5];
+};
+
+// Simulations of every batch live in one constant-memory table on each device (read by the step kernels through
+// scalar loads); a batch owns entries sim_base .. sim_base + its slot capacity.
+constexpr int kSimTable = 4096;
+
+struct BatchDev {
+    uint32_t sim_base;            // first table entry of this batch (slot s -> entry sim_base + s)
+    uint32_t* owner;              // global tile -> slot (kNoSlot for a free tile)
+    uint32_t* tile_base;          // slot -> first global tile
+    uint32_t ntiles;              // global tiles in use (high-water mark)
+    uint32_t nslots;
+    uint32_t cap;                 // capacity of each list
+    uint32_t* active;             // kShapes lists of cap entries
+    uint32_t* lava;               // kShapes lists of cap entries
+    uint32_t* wide;               // cap entries
+    BatchCtrl* ctrl;
+};
 
 struct McParams {
     int32_t nx, ny, nz;
@@ -146,8 +185,15 @@ void ca_launch_terrain(const CaParams& P, const CaDev& D, const int32_t* height,
 void ca_launch_seeds(const CaParams& P, const CaDev& D, const uint32_t* cells, const int32_t* flows,
                      const float* temps, int n, hipStream_t s);
 void ca_launch_init_flags(const CaParams& P, const CaDev& D, hipStream_t s);
-// One step: front list, D(n), rules, apply, finish.  ev != nullptr records 8 events around the kernels.
-void ca_launch_step(const CaParams& P, const CaDev& D, int grid_tiles, hipStream_t s, hipEvent_t* ev);
+// One step of every running simulation of the batch: front lists, D(n), rules, wide votes, apply, finish.
+// shapes = bit mask of the tile shapes present; ev != nullptr records 7 events around the kernel groups.
+void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, int grid_tiles, hipStream_t s,
+                    hipEvent_t* ev);
+// owner[base .. base + n) = slot
+void ca_launch_owner(uint32_t* owner, uint32_t base, uint32_t n, uint32_t slot, hipStream_t s);
+// Writes entry `index` of the device's simulation table (stream ordered; `sim` must stay valid until the
+// stream reaches the copy).
+void ca_write_sim(uint32_t index, const CaSim* sim, hipStream_t s);
 
 // Colour pass of the checkerboard sweep.  spin_bytes: 1, 2 or 4.
 void mc_launch_color(const McParams& P, void* spin, int spin_bytes, const uint32_t* list, const uint64_t* mob_thr,

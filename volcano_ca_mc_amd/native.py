@@ -27,7 +27,8 @@ class CaDesc(C.Structure):
                 ("neighborhood", C.c_int32), ("periodic", C.c_int32), ("seed", C.c_uint64),
                 ("max_steps", C.c_int32), ("eruption_steps", C.c_int32), ("erupt_temp", C.c_float),
                 ("solidus_temp", C.c_float), ("cooling_ambient", C.c_float), ("cooling_lateral", C.c_float),
-                ("cooling_fall", C.c_float), ("ponding", C.c_int32), ("dense", C.c_int32)]
+                ("cooling_fall", C.c_float), ("ponding", C.c_int32), ("dense", C.c_int32),
+                ("tile_shape", C.c_int32)]
 
 
 class Vent(C.Structure):
@@ -98,6 +99,8 @@ def lib() -> C.CDLL:
 EXPORTED = ("vca_version", "vca_last_error", "vca_ctx_create", "vca_ctx_destroy", "vca_ctx_synchronize",
             "vca_device_memory", "vca_ctx_set_timing", "vca_ctx_kernel_time", "vca_ca_create", "vca_ca_destroy",
             "vca_ca_run", "vca_ca_enqueue", "vca_ca_status", "vca_ca_histories", "vca_ca_download", "vca_ca_work",
+            "vca_ca_footprint", "vca_ca_tiles", "vca_batch_create", "vca_batch_destroy", "vca_batch_add",
+            "vca_batch_remove", "vca_batch_set_graph", "vca_batch_enqueue", "vca_batch_run", "vca_batch_info",
             "vca_mc_create", "vca_mc_create_from_ca", "vca_mc_destroy", "vca_mc_sweep", "vca_mc_counts_get",
             "vca_mc_measure", "vca_mc_download", "vca_mc_download_labels", "vca_mc_info",
             "vca_mc_download_mobility")
@@ -163,6 +166,23 @@ class CaParams:
     cooling_fall: float = 0.1
     ponding: bool = True
     dense: bool = False
+    tile_shape: int = 0  # 0 default; 1..5: 64x4x4, 64x2x8, 64x8x2, 64x1x16, 64x16x1
+
+
+def _ca_desc(dims, params: CaParams, neighborhood: str, periodic: bool) -> CaDesc:
+    nx, ny, nz = dims
+    return CaDesc(nx, ny, nz, int(params.classic), NEIGHBORHOODS[neighborhood], int(periodic), params.seed,
+                  params.max_steps, params.eruption_steps, params.erupt_temp, params.solidus_temp,
+                  params.cooling_ambient, params.cooling_lateral, params.cooling_fall, int(params.ponding),
+                  int(params.dense), int(params.tile_shape))
+
+
+def ca_footprint(dims, params: CaParams, neighborhood: str = "moore", periodic: bool = False) -> int:
+    """Device bytes a simulation of these dims takes (vca_ca_footprint)."""
+    d = _ca_desc(dims, params, neighborhood, periodic)
+    b = C.c_uint64()
+    _check(lib().vca_ca_footprint(C.byref(d), C.byref(b)))
+    return int(b.value)
 
 
 class LavaCA:
@@ -170,10 +190,7 @@ class LavaCA:
                  height: Optional[np.ndarray], spill: Optional[np.ndarray], vents: Sequence[Sequence[int]]):
         nx, ny, nz = dims
         self.ctx, self.dims, self.n = ctx, (nx, ny, nz), nx * ny * nz
-        d = CaDesc(nx, ny, nz, int(params.classic), NEIGHBORHOODS[neighborhood], int(periodic), params.seed,
-                   params.max_steps, params.eruption_steps, params.erupt_temp, params.solidus_temp,
-                   params.cooling_ambient, params.cooling_lateral, params.cooling_fall, int(params.ponding),
-                   int(params.dense))
+        d = _ca_desc(dims, params, neighborhood, periodic)
         v = (Vent * max(1, len(vents)))(*[Vent(*map(int, x)) for x in vents])
         h = None if height is None else np.ascontiguousarray(height, dtype=np.int32)
         s = None if spill is None else np.ascontiguousarray(spill, dtype=np.int32)
@@ -231,10 +248,59 @@ class LavaCA:
         _check(lib().vca_ca_download(self.h, _ptr(t), _ptr(f), _ptr(tp), _ptr(s), _ptr(a)))
         return {"type": t, "flow": f, "temp": tp, "solid": s, "arrival": a}
 
+    def tiles(self) -> int:
+        t = C.c_uint64()
+        _check(lib().vca_ca_tiles(self.h, C.byref(t)))
+        return int(t.value)
+
     def work(self):
         tv, ch, tc = C.c_uint64(), C.c_uint64(), C.c_int32()
         _check(lib().vca_ca_work(self.h, C.byref(tv), C.byref(ch), C.byref(tc)))
         return int(tv.value), int(ch.value), int(tc.value)
+
+
+class Batch:
+    """Simulations advanced together, one launch per kernel and step for all members (vca_batch_*)."""
+
+    def __init__(self, ctx: Context, max_sims: int, max_tiles: int, steps_per_graph: int = 0):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        _check(lib().vca_batch_create(ctx.h, max_sims, max_tiles, C.byref(self.h)))
+        if steps_per_graph:
+            self.set_graph(steps_per_graph)
+
+    def close(self):
+        if self.h:
+            lib().vca_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, ca: LavaCA):
+        _check(lib().vca_batch_add(self.h, ca.h))
+
+    def remove(self, ca: LavaCA):
+        _check(lib().vca_batch_remove(self.h, ca.h))
+
+    def set_graph(self, steps_per_graph: int):
+        _check(lib().vca_batch_set_graph(self.h, steps_per_graph))
+
+    def enqueue(self, steps: int):
+        _check(lib().vca_batch_enqueue(self.h, steps))
+
+    def run(self, max_steps: int = 1 << 30) -> int:
+        running = C.c_int32()
+        _check(lib().vca_batch_run(self.h, min(max_steps, 2 ** 31 - 1), C.byref(running)))
+        return int(running.value)
+
+    def info(self):
+        m, t, r = C.c_int32(), C.c_uint64(), C.c_int32()
+        _check(lib().vca_batch_info(self.h, C.byref(m), C.byref(t), C.byref(r)))
+        return int(m.value), int(t.value), int(r.value)
 
 
 class PottsMC:
