@@ -120,7 +120,8 @@ int vca_ca_tiles(vca_ca* ca, uint64_t* tiles);
 typedef struct vca_batch vca_batch;
 int vca_batch_create(vca_ctx* ctx, int32_t max_sims, uint64_t max_tiles, vca_batch** out);
 int vca_batch_destroy(vca_batch* b);           /* detaches the remaining members */
-int vca_batch_add(vca_batch* b, vca_ca* ca);    /* VCA_ERR_NOMEM when no slot or tile range is free */
+/* VCA_ERR_NOMEM when no slot or tile range is free; *slot (may be NULL) = the member's slot */
+int vca_batch_add(vca_batch* b, vca_ca* ca, int32_t* slot);
 int vca_batch_remove(vca_batch* b, vca_ca* ca);
 /* steps_per_graph > 0: step launches are captured once per membership as a hipGraph of that many steps
  * and replayed (steps past a member's stop are no-ops); 0: direct launches. */
@@ -130,6 +131,9 @@ int vca_batch_enqueue(vca_batch* b, int32_t steps);
  * *running = members still running. */
 int vca_batch_run(vca_batch* b, int32_t max_steps, int32_t* running);
 int vca_batch_info(vca_batch* b, int32_t* members, uint64_t* tiles_used, int32_t* running);
+/* Step count and stop reason of every slot in one device read: out[2*s] = steps, out[2*s+1] = stop reason
+ * (-1 for an empty slot); cap = entries of out / 2.  Synchronises the context stream. */
+int vca_batch_status(vca_batch* b, int32_t* out, int32_t cap);
 
 /* ---- Potts MC: replaces PottsMC ------------------------------------------------------------ */
 typedef struct {

@@ -30,7 +30,7 @@ HIP_FLAGS = [
 ]
 LIB_SOURCES = ["vca_device.hip", "vca_api.hip"]
 ENGINE_SOURCES = ["engine/json.cpp", "engine/config.cpp", "engine/terrain.cpp", "engine/writer.cpp",
-                  "engine/simulation.cpp", "engine/batch.cpp", "engine/main.cpp"]
+                  "engine/simulation.cpp", "engine/batch.cpp", "engine/pipeline.cpp", "engine/main.cpp"]
 
 
 def hipcc() -> str:

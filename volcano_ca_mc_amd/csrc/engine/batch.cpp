@@ -2,6 +2,7 @@
 // _*.json skipped) runs in its own child engine process into <out>/<name>/ with engine.log;
 // batch_summary.csv is rewritten after every item; exit code 5 when any item failed
 // (engine/src/app/Batch.cpp:147-418).  The parent never touches the GPU; each child owns it in turn.
+// With --gpu-batch the whole series runs in this process through the batched pipeline (pipeline.cpp).
 #include <spawn.h>
 #include <sys/wait.h>
 #include <fcntl.h>
@@ -25,17 +26,8 @@ namespace fs = std::filesystem;
 
 namespace {
 
-struct Item {
-    std::string name;
-    fs::path config, out;
-};
-struct Outcome {
-    Item item;
-    int exit_code = -1;
-    double wall_ms = 0.0;
-    std::string error;
-    bool ok() const { return exit_code == 0 && error.empty(); }
-};
+using Item = BatchItem;
+using Outcome = BatchOutcome;
 
 const char* const kSections[] = {"readConfig", "readInput", "allocate", "terrain", "vents", "caSim", "snapshots",
                                  "save", "mcInit", "mcSim", "mcStats", "mcSave", "total"};
@@ -145,7 +137,9 @@ std::vector<std::string> summary_row(const Outcome& o) {
     return row;
 }
 
-void write_summary(const fs::path& csv, const std::vector<Outcome>& outcomes) {
+}  // namespace
+
+void write_batch_summary(const fs::path& csv, const std::vector<BatchOutcome>& outcomes) {
     std::ofstream f(csv, std::ios::trunc);
     if (!f) throw std::runtime_error("Nie można zapisać podsumowania serii: " + csv.string());
     const auto cols = summary_columns();
@@ -157,6 +151,8 @@ void write_summary(const fs::path& csv, const std::vector<Outcome>& outcomes) {
         f << '\n';
     }
 }
+
+namespace {
 
 // Child engine process with stdout / stderr appended to log; returns its exit code (-1: did not start).
 int run_child(const fs::path& exe, const std::vector<std::string>& args, const fs::path& log, std::string& err) {
@@ -186,11 +182,11 @@ int run_child(const fs::path& exe, const std::vector<std::string>& args, const f
 
 }  // namespace
 
-int run_batch(const CliOptions& opt) {
+std::vector<BatchItem> batch_items(const CliOptions& opt, fs::path& out) {
     fs::path folder(opt.batch_dir);
     if (!fs::is_directory(folder)) throw CliError("--batch: folder z configami nie istnieje: " + opt.batch_dir);
     if (folder.filename().empty()) folder = folder.parent_path();
-    const fs::path out = opt.out_dir.empty() ? fs::path("results") / folder.filename() : fs::path(opt.out_dir);
+    out = opt.out_dir.empty() ? fs::path("results") / folder.filename() : fs::path(opt.out_dir);
     std::vector<Item> items;
     for (const auto& e : fs::directory_iterator(folder)) {
         if (!e.is_regular_file()) continue;
@@ -204,6 +200,15 @@ int run_batch(const CliOptions& opt) {
     std::sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.name < b.name; });
     if (items.empty())
         throw ConfigError("W folderze " + opt.batch_dir + " nie ma plików konfiguracyjnych *.json (series.json i pliki _*.json są pomijane)");
+    return items;
+}
+
+int run_batch(const CliOptions& opt) {
+    if (opt.gpu_batch) return run_batched(opt);
+    fs::path out;
+    const std::vector<Item> items = batch_items(opt, out);
+    const fs::path folder = fs::path(opt.batch_dir).filename().empty() ? fs::path(opt.batch_dir).parent_path()
+                                                                         : fs::path(opt.batch_dir);
     const fs::path exe = fs::read_symlink("/proc/self/exe");
     fs::create_directories(out);
     std::vector<std::string> extra;
@@ -255,7 +260,7 @@ int run_batch(const CliOptions& opt) {
             tail << "# koniec: kod wyjścia " << o.exit_code << ", czas " << b << " s\n";
         }
         outcomes.push_back(o);
-        write_summary(summary, outcomes);
+        write_batch_summary(summary, outcomes);
         std::cout << '[' << (i + 1) << '/' << n << "] " << it.name << ' ' << (o.ok() ? "ok" : "BŁĄD  " + o.error) << '\n'
                   << std::flush;
         progress.report("batch", static_cast<long long>(i + 1), n, true);

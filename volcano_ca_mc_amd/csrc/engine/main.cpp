@@ -5,7 +5,6 @@
 #include "simulation.hpp"
 
 namespace eng {
-int run_batch(const CliOptions& opt);
 
 std::string usage() {
     return "engine: symulacja erupcji wulkanu na GPU (MI355X): spływ lawy jako automat komórkowy i krystalizacja\n"
@@ -15,6 +14,8 @@ std::string usage() {
            "  engine --config <plik.json> --out <folder> [--set klucz=wartość]... [--progress]\n"
            "  engine --mc-from <folder_wyniku_CA> --config <mc.json> --out <folder> [--set klucz=wartość]...\n"
            "  engine --batch <folder_z_configami> --out <folder_wyników> [--set klucz=wartość]... [--progress]\n"
+           "  engine --batch <folder_z_configami> --gpu-batch --out <folder_wyników> [--set klucz=wartość]...\n"
+           "  engine --serve <zadania.jsonl> --out <folder_wyników> [--set klucz=wartość]...\n"
            "  engine --validate [--config <plik.json>] [--set klucz=wartość]... [--mc-from <folder>]\n"
            "  engine --print-defaults\n"
            "  engine --help\n"
@@ -27,6 +28,12 @@ std::string usage() {
            "  --threads N        skrót dla --set parallel.threads=N (akceptowane dla zgodności)\n"
            "  --mc-from <f>      samo Monte Carlo na wyniku CA z folderu f (ca_result.bin lub .txt)\n"
            "  --batch <f>        seria: wszystkie configi *.json z folderu f, każdy w osobnym procesie\n"
+           "  --gpu-batch        z --batch: cała seria w jednym procesie, kroki CA wszystkich przebiegów\n"
+           "                     wspólnymi wywołaniami GPU (wyniki identyczne jak przebiegi pojedyncze)\n"
+           "  --serve <plik>     strumień zadań: linie {\"at\": s, \"config\": ścieżka[, \"name\": n]}, przyjmowane\n"
+           "                     od chwili at, wspólne kroki GPU; serve_summary.json z opóźnieniami p50/p99\n"
+           "  --graph-steps N    długość grafu HIP kroków wsadowych (domyślnie 16, 0 = bez grafów)\n"
+           "  --max-jobs N       najwyżej N zadań naraz w potoku wsadowym (1 = po jednym; domyślnie bez limitu)\n"
            "  --progress         linie PROGRESS ca/mc/save/batch na stdout (dla GUI)\n"
            "  --validate         tylko walidacja configu; JSON {\"ok\", \"errors\", \"warnings\"} na stdout\n"
            "  --print-defaults   pełny domyślny config JSON\n"
@@ -48,6 +55,25 @@ CliOptions parse_cli(int argc, const char* const* argv) {
         else if (a == "--out") opt.out_dir = value(i, a);
         else if (a == "--mc-from") opt.mc_from = value(i, a);
         else if (a == "--batch") opt.batch_dir = value(i, a);
+        else if (a == "--gpu-batch") opt.gpu_batch = true;
+        else if (a == "--serve") opt.serve_path = value(i, a);
+        else if (a == "--max-jobs") {
+            const std::string v = value(i, a);
+            try {
+                opt.max_jobs = std::stoi(v);
+            } catch (...) {
+                throw CliError("--max-jobs wymaga liczby, podano: " + v);
+            }
+            if (opt.max_jobs < 0) throw CliError("--max-jobs nie może być ujemne");
+        } else if (a == "--graph-steps") {
+            const std::string v = value(i, a);
+            try {
+                opt.steps_per_graph = std::stoi(v);
+            } catch (...) {
+                throw CliError("--graph-steps wymaga liczby, podano: " + v);
+            }
+            if (opt.steps_per_graph < 0 || opt.steps_per_graph > 4096) throw CliError("--graph-steps poza zakresem 0..4096");
+        }
         else if (a == "--set") {
             const std::string kv = value(i, a);
             if (kv.find('=') == std::string::npos || kv.front() == '=')
@@ -74,6 +100,10 @@ CliOptions parse_cli(int argc, const char* const* argv) {
         throw CliError("--print-defaults nie przyjmuje innych opcji");
     if (!opt.batch_dir.empty() && (!opt.config_path.empty() || !opt.mc_from.empty() || opt.benchmark || opt.validate))
         throw CliError("--batch nie łączy się z --config, --mc-from, --benchmark ani --validate");
+    if (opt.gpu_batch && opt.batch_dir.empty()) throw CliError("--gpu-batch działa tylko z --batch");
+    if (!opt.serve_path.empty() && (!opt.batch_dir.empty() || !opt.config_path.empty() || !opt.mc_from.empty() ||
+                                    opt.validate))
+        throw CliError("--serve nie łączy się z --batch, --config, --mc-from ani --validate");
     if (opt.validate && (opt.benchmark || opt.progress || opt.timing_detail || opt.thread_stats))
         throw CliError("--validate nie łączy się z --benchmark, --progress, --timing-detail ani --thread-stats");
     if (opt.benchmark)
@@ -98,6 +128,7 @@ int main(int argc, char** argv) {
         }
         if (opt.validate) return run_validate(opt, std::cout);
         if (!opt.batch_dir.empty()) return run_batch(opt);
+        if (!opt.serve_path.empty()) return run_serve(opt);
         return run_from_cli(opt, since_start);
     } catch (const CliError& e) {
         std::cerr << "Błąd argumentów: " << e.what() << "\n\n" << usage();

@@ -210,56 +210,114 @@ Json ca_stats(const Config& c, const RunSummary& s) {
     return j;
 }
 
+vca_ca_desc make_ca_desc(const Config& cfg) {
+    vca_ca_desc d{};
+    d.nx = cfg.space.nx;
+    d.ny = cfg.space.ny;
+    d.nz = cfg.space.nz;
+    d.classic = cfg.ca.classic;
+    d.neighborhood = static_cast<int32_t>(cfg.neighborhood);
+    d.periodic = cfg.periodic;
+    d.seed = cfg.seed;
+    d.max_steps = cfg.ca.max_steps;
+    d.eruption_steps = cfg.vents.eruption_steps;
+    // CaParams narrows the double config values to float (Simulation.cpp:122-136)
+    d.erupt_temp = static_cast<float>(cfg.vents.erupt_temp);
+    d.solidus_temp = static_cast<float>(cfg.ca.solidus);
+    d.cooling_ambient = static_cast<float>(cfg.ca.cooling_ambient);
+    d.cooling_lateral = static_cast<float>(cfg.ca.cooling_lateral);
+    d.cooling_fall = static_cast<float>(cfg.ca.cooling_fall);
+    d.ponding = cfg.ca.ponding;
+    d.dense = cfg.ca.front ? 0 : 1;
+    return d;
+}
+
+CaInputs ca_inputs(const Config& cfg) {
+    // terrain + spill levels, craters (host, once per run)
+    CaInputs in;
+    Clock t;
+    in.hm = build_terrain(cfg);
+    if (!in.hm.empty()) in.spill = fill_depressions(in.hm, cfg.periodic, cfg.periodic);
+    in.terrain_ms = t.ms();
+    t.reset();
+    in.vents = build_vents(cfg, in.hm);
+    in.vents_ms = t.ms();
+    return in;
+}
+
+vca_ca* create_ca(const Config& cfg, const CaInputs& in, Gpu& gpu) {
+    // GPU state: allocation, rock columns, seeds, initial counts
+    const vca_ca_desc d = make_ca_desc(cfg);
+    std::vector<vca_vent> vv;
+    for (const Vent& v : in.vents) vv.push_back(vca_vent{v.x, v.y, v.z, v.flow_id});
+    vca_ca* h = nullptr;
+    vca_check(vca_ca_create(gpu.ctx(), &d, in.hm.empty() ? nullptr : in.hm.height.data(),
+                            in.spill.empty() ? nullptr : in.spill.height.data(), vv.data(), static_cast<int32_t>(vv.size()),
+                            &h),
+              "CA");
+    return h;
+}
+
+void ca_collect(vca_ca* ca, const Config& cfg, RunSummary& s) {
+    int32_t steps = 0, stop = 0;
+    vca_check(vca_ca_status(ca, &steps, &stop), "CA");
+    s.steps = steps;
+    s.stop = stop;
+    s.active.assign(static_cast<size_t>(steps), 0);
+    s.changed.assign(static_cast<size_t>(steps), 0);
+    int32_t n = 0;
+    vca_check(vca_ca_histories(ca, s.active.data(), s.changed.data(), steps, &n), "CA");
+    vca_check(vca_ca_work(ca, &s.tiles_visited, &s.changes, &s.tile_cells), "CA");
+    s.result.resize(cfg.space);
+    download(ca, s.result);
+}
+
+void save_ca(const Config& cfg, const RunSummary& s, const std::string& out_dir, Timing& timing) {
+    const fs::path out(out_dir);
+    Clock tf;
+    if (cfg.output.format == OutputFormat::Txt || cfg.output.format == OutputFormat::Both) {
+        tf.reset();
+        write_ca_txt((out / "ca_result.txt").string(), s.result);
+        timing.add_file((out / "ca_result.txt").string(), tf.ms());
+    }
+    if (cfg.output.format == OutputFormat::Bin || cfg.output.format == OutputFormat::Both) {
+        tf.reset();
+        write_ca_bin((out / "ca_result.bin").string(), s.result);
+        timing.add_file((out / "ca_result.bin").string(), tf.ms());
+    }
+}
+
+Json parallel_json(const Config& cfg) {
+    Json par = Json::object();
+    par["threads"] = cfg.parallel.threads;
+    par["schedule"] = cfg.parallel.schedule;
+    par["chunk"] = cfg.parallel.chunk;
+    par["affinity"] = cfg.parallel.affinity;
+    return par;
+}
+
+void write_config_used(const Config& cfg, const std::string& out_dir, Timing& timing) {
+    const fs::path out(out_dir);
+    Clock t;
+    write_json((out / "config_used.json").string(), cfg.effective);
+    timing.add_file((out / "config_used.json").string(), t.ms());
+}
+
 RunSummary run_simulation(const Config& cfg, const std::string& out_dir, Timing& timing, Progress& progress, Gpu& gpu) {
     RunSummary s;
     const fs::path out(out_dir);
-    {
-        Clock t;
-        write_json((out / "config_used.json").string(), cfg.effective);
-        timing.add_file((out / "config_used.json").string(), t.ms());
-    }
+    write_config_used(cfg, out_dir, timing);
     if (cfg.mc.full_only()) {
         s.ca_ran = false;
     } else {
-        // terrain + spill levels, craters (host, once per run)
+        const CaInputs in = ca_inputs(cfg);
+        s.vents = in.vents;
         Clock t;
-        Heightmap hm = build_terrain(cfg);
-        Heightmap spill;
-        if (!hm.empty()) spill = fill_depressions(hm, cfg.periodic, cfg.periodic);
-        const double terrain_ms = t.ms();
-        t.reset();
-        s.vents = build_vents(cfg, hm);
-        const double vents_ms = t.ms();
-        // GPU state: allocation, rock columns, seeds, initial counts
-        t.reset();
-        vca_ca_desc d{};
-        d.nx = cfg.space.nx;
-        d.ny = cfg.space.ny;
-        d.nz = cfg.space.nz;
-        d.classic = cfg.ca.classic;
-        d.neighborhood = static_cast<int32_t>(cfg.neighborhood);
-        d.periodic = cfg.periodic;
-        d.seed = cfg.seed;
-        d.max_steps = cfg.ca.max_steps;
-        d.eruption_steps = cfg.vents.eruption_steps;
-        // CaParams narrows the double config values to float (Simulation.cpp:122-136)
-        d.erupt_temp = static_cast<float>(cfg.vents.erupt_temp);
-        d.solidus_temp = static_cast<float>(cfg.ca.solidus);
-        d.cooling_ambient = static_cast<float>(cfg.ca.cooling_ambient);
-        d.cooling_lateral = static_cast<float>(cfg.ca.cooling_lateral);
-        d.cooling_fall = static_cast<float>(cfg.ca.cooling_fall);
-        d.ponding = cfg.ca.ponding;
-        d.dense = cfg.ca.front ? 0 : 1;
-        std::vector<vca_vent> vv;
-        for (const Vent& v : s.vents) vv.push_back(vca_vent{v.x, v.y, v.z, v.flow_id});
         CaHandle ca;
-        vca_check(vca_ca_create(gpu.ctx(), &d, hm.empty() ? nullptr : hm.height.data(),
-                                spill.empty() ? nullptr : spill.height.data(), vv.data(), static_cast<int32_t>(vv.size()),
-                                &ca.h),
-                  "CA");
+        ca.h = create_ca(cfg, in, gpu);
         timing.add("allocate", t.ms());
-        timing.add("terrain", terrain_ms);
-        timing.add("vents", vents_ms);
+        timing.add("terrain", in.terrain_ms);
+        timing.add("vents", in.vents_ms);
 
         // steps; snapshots (output.snapshotsEvery) and progress between chunks, outside caSim
         const int every = cfg.output.snapshots_every;
@@ -310,37 +368,14 @@ RunSummary run_simulation(const Config& cfg, const std::string& out_dir, Timing&
         if (progress.enabled()) progress.report("save", 0, 1, true);
 
         t.reset();
-        s.steps = steps;
-        s.stop = stop;
-        s.active.assign(static_cast<size_t>(steps), 0);
-        s.changed.assign(static_cast<size_t>(steps), 0);
-        int32_t n = 0;
-        vca_check(vca_ca_histories(ca.h, s.active.data(), s.changed.data(), steps, &n), "CA");
-        vca_check(vca_ca_work(ca.h, &s.tiles_visited, &s.changes, &s.tile_cells), "CA");
-        s.result.resize(cfg.space);
-        download(ca.h, s.result);
-        Clock tf;
-        if (cfg.output.format == OutputFormat::Txt || cfg.output.format == OutputFormat::Both) {
-            tf.reset();
-            write_ca_txt((out / "ca_result.txt").string(), s.result);
-            timing.add_file((out / "ca_result.txt").string(), tf.ms());
-        }
-        if (cfg.output.format == OutputFormat::Bin || cfg.output.format == OutputFormat::Both) {
-            tf.reset();
-            write_ca_bin((out / "ca_result.bin").string(), s.result);
-            timing.add_file((out / "ca_result.bin").string(), tf.ms());
-        }
+        ca_collect(ca.h, cfg, s);
+        save_ca(cfg, s, out_dir, timing);
         timing.add("save", t.ms());
     }
     Clock ts;
     Json stats = s.ca_ran ? ca_stats(cfg, s) : Json::object();
     if (s.ca_ran) timing.add("caStats", ts.ms());
-    Json par = Json::object();
-    par["threads"] = cfg.parallel.threads;
-    par["schedule"] = cfg.parallel.schedule;
-    par["chunk"] = cfg.parallel.chunk;
-    par["affinity"] = cfg.parallel.affinity;
-    stats["parallel"] = par;
+    stats["parallel"] = parallel_json(cfg);
     if (cfg.mc.enabled) {
         s.mc = run_mc(cfg, s.ca_ran ? &s.result : nullptr, Cooling::Full, timing, progress, out_dir, gpu);
         s.mc_ran = true;
@@ -355,22 +390,7 @@ RunSummary run_simulation(const Config& cfg, const std::string& out_dir, Timing&
     return s;
 }
 
-// ------------------------------------------------------------------------------------------------
-// MC stage
-// ------------------------------------------------------------------------------------------------
 namespace {
-
-struct McHandle {
-    vca_mc* h = nullptr;
-    ~McHandle() {
-        if (h) vca_mc_destroy(h);
-    }
-};
-
-struct ClassStat {
-    uint64_t grains = 0, cells = 0, largest = 0;
-    double mean() const { return grains > 0 ? double(cells) / double(grains) : 0.0; }
-};
 
 Json class_json(uint64_t cells, const ClassStat& a, const ClassStat& b) {
     Json j = Json::object();
@@ -385,16 +405,13 @@ Json class_json(uint64_t cells, const ClassStat& a, const ClassStat& b) {
 
 }  // namespace
 
-McSummary run_mc(const Config& cfg, const CaState* ca, Cooling cooling, Timing& timing, Progress& progress,
-                 const std::string& out_dir, Gpu& gpu) {
-    McSummary s;
-    s.full = ca == nullptr;
-    s.mcs = cfg.mc.mcs;
-    s.Q = cfg.mc.Q;
-    Clock t;
-    const Dims dims = s.full ? cfg.space : ca->dims;
-    s.dims = dims;
-    vca_mc_desc d{};
+McPrep mc_prepare(const Config& cfg, const CaState* ca, Cooling cooling) {
+    McPrep p;
+    p.full = ca == nullptr;
+    p.dims = p.full ? cfg.space : ca->dims;
+    const Dims& dims = p.dims;
+    vca_mc_desc& d = p.desc;
+    d = vca_mc_desc{};
     d.nx = dims.nx;
     d.ny = dims.ny;
     d.nz = dims.nz;
@@ -402,188 +419,221 @@ McSummary run_mc(const Config& cfg, const CaState* ca, Cooling cooling, Timing& 
     // full domain wraps every axis, basalt only x and y (McStage.cpp:102-106; PottsMC.cpp:18)
     d.periodic[0] = cfg.periodic && dims.nx > 1;
     d.periodic[1] = cfg.periodic && dims.ny > 1;
-    d.periodic[2] = cfg.periodic && s.full && dims.nz > 1;
+    d.periodic[2] = cfg.periodic && p.full && dims.nz > 1;
     d.seed = cfg.seed;
     d.Q = cfg.mc.Q;
     d.J = cfg.mc.J;
     d.kT = cfg.mc.kT;
     d.checkerboard = cfg.mc.checkerboard;
-    std::vector<uint8_t> domain;
-    std::vector<float> mobility;
-    bool dyn = false;
-    if (!s.full) {
+    if (!p.full) {
         const size_t n = ca->type.size();
-        domain.resize(n);
-        for (size_t i = 0; i < n; ++i) domain[i] = (ca->type[i] == VCA_BASALT || ca->type[i] == VCA_LAVA) ? 1 : 0;
+        p.domain.resize(n);
+        for (size_t i = 0; i < n; ++i) p.domain[i] = (ca->type[i] == VCA_BASALT || ca->type[i] == VCA_LAVA) ? 1 : 0;
         if (cooling != Cooling::None) {
             // CoolingMobility::fromCooling (CoolingMobility.cpp:8-129)
             uint32_t end = 0;
             for (size_t i = 0; i < n; ++i)
-                if (domain[i]) end = std::max<uint32_t>(end, std::max(ca->solid[i], ca->arrival[i]));
+                if (p.domain[i]) end = std::max<uint32_t>(end, std::max(ca->solid[i], ca->arrival[i]));
             std::vector<uint32_t> time(n, 0);
             uint32_t tmin = 0xFFFFFFFFu, tmax = 0;
             for (size_t i = 0; i < n; ++i) {
-                if (!domain[i]) continue;
+                if (!p.domain[i]) continue;
                 const uint32_t sd = ca->type[i] == VCA_BASALT ? ca->solid[i] : end;
                 time[i] = sd > ca->arrival[i] ? sd - ca->arrival[i] : 0;
                 tmin = std::min(tmin, time[i]);
                 tmax = std::max(tmax, time[i]);
             }
-            mobility.assign(n, 0.0f);
+            p.mobility.assign(n, 0.0f);
             for (size_t i = 0; i < n; ++i) {
-                if (!domain[i]) continue;
+                if (!p.domain[i]) continue;
                 if (tmax == tmin) {
-                    mobility[i] = 1.0f;
+                    p.mobility[i] = 1.0f;
                     continue;
                 }
                 const double norm = double(time[i] - tmin) / double(tmax - tmin);
-                mobility[i] = static_cast<float>(cfg.mc.mobility_min + (1.0 - cfg.mc.mobility_min) * norm);
+                p.mobility[i] = static_cast<float>(cfg.mc.mobility_min + (1.0 - cfg.mc.mobility_min) * norm);
             }
             if (cfg.mc.mobility_from_cooling) {
-                dyn = true;
-                s.mobility_used = true;
+                p.dynamics = true;
+                p.mobility_used = true;
             }
         }
         if (cooling == Cooling::None && cfg.mc.mobility_from_cooling)
-            s.warnings.push_back("wynik CA bez solidStep/arrivalStep (plik txt): mobilność ze stygnięcia wyłączona, "
+            p.warnings.push_back("wynik CA bez solidStep/arrivalStep (plik txt): mobilność ze stygnięcia wyłączona, "
                                  "MC jak dla mobilityFromCooling = false");
         if (cooling == Cooling::SolidOnly)
-            s.warnings.push_back("ca_result.bin w wersji 1 nie zawiera arrivalStep, czas w stanie płynnym liczony od "
+            p.warnings.push_back("ca_result.bin w wersji 1 nie zawiera arrivalStep, czas w stanie płynnym liczony od "
                                  "kroku 0 (= solidStep)");
     }
-    McHandle mc;
-    vca_check(vca_mc_create(gpu.ctx(), &d, s.full ? nullptr : domain.data(), mobility.empty() ? nullptr : mobility.data(),
-                            dyn ? 1 : 0, &mc.h),
-              "MC");
-    int32_t colors = 0;
-    uint64_t dom_cells = 0;
-    vca_check(vca_mc_info(mc.h, &colors, &dom_cells), "MC");
-    s.domain_cells = dom_cells;
-    if (dom_cells == 0)
-        s.warnings.push_back(s.full ? "domena MC jest pusta" : "domena MC jest pusta: wynik CA nie zawiera bazaltu");
-    timing.add("mcInit", t.ms());
+    return p;
+}
 
-    const bool has_classes = !mobility.empty();
-    auto measure = [&](McRow& row, ClassStat* cls, uint64_t* largest, uint64_t* high) {
-        vca_mc_stats st{};
-        vca_mc_class c[2]{};
-        uint64_t hi = 0;
-        vca_check(vca_mc_measure(mc.h, &st, has_classes ? c : nullptr, &hi), "MC");
-        row.mcs = st.mcs;
-        row.grains = st.grains;
-        row.mean_grain_size = st.mean_grain_size;
-        row.energy = st.energy;
-        if (cls) {
-            cls[0] = ClassStat{c[0].grains, c[0].grain_cells, c[0].largest};
-            cls[1] = ClassStat{c[1].grains, c[1].grain_cells, c[1].largest};
-        }
-        if (largest) *largest = st.largest;
-        if (high) *high = hi;
-    };
+McRun::McRun(const Config& cfg, McPrep prep, Gpu& gpu) : cfg_(cfg) {
+    s_.full = prep.full;
+    s_.mcs = cfg.mc.mcs;
+    s_.Q = cfg.mc.Q;
+    s_.dims = prep.dims;
+    s_.mobility_used = prep.mobility_used;
+    s_.warnings = std::move(prep.warnings);
+    has_classes_ = !prep.mobility.empty();
+    Clock t;
+    vca_check(vca_mc_create(gpu.ctx(), &prep.desc, s_.full ? nullptr : prep.domain.data(),
+                            prep.mobility.empty() ? nullptr : prep.mobility.data(), prep.dynamics ? 1 : 0, &h_),
+              "MC");
+    uint64_t dom_cells = 0;
+    vca_check(vca_mc_info(h_, &colors_, &dom_cells), "MC");
+    s_.domain_cells = dom_cells;
+    if (dom_cells == 0)
+        s_.warnings.push_back(s_.full ? "domena MC jest pusta" : "domena MC jest pusta: wynik CA nie zawiera bazaltu");
+    init_ms_ = t.ms();
+    t.reset();
+    McRow row0;
+    measure(row0, start_cls_, &largest_end_, &high_cells_);
+    s_.history.push_back(row0);
+    for (int c = 0; c < 2; ++c) end_cls_[c] = start_cls_[c];
+    stats_ms_ += t.ms();
+}
+
+McRun::~McRun() {
+    if (h_) vca_mc_destroy(h_);
+}
+
+void McRun::measure(McRow& row, ClassStat* cls, uint64_t* largest, uint64_t* high) {
+    vca_mc_stats st{};
+    vca_mc_class c[2]{};
+    uint64_t hi = 0;
+    vca_check(vca_mc_measure(h_, &st, has_classes_ ? c : nullptr, &hi), "MC");
+    row.mcs = st.mcs;
+    row.grains = st.grains;
+    row.mean_grain_size = st.mean_grain_size;
+    row.energy = st.energy;
+    if (cls) {
+        cls[0] = ClassStat{c[0].grains, c[0].grain_cells, c[0].largest};
+        cls[1] = ClassStat{c[1].grains, c[1].grain_cells, c[1].largest};
+    }
+    if (largest) *largest = st.largest;
+    if (high) *high = hi;
+}
+
+int McRun::next_stats_point() const {
+    return std::min(cfg_.mc.mcs, (k_ / cfg_.mc.stats_every + 1) * cfg_.mc.stats_every);
+}
+
+void McRun::advance(int next) {
+    next = std::min(next, cfg_.mc.mcs);
+    if (next <= k_) return;
+    vca_mc_counts counts{};
+    Clock tc;
+    vca_check(vca_mc_sweep(h_, next - k_, &counts), "MC");
+    sim_ms_ += tc.ms();
+    k_ = next;
+    if (k_ % cfg_.mc.stats_every == 0 || k_ == cfg_.mc.mcs) {
+        Clock tm;
+        McRow row;
+        const bool last = k_ == cfg_.mc.mcs;
+        measure(row, last ? end_cls_ : nullptr, last ? &largest_end_ : nullptr, nullptr);
+        const uint64_t att = counts.attempts - att_before_, acc = counts.accepted - acc_before_;
+        row.accept_rate = att > 0 ? double(acc) / double(att) : 0.0;
+        att_before_ = counts.attempts;
+        acc_before_ = counts.accepted;
+        s_.history.push_back(row);
+        stats_ms_ += tm.ms();
+    }
+}
+
+void McRun::download(std::vector<int32_t>& out) {
+    out.resize(s_.dims.cells());
+    vca_check(vca_mc_download(h_, out.data()), "MC");
+}
+
+McSummary McRun::finish(Timing& timing) {
+    // stats.json "mc" section (McStage::statsJson, McStage.cpp:326-383)
+    Clock t;
+    vca_mc_counts total{};
+    vca_check(vca_mc_counts_get(h_, &total), "MC");
+    const McRow& first = s_.history.front();
+    const McRow& last = s_.history.back();
+    Json j = Json::object();
+    j["domain"] = s_.full ? "full" : "basalt";
+    j["domainCells"] = Json(static_cast<unsigned long long>(s_.domain_cells));
+    j["space"] = Json::array({Json(s_.dims.nx), Json(s_.dims.ny), Json(s_.dims.nz)});
+    j["Q"] = cfg_.mc.Q;
+    j["mcs"] = cfg_.mc.mcs;
+    j["J"] = Json(cfg_.mc.J);
+    j["kT"] = Json(cfg_.mc.kT);
+    j["update"] = cfg_.mc.checkerboard ? "checkerboard" : "random";
+    if (cfg_.mc.checkerboard) j["colors"] = colors_;
+    j["neighborhood"] = neighborhood_name(cfg_.neighborhood);
+    j["boundary"] = cfg_.periodic ? "periodic" : "absorbing";
+    j["grainConnectivity"] = cfg_.neighborhood != Neighborhood::VonNeumann ? "moore" : "von_neumann";
+    j["mobility"] = s_.mobility_used ? "cooling" : "off";
+    if (s_.mobility_used) j["mobilityMin"] = Json(cfg_.mc.mobility_min);
+    j["grainsStart"] = Json(static_cast<unsigned long long>(first.grains));
+    j["grainsEnd"] = Json(static_cast<unsigned long long>(last.grains));
+    j["meanGrainSizeStart"] = Json(rounded(first.mean_grain_size, 3));
+    j["meanGrainSizeEnd"] = Json(rounded(last.mean_grain_size, 3));
+    j["largestGrainEnd"] = Json(static_cast<unsigned long long>(largest_end_));
+    j["energyStart"] = Json(first.energy);
+    j["energyEnd"] = Json(last.energy);
+    j["attempts"] = Json(static_cast<unsigned long long>(total.attempts));
+    j["accepted"] = Json(static_cast<unsigned long long>(total.accepted));
+    j["acceptedUphill"] = Json(static_cast<unsigned long long>(total.uphill));
+    if (has_classes_) {
+        Json bm = Json::object();
+        bm["threshold"] = Json(0.5);
+        bm["mobilityMin"] = Json(cfg_.mc.mobility_min);
+        bm["rule"] = "ziarno należy do klasy, w której jest większość jego komórek";
+        bm["high"] = class_json(high_cells_, start_cls_[0], end_cls_[0]);
+        bm["low"] = class_json(s_.domain_cells - high_cells_, start_cls_[1], end_cls_[1]);
+        j["byMobility"] = bm;
+    }
+    s_.stats = j;
+    download(s_.crystals);
+    timing.add("mcInit", init_ms_);
+    timing.add("mcSim", sim_ms_);
+    timing.add("mcStats", stats_ms_ + t.ms());
+    return std::move(s_);
+}
+
+McSummary run_mc(const Config& cfg, const CaState* ca, Cooling cooling, Timing& timing, Progress& progress,
+                 const std::string& out_dir, Gpu& gpu) {
+    Clock tp;
+    McPrep prep = mc_prepare(cfg, ca, cooling);
+    const double prep_ms = tp.ms();
+    McRun run(cfg, std::move(prep), gpu);
+    run.add_init_ms(prep_ms);
     const int snap_every = out_dir.empty() ? 0 : cfg.mc.snapshots_every;
     const fs::path snap_dir = fs::path(out_dir) / "mc_snapshots";
     double snap_ms = 0.0;
     std::vector<int32_t> snap;
     auto write_snapshot = [&](int k) {
         Clock ts;
-        snap.resize(dims.cells());
-        vca_check(vca_mc_download(mc.h, snap.data()), "MC");
+        run.download(snap);
         char name[32];
         std::snprintf(name, sizeof(name), "mcs_%06d.bin", k);
-        write_crystal_bin((snap_dir / name).string(), dims, snap);
+        write_crystal_bin((snap_dir / name).string(), run.dims(), snap);
         snap_ms += ts.ms();
     };
     if (snap_every > 0) {
         fs::create_directories(snap_dir);
         write_snapshot(0);
     }
-    double sim_ms = 0.0, stats_ms = 0.0;
-    ClassStat start_cls[2], end_cls[2];
-    uint64_t largest_end = 0, high_cells = 0;
-    t.reset();
-    McRow row0;
-    measure(row0, start_cls, &largest_end, &high_cells);
-    s.history.push_back(row0);
-    for (int c = 0; c < 2; ++c) end_cls[c] = start_cls[c];
-    stats_ms += t.ms();
     progress.report("mc", 0, cfg.mc.mcs, true);
-    uint64_t att_before = 0, acc_before = 0;
-    int k = 0;
-    while (k < cfg.mc.mcs) {
+    while (!run.done()) {
         // sweep up to the next measurement / snapshot point
-        int next = std::min(cfg.mc.mcs, (k / cfg.mc.stats_every + 1) * cfg.mc.stats_every);
+        int next = run.next_stats_point();
+        const int k = run.sweeps_done();
         if (snap_every > 0) next = std::min(next, (k / snap_every + 1) * snap_every);
         if (progress.enabled()) next = std::min(next, k + 16);
-        vca_mc_counts counts{};
-        Clock tc;
-        vca_check(vca_mc_sweep(mc.h, next - k, &counts), "MC");
-        sim_ms += tc.ms();
-        k = next;
-        if (k % cfg.mc.stats_every == 0 || k == cfg.mc.mcs) {
-            Clock tm;
-            McRow row;
-            const bool last = k == cfg.mc.mcs;
-            measure(row, last ? end_cls : nullptr, last ? &largest_end : nullptr, nullptr);
-            const uint64_t att = counts.attempts - att_before, acc = counts.accepted - acc_before;
-            row.accept_rate = att > 0 ? double(acc) / double(att) : 0.0;
-            att_before = counts.attempts;
-            acc_before = counts.accepted;
-            s.history.push_back(row);
-            stats_ms += tm.ms();
-        }
-        if (snap_every > 0 && (k % snap_every == 0 || k == cfg.mc.mcs)) write_snapshot(k);
-        progress.report("mc", k, cfg.mc.mcs, k == cfg.mc.mcs);
+        run.advance(next);
+        if (snap_every > 0 && (run.sweeps_done() % snap_every == 0 || run.done())) write_snapshot(run.sweeps_done());
+        progress.report("mc", run.sweeps_done(), cfg.mc.mcs, run.done());
     }
-    timing.add("mcSim", sim_ms);
+    McSummary s = run.finish(timing);
     if (snap_every > 0) {
         timing.add("mcSnapshots", snap_ms);
         timing.add_file(snap_dir.string(), snap_ms);
     }
-
-    // stats.json "mc" section (McStage::statsJson, McStage.cpp:326-383)
-    t.reset();
-    vca_mc_counts total{};
-    vca_check(vca_mc_counts_get(mc.h, &total), "MC");
-    const McRow& first = s.history.front();
-    const McRow& last = s.history.back();
-    Json j = Json::object();
-    j["domain"] = s.full ? "full" : "basalt";
-    j["domainCells"] = Json(static_cast<unsigned long long>(s.domain_cells));
-    j["space"] = Json::array({Json(dims.nx), Json(dims.ny), Json(dims.nz)});
-    j["Q"] = cfg.mc.Q;
-    j["mcs"] = cfg.mc.mcs;
-    j["J"] = Json(cfg.mc.J);
-    j["kT"] = Json(cfg.mc.kT);
-    j["update"] = cfg.mc.checkerboard ? "checkerboard" : "random";
-    if (cfg.mc.checkerboard) j["colors"] = colors;
-    j["neighborhood"] = neighborhood_name(cfg.neighborhood);
-    j["boundary"] = cfg.periodic ? "periodic" : "absorbing";
-    j["grainConnectivity"] = cfg.neighborhood != Neighborhood::VonNeumann ? "moore" : "von_neumann";
-    j["mobility"] = s.mobility_used ? "cooling" : "off";
-    if (s.mobility_used) j["mobilityMin"] = Json(cfg.mc.mobility_min);
-    j["grainsStart"] = Json(static_cast<unsigned long long>(first.grains));
-    j["grainsEnd"] = Json(static_cast<unsigned long long>(last.grains));
-    j["meanGrainSizeStart"] = Json(rounded(first.mean_grain_size, 3));
-    j["meanGrainSizeEnd"] = Json(rounded(last.mean_grain_size, 3));
-    j["largestGrainEnd"] = Json(static_cast<unsigned long long>(largest_end));
-    j["energyStart"] = Json(first.energy);
-    j["energyEnd"] = Json(last.energy);
-    j["attempts"] = Json(static_cast<unsigned long long>(total.attempts));
-    j["accepted"] = Json(static_cast<unsigned long long>(total.accepted));
-    j["acceptedUphill"] = Json(static_cast<unsigned long long>(total.uphill));
-    if (has_classes) {
-        Json bm = Json::object();
-        bm["threshold"] = Json(0.5);
-        bm["mobilityMin"] = Json(cfg.mc.mobility_min);
-        bm["rule"] = "ziarno należy do klasy, w której jest większość jego komórek";
-        bm["high"] = class_json(high_cells, start_cls[0], end_cls[0]);
-        bm["low"] = class_json(s.domain_cells - high_cells, start_cls[1], end_cls[1]);
-        j["byMobility"] = bm;
-    }
-    s.stats = j;
-    s.crystals.resize(dims.cells());
-    vca_check(vca_mc_download(mc.h, s.crystals.data()), "MC");
-    timing.add("mcStats", stats_ms + t.ms());
     return s;
 }
 
@@ -631,6 +681,24 @@ void env_info(Timing& timing, const Gpu& gpu) {
     timing.set_info("compiler", std::string("GCC ") + __VERSION__ + " + hipcc (gfx950)");
     timing.set_info("build", "Release");
 }
+}  // namespace
+
+void run_info(Timing& timing, const Config& cfg, const RunSummary& s, const Gpu& gpu) {
+    common_info(timing, cfg, cfg.space);
+    if (s.ca_ran) {
+        timing.set_info("steps", s.steps);
+        timing.set_info("grid", cfg.grid_matrix ? "matrix" : "vector");
+        timing.set_info("mode", cfg.ca.classic ? "classic" : "lava");
+        timing.set_info("stopReason", stop_text(s.stop));
+    } else {
+        timing.set_info("mode", "mc_full");
+    }
+    if (s.mc_ran) mc_info(timing, s.mc);
+    env_info(timing, gpu);
+}
+
+namespace {
+
 std::string times_line(const Timing& timing) {
     std::string line = "Czasy [ms]     ";
     const char* sep = "";
@@ -736,12 +804,7 @@ int run_from_cli(const CliOptions& opt, const Clock& since_start) {
         source["version"] = src.version;
         source["coolingData"] = cooling == Cooling::Full ? "solidStep+arrivalStep" : cooling == Cooling::SolidOnly ? "solidStep" : "brak";
         stats["source"] = source;
-        Json par = Json::object();
-        par["threads"] = cfg.parallel.threads;
-        par["schedule"] = cfg.parallel.schedule;
-        par["chunk"] = cfg.parallel.chunk;
-        par["affinity"] = cfg.parallel.affinity;
-        stats["parallel"] = par;
+        stats["parallel"] = parallel_json(cfg);
         stats["mc"] = mc.stats;
         Clock tw;
         write_json((out / "stats.json").string(), stats);
@@ -767,17 +830,7 @@ int run_from_cli(const CliOptions& opt, const Clock& since_start) {
     const RunSummary s = run_simulation(cfg, out_dir, timing, progress, gpu);
     timing.add("total", since_start.ms());
     const Dims& d = cfg.space;
-    common_info(timing, cfg, d);
-    if (s.ca_ran) {
-        timing.set_info("steps", s.steps);
-        timing.set_info("grid", cfg.grid_matrix ? "matrix" : "vector");
-        timing.set_info("mode", cfg.ca.classic ? "classic" : "lava");
-        timing.set_info("stopReason", stop_text(s.stop));
-    } else {
-        timing.set_info("mode", "mc_full");
-    }
-    if (s.mc_ran) mc_info(timing, s.mc);
-    env_info(timing, gpu);
+    run_info(timing, cfg, s, gpu);
     detail(&s);
     timing.record_files();
     write_json((fs::path(out_dir) / "times.json").string(), timing.to_json());

@@ -5,6 +5,7 @@
 #pragma once
 
 #include <chrono>
+#include <filesystem>
 #include <cstdint>
 #include <iostream>
 #include <string>
@@ -22,6 +23,10 @@ namespace eng {
 struct CliOptions {
     bool help = false, timing_detail = false, print_defaults = false, validate = false, progress = false;
     bool thread_stats = false, benchmark = false, quick = false;
+    bool gpu_batch = false;           // --gpu-batch: --batch in one process through the batched pipeline
+    int steps_per_graph = 16;         // --graph-steps: hipGraph length of the batched CA steps (0 = direct)
+    int max_jobs = 0;                 // --max-jobs: jobs admitted at once in the batched pipeline (0 = no limit)
+    std::string serve_path;           // --serve <jobs.jsonl>
     int repeats = 3;
     std::string config_path, out_dir, mc_from, batch_dir;
     std::vector<std::string> overrides;
@@ -114,6 +119,71 @@ struct RunSummary {
     McSummary mc;
 };
 
+// MC inputs derived on the host from the CA result: descriptor, basalt domain, cooling mobility
+// (CoolingMobility.cpp:8-129), warnings.
+struct McPrep {
+    bool full = false;
+    Dims dims;
+    vca_mc_desc desc{};
+    std::vector<uint8_t> domain;
+    std::vector<float> mobility;
+    bool dynamics = false, mobility_used = false;
+    std::vector<std::string> warnings;
+};
+McPrep mc_prepare(const Config& cfg, const CaState* ca, Cooling cooling);
+
+struct ClassStat {
+    uint64_t grains = 0, cells = 0, largest = 0;
+    double mean() const { return grains > 0 ? double(cells) / double(grains) : 0.0; }
+};
+
+// One MC stage on the GPU (McStage::run, McStage.cpp:93-301): created with the measurement at MCS 0, advanced
+// between measurement points (every statsEvery and the last sweep), then finished into the stats.json section
+// and the crystal orientations.  The solo driver and the batched pipeline both use it.
+class McRun {
+public:
+    McRun(const Config& cfg, McPrep prep, Gpu& gpu);
+    ~McRun();
+    McRun(const McRun&) = delete;
+    McRun& operator=(const McRun&) = delete;
+    int next_stats_point() const;
+    void advance(int next);  // sweeps up to `next` (<= mcs), measuring when it is a measurement point
+    bool done() const { return k_ >= cfg_.mc.mcs; }
+    int sweeps_done() const { return k_; }
+    const Dims& dims() const { return s_.dims; }
+    uint64_t domain_cells() const { return s_.domain_cells; }
+    void download(std::vector<int32_t>& out);
+    void add_init_ms(double ms) { init_ms_ += ms; }
+    McSummary finish(Timing& timing);
+
+private:
+    void measure(McRow& row, ClassStat* cls, uint64_t* largest, uint64_t* high);
+    const Config& cfg_;
+    McSummary s_;
+    vca_mc* h_ = nullptr;
+    int32_t colors_ = 0;
+    bool has_classes_ = false;
+    int k_ = 0;
+    double init_ms_ = 0.0, sim_ms_ = 0.0, stats_ms_ = 0.0;
+    ClassStat start_cls_[2], end_cls_[2];
+    uint64_t largest_end_ = 0, high_cells_ = 0, att_before_ = 0, acc_before_ = 0;
+};
+
+// Pieces of one run, shared by the solo driver and the batched pipeline.
+struct CaInputs {
+    Heightmap hm, spill;
+    std::vector<Vent> vents;
+    double terrain_ms = 0.0, vents_ms = 0.0;
+};
+vca_ca_desc make_ca_desc(const Config& cfg);
+CaInputs ca_inputs(const Config& cfg);
+vca_ca* create_ca(const Config& cfg, const CaInputs& in, Gpu& gpu);
+void ca_collect(vca_ca* ca, const Config& cfg, RunSummary& s);  // steps, histories, work, final state
+void save_ca(const Config& cfg, const RunSummary& s, const std::string& out_dir, Timing& timing);
+Json parallel_json(const Config& cfg);
+void write_config_used(const Config& cfg, const std::string& out_dir, Timing& timing);
+void run_info(Timing& timing, const Config& cfg, const RunSummary& s, const Gpu& gpu);
+
 const char* stop_text(int stop);
 Json ca_stats(const Config& c, const RunSummary& s);
 McSummary run_mc(const Config& cfg, const CaState* ca, Cooling cooling, Timing& timing, Progress& progress,
@@ -121,6 +191,24 @@ McSummary run_mc(const Config& cfg, const CaState* ca, Cooling cooling, Timing& 
 void save_mc(const Config& cfg, const McSummary& s, const std::string& out_dir, Timing& timing);
 RunSummary run_simulation(const Config& cfg, const std::string& out_dir, Timing& timing, Progress& progress, Gpu& gpu);
 int run_from_cli(const CliOptions& opt, const Clock& since_start);
+
+// --batch helpers (batch.cpp) and the batched pipeline (pipeline.cpp)
+struct BatchItem {
+    std::string name;
+    std::filesystem::path config, out;
+};
+struct BatchOutcome {
+    BatchItem item;
+    int exit_code = -1;
+    double wall_ms = 0.0;
+    std::string error;
+    bool ok() const { return exit_code == 0 && error.empty(); }
+};
+std::vector<BatchItem> batch_items(const CliOptions& opt, std::filesystem::path& out);
+void write_batch_summary(const std::filesystem::path& csv, const std::vector<BatchOutcome>& outcomes);
+int run_batch(const CliOptions& opt);
+int run_batched(const CliOptions& opt);
+int run_serve(const CliOptions& opt);
 int run_validate(const CliOptions& opt, std::ostream& out);
 std::string usage();
 CliOptions parse_cli(int argc, const char* const* argv);

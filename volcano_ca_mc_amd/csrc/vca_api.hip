@@ -278,7 +278,7 @@ struct Batch {
     vca_ctx* ctx = nullptr;
     int max_slots = 0;
     uint32_t cap = 0;                             // tiles
-    DevBuf owner, tile_base, active, lava, wide, ctrl;
+    DevBuf owner, tile_base, active, lava, wide, ctrl, status;
     uint32_t sim_base = 0;
     bool has_table = false;
     std::vector<CaSim> host_sims;                 // what the table entries hold (the copies read from here)
@@ -915,7 +915,7 @@ extern "C" int vca_batch_destroy(vca_batch* b) {
     });
 }
 
-extern "C" int vca_batch_add(vca_batch* b, vca_ca* ca) {
+extern "C" int vca_batch_add(vca_batch* b, vca_ca* ca, int32_t* slot) {
     return guarded([&] {
         require(b && ca, "NULL argument");
         require(ca->ctx == b->b.ctx, "the simulation and the batch use different contexts");
@@ -923,6 +923,28 @@ extern "C" int vca_batch_add(vca_batch* b, vca_ca* ca) {
         b->b.ctx->set_device();
         ca->slot = b->b.attach(ca);
         ca->member = &b->b;
+        if (slot) *slot = ca->slot;
+    });
+}
+
+extern "C" int vca_batch_status(vca_batch* b, int32_t* out, int32_t cap) {
+    return guarded([&] {
+        require(b && out && cap >= 0, "invalid argument");
+        Batch& bb = b->b;
+        bb.ctx->set_device();
+        if (!bb.status.p) bb.status.alloc(static_cast<size_t>(bb.max_slots) * 8);
+        ca_launch_status(bb.B, bb.status.as<int32_t>(), bb.ctx->stream);
+        check(hipGetLastError(), "batch status");
+        std::vector<int32_t> all(static_cast<size_t>(bb.B.nslots) * 2);
+        check(hipMemcpyAsync(all.data(), bb.status.p, all.size() * 4, hipMemcpyDeviceToHost, bb.ctx->stream),
+              "batch status");
+        check(hipStreamSynchronize(bb.ctx->stream), "batch status");
+        bb.ctx->drain();
+        for (int32_t k = 0; k < cap; ++k) {
+            const bool in = static_cast<uint32_t>(k) < bb.B.nslots && bb.members[static_cast<size_t>(k)];
+            out[2 * k] = in ? all[2 * static_cast<size_t>(k)] : 0;
+            out[2 * k + 1] = in ? all[2 * static_cast<size_t>(k) + 1] : -1;
+        }
     });
 }
 
@@ -955,19 +977,13 @@ extern "C" int vca_batch_enqueue(vca_batch* b, int32_t steps) {
 }
 
 namespace {
-// Running members: reads every member's step counter and stop flag (one copy each).
-int batch_running(Batch& b) {
+// Running members (one device read for all of them).
+int batch_running(vca_batch* b) {
+    Batch& bb = b->b;
+    std::vector<int32_t> st(static_cast<size_t>(bb.max_slots) * 2);
+    if (vca_batch_status(b, st.data(), bb.max_slots) != VCA_OK) throw VcaError(VCA_ERR_HIP, g_last_error);
     int running = 0;
-    for (vca_ca* c : b.members) {
-        if (!c) continue;
-        int32_t stop = 0;
-        check(hipMemcpyAsync(&stop, reinterpret_cast<char*>(c->ctrl.p) + offsetof(CaCtrl, stop), 4,
-                             hipMemcpyDeviceToHost, b.ctx->stream),
-              "read ctrl");
-        check(hipStreamSynchronize(b.ctx->stream), "batch status");
-        running += stop == VCA_RUNNING;
-    }
-    b.ctx->drain();
+    for (int k = 0; k < bb.max_slots; ++k) running += st[2 * static_cast<size_t>(k) + 1] == VCA_RUNNING;
     return running;
 }
 }  // namespace
@@ -976,13 +992,13 @@ extern "C" int vca_batch_run(vca_batch* b, int32_t max_steps, int32_t* running) 
     return guarded([&] {
         require(b && max_steps >= 0, "invalid argument");
         b->b.ctx->set_device();
-        int live = batch_running(b->b);
+        int live = batch_running(b);
         int remaining = max_steps;
         while (remaining > 0 && live > 0) {
             const int chunk = std::min(remaining, 128);
             b->b.enqueue(chunk);
             remaining -= chunk;
-            live = batch_running(b->b);
+            live = batch_running(b);
         }
         if (running) *running = live;
     });
@@ -999,7 +1015,7 @@ extern "C" int vca_batch_info(vca_batch* b, int32_t* members, uint64_t* tiles_us
                 if (c) t += c->P.ntiles;
             *tiles_used = t;
         }
-        if (running) *running = batch_running(b->b);
+        if (running) *running = batch_running(b);
     });
 }
 

@@ -1149,6 +1149,14 @@ __global__ void __launch_bounds__(kBlock) ca_finish_kernel(BatchDev B) {
     c->changes_total += changed;  // every change is one record
 }
 
+__global__ void __launch_bounds__(kBlock) ca_status_kernel(BatchDev B, int32_t* out) {
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < B.nslots; k += gridDim.x * kBlock) {
+        const CaSim& sim = g_sims[B.sim_base + k];
+        out[2 * k] = sim.running ? sim.D.ctrl->steps : 0;
+        out[2 * k + 1] = sim.running ? sim.D.ctrl->stop : -1;
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) ca_owner_kernel(uint32_t* owner, uint32_t base, uint32_t n, uint32_t slot) {
     for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) owner[base + j] = slot;
 }
@@ -1472,6 +1480,10 @@ void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, int 
     if (ev) hipEventRecord(ev[5], s);
     hipLaunchKernelGGL(ca_finish_kernel, dim3(B.nslots), dim3(kBlock), 0, s, B);
     if (ev) hipEventRecord(ev[6], s);
+}
+
+void ca_launch_status(const BatchDev& B, int32_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(ca_status_kernel, dim3(grid_for(B.nslots, 64)), dim3(kBlock), 0, s, B, out);
 }
 
 void ca_write_sim(uint32_t index, const CaSim* sim, hipStream_t s) {

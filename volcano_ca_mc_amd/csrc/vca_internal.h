@@ -191,6 +191,8 @@ void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, int 
                     hipEvent_t* ev);
 // owner[base .. base + n) = slot
 void ca_launch_owner(uint32_t* owner, uint32_t base, uint32_t n, uint32_t slot, hipStream_t s);
+// out[2*s] = steps, out[2*s+1] = stop (-1 when slot s is not running) for s < nslots.
+void ca_launch_status(const BatchDev& B, int32_t* out, hipStream_t s);
 // Writes entry `index` of the device's simulation table (stream ordered; `sim` must stay valid until the
 // stream reaches the copy).
 void ca_write_sim(uint32_t index, const CaSim* sim, hipStream_t s);

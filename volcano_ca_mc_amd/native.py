@@ -101,6 +101,7 @@ EXPORTED = ("vca_version", "vca_last_error", "vca_ctx_create", "vca_ctx_destroy"
             "vca_ca_run", "vca_ca_enqueue", "vca_ca_status", "vca_ca_histories", "vca_ca_download", "vca_ca_work",
             "vca_ca_footprint", "vca_ca_tiles", "vca_batch_create", "vca_batch_destroy", "vca_batch_add",
             "vca_batch_remove", "vca_batch_set_graph", "vca_batch_enqueue", "vca_batch_run", "vca_batch_info",
+            "vca_batch_status",
             "vca_mc_create", "vca_mc_create_from_ca", "vca_mc_destroy", "vca_mc_sweep", "vca_mc_counts_get",
             "vca_mc_measure", "vca_mc_download", "vca_mc_download_labels", "vca_mc_info",
             "vca_mc_download_mobility")
@@ -280,8 +281,16 @@ class Batch:
         except Exception:
             pass
 
-    def add(self, ca: LavaCA):
-        _check(lib().vca_batch_add(self.h, ca.h))
+    def add(self, ca: LavaCA) -> int:
+        slot = C.c_int32()
+        _check(lib().vca_batch_add(self.h, ca.h, C.byref(slot)))
+        return int(slot.value)
+
+    def status(self, cap: int):
+        """[(steps, stop)] per slot; stop -1 = empty slot."""
+        out = (C.c_int32 * (2 * cap))()
+        _check(lib().vca_batch_status(self.h, out, cap))
+        return [(out[2 * k], out[2 * k + 1]) for k in range(cap)]
 
     def remove(self, ca: LavaCA):
         _check(lib().vca_batch_remove(self.h, ca.h))
