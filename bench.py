@@ -105,8 +105,92 @@ def cpu_baseline(cfg_path: str, cells: int, tmp: str) -> dict | None:
                       f"mcSim {t['mcSim']:.0f} ms"}
 
 
+def _write_jobs(workloads, folder: str) -> None:
+    os.makedirs(folder, exist_ok=True)
+    for wl in workloads:
+        wl.write(folder)
+
+
+def _pipeline(args, out: str) -> dict:
+    t0 = time.time()
+    # the engine's per-job progress lines go to our stderr (the run's log)
+    r = subprocess.run([ENGINE, *args, "--out", out, "--set", "output.format=none"], stdout=sys.stderr,
+                       stderr=sys.stderr, timeout=3600)
+    if r.returncode != 0:
+        raise RuntimeError(f"engine {' '.join(args)} failed ({r.returncode})")
+    name = "serve_summary.json" if args[0] == "--serve" else "pipeline_summary.json"
+    with open(os.path.join(out, name)) as f:
+        s = json.load(f)
+    s["_wall_s"] = time.time() - t0
+    return s
+
+
+def _same_stats(a: str, b: str, names) -> bool:
+    for n in names:
+        with open(os.path.join(a, n, "stats.json"), encoding="utf-8") as f:
+            x = json.load(f)
+        with open(os.path.join(b, n, "stats.json"), encoding="utf-8") as f:
+            y = json.load(f)
+        if x != y:
+            return False
+    return True
+
+
+def bench_batched(a) -> dict:
+    """256 uneven jobs: the batched pipeline against the same jobs one at a time (--max-jobs 1), same process
+    and engine; stats.json of every job must be equal between the two."""
+    from volcano_ca_mc_amd import workloads as W
+    jobs = W.batched_mix(jobs=a.jobs)
+    with tempfile.TemporaryDirectory(prefix="vca_batched_") as tmp:
+        folder = os.path.join(tmp, "jobs")
+        _write_jobs(jobs, folder)
+        log(f"{len(jobs)} jobs written")
+        one = _pipeline(["--batch", folder, "--gpu-batch", "--max-jobs", "1"], os.path.join(tmp, "one"))
+        log(f"one at a time: {one['makespanS']:.2f} s, {one['jobsPerS']:.2f} jobs/s")
+        many = _pipeline(["--batch", folder, "--gpu-batch"], os.path.join(tmp, "many"))
+        log(f"batched: {many['makespanS']:.2f} s, {many['jobsPerS']:.2f} jobs/s")
+        same = _same_stats(os.path.join(tmp, "one"), os.path.join(tmp, "many"), [w.name for w in jobs])
+    return {"metric": "jobs/s (batched uneven simulations, bit-exact)", "value": many["jobsPerS"], "unit": "jobs/s",
+            "n_gpus": 1, "higher_is_better": True, "vs_one_at_a_time": many["jobsPerS"] / max(1e-9, one["jobsPerS"]),
+            "one_at_a_time_jobs_per_s": one["jobsPerS"], "makespan_s": many["makespanS"],
+            "one_at_a_time_makespan_s": one["makespanS"], "cell_updates_per_s": many["cellUpdatesPerS"],
+            "p50_job_ms": many["p50LatencyMs"], "p99_job_ms": many["p99LatencyMs"], "identical_stats": same,
+            "gpu_thread_ms": many.get("gpuThread"), "one_at_a_time_gpu_thread_ms": one.get("gpuThread"),
+            "dtype": "u8+i32+f32", "data": "synthetic (seeded terrains)",
+            "config": {"workload": f"{len(jobs)} jobs from one seed: 2D edges 128-2048, 3D edges 64-256, 200-800 steps, "
+                                   "50 Potts sweeps Q64", "parallelism": "single GPU"}}
+
+
+def bench_serving(a) -> dict:
+    from volcano_ca_mc_amd import workloads as W
+    arr = W.serving_stream(rate=a.rate, duration=a.duration, scale=a.scale)
+    with tempfile.TemporaryDirectory(prefix="vca_serve_") as tmp:
+        folder = os.path.join(tmp, "jobs")
+        _write_jobs([x.workload for x in arr], folder)
+        with open(os.path.join(tmp, "jobs.jsonl"), "w") as f:
+            for x in arr:
+                f.write(json.dumps({"at": x.t, "config": os.path.join(folder, x.workload.name + ".json"),
+                                    "name": x.workload.name}) + "\n")
+        log(f"{len(arr)} requests over {a.duration} s written")
+        s = _pipeline(["--serve", os.path.join(tmp, "jobs.jsonl")], os.path.join(tmp, "out"))
+    return {"metric": "served jobs/s with p50/p99 latency (bit-exact)", "value": s["jobsPerS"], "unit": "jobs/s",
+            "n_gpus": 1, "higher_is_better": True, "p50_job_ms": s["p50LatencyMs"], "p99_job_ms": s["p99LatencyMs"],
+            "offered_rate": a.rate, "makespan_s": s["makespanS"], "cell_updates_per_s": s["cellUpdatesPerS"],
+            "spin_attempts_per_s": s["spinAttemptsPerS"], "jobs": s["jobs"], "ok": s["ok"], "dtype": "u8+i32+f32",
+            "gpu_thread_ms": s.get("gpuThread"),
+            "data": "synthetic (seeded terrains)",
+            "config": {"workload": f"Poisson {a.rate}/s for {a.duration} s: 70% 2D 512x256 x300 steps, 25% 3D 256^3 x600, "
+                                   f"5% 768x768x256 x1200, 50-200 sweeps; grid scale {a.scale}",
+                       "parallelism": "single GPU"}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["cone", "batched", "serving"], default="cone",
+                    help="cone: the 3D cone job (default line); batched: 256 uneven jobs; serving: Poisson stream")
+    ap.add_argument("--jobs", type=int, default=256)
+    ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3, help="timed jobs")
     ap.add_argument("--warmup", type=int, default=1, help="untimed jobs")
@@ -119,6 +203,12 @@ def main() -> int:
         return 2
     if not os.path.exists(ENGINE):
         raise SystemExit(f"{ENGINE} missing: run `python -m volcano_ca_mc_amd.build` first")
+    if a.workload == "batched":
+        print(json.dumps(bench_batched(a)))
+        return 0
+    if a.workload == "serving":
+        print(json.dumps(bench_serving(a)))
+        return 0
     from volcano_ca_mc_amd import workloads as W
     n = max(16, int(round(512 * a.scale)))
     wl = W.cone_3d(n=n, nz=max(16, int(round(256 * a.scale))))

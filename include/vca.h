@@ -106,6 +106,24 @@ int vca_ca_download(vca_ca* ca, uint8_t* type, int32_t* flow, float* temp, uint1
 /* Work statistics: tiles visited and changes applied over all steps, tile size in cells. */
 int vca_ca_work(vca_ca* ca, uint64_t* tiles_visited, uint64_t* changes, int32_t* tile_cells);
 
+/* Everything vca_ca_status / vca_ca_histories / vca_ca_work / vca_ca_download give, in one call that reads on a
+ * stream of the simulation's own through pinned staging buffers, so it does not wait for other work of the
+ * context (another thread may keep advancing a batch meanwhile).  The simulation must have stopped and must not
+ * be in a batch (VCA_ERR_STATE otherwise).  Any array pointer may be NULL; histories get min(steps, cap) entries. */
+typedef struct {
+    int32_t steps, stop_reason;
+    uint64_t tiles_visited, changes;
+    uint8_t* type;
+    int32_t* flow;
+    float* temp;
+    uint16_t* solid;
+    uint16_t* arrival;
+    uint64_t* active;
+    uint64_t* changed;
+    int32_t cap;
+} vca_ca_result;
+int vca_ca_collect(vca_ca* ca, vca_ca_result* r);
+
 /* Device memory a simulation of these dims takes (state, change slots, tile lists), for admission control. */
 int vca_ca_footprint(const vca_ca_desc* desc, uint64_t* bytes);
 /* Tiles of the simulation (its share of a batch's tile range). */

@@ -45,10 +45,9 @@ struct Job {
     double prep_ms = 0.0;
     vca_ca* ca = nullptr;
     int slot = -1;
-    std::unique_ptr<McRun> mc;
+    int mc_stream = -1;
     Json stats = Json::object();
     uint64_t fp_ca = 0, fp_mc = 0;
-    Clock since_arrival;
     Clock ca_clock;
     double latency_ms = 0.0;
     double done_s = 0.0;
@@ -79,22 +78,30 @@ public:
                 if (j->slot >= 0) vca_batch_remove(batch_, j->ca);
                 vca_ca_destroy(j->ca);
             }
-            j->mc.reset();
         }
         vca_batch_destroy(batch_);
     }
 
     void run() {
-        const Clock start;
+        start_.reset();
+        const Clock& start = start_;
         size_t finished = 0;
         while (finished < jobs_.size()) {
             const double now = start.ms() / 1000.0;
             bool busy = false;
+            Clock ph;
             start_loading(now);
             harvest();
+            phase_ms_[0] += ph.ms();
+            ph.reset();
             busy |= admit();
+            phase_ms_[1] += ph.ms();
+            ph.reset();
             busy |= step_ca();
+            phase_ms_[2] += ph.ms();
+            ph.reset();
             busy |= step_mc();
+            phase_ms_[3] += ph.ms();
             finished = 0;
             for (auto& j : jobs_) {
                 if ((j->stage == Stage::Done || j->stage == Stage::Failed) && j->done_s == 0.0) {
@@ -108,9 +115,23 @@ public:
         makespan_s_ = start.ms() / 1000.0;
     }
     double makespan_s() const { return makespan_s_; }
+    // host time of the GPU thread per phase: host stages bookkeeping, admission (CA creation), CA chunks
+    // (enqueue, stop rules, result collection), MC intervals (creation, sweeps, measurements)
+    Json phases() const {
+        Json j = Json::object();
+        j["harvestMs"] = Json(phase_ms_[0]);
+        j["admitMs"] = Json(phase_ms_[1]);
+        j["caMs"] = Json(phase_ms_[2]);
+        j["caCollectMs"] = Json(collect_ms_);
+        j["mcMs"] = Json(phase_ms_[3]);
+        return j;
+    }
 
 private:
     static constexpr int kSlots = 1024;
+
+    // latency: from the job's arrival (not from when a worker picked it up)
+    double since_arrival(const Job& j) const { return start_.ms() - j.at * 1000.0; }
 
     int in_flight() const {
         int n = 0;
@@ -128,7 +149,6 @@ private:
             if (free_workers <= 0 || loaded >= 2 * static_cast<int>(workers_)) break;
             if (j.stage != Stage::Waiting || j.at > now) continue;
             j.stage = Stage::Loading;
-            j.since_arrival.reset();
             --free_workers;
             ++loaded;
             j.work = std::async(std::launch::async, [this, &j] {
@@ -163,6 +183,10 @@ private:
             try {
                 j.work.get();
             } catch (const std::exception& e) {
+                if (j.stage == Stage::Mc) {
+                    mc_busy_[static_cast<size_t>(j.mc_stream)] = 0;
+                    used_ -= j.fp_mc;
+                }
                 fail(j, e.what());
                 continue;
             }
@@ -178,12 +202,21 @@ private:
                     if (j.stage == Stage::Writing) write_async(j);
                     break;
                 case Stage::PostCa:
+                    vca_ca_destroy(j.ca);  // the batch's stream is idle here (between chunks)
+                    j.ca = nullptr;
+                    used_ -= j.fp_ca;
                     j.stage = j.cfg.mc.enabled ? Stage::McReady : Stage::Writing;
                     if (j.stage == Stage::Writing) write_async(j);
                     break;
+                case Stage::Mc:
+                    mc_busy_[static_cast<size_t>(j.mc_stream)] = 0;
+                    used_ -= j.fp_mc;
+                    j.stage = Stage::Done;
+                    j.latency_ms = since_arrival(j);
+                    break;
                 case Stage::Writing:
                     j.stage = Stage::Done;
-                    j.latency_ms = j.since_arrival.ms();
+                    j.latency_ms = since_arrival(j);
                     break;
                 default:
                     break;
@@ -200,12 +233,8 @@ private:
             j.slot = -1;
             used_ -= j.fp_ca;
         }
-        if (j.mc) {
-            j.mc.reset();
-            used_ -= j.fp_mc;
-        }
         j.stage = Stage::Failed;
-        j.latency_ms = j.since_arrival.ms();
+        j.latency_ms = since_arrival(j);
     }
 
     int admitted() const {
@@ -262,19 +291,19 @@ private:
             if (j.stage != Stage::Ca || st[2 * static_cast<size_t>(j.slot) + 1] == VCA_RUNNING) continue;
             try {
                 j.timing.add("caSim", j.ca_clock.ms());
+                const int32_t steps = st[2 * static_cast<size_t>(j.slot)];
                 Clock t;
-                ca_collect(j.ca, j.cfg, j.s);
                 vca_check(vca_batch_remove(batch_, j.ca), "CA");
-                vca_ca_destroy(j.ca);
-                j.ca = nullptr;
                 j.slot = -1;
-                used_ -= j.fp_ca;
-                const double collect_ms = t.ms();
+                collect_ms_ += t.ms();
+                // the final state is read on the simulation's own stream while the batch goes on; the GPU thread
+                // frees the simulation when this stage is done
                 j.stage = Stage::PostCa;
-                j.work = std::async(std::launch::async, [&j, collect_ms] {
+                j.work = std::async(std::launch::async, [&j, steps] {
                     Clock ts;
+                    ca_collect_stopped(j.ca, j.cfg, j.s, steps);
                     save_ca(j.cfg, j.s, j.out.string(), j.timing);
-                    j.timing.add("save", collect_ms + ts.ms());
+                    j.timing.add("save", ts.ms());
                     ts.reset();
                     j.stats = ca_stats(j.cfg, j.s);
                     j.timing.add("caStats", ts.ms());
@@ -292,59 +321,65 @@ private:
         return true;
     }
 
-    // MC stages: new ones start (memory permitting), running ones advance to their next measurement point
+    // MC stages run whole on worker threads, each on one of kMcStreams contexts of its own (so their small
+    // colour launches overlap each other and the batched CA steps); then the job's files are written there.
     bool step_mc() {
         bool any = false;
         for (auto& jp : jobs_) {
             Job& j = *jp;
-            try {
-                if (j.stage == Stage::McReady && !j.work.valid()) {
-                    if (used_ + j.fp_mc > budget_ && used_ > 0) continue;
-                    if (!j.s.ca_ran) j.stats["parallel"] = parallel_json(j.cfg);
-                    j.mc = std::make_unique<McRun>(j.cfg, std::move(j.prep), gpu_);
-                    j.mc->add_init_ms(j.prep_ms);
+            if (j.stage != Stage::McReady || j.work.valid()) continue;
+            if (used_ + j.fp_mc > budget_ && used_ > 0) continue;
+            int k = -1;
+            for (int c = 0; c < kMcStreams; ++c)
+                if (!mc_busy_[static_cast<size_t>(c)]) {
+                    k = c;
+                    break;
+                }
+            if (k < 0) break;
+            if (!mc_gpus_[static_cast<size_t>(k)]) mc_gpus_[static_cast<size_t>(k)] = std::make_unique<Gpu>();
+            mc_busy_[static_cast<size_t>(k)] = 1;
+            j.mc_stream = k;
+            used_ += j.fp_mc;
+            if (!j.s.ca_ran) j.stats["parallel"] = parallel_json(j.cfg);
+            j.stage = Stage::Mc;
+            any = true;
+            Gpu& g = *mc_gpus_[static_cast<size_t>(k)];
+            j.work = std::async(std::launch::async, [this, &j, &g] {
+                Clock tc;
+                {
+                    McRun run(j.cfg, std::move(j.prep), g);
+                    run.add_init_ms(j.prep_ms);
                     j.prep = McPrep{};
-                    used_ += j.fp_mc;
-                    j.stage = Stage::Mc;
-                    any = true;
+                    while (!run.done()) run.advance(run.next_stats_point());
+                    j.s.mc = run.finish(j.timing);
                 }
-                if (j.stage == Stage::Mc) {
-                    j.mc->advance(j.mc->next_stats_point());
-                    any = true;
-                    if (j.mc->done()) {
-                        j.s.mc = j.mc->finish(j.timing);
-                        j.s.mc_ran = true;
-                        j.mc.reset();
-                        used_ -= j.fp_mc;
-                        j.stats["mc"] = j.s.mc.stats;
-                        write_async(j);
-                    }
-                }
-            } catch (const std::exception& e) {
-                fail(j, e.what());
-            }
+                j.s.mc_ran = true;
+                j.stats["mc"] = j.s.mc.stats;
+                write_files(j);
+            });
         }
         return any;
     }
 
     // host stage: MC files, stats.json, times.json
+    void write_files(Job& j) {
+        const fs::path out = j.out;
+        j.s.result = CaState{};  // the CA state is written and no longer needed
+        if (j.s.mc_ran) save_mc(j.cfg, j.s.mc, out.string(), j.timing);
+        Clock tw;
+        write_json((out / "stats.json").string(), j.stats);
+        j.timing.add_file((out / "stats.json").string(), tw.ms());
+        j.timing.add(j.s.mc_ran ? "mcSave" : "save", tw.ms());
+        j.timing.add("total", since_arrival(j));
+        run_info(j.timing, j.cfg, j.s, gpu_);
+        j.timing.set_info("pipeline", "gpu-batch");
+        j.timing.record_files();
+        write_json((out / "times.json").string(), j.timing.to_json());
+        j.s.mc.crystals.clear();
+    }
     void write_async(Job& j) {
         j.stage = Stage::Writing;
-        j.s.result = CaState{};  // the CA state is written and no longer needed
-        j.work = std::async(std::launch::async, [this, &j] {
-            const fs::path out = j.out;
-            if (j.s.mc_ran) save_mc(j.cfg, j.s.mc, out.string(), j.timing);
-            Clock tw;
-            write_json((out / "stats.json").string(), j.stats);
-            j.timing.add_file((out / "stats.json").string(), tw.ms());
-            j.timing.add(j.s.mc_ran ? "mcSave" : "save", tw.ms());
-            j.timing.add("total", j.since_arrival.ms());
-            run_info(j.timing, j.cfg, j.s, gpu_);
-            j.timing.set_info("pipeline", "gpu-batch");
-            j.timing.record_files();
-            write_json((out / "times.json").string(), j.timing.to_json());
-            j.s.mc.crystals.clear();
-        });
+        j.work = std::async(std::launch::async, [this, &j] { write_files(j); });
     }
 
     void report(const Job& j) {
@@ -356,6 +391,9 @@ private:
     }
 
     static constexpr int kChunk = 32;
+    static constexpr int kMcStreams = 4;  // the device's hardware queues
+    std::unique_ptr<Gpu> mc_gpus_[kMcStreams];
+    int mc_busy_[kMcStreams] = {0, 0, 0, 0};
     const CliOptions& opt_;
     std::vector<std::unique_ptr<Job>>& jobs_;
     Gpu gpu_;
@@ -364,6 +402,9 @@ private:
     unsigned workers_ = 4;
     size_t reported_ = 0;
     double makespan_s_ = 0.0;
+    Clock start_;
+    double phase_ms_[4] = {0.0, 0.0, 0.0, 0.0};
+    double collect_ms_ = 0.0;
 };
 
 double percentile(std::vector<double> v, double p) {
@@ -371,6 +412,40 @@ double percentile(std::vector<double> v, double p) {
     std::sort(v.begin(), v.end());
     const size_t k = static_cast<size_t>(std::lround(p * static_cast<double>(v.size() - 1)));
     return v[std::min(k, v.size() - 1)];
+}
+
+// Throughput and latency of a finished pipeline run (serve_summary.json / pipeline_summary.json).
+Json pipeline_summary(const std::vector<std::unique_ptr<Job>>& jobs, double makespan, std::vector<double>& lat) {
+    double cell_updates = 0.0, spin_attempts = 0.0;
+    size_t ok = 0;
+    Json list = Json::array();
+    for (const auto& j : jobs) {
+        Json r = Json::object();
+        r["name"] = j->name;
+        r["at"] = Json(j->at);
+        r["latencyMs"] = Json(j->latency_ms);
+        r["status"] = j->stage == Stage::Done ? "ok" : "error";
+        if (!j->error.empty()) r["error"] = j->error;
+        if (j->stage == Stage::Done) {
+            ++ok;
+            lat.push_back(j->latency_ms);
+            if (j->s.ca_ran) cell_updates += static_cast<double>(j->cfg.space.cells()) * j->s.steps;
+            if (j->s.mc_ran) spin_attempts += static_cast<double>(j->s.mc.domain_cells) * j->s.mc.mcs;
+        }
+        list.push_back(r);
+    }
+    Json sum = Json::object();
+    sum["jobs"] = Json(static_cast<unsigned long long>(jobs.size()));
+    sum["ok"] = Json(static_cast<unsigned long long>(ok));
+    sum["makespanS"] = Json(makespan);
+    sum["jobsPerS"] = Json(makespan > 0 ? static_cast<double>(ok) / makespan : 0.0);
+    sum["p50LatencyMs"] = Json(percentile(lat, 0.50));
+    sum["p99LatencyMs"] = Json(percentile(lat, 0.99));
+    sum["cellUpdates"] = Json(cell_updates);
+    sum["cellUpdatesPerS"] = Json(makespan > 0 ? cell_updates / makespan : 0.0);
+    sum["spinAttemptsPerS"] = Json(makespan > 0 ? spin_attempts / makespan : 0.0);
+    sum["jobsList"] = list;
+    return sum;
 }
 
 }  // namespace
@@ -391,10 +466,12 @@ int run_batched(const CliOptions& opt) {
               << out.generic_string() << " ===\n"
               << std::flush;
     double makespan = 0.0;
+    Json phases;
     {
         Pipeline p(opt, jobs);
         p.run();
         makespan = p.makespan_s();
+        phases = p.phases();
     }
     std::vector<BatchOutcome> outcomes;
     size_t ok = 0;
@@ -409,6 +486,12 @@ int run_batched(const CliOptions& opt) {
     }
     const fs::path summary = out / "batch_summary.csv";
     write_batch_summary(summary, outcomes);
+    std::vector<double> lat;
+    Json ps = pipeline_summary(jobs, makespan, lat);
+    ps["maxJobs"] = opt.max_jobs;
+    ps["gpuThread"] = phases;
+    ps["graphSteps"] = opt.steps_per_graph;
+    write_json((out / "pipeline_summary.json").string(), ps);
     std::cout << "\nRazem: " << outcomes.size() << " przebiegów: " << ok << " ok, " << (outcomes.size() - ok)
               << " z błędem, " << makespan << " s\nPodsumowanie   " << summary.generic_string() << '\n';
     return ok == outcomes.size() ? 0 : 5;
@@ -438,40 +521,17 @@ int run_serve(const CliOptions& opt) {
     std::cout << "=== Serwowanie (--serve): " << jobs.size() << " zadań ze strumienia " << opt.serve_path << " ===\n"
               << std::flush;
     double makespan = 0.0;
+    Json phases;
     {
         Pipeline p(opt, jobs);
         p.run();
         makespan = p.makespan_s();
+        phases = p.phases();
     }
     std::vector<double> lat;
-    double cell_updates = 0.0, spin_attempts = 0.0;
-    size_t ok = 0;
-    Json list = Json::array();
-    for (const auto& j : jobs) {
-        Json r = Json::object();
-        r["name"] = j->name;
-        r["at"] = Json(j->at);
-        r["latencyMs"] = Json(j->latency_ms);
-        r["status"] = j->stage == Stage::Done ? "ok" : "error";
-        if (!j->error.empty()) r["error"] = j->error;
-        if (j->stage == Stage::Done) {
-            ++ok;
-            lat.push_back(j->latency_ms);
-            cell_updates += static_cast<double>(j->cfg.space.cells()) * j->s.steps;
-            if (j->s.mc_ran) spin_attempts += static_cast<double>(j->s.mc.domain_cells) * j->s.mc.mcs;
-        }
-        list.push_back(r);
-    }
-    Json sum = Json::object();
-    sum["jobs"] = Json(static_cast<unsigned long long>(jobs.size()));
-    sum["ok"] = Json(static_cast<unsigned long long>(ok));
-    sum["makespanS"] = Json(makespan);
-    sum["jobsPerS"] = Json(makespan > 0 ? static_cast<double>(ok) / makespan : 0.0);
-    sum["p50LatencyMs"] = Json(percentile(lat, 0.50));
-    sum["p99LatencyMs"] = Json(percentile(lat, 0.99));
-    sum["cellUpdatesPerS"] = Json(makespan > 0 ? cell_updates / makespan : 0.0);
-    sum["spinAttemptsPerS"] = Json(makespan > 0 ? spin_attempts / makespan : 0.0);
-    sum["jobsList"] = list;
+    Json sum = pipeline_summary(jobs, makespan, lat);
+    const unsigned long long ok = sum.at("ok").get_uint();
+    sum["gpuThread"] = phases;
     write_json((out / "serve_summary.json").string(), sum);
     std::cout << "\nRazem: " << ok << "/" << jobs.size() << " zadań ok, " << makespan << " s, p50 "
               << percentile(lat, 0.50) << " ms, p99 " << percentile(lat, 0.99) << " ms\nPodsumowanie   "

@@ -261,15 +261,45 @@ vca_ca* create_ca(const Config& cfg, const CaInputs& in, Gpu& gpu) {
 void ca_collect(vca_ca* ca, const Config& cfg, RunSummary& s) {
     int32_t steps = 0, stop = 0;
     vca_check(vca_ca_status(ca, &steps, &stop), "CA");
-    s.steps = steps;
-    s.stop = stop;
     s.active.assign(static_cast<size_t>(steps), 0);
     s.changed.assign(static_cast<size_t>(steps), 0);
-    int32_t n = 0;
-    vca_check(vca_ca_histories(ca, s.active.data(), s.changed.data(), steps, &n), "CA");
-    vca_check(vca_ca_work(ca, &s.tiles_visited, &s.changes, &s.tile_cells), "CA");
     s.result.resize(cfg.space);
-    download(ca, s.result);
+    vca_ca_result r{};
+    r.type = reinterpret_cast<uint8_t*>(s.result.type.data());
+    r.flow = s.result.flow.data();
+    r.temp = s.result.temp.data();
+    r.solid = s.result.solid.data();
+    r.arrival = s.result.arrival.data();
+    r.active = s.active.data();
+    r.changed = s.changed.data();
+    r.cap = steps;
+    vca_check(vca_ca_collect(ca, &r), "CA");
+    s.steps = r.steps;
+    s.stop = r.stop_reason;
+    s.tiles_visited = r.tiles_visited;
+    s.changes = r.changes;
+    s.tile_cells = 1024;
+}
+
+void ca_collect_stopped(vca_ca* ca, const Config& cfg, RunSummary& s, int32_t steps) {
+    s.active.assign(static_cast<size_t>(steps), 0);
+    s.changed.assign(static_cast<size_t>(steps), 0);
+    s.result.resize(cfg.space);
+    vca_ca_result r{};
+    r.type = reinterpret_cast<uint8_t*>(s.result.type.data());
+    r.flow = s.result.flow.data();
+    r.temp = s.result.temp.data();
+    r.solid = s.result.solid.data();
+    r.arrival = s.result.arrival.data();
+    r.active = s.active.data();
+    r.changed = s.changed.data();
+    r.cap = steps;
+    vca_check(vca_ca_collect(ca, &r), "CA");
+    s.steps = r.steps;
+    s.stop = r.stop_reason;
+    s.tiles_visited = r.tiles_visited;
+    s.changes = r.changes;
+    s.tile_cells = 1024;
 }
 
 void save_ca(const Config& cfg, const RunSummary& s, const std::string& out_dir, Timing& timing) {

@@ -179,6 +179,9 @@ vca_ca_desc make_ca_desc(const Config& cfg);
 CaInputs ca_inputs(const Config& cfg);
 vca_ca* create_ca(const Config& cfg, const CaInputs& in, Gpu& gpu);
 void ca_collect(vca_ca* ca, const Config& cfg, RunSummary& s);  // steps, histories, work, final state
+// Same for a simulation known to be stopped after `steps` steps and out of any batch: reads only on the
+// simulation's own stream (safe from a worker thread while the batch goes on).
+void ca_collect_stopped(vca_ca* ca, const Config& cfg, RunSummary& s, int32_t steps);
 void save_ca(const Config& cfg, const RunSummary& s, const std::string& out_dir, Timing& timing);
 Json parallel_json(const Config& cfg);
 void write_config_used(const Config& cfg, const std::string& out_dir, Timing& timing);

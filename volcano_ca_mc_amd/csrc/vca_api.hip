@@ -54,14 +54,33 @@ void require(bool ok, const std::string& msg) {
     if (!ok) throw VcaError(VCA_ERR_ARG, msg);
 }
 
-// Device allocation that remembers its size; freed with the owner.
+// Stream that device allocations of the current API call are ordered on (set by AllocStream for the call).
+thread_local hipStream_t t_alloc_stream = nullptr;
+struct AllocStream {
+    hipStream_t prev;
+    explicit AllocStream(hipStream_t s) : prev(t_alloc_stream) { t_alloc_stream = s; }
+    ~AllocStream() { t_alloc_stream = prev; }
+};
+
+// Device allocation that remembers its size; freed with the owner.  Inside an AllocStream scope it comes from
+// the device's stream-ordered pool (hipMallocAsync / hipFreeAsync on that stream): no device-wide
+// synchronisation per allocation, and freed blocks are reused by the next job.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    hipStream_t stream = nullptr;
+    bool pooled = false;
     void alloc(size_t n) {
         release();
         if (n == 0) n = 1;
-        hipError_t e = hipMalloc(&p, n);
+        hipError_t e;
+        if (t_alloc_stream) {
+            e = hipMallocAsync(&p, n, t_alloc_stream);
+            stream = t_alloc_stream;
+            pooled = e == hipSuccess;
+        } else {
+            e = hipMalloc(&p, n);
+        }
         if (e != hipSuccess) {
             p = nullptr;
             (void)hipGetLastError();
@@ -70,9 +89,15 @@ struct DevBuf {
         bytes = n;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (pooled)
+                (void)hipFreeAsync(p, stream);
+            else
+                (void)hipFree(p);
+        }
         p = nullptr;
         bytes = 0;
+        pooled = false;
     }
     ~DevBuf() { release(); }
     template <class T>
@@ -236,6 +261,7 @@ struct vca_ca {
         part, ctrl, hist_a, hist_c;
     uint64_t n = 0;
     std::unique_ptr<Batch> own;   // batch of one: the solo path
+    hipStream_t side = nullptr;   // reads of a stopped simulation (vca_ca_collect)
     Batch* member = nullptr;      // the shared batch this simulation is in
     int slot = -1;
 };
@@ -290,6 +316,7 @@ struct Batch {
     int steps_per_graph = 0;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
+    int capture_failures = 0;
 
     void init(vca_ctx* c, int slots, uint32_t tiles) {
         ctx = c;
@@ -465,18 +492,26 @@ struct Batch {
         hipStream_t s = ctx->stream;
         if (s == nullptr) return false;  // the legacy stream cannot be captured
         hipGraph_t g = nullptr;
-        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) {
             (void)hipGetLastError();
             return false;
         }
         const uint32_t sh = shapes();
         const bool lm = any_lava_mode();
         for (int k = 0; k < steps_per_graph; ++k) ca_launch_step(B, sh, lm, grid_tiles, s, nullptr);
-        check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+        // a capture invalidated by the runtime leaves nothing enqueued: the caller launches the steps directly
+        if (hipStreamEndCapture(s, &g) != hipSuccess || g == nullptr) {
+            (void)hipGetLastError();
+            if (g) (void)hipGraphDestroy(g);
+            return false;
+        }
         hipGraphExec_t exec = nullptr;
         const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
-        check(e, "hipGraphInstantiate");
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
         graph = exec;
         graph_steps = steps_per_graph;
         return true;
@@ -484,7 +519,8 @@ struct Batch {
     void enqueue(int steps) {
         if (steps <= 0 || count() == 0) return;
         if (steps_per_graph > 0 && !ctx->timing) {
-            if (!graph && !capture()) steps_per_graph = 0;
+            if (!graph && !capture()) ++capture_failures;
+            if (capture_failures >= 3) steps_per_graph = 0;  // this process cannot capture: direct launches
             while (graph && steps >= graph_steps) {
                 check(hipGraphLaunch(graph, ctx->stream), "hipGraphLaunch");
                 steps -= graph_steps;
@@ -543,6 +579,13 @@ extern "C" int vca_ctx_create(int device, void* stream, vca_ctx** out) {
             check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
             c->own_stream = true;
         }
+        // keep freed blocks of the stream-ordered pool for reuse instead of returning them at each sync
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+            uint64_t keep = ~0ULL;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+        (void)hipGetLastError();
         *out = c;
     });
 }
@@ -616,6 +659,7 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
                     "w trybie lava przestrzeń 2D to przekrój x-z (ny = 1)");
         }
         ctx->set_device();
+        AllocStream alloc_on(ctx->stream);
         auto ca = std::make_unique<vca_ca>();
         ca->ctx = ctx;
         ca->n = n;
@@ -766,7 +810,116 @@ extern "C" int vca_ca_destroy(vca_ca* ca) {
         (void)hipStreamSynchronize(ca->ctx->stream);
         if (ca->member) ca->member->detach(ca, ca->slot);
         ca->member = nullptr;
+        if (ca->side) {
+            (void)hipStreamSynchronize(ca->side);
+            (void)hipStreamDestroy(ca->side);
+        }
         delete ca;
+    });
+}
+
+namespace {
+// Pinned staging buffers shared by every collecting thread (allocated once, reused).
+class PinnedPool {
+public:
+    static constexpr size_t kBytes = 32u << 20;
+    static void* take() {
+        {
+            std::lock_guard<std::mutex> lock(mu());
+            if (!free_list().empty()) {
+                void* p = free_list().back();
+                free_list().pop_back();
+                return p;
+            }
+        }
+        void* p = nullptr;
+        check(hipHostMalloc(&p, kBytes, hipHostMallocDefault), "hipHostMalloc");
+        return p;
+    }
+    static void give(void* p) {
+        std::lock_guard<std::mutex> lock(mu());
+        free_list().push_back(p);
+    }
+
+private:
+    static std::mutex& mu() {
+        static std::mutex m;
+        return m;
+    }
+    static std::vector<void*>& free_list() {
+        static std::vector<void*> v;
+        return v;
+    }
+};
+
+// Device -> host copy of `bytes` through two pinned buffers on stream s (copy of one overlaps the host copy
+// of the other).
+void staged_download(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (!dst || bytes == 0) return;
+    void* buf[2] = {PinnedPool::take(), PinnedPool::take()};
+    hipEvent_t ev[2];
+    check(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming), "hipEventCreate");
+    check(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming), "hipEventCreate");
+    size_t pending[2] = {0, 0}, at[2] = {0, 0};
+    size_t off = 0;
+    int k = 0;
+    try {
+        while (off < bytes || pending[0] || pending[1]) {
+            if (pending[k]) {
+                check(hipEventSynchronize(ev[k]), "download");
+                std::memcpy(static_cast<char*>(dst) + at[k], buf[k], pending[k]);
+                pending[k] = 0;
+            }
+            if (off < bytes) {
+                const size_t n = std::min(PinnedPool::kBytes, bytes - off);
+                check(hipMemcpyAsync(buf[k], static_cast<const char*>(src) + off, n, hipMemcpyDeviceToHost, s),
+                      "download");
+                check(hipEventRecord(ev[k], s), "download");
+                pending[k] = n;
+                at[k] = off;
+                off += n;
+            }
+            k ^= 1;
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        PinnedPool::give(buf[0]);
+        PinnedPool::give(buf[1]);
+        throw;
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    PinnedPool::give(buf[0]);
+    PinnedPool::give(buf[1]);
+}
+}  // namespace
+
+extern "C" int vca_ca_collect(vca_ca* ca, vca_ca_result* r) {
+    return guarded([&] {
+        require(ca && r, "NULL argument");
+        if (ca->member) throw VcaError(VCA_ERR_STATE, "the simulation is in a batch: remove it first");
+        ca->ctx->set_device();
+        if (!ca->side) check(hipStreamCreateWithFlags(&ca->side, hipStreamNonBlocking), "hipStreamCreate");
+        CaCtrl c;
+        check(hipMemcpyAsync(&c, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, ca->side), "read ctrl");
+        check(hipStreamSynchronize(ca->side), "read ctrl");
+        if (c.stop == VCA_RUNNING) throw VcaError(VCA_ERR_STATE, "the simulation is still running");
+        r->steps = c.steps;
+        r->stop_reason = c.stop;
+        r->tiles_visited = c.tiles_visited;
+        r->changes = c.changes_total;
+        const size_t m = static_cast<size_t>(std::min(c.steps, std::max(0, r->cap)));
+        if (m > 0 && r->active) staged_download(r->active, ca->hist_a.p, m * 8, ca->side);
+        if (m > 0 && r->changed) staged_download(r->changed, ca->hist_c.p, m * 8, ca->side);
+        const uint64_t n = ca->n;
+        staged_download(r->type, ca->type.p, n, ca->side);
+        staged_download(r->flow, ca->flow.p, n * 4, ca->side);
+        staged_download(r->temp, ca->temp.p, n * 4, ca->side);
+        staged_download(r->solid, ca->solid.p, n * 2, ca->side);
+        staged_download(r->arrival, ca->arrival.p, n * 2, ca->side);
+        check(hipStreamSynchronize(ca->side), "collect");
     });
 }
 
@@ -825,9 +978,12 @@ extern "C" int vca_ca_histories(vca_ca* ca, uint64_t* active, uint64_t* changed,
         ca_status_sync(ca, &steps, nullptr);
         const int32_t m = std::min(steps, std::max(0, cap));
         if (m > 0 && active)
-            check(hipMemcpy(active, ca->hist_a.p, static_cast<size_t>(m) * 8, hipMemcpyDeviceToHost), "hist");
+            check(hipMemcpyAsync(active, ca->hist_a.p, static_cast<size_t>(m) * 8, hipMemcpyDeviceToHost, ca->ctx->stream),
+                  "hist");
         if (m > 0 && changed)
-            check(hipMemcpy(changed, ca->hist_c.p, static_cast<size_t>(m) * 8, hipMemcpyDeviceToHost), "hist");
+            check(hipMemcpyAsync(changed, ca->hist_c.p, static_cast<size_t>(m) * 8, hipMemcpyDeviceToHost, ca->ctx->stream),
+                  "hist");
+        check(hipStreamSynchronize(ca->ctx->stream), "hist");
         if (n) *n = steps;
     });
 }
@@ -839,11 +995,12 @@ extern "C" int vca_ca_download(vca_ca* ca, uint8_t* type, int32_t* flow, float* 
         ca->ctx->set_device();
         check(hipStreamSynchronize(ca->ctx->stream), "sync");
         const uint64_t n = ca->n;
-        if (type) check(hipMemcpy(type, ca->type.p, n, hipMemcpyDeviceToHost), "download");
-        if (flow) check(hipMemcpy(flow, ca->flow.p, n * 4, hipMemcpyDeviceToHost), "download");
-        if (temp) check(hipMemcpy(temp, ca->temp.p, n * 4, hipMemcpyDeviceToHost), "download");
-        if (solid) check(hipMemcpy(solid, ca->solid.p, n * 2, hipMemcpyDeviceToHost), "download");
-        if (arrival) check(hipMemcpy(arrival, ca->arrival.p, n * 2, hipMemcpyDeviceToHost), "download");
+        if (type) check(hipMemcpyAsync(type, ca->type.p, n, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
+        if (flow) check(hipMemcpyAsync(flow, ca->flow.p, n * 4, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
+        if (temp) check(hipMemcpyAsync(temp, ca->temp.p, n * 4, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
+        if (solid) check(hipMemcpyAsync(solid, ca->solid.p, n * 2, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
+        if (arrival) check(hipMemcpyAsync(arrival, ca->arrival.p, n * 2, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
+        check(hipStreamSynchronize(ca->ctx->stream), "download");
     });
 }
 
@@ -894,6 +1051,7 @@ extern "C" int vca_batch_create(vca_ctx* ctx, int32_t max_sims, uint64_t max_til
         require(max_sims >= 1 && max_sims <= kMaxSlots, "max_sims poza zakresem 1..1024");
         require(max_tiles >= 1 && max_tiles <= kTileMask, "max_tiles poza zakresem");
         ctx->set_device();
+        AllocStream alloc_on(ctx->stream);
         auto b = std::make_unique<vca_batch>();
         b->b.init(ctx, max_sims, static_cast<uint32_t>(max_tiles));
         check(hipStreamSynchronize(ctx->stream), "batch create");
@@ -932,6 +1090,7 @@ extern "C" int vca_batch_status(vca_batch* b, int32_t* out, int32_t cap) {
         require(b && out && cap >= 0, "invalid argument");
         Batch& bb = b->b;
         bb.ctx->set_device();
+        AllocStream alloc_on(bb.ctx->stream);
         if (!bb.status.p) bb.status.alloc(static_cast<size_t>(bb.max_slots) * 8);
         ca_launch_status(bb.B, bb.status.as<int32_t>(), bb.ctx->stream);
         check(hipGetLastError(), "batch status");
@@ -1083,6 +1242,7 @@ extern "C" int vca_mc_create(vca_ctx* ctx, const vca_mc_desc* d, const uint8_t* 
         const uint64_t n = static_cast<uint64_t>(d->nx) * d->ny * d->nz;
         require(n <= 2000000000ULL, "przestrzeń ma za dużo komórek");
         ctx->set_device();
+        AllocStream alloc_on(ctx->stream);
         auto mc = std::make_unique<vca_mc>();
         mc->ctx = ctx;
         mc->n = n;
@@ -1280,6 +1440,7 @@ extern "C" int vca_mc_measure(vca_mc* mc, vca_mc_stats* st, vca_mc_class* classe
     return guarded([&] {
         require(mc && st, "NULL argument");
         mc->ctx->set_device();
+        AllocStream alloc_on(mc->ctx->stream);
         hipStream_t s = mc->ctx->stream;
         const uint64_t n = mc->n;
         if (!mc->parent.p) {
@@ -1327,11 +1488,13 @@ extern "C" int vca_mc_download(vca_mc* mc, int32_t* crystal) {
         check(hipStreamSynchronize(mc->ctx->stream), "sync");
         const uint64_t n = mc->n;
         if (mc->spin_bytes == 4) {
-            check(hipMemcpy(crystal, mc->spin.p, n * 4, hipMemcpyDeviceToHost), "download");
+            check(hipMemcpyAsync(crystal, mc->spin.p, n * 4, hipMemcpyDeviceToHost, mc->ctx->stream), "download");
+            check(hipStreamSynchronize(mc->ctx->stream), "download");
             return;
         }
         std::vector<uint8_t> raw(n * static_cast<uint64_t>(mc->spin_bytes));
-        check(hipMemcpy(raw.data(), mc->spin.p, raw.size(), hipMemcpyDeviceToHost), "download");
+        check(hipMemcpyAsync(raw.data(), mc->spin.p, raw.size(), hipMemcpyDeviceToHost, mc->ctx->stream), "download");
+        check(hipStreamSynchronize(mc->ctx->stream), "download");
         if (mc->spin_bytes == 1)
             for (uint64_t i = 0; i < n; ++i) crystal[i] = raw[i];
         else
@@ -1366,6 +1529,7 @@ extern "C" int vca_mc_download_mobility(vca_mc* mc, float* mobility) {
         require(mc && mobility, "NULL argument");
         require(mc->has_mob_cells, "brak mobilności");
         mc->ctx->set_device();
-        check(hipMemcpy(mobility, mc->mob_cells.p, mc->n * 4, hipMemcpyDeviceToHost), "mobility");
+        check(hipMemcpyAsync(mobility, mc->mob_cells.p, mc->n * 4, hipMemcpyDeviceToHost, mc->ctx->stream), "mobility");
+        check(hipStreamSynchronize(mc->ctx->stream), "mobility");
     });
 }

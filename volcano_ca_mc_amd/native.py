@@ -101,7 +101,7 @@ EXPORTED = ("vca_version", "vca_last_error", "vca_ctx_create", "vca_ctx_destroy"
             "vca_ca_run", "vca_ca_enqueue", "vca_ca_status", "vca_ca_histories", "vca_ca_download", "vca_ca_work",
             "vca_ca_footprint", "vca_ca_tiles", "vca_batch_create", "vca_batch_destroy", "vca_batch_add",
             "vca_batch_remove", "vca_batch_set_graph", "vca_batch_enqueue", "vca_batch_run", "vca_batch_info",
-            "vca_batch_status",
+            "vca_batch_status", "vca_ca_collect",
             "vca_mc_create", "vca_mc_create_from_ca", "vca_mc_destroy", "vca_mc_sweep", "vca_mc_counts_get",
             "vca_mc_measure", "vca_mc_download", "vca_mc_download_labels", "vca_mc_info",
             "vca_mc_download_mobility")
