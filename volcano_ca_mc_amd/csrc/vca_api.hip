@@ -261,7 +261,7 @@ struct vca_ca {
         part, ctrl, hist_a, hist_c;
     uint64_t n = 0;
     std::unique_ptr<Batch> own;   // batch of one: the solo path
-    hipStream_t side = nullptr;   // reads of a stopped simulation (vca_ca_collect)
+    hipStream_t side = nullptr;   // during vca_ca_collect: the pooled stream its reads go on
     Batch* member = nullptr;      // the shared batch this simulation is in
     int slot = -1;
 };
@@ -439,7 +439,6 @@ struct Batch {
         check(hipMemcpyAsync(tile_base.as<uint32_t>() + slot, &base, 4, hipMemcpyHostToDevice, s), "batch add");
         ca_launch_owner(owner.as<uint32_t>(), base, ca->P.ntiles, static_cast<uint32_t>(slot), s);
         check(hipGetLastError(), "batch add");
-        check(hipStreamSynchronize(s), "batch add");
         members[static_cast<size_t>(slot)] = ca;
         bases.resize(static_cast<size_t>(max_slots));
         bases[static_cast<size_t>(slot)] = base;
@@ -453,7 +452,6 @@ struct Batch {
         const uint32_t base = bases[static_cast<size_t>(slot)];
         ca_launch_owner(owner.as<uint32_t>(), base, ca->P.ntiles, kNoSlot, s);
         check(hipGetLastError(), "batch remove");
-        check(hipStreamSynchronize(s), "batch remove");
         give_range(base, ca->P.ntiles);
         members[static_cast<size_t>(slot)] = nullptr;
         refresh();
@@ -768,7 +766,7 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
             h.alloc(cols * 4);
             check(hipMemcpyAsync(h.p, height, cols * 4, hipMemcpyHostToDevice, s), "copy height");
             ca_launch_terrain(P, D, h.as<int32_t>(), s);
-            check(hipStreamSynchronize(s), "terrain");
+            check(hipGetLastError(), "terrain");
         }
         if (nvents > 0) {
             DevBuf c, f, t;
@@ -779,26 +777,17 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
             check(hipMemcpyAsync(f.p, vflow.data(), vflow.size() * 4, hipMemcpyHostToDevice, s), "copy");
             check(hipMemcpyAsync(t.p, vtemp.data(), vtemp.size() * 4, hipMemcpyHostToDevice, s), "copy");
             ca_launch_seeds(P, D, c.as<uint32_t>(), f.as<int32_t>(), t.as<float>(), nvents, s);
-            check(hipStreamSynchronize(s), "seeds");
+            check(hipGetLastError(), "seeds");
+            if (!c.pooled) check(hipStreamSynchronize(s), "seeds");  // a non-pooled free would not wait
         }
+        // countInitialState (LavaCA.hpp:1252-1280) and the initial stop rule, on the device
         ca_launch_init_flags(P, D, s);
-        CaCtrl c0;
-        check(hipMemcpyAsync(&c0, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, s), "read ctrl");
-        check(hipStreamSynchronize(s), "init");
-        // countInitialState (LavaCA.hpp:1252-1280)
-        int32_t stop = VCA_RUNNING;
-        if (!classic && c0.lava_total == 0)
-            stop = VCA_NO_LAVA;
-        else if (classic && c0.air_total == 0)
-            stop = VCA_NO_AIR;
-        else if (d->max_steps <= 0)
-            stop = VCA_MAX_STEPS;
-        c0.stop = stop;
-        check(hipMemcpyAsync(ca->ctrl.p, &c0, sizeof(CaCtrl), hipMemcpyHostToDevice, s), "write ctrl");
-        check(hipStreamSynchronize(s), "init");
+        ca_launch_init_stop(P, D, s);
+        check(hipGetLastError(), "init");
         ca->own = std::make_unique<Batch>();
         ca->own->init(ctx, 1, P.ntiles);
         ca->own->attach(ca.get());
+        check(hipStreamSynchronize(s), "init");
         *out = ca.release();
     });
 }
@@ -810,15 +799,43 @@ extern "C" int vca_ca_destroy(vca_ca* ca) {
         (void)hipStreamSynchronize(ca->ctx->stream);
         if (ca->member) ca->member->detach(ca, ca->slot);
         ca->member = nullptr;
-        if (ca->side) {
-            (void)hipStreamSynchronize(ca->side);
-            (void)hipStreamDestroy(ca->side);
-        }
         delete ca;
     });
 }
 
 namespace {
+// Side streams of collecting simulations, reused (creating and destroying a stream per job costs ~ms).
+class StreamPool {
+public:
+    static hipStream_t take() {
+        {
+            std::lock_guard<std::mutex> lock(mu());
+            if (!free_list().empty()) {
+                hipStream_t s = free_list().back();
+                free_list().pop_back();
+                return s;
+            }
+        }
+        hipStream_t s = nullptr;
+        check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        return s;
+    }
+    static void give(hipStream_t s) {
+        std::lock_guard<std::mutex> lock(mu());
+        free_list().push_back(s);
+    }
+
+private:
+    static std::mutex& mu() {
+        static std::mutex m;
+        return m;
+    }
+    static std::vector<hipStream_t>& free_list() {
+        static std::vector<hipStream_t> v;
+        return v;
+    }
+};
+
 // Pinned staging buffers shared by every collecting thread (allocated once, reused).
 class PinnedPool {
 public:
@@ -901,7 +918,14 @@ extern "C" int vca_ca_collect(vca_ca* ca, vca_ca_result* r) {
         require(ca && r, "NULL argument");
         if (ca->member) throw VcaError(VCA_ERR_STATE, "the simulation is in a batch: remove it first");
         ca->ctx->set_device();
-        if (!ca->side) check(hipStreamCreateWithFlags(&ca->side, hipStreamNonBlocking), "hipStreamCreate");
+        struct Side {
+            hipStream_t s = StreamPool::take();
+            ~Side() {
+                (void)hipStreamSynchronize(s);
+                StreamPool::give(s);
+            }
+        } side;
+        ca->side = side.s;
         CaCtrl c;
         check(hipMemcpyAsync(&c, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, ca->side), "read ctrl");
         check(hipStreamSynchronize(ca->side), "read ctrl");
@@ -920,6 +944,7 @@ extern "C" int vca_ca_collect(vca_ca* ca, vca_ca_result* r) {
         staged_download(r->solid, ca->solid.p, n * 2, ca->side);
         staged_download(r->arrival, ca->arrival.p, n * 2, ca->side);
         check(hipStreamSynchronize(ca->side), "collect");
+        ca->side = nullptr;
     });
 }
 

@@ -356,6 +356,21 @@ __global__ void ca_seeds_kernel(CaDev D, const uint32_t* cells, const int32_t* f
     }
 }
 
+// The stop rule before the first step (LavaCA.hpp:1252-1280): no lava (lava mode), no air (classic mode),
+// or maxSteps 0.
+__global__ void ca_init_stop_kernel(CaParams P, CaDev D) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    CaCtrl* c = D.ctrl;
+    int32_t stop = 0;
+    if (!P.classic && c->lava_total == 0)
+        stop = 1;  // NoLava
+    else if (P.classic && c->air_total == 0)
+        stop = 2;  // NoAir
+    else if (P.max_steps <= 0)
+        stop = 4;  // MaxSteps
+    c->stop = stop;
+}
+
 // Tile flags and the initial LAVA / AIR counts (countInitialState, LavaCA.hpp:1252-1280).
 __global__ void __launch_bounds__(kBlock) ca_init_flags_kernel(CaParams P, CaDev D) {
     for (uint32_t t = blockIdx.x; t < P.ntiles; t += gridDim.x) {
@@ -1443,6 +1458,10 @@ void ca_launch_terrain(const CaParams& P, const CaDev& D, const int32_t* height,
 void ca_launch_seeds(const CaParams&, const CaDev& D, const uint32_t* cells, const int32_t* flows, const float* temps,
                      int n, hipStream_t s) {
     hipLaunchKernelGGL(ca_seeds_kernel, dim3(1), dim3(64), 0, s, D, cells, flows, temps, n);
+}
+
+void ca_launch_init_stop(const CaParams& P, const CaDev& D, hipStream_t s) {
+    hipLaunchKernelGGL(ca_init_stop_kernel, dim3(1), dim3(64), 0, s, P, D);
 }
 
 void ca_launch_init_flags(const CaParams& P, const CaDev& D, hipStream_t s) {

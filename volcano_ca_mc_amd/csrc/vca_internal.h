@@ -185,6 +185,7 @@ void ca_launch_terrain(const CaParams& P, const CaDev& D, const int32_t* height,
 void ca_launch_seeds(const CaParams& P, const CaDev& D, const uint32_t* cells, const int32_t* flows,
                      const float* temps, int n, hipStream_t s);
 void ca_launch_init_flags(const CaParams& P, const CaDev& D, hipStream_t s);
+void ca_launch_init_stop(const CaParams& P, const CaDev& D, hipStream_t s);
 // One step of every running simulation of the batch: front lists, D(n), rules, wide votes, apply, finish.
 // shapes = bit mask of the tile shapes present; ev != nullptr records 7 events around the kernel groups.
 void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, int grid_tiles, hipStream_t s,
