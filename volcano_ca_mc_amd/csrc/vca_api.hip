@@ -330,7 +330,7 @@ struct Batch {
         check(hipGetLastError(), "simulation table");
         owner.alloc(static_cast<size_t>(cap) * 4);
         tile_base.alloc(static_cast<size_t>(slots) * 4);
-        active.alloc(static_cast<size_t>(cap) * kShapes * 4);
+        active.alloc(static_cast<size_t>(cap) * kShapes * 8);
         lava.alloc(static_cast<size_t>(cap) * kShapes * 4);
         wide.alloc(static_cast<size_t>(cap) * 4);
         ctrl.alloc(sizeof(BatchCtrl));
@@ -345,7 +345,7 @@ struct Batch {
         B.ntiles = 0;
         B.nslots = 0;
         B.cap = cap;
-        B.active = active.as<uint32_t>();
+        B.active = active.as<uint64_t>();
         B.lava = lava.as<uint32_t>();
         B.wide = wide.as<uint32_t>();
         B.ctrl = ctrl.as<BatchCtrl>();
@@ -1054,7 +1054,7 @@ extern "C" int vca_ca_footprint(const vca_ca_desc* d, uint64_t* bytes) {
         const uint64_t cols = static_cast<uint64_t>(d->nx) * d->ny;
         // state 14 B/cell, record slots 16 B per tile cell, per-tile counts and flags, the batch-of-one lists,
         // spill levels, histories
-        *bytes = n * 14 + tiles * (kTileCells * sizeof(Change) + 9 + (2 * kShapes + 2) * 4) + cols * 4 +
+        *bytes = n * 14 + tiles * (kTileCells * sizeof(Change) + 9 + (3 * kShapes + 2) * 4) + cols * 4 +
                  static_cast<uint64_t>(std::max(1, d->max_steps)) * 16 + kPartRows * sizeof(CaPart) + 4096;
     });
 }

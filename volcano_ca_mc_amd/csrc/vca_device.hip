@@ -414,6 +414,7 @@ __global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B) {
     bool act = false, lava = false;
     int shape = -1;
     uint32_t entry = 0;
+    uint32_t step = 0;
     if (g < B.ntiles) {
         const uint32_t slot = B.owner[g];
         if (slot != kNoSlot && g_sims[B.sim_base + slot].running && !g_sims[B.sim_base + slot].D.ctrl->stop) {
@@ -424,6 +425,7 @@ __global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B) {
             lava = !P.classic && (f & kFlagLava);
             shape = g_sims[B.sim_base + slot].shape;
             entry = (slot << kTileBits) | t;
+            step = static_cast<uint32_t>(g_sims[B.sim_base + slot].D.ctrl->steps);
             if (P.dense || lava || (P.classic && (f & kFlagLava) && (f & kFlagAir))) {
                 act = true;
             } else if (f & kFlagAir) {
@@ -452,7 +454,7 @@ __global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B) {
         const bool a = act && shape == k;
         if (__ballot(a)) {
             const uint32_t slot = wave_append(a, &B.ctrl->nactive[k]);
-            if (a) B.active[k * B.cap + slot] = entry;
+            if (a) B.active[k * B.cap + slot] = (static_cast<uint64_t>(step) << 32) | entry;
         }
         const bool l = lava && shape == k;
         if (__ballot(l)) {
@@ -501,12 +503,13 @@ struct HaloTypes {
     uint8_t edge[2 * Halo<TY, TZ>::R];
 };
 
-// Stages the types of the tile at o and its halo, row by row, with the row lava masks.  All loads of a
-// thread are issued before its first LDS store, so a tile costs one memory latency.  Returns whether this
-// thread saw lava; own_air |= this thread saw AIR among the tile's own cells.
+// Stages the types of the tile at o and its halo, row by row, with the row lava masks, and (with_spill)
+// the spill heights of the LX x LY halo columns (0 where a column holds no cells).  Every load of a thread
+// is issued before its first LDS store, so a tile costs one memory latency.  Returns whether this thread saw
+// lava; own_air |= this thread saw AIR among the tile's own cells.
 template <int TY, int TZ>
 __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileCoord& o, HaloTypes<TY, TZ>& S,
-                                  int& own_air) {
+                                  bool with_spill, int32_t* s_spill, int& own_air) {
     using H = Halo<TY, TZ>;
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     const int lx = static_cast<int>(threadIdx.x & 63u);
@@ -528,6 +531,19 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
         const int ex = halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1);
         if (base != kNoCell && ex >= 0) e = D.type[base + static_cast<uint32_t>(ex)];
     }
+    int32_t sp[H::KC];
+    if (with_spill) {
+#pragma unroll
+        for (int k = 0; k < H::KC; ++k) {
+            const int q = t + k * kBlock;
+            sp[k] = 0;
+            if (q < H::LXY) {
+                int sx = o.x0 - 1 + q % H::LX, sy = o.y0 - 1 + q / H::LX;
+                if (sx <= P.nx && sy <= P.ny && wrap_axis(sx, P.nx, P.per[0]) && wrap_axis(sy, P.ny, P.per[1]))
+                    sp[k] = D.spill[static_cast<uint32_t>(sx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(sy)];
+            }
+        }
+    }
     bool lava = false;
 #pragma unroll
     for (int k = 0; k < H::KR; ++k) {
@@ -546,29 +562,14 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
         S.edge[t] = e == kLava ? 1 : 0;
         lava = lava || e == kLava;
     }
-    return lava;
-}
-
-// Spill heights of the LX x LY halo columns into LDS (0 where the column holds no cells).
-template <int TY, int TZ>
-__device__ inline void stage_spill(const CaParams& P, const CaDev& D, const TileCoord& o, int32_t* s_spill) {
-    using H = Halo<TY, TZ>;
-    int32_t v[H::KC];
+    if (with_spill) {
 #pragma unroll
-    for (int k = 0; k < H::KC; ++k) {
-        const int q = static_cast<int>(threadIdx.x) + k * kBlock;
-        v[k] = 0;
-        if (q < H::LXY) {
-            int gx = o.x0 - 1 + q % H::LX, gy = o.y0 - 1 + q / H::LX;
-            if (gx <= P.nx && gy <= P.ny && wrap_axis(gx, P.nx, P.per[0]) && wrap_axis(gy, P.ny, P.per[1]))
-                v[k] = D.spill[static_cast<uint32_t>(gx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(gy)];
+        for (int k = 0; k < H::KC; ++k) {
+            const int q = t + k * kBlock;
+            if (q < H::LXY) s_spill[q] = sp[k];
         }
     }
-#pragma unroll
-    for (int k = 0; k < H::KC; ++k) {
-        const int q = static_cast<int>(threadIdx.x) + k * kBlock;
-        if (q < H::LXY) s_spill[q] = v[k];
-    }
+    return lava;
 }
 
 // dropDepth (LavaCA.hpp:404-414) of the AIR cell at halo position c, which sits in halo layer lz + 1 of the
@@ -615,8 +616,7 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
         const TileCoord o = tile_origin(P, e & kTileMask);
         __syncthreads();  // the previous tile's LDS reads are done
         int own_air = 0;
-        stage_rows<TY, TZ>(P, D, o, S, own_air);
-        if (has_spill) stage_spill<TY, TZ>(P, D, o, s_spill);
+        stage_rows<TY, TZ>(P, D, o, S, has_spill, s_spill, own_air);
         if (threadIdx.x == 0) s_n = 0;
         __syncthreads();
         {
@@ -743,22 +743,29 @@ __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY
 // all independent coalesced loads), so an AIR cell folds its 26 neighbour types into a bitmask without
 // branches and gathers its candidates from LDS.  Cells whose vote overflows the four register slots are
 // handed to ca_wide.
+// Register budget for five waves per SIMD (five blocks per CU; the LDS tile allows five too): more tiles in
+// flight hide the staging latency better than the few registers the limit costs.
 template <int TY, int TZ>
-__global__ void __launch_bounds__(kBlock) ca_rules_kernel(BatchDev B, int shape) {
+__global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int shape) {
     using H = Halo<TY, TZ>;
     constexpr int LX = H::LX, LY = H::LY;
     __shared__ RulesTile<TY, TZ> T;
     __shared__ uint32_t s_nchg, s_nlist, s_nwide;
-    __shared__ unsigned long long s_red[3][kBlock / 64];
+    __shared__ uint32_t s_red[2][kBlock / 64];
+    __shared__ uint32_t s_wflags[2];  // per-wave flag bytes: [0] staging, [1] end of tile
+    const uint64_t* tiles = B.active + shape * B.cap;
+    // the first entry is read together with the list length (blockIdx.x < gridDim.x <= cap: in bounds)
+    uint64_t next = tiles[blockIdx.x];
     const uint32_t nact = B.ctrl->nactive[shape];
-    const uint32_t* tiles = B.active + shape * B.cap;
     for (uint32_t s = blockIdx.x; s < nact; s += gridDim.x) {
-        const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[s]);  // uniform: one entry per block
+        // uniform: one entry per block; low word slot << kTileBits | tile, high word the step
+        const uint32_t e = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(next));
+        const int32_t t = static_cast<int32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(next >> 32)));
+        if (s + gridDim.x < nact) next = tiles[s + gridDim.x];
         const CaSim& sim = g_sims[B.sim_base + (e >> kTileBits)];
         const CaParams& P = sim.P;
         const CaDev& D = sim.D;
         const uint32_t tile = e & kTileMask;
-        const int32_t t = D.ctrl->steps;
         const uint64_t prefix = hash_prefix(P.seed, static_cast<uint64_t>(t));
         const bool erupting = t < P.eruption_steps;
         const uint16_t stamp = static_cast<uint16_t>(t < 65535 ? t : 65535);
@@ -768,20 +775,27 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(BatchDev B, int shape)
         const uint32_t rules_mask =
             P.classic ? P.nb.any_mask
                       : ((P.nb.any_mask & ~0x1FFu) | (P.ponding ? (P.nb.any_mask & (1u << 4)) : 0u));
-        unsigned long long lava_after = 0, born = 0, changed = 0;
+        uint32_t w_lava = 0, w_born = 0;  // wave-uniform counts: lava cells after the step, born cells
         const TileCoord o = tile_origin(P, tile);
         __syncthreads();  // the previous tile's LDS reads are done
         int own_air = 0;
-        const bool saw_lava = stage_rows<TY, TZ>(P, D, o, T.S, own_air);
-        if (has_spill) stage_spill<TY, TZ>(P, D, o, T.spill);
+        const bool saw_lava = stage_rows<TY, TZ>(P, D, o, T.S, has_spill, T.spill, own_air);
         if (threadIdx.x == 0) {
             s_nchg = 0;
             s_nlist = 0;
             s_nwide = 0;
         }
-        if (!__syncthreads_or(saw_lava)) {
+        {
+            // block OR of saw_lava (bit 0) and own_air (bit 1): one flag byte per wave
+            const uint32_t f = (__ballot(saw_lava) ? 1u : 0u) | (__ballot(own_air) ? 2u : 0u);
+            if ((threadIdx.x & 63u) == 0)
+                reinterpret_cast<uint8_t*>(&s_wflags[0])[threadIdx.x >> 6] = static_cast<uint8_t>(f);
+        }
+        __syncthreads();
+        const uint32_t stage_flags = s_wflags[0];
+        if (!(stage_flags & 0x01010101u)) {
             // no lava in the tile or its halo: no cell changes; the tile keeps AIR iff it has AIR
-            const int any_air = __syncthreads_or(own_air);
+            const int any_air = (stage_flags & 0x02020202u) != 0;
             if (threadIdx.x == 0) {
                 D.tile_flags[tile] = static_cast<uint8_t>(any_air ? kFlagAir : 0);
                 D.tile_nchg[tile] = 0;
@@ -885,7 +899,7 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(BatchDev B, int shape)
             const int ly = row % TY, lz = row / TY;
             const int x = o.x0 + lx, y = o.y0 + ly, z = o.z0 + lz;
             const int c = (lx + 1) + LX * ((ly + 1) + LY * (lz + 1));
-            bool want = false, wide = false;
+            bool want = false, wide = false, stay = false, newborn = false;
             Change ch;
             uint32_t i = 0;
             if (j < nlist) {
@@ -907,14 +921,12 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(BatchDev B, int shape)
                         if (tc < P.solidus) {
                             want = true;
                             ch = Change{i, f, tc, stamp, kBasalt, kKindSolidify};
-                            ++changed;
                         } else {
-                            ++lava_after;
+                            stay = true;
                             tile_lava = 1;
                             if (tc != before) {
                                 want = true;
                                 ch = Change{i, f, tc, 0, kLava, kKindTemp};
-                                ++changed;
                             }
                         }
                     }
@@ -964,15 +976,15 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(BatchDev B, int shape)
                     } else if (has && (P.classic || wt >= P.solidus)) {
                         want = true;
                         ch = Change{i, wf, P.classic ? 0.0f : wt, stamp, kLava, kKindBorn};
-                        ++lava_after;
-                        ++born;
-                        ++changed;
+                        newborn = true;
                         tile_lava = 1;
                     } else {
                         tile_air = 1;
                     }
                 }
             }
+            w_lava += static_cast<uint32_t>(__popcll(__ballot(stay || newborn)));
+            w_born += static_cast<uint32_t>(__popcll(__ballot(newborn)));
             Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
             const uint32_t slot = wave_append(want, &s_nchg);
             if (want) rec[slot] = ch;
@@ -983,32 +995,34 @@ __global__ void __launch_bounds__(kBlock) ca_rules_kernel(BatchDev B, int shape)
                 if (wide) rec[kTileCells - 1 - wslot] = Change{i, 0, 0.0f, 0, 0, kKindWide};
             }
         }
-        // tile flags, record counts and the tile's counters (row tile % kPartRows of its simulation)
-        lava_after = wave_sum(lava_after);
-        born = wave_sum(born);
-        changed = wave_sum(changed);
-        if ((threadIdx.x & 63u) == 0) {
-            s_red[0][threadIdx.x >> 6] = lava_after;
-            s_red[1][threadIdx.x >> 6] = born;
-            s_red[2][threadIdx.x >> 6] = changed;
+        // tile flags, record counts and the tile's counters (row tile % kPartRows of its simulation); every
+        // record is one changed cell
+        {
+            const uint32_t f = (__ballot(tile_lava) ? static_cast<uint32_t>(kFlagLava) : 0u) |
+                               (__ballot(tile_air) ? static_cast<uint32_t>(kFlagAir) : 0u);
+            if ((threadIdx.x & 63u) == 0) {
+                s_red[0][threadIdx.x >> 6] = w_lava;
+                s_red[1][threadIdx.x >> 6] = w_born;
+                reinterpret_cast<uint8_t*>(&s_wflags[1])[threadIdx.x >> 6] = static_cast<uint8_t>(f);
+            }
         }
-        const int any_lava = __syncthreads_or(tile_lava);
-        const int any_air = __syncthreads_or(tile_air);
+        __syncthreads();
         if (threadIdx.x == 0) {
-            D.tile_flags[tile] = static_cast<uint8_t>((any_lava ? kFlagLava : 0) | (any_air ? kFlagAir : 0));
-            D.tile_nchg[tile] = s_nchg;
+            const uint32_t wf = s_wflags[1];
+            D.tile_flags[tile] = static_cast<uint8_t>((wf | wf >> 8 | wf >> 16 | wf >> 24) & 0xFFu);
+            const uint32_t nchg = s_nchg;
+            D.tile_nchg[tile] = nchg;
             D.tile_nwide[tile] = s_nwide;
             if (s_nwide) B.wide[atomicAdd(&B.ctrl->nwide, 1u)] = e;
-            unsigned long long r0 = 0, r1 = 0, r2 = 0;
+            uint32_t r0 = 0, r1 = 0;
             for (int w = 0; w < kBlock / 64; ++w) {
                 r0 += s_red[0][w];
                 r1 += s_red[1][w];
-                r2 += s_red[2][w];
             }
             CaPart* row = &D.part[tile % kPartRows];
-            if (r0) atomicAdd(&row->lava_after, r0);
-            if (r1) atomicAdd(&row->born, r1);
-            if (r2) atomicAdd(&row->changed, r2);
+            if (r0) atomicAdd(&row->lava_after, static_cast<unsigned long long>(r0));
+            if (r1) atomicAdd(&row->born, static_cast<unsigned long long>(r1));
+            if (nchg) atomicAdd(&row->changed, static_cast<unsigned long long>(nchg));
             atomicAdd(&row->tiles, 1ULL);
         }
     }
@@ -1074,9 +1088,10 @@ __global__ void __launch_bounds__(kBlock) ca_apply_kernel(BatchDev B) {
 #pragma unroll 1
     for (int k = 0; k < kShapes; ++k) {
         const uint32_t nact = B.ctrl->nactive[k];
-        const uint32_t* tiles = B.active + k * B.cap;
+        const uint64_t* tiles = B.active + k * B.cap;
         for (uint32_t s = blockIdx.x; s < nact; s += gridDim.x) {
-            const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[s]);  // uniform: one entry per block
+            // uniform: one entry per block
+            const uint32_t e = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(tiles[s]));
             const CaDev& D = g_sims[B.sim_base + (e >> kTileBits)].D;
             const uint32_t tile = e & kTileMask;
             const uint32_t n = D.tile_nchg[tile];

@@ -140,7 +140,8 @@ struct CaSim {
 };
 
 // A batch: simulations in slots, their tiles in one global tile range.  Work-list entries are
-// slot << kTileBits | tile (tile local to the simulation).
+// slot << kTileBits | tile (tile local to the simulation); the visited-tile lists carry the simulation's step
+// count in the high word as well, so the rules kernel reads both with one load.
 constexpr int kTileBits = 22;
 constexpr uint32_t kTileMask = (1u << kTileBits) - 1u;
 constexpr int kMaxSlots = 1024;
@@ -164,7 +165,7 @@ struct BatchDev {
     uint32_t ntiles;              // global tiles in use (high-water mark)
     uint32_t nslots;
     uint32_t cap;                 // capacity of each list
-    uint32_t* active;             // kShapes lists of cap entries
+    uint64_t* active;             // kShapes lists of cap entries: step << 32 | entry
     uint32_t* lava;               // kShapes lists of cap entries
     uint32_t* wide;               // cap entries
     BatchCtrl* ctrl;
