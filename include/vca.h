@@ -124,6 +124,18 @@ typedef struct {
 } vca_ca_result;
 int vca_ca_collect(vca_ca* ca, vca_ca_result* r);
 
+/* flowSums of the final state (Simulation.cpp:408-466) on the device: for flowId f = 1..nflows, lava[f] / basalt[f]
+ * = LAVA / BASALT cells of the flow and maxd2[f] = the largest (x - vx)^2 + (y - vy)^2 over them, (vx, vy) =
+ * vents[f-1] (arrays of nflows + 1 entries; entry 0 and flows without cells are 0).  The reach is hypot of the
+ * farthest cell: every cell at that squared distance has the same true distance, and vca_ca_flow_points tells
+ * which lattice points of the circle occur when the host libm's hypot differs between them.  Like vca_ca_collect:
+ * the simulation must have stopped and must not be in a batch; reads on a stream of its own. */
+int vca_ca_flow_stats(vca_ca* ca, const vca_vent* vents, int32_t nflows, uint64_t* lava, uint64_t* basalt,
+                      uint64_t* maxd2);
+/* seen[a] = 1 when a LAVA / BASALT cell of flow `flow` lies at |x - vent.x| = a, (x - vent.x)^2 + (y - vent.y)^2
+ * = d2 (a < cap). */
+int vca_ca_flow_points(vca_ca* ca, const vca_vent* vent, int32_t flow, uint64_t d2, uint8_t* seen, int32_t cap);
+
 /* Device memory a simulation of these dims takes (state, change slots, tile lists), for admission control. */
 int vca_ca_footprint(const vca_ca_desc* desc, uint64_t* bytes);
 /* Tiles of the simulation (its share of a batch's tile range). */
@@ -187,8 +199,9 @@ typedef struct {
  * 53-bit integer thresholds on the host, so the device never evaluates exp. */
 int vca_mc_create(vca_ctx* ctx, const vca_mc_desc* desc, const uint8_t* domain, const float* mobility,
                   int32_t mobility_dynamics, vca_mc** out);
-/* Same, but the basalt domain and the cooling mobility (CoolingMobility.cpp:8-129) are derived from the
- * final CA state held by `ca` (same dims), without the caller downloading it. */
+/* Same, but the basalt domain and the cooling mobility (CoolingMobility.cpp:8-129) are derived on the device from
+ * the final CA state held by `ca` (same dims and device, stopped, not in a batch), without the caller downloading
+ * it.  ctx may be another context than the CA's. */
 int vca_mc_create_from_ca(vca_ctx* ctx, const vca_mc_desc* desc, vca_ca* ca, int32_t use_mobility,
                           double mobility_min, vca_mc** out);
 int vca_mc_destroy(vca_mc* mc);

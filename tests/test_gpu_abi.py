@@ -169,3 +169,58 @@ def test_mc_from_ca_matches_host_domain(ctx):
     assert np.array_equal(a.crystals(), b.crystals())
     assert a.counters() == b.counters()
     assert a.measure() == b.measure()
+
+
+def _flow_sums_numpy(st, dims, vents):
+    """flowSums (Simulation.cpp:408-466) restated with numpy: per flowId lava / basalt cells and max squared
+    horizontal vent distance."""
+    nx, ny, nz = dims
+    n = len(vents)
+    t, f = st["type"], st["flow"]
+    i = np.arange(t.size, dtype=np.int64)
+    x, y = i % nx, (i // nx) % ny
+    ok = ((t == 2) | (t == 3)) & (f >= 1) & (f <= n)
+    la, ba, d2 = np.zeros(n + 1, np.uint64), np.zeros(n + 1, np.uint64), np.zeros(n + 1, np.uint64)
+    if n == 0:
+        return la, ba, d2, x, y, ok
+    fi = f[ok].astype(np.int64)
+    la += np.bincount(fi[t[ok] == 2], minlength=n + 1).astype(np.uint64)
+    ba += np.bincount(fi[t[ok] == 3], minlength=n + 1).astype(np.uint64)
+    v = np.asarray(vents, dtype=np.int64)
+    dd = (x[ok] - v[fi - 1, 0]) ** 2 + (y[ok] - v[fi - 1, 1]) ** 2
+    np.maximum.at(d2, fi, dd.astype(np.uint64))
+    return la, ba, d2, x, y, ok
+
+
+def _lava_cases():
+    return [c for c in _ca_cases() if _inputs(c)[1]["ca"]["mode"] == "lava"]
+
+
+@pytest.mark.parametrize("name", _lava_cases())
+def test_flow_stats_on_device_match_host_sums(ctx, name):
+    """vca_ca_flow_stats / vca_ca_flow_points against the same sums over the downloaded final state."""
+    d, cfg = _inputs(name)
+    print(f"case {name}: dims {cfg['space']}", flush=True)
+    hm = O.terrain_for(cfg, d)
+    per = cfg["boundary"] == "periodic"
+    spill = O.fill_depressions(hm, per, per)
+    vents = O.vents_for(cfg, hm)
+    gpu_ca = _gpu_ca(ctx, cfg, hm, spill, vents)
+    gpu_ca.run()
+    st = gpu_ca.result()
+    vl = vents.tolist()
+    la, ba, d2 = gpu_ca.flow_stats(vl)
+    hla, hba, hd2, x, y, ok = _flow_sums_numpy(st, gpu_ca.dims, vl)
+    assert np.array_equal(la, hla) and np.array_equal(ba, hba) and np.array_equal(d2, hd2), name
+    for f in range(1, len(vl) + 1):
+        if la[f] + ba[f] == 0:
+            continue
+        cap = int(np.sqrt(float(d2[f]))) + 2
+        seen = gpu_ca.flow_points(vl[f - 1], f, int(d2[f]), cap)
+        sel = ok & (st["flow"] == f)
+        dx = x[sel] - vl[f - 1][0]
+        dy = y[sel] - vl[f - 1][1]
+        far = np.unique(np.abs(dx[dx * dx + dy * dy == int(d2[f])]))
+        expect = np.zeros(cap, np.uint8)
+        expect[far] = 1
+        assert np.array_equal(seen, expect), (name, f)

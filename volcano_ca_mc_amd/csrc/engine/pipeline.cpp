@@ -202,15 +202,16 @@ private:
                     if (j.stage == Stage::Writing) write_async(j);
                     break;
                 case Stage::PostCa:
-                    vca_ca_destroy(j.ca);  // the batch's stream is idle here (between chunks)
-                    j.ca = nullptr;
-                    used_ -= j.fp_ca;
+                    // the MC takes its domain from the simulation's final state on the device: the simulation is
+                    // freed once the MC stage is done
+                    if (!j.cfg.mc.enabled) release_ca(j);
                     j.stage = j.cfg.mc.enabled ? Stage::McReady : Stage::Writing;
                     if (j.stage == Stage::Writing) write_async(j);
                     break;
                 case Stage::Mc:
                     mc_busy_[static_cast<size_t>(j.mc_stream)] = 0;
                     used_ -= j.fp_mc;
+                    release_ca(j);
                     j.stage = Stage::Done;
                     j.latency_ms = since_arrival(j);
                     break;
@@ -222,6 +223,14 @@ private:
                     break;
             }
         }
+    }
+
+    // on the GPU thread, between chunks
+    void release_ca(Job& j) {
+        if (!j.ca) return;
+        vca_ca_destroy(j.ca);
+        j.ca = nullptr;
+        used_ -= j.fp_ca;
     }
 
     void fail(Job& j, const std::string& what) {
@@ -291,7 +300,6 @@ private:
             if (j.stage != Stage::Ca || st[2 * static_cast<size_t>(j.slot) + 1] == VCA_RUNNING) continue;
             try {
                 j.timing.add("caSim", j.ca_clock.ms());
-                const int32_t steps = st[2 * static_cast<size_t>(j.slot)];
                 Clock t;
                 vca_check(vca_batch_remove(batch_, j.ca), "CA");
                 j.slot = -1;
@@ -299,20 +307,18 @@ private:
                 // the final state is read on the simulation's own stream while the batch goes on; the GPU thread
                 // frees the simulation when this stage is done
                 j.stage = Stage::PostCa;
-                j.work = std::async(std::launch::async, [&j, steps] {
+                j.work = std::async(std::launch::async, [&j] {
                     Clock ts;
-                    ca_collect_stopped(j.ca, j.cfg, j.s, steps);
+                    ca_collect(j.ca, j.cfg, j.s, ca_state_needed(j.cfg));
                     save_ca(j.cfg, j.s, j.out.string(), j.timing);
                     j.timing.add("save", ts.ms());
                     ts.reset();
+                    ca_flow_sums(j.ca, j.s);
                     j.stats = ca_stats(j.cfg, j.s);
                     j.timing.add("caStats", ts.ms());
                     j.stats["parallel"] = parallel_json(j.cfg);
-                    if (j.cfg.mc.enabled) {
-                        Clock tp;
-                        j.prep = mc_prepare(j.cfg, &j.s.result, Cooling::Full);
-                        j.prep_ms = tp.ms();
-                    }
+                    j.s.result = CaState{};  // written; the MC reads the state on the device
+                    if (j.cfg.mc.enabled) j.prep = mc_prepare_from_ca(j.cfg, j.ca);
                 });
             } catch (const std::exception& e) {
                 fail(j, e.what());

@@ -149,19 +149,25 @@ Json ca_stats(const Config& c, const RunSummary& s) {
     const size_t flows = s.vents.size();
     std::vector<uint64_t> lava(flows + 1, 0), basalt(flows + 1, 0);
     std::vector<double> reach(flows + 1, 0.0);
-    const Dims& d = s.result.dims;
-    size_t i = 0;
-    for (int z = 0; z < d.nz; ++z)
-        for (int y = 0; y < d.ny; ++y)
-            for (int x = 0; x < d.nx; ++x, ++i) {
-                const uint8_t t = s.result.type[i];
-                const int32_t f = s.result.flow[i];
-                if ((t != VCA_LAVA && t != VCA_BASALT) || f <= 0 || static_cast<size_t>(f) > flows) continue;
-                const size_t fi = static_cast<size_t>(f);
-                (t == VCA_LAVA ? lava : basalt)[fi] += 1;
-                const Vent& v = s.vents[fi - 1];
-                reach[fi] = std::max(reach[fi], std::hypot(double(x - v.x), double(y - v.y)));
-            }
+    if (s.flows_ready) {
+        lava = s.flow_lava;
+        basalt = s.flow_basalt;
+        reach = s.flow_reach;
+    } else {
+        const Dims& d = s.result.dims;
+        size_t i = 0;
+        for (int z = 0; z < d.nz; ++z)
+            for (int y = 0; y < d.ny; ++y)
+                for (int x = 0; x < d.nx; ++x, ++i) {
+                    const uint8_t t = s.result.type[i];
+                    const int32_t f = s.result.flow[i];
+                    if ((t != VCA_LAVA && t != VCA_BASALT) || f <= 0 || static_cast<size_t>(f) > flows) continue;
+                    const size_t fi = static_cast<size_t>(f);
+                    (t == VCA_LAVA ? lava : basalt)[fi] += 1;
+                    const Vent& v = s.vents[fi - 1];
+                    reach[fi] = std::max(reach[fi], std::hypot(double(x - v.x), double(y - v.y)));
+                }
+    }
     Json j = Json::object();
     j["mode"] = c.ca.classic ? "classic" : "lava";
     j["steps"] = s.steps;
@@ -258,22 +264,29 @@ vca_ca* create_ca(const Config& cfg, const CaInputs& in, Gpu& gpu) {
     return h;
 }
 
-void ca_collect(vca_ca* ca, const Config& cfg, RunSummary& s) {
-    int32_t steps = 0, stop = 0;
-    vca_check(vca_ca_status(ca, &steps, &stop), "CA");
-    s.active.assign(static_cast<size_t>(steps), 0);
-    s.changed.assign(static_cast<size_t>(steps), 0);
-    s.result.resize(cfg.space);
+bool ca_state_needed(const Config& cfg) { return cfg.output.format != OutputFormat::None; }
+
+void ca_collect(vca_ca* ca, const Config& cfg, RunSummary& s, bool with_state) {
     vca_ca_result r{};
-    r.type = reinterpret_cast<uint8_t*>(s.result.type.data());
-    r.flow = s.result.flow.data();
-    r.temp = s.result.temp.data();
-    r.solid = s.result.solid.data();
-    r.arrival = s.result.arrival.data();
+    // histories of at most maxSteps entries (no status read on the context's stream, which may be busy with a
+    // batch)
+    const int32_t cap = std::max(1, cfg.ca.max_steps);
+    s.active.assign(static_cast<size_t>(cap), 0);
+    s.changed.assign(static_cast<size_t>(cap), 0);
+    if (with_state) {
+        s.result.resize(cfg.space);
+        r.type = reinterpret_cast<uint8_t*>(s.result.type.data());
+        r.flow = s.result.flow.data();
+        r.temp = s.result.temp.data();
+        r.solid = s.result.solid.data();
+        r.arrival = s.result.arrival.data();
+    }
     r.active = s.active.data();
     r.changed = s.changed.data();
-    r.cap = steps;
+    r.cap = cap;
     vca_check(vca_ca_collect(ca, &r), "CA");
+    s.active.resize(static_cast<size_t>(std::min(r.steps, cap)));
+    s.changed.resize(static_cast<size_t>(std::min(r.steps, cap)));
     s.steps = r.steps;
     s.stop = r.stop_reason;
     s.tiles_visited = r.tiles_visited;
@@ -281,25 +294,54 @@ void ca_collect(vca_ca* ca, const Config& cfg, RunSummary& s) {
     s.tile_cells = 1024;
 }
 
-void ca_collect_stopped(vca_ca* ca, const Config& cfg, RunSummary& s, int32_t steps) {
-    s.active.assign(static_cast<size_t>(steps), 0);
-    s.changed.assign(static_cast<size_t>(steps), 0);
-    s.result.resize(cfg.space);
-    vca_ca_result r{};
-    r.type = reinterpret_cast<uint8_t*>(s.result.type.data());
-    r.flow = s.result.flow.data();
-    r.temp = s.result.temp.data();
-    r.solid = s.result.solid.data();
-    r.arrival = s.result.arrival.data();
-    r.active = s.active.data();
-    r.changed = s.changed.data();
-    r.cap = steps;
-    vca_check(vca_ca_collect(ca, &r), "CA");
-    s.steps = r.steps;
-    s.stop = r.stop_reason;
-    s.tiles_visited = r.tiles_visited;
-    s.changes = r.changes;
-    s.tile_cells = 1024;
+namespace {
+uint64_t isqrt(uint64_t v) {
+    uint64_t r = static_cast<uint64_t>(std::sqrt(static_cast<double>(v)));
+    while (r > 0 && r * r > v) --r;
+    while ((r + 1) * (r + 1) <= v) ++r;
+    return r;
+}
+
+// max hypot(x - vx, y - vy) over the flow's cells, from their largest squared distance d2: the lattice points
+// (a, b), a^2 + b^2 = d2, all lie at the same true distance; the host libm's hypot may still differ between
+// them in the last place, and then the device says which of them the flow reaches.
+double exact_reach(vca_ca* ca, const Vent& v, int32_t flow, uint64_t d2) {
+    std::vector<std::pair<uint64_t, uint64_t>> pts;
+    const uint64_t r = isqrt(d2);
+    for (uint64_t a = 0; a <= r; ++a) {
+        const uint64_t rest = d2 - a * a, b = isqrt(rest);
+        if (b * b == rest) pts.emplace_back(a, b);
+    }
+    if (pts.empty()) return 0.0;
+    const double h0 = std::hypot(double(pts[0].first), double(pts[0].second));
+    bool same = true;
+    for (const auto& p : pts) same = same && std::hypot(double(p.first), double(p.second)) == h0;
+    if (same) return h0;
+    std::vector<uint8_t> seen(static_cast<size_t>(r) + 1, 0);
+    const vca_vent vv{v.x, v.y, v.z, flow};
+    vca_check(vca_ca_flow_points(ca, &vv, flow, d2, seen.data(), static_cast<int32_t>(seen.size())), "CA");
+    double best = 0.0;
+    for (const auto& p : pts)
+        if (seen[static_cast<size_t>(p.first)]) best = std::max(best, std::hypot(double(p.first), double(p.second)));
+    return best;
+}
+}  // namespace
+
+void ca_flow_sums(vca_ca* ca, RunSummary& s) {
+    const size_t flows = s.vents.size();
+    std::vector<vca_vent> vv(flows);
+    for (size_t f = 0; f < flows; ++f) vv[f] = vca_vent{s.vents[f].x, s.vents[f].y, s.vents[f].z, int32_t(f + 1)};
+    s.flow_lava.assign(flows + 1, 0);
+    s.flow_basalt.assign(flows + 1, 0);
+    s.flow_reach.assign(flows + 1, 0.0);
+    std::vector<uint64_t> d2(flows + 1, 0);
+    vca_check(vca_ca_flow_stats(ca, vv.data(), static_cast<int32_t>(flows), s.flow_lava.data(), s.flow_basalt.data(),
+                                d2.data()),
+              "CA");
+    for (size_t f = 1; f <= flows; ++f)
+        if (s.flow_lava[f] + s.flow_basalt[f] > 0)
+            s.flow_reach[f] = exact_reach(ca, s.vents[f - 1], static_cast<int32_t>(f), d2[f]);
+    s.flows_ready = true;
 }
 
 void save_ca(const Config& cfg, const RunSummary& s, const std::string& out_dir, Timing& timing) {
@@ -337,13 +379,13 @@ RunSummary run_simulation(const Config& cfg, const std::string& out_dir, Timing&
     RunSummary s;
     const fs::path out(out_dir);
     write_config_used(cfg, out_dir, timing);
+    CaHandle ca;  // kept until the MC has taken its domain from the final state
     if (cfg.mc.full_only()) {
         s.ca_ran = false;
     } else {
         const CaInputs in = ca_inputs(cfg);
         s.vents = in.vents;
         Clock t;
-        CaHandle ca;
         ca.h = create_ca(cfg, in, gpu);
         timing.add("allocate", t.ms());
         timing.add("terrain", in.terrain_ms);
@@ -398,16 +440,19 @@ RunSummary run_simulation(const Config& cfg, const std::string& out_dir, Timing&
         if (progress.enabled()) progress.report("save", 0, 1, true);
 
         t.reset();
-        ca_collect(ca.h, cfg, s);
+        ca_collect(ca.h, cfg, s, ca_state_needed(cfg));
         save_ca(cfg, s, out_dir, timing);
         timing.add("save", t.ms());
     }
     Clock ts;
+    if (s.ca_ran) ca_flow_sums(ca.h, s);
     Json stats = s.ca_ran ? ca_stats(cfg, s) : Json::object();
     if (s.ca_ran) timing.add("caStats", ts.ms());
     stats["parallel"] = parallel_json(cfg);
     if (cfg.mc.enabled) {
-        s.mc = run_mc(cfg, s.ca_ran ? &s.result : nullptr, Cooling::Full, timing, progress, out_dir, gpu);
+        Clock tp;
+        McPrep prep = s.ca_ran ? mc_prepare_from_ca(cfg, ca.h) : mc_prepare(cfg, nullptr, Cooling::Full);
+        s.mc = run_mc(cfg, std::move(prep), tp.ms(), timing, progress, out_dir, gpu);
         s.mc_ran = true;
         if (progress.enabled()) progress.report("save", 0, 1, true);
         save_mc(cfg, s.mc, out_dir, timing);
@@ -434,6 +479,16 @@ Json class_json(uint64_t cells, const ClassStat& a, const ClassStat& b) {
 }
 
 }  // namespace
+
+McPrep mc_prepare_from_ca(const Config& cfg, vca_ca* ca) {
+    McPrep p = mc_prepare(cfg, nullptr, Cooling::Full);  // descriptor of the full domain ...
+    p.full = false;                                       // ... then the basalt domain's periodicity (x, y only)
+    p.desc.periodic[2] = 0;
+    p.from_ca = ca;
+    p.dynamics = cfg.mc.mobility_from_cooling;
+    p.mobility_used = cfg.mc.mobility_from_cooling;
+    return p;
+}
 
 McPrep mc_prepare(const Config& cfg, const CaState* ca, Cooling cooling) {
     McPrep p;
@@ -505,11 +560,16 @@ McRun::McRun(const Config& cfg, McPrep prep, Gpu& gpu) : cfg_(cfg) {
     s_.dims = prep.dims;
     s_.mobility_used = prep.mobility_used;
     s_.warnings = std::move(prep.warnings);
-    has_classes_ = !prep.mobility.empty();
+    has_classes_ = prep.has_mobility();
     Clock t;
-    vca_check(vca_mc_create(gpu.ctx(), &prep.desc, s_.full ? nullptr : prep.domain.data(),
-                            prep.mobility.empty() ? nullptr : prep.mobility.data(), prep.dynamics ? 1 : 0, &h_),
-              "MC");
+    if (prep.from_ca)
+        vca_check(vca_mc_create_from_ca(gpu.ctx(), &prep.desc, prep.from_ca, prep.dynamics ? 1 : 0,
+                                        cfg.mc.mobility_min, &h_),
+                  "MC");
+    else
+        vca_check(vca_mc_create(gpu.ctx(), &prep.desc, s_.full ? nullptr : prep.domain.data(),
+                                prep.mobility.empty() ? nullptr : prep.mobility.data(), prep.dynamics ? 1 : 0, &h_),
+                  "MC");
     uint64_t dom_cells = 0;
     vca_check(vca_mc_info(h_, &colors_, &dom_cells), "MC");
     s_.domain_cells = dom_cells;
@@ -629,7 +689,11 @@ McSummary run_mc(const Config& cfg, const CaState* ca, Cooling cooling, Timing& 
                  const std::string& out_dir, Gpu& gpu) {
     Clock tp;
     McPrep prep = mc_prepare(cfg, ca, cooling);
-    const double prep_ms = tp.ms();
+    return run_mc(cfg, std::move(prep), tp.ms(), timing, progress, out_dir, gpu);
+}
+
+McSummary run_mc(const Config& cfg, McPrep prep, double prep_ms, Timing& timing, Progress& progress,
+                 const std::string& out_dir, Gpu& gpu) {
     McRun run(cfg, std::move(prep), gpu);
     run.add_init_ms(prep_ms);
     const int snap_every = out_dir.empty() ? 0 : cfg.mc.snapshots_every;

@@ -114,7 +114,11 @@ struct RunSummary {
     std::vector<uint64_t> active, changed;
     uint64_t tiles_visited = 0, changes = 0;
     int32_t tile_cells = 0;
-    CaState result;
+    CaState result;               // the final state, when something on the host needs it (result files)
+    // flowSums of the final state (per flowId, entry 0 unused), computed on the device
+    bool flows_ready = false;
+    std::vector<uint64_t> flow_lava, flow_basalt;
+    std::vector<double> flow_reach;
     bool mc_ran = false;
     McSummary mc;
 };
@@ -127,10 +131,14 @@ struct McPrep {
     vca_mc_desc desc{};
     std::vector<uint8_t> domain;
     std::vector<float> mobility;
+    vca_ca* from_ca = nullptr;    // domain and cooling mobility taken on the device from this stopped CA
     bool dynamics = false, mobility_used = false;
     std::vector<std::string> warnings;
+    bool has_mobility() const { return from_ca != nullptr || !mobility.empty(); }
 };
 McPrep mc_prepare(const Config& cfg, const CaState* ca, Cooling cooling);
+// The MC of a run whose CA result is still on the device (Cooling::Full): nothing is downloaded.
+McPrep mc_prepare_from_ca(const Config& cfg, vca_ca* ca);
 
 struct ClassStat {
     uint64_t grains = 0, cells = 0, largest = 0;
@@ -178,10 +186,13 @@ struct CaInputs {
 vca_ca_desc make_ca_desc(const Config& cfg);
 CaInputs ca_inputs(const Config& cfg);
 vca_ca* create_ca(const Config& cfg, const CaInputs& in, Gpu& gpu);
-void ca_collect(vca_ca* ca, const Config& cfg, RunSummary& s);  // steps, histories, work, final state
-// Same for a simulation known to be stopped after `steps` steps and out of any batch: reads only on the
-// simulation's own stream (safe from a worker thread while the batch goes on).
-void ca_collect_stopped(vca_ca* ca, const Config& cfg, RunSummary& s, int32_t steps);
+// Whether anything on the host needs the final CA state (result files).
+bool ca_state_needed(const Config& cfg);
+// Steps, histories, work counters and (with_state) the final state of a stopped simulation that is out of any
+// batch; reads only on the simulation's own stream (safe from a worker thread while a batch goes on).
+void ca_collect(vca_ca* ca, const Config& cfg, RunSummary& s, bool with_state);
+// flowSums (Simulation.cpp:408-466) on the device into s.flow_*.
+void ca_flow_sums(vca_ca* ca, RunSummary& s);
 void save_ca(const Config& cfg, const RunSummary& s, const std::string& out_dir, Timing& timing);
 Json parallel_json(const Config& cfg);
 void write_config_used(const Config& cfg, const std::string& out_dir, Timing& timing);
@@ -190,6 +201,8 @@ void run_info(Timing& timing, const Config& cfg, const RunSummary& s, const Gpu&
 const char* stop_text(int stop);
 Json ca_stats(const Config& c, const RunSummary& s);
 McSummary run_mc(const Config& cfg, const CaState* ca, Cooling cooling, Timing& timing, Progress& progress,
+                 const std::string& out_dir, Gpu& gpu);
+McSummary run_mc(const Config& cfg, McPrep prep, double prep_ms, Timing& timing, Progress& progress,
                  const std::string& out_dir, Gpu& gpu);
 void save_mc(const Config& cfg, const McSummary& s, const std::string& out_dir, Timing& timing);
 RunSummary run_simulation(const Config& cfg, const std::string& out_dir, Timing& timing, Progress& progress, Gpu& gpu);

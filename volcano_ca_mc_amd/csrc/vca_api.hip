@@ -948,6 +948,84 @@ extern "C" int vca_ca_collect(vca_ca* ca, vca_ca_result* r) {
     });
 }
 
+namespace {
+// A pooled stream for reads of a stopped simulation outside its context's stream (the context may keep advancing
+// a batch meanwhile); synchronised and returned on scope exit.
+struct SideStream {
+    hipStream_t s = StreamPool::take();
+    SideStream() = default;
+    SideStream(const SideStream&) = delete;
+    SideStream& operator=(const SideStream&) = delete;
+    ~SideStream() {
+        (void)hipStreamSynchronize(s);
+        StreamPool::give(s);
+    }
+};
+
+void require_stopped(vca_ca* ca, hipStream_t s) {
+    if (ca->member) throw VcaError(VCA_ERR_STATE, "the simulation is in a batch: remove it first");
+    CaCtrl c;
+    check(hipMemcpyAsync(&c, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, s), "read ctrl");
+    check(hipStreamSynchronize(s), "read ctrl");
+    if (c.stop == VCA_RUNNING) throw VcaError(VCA_ERR_STATE, "the simulation is still running");
+}
+}  // namespace
+
+extern "C" int vca_ca_flow_stats(vca_ca* ca, const vca_vent* vents, int32_t nflows, uint64_t* lava, uint64_t* basalt,
+                                 uint64_t* maxd2) {
+    return guarded([&] {
+        require(ca && nflows >= 0, "invalid argument");
+        require(nflows == 0 || (vents && lava && basalt && maxd2), "NULL argument");
+        ca->ctx->set_device();
+        SideStream side;
+        require_stopped(ca, side.s);
+        const size_t m = static_cast<size_t>(nflows) + 1;
+        std::vector<int32_t> xy(2 * m, 0);
+        for (int32_t f = 0; f < nflows; ++f) {
+            xy[2 * static_cast<size_t>(f)] = vents[f].x;
+            xy[2 * static_cast<size_t>(f) + 1] = vents[f].y;
+        }
+        AllocStream alloc_on(side.s);
+        DevBuf dxy, out;
+        dxy.alloc(xy.size() * 4);
+        out.alloc(3 * m * 8);
+        check(hipMemcpyAsync(dxy.p, xy.data(), xy.size() * 4, hipMemcpyHostToDevice, side.s), "copy vents");
+        check(hipMemsetAsync(out.p, 0, 3 * m * 8, side.s), "memset");
+        unsigned long long* o = out.as<unsigned long long>();
+        if (nflows > 0) ca_launch_flow_sums(ca->P, ca->D, dxy.as<int32_t>(), nflows, o, o + m, o + 2 * m, side.s);
+        check(hipGetLastError(), "flow sums");
+        std::vector<uint64_t> h(3 * m);
+        check(hipMemcpyAsync(h.data(), out.p, 3 * m * 8, hipMemcpyDeviceToHost, side.s), "read flow sums");
+        check(hipStreamSynchronize(side.s), "flow sums");
+        for (size_t f = 0; f < m && nflows > 0; ++f) {
+            lava[f] = h[f];
+            basalt[f] = h[m + f];
+            maxd2[f] = h[2 * m + f];
+        }
+    });
+}
+
+extern "C" int vca_ca_flow_points(vca_ca* ca, const vca_vent* vent, int32_t flow, uint64_t d2, uint8_t* seen,
+                                  int32_t cap) {
+    return guarded([&] {
+        require(ca && vent && seen && cap > 0, "invalid argument");
+        ca->ctx->set_device();
+        SideStream side;
+        require_stopped(ca, side.s);
+        AllocStream alloc_on(side.s);
+        DevBuf buf;
+        buf.alloc(static_cast<size_t>(cap) * 4);
+        check(hipMemsetAsync(buf.p, 0, static_cast<size_t>(cap) * 4, side.s), "memset");
+        ca_launch_flow_points(ca->P, ca->D, flow, vent->x, vent->y, d2, buf.as<uint32_t>(), static_cast<uint32_t>(cap),
+                              side.s);
+        check(hipGetLastError(), "flow points");
+        std::vector<uint32_t> h(static_cast<size_t>(cap));
+        check(hipMemcpyAsync(h.data(), buf.p, h.size() * 4, hipMemcpyDeviceToHost, side.s), "read flow points");
+        check(hipStreamSynchronize(side.s), "flow points");
+        for (int32_t k = 0; k < cap; ++k) seen[k] = h[static_cast<size_t>(k)] ? 1 : 0;
+    });
+}
+
 static void ca_status_sync(vca_ca* ca, int32_t* steps, int32_t* stop) {
     CaCtrl c;
     check(hipMemcpyAsync(&c, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, ca->ctx->stream), "read ctrl");
@@ -1247,13 +1325,110 @@ Colouring make_colouring(int kind, const int dims[3], const int per[3]) {
     return c;
 }
 
-// Exact integer form of `toUniform(r) >= m` (skip) for a float mobility m: skip iff (r >> 11) >= thr.
-uint64_t mobility_threshold(float m) {
-    const double two53 = 9007199254740992.0;
-    if (std::isnan(m)) return 1ULL << 53;  // u >= NaN is false: never skipped
-    if (!(m > 0.0f)) return 0;             // u >= m for every u in [0, 1)
-    const double v = std::ceil(static_cast<double>(m) * two53);
-    return v >= two53 ? (1ULL << 53) : static_cast<uint64_t>(v);
+
+}  // namespace
+
+namespace {
+
+// PottsMC ctor + setDomain + setCheckerboard + initialize (PottsMC.cpp:14-143) with the domain given on the device
+// (L.dom: nothing, 0/1 bytes or the CA cell types) and the mobility, when there is one, already in mc->mob_cells.
+// The domain lists are built on the device: colour-major, global index ascending within a colour (buildColors),
+// or index order for the sequential random mode; one host read of the colour totals.
+std::unique_ptr<vca_mc> mc_build(vca_ctx* ctx, const vca_mc_desc* d, McListSrc L, std::unique_ptr<vca_mc> mc,
+                                 bool mobility_dynamics) {
+    hipStream_t s = ctx->stream;
+    McParams& P = mc->P;
+    const uint64_t n = mc->n;
+    const int dims[3] = {d->nx, d->ny, d->nz};
+    const Colouring col = make_colouring(d->neighborhood, dims, P.per);
+    L.n = n;
+    L.ncol = mc->checkerboard ? col.count : 1;
+    L.parity = col.parity ? 1 : 0;
+    for (int a = 0; a < 3; ++a) {
+        L.dims[a] = dims[a];
+        L.per[a] = P.per[a];
+    }
+    for (int k = 0; k < 27; ++k) L.index[k] = col.index[k];
+    mc->colors = mc->checkerboard ? col.count : 0;
+    const uint64_t nseg = mc_list_segments(n);
+    DevBuf cnt, totals;
+    cnt.alloc(static_cast<size_t>(L.ncol) * nseg * 4);
+    totals.alloc(static_cast<size_t>(L.ncol) * 4 * 2);
+    mc_launch_list_count(L, cnt.as<uint32_t>(), totals.as<uint32_t>(), s);
+    check(hipGetLastError(), "mc lists");
+    std::vector<uint32_t> tot(static_cast<size_t>(L.ncol));
+    check(hipMemcpyAsync(tot.data(), totals.p, tot.size() * 4, hipMemcpyDeviceToHost, s), "read colour totals");
+    check(hipStreamSynchronize(s), "mc lists");
+    mc->color_start.assign(1, 0);
+    uint64_t sum = 0;
+    for (uint32_t t : tot) {
+        sum += t;
+        mc->color_start.push_back(static_cast<uint32_t>(sum));
+    }
+    mc->ndomain = static_cast<uint32_t>(sum);
+    // first slot of each colour, after the totals in the same buffer
+    check(hipMemcpyAsync(totals.as<uint32_t>() + L.ncol, mc->color_start.data(), static_cast<size_t>(L.ncol) * 4,
+                         hipMemcpyHostToDevice, s),
+          "copy colour starts");
+    mc->list.alloc(static_cast<size_t>(mc->ndomain) * 4);
+    uint64_t* thr = nullptr;
+    if (mc->has_mob_cells && mobility_dynamics) {
+        mc->mob_thr.alloc(static_cast<size_t>(mc->ndomain) * 8);
+        thr = mc->mob_thr.as<uint64_t>();
+        mc->has_mob_dyn = true;
+    }
+    mc_launch_list_scatter(L, cnt.as<uint32_t>(), totals.as<uint32_t>() + L.ncol, mc->list.as<uint32_t>(),
+                           mc->has_mob_cells ? mc->mob_cells.as<float>() : nullptr, thr, s);
+    check(hipGetLastError(), "mc lists");
+    mc->spin.alloc(n * static_cast<uint64_t>(mc->spin_bytes));
+    check(hipMemsetAsync(mc->spin.p, 0, n * static_cast<uint64_t>(mc->spin_bytes), s), "memset");
+    // Metropolis table T[before][after] (PottsMC.cpp:431-436): dE = J*after - J*before in double;
+    // accept-always for dE <= 0, else (r >> 11) < ceil(exp(-dE/kT) * 2^53) (exp from the host libm).
+    std::vector<uint64_t> acc(27 * 27, 0);
+    for (int b = 0; b < 27; ++b)
+        for (int a = 0; a < 27; ++a) {
+            const double dE = d->J * a - d->J * b;
+            uint64_t t = 0;
+            if (dE <= 0.0) {
+                t = kDownhill;
+            } else if (d->kT > 0.0) {
+                const double v = std::ceil(std::exp(-dE / d->kT) * 9007199254740992.0);
+                t = v >= 9007199254740992.0 ? (1ULL << 53) : static_cast<uint64_t>(v);
+            }
+            acc[static_cast<size_t>(b) * 27 + a] = t;
+        }
+    mc->accept.alloc(acc.size() * 8);
+    check(hipMemcpyAsync(mc->accept.p, acc.data(), acc.size() * 8, hipMemcpyHostToDevice, s), "copy table");
+    mc->counters.alloc(kMcRows * kMcCounters * 8);
+    check(hipMemsetAsync(mc->counters.p, 0, kMcRows * kMcCounters * 8, s), "memset");
+    mc->gout.alloc(kMcRows * kMcSlots * 8);
+    mc_launch_init_spins(P, mc->spin.p, mc->spin_bytes, mc->list.as<uint32_t>(), mc->ndomain, s);
+    check(hipStreamSynchronize(s), "mc init");  // also: the temporary buffers and the caller's inputs are free
+    return mc;
+}
+
+std::unique_ptr<vca_mc> mc_new(vca_ctx* ctx, const vca_mc_desc* d) {
+    require(d->Q >= 2, "MC wymaga co najmniej 2 orientacji (Q >= 2)");
+    require(d->nx >= 1 && d->ny >= 1 && d->nz >= 1, "wymiary muszą być >= 1");
+    const uint64_t n = static_cast<uint64_t>(d->nx) * d->ny * d->nz;
+    require(n <= 2000000000ULL, "przestrzeń ma za dużo komórek");
+    auto mc = std::make_unique<vca_mc>();
+    mc->ctx = ctx;
+    mc->n = n;
+    McParams& P = mc->P;
+    P.nx = d->nx;
+    P.ny = d->ny;
+    P.nz = d->nz;
+    const int dims[3] = {d->nx, d->ny, d->nz};
+    for (int a = 0; a < 3; ++a) P.per[a] = d->periodic[a] && dims[a] > 1;
+    P.seed = d->seed;
+    P.Q = d->Q;
+    P.nb = make_nb(d->neighborhood, d->nx, d->ny, d->nz);
+    mc->checkerboard = d->checkerboard != 0;
+    mc->J = d->J;
+    mc->kT = d->kT;
+    mc->spin_bytes = d->Q <= 255 ? 1 : (d->Q <= 65535 ? 2 : 4);
+    return mc;
 }
 
 }  // namespace
@@ -1262,94 +1437,24 @@ extern "C" int vca_mc_create(vca_ctx* ctx, const vca_mc_desc* d, const uint8_t* 
                              int32_t mobility_dynamics, vca_mc** out) {
     return guarded([&] {
         require(ctx && d && out, "NULL argument");
-        require(d->Q >= 2, "MC wymaga co najmniej 2 orientacji (Q >= 2)");
-        require(d->nx >= 1 && d->ny >= 1 && d->nz >= 1, "wymiary muszą być >= 1");
-        const uint64_t n = static_cast<uint64_t>(d->nx) * d->ny * d->nz;
-        require(n <= 2000000000ULL, "przestrzeń ma za dużo komórek");
         ctx->set_device();
         AllocStream alloc_on(ctx->stream);
-        auto mc = std::make_unique<vca_mc>();
-        mc->ctx = ctx;
-        mc->n = n;
-        McParams& P = mc->P;
-        P.nx = d->nx;
-        P.ny = d->ny;
-        P.nz = d->nz;
-        const int dims[3] = {d->nx, d->ny, d->nz};
-        for (int a = 0; a < 3; ++a) P.per[a] = d->periodic[a] && dims[a] > 1;
-        P.seed = d->seed;
-        P.Q = d->Q;
-        P.nb = make_nb(d->neighborhood, d->nx, d->ny, d->nz);
-        mc->checkerboard = d->checkerboard != 0;
-        mc->J = d->J;
-        mc->kT = d->kT;
-        mc->spin_bytes = d->Q <= 255 ? 1 : (d->Q <= 65535 ? 2 : 4);
-
-        // Domain lists: colour-major, global index ascending within a colour (buildColors), or index order
-        // for the sequential random mode (initialize, PottsMC.cpp:104-143).
-        std::vector<uint32_t> list;
-        const Colouring col = make_colouring(d->neighborhood, dims, P.per);
-        if (mc->checkerboard) {
-            mc->colors = col.count;
-            std::vector<std::vector<uint32_t>> per_col(static_cast<size_t>(col.count));
-            uint64_t g = 0;
-            for (int z = 0; z < d->nz; ++z)
-                for (int y = 0; y < d->ny; ++y)
-                    for (int x = 0; x < d->nx; ++x, ++g)
-                        if (!domain || domain[g]) per_col[static_cast<size_t>(col.of(x, y, z))].push_back(static_cast<uint32_t>(g));
-            mc->color_start.push_back(0);
-            for (auto& v : per_col) {
-                list.insert(list.end(), v.begin(), v.end());
-                mc->color_start.push_back(static_cast<uint32_t>(list.size()));
-            }
-        } else {
-            mc->colors = 0;
-            for (uint64_t g = 0; g < n; ++g)
-                if (!domain || domain[g]) list.push_back(static_cast<uint32_t>(g));
-            mc->color_start = {0, static_cast<uint32_t>(list.size())};
-        }
-        mc->ndomain = static_cast<uint32_t>(list.size());
+        auto mc = mc_new(ctx, d);
         hipStream_t s = ctx->stream;
-        mc->spin.alloc(n * static_cast<uint64_t>(mc->spin_bytes));
-        check(hipMemsetAsync(mc->spin.p, 0, n * static_cast<uint64_t>(mc->spin_bytes), s), "memset");
-        mc->list.alloc(list.size() * 4);
-        check(hipMemcpyAsync(mc->list.p, list.data(), list.size() * 4, hipMemcpyHostToDevice, s), "copy list");
-        std::vector<uint64_t> thr;
+        McListSrc L{};
+        DevBuf dom;
+        if (domain) {
+            dom.alloc(mc->n);
+            check(hipMemcpyAsync(dom.p, domain, mc->n, hipMemcpyHostToDevice, s), "copy domain");
+            L.mode = 1;
+            L.dom = dom.as<uint8_t>();
+        }
         if (mobility) {
             mc->has_mob_cells = true;
-            mc->mob_cells.alloc(n * 4);
-            check(hipMemcpyAsync(mc->mob_cells.p, mobility, n * 4, hipMemcpyHostToDevice, s), "copy mobility");
-            if (mobility_dynamics) {
-                thr.resize(list.size());
-                for (size_t j = 0; j < list.size(); ++j) thr[j] = mobility_threshold(mobility[list[j]]);
-                mc->mob_thr.alloc(thr.size() * 8);
-                check(hipMemcpyAsync(mc->mob_thr.p, thr.data(), thr.size() * 8, hipMemcpyHostToDevice, s), "copy thr");
-                mc->has_mob_dyn = true;
-            }
+            mc->mob_cells.alloc(mc->n * 4);
+            check(hipMemcpyAsync(mc->mob_cells.p, mobility, mc->n * 4, hipMemcpyHostToDevice, s), "copy mobility");
         }
-        // Metropolis table T[before][after] (PottsMC.cpp:431-436): dE = J*after - J*before in double;
-        // accept-always for dE <= 0, else (r >> 11) < ceil(exp(-dE/kT) * 2^53) (exp from the host libm).
-        std::vector<uint64_t> acc(27 * 27, 0);
-        for (int b = 0; b < 27; ++b)
-            for (int a = 0; a < 27; ++a) {
-                const double dE = d->J * a - d->J * b;
-                uint64_t t = 0;
-                if (dE <= 0.0) {
-                    t = kDownhill;
-                } else if (d->kT > 0.0) {
-                    const double v = std::ceil(std::exp(-dE / d->kT) * 9007199254740992.0);
-                    t = v >= 9007199254740992.0 ? (1ULL << 53) : static_cast<uint64_t>(v);
-                }
-                acc[static_cast<size_t>(b) * 27 + a] = t;
-            }
-        mc->accept.alloc(acc.size() * 8);
-        check(hipMemcpyAsync(mc->accept.p, acc.data(), acc.size() * 8, hipMemcpyHostToDevice, s), "copy table");
-        mc->counters.alloc(kMcRows * kMcCounters * 8);
-        check(hipMemsetAsync(mc->counters.p, 0, kMcRows * kMcCounters * 8, s), "memset");
-        mc->gout.alloc(kMcRows * kMcSlots * 8);
-        mc_launch_init_spins(P, mc->spin.p, mc->spin_bytes, mc->list.as<uint32_t>(), mc->ndomain, s);
-        check(hipStreamSynchronize(s), "mc init");
-        *out = mc.release();
+        *out = mc_build(ctx, d, L, std::move(mc), mobility_dynamics != 0).release();
     });
 }
 
@@ -1358,41 +1463,32 @@ extern "C" int vca_mc_create_from_ca(vca_ctx* ctx, const vca_mc_desc* d, vca_ca*
     return guarded([&] {
         require(ctx && d && ca && out, "NULL argument");
         require(ca->P.nx == d->nx && ca->P.ny == d->ny && ca->P.nz == d->nz, "wymiary MC różnią się od CA");
-        const uint64_t n = ca->n;
-        std::vector<uint8_t> type(n);
-        std::vector<uint16_t> solid(n), arrival(n);
-        if (vca_ca_download(ca, type.data(), nullptr, nullptr, solid.data(), arrival.data()) != VCA_OK)
-            throw VcaError(VCA_ERR_HIP, g_last_error);
-        // CoolingMobility (CoolingMobility.cpp:8-129)
-        std::vector<uint8_t> dom(n);
-        uint32_t end = 0;
-        for (uint64_t i = 0; i < n; ++i) {
-            dom[i] = (type[i] == VCA_BASALT || type[i] == VCA_LAVA) ? 1 : 0;
-            if (dom[i]) end = std::max<uint32_t>(end, std::max(solid[i], arrival[i]));
+        require(ca->ctx->device == ctx->device, "MC i CA na różnych urządzeniach");
+        require(!ca->member, "the simulation is in a batch: remove it first");
+        ctx->set_device();
+        AllocStream alloc_on(ctx->stream);
+        auto mc = mc_new(ctx, d);
+        hipStream_t s = ctx->stream;
+        // a stopped simulation's state is final (later steps are no-ops), so ctx may be another context whose
+        // stream is not ordered after the CA's
+        {
+            SideStream side;
+            require_stopped(ca, side.s);
         }
-        std::vector<uint32_t> time(n, 0);
-        uint32_t tmin = 0xFFFFFFFFu, tmax = 0;
-        for (uint64_t i = 0; i < n; ++i) {
-            if (!dom[i]) continue;
-            const uint32_t sd = type[i] == VCA_BASALT ? solid[i] : end;
-            time[i] = sd > arrival[i] ? sd - arrival[i] : 0;
-            tmin = std::min(tmin, time[i]);
-            tmax = std::max(tmax, time[i]);
-        }
-        std::vector<float> mob(n, 0.0f);
-        for (uint64_t i = 0; i < n; ++i) {
-            if (!dom[i]) continue;
-            if (tmax == tmin) {
-                mob[i] = 1.0f;
-                continue;
-            }
-            const double norm = double(time[i] - tmin) / double(tmax - tmin);
-            mob[i] = static_cast<float>(mobility_min + (1.0 - mobility_min) * norm);
-        }
-        vca_mc* mc = nullptr;
-        if (vca_mc_create(ctx, d, dom.data(), mob.data(), use_mobility, &mc) != VCA_OK)
-            throw VcaError(VCA_ERR_ARG, g_last_error);
-        *out = mc;
+        // CoolingMobility (CoolingMobility.cpp:8-129) from the CA result on the device
+        mc->has_mob_cells = true;
+        mc->mob_cells.alloc(mc->n * 4);
+        DevBuf red;
+        red.alloc(16);
+        const uint32_t init[4] = {0, 0xFFFFFFFFu, 0, 0};
+        check(hipMemcpyAsync(red.p, init, sizeof init, hipMemcpyHostToDevice, s), "copy");
+        mc_launch_cooling(mc->n, ca->type.as<uint8_t>(), ca->solid.as<uint16_t>(), ca->arrival.as<uint16_t>(),
+                          red.as<uint32_t>(), mobility_min, mc->mob_cells.as<float>(), s);
+        check(hipGetLastError(), "cooling mobility");
+        McListSrc L{};
+        L.mode = 2;
+        L.dom = ca->type.as<uint8_t>();
+        *out = mc_build(ctx, d, L, std::move(mc), use_mobility != 0).release();
     });
 }
 

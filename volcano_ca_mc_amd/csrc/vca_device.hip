@@ -1463,6 +1463,10 @@ inline unsigned grid_for(uint64_t n, unsigned cap = 65535u * 4u) {
     const uint64_t g = (n + kBlock - 1) / kBlock;
     return static_cast<unsigned>(g < 1 ? 1 : (g > cap ? cap : g));
 }
+// One block per item (row, segment), at most cap blocks.
+inline unsigned blocks_for(uint64_t items, unsigned cap) {
+    return static_cast<unsigned>(items < 1 ? 1 : (items > cap ? cap : items));
+}
 }  // namespace
 
 void ca_launch_terrain(const CaParams& P, const CaDev& D, const int32_t* height, hipStream_t s) {
@@ -1599,6 +1603,338 @@ void mc_launch_labels(const McParams& P, const void* spin, int bytes, const uint
     const uint64_t n = static_cast<uint64_t>(P.nx) * P.ny * P.nz;
     VCA_SPIN_DISPATCH(bytes, hipLaunchKernelGGL(mc_labels_kernel<S>, dim3(grid_for(n, 16384)), dim3(kBlock), 0, s, n,
                                                 static_cast<const S*>(spin), parent, out));
+}
+
+// ------------------------------------------------------------------------------------------------
+// After the CA: flow sums of the final state and the MC domain lists, on the device
+// ------------------------------------------------------------------------------------------------
+
+// flowSums (Simulation.cpp:408-466): per flowId f in 1..nflows the LAVA and BASALT cells and the largest squared
+// horizontal distance to the flow's vent.  One block per grid row (y, z); a wave folds its lanes flow by flow
+// (lanes of one wave nearly always share one flow), so each (wave, flow) pair costs one atomic per counter.
+__global__ void __launch_bounds__(kBlock) ca_flow_sums_kernel(CaParams P, CaDev D, const int32_t* vent_xy,
+                                                               int32_t nflows, unsigned long long* lava,
+                                                               unsigned long long* basalt,
+                                                               unsigned long long* maxd2) {
+    const uint32_t rows = static_cast<uint32_t>(P.ny) * static_cast<uint32_t>(P.nz);
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    for (uint32_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int y = static_cast<int>(r % static_cast<uint32_t>(P.ny));
+        const uint32_t row = r * static_cast<uint32_t>(P.nx);
+        for (int x0 = 0; x0 < P.nx; x0 += kBlock) {
+            const int x = x0 + static_cast<int>(threadIdx.x);
+            bool valid = false, is_lava = false;
+            int32_t f = 0;
+            unsigned long long d2 = 0;
+            if (x < P.nx) {
+                const uint8_t t = D.type[row + static_cast<uint32_t>(x)];
+                if (t == kLava || t == kBasalt) {
+                    f = D.flow[row + static_cast<uint32_t>(x)];
+                    valid = f >= 1 && f <= nflows;
+                    is_lava = t == kLava;
+                    if (valid) {
+                        const long long dx = x - vent_xy[2 * (f - 1)], dy = y - vent_xy[2 * (f - 1) + 1];
+                        d2 = static_cast<unsigned long long>(dx * dx + dy * dy);
+                    }
+                }
+            }
+            unsigned long long pend = __ballot(valid);
+            while (pend) {
+                const int leader = __ffsll(static_cast<long long>(pend)) - 1;
+                const int32_t fl = __shfl(f, leader);
+                const bool mine = valid && f == fl;
+                const unsigned long long m = __ballot(mine);
+                const unsigned long long ml = __ballot(mine && is_lava);
+                unsigned long long mx = mine ? d2 : 0ULL;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    const unsigned long long v = __shfl_xor(mx, o);
+                    mx = v > mx ? v : mx;
+                }
+                if (lane == leader) {
+                    const unsigned long long nl = static_cast<unsigned long long>(__popcll(ml));
+                    const unsigned long long nb = static_cast<unsigned long long>(__popcll(m)) - nl;
+                    if (nl) atomicAdd(&lava[fl], nl);
+                    if (nb) atomicAdd(&basalt[fl], nb);
+                    atomicMax(&maxd2[fl], mx);
+                }
+                pend &= ~m;
+            }
+        }
+    }
+}
+
+// Marks seen[|dx|] for the LAVA / BASALT cells of flow `flow` whose squared distance to (vx, vy) is d2: which
+// lattice points of that circle the flow reaches (the reach is the largest hypot among them).
+__global__ void __launch_bounds__(kBlock) ca_flow_points_kernel(CaParams P, CaDev D, int32_t flow, int32_t vx,
+                                                                 int32_t vy, unsigned long long d2, uint32_t* seen,
+                                                                 uint32_t cap) {
+    const uint32_t rows = static_cast<uint32_t>(P.ny) * static_cast<uint32_t>(P.nz);
+    for (uint32_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int y = static_cast<int>(r % static_cast<uint32_t>(P.ny));
+        const uint32_t row = r * static_cast<uint32_t>(P.nx);
+        for (int x = static_cast<int>(threadIdx.x); x < P.nx; x += kBlock) {
+            const uint8_t t = D.type[row + static_cast<uint32_t>(x)];
+            if (t != kLava && t != kBasalt) continue;
+            if (D.flow[row + static_cast<uint32_t>(x)] != flow) continue;
+            const long long dx = x - vx, dy = y - vy;
+            if (static_cast<unsigned long long>(dx * dx + dy * dy) != d2) continue;
+            const unsigned long long a = static_cast<unsigned long long>(dx < 0 ? -dx : dx);
+            if (a < cap) seen[a] = 1u;
+        }
+    }
+}
+
+// Cooling mobility (CoolingMobility.cpp:8-129), three passes over the CA result: red[0] = end step (largest
+// solidStep / arrivalStep of the domain), red[1] / red[2] = smallest / largest time in the liquid state, then
+// the mobility of every cell (0 outside the domain).  Same double arithmetic as the host formula.
+__device__ inline bool mc_domain_cell(uint8_t t) { return t == kBasalt || t == kLava; }
+
+__global__ void __launch_bounds__(kBlock) mc_cool_end_kernel(uint64_t n, const uint8_t* type, const uint16_t* solid,
+                                                              const uint16_t* arrival, uint32_t* red) {
+    uint32_t e = 0;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * kBlock)
+        if (mc_domain_cell(type[i])) {
+            const uint32_t a = solid[i], b = arrival[i];
+            e = a > e ? a : e;
+            e = b > e ? b : e;
+        }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t v = __shfl_xor(e, o);
+        e = v > e ? v : e;
+    }
+    if ((threadIdx.x & 63u) == 0 && e) atomicMax(&red[0], e);
+}
+
+__device__ inline uint32_t mc_cool_time(uint8_t t, uint32_t solid, uint32_t arrival, uint32_t end) {
+    const uint32_t sd = t == kBasalt ? solid : end;
+    return sd > arrival ? sd - arrival : 0u;
+}
+
+__global__ void __launch_bounds__(kBlock) mc_cool_range_kernel(uint64_t n, const uint8_t* type, const uint16_t* solid,
+                                                                const uint16_t* arrival, uint32_t* red) {
+    const uint32_t end = red[0];
+    uint32_t lo = 0xFFFFFFFFu, hi = 0;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        const uint8_t t = type[i];
+        if (!mc_domain_cell(t)) continue;
+        const uint32_t tm = mc_cool_time(t, solid[i], arrival[i], end);
+        lo = tm < lo ? tm : lo;
+        hi = tm > hi ? tm : hi;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        if (lo != 0xFFFFFFFFu) atomicMin(&red[1], lo);
+        if (hi) atomicMax(&red[2], hi);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) mc_cool_mobility_kernel(uint64_t n, const uint8_t* type,
+                                                                   const uint16_t* solid, const uint16_t* arrival,
+                                                                   const uint32_t* red, double mobility_min,
+                                                                   float* mob) {
+    const uint32_t end = red[0], tmin = red[1], tmax = red[2];
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        const uint8_t t = type[i];
+        float m = 0.0f;
+        if (mc_domain_cell(t)) {
+            if (tmax == tmin) {
+                m = 1.0f;
+            } else {
+                const uint32_t tm = mc_cool_time(t, solid[i], arrival[i], end);
+                const double norm = static_cast<double>(tm - tmin) / static_cast<double>(tmax - tmin);
+                m = static_cast<float>(mobility_min + (1.0 - mobility_min) * norm);
+            }
+        }
+        mob[i] = m;
+    }
+}
+
+// Exact integer form of `toUniform(r) >= m` (the mobility skip) for a float mobility m: skip iff (r >> 11) >= thr.
+__device__ inline uint64_t mobility_threshold_dev(float m) {
+    const double two53 = 9007199254740992.0;
+    if (m != m) return 1ULL << 53;  // u >= NaN is false: never skipped
+    if (!(m > 0.0f)) return 0;      // u >= m for every u in [0, 1)
+    const double v = ceil(static_cast<double>(m) * two53);
+    return v >= two53 ? (1ULL << 53) : static_cast<uint64_t>(v);
+}
+
+// Domain lists of the MC, colour-major with ascending global index inside each colour (buildColors,
+// PottsMC.cpp:145-224), built by an ordered compaction: per-segment colour counts, a scan per colour, then each
+// segment writes its cells at their ranks.  Segments are kMcSeg consecutive cells, processed 256 at a time.
+constexpr int kMcSeg = 4096;
+
+__device__ inline bool mc_list_member(const McListSrc& L, uint64_t i) {
+    if (L.mode == 0) return true;
+    const uint8_t v = L.dom[i];
+    return L.mode == 1 ? v != 0 : mc_domain_cell(v);
+}
+__device__ inline int mc_list_colour(const McListSrc& L, uint64_t i) {
+    if (L.ncol <= 1) return 0;
+    // i < n <= 2e9: 32-bit division
+    const uint32_t nx = static_cast<uint32_t>(L.dims[0]), ny = static_cast<uint32_t>(L.dims[1]);
+    const uint32_t ii = static_cast<uint32_t>(i);
+    const uint32_t x = ii % nx, q = ii / nx;
+    const uint32_t y = q % ny, z = q / ny;
+    if (L.parity) return static_cast<int>((x + y + z) & 1u);
+    const uint32_t c[3] = {x, y, z};
+    int k = 0, mul = 1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int cls = (L.per[a] && (L.dims[a] & 1) && c[a] == static_cast<uint32_t>(L.dims[a] - 1)) ? 2
+                                                                                                     : static_cast<int>(c[a] & 1u);
+        k += cls * mul;
+        mul *= 3;
+    }
+    return L.index[k];
+}
+
+__global__ void __launch_bounds__(kBlock) mc_list_count_kernel(McListSrc L, uint32_t* cnt) {
+    __shared__ uint32_t s_cnt[27];
+    const uint64_t nseg = (L.n + kMcSeg - 1) / kMcSeg;
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    for (uint64_t g = blockIdx.x; g < nseg; g += gridDim.x) {
+        __syncthreads();
+        if (threadIdx.x < 27) s_cnt[threadIdx.x] = 0;
+        __syncthreads();
+        for (int r = 0; r < kMcSeg; r += kBlock) {
+            const uint64_t i = g * kMcSeg + static_cast<uint64_t>(r) + threadIdx.x;
+            const bool in = i < L.n && mc_list_member(L, i);
+            const int c = in ? mc_list_colour(L, i) : 0;
+            unsigned long long pend = __ballot(in);
+            while (pend) {
+                const int leader = __ffsll(static_cast<long long>(pend)) - 1;
+                const int cl = __shfl(c, leader);
+                const unsigned long long m = __ballot(in && c == cl);
+                if (lane == leader) atomicAdd(&s_cnt[cl], static_cast<uint32_t>(__popcll(m)));
+                pend &= ~m;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < static_cast<unsigned>(L.ncol)) cnt[static_cast<uint64_t>(threadIdx.x) * nseg + g] = s_cnt[threadIdx.x];
+    }
+}
+
+// Exclusive scan of each colour's segment counts (one block per colour); totals[c] = the colour's cells.
+__global__ void __launch_bounds__(1024) mc_list_scan_kernel(uint32_t* cnt, uint64_t nseg, uint32_t* totals) {
+    __shared__ uint32_t s_w[16];
+    __shared__ uint32_t s_carry;
+    uint32_t* v = cnt + static_cast<uint64_t>(blockIdx.x) * nseg;
+    const int lane = static_cast<int>(threadIdx.x & 63u), w = static_cast<int>(threadIdx.x >> 6);
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (uint64_t b = 0; b < nseg; b += 1024) {
+        const uint64_t j = b + threadIdx.x;
+        const uint32_t x = j < nseg ? v[j] : 0u;
+        uint32_t inc = x;  // inclusive scan inside the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(inc, o);
+            if (lane >= o) inc += y;
+        }
+        if (lane == 63) s_w[w] = inc;
+        __syncthreads();
+        uint32_t before = s_carry;
+        for (int k = 0; k < w; ++k) before += s_w[k];
+        if (j < nseg) v[j] = before + inc - x;
+        __syncthreads();
+        if (threadIdx.x == 1023) s_carry = before + inc;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = s_carry;
+}
+
+__global__ void __launch_bounds__(kBlock) mc_list_scatter_kernel(McListSrc L, const uint32_t* off,
+                                                                  const uint32_t* colour_base, uint32_t* list,
+                                                                  const float* mob, uint64_t* thr) {
+    __shared__ uint32_t s_off[27];
+    __shared__ uint32_t s_wt[27][kBlock / 64];
+    const uint64_t nseg = (L.n + kMcSeg - 1) / kMcSeg;
+    const int lane = static_cast<int>(threadIdx.x & 63u), w = static_cast<int>(threadIdx.x >> 6);
+    const unsigned long long below = (1ULL << lane) - 1ULL;
+    for (uint64_t g = blockIdx.x; g < nseg; g += gridDim.x) {
+        __syncthreads();
+        if (threadIdx.x < static_cast<unsigned>(L.ncol))
+            s_off[threadIdx.x] = colour_base[threadIdx.x] + off[static_cast<uint64_t>(threadIdx.x) * nseg + g];
+        for (int r = 0; r < kMcSeg; r += kBlock) {
+            const uint64_t i = g * kMcSeg + static_cast<uint64_t>(r) + threadIdx.x;
+            const bool in = i < L.n && mc_list_member(L, i);
+            const int c = in ? mc_list_colour(L, i) : 0;
+            if (lane < L.ncol) s_wt[lane][w] = 0;  // this wave's column (in program order before its writes)
+            uint32_t rank = 0;
+            unsigned long long pend = __ballot(in);
+            while (pend) {
+                const int leader = __ffsll(static_cast<long long>(pend)) - 1;
+                const int cl = __shfl(c, leader);
+                const unsigned long long m = __ballot(in && c == cl);
+                if (in && c == cl) rank = static_cast<uint32_t>(__popcll(m & below));
+                if (lane == leader) s_wt[cl][w] = static_cast<uint32_t>(__popcll(m));
+                pend &= ~m;
+            }
+            __syncthreads();
+            if (in) {
+                uint32_t pos = s_off[c] + rank;
+                for (int k = 0; k < w; ++k) pos += s_wt[c][k];
+                list[pos] = static_cast<uint32_t>(i);
+                if (thr) thr[pos] = mobility_threshold_dev(mob[i]);
+            }
+            __syncthreads();
+            if (threadIdx.x < static_cast<unsigned>(L.ncol)) {
+                uint32_t add = 0;
+                for (int k = 0; k < kBlock / 64; ++k) add += s_wt[threadIdx.x][k];
+                s_off[threadIdx.x] += add;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+void ca_launch_flow_sums(const CaParams& P, const CaDev& D, const int32_t* vent_xy, int32_t nflows,
+                         unsigned long long* lava, unsigned long long* basalt, unsigned long long* maxd2,
+                         hipStream_t s) {
+    const uint64_t rows = static_cast<uint64_t>(P.ny) * P.nz;
+    hipLaunchKernelGGL(ca_flow_sums_kernel, dim3(blocks_for(rows, 16384)), dim3(kBlock), 0, s, P, D, vent_xy, nflows,
+                       lava, basalt, maxd2);
+}
+
+void ca_launch_flow_points(const CaParams& P, const CaDev& D, int32_t flow, int32_t vx, int32_t vy,
+                           unsigned long long d2, uint32_t* seen, uint32_t cap, hipStream_t s) {
+    const uint64_t rows = static_cast<uint64_t>(P.ny) * P.nz;
+    hipLaunchKernelGGL(ca_flow_points_kernel, dim3(blocks_for(rows, 16384)), dim3(kBlock), 0, s, P, D, flow, vx, vy,
+                       d2, seen, cap);
+}
+
+void mc_launch_cooling(uint64_t n, const uint8_t* type, const uint16_t* solid, const uint16_t* arrival,
+                       uint32_t* red, double mobility_min, float* mob, hipStream_t s) {
+    const unsigned g = grid_for(n, 8192);
+    hipLaunchKernelGGL(mc_cool_end_kernel, dim3(g), dim3(kBlock), 0, s, n, type, solid, arrival, red);
+    hipLaunchKernelGGL(mc_cool_range_kernel, dim3(g), dim3(kBlock), 0, s, n, type, solid, arrival, red);
+    hipLaunchKernelGGL(mc_cool_mobility_kernel, dim3(grid_for(n, 65536)), dim3(kBlock), 0, s, n, type, solid, arrival,
+                       red, mobility_min, mob);
+}
+
+uint64_t mc_list_segments(uint64_t n) { return (n + kMcSeg - 1) / kMcSeg; }
+
+void mc_launch_list_count(const McListSrc& L, uint32_t* cnt, uint32_t* totals, hipStream_t s) {
+    const uint64_t nseg = mc_list_segments(L.n);
+    hipLaunchKernelGGL(mc_list_count_kernel, dim3(blocks_for(nseg, 16384)), dim3(kBlock), 0, s, L, cnt);
+    hipLaunchKernelGGL(mc_list_scan_kernel, dim3(static_cast<unsigned>(L.ncol)), dim3(1024), 0, s, cnt, nseg, totals);
+}
+
+void mc_launch_list_scatter(const McListSrc& L, const uint32_t* off, const uint32_t* colour_base, uint32_t* list,
+                            const float* mob, uint64_t* thr, hipStream_t s) {
+    const uint64_t nseg = mc_list_segments(L.n);
+    hipLaunchKernelGGL(mc_list_scatter_kernel, dim3(blocks_for(nseg, 16384)), dim3(kBlock), 0, s, L, off, colour_base,
+                       list, mob, thr);
 }
 
 }  // namespace vca

@@ -181,6 +181,20 @@ struct McParams {
 
 constexpr uint64_t kDownhill = ~0ULL;  // accept-table entry for dE <= 0
 
+// Source of the MC domain lists built on the device: mode 0 every cell, 1 dom[i] != 0, 2 dom = CA cell types
+// (LAVA or BASALT).  ncol colours (1 = one list in index order); parity colouring or index[] over the axis
+// classes as in buildColors (PottsMC.cpp:145-224).
+struct McListSrc {
+    uint64_t n;
+    const uint8_t* dom;
+    int32_t mode;
+    int32_t ncol;
+    int32_t parity;
+    int32_t dims[3];
+    int32_t per[3];
+    int32_t index[27];
+};
+
 // ---- launchers (vca_device.hip) ----
 void ca_launch_terrain(const CaParams& P, const CaDev& D, const int32_t* height, hipStream_t s);
 void ca_launch_seeds(const CaParams& P, const CaDev& D, const uint32_t* cells, const int32_t* flows,
@@ -198,6 +212,23 @@ void ca_launch_status(const BatchDev& B, int32_t* out, hipStream_t s);
 // Writes entry `index` of the device's simulation table (stream ordered; `sim` must stay valid until the
 // stream reaches the copy).
 void ca_write_sim(uint32_t index, const CaSim* sim, hipStream_t s);
+
+// After the CA: per-flow LAVA / BASALT counts and largest squared vent distance (arrays of nflows + 1, zeroed by
+// the caller; vent_xy = x, y of flow f at 2*(f-1)); seen[|dx|] = 1 for the cells of one flow at squared distance d2.
+void ca_launch_flow_sums(const CaParams& P, const CaDev& D, const int32_t* vent_xy, int32_t nflows,
+                         unsigned long long* lava, unsigned long long* basalt, unsigned long long* maxd2,
+                         hipStream_t s);
+void ca_launch_flow_points(const CaParams& P, const CaDev& D, int32_t flow, int32_t vx, int32_t vy,
+                           unsigned long long d2, uint32_t* seen, uint32_t cap, hipStream_t s);
+// Cooling mobility of every cell from the CA result; red[3] = {0, 0xFFFFFFFF, 0} on entry (end, tmin, tmax).
+void mc_launch_cooling(uint64_t n, const uint8_t* type, const uint16_t* solid, const uint16_t* arrival,
+                       uint32_t* red, double mobility_min, float* mob, hipStream_t s);
+// Ordered domain lists: count + scan (cnt: ncol * segments entries; totals: ncol), then scatter (colour_base =
+// first list slot of each colour; thr != nullptr also writes the mobility threshold of each entry from mob).
+uint64_t mc_list_segments(uint64_t n);
+void mc_launch_list_count(const McListSrc& L, uint32_t* cnt, uint32_t* totals, hipStream_t s);
+void mc_launch_list_scatter(const McListSrc& L, const uint32_t* off, const uint32_t* colour_base, uint32_t* list,
+                            const float* mob, uint64_t* thr, hipStream_t s);
 
 // Colour pass of the checkerboard sweep.  spin_bytes: 1, 2 or 4.
 void mc_launch_color(const McParams& P, void* spin, int spin_bytes, const uint32_t* list, const uint64_t* mob_thr,

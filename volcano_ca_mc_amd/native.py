@@ -80,6 +80,7 @@ def lib() -> C.CDLL:
         "vca_ca_status": [vp, C.POINTER(i32), C.POINTER(i32)],
         "vca_ca_histories": [vp, vp, vp, i32, C.POINTER(i32)], "vca_ca_download": [vp, vp, vp, vp, vp, vp],
         "vca_ca_work": [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(i32)],
+        "vca_ca_flow_stats": [vp, vp, i32, vp, vp, vp], "vca_ca_flow_points": [vp, vp, i32, u64, vp, i32],
         "vca_mc_create": [vp, C.POINTER(McDesc), vp, vp, i32, C.POINTER(vp)],
         "vca_mc_create_from_ca": [vp, C.POINTER(McDesc), vp, i32, C.c_double, C.POINTER(vp)],
         "vca_mc_destroy": [vp], "vca_mc_sweep": [vp, i32, C.POINTER(McCounts)],
@@ -101,7 +102,7 @@ EXPORTED = ("vca_version", "vca_last_error", "vca_ctx_create", "vca_ctx_destroy"
             "vca_ca_run", "vca_ca_enqueue", "vca_ca_status", "vca_ca_histories", "vca_ca_download", "vca_ca_work",
             "vca_ca_footprint", "vca_ca_tiles", "vca_batch_create", "vca_batch_destroy", "vca_batch_add",
             "vca_batch_remove", "vca_batch_set_graph", "vca_batch_enqueue", "vca_batch_run", "vca_batch_info",
-            "vca_batch_status", "vca_ca_collect",
+            "vca_batch_status", "vca_ca_collect", "vca_ca_flow_stats", "vca_ca_flow_points",
             "vca_mc_create", "vca_mc_create_from_ca", "vca_mc_destroy", "vca_mc_sweep", "vca_mc_counts_get",
             "vca_mc_measure", "vca_mc_download", "vca_mc_download_labels", "vca_mc_info",
             "vca_mc_download_mobility")
@@ -258,6 +259,20 @@ class LavaCA:
         tv, ch, tc = C.c_uint64(), C.c_uint64(), C.c_int32()
         _check(lib().vca_ca_work(self.h, C.byref(tv), C.byref(ch), C.byref(tc)))
         return int(tv.value), int(ch.value), int(tc.value)
+
+    def flow_stats(self, vents: Sequence[Sequence[int]]):
+        """Per flowId 1..len(vents): (lava cells, basalt cells, largest squared vent distance), on the device."""
+        m = len(vents) + 1
+        v = (Vent * max(1, len(vents)))(*[Vent(*map(int, x)) for x in vents])
+        la, ba, d2 = (np.zeros(m, np.uint64) for _ in range(3))
+        _check(lib().vca_ca_flow_stats(self.h, C.cast(v, C.c_void_p), len(vents), _ptr(la), _ptr(ba), _ptr(d2)))
+        return la, ba, d2
+
+    def flow_points(self, vent: Sequence[int], flow: int, d2: int, cap: int) -> np.ndarray:
+        v = Vent(*map(int, vent))
+        seen = np.zeros(cap, np.uint8)
+        _check(lib().vca_ca_flow_points(self.h, C.byref(v), flow, d2, _ptr(seen), cap))
+        return seen
 
 
 class Batch:
