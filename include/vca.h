@@ -79,7 +79,7 @@ typedef struct {
     int32_t ponding;             /* CaParams::ponding */
     int32_t dense;               /* 0: visit only tiles near lava (ca.sweep = front); 1: every tile (full) */
     int32_t tile_shape;          /* 0: default for the dims; 1..5: 64x4x4, 64x2x8, 64x8x2, 64x1x16, 64x16x1
-                                    (results do not depend on it; see vca_autotune_shape) */
+                                    (results do not depend on it; vca_ca_autotune picks one) */
 } vca_ca_desc;
 
 typedef struct {
@@ -138,6 +138,19 @@ int vca_ca_flow_points(vca_ca* ca, const vca_vent* vent, int32_t flow, uint64_t 
 
 /* Device memory a simulation of these dims takes (state, change slots, tile lists), for admission control. */
 int vca_ca_footprint(const vca_ca_desc* desc, uint64_t* bytes);
+/* Autotuner (no counterpart in the reference): runs the first probe_steps steps of this simulation once per
+ * candidate launch configuration — every tile shape that suits the dims, each with the active-front compaction
+ * (dense = 0) and without it (dense = 1) — and returns the fastest by the device time of a whole step.  Results do
+ * not depend on either choice, so the caller may put the winner into the descriptor (tile_shape, dense) of the real
+ * run.  The context's kernel timing state is left as it was. */
+typedef struct {
+    int32_t tile_shape;          /* 1..5 */
+    int32_t dense;
+    int32_t candidates, probe_steps;
+    float ms_per_step[2][5];     /* [dense][tile_shape - 1], 0 = not tried */
+} vca_tune_result;
+int vca_ca_autotune(vca_ctx* ctx, const vca_ca_desc* desc, const int32_t* height, const int32_t* spill,
+                    const vca_vent* vents, int32_t nvents, int32_t probe_steps, vca_tune_result* out);
 /* Tiles of the simulation (its share of a batch's tile range). */
 int vca_ca_tiles(vca_ca* ca, uint64_t* tiles);
 

@@ -62,3 +62,25 @@ def test_engine_batch_runs_children_and_reports_failures(tmp_path):
     assert len(rows) == 3
     assert rows[1].startswith("a_good,ok,0") and rows[2].startswith("b_bad,error,1")
     assert (out / "a_good" / "ca_result.bin").exists()
+
+
+@pytest.mark.parametrize("name", ["lava3d_moore_absorbing", "lava2d_von_neumann_absorbing"])
+def test_engine_autotune_keeps_outputs(tmp_path, name):
+    """--autotune: probes every tile shape x front mode on the run's own inputs, runs with the fastest, and the
+    result files stay identical to the reference's."""
+    d = os.path.join(GOLD, "cases", name)
+    print(f"engine --autotune case {name}", flush=True)
+    r = subprocess.run([ENGINE, "--config", os.path.join(d, "config.json"), "--out", str(tmp_path), "--autotune", "8"],
+                       cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    p = os.path.join(d, "ca_result.bin.gz")
+    assert (tmp_path / "ca_result.bin").read_bytes() == gzip.decompress(open(p, "rb").read())
+    got = json.loads((tmp_path / "stats.json").read_text(encoding="utf-8"))
+    got.pop("parallel", None)
+    assert got == json.load(open(os.path.join(d, "stats.json"), encoding="utf-8"))
+    tune = json.loads((tmp_path / "times.json").read_text())["autotune"]
+    cfg = json.load(open(os.path.join(d, "config.json")))
+    three_d = cfg["space"].get("ny", 1) > 1 and cfg["space"].get("nz", 1) > 1
+    assert len(tune["msPerStep"]) == (10 if three_d else 2)
+    assert tune["sweep"] in ("front", "full") and tune["probeSteps"] == 8
+    assert min(tune["msPerStep"].values()) == tune["msPerStep"][f"{tune['sweep']} {tune['tileShape']}"]

@@ -34,6 +34,8 @@ std::string usage() {
            "                     od chwili at, wspólne kroki GPU; serve_summary.json z opóźnieniami p50/p99\n"
            "  --graph-steps N    długość grafu HIP kroków wsadowych (domyślnie 16, 0 = bez grafów)\n"
            "  --max-jobs N       najwyżej N zadań naraz w potoku wsadowym (1 = po jednym; domyślnie bez limitu)\n"
+           "  --autotune N       przed przebiegiem CA: N kroków próbnych na każdy kształt kafla i tryb frontu,\n"
+           "                     przebieg z najszybszym (wyniki bez zmian; wybór w times.json)\n"
            "  --progress         linie PROGRESS ca/mc/save/batch na stdout (dla GUI)\n"
            "  --validate         tylko walidacja configu; JSON {\"ok\", \"errors\", \"warnings\"} na stdout\n"
            "  --print-defaults   pełny domyślny config JSON\n"
@@ -65,6 +67,14 @@ CliOptions parse_cli(int argc, const char* const* argv) {
                 throw CliError("--max-jobs wymaga liczby, podano: " + v);
             }
             if (opt.max_jobs < 0) throw CliError("--max-jobs nie może być ujemne");
+        } else if (a == "--autotune") {
+            const std::string v = value(i, a);
+            try {
+                opt.autotune_steps = std::stoi(v);
+            } catch (...) {
+                throw CliError("--autotune wymaga liczby, podano: " + v);
+            }
+            if (opt.autotune_steps < 1 || opt.autotune_steps > 65535) throw CliError("--autotune poza zakresem 1..65535");
         } else if (a == "--graph-steps") {
             const std::string v = value(i, a);
             try {

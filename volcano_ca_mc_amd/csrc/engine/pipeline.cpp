@@ -17,6 +17,7 @@
 #include <fstream>
 #include <future>
 #include <iostream>
+#include <map>
 #include <memory>
 #include <sstream>
 #include <thread>
@@ -263,6 +264,7 @@ private:
             if (opt_.max_jobs > 0 && admitted() >= opt_.max_jobs) break;
             if (used_ + j.fp_ca > budget_ && used_ > 0) break;
             try {
+                if (opt_.autotune_steps > 0) tuned(j);
                 Clock t;
                 j.ca = create_ca(j.cfg, j.in, gpu_);
                 int32_t slot = -1;
@@ -285,6 +287,24 @@ private:
             }
         }
         return any;
+    }
+
+    // --autotune: one probe per class of runs (dims, neighbourhood, mode, boundary), reused by the others
+    void tuned(Job& j) {
+        const Config& c = j.cfg;
+        std::ostringstream key;
+        key << c.space.nx << 'x' << c.space.ny << 'x' << c.space.nz << '/' << static_cast<int>(c.neighborhood) << '/'
+            << c.ca.classic << '/' << c.periodic;
+        auto it = tune_cache_.find(key.str());
+        if (it == tune_cache_.end()) {
+            autotune_ca(j.cfg, j.in, gpu_, opt_.autotune_steps);
+            j.timing.add("autotune", j.in.tune_ms);
+            tune_cache_[key.str()] = j.in.tune;
+        } else {
+            j.in.tuned = true;
+            j.in.tune = it->second;
+        }
+        j.timing.set_info("autotune", autotune_json(j.in));
     }
 
     // one chunk of batched CA steps, then the stop rules of every member in one read
@@ -410,6 +430,7 @@ private:
     double makespan_s_ = 0.0;
     Clock start_;
     double phase_ms_[4] = {0.0, 0.0, 0.0, 0.0};
+    std::map<std::string, vca_tune_result> tune_cache_;
     double collect_ms_ = 0.0;
 };
 

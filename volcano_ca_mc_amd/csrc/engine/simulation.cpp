@@ -251,9 +251,42 @@ CaInputs ca_inputs(const Config& cfg) {
     return in;
 }
 
+void autotune_ca(const Config& cfg, CaInputs& in, Gpu& gpu, int probe_steps) {
+    Clock t;
+    const vca_ca_desc d = make_ca_desc(cfg);
+    std::vector<vca_vent> vv;
+    for (const Vent& v : in.vents) vv.push_back(vca_vent{v.x, v.y, v.z, v.flow_id});
+    vca_check(vca_ca_autotune(gpu.ctx(), &d, in.hm.empty() ? nullptr : in.hm.height.data(),
+                              in.spill.empty() ? nullptr : in.spill.height.data(), vv.data(),
+                              static_cast<int32_t>(vv.size()), probe_steps, &in.tune),
+              "CA autotune");
+    in.tuned = true;
+    in.tune_ms = t.ms();
+}
+
+Json autotune_json(const CaInputs& in) {
+    static const char* kShape[5] = {"64x4x4", "64x2x8", "64x8x2", "64x1x16", "64x16x1"};
+    Json j = Json::object();
+    j["tileShape"] = kShape[std::max(1, std::min(5, in.tune.tile_shape)) - 1];
+    j["sweep"] = in.tune.dense ? "full" : "front";
+    j["probeSteps"] = in.tune.probe_steps;
+    Json c = Json::object();
+    for (int dn = 0; dn < 2; ++dn)
+        for (int sh = 0; sh < 5; ++sh)
+            if (in.tune.ms_per_step[dn][sh] > 0.0f)
+                c[std::string(dn ? "full " : "front ") + kShape[sh]] = Json(double(in.tune.ms_per_step[dn][sh]));
+    j["msPerStep"] = c;
+    j["ms"] = Json(in.tune_ms);
+    return j;
+}
+
 vca_ca* create_ca(const Config& cfg, const CaInputs& in, Gpu& gpu) {
     // GPU state: allocation, rock columns, seeds, initial counts
-    const vca_ca_desc d = make_ca_desc(cfg);
+    vca_ca_desc d = make_ca_desc(cfg);
+    if (in.tuned) {
+        d.tile_shape = in.tune.tile_shape;
+        d.dense = in.tune.dense;
+    }
     std::vector<vca_vent> vv;
     for (const Vent& v : in.vents) vv.push_back(vca_vent{v.x, v.y, v.z, v.flow_id});
     vca_ca* h = nullptr;
@@ -375,7 +408,8 @@ void write_config_used(const Config& cfg, const std::string& out_dir, Timing& ti
     timing.add_file((out / "config_used.json").string(), t.ms());
 }
 
-RunSummary run_simulation(const Config& cfg, const std::string& out_dir, Timing& timing, Progress& progress, Gpu& gpu) {
+RunSummary run_simulation(const Config& cfg, const std::string& out_dir, Timing& timing, Progress& progress, Gpu& gpu,
+                          int autotune_steps) {
     RunSummary s;
     const fs::path out(out_dir);
     write_config_used(cfg, out_dir, timing);
@@ -383,8 +417,13 @@ RunSummary run_simulation(const Config& cfg, const std::string& out_dir, Timing&
     if (cfg.mc.full_only()) {
         s.ca_ran = false;
     } else {
-        const CaInputs in = ca_inputs(cfg);
+        CaInputs in = ca_inputs(cfg);
         s.vents = in.vents;
+        if (autotune_steps > 0) {
+            autotune_ca(cfg, in, gpu, autotune_steps);
+            timing.add("autotune", in.tune_ms);
+            timing.set_info("autotune", autotune_json(in));
+        }
         Clock t;
         ca.h = create_ca(cfg, in, gpu);
         timing.add("allocate", t.ms());
@@ -678,7 +717,8 @@ McSummary McRun::finish(Timing& timing) {
         j["byMobility"] = bm;
     }
     s_.stats = j;
-    download(s_.crystals);
+    // the orientations go to the host only for mc_result.bin / .txt
+    if (cfg_.output.format != OutputFormat::None) download(s_.crystals);
     timing.add("mcInit", init_ms_);
     timing.add("mcSim", sim_ms_);
     timing.add("mcStats", stats_ms_ + t.ms());
@@ -921,7 +961,7 @@ int run_from_cli(const CliOptions& opt, const Clock& since_start) {
         return 0;
     }
 
-    const RunSummary s = run_simulation(cfg, out_dir, timing, progress, gpu);
+    const RunSummary s = run_simulation(cfg, out_dir, timing, progress, gpu, opt.autotune_steps);
     timing.add("total", since_start.ms());
     const Dims& d = cfg.space;
     run_info(timing, cfg, s, gpu);

@@ -26,6 +26,7 @@ struct CliOptions {
     bool gpu_batch = false;           // --gpu-batch: --batch in one process through the batched pipeline
     int steps_per_graph = 16;         // --graph-steps: hipGraph length of the batched CA steps (0 = direct)
     int max_jobs = 0;                 // --max-jobs: jobs admitted at once in the batched pipeline (0 = no limit)
+    int autotune_steps = 0;           // --autotune: probe steps per launch configuration before the CA (0 = off)
     std::string serve_path;           // --serve <jobs.jsonl>
     int repeats = 3;
     std::string config_path, out_dir, mc_from, batch_dir;
@@ -182,7 +183,13 @@ struct CaInputs {
     Heightmap hm, spill;
     std::vector<Vent> vents;
     double terrain_ms = 0.0, vents_ms = 0.0;
+    bool tuned = false;               // tile shape and front mode chosen by vca_ca_autotune
+    vca_tune_result tune{};
+    double tune_ms = 0.0;
 };
+// Probes the launch configurations on this run's inputs (vca_ca_autotune) and keeps the fastest in `in`.
+void autotune_ca(const Config& cfg, CaInputs& in, Gpu& gpu, int probe_steps);
+Json autotune_json(const CaInputs& in);
 vca_ca_desc make_ca_desc(const Config& cfg);
 CaInputs ca_inputs(const Config& cfg);
 vca_ca* create_ca(const Config& cfg, const CaInputs& in, Gpu& gpu);
@@ -205,7 +212,8 @@ McSummary run_mc(const Config& cfg, const CaState* ca, Cooling cooling, Timing& 
 McSummary run_mc(const Config& cfg, McPrep prep, double prep_ms, Timing& timing, Progress& progress,
                  const std::string& out_dir, Gpu& gpu);
 void save_mc(const Config& cfg, const McSummary& s, const std::string& out_dir, Timing& timing);
-RunSummary run_simulation(const Config& cfg, const std::string& out_dir, Timing& timing, Progress& progress, Gpu& gpu);
+RunSummary run_simulation(const Config& cfg, const std::string& out_dir, Timing& timing, Progress& progress, Gpu& gpu,
+                          int autotune_steps = 0);
 int run_from_cli(const CliOptions& opt, const Clock& since_start);
 
 // --batch helpers (batch.cpp) and the batched pipeline (pipeline.cpp)

@@ -1137,6 +1137,67 @@ extern "C" int vca_ca_footprint(const vca_ca_desc* d, uint64_t* bytes) {
     });
 }
 
+extern "C" int vca_ca_autotune(vca_ctx* ctx, const vca_ca_desc* desc, const int32_t* height, const int32_t* spill,
+                               const vca_vent* vents, int32_t nvents, int32_t probe_steps, vca_tune_result* out) {
+    return guarded([&] {
+        require(ctx && desc && out && probe_steps > 0, "invalid argument");
+        ctx->set_device();
+        ctx->drain();
+        // the caller's timing state is kept aside and restored
+        const bool timing = ctx->timing;
+        std::map<std::string, KernelTime> saved;
+        saved.swap(ctx->times);
+        *out = vca_tune_result{};
+        out->probe_steps = probe_steps;
+        std::vector<int> shapes;
+        if (desc->ny == 1)
+            shapes = {4};  // x-z plane: 64 x 1 x 16
+        else if (desc->nz == 1)
+            shapes = {5};  // x-y plane: 64 x 16 x 1
+        else
+            shapes = {1, 2, 3, 4, 5};
+        float best = 0.0f;
+        try {
+            ctx->timing = true;
+            for (int dense = 0; dense < 2; ++dense)
+                for (int sh : shapes) {
+                    vca_ca_desc d = *desc;
+                    d.tile_shape = sh;
+                    d.dense = dense;
+                    vca_ca* ca = nullptr;
+                    if (vca_ca_create(ctx, &d, height, spill, vents, nvents, &ca) != VCA_OK)
+                        throw VcaError(VCA_ERR_ARG, g_last_error);
+                    ctx->drain();
+                    ctx->times.clear();
+                    int32_t steps = 0, stop = 0;
+                    const int rc = vca_ca_run(ca, probe_steps, &steps, &stop);
+                    ctx->drain();
+                    vca_ca_destroy(ca);
+                    if (rc != VCA_OK) throw VcaError(rc, g_last_error);
+                    const KernelTime& k = ctx->times["ca_step"];
+                    const float ms = k.launches ? static_cast<float>(k.ms / static_cast<double>(k.launches)) : 0.0f;
+                    out->ms_per_step[dense][sh - 1] = ms;
+                    ++out->candidates;
+                    if (k.launches && (best == 0.0f || ms < best)) {
+                        best = ms;
+                        out->tile_shape = sh;
+                        out->dense = dense;
+                    }
+                }
+        } catch (...) {
+            ctx->timing = timing;
+            ctx->times.swap(saved);
+            throw;
+        }
+        ctx->timing = timing;
+        ctx->times.swap(saved);
+        if (out->tile_shape == 0) {  // nothing ran (the simulation stops before its first step)
+            out->tile_shape = default_shape(desc->nx, desc->ny, desc->nz) + 1;
+            out->dense = desc->dense;
+        }
+    });
+}
+
 extern "C" int vca_ca_tiles(vca_ca* ca, uint64_t* tiles) {
     return guarded([&] {
         require(ca && tiles, "NULL argument");

@@ -35,6 +35,11 @@ class Vent(C.Structure):
     _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("z", C.c_int32), ("flow_id", C.c_int32)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("tile_shape", C.c_int32), ("dense", C.c_int32), ("candidates", C.c_int32),
+                ("probe_steps", C.c_int32), ("ms_per_step", (C.c_float * 5) * 2)]
+
+
 class McDesc(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("neighborhood", C.c_int32),
                 ("periodic", C.c_int32 * 3), ("seed", C.c_uint64), ("Q", C.c_int32), ("J", C.c_double),
@@ -81,6 +86,7 @@ def lib() -> C.CDLL:
         "vca_ca_histories": [vp, vp, vp, i32, C.POINTER(i32)], "vca_ca_download": [vp, vp, vp, vp, vp, vp],
         "vca_ca_work": [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(i32)],
         "vca_ca_flow_stats": [vp, vp, i32, vp, vp, vp], "vca_ca_flow_points": [vp, vp, i32, u64, vp, i32],
+        "vca_ca_autotune": [vp, C.POINTER(CaDesc), vp, vp, vp, i32, i32, C.POINTER(TuneResult)],
         "vca_mc_create": [vp, C.POINTER(McDesc), vp, vp, i32, C.POINTER(vp)],
         "vca_mc_create_from_ca": [vp, C.POINTER(McDesc), vp, i32, C.c_double, C.POINTER(vp)],
         "vca_mc_destroy": [vp], "vca_mc_sweep": [vp, i32, C.POINTER(McCounts)],
@@ -102,7 +108,7 @@ EXPORTED = ("vca_version", "vca_last_error", "vca_ctx_create", "vca_ctx_destroy"
             "vca_ca_run", "vca_ca_enqueue", "vca_ca_status", "vca_ca_histories", "vca_ca_download", "vca_ca_work",
             "vca_ca_footprint", "vca_ca_tiles", "vca_batch_create", "vca_batch_destroy", "vca_batch_add",
             "vca_batch_remove", "vca_batch_set_graph", "vca_batch_enqueue", "vca_batch_run", "vca_batch_info",
-            "vca_batch_status", "vca_ca_collect", "vca_ca_flow_stats", "vca_ca_flow_points",
+            "vca_batch_status", "vca_ca_collect", "vca_ca_flow_stats", "vca_ca_flow_points", "vca_ca_autotune",
             "vca_mc_create", "vca_mc_create_from_ca", "vca_mc_destroy", "vca_mc_sweep", "vca_mc_counts_get",
             "vca_mc_measure", "vca_mc_download", "vca_mc_download_labels", "vca_mc_info",
             "vca_mc_download_mobility")
@@ -185,6 +191,22 @@ def ca_footprint(dims, params: CaParams, neighborhood: str = "moore", periodic: 
     b = C.c_uint64()
     _check(lib().vca_ca_footprint(C.byref(d), C.byref(b)))
     return int(b.value)
+
+
+def ca_autotune(ctx: "Context", dims, params: CaParams, neighborhood: str, periodic: bool,
+                height: Optional[np.ndarray], spill: Optional[np.ndarray], vents: Sequence[Sequence[int]],
+                probe_steps: int = 64) -> dict:
+    """vca_ca_autotune: the fastest (tile_shape, dense) over probe_steps steps of this simulation, and the
+    measured ms per step of every candidate, keyed (dense, tile_shape)."""
+    d = _ca_desc(dims, params, neighborhood, periodic)
+    v = (Vent * max(1, len(vents)))(*[Vent(*map(int, x)) for x in vents])
+    h = None if height is None else np.ascontiguousarray(height, dtype=np.int32)
+    s = None if spill is None else np.ascontiguousarray(spill, dtype=np.int32)
+    r = TuneResult()
+    _check(lib().vca_ca_autotune(ctx.h, C.byref(d), _ptr(h), _ptr(s), C.cast(v, C.c_void_p), len(vents), probe_steps,
+                                 C.byref(r)))
+    ms = {(dn, sh + 1): float(r.ms_per_step[dn][sh]) for dn in range(2) for sh in range(5) if r.ms_per_step[dn][sh] > 0}
+    return {"tile_shape": int(r.tile_shape), "dense": int(r.dense), "candidates": int(r.candidates), "ms": ms}
 
 
 class LavaCA:
