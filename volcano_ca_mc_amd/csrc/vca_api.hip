@@ -690,6 +690,8 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         const uint64_t ntiles = static_cast<uint64_t>(P.ntx) * P.nty * P.ntz;
         require(ntiles <= kTileMask, "za dużo kafli dla tej siatki");
         P.ntiles = static_cast<uint32_t>(ntiles);
+        magic_div(static_cast<uint32_t>(P.ntx), P.div_ntx_m, P.div_ntx_p);
+        magic_div(static_cast<uint32_t>(P.nty), P.div_nty_m, P.div_nty_p);
 
         // vents: flowId -> cell (isVent, LavaCA.hpp:335-341)
         int32_t maxf = 0;

@@ -46,6 +46,13 @@ __device__ inline unsigned long long wave_sum(unsigned long long v) {
     return v;
 }
 
+// v of lane `lane` (uniform) into scalar registers.
+__device__ inline uint64_t readlane64(uint64_t v, int lane) {
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v)), lane));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v >> 32)), lane));
+    return static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
+}
+
 // Wave-aggregated slot reservation: every lane of the wave must call this (pred may differ per lane).
 __device__ inline uint32_t wave_append(bool pred, uint32_t* counter) {
     const unsigned long long mask = __ballot(pred);
@@ -114,10 +121,11 @@ struct TileCoord {
     int x0, y0, z0;
 };
 __device__ inline TileCoord tile_origin(const CaParams& P, uint32_t t) {
-    const int tx = static_cast<int>(t % static_cast<uint32_t>(P.ntx));
-    const uint32_t r = t / static_cast<uint32_t>(P.ntx);
-    const int ty = static_cast<int>(r % static_cast<uint32_t>(P.nty));
-    const int tz = static_cast<int>(r / static_cast<uint32_t>(P.nty));
+    const uint32_t r = div_magic(t, P.div_ntx_m, P.div_ntx_p);
+    const int tx = static_cast<int>(t - r * static_cast<uint32_t>(P.ntx));
+    const uint32_t tz_ = div_magic(r, P.div_nty_m, P.div_nty_p);
+    const int ty = static_cast<int>(r - tz_ * static_cast<uint32_t>(P.nty));
+    const int tz = static_cast<int>(tz_);
     return TileCoord{tx * kTileX, ty * P.ty, tz * P.tz};
 }
 // Cell k (0..3) of thread threadIdx.x inside the tile.
@@ -150,61 +158,63 @@ __device__ int drop_depth(const CaDev& D, uint32_t col, uint32_t plane, int z) {
     return kMaxDrop;
 }
 
-// Candidate vote with 4 register slots (engine/src/ca/CaRules.hpp:127-183).  More than 4 distinct flowIds
-// among the candidates sets `overflow`; the caller then re-runs the gather with VoteWide.
+// h % m for a small m >= 1 (m < 2^16) without the 64-bit division routine: 2^32 = q*m + r32 gives
+// h = hi * 2^32 + lo == (hi mod m) * r32 + lo mod m  (mod m).
+__device__ inline uint32_t mod_small(uint64_t h, uint32_t m) {
+    const uint32_t r32 = (0u - m) % m;  // 2^32 mod m
+    const uint32_t a = static_cast<uint32_t>(h >> 32) % m, b = static_cast<uint32_t>(h) % m;
+    return (a * r32 + b) % m;
+}
+
+// Candidate vote with 4 register slots (engine/src/ca/CaRules.hpp:127-183), updated with selects only (no
+// per-lane branches).  More than 4 distinct flowIds among the candidates sets `overflow`; the caller then hands
+// the cell to a full vote.
 struct Vote4 {
     int n = 0;
     bool overflow = false;
     int32_t id[4];
     int cnt[4];
     float mt[4];
-    __device__ void add(int32_t f, float t) {
+    // add(f, t) when ok, nothing otherwise
+    __device__ void add_if(bool ok, int32_t f, float t) {
         bool found = false;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            if (s < n && id[s] == f) {
-                ++cnt[s];
-                if (mt[s] < t) mt[s] = t;  // std::max(a, b) == (a < b ? b : a)
-                found = true;
-            }
+            const bool m = ok && s < n && id[s] == f;
+            cnt[s] = m ? cnt[s] + 1 : cnt[s];
+            mt[s] = (m && mt[s] < t) ? t : mt[s];  // std::max(a, b) == (a < b ? b : a)
+            found = found || m;
         }
-        if (found) return;
-        if (n == 4) {
-            overflow = true;
-            return;
-        }
+        const bool ins = ok && !found;
+        overflow = overflow || (ins && n == 4);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            if (s == n) {
-                id[s] = f;
-                cnt[s] = 1;
-                mt[s] = t;
-            }
+            const bool here = ins && n == s;
+            id[s] = here ? f : id[s];
+            cnt[s] = here ? 1 : cnt[s];
+            mt[s] = here ? t : mt[s];
         }
-        ++n;
+        n = (ins && n < 4) ? n + 1 : n;
     }
+    __device__ void add(int32_t f, float t) { add_if(true, f, t); }
     // CandidateVote::winner: most votes; ties -> ascending flowId, pick hash % k.
     __device__ void winner(uint64_t h_index, int32_t& flow, float& temp) const {
         int best = 0;
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
-            if (s < n && cnt[s] > best) best = cnt[s];
+        for (int s = 0; s < 4; ++s) best = (s < n && cnt[s] > best) ? cnt[s] : best;
         int k = 0;
 #pragma unroll
         for (int s = 0; s < 4; ++s) k += (s < n && cnt[s] == best) ? 1 : 0;
-        int pick = 0;
-        if (n > 1 && k > 1) pick = static_cast<int>(splitmix64(h_index ^ kSaltTieBreak) % static_cast<uint64_t>(k));
+        const uint32_t pick =
+            (n > 1 && k > 1) ? mod_small(splitmix64(h_index ^ kSaltTieBreak), static_cast<uint32_t>(k)) : 0u;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            if (s < n && cnt[s] == best) {
-                int rank = 0;
+            uint32_t rank = 0;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) rank += (q < n && cnt[q] == best && id[q] < id[s]) ? 1 : 0;
-                if (rank == pick) {
-                    flow = id[s];
-                    temp = mt[s];
-                }
-            }
+            for (int q = 0; q < 4; ++q) rank += (q < n && cnt[q] == best && id[q] < id[s]) ? 1u : 0u;
+            const bool take = s < n && cnt[s] == best && rank == pick;
+            flow = take ? id[s] : flow;
+            temp = take ? mt[s] : temp;
         }
     }
 };
@@ -235,7 +245,7 @@ struct VoteWide {
         int k = 0;
         for (int s = 0; s < n; ++s) k += cnt[s] == best ? 1 : 0;
         int pick = 0;
-        if (n > 1 && k > 1) pick = static_cast<int>(splitmix64(h_index ^ kSaltTieBreak) % static_cast<uint64_t>(k));
+        if (n > 1 && k > 1) pick = static_cast<int>(mod_small(splitmix64(h_index ^ kSaltTieBreak), static_cast<uint32_t>(k)));
         for (int s = 0; s < n; ++s) {
             if (cnt[s] != best) continue;
             int rank = 0;
@@ -429,10 +439,11 @@ __global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B) {
             if (P.dense || lava || (P.classic && (f & kFlagLava) && (f & kFlagAir))) {
                 act = true;
             } else if (f & kFlagAir) {
-                const int tx = static_cast<int>(t % static_cast<uint32_t>(P.ntx));
-                const uint32_t r = t / static_cast<uint32_t>(P.ntx);
-                const int ty = static_cast<int>(r % static_cast<uint32_t>(P.nty));
-                const int tz = static_cast<int>(r / static_cast<uint32_t>(P.nty));
+                const uint32_t r = div_magic(t, P.div_ntx_m, P.div_ntx_p);
+                const int tx = static_cast<int>(t - r * static_cast<uint32_t>(P.ntx));
+                const uint32_t tzu = div_magic(r, P.div_nty_m, P.div_nty_p);
+                const int ty = static_cast<int>(r - tzu * static_cast<uint32_t>(P.nty));
+                const int tz = static_cast<int>(tzu);
                 uint8_t any = 0;  // all 26 neighbour flags are loaded together (no early exit)
 #pragma unroll
                 for (int q = 0; q < 27; ++q) {
@@ -481,18 +492,24 @@ constexpr uint32_t kNoCell = 0xFFFFFFFFu;
 
 // Global index of x = 0 in halo row r of the tile at o, or kNoCell when the row holds no cells.  Rows past the
 // far side of a partial tile (y > ny, ...) neighbour no cell and hold none either.
+// Coordinate v in [-1, n] of a halo position: its wrapped value, or -1 when it holds no cell (absorbing axis,
+// or past the far side of a partial tile).  Selects only.
+__device__ inline int halo_coord(int v, int n, int per) {
+    const bool in = static_cast<unsigned>(v) < static_cast<unsigned>(n);
+    const bool ok = v <= n && (in || per);
+    const int w = in ? v : (v < 0 ? v + n : v - n);
+    return ok ? w : -1;
+}
 template <int TY, int TZ>
 __device__ inline uint32_t halo_row(const CaParams& P, const TileCoord& o, int r) {
-    int gy = o.y0 - 1 + r % Halo<TY, TZ>::LY, gz = o.z0 - 1 + r / Halo<TY, TZ>::LY;
-    if (gy > P.ny || gz > P.nz || !wrap_axis(gy, P.ny, P.per[1]) || !wrap_axis(gz, P.nz, P.per[2])) return kNoCell;
-    return static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(gy) +
-           static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny) * static_cast<uint32_t>(gz);
+    const int gy = halo_coord(o.y0 - 1 + r % Halo<TY, TZ>::LY, P.ny, P.per[1]);
+    const int gz = halo_coord(o.z0 - 1 + r / Halo<TY, TZ>::LY, P.nz, P.per[2]);
+    const uint32_t base = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(gy) +
+                          static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny) * static_cast<uint32_t>(gz);
+    return (gy < 0 || gz < 0) ? kNoCell : base;
 }
 // Global x of halo x position gx in [-1, nx], or -1 when it holds no cell.
-__device__ inline int halo_x(const CaParams& P, int gx) {
-    if (gx > P.nx || !wrap_axis(gx, P.nx, P.per[0])) return -1;
-    return gx;
-}
+__device__ inline int halo_x(const CaParams& P, int gx) { return halo_coord(gx, P.nx, P.per[0]); }
 
 // Halo of one tile in LDS with per-row lava masks: bit lx of lava[r] = halo column lx + 1 of row r holds
 // LAVA; edge[2r] / edge[2r + 1] = the left / right edge column of row r holds LAVA.
@@ -507,29 +524,42 @@ struct HaloTypes {
 // the spill heights of the LX x LY halo columns (0 where a column holds no cells).  Every load of a thread
 // is issued before its first LDS store, so a tile costs one memory latency.  Returns whether this thread saw
 // lava; own_air |= this thread saw AIR among the tile's own cells.
+// Halo row bases of the tile at o, one per lane: lane j < R holds halo_row(j).  Rows are then read with
+// v_readlane (scalar row index) instead of each wave redoing the row arithmetic in scalar code.
+template <int TY, int TZ>
+__device__ inline uint32_t halo_row_lanes(const CaParams& P, const TileCoord& o) {
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    return lane < Halo<TY, TZ>::R ? halo_row<TY, TZ>(P, o, lane) : kNoCell;
+}
+__device__ inline uint32_t row_of(uint32_t row_base, int r) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(row_base), r));
+}
+
 template <int TY, int TZ>
 __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileCoord& o, HaloTypes<TY, TZ>& S,
-                                  bool with_spill, int32_t* s_spill, int& own_air) {
+                                  bool with_spill, int32_t* s_spill, int& own_air, uint32_t row_base, int gx) {
     using H = Halo<TY, TZ>;
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     const int lx = static_cast<int>(threadIdx.x & 63u);
     const int t = static_cast<int>(threadIdx.x);
-    const int gx = halo_x(P, o.x0 + lx);  // x == nx is the wrapped x = 0 of a periodic partial tile
     uint32_t v[H::KR];  // 32-bit so that no load result is packed (and waited for) before the stores
 #pragma unroll
     for (int k = 0; k < H::KR; ++k) {
         const int r = w + 4 * k;
         v[k] = kVoidLds;
         if (r < H::R) {
-            const uint32_t base = halo_row<TY, TZ>(P, o, r);
+            const uint32_t base = row_of(row_base, r);
             if (base != kNoCell && gx >= 0) v[k] = D.type[base + static_cast<uint32_t>(gx)];
         }
     }
     uint32_t e = kVoidLds;
-    if (t < 2 * H::R) {
-        const uint32_t base = halo_row<TY, TZ>(P, o, t >> 1);
-        const int ex = halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1);
-        if (base != kNoCell && ex >= 0) e = D.type[base + static_cast<uint32_t>(ex)];
+    {
+        // thread t < 2R: edge column (t & 1) of row t >> 1 (rows 0..31 in wave 0, 32..35 in wave 1)
+        const uint32_t base = static_cast<uint32_t>(__shfl(static_cast<int>(row_base), (t >> 1) & 63));
+        if (t < 2 * H::R) {
+            const int ex = halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1);
+            if (base != kNoCell && ex >= 0) e = D.type[base + static_cast<uint32_t>(ex)];
+        }
     }
     int32_t sp[H::KC];
     if (with_spill) {
@@ -616,7 +646,9 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
         const TileCoord o = tile_origin(P, e & kTileMask);
         __syncthreads();  // the previous tile's LDS reads are done
         int own_air = 0;
-        stage_rows<TY, TZ>(P, D, o, S, has_spill, s_spill, own_air);
+        const uint32_t row_base = halo_row_lanes<TY, TZ>(P, o);
+        const int gx = halo_x(P, o.x0 + static_cast<int>(threadIdx.x & 63u));
+        stage_rows<TY, TZ>(P, D, o, S, has_spill, s_spill, own_air, row_base, gx);
         if (threadIdx.x == 0) s_n = 0;
         __syncthreads();
         {
@@ -682,6 +714,9 @@ struct RulesTile {
     int32_t flow[H::N];
     float temp[H::N];
     int32_t spill[H::LXY];
+    int16_t koff[27];           // LDS offset of neighbour position k: dx + LX * dy + LXY * dz
+    // LAVA bits of halo row r over all LX = 66 positions (bit p = halo column p), as three 32-bit parts + 0
+    uint32_t row66[H::R][4];
 };
 
 // gatherLavaCandidates (LavaCA.hpp:465-519) / the classic vote (LavaCA.hpp:1165-1202) for the AIR cell at LDS
@@ -691,50 +726,47 @@ template <int TY, int TZ, class V>
 __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY, TZ>& T, int c, int lz, int z,
                            uint32_t col, uint32_t plane, int spill_c, bool has_spill, uint32_t bits, V& v) {
     constexpr int LX = Halo<TY, TZ>::LX, LXY = Halo<TY, TZ>::LXY;
-    int cdrop = -100;
+    if (P.classic) {
+        #pragma clang loop unroll(disable)
+        while (bits) {
+            const int k = __ffs(bits) - 1;
+            bits &= bits - 1;
+            v.add(T.flow[c + T.koff[k]], 0.0f);
+        }
+        return;
+    }
+    // dropDepth of the cell, once, when a side or diagonal-up neighbour may need it (pure function of the cell)
+    const int cdrop = (bits & ~((1u << 4) | (1u << 22)))
+                          ? drop_depth_lds<TY, TZ>(D, T.S.type, c, lz, z, col, plane, spill_c, has_spill)
+                          : -100;
     #pragma clang loop unroll(disable)
     while (bits) {
         const int k = __ffs(bits) - 1;
         bits &= bits - 1;
-        const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
-        const int n = c + dx + LX * dy + LXY * dz;
-        if (P.classic) {
-            v.add(T.flow[n], 0.0f);
-            continue;
-        }
-        float cooling;
-        if (dz < 0) {
-            // tryPonding (LavaCA.hpp:443-461); only k == 4 (directly below) is in the mask in lava mode
-            if (z > spill_c || z == 0) continue;
-            bool ok = true;
-            if (z - 1 > 0) {  // hasSupport(z - 1)
+        const int n = c + T.koff[k];  // dx + LX * dy + LXY * dz of position k
+        // k >= 18: dz > 0 (k == 22 directly above); k == 4: directly below (ponding); else dz == 0
+        const bool up = k >= 18;
+        const int d = T.desc[n];
+        bool ok = d >= 0 && (up ? (cdrop == 0 && d == 1) : (cdrop == d && !(d == 1 && P.nb.diagonal_falls)));
+        ok = k == 22 || ok;
+        if (k == 4) {
+            // tryPonding (LavaCA.hpp:443-461); only k == 4 of the dz < 0 positions is in the mask in lava mode
+            ok = !(z > spill_c || z == 0);
+            if (ok && z - 1 > 0) {  // hasSupport(z - 1)
                 const uint8_t b2 = lz >= 1 ? T.S.type[c - 2 * LXY] : D.type[col + plane * static_cast<uint32_t>(z - 2)];
                 ok = b2 == kRock || b2 == kBasalt || (b2 == kLava && has_spill && z - 2 < spill_c);
             }
-            if (!ok) continue;
-            const uint8_t h0 = T.S.type[n - 1], h1 = T.S.type[n + 1];
-            bool closed = h0 != kAir && h0 != kVoidLds && h1 != kAir && h1 != kVoidLds;
-            if (P.ny > 1) {
-                const uint8_t h2 = T.S.type[n - LX], h3 = T.S.type[n + LX];
-                closed = closed && h2 != kAir && h2 != kVoidLds && h3 != kAir && h3 != kVoidLds;
-            }
-            if (!closed) continue;
-            cooling = P.cool_lat;
-        } else if (dz > 0 && dx == 0 && dy == 0) {
-            cooling = P.cool_fall;
-        } else {
-            const int d = T.desc[n];
-            if (d < 0) continue;
-            if (cdrop == -100) cdrop = drop_depth_lds<TY, TZ>(D, T.S.type, c, lz, z, col, plane, spill_c, has_spill);
-            if (dz > 0) {
-                if (cdrop != 0 || d != 1) continue;
-                cooling = P.cool_fall;
-            } else {
-                if (cdrop != d || (d == 1 && P.nb.diagonal_falls)) continue;
-                cooling = P.cool_lat;
+            if (ok) {
+                const uint8_t h0 = T.S.type[n - 1], h1 = T.S.type[n + 1];
+                bool closed = h0 != kAir && h0 != kVoidLds && h1 != kAir && h1 != kVoidLds;
+                if (P.ny > 1) {
+                    const uint8_t h2 = T.S.type[n - LX], h3 = T.S.type[n + LX];
+                    closed = closed && h2 != kAir && h2 != kVoidLds && h3 != kAir && h3 != kVoidLds;
+                }
+                ok = closed;
             }
         }
-        v.add(T.flow[n], T.temp[n] - cooling);
+        v.add_if(ok, T.flow[n], T.temp[n] - (up ? P.cool_fall : P.cool_lat));
     }
 }
 
@@ -753,6 +785,10 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
     __shared__ uint32_t s_nchg, s_nlist, s_nwide;
     __shared__ uint32_t s_red[2][kBlock / 64];
     __shared__ uint32_t s_wflags[2];  // per-wave flag bytes: [0] staging, [1] end of tile
+    if (threadIdx.x < 27) {
+        const int k = static_cast<int>(threadIdx.x);
+        T.koff[k] = static_cast<int16_t>((k % 3 - 1) + LX * ((k / 3) % 3 - 1) + H::LXY * (k / 9 - 1));
+    }
     const uint64_t* tiles = B.active + shape * B.cap;
     // the first entry is read together with the list length (blockIdx.x < gridDim.x <= cap: in bounds)
     uint64_t next = tiles[blockIdx.x];
@@ -779,7 +815,11 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
         const TileCoord o = tile_origin(P, tile);
         __syncthreads();  // the previous tile's LDS reads are done
         int own_air = 0;
-        const bool saw_lava = stage_rows<TY, TZ>(P, D, o, T.S, has_spill, T.spill, own_air);
+        // lane j: base of halo row j; gx: this lane's column (x == nx is the wrapped x = 0 of a periodic partial
+        // tile, -1 for no cell)
+        const uint32_t row_base = halo_row_lanes<TY, TZ>(P, o);
+        const int gx = halo_x(P, o.x0 + static_cast<int>(threadIdx.x & 63u));
+        const bool saw_lava = stage_rows<TY, TZ>(P, D, o, T.S, has_spill, T.spill, own_air, row_base, gx);
         if (threadIdx.x == 0) {
             s_nchg = 0;
             s_nlist = 0;
@@ -821,7 +861,7 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
                 ds[k] = -1;
                 if (r < H::R && ((T.S.lava[r] >> lx) & 1u)) {
                     lava |= 1u << k;
-                    const uint32_t g = halo_row<TY, TZ>(P, o, r) + static_cast<uint32_t>(halo_x(P, o.x0 + lx));
+                    const uint32_t g = row_of(row_base, r) + static_cast<uint32_t>(gx);
                     fl[k] = D.flow[g];
                     if (!P.classic) {
                         tp[k] = D.temp[g];
@@ -833,9 +873,9 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
             float etp = 0.0f;
             int32_t eds = -1;
             const bool elava = t < 2 * H::R && T.S.edge[t];
+            const uint32_t ebase = static_cast<uint32_t>(__shfl(static_cast<int>(row_base), (t >> 1) & 63));
             if (elava) {
-                const uint32_t g = halo_row<TY, TZ>(P, o, t >> 1) +
-                                   static_cast<uint32_t>(halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1));
+                const uint32_t g = ebase + static_cast<uint32_t>(halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1));
                 efl = D.flow[g];
                 if (!P.classic) {
                     etp = D.temp[g];
@@ -857,17 +897,27 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
                 T.temp[q] = etp;
                 T.desc[q] = static_cast<int8_t>(eds);
             }
+            if (t < H::R) {
+                const uint64_t m = T.S.lava[t];
+                const uint64_t lo = (m << 1) | static_cast<uint64_t>(T.S.edge[2 * t]);
+                const uint32_t hi = static_cast<uint32_t>(m >> 63) | (static_cast<uint32_t>(T.S.edge[2 * t + 1]) << 1);
+                T.row66[t][0] = static_cast<uint32_t>(lo);
+                T.row66[t][1] = static_cast<uint32_t>(lo >> 32);
+                T.row66[t][2] = hi;
+                T.row66[t][3] = 0;
+            }
         }
         int tile_lava = 0, tile_air = 0;
         {
             // candidate cells: LAVA, or AIR within one cell of a LAVA position (row masks of the 3 x 3 halo
             // rows around the cell's row, dilated by one in x); every other cell keeps its state
             const int lx = static_cast<int>(threadIdx.x & 63u);
-            const int w = static_cast<int>(threadIdx.x >> 6);
-#pragma unroll 1
-            for (int k = 0; k < 4; ++k) {
-                const int row = w + 4 * k;
-                const int rc = (row % TY + 1) + LY * (row / TY + 1);
+            const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+            // lane j < 16 folds the 3 x 3 halo rows around the tile's row j; each wave then reads its four rows'
+            // masks from those lanes
+            uint64_t near_j = 0;
+            if (lx < 16) {
+                const int rc = (lx % TY + 1) + LY * (lx / TY + 1);
                 uint64_t u = 0;
                 uint32_t el = 0, er = 0;
 #pragma unroll
@@ -879,8 +929,13 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
                         el |= T.S.edge[2 * rr];
                         er |= T.S.edge[2 * rr + 1];
                     }
-                const uint64_t near = u | (u << 1) | (u >> 1) | static_cast<uint64_t>(el) |
-                                      (static_cast<uint64_t>(er) << 63);
+                near_j = u | (u << 1) | (u >> 1) | static_cast<uint64_t>(el) | (static_cast<uint64_t>(er) << 63);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int row = w + 4 * k;
+                const int rc = (row % TY + 1) + LY * (row / TY + 1);
+                const uint64_t near = readlane64(near_j, row);
                 // positions past the grid's far side hold wrapped halo cells of a periodic partial tile
                 const bool inside = o.x0 + lx < P.nx && o.y0 + row % TY < P.ny && o.z0 + row / TY < P.nz;
                 const uint8_t ty = inside ? T.S.type[(lx + 1) + LX * rc] : kVoidLds;
@@ -934,15 +989,16 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
                     // LAVA among the 26 neighbours, three x positions at a time from the row masks
                     uint32_t lava_bits = 0;
                     const int rc = (ly + 1) + LY * (lz + 1);
+                    // halo positions lx .. lx + 2 of each 66-position row word: one funnel shift of two of its
+                    // three 32-bit parts
+                    const uint32_t* wbase = &T.row66[0][0] + (lx >> 5);
+                    const uint32_t sh = static_cast<uint32_t>(lx) & 31u;
 #pragma unroll
                     for (int dz = -1; dz <= 1; ++dz)
 #pragma unroll
                         for (int dy = -1; dy <= 1; ++dy) {
                             const int rr = rc + dy + LY * dz;
-                            const uint64_t m = T.S.lava[rr];
-                            const uint64_t sh = lx == 0 ? ((m << 1) | T.S.edge[2 * rr]) : (m >> (lx - 1));
-                            uint32_t b3 = static_cast<uint32_t>(sh) & 7u;
-                            if (lx == 63) b3 = (b3 & 3u) | (static_cast<uint32_t>(T.S.edge[2 * rr + 1]) << 2);
+                            const uint32_t b3 = __builtin_amdgcn_alignbit(wbase[4 * rr + 1], wbase[4 * rr], sh) & 7u;
                             lava_bits |= b3 << (3 * (dy + 1) + 9 * (dz + 1));
                         }
                     lava_bits &= rules_mask & ~(1u << 13);
@@ -954,8 +1010,8 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
                         const uint64_t h_index = splitmix64(prefix ^ i);
                         uint32_t mask = P.nb.mask[0];
                         if (P.nb.variants > 1)
-                            mask = nb_mask(P.nb, static_cast<uint32_t>(splitmix64(h_index ^ kSaltNeighborhood) %
-                                                              static_cast<uint64_t>(P.nb.variants)));
+                            mask = nb_mask(P.nb, mod_small(splitmix64(h_index ^ kSaltNeighborhood),
+                                                           static_cast<uint32_t>(P.nb.variants)));
                         const uint32_t bits = lava_bits & mask;
                         if (bits) {
                             const int spill_c = has_spill ? T.spill[(lx + 1) + LX * (ly + 1)] : 0;
@@ -1058,8 +1114,8 @@ __global__ void __launch_bounds__(kBlock) ca_wide_kernel(BatchDev B) {
             const uint64_t h_index = splitmix64(prefix ^ i);
             uint32_t mask = P.nb.mask[0];
             if (P.nb.variants > 1)
-                mask = nb_mask(P.nb, static_cast<uint32_t>(splitmix64(h_index ^ kSaltNeighborhood) %
-                                                              static_cast<uint64_t>(P.nb.variants)));
+                mask = nb_mask(P.nb, mod_small(splitmix64(h_index ^ kSaltNeighborhood),
+                                               static_cast<uint32_t>(P.nb.variants)));
             VoteWide v;
             if (P.classic)
                 gather_classic(P, D, x, y, z, mask, v);
@@ -1223,7 +1279,9 @@ __device__ inline void mc_trial(const McParams& P, S* spin, uint32_t g, uint64_t
     int x, y, z;
     cell_xyz(P, g, x, y, z);
     const int32_t cur = static_cast<int32_t>(spin[g]);
-    int32_t prop = 1 + static_cast<int32_t>(splitmix64(h ^ kSaltMcOrient) % static_cast<uint64_t>(P.Q - 1));
+    const uint64_t ho = splitmix64(h ^ kSaltMcOrient);
+    const uint32_t qm = static_cast<uint32_t>(P.Q - 1);
+    int32_t prop = 1 + static_cast<int32_t>(qm < 65536u ? mod_small(ho, qm) : ho % qm);
     if (prop >= cur) ++prop;
     const uint32_t mask = P.nb.mask[var];
     int in_dom = 0, same_a = 0, same_b = 0;
@@ -1261,7 +1319,8 @@ __global__ void __launch_bounds__(kBlock) mc_color_kernel(McParams P, S* spin, c
         const uint64_t h = splitmix64(prefix ^ g);
         if (mob_thr == nullptr || (splitmix64(h ^ kSaltMcMobility) >> 11) < mob_thr[j]) {
             const int var = P.nb.variants > 1
-                                ? static_cast<int>(splitmix64(h ^ kSaltNeighborhood) % static_cast<uint64_t>(P.nb.variants))
+                                ? static_cast<int>(mod_small(splitmix64(h ^ kSaltNeighborhood),
+                                                             static_cast<uint32_t>(P.nb.variants)))
                                 : 0;
             mc_trial(P, spin, g, h, var, accept, c[0], c[1], c[2]);
         }
@@ -1281,8 +1340,8 @@ __global__ void mc_random_kernel(McParams P, S* spin, const uint32_t* list, cons
         const uint32_t g = list[kk];
         if (mob_thr != nullptr && (splitmix64(h ^ kSaltMcMobility) >> 11) >= mob_thr[kk]) continue;
         const int var = P.nb.variants > 1
-                            ? static_cast<int>(splitmix64(splitmix64(prefix ^ g) ^ kSaltNeighborhood) %
-                                               static_cast<uint64_t>(P.nb.variants))
+                            ? static_cast<int>(mod_small(splitmix64(splitmix64(prefix ^ g) ^ kSaltNeighborhood),
+                                                         static_cast<uint32_t>(P.nb.variants)))
                             : 0;
         mc_trial(P, spin, g, h, var, accept, att, acc, up);
     }
@@ -1317,7 +1376,8 @@ __global__ void __launch_bounds__(kBlock) mc_energy_kernel(McParams P, const S* 
         if (P.nb.variants == 1) {
             mask = P.nb.mask[0] & ~((1u << 14) - 1u);
         } else {
-            mask = P.nb.mask[splitmix64(splitmix64(prefix ^ g) ^ kSaltNeighborhood) % static_cast<uint64_t>(P.nb.variants)];
+            mask = nb_mask(P.nb, mod_small(splitmix64(splitmix64(prefix ^ g) ^ kSaltNeighborhood),
+                                           static_cast<uint32_t>(P.nb.variants)));
         }
         int in_dom = 0, same = 0;
 #pragma unroll

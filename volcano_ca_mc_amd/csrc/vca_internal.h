@@ -71,8 +71,23 @@ struct CaParams {
     int32_t ty, tz;               // tile extent in y and z
     int32_t ntx, nty, ntz;        // tiles per axis
     uint32_t ntiles;
+    // t / ntx and r / nty for t, r < 2^23 as (x * m) >> p (exact; see magic_div)
+    uint32_t div_ntx_m, div_nty_m;
+    int32_t div_ntx_p, div_nty_p;
     int32_t nvent_slots;
 };
+
+// Division by an invariant d >= 1 for dividends x < 2^23: p = 23 + ceil(log2 d), m = ceil(2^p / d) (< 2^24);
+// then floor(x / d) == (x * m) >> p, because m * d - 2^p < d <= 2^(p - 23) and x < 2^23.
+inline void magic_div(uint32_t d, uint32_t& m, int32_t& p) {
+    int l = 0;
+    while ((1ULL << l) < d) ++l;
+    p = 23 + l;
+    m = static_cast<uint32_t>(((1ULL << p) + d - 1) / d);
+}
+__host__ __device__ inline uint32_t div_magic(uint32_t x, uint32_t m, int32_t p) {
+    return static_cast<uint32_t>((static_cast<uint64_t>(x) * m) >> p);
+}
 
 // One change of one cell in one step (applied after every rule of the step was evaluated).
 struct Change {
