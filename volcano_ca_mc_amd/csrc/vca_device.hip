@@ -844,6 +844,7 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
             }
             continue;
         }
+        int tile_lava = 0, tile_air = 0;
         {
             // flowId / temperature / D(n) of the LAVA positions, all loads in flight before the stores
             const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
@@ -882,6 +883,45 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
                     eds = D.descent[g];
                 }
             }
+            {
+                // candidate cells (while the lava loads above are in flight): LAVA, or AIR next to a LAVA position
+                // the rules read -- the 3 x 3 halo rows around the cell's row dilated by one in x in classic mode;
+                // in lava mode the rows of the cell's layer and the one above (dz >= 0) plus, with ponding, the cell
+                // directly below.  Every other cell keeps its state.
+                // lane j < 16 folds the halo rows around the tile's row j; each wave then reads its four rows' masks
+                // from those lanes
+                uint64_t near_j = 0;
+                if (lx < 16) {
+                    const int rc = (lx % TY + 1) + LY * (lx / TY + 1);
+                    uint64_t u = 0;
+                    uint32_t el = 0, er = 0;
+#pragma unroll
+                    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+                        for (int dy = -1; dy <= 1; ++dy) {
+                            if (dz < 0 && !P.classic) continue;
+                            const int rr = rc + dy + LY * dz;
+                            u |= T.S.lava[rr];
+                            el |= T.S.edge[2 * rr];
+                            er |= T.S.edge[2 * rr + 1];
+                        }
+                    near_j = u | (u << 1) | (u >> 1) | static_cast<uint64_t>(el) | (static_cast<uint64_t>(er) << 63);
+                    if (!P.classic && P.ponding) near_j |= T.S.lava[rc - LY];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int row = w + 4 * k;
+                    const int rc = (row % TY + 1) + LY * (row / TY + 1);
+                    const uint64_t near = readlane64(near_j, row);
+                    // positions past the grid's far side hold wrapped halo cells of a periodic partial tile
+                    const bool inside = o.x0 + lx < P.nx && o.y0 + row % TY < P.ny && o.z0 + row / TY < P.nz;
+                    const uint8_t ty = inside ? T.S.type[(lx + 1) + LX * rc] : kVoidLds;
+                    const bool cand = ty == kLava || (ty == kAir && ((near >> lx) & 1u));
+                    if (ty == kAir && !cand) tile_air = 1;
+                    const uint32_t slot = wave_append(cand, &s_nlist);
+                    if (cand) T.list[slot] = static_cast<uint16_t>(lx + 64 * row);
+                }
+            }
 #pragma unroll
             for (int k = 0; k < H::KR; ++k) {
                 if ((lava >> k) & 1u) {
@@ -905,44 +945,6 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
                 T.row66[t][1] = static_cast<uint32_t>(lo >> 32);
                 T.row66[t][2] = hi;
                 T.row66[t][3] = 0;
-            }
-        }
-        int tile_lava = 0, tile_air = 0;
-        {
-            // candidate cells: LAVA, or AIR within one cell of a LAVA position (row masks of the 3 x 3 halo
-            // rows around the cell's row, dilated by one in x); every other cell keeps its state
-            const int lx = static_cast<int>(threadIdx.x & 63u);
-            const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-            // lane j < 16 folds the 3 x 3 halo rows around the tile's row j; each wave then reads its four rows'
-            // masks from those lanes
-            uint64_t near_j = 0;
-            if (lx < 16) {
-                const int rc = (lx % TY + 1) + LY * (lx / TY + 1);
-                uint64_t u = 0;
-                uint32_t el = 0, er = 0;
-#pragma unroll
-                for (int dz = -1; dz <= 1; ++dz)
-#pragma unroll
-                    for (int dy = -1; dy <= 1; ++dy) {
-                        const int rr = rc + dy + LY * dz;
-                        u |= T.S.lava[rr];
-                        el |= T.S.edge[2 * rr];
-                        er |= T.S.edge[2 * rr + 1];
-                    }
-                near_j = u | (u << 1) | (u >> 1) | static_cast<uint64_t>(el) | (static_cast<uint64_t>(er) << 63);
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int row = w + 4 * k;
-                const int rc = (row % TY + 1) + LY * (row / TY + 1);
-                const uint64_t near = readlane64(near_j, row);
-                // positions past the grid's far side hold wrapped halo cells of a periodic partial tile
-                const bool inside = o.x0 + lx < P.nx && o.y0 + row % TY < P.ny && o.z0 + row / TY < P.nz;
-                const uint8_t ty = inside ? T.S.type[(lx + 1) + LX * rc] : kVoidLds;
-                const bool cand = ty == kLava || (ty == kAir && ((near >> lx) & 1u));
-                if (ty == kAir && !cand) tile_air = 1;
-                const uint32_t slot = wave_append(cand, &s_nlist);
-                if (cand) T.list[slot] = static_cast<uint16_t>(lx + 64 * row);
             }
         }
         __syncthreads();  // list and lava data of the tile are in LDS
