@@ -14,7 +14,8 @@ statistics and file output are outside them (output.format = none).
 Printed: one JSON line.  value = CA cell-updates/s = cells x executed steps / caSim (the reference's
 "ns per cell and step" unit); mc_value = MC spin-attempts/s = domain cells x sweeps / mcSim.
 roofline: the dominant kernel's algorithmic bytes per launch (26 B per processed cell-step for the CA
-kernels) / its mean launch time from HIP events recorded on the engine's stream during the timed runs;
+kernels) / its mean launch time from HIP events recorded on the engine's stream during the timed runs (on every
+10th CA step: events on every launch would add their own cost to caSim);
 traffic = measured HBM bytes per launch of that kernel from the committed rocprofv3 counter summary.
 cpu_baseline: the reference engine (oracle/_ref/engine, built from the reference sources) on the host
 cores, on a bounded sample (the full CA phase, the first 20 sweeps) of the same workload.
@@ -36,6 +37,7 @@ sys.path.insert(0, ROOT)
 ENGINE = os.path.join(ROOT, "volcano_ca_mc_amd", "_lib", "engine")
 REF_ENGINE = os.path.join(ROOT, "oracle", "_ref", "engine")
 HBM_PEAK_GBS = 8000.0
+TIMING_SAMPLE = 10  # kernel events on every 10th CA step and MC sweep
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
 
 
@@ -216,7 +218,9 @@ def main() -> int:
     cells = nx * ny * nz
     with tempfile.TemporaryDirectory(prefix="vca_bench_") as tmp:
         cfg_path = wl.write(tmp)
-        extra = ["--timing-detail", "--set", "output.format=none"]
+        # kernel times from HIP events on every TIMING_SAMPLE-th CA step / MC sweep of the timed jobs: timing every
+        # launch would add its own events' cost to the measured caSim
+        extra = ["--timing-sample", str(TIMING_SAMPLE), "--set", "output.format=none"]
         runs = []
         for i in range(a.warmup + a.steps):
             t = run_engine(ENGINE, cfg_path, os.path.join(tmp, f"run{i}"), extra)
@@ -255,7 +259,7 @@ def main() -> int:
         w = runs[0].get("detail", {}).get("caWork", {})
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_k,
-                    "kernel_mean_ms": round(mean_ms, 5), "algorithmic_bytes_per_launch": round(bytes_per_launch[dom_k]),
+                    "kernel_mean_ms": round(mean_ms, 5), "timed_every_nth_step": TIMING_SAMPLE, "algorithmic_bytes_per_launch": round(bytes_per_launch[dom_k]),
                     "bytes_per_unit": f"{CA_BYTES_PER_CELL} B per processed cell-step" if dom_k == "ca_rules" else None,
                     "processed_cells_per_launch": round(w.get("tilesVisited", 0) * w.get("tileCells", 0) /
                                                         max(1, steps)),

@@ -238,8 +238,10 @@ typedef struct {
     double ms;          /* summed device time of the kernel */
     uint64_t launches;  /* launches timed */
 } vca_kernel_time;
-/* Enables per-launch HIP-event timing of every kernel of the CA step and the MC sweep.  Events come from a
- * pool and are read back at the synchronisation points the calls already have (no extra host sync). */
+/* Enables per-launch HIP-event timing of the kernels of the CA step and the MC sweep: on = 1 times every launch,
+ * on = N > 1 the launches of every N-th CA step and every N-th MC sweep (the averages per launch then come from a
+ * regular sample of the run at a fraction of the events' own cost), 0 turns timing off.  Events come from a pool
+ * and are read back at the synchronisation points the calls already have (no extra host sync). */
 int vca_ctx_set_timing(vca_ctx* ctx, int32_t on);
 /* name: "ca_front", "ca_descent", "ca_rules", "ca_wide", "ca_apply", "ca_finish", "ca_step" (whole step),
  * "mc_color" (one colour launch), "mc_sweep" (whole sweep) */

@@ -34,6 +34,7 @@ std::string usage() {
            "                     od chwili at, wspólne kroki GPU; serve_summary.json z opóźnieniami p50/p99\n"
            "  --graph-steps N    długość grafu HIP kroków wsadowych (domyślnie 16, 0 = bez grafów)\n"
            "  --max-jobs N       najwyżej N zadań naraz w potoku wsadowym (1 = po jednym; domyślnie bez limitu)\n"
+           "  --timing-sample N  jak --timing-detail, ale czasy kerneli z co N-tego kroku CA i przemiatania MC\n"
            "  --autotune N       przed przebiegiem CA: N kroków próbnych na każdy kształt kafla i tryb frontu,\n"
            "                     przebieg z najszybszym (wyniki bez zmian; wybór w times.json)\n"
            "  --progress         linie PROGRESS ca/mc/save/batch na stdout (dla GUI)\n"
@@ -99,6 +100,16 @@ CliOptions parse_cli(int argc, const char* const* argv) {
         else if (a == "--high-priority") opt.overrides.push_back("parallel.priority=high");
         else if (a == "--benchmark") opt.benchmark = true;
         else if (a == "--timing-detail") opt.timing_detail = true;
+        else if (a == "--timing-sample") {
+            const std::string v = value(i, a);
+            try {
+                opt.timing_every = std::stoi(v);
+            } catch (...) {
+                throw CliError("--timing-sample wymaga liczby, podano: " + v);
+            }
+            if (opt.timing_every < 1) throw CliError("--timing-sample musi być >= 1");
+            opt.timing_detail = true;
+        }
         else if (a == "--print-defaults") opt.print_defaults = true;
         else if (a == "--validate") opt.validate = true;
         else if (a == "--progress") opt.progress = true;

@@ -863,12 +863,13 @@ int run_from_cli(const CliOptions& opt, const Clock& since_start) {
     }
     fs::create_directories(out_dir);
     Gpu gpu;
-    if (opt.timing_detail) vca_check(vca_ctx_set_timing(gpu.ctx(), 1), "GPU");
+    if (opt.timing_detail) vca_check(vca_ctx_set_timing(gpu.ctx(), std::max(1, opt.timing_every)), "GPU");
     // --timing-detail: device time of every kernel (HIP events on the engine's stream) and CA work counters
     auto detail = [&](const RunSummary* s) {
         if (!opt.timing_detail) return;
         Json d = Json::object();
         d["unit"] = "ms";
+        d["sampleEvery"] = std::max(1, opt.timing_every);  // launches of every N-th CA step / MC sweep timed
         Json k = Json::object();
         for (const char* name : {"ca_front", "ca_descent", "ca_rules", "ca_wide", "ca_apply", "ca_finish", "ca_step",
                                   "mc_color", "mc_sweep"}) {

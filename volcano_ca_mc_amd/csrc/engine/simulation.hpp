@@ -22,6 +22,7 @@ namespace eng {
 
 struct CliOptions {
     bool help = false, timing_detail = false, print_defaults = false, validate = false, progress = false;
+    int timing_every = 1;             // --timing-sample N: --timing-detail on every N-th CA step / MC sweep
     bool thread_stats = false, benchmark = false, quick = false;
     bool gpu_batch = false;           // --gpu-batch: --batch in one process through the batched pipeline
     int steps_per_graph = 16;         // --graph-steps: hipGraph length of the batched CA steps (0 = direct)

@@ -206,6 +206,11 @@ struct vca_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     bool timing = false;
+    // timing period: the launches of every timing_every-th CA step and MC sweep are timed (1 = all of them)
+    uint32_t timing_every = 1;
+    uint64_t ca_tick = 0, mc_tick = 0;
+    bool time_ca_step() { return timing && (ca_tick++ % timing_every) == 0; }
+    bool time_mc_sweep() { return timing && (mc_tick++ % timing_every) == 0; }
     std::map<std::string, KernelTime> times;
     // Event pool for per-launch timing: events are recorded around launches without any host wait and
     // drained (elapsed times summed per kernel name) at the next point where the host synchronises anyway.
@@ -463,7 +468,7 @@ struct Batch {
         const uint32_t sh = shapes();
         const bool lm = any_lava_mode();
         for (int k = 0; k < steps; ++k) {
-            if (!c->timing) {
+            if (!c->time_ca_step()) {
                 ca_launch_step(B, sh, lm, grid_tiles, c->stream, nullptr);
                 continue;
             }
@@ -622,7 +627,10 @@ extern "C" int vca_ctx_set_timing(vca_ctx* ctx, int32_t on) {
         require(ctx, "ctx == NULL");
         ctx->set_device();
         ctx->drain();
+        require(on >= 0, "invalid argument");
         ctx->timing = on != 0;
+        ctx->timing_every = on > 1 ? static_cast<uint32_t>(on) : 1u;
+        ctx->ca_tick = ctx->mc_tick = 0;
         if (ctx->timing) ctx->times.clear();
     });
 }
@@ -1592,22 +1600,23 @@ extern "C" int vca_mc_sweep(vca_mc* mc, int32_t nsweeps, vca_mc_counts* counts) 
         const uint64_t* thr = mc->has_mob_dyn ? mc->mob_thr.as<uint64_t>() : nullptr;
         for (int k = 0; k < nsweeps; ++k) {
             const uint64_t prefix = hash_prefix(mc->P.seed, static_cast<uint64_t>(mc->sweeps));
-            const size_t t0 = c->timing ? c->record() : 0;
+            const bool tm = c->time_mc_sweep();
+            const size_t t0 = tm ? c->record() : 0;
             if (mc->checkerboard) {
                 for (int col = 0; col < mc->colors; ++col) {
                     const uint32_t b = mc->color_start[static_cast<size_t>(col)];
                     const uint32_t e = mc->color_start[static_cast<size_t>(col) + 1];
-                    const size_t k0 = c->timing ? c->record() : 0;
+                    const size_t k0 = tm ? c->record() : 0;
                     mc_launch_color(mc->P, mc->spin.p, mc->spin_bytes, mc->list.as<uint32_t>() + b, thr ? thr + b : nullptr,
                                     e - b, prefix, mc->accept.as<uint64_t>(),
                                     mc->counters.as<unsigned long long>(), s);
-                    if (c->timing && e > b) c->mark("mc_color", k0, c->record());
+                    if (tm && e > b) c->mark("mc_color", k0, c->record());
                 }
             } else {
                 mc_launch_random(mc->P, mc->spin.p, mc->spin_bytes, mc->list.as<uint32_t>(), thr, mc->ndomain, prefix,
                                  mc->accept.as<uint64_t>(), mc->counters.as<unsigned long long>(), s);
             }
-            if (c->timing) {
+            if (tm) {
                 c->mark("mc_sweep", t0, c->record());
                 if (c->used > 4096) c->drain();
             }

@@ -46,6 +46,15 @@ __device__ inline unsigned long long wave_sum(unsigned long long v) {
     return v;
 }
 
+// threadIdx.x as a value the compiler cannot hoist out of a loop: per-lane index arithmetic derived from it is
+// recomputed where it is used (a few instructions) instead of being kept live -- and spilled -- across a loop.
+// The asm block is empty (no instruction, no memory access).
+__device__ inline unsigned opaque_tid() {
+    unsigned t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 // v of lane `lane` (uniform) into scalar registers.
 __device__ inline uint64_t readlane64(uint64_t v, int lane) {
     const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v)), lane));
@@ -528,7 +537,7 @@ struct HaloTypes {
 // v_readlane (scalar row index) instead of each wave redoing the row arithmetic in scalar code.
 template <int TY, int TZ>
 __device__ inline uint32_t halo_row_lanes(const CaParams& P, const TileCoord& o) {
-    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const int lane = static_cast<int>(opaque_tid() & 63u);
     return lane < Halo<TY, TZ>::R ? halo_row<TY, TZ>(P, o, lane) : kNoCell;
 }
 __device__ inline uint32_t row_of(uint32_t row_base, int r) {
@@ -539,9 +548,9 @@ template <int TY, int TZ>
 __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileCoord& o, HaloTypes<TY, TZ>& S,
                                   bool with_spill, int32_t* s_spill, int& own_air, uint32_t row_base, int gx) {
     using H = Halo<TY, TZ>;
-    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-    const int lx = static_cast<int>(threadIdx.x & 63u);
-    const int t = static_cast<int>(threadIdx.x);
+    const int t = static_cast<int>(opaque_tid());
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int lx = t & 63;
     uint32_t v[H::KR];  // 32-bit so that no load result is packed (and waited for) before the stores
 #pragma unroll
     for (int k = 0; k < H::KR; ++k) {
@@ -794,6 +803,7 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
     uint64_t next = tiles[blockIdx.x];
     const uint32_t nact = B.ctrl->nactive[shape];
     for (uint32_t s = blockIdx.x; s < nact; s += gridDim.x) {
+        const unsigned tid = opaque_tid();  // see opaque_tid
         // uniform: one entry per block; low word slot << kTileBits | tile, high word the step
         const uint32_t e = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(next));
         const int32_t t = static_cast<int32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(next >> 32)));
@@ -818,9 +828,9 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
         // lane j: base of halo row j; gx: this lane's column (x == nx is the wrapped x = 0 of a periodic partial
         // tile, -1 for no cell)
         const uint32_t row_base = halo_row_lanes<TY, TZ>(P, o);
-        const int gx = halo_x(P, o.x0 + static_cast<int>(threadIdx.x & 63u));
+        const int gx = halo_x(P, o.x0 + static_cast<int>(tid & 63u));
         const bool saw_lava = stage_rows<TY, TZ>(P, D, o, T.S, has_spill, T.spill, own_air, row_base, gx);
-        if (threadIdx.x == 0) {
+        if (tid == 0) {
             s_nchg = 0;
             s_nlist = 0;
             s_nwide = 0;
@@ -828,15 +838,15 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
         {
             // block OR of saw_lava (bit 0) and own_air (bit 1): one flag byte per wave
             const uint32_t f = (__ballot(saw_lava) ? 1u : 0u) | (__ballot(own_air) ? 2u : 0u);
-            if ((threadIdx.x & 63u) == 0)
-                reinterpret_cast<uint8_t*>(&s_wflags[0])[threadIdx.x >> 6] = static_cast<uint8_t>(f);
+            if ((tid & 63u) == 0)
+                reinterpret_cast<uint8_t*>(&s_wflags[0])[tid >> 6] = static_cast<uint8_t>(f);
         }
         __syncthreads();
         const uint32_t stage_flags = s_wflags[0];
         if (!(stage_flags & 0x01010101u)) {
             // no lava in the tile or its halo: no cell changes; the tile keeps AIR iff it has AIR
             const int any_air = (stage_flags & 0x02020202u) != 0;
-            if (threadIdx.x == 0) {
+            if (tid == 0) {
                 D.tile_flags[tile] = static_cast<uint8_t>(any_air ? kFlagAir : 0);
                 D.tile_nchg[tile] = 0;
                 D.tile_nwide[tile] = 0;
@@ -847,9 +857,9 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
         int tile_lava = 0, tile_air = 0;
         {
             // flowId / temperature / D(n) of the LAVA positions, all loads in flight before the stores
-            const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-            const int lx = static_cast<int>(threadIdx.x & 63u);
-            const int t = static_cast<int>(threadIdx.x);
+            const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
+            const int lx = static_cast<int>(tid & 63u);
+            const int t = static_cast<int>(tid);
             int32_t fl[H::KR];
             float tp[H::KR];
             int32_t ds[H::KR];  // 32-bit: see stage_rows
@@ -950,7 +960,7 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
         __syncthreads();  // list and lava data of the tile are in LDS
         const uint32_t nlist = s_nlist;
         for (uint32_t j0 = 0; j0 < nlist; j0 += kBlock) {
-            const uint32_t j = j0 + threadIdx.x;
+            const uint32_t j = j0 + tid;
             const int e = j < nlist ? T.list[j] : 0;
             const int lx = e & 63, row = e >> 6;
             const int ly = row % TY, lz = row / TY;
@@ -1058,14 +1068,14 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
         {
             const uint32_t f = (__ballot(tile_lava) ? static_cast<uint32_t>(kFlagLava) : 0u) |
                                (__ballot(tile_air) ? static_cast<uint32_t>(kFlagAir) : 0u);
-            if ((threadIdx.x & 63u) == 0) {
-                s_red[0][threadIdx.x >> 6] = w_lava;
-                s_red[1][threadIdx.x >> 6] = w_born;
-                reinterpret_cast<uint8_t*>(&s_wflags[1])[threadIdx.x >> 6] = static_cast<uint8_t>(f);
+            if ((tid & 63u) == 0) {
+                s_red[0][tid >> 6] = w_lava;
+                s_red[1][tid >> 6] = w_born;
+                reinterpret_cast<uint8_t*>(&s_wflags[1])[tid >> 6] = static_cast<uint8_t>(f);
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (tid == 0) {
             const uint32_t wf = s_wflags[1];
             D.tile_flags[tile] = static_cast<uint8_t>((wf | wf >> 8 | wf >> 16 | wf >> 24) & 0xFFu);
             const uint32_t nchg = s_nchg;
