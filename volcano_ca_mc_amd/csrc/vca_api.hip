@@ -262,7 +262,8 @@ struct vca_ca {
     CaParams P{};
     CaDev D{};
     int shape = 0;
-    DevBuf type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, changes, tile_nchg, tile_nwide,
+    DevBuf type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, tile_near, tile_halo, changes, tile_nchg,
+        tile_nwide,
         part, ctrl, hist_a, hist_c;
     uint64_t n = 0;
     std::unique_ptr<Batch> own;   // batch of one: the solo path
@@ -729,6 +730,8 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         ca->descent.alloc(n);
         ca->vent_cell.alloc(vent_table.size() * 4);
         ca->tile_flags.alloc(P.ntiles);
+        ca->tile_near.alloc(P.ntiles);
+        ca->tile_halo.alloc(static_cast<size_t>(P.ntiles) * kHaloBytes);
         ca->changes.alloc(static_cast<size_t>(P.ntiles) * kTileCells * sizeof(Change));
         ca->tile_nchg.alloc(static_cast<size_t>(P.ntiles) * 4);
         ca->tile_nwide.alloc(static_cast<size_t>(P.ntiles) * 4);
@@ -744,6 +747,8 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         check(hipMemsetAsync(ca->arrival.p, 0, n * 2, s), "memset");
         check(hipMemsetAsync(ca->descent.p, 0xFF, n, s), "memset");
         check(hipMemsetAsync(ca->ctrl.p, 0, sizeof(CaCtrl), s), "memset");
+        check(hipMemsetAsync(ca->tile_near.p, 0, P.ntiles, s), "memset");
+        check(hipMemsetAsync(ca->tile_halo.p, 0, static_cast<size_t>(P.ntiles) * kHaloBytes, s), "memset");
         check(hipMemsetAsync(ca->part.p, 0, static_cast<size_t>(kPartRows) * sizeof(CaPart), s), "memset");
         check(hipMemsetAsync(ca->hist_a.p, 0, hcap * 8, s), "memset");
         check(hipMemsetAsync(ca->hist_c.p, 0, hcap * 8, s), "memset");
@@ -759,6 +764,8 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         D.descent = ca->descent.as<int8_t>();
         D.vent_cell = ca->vent_cell.as<uint32_t>();
         D.tile_flags = ca->tile_flags.as<uint8_t>();
+        D.tile_near = ca->tile_near.as<uint8_t>();
+        D.tile_halo = ca->tile_halo.as<uint8_t>();
         D.changes = ca->changes.as<Change>();
         D.tile_nchg = ca->tile_nchg.as<uint32_t>();
         D.tile_nwide = ca->tile_nwide.as<uint32_t>();
@@ -1140,9 +1147,10 @@ extern "C" int vca_ca_footprint(const vca_ca_desc* d, uint64_t* bytes) {
                                static_cast<uint64_t>((d->ny + kShapeTY[shape] - 1) / kShapeTY[shape]) *
                                static_cast<uint64_t>((d->nz + kShapeTZ[shape] - 1) / kShapeTZ[shape]);
         const uint64_t cols = static_cast<uint64_t>(d->nx) * d->ny;
-        // state 14 B/cell, record slots 16 B per tile cell, per-tile counts and flags, the batch-of-one lists,
+        // state 14 B/cell, record slots 16 B per tile cell, per-tile counts, flags and neighbour marks, the
+        // batch-of-one lists,
         // spill levels, histories
-        *bytes = n * 14 + tiles * (kTileCells * sizeof(Change) + 9 + (3 * kShapes + 2) * 4) + cols * 4 +
+        *bytes = n * 14 + tiles * (kTileCells * sizeof(Change) + 10 + kHaloBytes + (3 * kShapes + 2) * 4) + cols * 4 +
                  static_cast<uint64_t>(std::max(1, d->max_steps)) * 16 + kPartRows * sizeof(CaPart) + 4096;
     });
 }

@@ -137,6 +137,44 @@ __device__ inline TileCoord tile_origin(const CaParams& P, uint32_t t) {
     const int tz = static_cast<int>(tz_);
     return TileCoord{tx * kTileX, ty * P.ty, tz * P.tz};
 }
+// Lane q < 27 (q != 13) of the calling wave marks neighbour tile q of tile (tx, ty, tz) as next to lava (periodic
+// axes wrap, tiles past an absorbing edge do not exist).
+__device__ inline void mark_near(const CaParams& P, const CaDev& D, int tx, int ty, int tz, int q) {
+    if (q >= 27 || q == 13) return;
+    int qx = tx + q % 3 - 1, qy = ty + (q / 3) % 3 - 1, qz = tz + q / 9 - 1;
+    if (!wrap_axis(qx, P.ntx, P.per[0]) || !wrap_axis(qy, P.nty, P.per[1]) || !wrap_axis(qz, P.ntz, P.per[2])) return;
+    D.tile_near[static_cast<uint32_t>(qx) +
+                static_cast<uint32_t>(P.ntx) * (static_cast<uint32_t>(qy) + static_cast<uint32_t>(P.nty) * static_cast<uint32_t>(qz))] = 1;
+}
+
+// Directions q = (dx+1) + 3(dy+1) + 9(dz+1) toward which the cell at (lx, ly, lz) of its tile lies on the boundary:
+// per axis 0 always, -1 at the first cell, +1 at the last real cell (xm / ym / zm, which is short of the tile's edge
+// in a partial tile).  Bit 13 (the tile itself) is clear.
+__device__ inline uint32_t boundary_dirs(int lx, int ly, int lz, int xm, int ym, int zm) {
+    const uint32_t ax = 2u | (lx == 0 ? 1u : 0u) | (lx == xm ? 4u : 0u);
+    const uint32_t ay = 2u | (ly == 0 ? 1u : 0u) | (ly == ym ? 4u : 0u);
+    const uint32_t az = 2u | (lz == 0 ? 1u : 0u) | (lz == zm ? 4u : 0u);
+    const uint32_t ysel = ((ay & 1u) ? 0x007u : 0u) | ((ay & 2u) ? 0x038u : 0u) | ((ay & 4u) ? 0x1C0u : 0u);
+    const uint32_t xy = (ax * 0x49u) & ysel;  // 0x49: the 3 bits of ax repeated at i + 3j, j = 0..2
+    const uint32_t zsel = ((az & 1u) ? 0x1FFu : 0u) | ((az & 2u) ? 0x3FE00u : 0u) | ((az & 4u) ? 0x7FC0000u : 0u);
+    return ((xy * 0x40201u) & zsel) & ~(1u << 13);  // 0x40201: xy repeated at 9k, k = 0..2
+}
+// Lane q < 27 of the calling wave: when bit q of bnd is set, tell neighbour tile q of tile (tx, ty, tz) that this
+// tile holds lava next to it (byte 26 - q of the neighbour: the direction back to this tile).
+__device__ inline void mark_halo(const CaParams& P, const CaDev& D, int tx, int ty, int tz, int q, uint32_t bnd) {
+    if (q >= 27 || !((bnd >> q) & 1u)) return;
+    int qx = tx + q % 3 - 1, qy = ty + (q / 3) % 3 - 1, qz = tz + q / 9 - 1;
+    if (!wrap_axis(qx, P.ntx, P.per[0]) || !wrap_axis(qy, P.nty, P.per[1]) || !wrap_axis(qz, P.ntz, P.per[2])) return;
+    const uint32_t n = static_cast<uint32_t>(qx) + static_cast<uint32_t>(P.ntx) *
+                                                       (static_cast<uint32_t>(qy) + static_cast<uint32_t>(P.nty) * static_cast<uint32_t>(qz));
+    D.tile_halo[static_cast<size_t>(n) * kHaloBytes + static_cast<uint32_t>(26 - q)] = 1;
+}
+__device__ inline unsigned wave_or(unsigned v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v |= static_cast<unsigned>(__shfl_xor(static_cast<int>(v), o));
+    return v;
+}
+
 // Cell k (0..3) of thread threadIdx.x inside the tile.
 __device__ inline void tile_cell(const CaParams& P, const TileCoord& o, int k, int& x, int& y, int& z) {
     const int lx = static_cast<int>(threadIdx.x & 63u);
@@ -392,9 +430,13 @@ __global__ void ca_init_stop_kernel(CaParams P, CaDev D) {
 
 // Tile flags and the initial LAVA / AIR counts (countInitialState, LavaCA.hpp:1252-1280).
 __global__ void __launch_bounds__(kBlock) ca_init_flags_kernel(CaParams P, CaDev D) {
+    __shared__ uint32_t s_bnd[kBlock / 64];
     for (uint32_t t = blockIdx.x; t < P.ntiles; t += gridDim.x) {
         const TileCoord o = tile_origin(P, t);
+        const int xm = min(kTileX - 1, P.nx - 1 - o.x0), ym = min(P.ty - 1, P.ny - 1 - o.y0),
+                  zm = min(P.tz - 1, P.nz - 1 - o.z0);
         int lava = 0, air = 0;
+        uint32_t bnd = 0;  // directions in which this tile holds lava on its boundary (mark_halo)
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             int x, y, z;
@@ -405,8 +447,11 @@ __global__ void __launch_bounds__(kBlock) ca_init_flags_kernel(CaParams P, CaDev
                                                                          static_cast<uint32_t>(P.ny) * z)];
                 lava += ty == kLava;
                 air += ty == kAir;
+                if (ty == kLava) bnd |= boundary_dirs(x - o.x0, y - o.y0, z - o.z0, xm, ym, zm);
             }
         }
+        bnd = wave_or(bnd);
+        if ((threadIdx.x & 63u) == 0) s_bnd[threadIdx.x >> 6] = bnd;
         const unsigned long long wl = wave_sum(static_cast<unsigned long long>(lava));
         const unsigned long long wa = wave_sum(static_cast<unsigned long long>(air));
         if ((threadIdx.x & 63u) == 0) {
@@ -416,6 +461,14 @@ __global__ void __launch_bounds__(kBlock) ca_init_flags_kernel(CaParams P, CaDev
         const int any_lava = __syncthreads_or(lava);
         const int any_air = __syncthreads_or(air);
         if (threadIdx.x == 0) D.tile_flags[t] = static_cast<uint8_t>((any_lava ? kFlagLava : 0) | (any_air ? kFlagAir : 0));
+        if (any_lava) {
+            const int tx = o.x0 / kTileX, ty = o.y0 / P.ty, tz = o.z0 / P.tz;
+            mark_near(P, D, tx, ty, tz, static_cast<int>(threadIdx.x));
+            uint32_t all = 0;
+            for (int w = 0; w < kBlock / 64; ++w) all |= s_bnd[w];  // written before the barriers above
+            mark_halo(P, D, tx, ty, tz, static_cast<int>(threadIdx.x), all);
+        }
+        __syncthreads();  // s_bnd is rewritten by the next tile
     }
 }
 
@@ -423,63 +476,110 @@ __global__ void __launch_bounds__(kBlock) ca_init_flags_kernel(CaParams P, CaDev
 // CA step kernels
 // ------------------------------------------------------------------------------------------------
 
-// Visited-tile lists of one step of every running simulation of the batch (one lane per global tile).  A tile
+constexpr int kFrontPer = 8;  // tiles per thread of ca_front
+
+// Visited-tile lists of one step of every running simulation of the batch (kFrontPer global tiles per thread).  A tile
 // is visited when it holds lava (lava cools) or holds AIR next to a tile with lava: the reference's "may
 // change" set (CaRules.hpp:69-79, LavaCA.hpp:844-871) at tile granularity.  In lava mode the tiles holding
-// lava also go to a second list: the cells whose D(n) is needed.  One list per tile shape; wave ballot + one
-// atomic per wave appends the entries.
-__global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B) {
-    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-    bool act = false, lava = false;
-    int shape = -1;
-    uint32_t entry = 0;
-    uint32_t step = 0;
-    if (g < B.ntiles) {
-        const uint32_t slot = B.owner[g];
-        if (slot != kNoSlot && g_sims[B.sim_base + slot].running && !g_sims[B.sim_base + slot].D.ctrl->stop) {
-            const CaParams& P = g_sims[B.sim_base + slot].P;
-            const uint32_t t = g - B.tile_base[slot];
-            const uint8_t* flags = g_sims[B.sim_base + slot].D.tile_flags;
-            const uint8_t f = flags[t];
-            lava = !P.classic && (f & kFlagLava);
-            shape = g_sims[B.sim_base + slot].shape;
-            entry = (slot << kTileBits) | t;
-            step = static_cast<uint32_t>(g_sims[B.sim_base + slot].D.ctrl->steps);
-            if (P.dense || lava || (P.classic && (f & kFlagLava) && (f & kFlagAir))) {
-                act = true;
-            } else if (f & kFlagAir) {
-                const uint32_t r = div_magic(t, P.div_ntx_m, P.div_ntx_p);
-                const int tx = static_cast<int>(t - r * static_cast<uint32_t>(P.ntx));
-                const uint32_t tzu = div_magic(r, P.div_nty_m, P.div_nty_p);
-                const int ty = static_cast<int>(r - tzu * static_cast<uint32_t>(P.nty));
-                const int tz = static_cast<int>(tzu);
-                uint8_t any = 0;  // all 26 neighbour flags are loaded together (no early exit)
+// lava also go to a second list: the cells whose D(n) is needed.  One list per tile shape; a block counts its
+// entries by wave ballots and reserves them with one atomic per list.
+__global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B, uint32_t shapes) {
+    // lists 2*shape (visited) and 2*shape + 1 (lava); per wave entry counts, then each block's base from one atomic
+    // per list (a device-wide counter is the contended resource: one atomic per block, not per wave)
+    __shared__ uint32_t s_cnt[2 * kShapes][kBlock / 64];
+    __shared__ uint32_t s_base[2 * kShapes];
+    const int w = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63u);
+    const unsigned long long below = (1ULL << lane) - 1ULL;
+    uint64_t ent[kFrontPer];
+    uint32_t code[kFrontPer];  // bit 0 visited, bit 1 lava, bits 2.. shape
 #pragma unroll
-                for (int q = 0; q < 27; ++q) {
-                    if (q == 13) continue;
-                    int qx = tx + q % 3 - 1, qy = ty + (q / 3) % 3 - 1, qz = tz + q / 9 - 1;
-                    if (!wrap_axis(qx, P.ntx, P.per[0]) || !wrap_axis(qy, P.nty, P.per[1]) ||
-                        !wrap_axis(qz, P.ntz, P.per[2]))
-                        continue;
-                    any |= flags[static_cast<uint32_t>(qx) +
-                                 static_cast<uint32_t>(P.ntx) *
-                                     (static_cast<uint32_t>(qy) + static_cast<uint32_t>(P.nty) * static_cast<uint32_t>(qz))];
+    for (int k = 0; k < kFrontPer; ++k) {
+        const uint32_t g = (blockIdx.x * kFrontPer + k) * kBlock + threadIdx.x;
+        const uint32_t slot = g < B.ntiles ? B.owner[g] : kNoSlot;
+        // a wave's 64 tiles nearly always belong to one simulation: then its parameters are read once, as scalars
+        const uint32_t s0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(slot)));
+        const bool uniform = __ballot(slot != s0) == 0;
+        ent[k] = 0;
+        code[k] = 0;
+        auto visit = [&](const CaSim& sim, uint32_t sl) {
+            if (!sim.running || sim.D.ctrl->stop) return;
+            const CaParams& P = sim.P;
+            const uint32_t t = g - B.tile_base[sl];
+            const uint8_t f = sim.D.tile_flags[t];
+            const bool lava = !P.classic && (f & kFlagLava);
+            // a neighbour tile with lava marked this tile when its own step ended (mark_near); lava tiles are
+            // visited every step in lava mode, so their marks are renewed and this step's are cleared here
+            uint8_t* near = sim.D.tile_near;
+            const uint8_t nb = near[t];
+            if (nb && !P.classic) near[t] = 0;
+            const bool act =
+                P.dense || lava || (P.classic && (f & kFlagLava) && (f & kFlagAir)) || ((f & kFlagAir) && nb);
+            // the rules stage a visited tile only when it or its 1-cell halo holds lava: its own flag, or a
+            // neighbour's halo byte (mark_halo); the bytes are cleared for the next step in lava mode
+            bool stage = (f & kFlagLava) != 0;
+            if (act) {
+                uint4* hb = reinterpret_cast<uint4*>(sim.D.tile_halo + static_cast<size_t>(t) * kHaloBytes);
+                bool any = stage;  // a tile with lava is staged anyway: its bytes are only cleared
+                if (!stage) {
+                    const uint4 h0 = hb[0], h1 = hb[1];
+                    any = (h0.x | h0.y | h0.z | h0.w | h1.x | h1.y | h1.z | h1.w) != 0;
+                    stage = any;
                 }
-                act = (any & kFlagLava) != 0;
+                if (any && !P.classic) {
+                    hb[0] = make_uint4(0, 0, 0, 0);
+                    hb[1] = make_uint4(0, 0, 0, 0);
+                }
             }
+            ent[k] = (static_cast<uint64_t>(static_cast<uint32_t>(sim.D.ctrl->steps)) << 32) | (sl << kTileBits) | t |
+                     (stage ? kEntryStage : 0ULL);
+            code[k] = (act ? 1u : 0u) | (lava ? 2u : 0u) | (static_cast<uint32_t>(sim.shape) << 2);
+        };
+        if (uniform) {
+            if (s0 != kNoSlot) visit(g_sims[B.sim_base + s0], s0);
+        } else if (slot != kNoSlot) {
+            visit(g_sims[B.sim_base + slot], slot);
         }
     }
 #pragma unroll
-    for (int k = 0; k < kShapes; ++k) {
-        const bool a = act && shape == k;
-        if (__ballot(a)) {
-            const uint32_t slot = wave_append(a, &B.ctrl->nactive[k]);
-            if (a) B.active[k * B.cap + slot] = (static_cast<uint64_t>(step) << 32) | entry;
+    for (int l = 0; l < 2 * kShapes; ++l) {
+        if (!((shapes >> (l >> 1)) & 1u)) continue;
+        const uint32_t want = (static_cast<uint32_t>(l >> 1) << 2) | ((l & 1) ? 2u : 1u);
+        uint32_t n = 0;
+#pragma unroll
+        for (int k = 0; k < kFrontPer; ++k) {
+            const bool m = (code[k] >> 2) == (want >> 2) && (code[k] & want & 3u);
+            n += static_cast<uint32_t>(__popcll(__ballot(m)));
         }
-        const bool l = lava && shape == k;
-        if (__ballot(l)) {
-            const uint32_t slot = wave_append(l, &B.ctrl->nlava[k]);
-            if (l) B.lava[k * B.cap + slot] = entry;
+        if (lane == 0) s_cnt[l][w] = n;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * kShapes) {
+        const int l = static_cast<int>(threadIdx.x);
+        uint32_t tot = 0;
+        if ((shapes >> (l >> 1)) & 1u)
+            for (int q = 0; q < kBlock / 64; ++q) tot += s_cnt[l][q];
+        uint32_t* counter = (l & 1) ? &B.ctrl->nlava[l >> 1] : &B.ctrl->nactive[l >> 1];
+        s_base[l] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < 2 * kShapes; ++l) {
+        if (!((shapes >> (l >> 1)) & 1u)) continue;
+        const uint32_t want = (static_cast<uint32_t>(l >> 1) << 2) | ((l & 1) ? 2u : 1u);
+        uint32_t pos = s_base[l];
+        for (int q = 0; q < w; ++q) pos += s_cnt[l][q];
+#pragma unroll
+        for (int k = 0; k < kFrontPer; ++k) {
+            const bool m = (code[k] >> 2) == (want >> 2) && (code[k] & want & 3u);
+            const unsigned long long mb = __ballot(m);
+            if (m) {
+                const uint32_t at = pos + static_cast<uint32_t>(__popcll(mb & below));
+                if (l & 1)
+                    B.lava[(l >> 1) * B.cap + at] = static_cast<uint32_t>(ent[k]);
+                else
+                    B.active[(l >> 1) * B.cap + at] = ent[k];
+            }
+            pos += static_cast<uint32_t>(__popcll(mb));
         }
     }
 }
@@ -792,7 +892,7 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
     constexpr int LX = H::LX, LY = H::LY;
     __shared__ RulesTile<TY, TZ> T;
     __shared__ uint32_t s_nchg, s_nlist, s_nwide;
-    __shared__ uint32_t s_red[2][kBlock / 64];
+    __shared__ uint32_t s_red[3][kBlock / 64];
     __shared__ uint32_t s_wflags[2];  // per-wave flag bytes: [0] staging, [1] end of tile
     if (threadIdx.x < 27) {
         const int k = static_cast<int>(threadIdx.x);
@@ -804,14 +904,26 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
     const uint32_t nact = B.ctrl->nactive[shape];
     for (uint32_t s = blockIdx.x; s < nact; s += gridDim.x) {
         const unsigned tid = opaque_tid();  // see opaque_tid
-        // uniform: one entry per block; low word slot << kTileBits | tile, high word the step
+        // uniform: one entry per block; low word slot << kTileBits | tile, then the step and kEntryStage
         const uint32_t e = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(next));
-        const int32_t t = static_cast<int32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(next >> 32)));
+        const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(next >> 32)));
+        const int32_t t = static_cast<int32_t>(hi & 0xFFFFu);
+        const bool stage = (hi >> 31) != 0;
         if (s + gridDim.x < nact) next = tiles[s + gridDim.x];
         const CaSim& sim = g_sims[B.sim_base + (e >> kTileBits)];
         const CaParams& P = sim.P;
         const CaDev& D = sim.D;
         const uint32_t tile = e & kTileMask;
+        if (!stage) {
+            // neither the tile nor its 1-cell halo holds lava (the tile's flag and its neighbours' halo bytes): no
+            // cell of the tile can change, and its flags stay as they are
+            if (tid == 0) {
+                D.tile_nchg[tile] = 0;
+                D.tile_nwide[tile] = 0;
+                atomicAdd(&D.part[tile % kPartRows].tiles, 1ULL);
+            }
+            continue;
+        }
         const uint64_t prefix = hash_prefix(P.seed, static_cast<uint64_t>(t));
         const bool erupting = t < P.eruption_steps;
         const uint16_t stamp = static_cast<uint16_t>(t < 65535 ? t : 65535);
@@ -959,6 +1071,11 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
         }
         __syncthreads();  // list and lava data of the tile are in LDS
         const uint32_t nlist = s_nlist;
+        // directions in which the tile holds lava on its boundary after the step (mark_halo): cells that stay or
+        // become LAVA, and the cells left to ca_wide (either outcome)
+        const int xm = min(kTileX - 1, P.nx - 1 - o.x0), ym = min(TY - 1, P.ny - 1 - o.y0),
+                  zm = min(TZ - 1, P.nz - 1 - o.z0);
+        uint32_t bnd = 0;
         for (uint32_t j0 = 0; j0 < nlist; j0 += kBlock) {
             const uint32_t j = j0 + tid;
             const int e = j < nlist ? T.list[j] : 0;
@@ -976,6 +1093,7 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
                 if (ty == kLava) {
                     if (P.classic) {
                         tile_lava = 1;
+                        bnd |= boundary_dirs(lx, ly, lz, xm, ym, zm);
                     } else {
                         // rule 2 (LavaCA.hpp:1129-1147, 973-994)
                         const int32_t f = T.flow[c];
@@ -1053,6 +1171,7 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
             }
             w_lava += static_cast<uint32_t>(__popcll(__ballot(stay || newborn)));
             w_born += static_cast<uint32_t>(__popcll(__ballot(newborn)));
+            if (stay || newborn || wide) bnd |= boundary_dirs(lx, ly, lz, xm, ym, zm);
             Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
             const uint32_t slot = wave_append(want, &s_nchg);
             if (want) rec[slot] = ch;
@@ -1068,13 +1187,22 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
         {
             const uint32_t f = (__ballot(tile_lava) ? static_cast<uint32_t>(kFlagLava) : 0u) |
                                (__ballot(tile_air) ? static_cast<uint32_t>(kFlagAir) : 0u);
+            bnd = wave_or(bnd);
             if ((tid & 63u) == 0) {
                 s_red[0][tid >> 6] = w_lava;
                 s_red[1][tid >> 6] = w_born;
+                s_red[2][tid >> 6] = bnd;
                 reinterpret_cast<uint8_t*>(&s_wflags[1])[tid >> 6] = static_cast<uint8_t>(f);
             }
         }
         __syncthreads();
+        // a tile holding lava after the step marks its neighbours for the next step's front: any lava (visit), and
+        // lava next to each neighbour (stage)
+        if ((s_wflags[1] & 0x01010101u) != 0) {
+            const int tx = o.x0 / kTileX, tyy = o.y0 / TY, tz = o.z0 / TZ;
+            mark_near(P, D, tx, tyy, tz, static_cast<int>(tid));
+            mark_halo(P, D, tx, tyy, tz, static_cast<int>(tid), s_red[2][0] | s_red[2][1] | s_red[2][2] | s_red[2][3]);
+        }
         if (tid == 0) {
             const uint32_t wf = s_wflags[1];
             D.tile_flags[tile] = static_cast<uint8_t>((wf | wf >> 8 | wf >> 16 | wf >> 24) & 0xFFu);
@@ -1575,7 +1703,8 @@ void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, int 
                     hipEvent_t* ev) {
     const unsigned gt = static_cast<unsigned>(grid_tiles);
     if (ev) hipEventRecord(ev[0], s);
-    hipLaunchKernelGGL(ca_front_kernel, dim3(grid_for(B.ntiles)), dim3(kBlock), 0, s, B);
+    hipLaunchKernelGGL(ca_front_kernel, dim3(grid_for((B.ntiles + kFrontPer - 1) / kFrontPer)), dim3(kBlock), 0, s,
+                       B, shapes);
     if (ev) hipEventRecord(ev[1], s);
     if (any_lava_mode)
         for (int k = 0; k < kShapes; ++k)

@@ -131,6 +131,10 @@ struct CaDev {
     int32_t* spill;               // nx*ny, nullptr in classic mode
     uint32_t* vent_cell;          // by flowId, 0xFFFFFFFF = none
     uint8_t* tile_flags;          // bit0: has lava, bit1: has air
+    uint8_t* tile_near;           // 1: one of the 26 neighbour tiles held lava when its last step ended (set by
+                                  // that tile's rules, read and cleared by ca_front in lava mode; kept in classic)
+    uint8_t* tile_halo;           // kHaloBytes per tile: byte q = 1 when the neighbour tile in direction q holds lava
+                                  // in its cells next to this tile (same writers / reader / clearing as tile_near)
     Change* changes;              // kTileCells record slots per tile: tile t owns slots t*kTileCells ..
     uint32_t* tile_nchg;          // records of each visited tile this step (from slot 0 up)
     uint32_t* tile_nwide;         // cells left to ca_wide (kKindWide records from slot kTileCells-1 down)
@@ -140,6 +144,9 @@ struct CaDev {
     unsigned long long* hist_changed;
 };
 constexpr uint8_t kFlagLava = 1, kFlagAir = 2;
+constexpr int kHaloBytes = 32;   // 27 direction bytes per tile, padded
+// Visited-tile entry bit: the tile or its 1-cell halo holds lava, so the rules must stage it (ca_front sets it).
+constexpr uint64_t kEntryStage = 1ULL << 63;
 
 // Tile shapes (64 x TY x TZ): 0 = 4x4, 1 = 2x8, 2 = 8x2 (3D), 3 = 1x16 (x-z plane, ny = 1), 4 = 16x1 (nz = 1).
 constexpr int kShapes = 5;
