@@ -1281,18 +1281,23 @@ __global__ void __launch_bounds__(kBlock) ca_wide_kernel(BatchDev B) {
 
 // Scatters each visited tile's change records into the state (LavaCA.hpp:1052-1066).
 __global__ void __launch_bounds__(kBlock) ca_apply_kernel(BatchDev B) {
+    // one wave per visited tile; tiles settled without staging (no kEntryStage) have no records
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const uint32_t waves = gridDim.x * (kBlock / 64);
 #pragma unroll 1
     for (int k = 0; k < kShapes; ++k) {
         const uint32_t nact = B.ctrl->nactive[k];
         const uint64_t* tiles = B.active + k * B.cap;
-        for (uint32_t s = blockIdx.x; s < nact; s += gridDim.x) {
-            // uniform: one entry per block
-            const uint32_t e = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(tiles[s]));
+        for (uint32_t s = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); s < nact; s += waves) {
+            const uint64_t ent = tiles[s];
+            const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(ent >> 32)));
+            if (!(hi >> 31)) continue;
+            const uint32_t e = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(ent)));
             const CaDev& D = g_sims[B.sim_base + (e >> kTileBits)].D;
             const uint32_t tile = e & kTileMask;
             const uint32_t n = D.tile_nchg[tile];
             const Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
-            for (uint32_t j = threadIdx.x; j < n; j += kBlock) {
+            for (uint32_t j = static_cast<uint32_t>(lane); j < n; j += 64) {
                 const Change c = rec[j];
                 D.type[c.idx] = c.type;
                 D.temp[c.idx] = c.temp;
@@ -1715,7 +1720,7 @@ void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, int 
     if (ev) hipEventRecord(ev[3], s);
     hipLaunchKernelGGL(ca_wide_kernel, dim3(kWideBlocks), dim3(kBlock), 0, s, B);
     if (ev) hipEventRecord(ev[4], s);
-    hipLaunchKernelGGL(ca_apply_kernel, dim3(gt), dim3(kBlock), 0, s, B);
+    hipLaunchKernelGGL(ca_apply_kernel, dim3((gt + 3) / 4), dim3(kBlock), 0, s, B);
     if (ev) hipEventRecord(ev[5], s);
     hipLaunchKernelGGL(ca_finish_kernel, dim3(B.nslots), dim3(kBlock), 0, s, B);
     if (ev) hipEventRecord(ev[6], s);
