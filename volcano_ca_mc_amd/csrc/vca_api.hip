@@ -272,29 +272,68 @@ struct vca_ca {
     int slot = -1;
 };
 
-// Entries of the per-device simulation table (kSimTable) handed out to batches as contiguous ranges.
+// Entries of the per-device simulation table (kSimTable) handed out to batches as contiguous ranges.  A range
+// is given back with an event recorded on the stream of the batch that used it: it is reused only once that
+// event has completed (kernels still queued may read the entries), without the releasing thread waiting.
 class SimTable {
 public:
     static uint32_t take(int device, int n) {
         std::lock_guard<std::mutex> lock(mu());
         std::vector<char>& used = map()[device];
         used.resize(kSimTable, 0);
-        for (int b = 0; b + n <= kSimTable; ++b) {
-            bool ok = true;
-            for (int k = 0; k < n && ok; ++k) ok = !used[static_cast<size_t>(b + k)];
-            if (!ok) continue;
-            for (int k = 0; k < n; ++k) used[static_cast<size_t>(b + k)] = 1;
-            return static_cast<uint32_t>(b);
+        for (int pass = 0; pass < 2; ++pass) {
+            reap(device, pass == 1);  // second pass: wait for the pending releases
+            for (int b = 0; b + n <= kSimTable; ++b) {
+                bool ok = true;
+                for (int k = 0; k < n && ok; ++k) ok = !used[static_cast<size_t>(b + k)];
+                if (!ok) continue;
+                for (int k = 0; k < n; ++k) used[static_cast<size_t>(b + k)] = 1;
+                return static_cast<uint32_t>(b);
+            }
         }
         throw VcaError(VCA_ERR_NOMEM, "simulation table full (" + std::to_string(kSimTable) + " entries per device)");
     }
-    static void give(int device, uint32_t base, int n) {
+    // entries become free when `after` completes (nullptr: now); takes ownership of the event
+    static void give(int device, uint32_t base, int n, hipEvent_t after) {
         std::lock_guard<std::mutex> lock(mu());
-        std::vector<char>& used = map()[device];
-        for (int k = 0; k < n && base + k < used.size(); ++k) used[base + k] = 0;
+        pending().push_back(Pending{device, base, n, after});
+        reap(device, false);
     }
 
 private:
+    struct Pending {
+        int device;
+        uint32_t base;
+        int n;
+        hipEvent_t ev;
+    };
+    static void reap(int device, bool wait) {
+        std::vector<Pending>& p = pending();
+        std::vector<char>& used = map()[device];
+        used.resize(kSimTable, 0);
+        for (size_t k = 0; k < p.size();) {
+            if (p[k].device != device) {
+                ++k;
+                continue;
+            }
+            if (p[k].ev) {
+                const hipError_t e = wait ? hipEventSynchronize(p[k].ev) : hipEventQuery(p[k].ev);
+                if (e == hipErrorNotReady) {
+                    ++k;
+                    continue;
+                }
+                (void)hipGetLastError();
+                (void)hipEventDestroy(p[k].ev);
+            }
+            for (int q = 0; q < p[k].n && p[k].base + q < used.size(); ++q) used[p[k].base + q] = 0;
+            p[k] = p.back();
+            p.pop_back();
+        }
+    }
+    static std::vector<Pending>& pending() {
+        static std::vector<Pending> v;
+        return v;
+    }
     static std::mutex& mu() {
         static std::mutex m;
         return m;
@@ -360,8 +399,15 @@ struct Batch {
     ~Batch() {
         drop_graph();
         if (has_table) {
-            (void)hipStreamSynchronize(ctx->stream);
-            SimTable::give(ctx->device, sim_base, max_slots);
+            hipEvent_t ev = nullptr;
+            if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+                hipEventRecord(ev, ctx->stream) != hipSuccess) {
+                (void)hipGetLastError();
+                if (ev) (void)hipEventDestroy(ev);
+                ev = nullptr;
+                (void)hipStreamSynchronize(ctx->stream);
+            }
+            SimTable::give(ctx->device, sim_base, max_slots, ev);
         }
     }
     void drop_graph() {
@@ -813,7 +859,8 @@ extern "C" int vca_ca_destroy(vca_ca* ca) {
     return guarded([&] {
         if (!ca) return;
         ca->ctx->set_device();
-        (void)hipStreamSynchronize(ca->ctx->stream);
+        // no host wait: the device buffers are freed in stream order after the work already queued on the
+        // context's stream (stream-ordered pool), and the simulation table entries are reused only after it
         if (ca->member) ca->member->detach(ca, ca->slot);
         ca->member = nullptr;
         delete ca;
