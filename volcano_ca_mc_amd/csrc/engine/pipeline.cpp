@@ -30,7 +30,7 @@ namespace fs = std::filesystem;
 
 namespace {
 
-enum class Stage { Waiting, Loading, Ready, Ca, PostCa, McReady, Mc, Writing, Done, Failed };
+enum class Stage { Waiting, Loading, Ready, Creating, Created, Ca, PostCa, McReady, Mc, Writing, Done, Failed };
 
 struct Job {
     std::string name;
@@ -92,6 +92,7 @@ public:
             bool busy = false;
             Clock ph;
             start_loading(now);
+            spawn_ms_ += ph.ms();
             harvest();
             phase_ms_[0] += ph.ms();
             ph.reset();
@@ -124,6 +125,11 @@ public:
         j["admitMs"] = Json(phase_ms_[1]);
         j["caMs"] = Json(phase_ms_[2]);
         j["caCollectMs"] = Json(collect_ms_);
+        j["loadSpawnMs"] = Json(spawn_ms_);
+        j["releaseMs"] = Json(release_ms_);
+        j["batchAddMs"] = Json(create_ms_);
+        j["caEnqueueMs"] = Json(enqueue_ms_);
+        j["caStatusMs"] = Json(status_ms_);
         j["mcMs"] = Json(phase_ms_[3]);
         return j;
     }
@@ -184,6 +190,7 @@ private:
             try {
                 j.work.get();
             } catch (const std::exception& e) {
+                if (j.stage == Stage::Creating) used_ -= j.fp_ca;
                 if (j.stage == Stage::Mc) {
                     mc_busy_[static_cast<size_t>(j.mc_stream)] = 0;
                     used_ -= j.fp_mc;
@@ -201,6 +208,9 @@ private:
                         j.prep_ms = tp.ms();
                     }
                     if (j.stage == Stage::Writing) write_async(j);
+                    break;
+                case Stage::Creating:
+                    j.stage = Stage::Created;
                     break;
                 case Stage::PostCa:
                     // the MC takes its domain from the simulation's final state on the device: the simulation is
@@ -229,7 +239,9 @@ private:
     // on the GPU thread, between chunks
     void release_ca(Job& j) {
         if (!j.ca) return;
+        Clock t;
         vca_ca_destroy(j.ca);
+        release_ms_ += t.ms();
         j.ca = nullptr;
         used_ -= j.fp_ca;
     }
@@ -250,41 +262,53 @@ private:
     int admitted() const {
         int n = 0;
         for (auto& j : jobs_)
-            n += j->stage == Stage::Ca || j->stage == Stage::PostCa || j->stage == Stage::McReady || j->stage == Stage::Mc;
+            n += j->stage == Stage::Creating || j->stage == Stage::Created || j->stage == Stage::Ca ||
+                 j->stage == Stage::PostCa || j->stage == Stage::McReady || j->stage == Stage::Mc;
         return n;
     }
 
     // arrival order; a job waits while its footprint does not fit next to the running ones (or while
-    // --max-jobs jobs are between admission and their result files)
+    // --max-jobs jobs are between admission and their result files).  Simulations are created on worker threads
+    // (allocation, rock columns, seeds on a stream of their own) and join the batch here, between chunks.
     bool admit() {
         bool any = false;
+        for (auto& jp : jobs_) {
+            Job& j = *jp;
+            if (j.stage != Stage::Created) continue;
+            Clock t;
+            int32_t slot = -1;
+            if (vca_batch_add(batch_, j.ca, &slot) != VCA_OK) break;  // no slot / tile range: retry later
+            create_ms_ += t.ms();
+            j.slot = slot;
+            j.stage = Stage::Ca;
+            j.ca_clock.reset();
+            any = true;
+        }
+        int free_workers = static_cast<int>(workers_) - in_flight();
         for (auto& jp : jobs_) {
             Job& j = *jp;
             if (j.stage != Stage::Ready || j.work.valid()) continue;
             if (opt_.max_jobs > 0 && admitted() >= opt_.max_jobs) break;
             if (used_ + j.fp_ca > budget_ && used_ > 0) break;
+            if (free_workers <= 0) break;
             try {
                 if (opt_.autotune_steps > 0) tuned(j);
+            } catch (const std::exception& e) {
+                fail(j, e.what());
+                continue;
+            }
+            used_ += j.fp_ca;
+            j.stage = Stage::Creating;
+            --free_workers;
+            any = true;
+            j.work = std::async(std::launch::async, [this, &j] {
                 Clock t;
                 j.ca = create_ca(j.cfg, j.in, gpu_);
-                int32_t slot = -1;
-                if (vca_batch_add(batch_, j.ca, &slot) != VCA_OK) {  // no slot / tile range: retry later
-                    vca_ca_destroy(j.ca);
-                    j.ca = nullptr;
-                    break;
-                }
-                j.slot = slot;
                 j.timing.add("allocate", t.ms());
                 j.timing.add("terrain", j.in.terrain_ms);
                 j.timing.add("vents", j.in.vents_ms);
                 j.in = CaInputs{};  // host terrain no longer needed
-                used_ += j.fp_ca;
-                j.stage = Stage::Ca;
-                j.ca_clock.reset();
-                any = true;
-            } catch (const std::exception& e) {
-                fail(j, e.what());
-            }
+            });
         }
         return any;
     }
@@ -312,9 +336,13 @@ private:
         int running = 0;
         for (auto& j : jobs_) running += j->stage == Stage::Ca;
         if (running == 0) return false;
+        Clock te;
         vca_check(vca_batch_enqueue(batch_, kChunk), "CA");
+        enqueue_ms_ += te.ms();
+        te.reset();
         std::vector<int32_t> st(2 * kSlots);
         vca_check(vca_batch_status(batch_, st.data(), kSlots), "CA");
+        status_ms_ += te.ms();
         for (auto& jp : jobs_) {
             Job& j = *jp;
             if (j.stage != Stage::Ca || st[2 * static_cast<size_t>(j.slot) + 1] == VCA_RUNNING) continue;
@@ -432,6 +460,7 @@ private:
     double phase_ms_[4] = {0.0, 0.0, 0.0, 0.0};
     std::map<std::string, vca_tune_result> tune_cache_;
     double collect_ms_ = 0.0;
+    double spawn_ms_ = 0.0, release_ms_ = 0.0, create_ms_ = 0.0, enqueue_ms_ = 0.0, status_ms_ = 0.0;
 };
 
 double percentile(std::vector<double> v, double p) {

@@ -257,16 +257,36 @@ struct vca_ctx {
 
 struct Batch;
 
+// A region of a simulation's single device allocation.
+struct Span {
+    void* p = nullptr;
+    size_t bytes = 0;
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+// Offsets of the regions of one allocation, each 256-byte aligned.
+struct Layout {
+    size_t total = 0;
+    size_t add(size_t bytes) {
+        const size_t at = total;
+        total += (std::max<size_t>(bytes, 1) + 255) & ~static_cast<size_t>(255);
+        return at;
+    }
+};
+
 struct vca_ca {
     vca_ctx* ctx = nullptr;
     CaParams P{};
     CaDev D{};
     int shape = 0;
-    DevBuf type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, tile_near, tile_halo, changes, tile_nchg,
-        tile_nwide,
-        part, ctrl, hist_a, hist_c;
+    DevBuf arena;  // every region below: one allocation and one free per simulation
+    Span type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, tile_near, tile_halo, changes, tile_nchg,
+        tile_nwide, part, ctrl, hist_a, hist_c;
     uint64_t n = 0;
-    std::unique_ptr<Batch> own;   // batch of one: the solo path
+    std::unique_ptr<Batch> own;   // batch of one: the solo path (made on the first solo enqueue)
     hipStream_t side = nullptr;   // during vca_ca_collect: the pooled stream its reads go on
     Batch* member = nullptr;      // the shared batch this simulation is in
     int slot = -1;
@@ -696,6 +716,106 @@ extern "C" int vca_ctx_kernel_time(vca_ctx* ctx, const char* name, vca_kernel_ti
 // ===================================================================================================
 // CA
 // ===================================================================================================
+namespace {
+// Regions of a simulation's single allocation (creation and the footprint estimate use the same plan).  The
+// regions cleared at creation come first, so that one memset covers them.  Creation inputs (rock heights, seed
+// cells) are staged in the step records, which no kernel reads before the first step.
+struct CaPlan {
+    Layout L;
+    size_t hcap = 1;
+    size_t type = 0, flow = 0, temp = 0, solid = 0, arrival = 0, ctrl = 0, tile_near = 0, tile_halo = 0, part = 0,
+           hist_a = 0, hist_c = 0;
+    size_t zero_bytes = 0;
+    size_t descent = 0, vent_cell = 0, tile_flags = 0, changes = 0, tile_nchg = 0, tile_nwide = 0, spill = 0;
+    size_t staging = 0, st_height = 0, st_cells = 0, st_flows = 0, st_temps = 0, st_bytes = 0;
+};
+
+CaPlan ca_plan(uint64_t n, uint32_t ntiles, size_t cols, bool lava, size_t vent_slots, size_t nvents, int max_steps) {
+    CaPlan c;
+    Layout& L = c.L;
+    c.hcap = static_cast<size_t>(std::max(1, max_steps));
+    c.type = L.add(n);
+    c.flow = L.add(n * 4);
+    c.temp = L.add(n * 4);
+    c.solid = L.add(n * 2);
+    c.arrival = L.add(n * 2);
+    c.ctrl = L.add(sizeof(CaCtrl));
+    c.tile_near = L.add(ntiles);
+    c.tile_halo = L.add(static_cast<size_t>(ntiles) * kHaloBytes);
+    c.part = L.add(static_cast<size_t>(kPartRows) * sizeof(CaPart));
+    c.hist_a = L.add(c.hcap * 8);
+    c.hist_c = L.add(c.hcap * 8);
+    c.zero_bytes = L.total;
+    c.descent = L.add(n);
+    c.vent_cell = L.add(vent_slots * 4);
+    c.tile_flags = L.add(ntiles);
+    const size_t records = static_cast<size_t>(ntiles) * kTileCells * sizeof(Change);
+    c.changes = L.add(records);
+    c.tile_nchg = L.add(static_cast<size_t>(ntiles) * 4);
+    c.tile_nwide = L.add(static_cast<size_t>(ntiles) * 4);
+    if (lava) c.spill = L.add(cols * 4);
+    Layout st;
+    c.st_height = st.add(lava ? cols * 4 : 0);
+    c.st_cells = st.add(nvents * 4);
+    c.st_flows = st.add(nvents * 4);
+    c.st_temps = st.add(nvents * 4);
+    c.st_bytes = st.total;
+    c.staging = st.total <= records ? c.changes : L.add(st.total);
+    return c;
+}
+}  // namespace
+
+namespace {
+// Streams for work outside a context's stream (creation, reads of stopped simulations), reused per device
+// (creating and destroying a stream per job costs ~ms).
+class StreamPool {
+public:
+    static hipStream_t take(int device) {
+        {
+            std::lock_guard<std::mutex> lock(mu());
+            std::vector<hipStream_t>& v = free_list()[device];
+            if (!v.empty()) {
+                hipStream_t s = v.back();
+                v.pop_back();
+                return s;
+            }
+        }
+        hipStream_t s = nullptr;
+        check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        return s;
+    }
+    static void give(int device, hipStream_t s) {
+        std::lock_guard<std::mutex> lock(mu());
+        free_list()[device].push_back(s);
+    }
+
+private:
+    static std::mutex& mu() {
+        static std::mutex m;
+        return m;
+    }
+    static std::map<int, std::vector<hipStream_t>>& free_list() {
+        static std::map<int, std::vector<hipStream_t>> v;
+        return v;
+    }
+};
+
+// A pooled stream of the context's device, for work that must not go on the context's stream (which may be
+// advancing a batch, or under capture, on another thread meanwhile); synchronised and returned on scope exit.
+struct SideStream {
+    int device;
+    hipStream_t s;
+    explicit SideStream(const vca_ctx* ctx) : device(ctx->device), s(StreamPool::take(ctx->device)) {}
+    SideStream(const SideStream&) = delete;
+    SideStream& operator=(const SideStream&) = delete;
+    ~SideStream() {
+        (void)hipStreamSynchronize(s);
+        StreamPool::give(device, s);
+    }
+};
+
+}  // namespace
+
 extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* height, const int32_t* spill,
                              const vca_vent* vents, int32_t nvents, vca_ca** out) {
     return guarded([&] {
@@ -712,7 +832,6 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
                     "w trybie lava przestrzeń 2D to przekrój x-z (ny = 1)");
         }
         ctx->set_device();
-        AllocStream alloc_on(ctx->stream);
         auto ca = std::make_unique<vca_ca>();
         ca->ctx = ctx;
         ca->n = n;
@@ -767,37 +886,44 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
                 if (vflow[k] >= 0) vent_table[static_cast<size_t>(vflow[k])] = vcell[k];
         P.nvent_slots = maxf + 1;
 
-        hipStream_t s = ctx->stream;
-        ca->type.alloc(n);
-        ca->flow.alloc(n * 4);
-        ca->temp.alloc(n * 4);
-        ca->solid.alloc(n * 2);
-        ca->arrival.alloc(n * 2);
-        ca->descent.alloc(n);
-        ca->vent_cell.alloc(vent_table.size() * 4);
-        ca->tile_flags.alloc(P.ntiles);
-        ca->tile_near.alloc(P.ntiles);
-        ca->tile_halo.alloc(static_cast<size_t>(P.ntiles) * kHaloBytes);
-        ca->changes.alloc(static_cast<size_t>(P.ntiles) * kTileCells * sizeof(Change));
-        ca->tile_nchg.alloc(static_cast<size_t>(P.ntiles) * 4);
-        ca->tile_nwide.alloc(static_cast<size_t>(P.ntiles) * 4);
-        ca->part.alloc(static_cast<size_t>(kPartRows) * sizeof(CaPart));
-        ca->ctrl.alloc(sizeof(CaCtrl));
-        const size_t hcap = static_cast<size_t>(std::max(1, d->max_steps));
-        ca->hist_a.alloc(hcap * 8);
-        ca->hist_c.alloc(hcap * 8);
-        check(hipMemsetAsync(ca->type.p, 0, n, s), "memset");
-        check(hipMemsetAsync(ca->flow.p, 0, n * 4, s), "memset");
-        check(hipMemsetAsync(ca->temp.p, 0, n * 4, s), "memset");
-        check(hipMemsetAsync(ca->solid.p, 0, n * 2, s), "memset");
-        check(hipMemsetAsync(ca->arrival.p, 0, n * 2, s), "memset");
+        // Creation runs on a pooled stream of its own: the context's stream may be advancing (or capturing) a batch
+        // on another thread meanwhile.  One allocation holds every region.
+        SideStream side(ctx);
+        hipStream_t s = side.s;
+        const size_t cols = static_cast<size_t>(d->nx) * d->ny;
+        const CaPlan plan = ca_plan(n, P.ntiles, cols, !classic, vent_table.size(), static_cast<size_t>(nvents), d->max_steps);
+        {
+            AllocStream alloc_on(s);
+            ca->arena.alloc(plan.L.total);
+        }
+        // freed on the context's stream, after the simulation's last queued step (creation is synchronised below)
+        ca->arena.stream = ctx->stream;
+        char* base = static_cast<char*>(ca->arena.p);
+        auto region = [&](Span& sp, size_t off, size_t bytes) {
+            sp.p = base + off;
+            sp.bytes = bytes;
+        };
+        region(ca->type, plan.type, n);
+        region(ca->flow, plan.flow, n * 4);
+        region(ca->temp, plan.temp, n * 4);
+        region(ca->solid, plan.solid, n * 2);
+        region(ca->arrival, plan.arrival, n * 2);
+        region(ca->ctrl, plan.ctrl, sizeof(CaCtrl));
+        region(ca->tile_near, plan.tile_near, P.ntiles);
+        region(ca->tile_halo, plan.tile_halo, static_cast<size_t>(P.ntiles) * kHaloBytes);
+        region(ca->part, plan.part, static_cast<size_t>(kPartRows) * sizeof(CaPart));
+        region(ca->hist_a, plan.hist_a, plan.hcap * 8);
+        region(ca->hist_c, plan.hist_c, plan.hcap * 8);
+        region(ca->descent, plan.descent, n);
+        region(ca->vent_cell, plan.vent_cell, vent_table.size() * 4);
+        region(ca->tile_flags, plan.tile_flags, P.ntiles);
+        region(ca->changes, plan.changes, static_cast<size_t>(P.ntiles) * kTileCells * sizeof(Change));
+        region(ca->tile_nchg, plan.tile_nchg, static_cast<size_t>(P.ntiles) * 4);
+        region(ca->tile_nwide, plan.tile_nwide, static_cast<size_t>(P.ntiles) * 4);
+        if (!classic) region(ca->spill, plan.spill, cols * 4);
+        // state, counters, neighbour marks, partials and histories start at zero; descent at "none"
+        check(hipMemsetAsync(base, 0, plan.zero_bytes, s), "memset");
         check(hipMemsetAsync(ca->descent.p, 0xFF, n, s), "memset");
-        check(hipMemsetAsync(ca->ctrl.p, 0, sizeof(CaCtrl), s), "memset");
-        check(hipMemsetAsync(ca->tile_near.p, 0, P.ntiles, s), "memset");
-        check(hipMemsetAsync(ca->tile_halo.p, 0, static_cast<size_t>(P.ntiles) * kHaloBytes, s), "memset");
-        check(hipMemsetAsync(ca->part.p, 0, static_cast<size_t>(kPartRows) * sizeof(CaPart), s), "memset");
-        check(hipMemsetAsync(ca->hist_a.p, 0, hcap * 8, s), "memset");
-        check(hipMemsetAsync(ca->hist_c.p, 0, hcap * 8, s), "memset");
         check(hipMemcpyAsync(ca->vent_cell.p, vent_table.data(), vent_table.size() * 4, hipMemcpyHostToDevice, s),
               "copy vents");
 
@@ -820,36 +946,31 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         D.hist_active = ca->hist_a.as<unsigned long long>();
         D.hist_changed = ca->hist_c.as<unsigned long long>();
         D.spill = nullptr;
-        const size_t cols = static_cast<size_t>(d->nx) * d->ny;
+        char* staging = base + plan.staging;
         if (!classic) {
-            ca->spill.alloc(cols * 4);
             D.spill = ca->spill.as<int32_t>();
             check(hipMemcpyAsync(ca->spill.p, spill, cols * 4, hipMemcpyHostToDevice, s), "copy spill");
-            DevBuf h;
-            h.alloc(cols * 4);
-            check(hipMemcpyAsync(h.p, height, cols * 4, hipMemcpyHostToDevice, s), "copy height");
-            ca_launch_terrain(P, D, h.as<int32_t>(), s);
+            check(hipMemcpyAsync(staging + plan.st_height, height, cols * 4, hipMemcpyHostToDevice, s), "copy height");
+            ca_launch_terrain(P, D, reinterpret_cast<const int32_t*>(staging + plan.st_height), s);
             check(hipGetLastError(), "terrain");
         }
+        std::vector<char> seeds;  // kept until the synchronisation below (the upload may read it until then)
         if (nvents > 0) {
-            DevBuf c, f, t;
-            c.alloc(vcell.size() * 4);
-            f.alloc(vflow.size() * 4);
-            t.alloc(vtemp.size() * 4);
-            check(hipMemcpyAsync(c.p, vcell.data(), vcell.size() * 4, hipMemcpyHostToDevice, s), "copy");
-            check(hipMemcpyAsync(f.p, vflow.data(), vflow.size() * 4, hipMemcpyHostToDevice, s), "copy");
-            check(hipMemcpyAsync(t.p, vtemp.data(), vtemp.size() * 4, hipMemcpyHostToDevice, s), "copy");
-            ca_launch_seeds(P, D, c.as<uint32_t>(), f.as<int32_t>(), t.as<float>(), nvents, s);
+            // cells, flows and temperatures in one upload
+            seeds.assign(plan.st_bytes - plan.st_cells, 0);
+            std::memcpy(seeds.data(), vcell.data(), vcell.size() * 4);
+            std::memcpy(seeds.data() + (plan.st_flows - plan.st_cells), vflow.data(), vflow.size() * 4);
+            std::memcpy(seeds.data() + (plan.st_temps - plan.st_cells), vtemp.data(), vtemp.size() * 4);
+            check(hipMemcpyAsync(staging + plan.st_cells, seeds.data(), seeds.size(), hipMemcpyHostToDevice, s), "copy");
+            ca_launch_seeds(P, D, reinterpret_cast<const uint32_t*>(staging + plan.st_cells),
+                            reinterpret_cast<const int32_t*>(staging + plan.st_flows),
+                            reinterpret_cast<const float*>(staging + plan.st_temps), nvents, s);
             check(hipGetLastError(), "seeds");
-            if (!c.pooled) check(hipStreamSynchronize(s), "seeds");  // a non-pooled free would not wait
         }
         // countInitialState (LavaCA.hpp:1252-1280) and the initial stop rule, on the device
         ca_launch_init_flags(P, D, s);
         ca_launch_init_stop(P, D, s);
         check(hipGetLastError(), "init");
-        ca->own = std::make_unique<Batch>();
-        ca->own->init(ctx, 1, P.ntiles);
-        ca->own->attach(ca.get());
         check(hipStreamSynchronize(s), "init");
         *out = ca.release();
     });
@@ -868,38 +989,6 @@ extern "C" int vca_ca_destroy(vca_ca* ca) {
 }
 
 namespace {
-// Side streams of collecting simulations, reused (creating and destroying a stream per job costs ~ms).
-class StreamPool {
-public:
-    static hipStream_t take() {
-        {
-            std::lock_guard<std::mutex> lock(mu());
-            if (!free_list().empty()) {
-                hipStream_t s = free_list().back();
-                free_list().pop_back();
-                return s;
-            }
-        }
-        hipStream_t s = nullptr;
-        check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
-        return s;
-    }
-    static void give(hipStream_t s) {
-        std::lock_guard<std::mutex> lock(mu());
-        free_list().push_back(s);
-    }
-
-private:
-    static std::mutex& mu() {
-        static std::mutex m;
-        return m;
-    }
-    static std::vector<hipStream_t>& free_list() {
-        static std::vector<hipStream_t> v;
-        return v;
-    }
-};
-
 // Pinned staging buffers shared by every collecting thread (allocated once, reused).
 class PinnedPool {
 public:
@@ -982,13 +1071,7 @@ extern "C" int vca_ca_collect(vca_ca* ca, vca_ca_result* r) {
         require(ca && r, "NULL argument");
         if (ca->member) throw VcaError(VCA_ERR_STATE, "the simulation is in a batch: remove it first");
         ca->ctx->set_device();
-        struct Side {
-            hipStream_t s = StreamPool::take();
-            ~Side() {
-                (void)hipStreamSynchronize(s);
-                StreamPool::give(s);
-            }
-        } side;
+        SideStream side(ca->ctx);
         ca->side = side.s;
         CaCtrl c;
         check(hipMemcpyAsync(&c, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, ca->side), "read ctrl");
@@ -1013,19 +1096,6 @@ extern "C" int vca_ca_collect(vca_ca* ca, vca_ca_result* r) {
 }
 
 namespace {
-// A pooled stream for reads of a stopped simulation outside its context's stream (the context may keep advancing
-// a batch meanwhile); synchronised and returned on scope exit.
-struct SideStream {
-    hipStream_t s = StreamPool::take();
-    SideStream() = default;
-    SideStream(const SideStream&) = delete;
-    SideStream& operator=(const SideStream&) = delete;
-    ~SideStream() {
-        (void)hipStreamSynchronize(s);
-        StreamPool::give(s);
-    }
-};
-
 void require_stopped(vca_ca* ca, hipStream_t s) {
     if (ca->member) throw VcaError(VCA_ERR_STATE, "the simulation is in a batch: remove it first");
     CaCtrl c;
@@ -1041,7 +1111,7 @@ extern "C" int vca_ca_flow_stats(vca_ca* ca, const vca_vent* vents, int32_t nflo
         require(ca && nflows >= 0, "invalid argument");
         require(nflows == 0 || (vents && lava && basalt && maxd2), "NULL argument");
         ca->ctx->set_device();
-        SideStream side;
+        SideStream side(ca->ctx);
         require_stopped(ca, side.s);
         const size_t m = static_cast<size_t>(nflows) + 1;
         std::vector<int32_t> xy(2 * m, 0);
@@ -1074,7 +1144,7 @@ extern "C" int vca_ca_flow_points(vca_ca* ca, const vca_vent* vent, int32_t flow
     return guarded([&] {
         require(ca && vent && seen && cap > 0, "invalid argument");
         ca->ctx->set_device();
-        SideStream side;
+        SideStream side(ca->ctx);
         require_stopped(ca, side.s);
         AllocStream alloc_on(side.s);
         DevBuf buf;
@@ -1104,7 +1174,13 @@ extern "C" int vca_ca_enqueue(vca_ca* ca, int32_t new_steps) {
         require(ca, "ca == NULL");
         if (ca->member) throw VcaError(VCA_ERR_STATE, "the simulation is in a batch: advance it with vca_batch_*");
         ca->ctx->set_device();
-        // the private batch's members[0] is this simulation; its slot there is 0
+        if (!ca->own) {  // the private batch (members[0] is this simulation) is made on the first solo enqueue
+            AllocStream alloc_on(ca->ctx->stream);
+            auto own = std::make_unique<Batch>();
+            own->init(ca->ctx, 1, ca->P.ntiles);
+            own->attach(ca);
+            ca->own = std::move(own);
+        }
         ca->own->enqueue(new_steps);
     });
 }
@@ -1194,11 +1270,10 @@ extern "C" int vca_ca_footprint(const vca_ca_desc* d, uint64_t* bytes) {
                                static_cast<uint64_t>((d->ny + kShapeTY[shape] - 1) / kShapeTY[shape]) *
                                static_cast<uint64_t>((d->nz + kShapeTZ[shape] - 1) / kShapeTZ[shape]);
         const uint64_t cols = static_cast<uint64_t>(d->nx) * d->ny;
-        // state 14 B/cell, record slots 16 B per tile cell, per-tile counts, flags and neighbour marks, the
-        // batch-of-one lists,
-        // spill levels, histories
-        *bytes = n * 14 + tiles * (kTileCells * sizeof(Change) + 10 + kHaloBytes + (3 * kShapes + 2) * 4) + cols * 4 +
-                 static_cast<uint64_t>(std::max(1, d->max_steps)) * 16 + kPartRows * sizeof(CaPart) + 4096;
+        // the simulation's allocation (vents: a few slots), and the batch-of-one lists a solo run adds
+        const CaPlan plan = ca_plan(n, static_cast<uint32_t>(std::min<uint64_t>(tiles, kTileMask)), cols, d->classic == 0,
+                                    64, 64, d->max_steps);
+        *bytes = plan.L.total + tiles * (kShapes * 12 + 8) + 4096;
     });
 }
 
@@ -1598,7 +1673,7 @@ extern "C" int vca_mc_create_from_ca(vca_ctx* ctx, const vca_mc_desc* d, vca_ca*
         // a stopped simulation's state is final (later steps are no-ops), so ctx may be another context whose
         // stream is not ordered after the CA's
         {
-            SideStream side;
+            SideStream side(ca->ctx);
             require_stopped(ca, side.s);
         }
         // CoolingMobility (CoolingMobility.cpp:8-129) from the CA result on the device
