@@ -263,6 +263,8 @@ def main() -> int:
                     "bytes_per_unit": f"{CA_BYTES_PER_CELL} B per processed cell-step" if dom_k == "ca_rules" else None,
                     "processed_cells_per_launch": round(w.get("tilesVisited", 0) * w.get("tileCells", 0) /
                                                         max(1, steps)),
+                    "staged_cells_per_launch": round(w.get("tilesStaged", 0) * w.get("tileCells", 0) /
+                                                     max(1, steps)),
                     "minimal_bytes_per_launch": round(minimal_bytes(runs[0])),
                     "kernel_share": {k: round(v[0] / sum(x[0] for x in ktot.values()), 3) for k, v in ktot.items()}}
         cpu = None if a.no_cpu else cpu_baseline(cfg_path, cells, tmp)

@@ -121,6 +121,7 @@ typedef struct {
     uint64_t* active;
     uint64_t* changed;
     int32_t cap;
+    uint64_t tiles_staged; /* visited tiles whose halo held lava (the rest settle without being read) */
 } vca_ca_result;
 int vca_ca_collect(vca_ca* ca, vca_ca_result* r);
 

@@ -323,6 +323,7 @@ void ca_collect(vca_ca* ca, const Config& cfg, RunSummary& s, bool with_state) {
     s.steps = r.steps;
     s.stop = r.stop_reason;
     s.tiles_visited = r.tiles_visited;
+    s.tiles_staged = r.tiles_staged;
     s.changes = r.changes;
     s.tile_cells = 1024;
 }
@@ -885,6 +886,7 @@ int run_from_cli(const CliOptions& opt, const Clock& since_start) {
         if (s && s->ca_ran) {
             Json w = Json::object();
             w["tilesVisited"] = Json(static_cast<unsigned long long>(s->tiles_visited));
+            w["tilesStaged"] = Json(static_cast<unsigned long long>(s->tiles_staged));
             w["tileCells"] = s->tile_cells;
             w["changes"] = Json(static_cast<unsigned long long>(s->changes));
             uint64_t lava = 0;

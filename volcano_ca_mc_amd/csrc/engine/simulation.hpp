@@ -114,7 +114,7 @@ struct RunSummary {
     int stop = 0;
     std::vector<Vent> vents;
     std::vector<uint64_t> active, changed;
-    uint64_t tiles_visited = 0, changes = 0;
+    uint64_t tiles_visited = 0, tiles_staged = 0, changes = 0;
     int32_t tile_cells = 0;
     CaState result;               // the final state, when something on the host needs it (result files)
     // flowSums of the final state (per flowId, entry 0 unused), computed on the device

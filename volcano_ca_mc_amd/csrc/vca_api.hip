@@ -1080,6 +1080,7 @@ extern "C" int vca_ca_collect(vca_ca* ca, vca_ca_result* r) {
         r->steps = c.steps;
         r->stop_reason = c.stop;
         r->tiles_visited = c.tiles_visited;
+        r->tiles_staged = c.tiles_staged;
         r->changes = c.changes_total;
         const size_t m = static_cast<size_t>(std::min(c.steps, std::max(0, r->cap)));
         if (m > 0 && r->active) staged_download(r->active, ca->hist_a.p, m * 8, ca->side);

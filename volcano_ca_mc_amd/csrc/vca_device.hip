@@ -501,8 +501,8 @@ __global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B, uint32_t s
         const bool uniform = __ballot(slot != s0) == 0;
         ent[k] = 0;
         code[k] = 0;
-        auto visit = [&](const CaSim& sim, uint32_t sl) {
-            if (!sim.running || sim.D.ctrl->stop) return;
+        auto visit = [&](const CaSim& sim, uint32_t sl) -> bool {
+            if (!sim.running || sim.D.ctrl->stop) return false;
             const CaParams& P = sim.P;
             const uint32_t t = g - B.tile_base[sl];
             const uint8_t f = sim.D.tile_flags[t];
@@ -514,8 +514,9 @@ __global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B, uint32_t s
             if (nb && !P.classic) near[t] = 0;
             const bool act =
                 P.dense || lava || (P.classic && (f & kFlagLava) && (f & kFlagAir)) || ((f & kFlagAir) && nb);
-            // the rules stage a visited tile only when it or its 1-cell halo holds lava: its own flag, or a
-            // neighbour's halo byte (mark_halo); the bytes are cleared for the next step in lava mode
+            // a visited tile goes to the rules only when it or its 1-cell halo holds lava: its own flag, or a
+            // neighbour's halo byte (mark_halo); the bytes are cleared for the next step in lava mode.  Any other
+            // visited tile is settled here: none of its cells can change, its flags stay as they are.
             bool stage = (f & kFlagLava) != 0;
             if (act) {
                 uint4* hb = reinterpret_cast<uint4*>(sim.D.tile_halo + static_cast<size_t>(t) * kHaloBytes);
@@ -530,14 +531,21 @@ __global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B, uint32_t s
                     hb[1] = make_uint4(0, 0, 0, 0);
                 }
             }
-            ent[k] = (static_cast<uint64_t>(static_cast<uint32_t>(sim.D.ctrl->steps)) << 32) | (sl << kTileBits) | t |
-                     (stage ? kEntryStage : 0ULL);
-            code[k] = (act ? 1u : 0u) | (lava ? 2u : 0u) | (static_cast<uint32_t>(sim.shape) << 2);
+            ent[k] = (static_cast<uint64_t>(static_cast<uint32_t>(sim.D.ctrl->steps)) << 32) | (sl << kTileBits) | t;
+            code[k] = (act && stage ? 1u : 0u) | (lava ? 2u : 0u) | (static_cast<uint32_t>(sim.shape) << 2);
+            return act;
         };
+        // visited tiles are counted here (staged or settled): one atomic per wave when the wave is one simulation's
         if (uniform) {
-            if (s0 != kNoSlot) visit(g_sims[B.sim_base + s0], s0);
+            if (s0 != kNoSlot) {
+                const CaSim& sim = g_sims[B.sim_base + s0];
+                const unsigned long long m = __ballot(visit(sim, s0));
+                if (m && lane == 0)
+                    atomicAdd(&sim.D.part[(g >> 6) % kPartRows].tiles, static_cast<unsigned long long>(__popcll(m)));
+            }
         } else if (slot != kNoSlot) {
-            visit(g_sims[B.sim_base + slot], slot);
+            const CaSim& sim = g_sims[B.sim_base + slot];
+            if (visit(sim, slot)) atomicAdd(&sim.D.part[(g - B.tile_base[slot]) % kPartRows].tiles, 1ULL);
         }
     }
 #pragma unroll
@@ -904,26 +912,14 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
     const uint32_t nact = B.ctrl->nactive[shape];
     for (uint32_t s = blockIdx.x; s < nact; s += gridDim.x) {
         const unsigned tid = opaque_tid();  // see opaque_tid
-        // uniform: one entry per block; low word slot << kTileBits | tile, then the step and kEntryStage
+        // uniform: one entry per block; low word slot << kTileBits | tile, high word the simulation's step
         const uint32_t e = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(next));
-        const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(next >> 32)));
-        const int32_t t = static_cast<int32_t>(hi & 0xFFFFu);
-        const bool stage = (hi >> 31) != 0;
+        const int32_t t = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(next >> 32));
         if (s + gridDim.x < nact) next = tiles[s + gridDim.x];
         const CaSim& sim = g_sims[B.sim_base + (e >> kTileBits)];
         const CaParams& P = sim.P;
         const CaDev& D = sim.D;
         const uint32_t tile = e & kTileMask;
-        if (!stage) {
-            // neither the tile nor its 1-cell halo holds lava (the tile's flag and its neighbours' halo bytes): no
-            // cell of the tile can change, and its flags stay as they are
-            if (tid == 0) {
-                D.tile_nchg[tile] = 0;
-                D.tile_nwide[tile] = 0;
-                atomicAdd(&D.part[tile % kPartRows].tiles, 1ULL);
-            }
-            continue;
-        }
         const uint64_t prefix = hash_prefix(P.seed, static_cast<uint64_t>(t));
         const bool erupting = t < P.eruption_steps;
         const uint16_t stamp = static_cast<uint16_t>(t < 65535 ? t : 65535);
@@ -962,7 +958,7 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
                 D.tile_flags[tile] = static_cast<uint8_t>(any_air ? kFlagAir : 0);
                 D.tile_nchg[tile] = 0;
                 D.tile_nwide[tile] = 0;
-                atomicAdd(&D.part[tile % kPartRows].tiles, 1ULL);
+                atomicAdd(&D.part[tile % kPartRows].staged, 1ULL);
             }
             continue;
         }
@@ -1219,7 +1215,7 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
             if (r0) atomicAdd(&row->lava_after, static_cast<unsigned long long>(r0));
             if (r1) atomicAdd(&row->born, static_cast<unsigned long long>(r1));
             if (nchg) atomicAdd(&row->changed, static_cast<unsigned long long>(nchg));
-            atomicAdd(&row->tiles, 1ULL);
+            atomicAdd(&row->staged, 1ULL);
         }
     }
 }
@@ -1281,7 +1277,7 @@ __global__ void __launch_bounds__(kBlock) ca_wide_kernel(BatchDev B) {
 
 // Scatters each visited tile's change records into the state (LavaCA.hpp:1052-1066).
 __global__ void __launch_bounds__(kBlock) ca_apply_kernel(BatchDev B) {
-    // one wave per visited tile; tiles settled without staging (no kEntryStage) have no records
+    // one wave per staged tile
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const uint32_t waves = gridDim.x * (kBlock / 64);
 #pragma unroll 1
@@ -1289,10 +1285,7 @@ __global__ void __launch_bounds__(kBlock) ca_apply_kernel(BatchDev B) {
         const uint32_t nact = B.ctrl->nactive[k];
         const uint64_t* tiles = B.active + k * B.cap;
         for (uint32_t s = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); s < nact; s += waves) {
-            const uint64_t ent = tiles[s];
-            const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(ent >> 32)));
-            if (!(hi >> 31)) continue;
-            const uint32_t e = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(ent)));
+            const uint32_t e = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(tiles[s])));
             const CaDev& D = g_sims[B.sim_base + (e >> kTileBits)].D;
             const uint32_t tile = e & kTileMask;
             const uint32_t n = D.tile_nchg[tile];
@@ -1315,7 +1308,7 @@ __global__ void __launch_bounds__(kBlock) ca_apply_kernel(BatchDev B) {
 // finishStep (LavaCA.hpp:1206-1229) + frontCounts (LavaCA.hpp:865-871), one block per slot.  Block 0 also
 // clears the batch's list counters for the next step (every kernel of this step has finished with them).
 __global__ void __launch_bounds__(kBlock) ca_finish_kernel(BatchDev B) {
-    __shared__ unsigned long long red[4][kBlock / 64];
+    __shared__ unsigned long long red[5][kBlock / 64];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         for (int k = 0; k < kShapes; ++k) {
             B.ctrl->nactive[k] = 0;
@@ -1329,34 +1322,38 @@ __global__ void __launch_bounds__(kBlock) ca_finish_kernel(BatchDev B) {
     const CaDev& D = sim.D;
     CaCtrl* c = D.ctrl;
     if (c->stop) return;
-    unsigned long long la = 0, bo = 0, ch = 0, tl = 0;
+    unsigned long long la = 0, bo = 0, ch = 0, tl = 0, st = 0;
     for (int r = static_cast<int>(threadIdx.x); r < kPartRows; r += kBlock) {
         CaPart* row = &D.part[r];
         la += row->lava_after;
         bo += row->born;
         ch += row->changed;
         tl += row->tiles;
-        *row = CaPart{0, 0, 0, 0};  // cleared for the next step
+        st += row->staged;
+        *row = CaPart{0, 0, 0, 0, 0};  // cleared for the next step
     }
     la = wave_sum(la);
     bo = wave_sum(bo);
     ch = wave_sum(ch);
     tl = wave_sum(tl);
+    st = wave_sum(st);
     const int w = static_cast<int>(threadIdx.x >> 6);
     if ((threadIdx.x & 63u) == 0) {
         red[0][w] = la;
         red[1][w] = bo;
         red[2][w] = ch;
         red[3][w] = tl;
+        red[4][w] = st;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    unsigned long long lava_after = 0, born = 0, changed = 0, tiles = 0;
+    unsigned long long lava_after = 0, born = 0, changed = 0, tiles = 0, staged = 0;
     for (int k = 0; k < kBlock / 64; ++k) {
         lava_after += red[0][k];
         born += red[1][k];
         changed += red[2][k];
         tiles += red[3][k];
+        staged += red[4][k];
     }
     const int32_t t = c->steps;
     const unsigned long long active = P.classic ? c->lava_total + born : lava_after;
@@ -1377,6 +1374,7 @@ __global__ void __launch_bounds__(kBlock) ca_finish_kernel(BatchDev B) {
         stop = 4;  // MaxSteps
     c->stop = stop;
     c->tiles_visited += tiles;
+    c->tiles_staged += staged;
     c->changes_total += changed;  // every change is one record
 }
 

@@ -105,12 +105,14 @@ struct CaCtrl {
     int32_t stop;
     unsigned long long lava_total, air_total;
     unsigned long long tiles_visited, changes_total;
+    unsigned long long tiles_staged;
 };
 
-// Counters of one step spread over kPartRows rows (the rules kernel adds a visited tile's counts to row
-// tile % kPartRows; ca_finish sums and clears them), so that no counter is one address every block updates.
+// Counters of one step spread over kPartRows rows (a tile's counts go to row tile % kPartRows: visited tiles
+// from ca_front, the rest from the rules kernel; ca_finish sums and clears them), so that no counter is one
+// address every block updates.
 struct CaPart {
-    unsigned long long lava_after, born, changed, tiles;
+    unsigned long long lava_after, born, changed, tiles, staged;
 };
 constexpr int kPartRows = 256;
 
@@ -145,8 +147,6 @@ struct CaDev {
 };
 constexpr uint8_t kFlagLava = 1, kFlagAir = 2;
 constexpr int kHaloBytes = 32;   // 27 direction bytes per tile, padded
-// Visited-tile entry bit: the tile or its 1-cell halo holds lava, so the rules must stage it (ca_front sets it).
-constexpr uint64_t kEntryStage = 1ULL << 63;
 
 // Tile shapes (64 x TY x TZ): 0 = 4x4, 1 = 2x8, 2 = 8x2 (3D), 3 = 1x16 (x-z plane, ny = 1), 4 = 16x1 (nz = 1).
 constexpr int kShapes = 5;
