@@ -10,7 +10,7 @@
 //                start-of-step state only and write the cell's new state as a record into the tile's own
 //                slot range of the change buffer (LDS counter, no global atomics), count lava/born/changed,
 //                and set the tile's lava/air flags from the new state (LavaCA.hpp:963-1048, 1109-1202).
-//   ca_wide    : the few AIR cells whose vote has more than four distinct flowIds (full vote).
+//   ca_wide    : the few AIR cells whose candidates carry several flowIds (full vote).
 //   ca_apply   : scatters the records of every visited tile into the state (LavaCA.hpp:1052-1066).
 //   ca_finish  : counter rows -> histories, stop rule (LavaCA.hpp:1206-1229).
 // Every kernel first reads the device stop flag and returns when a stop rule already fired, so the host
@@ -213,60 +213,32 @@ __device__ inline uint32_t mod_small(uint64_t h, uint32_t m) {
     return (a * r32 + b) % m;
 }
 
-// Candidate vote with 4 register slots (engine/src/ca/CaRules.hpp:127-183), updated with selects only (no
-// per-lane branches).  More than 4 distinct flowIds among the candidates sets `overflow`; the caller then hands
-// the cell to a full vote.
-struct Vote4 {
-    int n = 0;
+// CandidateVote (engine/src/ca/CaRules.hpp:127-183) in registers for the common case, a cell whose candidates
+// all carry one flowId: the winner is that flowId with the largest candidate temperature (most votes, and no tie
+// to break).  A second flowId among the candidates sets overflow: such cells (where flows meet) are decided by
+// ca_wide.
+struct VoteFirst {
+    int n = 0;             // candidates counted (all of flowId id)
     bool overflow = false;
-    int32_t id[4];
-    int cnt[4];
-    float mt[4];
+    int32_t id = 0;
+    float mt = 0.0f;
     // add(f, t) when ok, nothing otherwise
     __device__ void add_if(bool ok, int32_t f, float t) {
-        bool found = false;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const bool m = ok && s < n && id[s] == f;
-            cnt[s] = m ? cnt[s] + 1 : cnt[s];
-            mt[s] = (m && mt[s] < t) ? t : mt[s];  // std::max(a, b) == (a < b ? b : a)
-            found = found || m;
-        }
-        const bool ins = ok && !found;
-        overflow = overflow || (ins && n == 4);
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const bool here = ins && n == s;
-            id[s] = here ? f : id[s];
-            cnt[s] = here ? 1 : cnt[s];
-            mt[s] = here ? t : mt[s];
-        }
-        n = (ins && n < 4) ? n + 1 : n;
+        const bool first = ok && n == 0;
+        const bool same = ok && (first || f == id);
+        overflow = overflow || (ok && !same);
+        id = first ? f : id;
+        mt = first ? t : ((same && mt < t) ? t : mt);  // std::max(a, b) == (a < b ? b : a)
+        n += same ? 1 : 0;
     }
     __device__ void add(int32_t f, float t) { add_if(true, f, t); }
-    // CandidateVote::winner: most votes; ties -> ascending flowId, pick hash % k.
-    __device__ void winner(uint64_t h_index, int32_t& flow, float& temp) const {
-        int best = 0;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) best = (s < n && cnt[s] > best) ? cnt[s] : best;
-        int k = 0;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) k += (s < n && cnt[s] == best) ? 1 : 0;
-        const uint32_t pick =
-            (n > 1 && k > 1) ? mod_small(splitmix64(h_index ^ kSaltTieBreak), static_cast<uint32_t>(k)) : 0u;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            uint32_t rank = 0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) rank += (q < n && cnt[q] == best && id[q] < id[s]) ? 1u : 0u;
-            const bool take = s < n && cnt[s] == best && rank == pick;
-            flow = take ? id[s] : flow;
-            temp = take ? mt[s] : temp;
-        }
+    __device__ void winner(int32_t& flow, float& temp) const {
+        flow = id;
+        temp = mt;
     }
 };
 
-// Vote over any number of distinct flowIds (private memory; only used when Vote4 overflowed).
+// Vote over any number of distinct flowIds (private memory; the cells VoteFirst left to ca_wide).
 struct VoteWide {
     int n = 0;
     bool overflow = false;
@@ -890,7 +862,7 @@ __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY
 // Rules of every cell of the visited tiles (LavaCA.hpp:963-1048, 1109-1202).  The tile and a 1-cell halo
 // are staged in LDS first (types for every position, then flowId / temperature / D(n) for LAVA positions,
 // all independent coalesced loads), so an AIR cell folds its 26 neighbour types into a bitmask without
-// branches and gathers its candidates from LDS.  Cells whose vote overflows the four register slots are
+// branches and gathers its candidates from LDS.  Cells whose candidates carry more than one flowId are
 // handed to ca_wide.
 // Register budget for five waves per SIMD (five blocks per CU; the LDS tile allows five too): more tiles in
 // flight hide the staging latency better than the few registers the limit costs.
@@ -1141,14 +1113,14 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
                         const uint32_t bits = lava_bits & mask;
                         if (bits) {
                             const int spill_c = has_spill ? T.spill[(lx + 1) + LX * (ly + 1)] : 0;
-                            Vote4 v;
+                            VoteFirst v;
                             gather_lds(P, D, T, c, lz, z, col, plane, spill_c, has_spill, bits, v);
                             has = v.n > 0;
                             if (v.overflow) {
-                                wide = true;  // ca_wide decides this cell
+                                wide = true;  // several flowIds: ca_wide decides this cell
                                 has = false;
                             } else if (has) {
-                                v.winner(h_index, wf, wt);
+                                v.winner(wf, wt);
                             }
                         }
                     }
@@ -1220,7 +1192,7 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
     }
 }
 
-// AIR cells whose candidates carry more than four distinct flowIds (rare): one block per tile that has such
+// AIR cells whose candidates carry more than one flowId (where flows meet): one block per tile that has such
 // cells; the full vote over private memory, then the record and counters the rules kernel left to this
 // kernel.  Reads the start-of-step state only, so it runs between the rules and the apply launches.
 __global__ void __launch_bounds__(kBlock) ca_wide_kernel(BatchDev B) {
