@@ -803,7 +803,6 @@ struct RulesTile {
     int32_t flow[H::N];
     float temp[H::N];
     int32_t spill[H::LXY];
-    int16_t koff[27];           // LDS offset of neighbour position k: dx + LX * dy + LXY * dz
     // LAVA bits of halo row r over all LX = 66 positions (bit p = halo column p), as three 32-bit parts + 0
     uint32_t row66[H::R][4];
 };
@@ -811,51 +810,81 @@ struct RulesTile {
 // gatherLavaCandidates (LavaCA.hpp:465-519) / the classic vote (LavaCA.hpp:1165-1202) for the AIR cell at LDS
 // position c; bits = the cell's neighbour positions (k as in vca_internal.h) that hold LAVA and that the rules
 // read.  Only ponding below the staged layers and deep drop depths touch global memory.
+// LDS offset of neighbour position k (< 27; dx = k % 3 - 1, dy = k / 3 % 3 - 1, dz = k / 9 - 1) in a halo of rows
+// of LX positions and layers of LXY: k / 9 and r / 3 (r < 9) by multiply-shift.
+template <int LX, int LXY>
+__device__ inline int koff(int k) {
+    const int dz = (k * 57) >> 9;
+    const int r = k - 9 * dz;
+    const int dy = (r * 11) >> 5;
+    const int dx = r - 3 * dy;
+    return dx + LX * dy + LXY * dz - (1 + LX + LXY);
+}
+
 template <int TY, int TZ, class V>
 __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY, TZ>& T, int c, int lz, int z,
                            uint32_t col, uint32_t plane, int spill_c, bool has_spill, uint32_t bits, V& v) {
     constexpr int LX = Halo<TY, TZ>::LX, LXY = Halo<TY, TZ>::LXY;
+    // positions are taken two at a time so that their LDS reads are in flight together (the vote does not depend
+    // on the order of its candidates)
     if (P.classic) {
         #pragma clang loop unroll(disable)
         while (bits) {
-            const int k = __ffs(bits) - 1;
+            const int k1 = __builtin_ctz(bits);
             bits &= bits - 1;
-            v.add(T.flow[c + T.koff[k]], 0.0f);
+            const bool two = bits != 0;
+            const int k2 = two ? __builtin_ctz(bits) : k1;
+            bits &= bits - 1;
+            const int32_t f1 = T.flow[c + koff<LX, LXY>(k1)], f2 = T.flow[c + koff<LX, LXY>(k2)];
+            v.add(f1, 0.0f);
+            v.add_if(two, f2, 0.0f);
         }
         return;
     }
+    if (bits & (1u << 4)) {
+        // tryPonding (LavaCA.hpp:443-461) into the cell from directly below, the only dz < 0 position in the
+        // lava-mode mask
+        bits &= ~(1u << 4);
+        const int n = c - LXY;
+        bool ok = !(z > spill_c || z == 0);
+        if (ok && z - 1 > 0) {  // hasSupport(z - 1)
+            const uint8_t b2 = lz >= 1 ? T.S.type[c - 2 * LXY] : D.type[col + plane * static_cast<uint32_t>(z - 2)];
+            ok = b2 == kRock || b2 == kBasalt || (b2 == kLava && has_spill && z - 2 < spill_c);
+        }
+        if (ok) {
+            const uint8_t h0 = T.S.type[n - 1], h1 = T.S.type[n + 1];
+            bool closed = h0 != kAir && h0 != kVoidLds && h1 != kAir && h1 != kVoidLds;
+            if (P.ny > 1) {
+                const uint8_t h2 = T.S.type[n - LX], h3 = T.S.type[n + LX];
+                closed = closed && h2 != kAir && h2 != kVoidLds && h3 != kAir && h3 != kVoidLds;
+            }
+            ok = closed;
+        }
+        v.add_if(ok, T.flow[n], T.temp[n] - P.cool_lat);
+    }
     // dropDepth of the cell, once, when a side or diagonal-up neighbour may need it (pure function of the cell)
-    const int cdrop = (bits & ~((1u << 4) | (1u << 22)))
-                          ? drop_depth_lds<TY, TZ>(D, T.S.type, c, lz, z, col, plane, spill_c, has_spill)
-                          : -100;
+    const int cdrop = (bits & ~(1u << 22)) ? drop_depth_lds<TY, TZ>(D, T.S.type, c, lz, z, col, plane, spill_c, has_spill)
+                                           : -100;
+    const bool diag = P.nb.diagonal_falls != 0;
+    // k >= 18: dz > 0 (k == 22 directly above, a free fall); else dz == 0
+    auto accept = [&](int k, int d) {
+        const bool up = k >= 18;
+        const bool ok = d >= 0 && (up ? (cdrop == 0 && d == 1) : (cdrop == d && !(d == 1 && diag)));
+        return k == 22 || ok;
+    };
     #pragma clang loop unroll(disable)
     while (bits) {
-        const int k = __ffs(bits) - 1;
+        const int k1 = __builtin_ctz(bits);
         bits &= bits - 1;
-        const int n = c + T.koff[k];  // dx + LX * dy + LXY * dz of position k
-        // k >= 18: dz > 0 (k == 22 directly above); k == 4: directly below (ponding); else dz == 0
-        const bool up = k >= 18;
-        const int d = T.desc[n];
-        bool ok = d >= 0 && (up ? (cdrop == 0 && d == 1) : (cdrop == d && !(d == 1 && P.nb.diagonal_falls)));
-        ok = k == 22 || ok;
-        if (k == 4) {
-            // tryPonding (LavaCA.hpp:443-461); only k == 4 of the dz < 0 positions is in the mask in lava mode
-            ok = !(z > spill_c || z == 0);
-            if (ok && z - 1 > 0) {  // hasSupport(z - 1)
-                const uint8_t b2 = lz >= 1 ? T.S.type[c - 2 * LXY] : D.type[col + plane * static_cast<uint32_t>(z - 2)];
-                ok = b2 == kRock || b2 == kBasalt || (b2 == kLava && has_spill && z - 2 < spill_c);
-            }
-            if (ok) {
-                const uint8_t h0 = T.S.type[n - 1], h1 = T.S.type[n + 1];
-                bool closed = h0 != kAir && h0 != kVoidLds && h1 != kAir && h1 != kVoidLds;
-                if (P.ny > 1) {
-                    const uint8_t h2 = T.S.type[n - LX], h3 = T.S.type[n + LX];
-                    closed = closed && h2 != kAir && h2 != kVoidLds && h3 != kAir && h3 != kVoidLds;
-                }
-                ok = closed;
-            }
-        }
-        v.add_if(ok, T.flow[n], T.temp[n] - (up ? P.cool_fall : P.cool_lat));
+        const bool two = bits != 0;
+        const int k2 = two ? __builtin_ctz(bits) : k1;
+        bits &= bits - 1;
+        const int n1 = c + koff<LX, LXY>(k1), n2 = c + koff<LX, LXY>(k2);
+        const int d1 = T.desc[n1], d2 = T.desc[n2];
+        const int32_t f1 = T.flow[n1], f2 = T.flow[n2];
+        const float t1 = T.temp[n1], t2 = T.temp[n2];
+        v.add_if(accept(k1, d1), f1, t1 - (k1 >= 18 ? P.cool_fall : P.cool_lat));
+        v.add_if(two && accept(k2, d2), f2, t2 - (k2 >= 18 ? P.cool_fall : P.cool_lat));
     }
 }
 
@@ -874,10 +903,6 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
     __shared__ uint32_t s_nchg, s_nlist, s_nwide;
     __shared__ uint32_t s_red[3][kBlock / 64];
     __shared__ uint32_t s_wflags[2];  // per-wave flag bytes: [0] staging, [1] end of tile
-    if (threadIdx.x < 27) {
-        const int k = static_cast<int>(threadIdx.x);
-        T.koff[k] = static_cast<int16_t>((k % 3 - 1) + LX * ((k / 3) % 3 - 1) + H::LXY * (k / 9 - 1));
-    }
     const uint64_t* tiles = B.active + shape * B.cap;
     // the first entry is read together with the list length (blockIdx.x < gridDim.x <= cap: in bounds)
     uint64_t next = tiles[blockIdx.x];
