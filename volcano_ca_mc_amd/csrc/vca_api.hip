@@ -287,6 +287,7 @@ struct vca_ca {
         tile_nwide, part, ctrl, hist_a, hist_c;
     uint64_t n = 0;
     std::unique_ptr<Batch> own;   // batch of one: the solo path (made on the first solo enqueue)
+    bool flow16 = true;           // every flowId the simulation can hold (its vents') fits int16_t
     hipStream_t side = nullptr;   // during vca_ca_collect: the pooled stream its reads go on
     Batch* member = nullptr;      // the shared batch this simulation is in
     int slot = -1;
@@ -446,6 +447,11 @@ struct Batch {
             if (c && !c->P.classic) return true;
         return false;
     }
+    bool flow16() const {
+        for (const vca_ca* c : members)
+            if (c && !c->flow16) return false;
+        return true;
+    }
     int count() const {
         int k = 0;
         for (const vca_ca* c : members) k += c != nullptr;
@@ -533,10 +539,10 @@ struct Batch {
     void launch_direct(int steps) {
         vca_ctx* c = ctx;
         const uint32_t sh = shapes();
-        const bool lm = any_lava_mode();
+        const bool lm = any_lava_mode(), f16 = flow16();
         for (int k = 0; k < steps; ++k) {
             if (!c->time_ca_step()) {
-                ca_launch_step(B, sh, lm, grid_tiles, c->stream, nullptr);
+                ca_launch_step(B, sh, lm, f16, grid_tiles, c->stream, nullptr);
                 continue;
             }
             hipEvent_t ev[7];
@@ -545,7 +551,7 @@ struct Batch {
                 ev[e] = c->take();
                 idx[e] = c->used - 1;
             }
-            ca_launch_step(B, sh, lm, grid_tiles, c->stream, ev);
+            ca_launch_step(B, sh, lm, f16, grid_tiles, c->stream, ev);
             c->mark("ca_front", idx[0], idx[1]);
             if (lm) c->mark("ca_descent", idx[1], idx[2]);
             c->mark("ca_rules", idx[2], idx[3]);
@@ -567,8 +573,8 @@ struct Batch {
             return false;
         }
         const uint32_t sh = shapes();
-        const bool lm = any_lava_mode();
-        for (int k = 0; k < steps_per_graph; ++k) ca_launch_step(B, sh, lm, grid_tiles, s, nullptr);
+        const bool lm = any_lava_mode(), f16 = flow16();
+        for (int k = 0; k < steps_per_graph; ++k) ca_launch_step(B, sh, lm, f16, grid_tiles, s, nullptr);
         // a capture invalidated by the runtime leaves nothing enqueued: the caller launches the steps directly
         if (hipStreamEndCapture(s, &g) != hipSuccess || g == nullptr) {
             (void)hipGetLastError();
@@ -874,6 +880,9 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         std::vector<float> vtemp(static_cast<size_t>(nvents), classic ? 0.0f : d->erupt_temp);
         for (int k = 0; k < nvents; ++k) {
             const vca_vent& v = vents[k];
+            // cells take their flowId from a vent (seed) or from a neighbour's vote: the vents' ids are every
+            // flowId the simulation can hold
+            ca->flow16 = ca->flow16 && v.flow_id >= INT16_MIN && v.flow_id <= INT16_MAX;
             require(v.x >= 0 && v.x < d->nx && v.y >= 0 && v.y < d->ny && v.z >= 0 && v.z < d->nz,
                     "krater poza siatką");
             vcell[k] = static_cast<uint32_t>(v.x + static_cast<uint64_t>(d->nx) * (v.y + static_cast<uint64_t>(d->ny) * v.z));

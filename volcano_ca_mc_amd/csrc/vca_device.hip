@@ -48,11 +48,19 @@ __device__ inline unsigned long long wave_sum(unsigned long long v) {
 
 // threadIdx.x as a value the compiler cannot hoist out of a loop: per-lane index arithmetic derived from it is
 // recomputed where it is used (a few instructions) instead of being kept live -- and spilled -- across a loop.
-// The asm block is empty (no instruction, no memory access).
+// The lane comes from two v_mbcnt instructions in an asm block (no memory access), the wave's first thread from
+// a scalar register, so no vector register holds the thread index across the loop either.
 __device__ inline unsigned opaque_tid() {
-    unsigned t = threadIdx.x;
-    asm volatile("" : "+v"(t));
-    return t;
+    unsigned lane;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane));
+    return (static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x))) & ~63u) | lane;
+}
+// 0 that the compiler cannot hoist out of a loop (a hoisted constant held in a vector register across the loop
+// gets spilled).  One v_mov in an asm block.
+__device__ inline uint32_t opaque_zero() {
+    uint32_t z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return z;
 }
 
 // v of lane `lane` (uniform) into scalar registers.
@@ -794,13 +802,14 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
 
 // Tile of the rules kernel in LDS: types of every halo position, flowId / temperature / D(n) of the LAVA
 // positions, spill heights of the halo columns.
-template <int TY, int TZ>
+// F: the LDS type of flowIds (int16_t when every flowId of the batch fits, which keeps six tiles per CU)
+template <int TY, int TZ, class F>
 struct RulesTile {
     using H = Halo<TY, TZ>;
     HaloTypes<TY, TZ> S;
     uint16_t list[kTileCells];  // candidate cells of the tile (tile-local index lx + 64 * row)
     int8_t desc[H::N];
-    int32_t flow[H::N];
+    F flow[H::N];
     float temp[H::N];
     int32_t spill[H::LXY];
     // LAVA bits of halo row r over all LX = 66 positions (bit p = halo column p), as three 32-bit parts + 0
@@ -821,8 +830,8 @@ __device__ inline int koff(int k) {
     return dx + LX * dy + LXY * dz - (1 + LX + LXY);
 }
 
-template <int TY, int TZ, class V>
-__device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY, TZ>& T, int c, int lz, int z,
+template <int TY, int TZ, class F, class V>
+__device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY, TZ, F>& T, int c, int lz, int z,
                            uint32_t col, uint32_t plane, int spill_c, bool has_spill, uint32_t bits, V& v) {
     constexpr int LX = Halo<TY, TZ>::LX, LXY = Halo<TY, TZ>::LXY;
     // positions are taken two at a time so that their LDS reads are in flight together (the vote does not depend
@@ -893,13 +902,17 @@ __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY
 // all independent coalesced loads), so an AIR cell folds its 26 neighbour types into a bitmask without
 // branches and gathers its candidates from LDS.  Cells whose candidates carry more than one flowId are
 // handed to ca_wide.
-// Register budget for five waves per SIMD (five blocks per CU; the LDS tile allows five too): more tiles in
-// flight hide the staging latency better than the few registers the limit costs.
-template <int TY, int TZ>
-__global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int shape) {
+// Register budget: as many waves per SIMD as tiles per CU fit in LDS (the 4 x 4 tile with 16-bit flowIds: six,
+// 80 registers; the others five at most, 96 registers) -- more tiles in flight hide the staging latency better
+// than the few registers the limit costs.
+template <int TY, int TZ, class F>
+constexpr int kRulesWaves = (TY == 4 && TZ == 4 && sizeof(F) == 2) ? 6 : 5;
+
+template <int TY, int TZ, class F>
+__global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_kernel(BatchDev B, int shape) {
     using H = Halo<TY, TZ>;
     constexpr int LX = H::LX, LY = H::LY;
-    __shared__ RulesTile<TY, TZ> T;
+    __shared__ RulesTile<TY, TZ, F> T;
     __shared__ uint32_t s_nchg, s_nlist, s_nwide;
     __shared__ uint32_t s_red[3][kBlock / 64];
     __shared__ uint32_t s_wflags[2];  // per-wave flag bytes: [0] staging, [1] end of tile
@@ -936,9 +949,10 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
         const int gx = halo_x(P, o.x0 + static_cast<int>(tid & 63u));
         const bool saw_lava = stage_rows<TY, TZ>(P, D, o, T.S, has_spill, T.spill, own_air, row_base, gx);
         if (tid == 0) {
-            s_nchg = 0;
-            s_nlist = 0;
-            s_nwide = 0;
+            const uint32_t zero = opaque_zero();
+            s_nchg = zero;
+            s_nlist = zero;
+            s_nwide = zero;
         }
         {
             // block OR of saw_lava (bit 0) and own_air (bit 1): one flag byte per wave
@@ -1041,14 +1055,14 @@ __global__ void __launch_bounds__(kBlock, 5) ca_rules_kernel(BatchDev B, int sha
             for (int k = 0; k < H::KR; ++k) {
                 if ((lava >> k) & 1u) {
                     const int q = lx + 1 + H::LX * (w + 4 * k);
-                    T.flow[q] = fl[k];
+                    T.flow[q] = static_cast<F>(fl[k]);
                     T.temp[q] = tp[k];
                     T.desc[q] = static_cast<int8_t>(ds[k]);
                 }
             }
             if (elava) {
                 const int q = ((t & 1) ? H::LX - 1 : 0) + H::LX * (t >> 1);
-                T.flow[q] = efl;
+                T.flow[q] = static_cast<F>(efl);
                 T.temp[q] = etp;
                 T.desc[q] = static_cast<int8_t>(eds);
             }
@@ -1699,8 +1713,19 @@ void ca_launch_init_flags(const CaParams& P, const CaDev& D, hipStream_t s) {
         }                                                                                              \
     } while (0)
 
-void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, int grid_tiles, hipStream_t s,
-                    hipEvent_t* ev) {
+#define VCA_RULES_DISPATCH(k, F, ...)                                                                  \
+    do {                                                                                               \
+        switch (k) {                                                                                   \
+            case 0: hipLaunchKernelGGL((ca_rules_kernel<4, 4, F>), __VA_ARGS__, 0); break;            \
+            case 1: hipLaunchKernelGGL((ca_rules_kernel<2, 8, F>), __VA_ARGS__, 1); break;            \
+            case 2: hipLaunchKernelGGL((ca_rules_kernel<8, 2, F>), __VA_ARGS__, 2); break;            \
+            case 3: hipLaunchKernelGGL((ca_rules_kernel<1, 16, F>), __VA_ARGS__, 3); break;           \
+            default: hipLaunchKernelGGL((ca_rules_kernel<16, 1, F>), __VA_ARGS__, 4); break;          \
+        }                                                                                              \
+    } while (0)
+
+void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, bool flow16, int grid_tiles,
+                    hipStream_t s, hipEvent_t* ev) {
     const unsigned gt = static_cast<unsigned>(grid_tiles);
     if (ev) hipEventRecord(ev[0], s);
     hipLaunchKernelGGL(ca_front_kernel, dim3(grid_for((B.ntiles + kFrontPer - 1) / kFrontPer)), dim3(kBlock), 0, s,
@@ -1711,7 +1736,12 @@ void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, int 
             if ((shapes >> k) & 1u) VCA_SHAPE_DISPATCH(k, ca_descent_kernel, dim3(gt), dim3(kBlock), 0, s, B);
     if (ev) hipEventRecord(ev[2], s);
     for (int k = 0; k < kShapes; ++k)
-        if ((shapes >> k) & 1u) VCA_SHAPE_DISPATCH(k, ca_rules_kernel, dim3(gt), dim3(kBlock), 0, s, B);
+        if ((shapes >> k) & 1u) {
+            if (flow16)
+                VCA_RULES_DISPATCH(k, int16_t, dim3(gt), dim3(kBlock), 0, s, B);
+            else
+                VCA_RULES_DISPATCH(k, int32_t, dim3(gt), dim3(kBlock), 0, s, B);
+        }
     if (ev) hipEventRecord(ev[3], s);
     hipLaunchKernelGGL(ca_wide_kernel, dim3(kWideBlocks), dim3(kBlock), 0, s, B);
     if (ev) hipEventRecord(ev[4], s);

@@ -224,9 +224,10 @@ void ca_launch_seeds(const CaParams& P, const CaDev& D, const uint32_t* cells, c
 void ca_launch_init_flags(const CaParams& P, const CaDev& D, hipStream_t s);
 void ca_launch_init_stop(const CaParams& P, const CaDev& D, hipStream_t s);
 // One step of every running simulation of the batch: front lists, D(n), rules, wide votes, apply, finish.
-// shapes = bit mask of the tile shapes present; ev != nullptr records 7 events around the kernel groups.
-void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, int grid_tiles, hipStream_t s,
-                    hipEvent_t* ev);
+// shapes = bit mask of the tile shapes present; flow16 = every flowId of the batch fits int16_t (the rules kernel
+// then holds them in 16 bits of LDS); ev != nullptr records 7 events around the kernel groups.
+void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, bool flow16, int grid_tiles,
+                    hipStream_t s, hipEvent_t* ev);
 // owner[base .. base + n) = slot
 void ca_launch_owner(uint32_t* owner, uint32_t base, uint32_t n, uint32_t slot, hipStream_t s);
 // out[2*s] = steps, out[2*s+1] = stop (-1 when slot s is not running) for s < nslots.
