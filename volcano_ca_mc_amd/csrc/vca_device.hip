@@ -582,7 +582,6 @@ struct Halo {
     static constexpr int LX = kTileX + 2, LY = TY + 2, LZ = TZ + 2, LXY = LX * LY, N = LXY * LZ;
     static constexpr int R = LY * LZ;                                   // halo rows
     static constexpr int KR = (R + kBlock / 64 - 1) / (kBlock / 64);    // rows per wave
-    static constexpr int KC = (LXY + kBlock - 1) / kBlock;              // spill column rounds per thread
     static_assert(2 * R <= kBlock, "edge columns: one thread each");
     static_assert(TY * TZ == 16, "four waves x four rows per tile");
 };
@@ -620,7 +619,8 @@ struct HaloTypes {
 };
 
 // Stages the types of the tile at o and its halo, row by row, with the row lava masks, and (with_spill)
-// the spill heights of the LX x LY halo columns (0 where a column holds no cells).  Every load of a thread
+// the spill heights of the LX x LY halo columns, or with OWN of the tile's own 64 x TY columns only (index
+// lx + 64 * ly; 0 where a column holds no cells).  Every load of a thread
 // is issued before its first LDS store, so a tile costs one memory latency.  Returns whether this thread saw
 // lava; own_air |= this thread saw AIR among the tile's own cells.
 // Halo row bases of the tile at o, one per lane: lane j < R holds halo_row(j).  Rows are then read with
@@ -634,10 +634,12 @@ __device__ inline uint32_t row_of(uint32_t row_base, int r) {
     return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(row_base), r));
 }
 
-template <int TY, int TZ>
+template <int TY, int TZ, bool OWN = false>
 __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileCoord& o, HaloTypes<TY, TZ>& S,
                                   bool with_spill, int32_t* s_spill, int& own_air, uint32_t row_base, int gx) {
     using H = Halo<TY, TZ>;
+    constexpr int NS = OWN ? kTileX * TY : H::LXY;     // spill columns staged
+    constexpr int KS = (NS + kBlock - 1) / kBlock;     // per thread
     const int t = static_cast<int>(opaque_tid());
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int lx = t & 63;
@@ -660,16 +662,23 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
             if (base != kNoCell && ex >= 0) e = D.type[base + static_cast<uint32_t>(ex)];
         }
     }
-    int32_t sp[H::KC];
+    int32_t sp[KS];
     if (with_spill) {
 #pragma unroll
-        for (int k = 0; k < H::KC; ++k) {
+        for (int k = 0; k < KS; ++k) {
             const int q = t + k * kBlock;
             sp[k] = 0;
-            if (q < H::LXY) {
-                int sx = o.x0 - 1 + q % H::LX, sy = o.y0 - 1 + q / H::LX;
-                if (sx <= P.nx && sy <= P.ny && wrap_axis(sx, P.nx, P.per[0]) && wrap_axis(sy, P.ny, P.per[1]))
-                    sp[k] = D.spill[static_cast<uint32_t>(sx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(sy)];
+            if (q < NS) {
+                if (OWN) {
+                    const int sx = o.x0 + q % kTileX, sy = o.y0 + q / kTileX;
+                    if (sx < P.nx && sy < P.ny)
+                        sp[k] = D.spill[static_cast<uint32_t>(sx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(sy)];
+                } else {
+                    int sx = o.x0 - 1 + q % H::LX, sy = o.y0 - 1 + q / H::LX;
+                    if (sx <= P.nx && sy <= P.ny && wrap_axis(sx, P.nx, P.per[0]) && wrap_axis(sy, P.ny, P.per[1]))
+                        sp[k] = D.spill[static_cast<uint32_t>(sx) +
+                                        static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(sy)];
+                }
             }
         }
     }
@@ -693,9 +702,9 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
     }
     if (with_spill) {
 #pragma unroll
-        for (int k = 0; k < H::KC; ++k) {
+        for (int k = 0; k < KS; ++k) {
             const int q = t + k * kBlock;
-            if (q < H::LXY) s_spill[q] = sp[k];
+            if (q < NS) s_spill[q] = sp[k];
         }
     }
     return lava;
@@ -803,7 +812,7 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
 }
 
 // Tile of the rules kernel in LDS: types of every halo position, flowId / temperature / D(n) of the LAVA
-// positions, spill heights of the halo columns.
+// positions, spill heights of the tile's own columns.
 // F: the LDS type of flowIds (int16_t when every flowId of the batch fits, which keeps six tiles per CU)
 template <int TY, int TZ, class F>
 struct RulesTile {
@@ -813,7 +822,7 @@ struct RulesTile {
     int8_t desc[H::N];
     F flow[H::N];
     float temp[H::N];
-    int32_t spill[H::LXY];
+    int32_t spill[kTileX * TY];  // spill heights of the tile's own columns (lx + 64 * ly)
     // LAVA bits of halo row r over all LX = 66 positions (bit p = halo column p), as three 32-bit parts + 0
     uint32_t row66[H::R][4];
 };
@@ -949,7 +958,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         // tile, -1 for no cell)
         const uint32_t row_base = halo_row_lanes<TY, TZ>(P, o);
         const int gx = halo_x(P, o.x0 + static_cast<int>(tid & 63u));
-        const bool saw_lava = stage_rows<TY, TZ>(P, D, o, T.S, has_spill, T.spill, own_air, row_base, gx);
+        const bool saw_lava = stage_rows<TY, TZ, true>(P, D, o, T.S, has_spill, T.spill, own_air, row_base, gx);
         if (tid == 0) {
             const uint32_t zero = opaque_zero();
             s_nchg = zero;
@@ -1153,7 +1162,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                                                            static_cast<uint32_t>(P.nb.variants)));
                         const uint32_t bits = lava_bits & mask;
                         if (bits) {
-                            const int spill_c = has_spill ? T.spill[(lx + 1) + LX * (ly + 1)] : 0;
+                            const int spill_c = has_spill ? T.spill[lx + kTileX * ly] : 0;
                             VoteFirst v;
                             gather_lds(P, D, T, c, lz, z, col, plane, spill_c, has_spill, bits, v);
                             has = v.n > 0;
