@@ -924,7 +924,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
     using H = Halo<TY, TZ>;
     constexpr int LX = H::LX, LY = H::LY;
     __shared__ RulesTile<TY, TZ, F> T;
-    __shared__ uint32_t s_nchg, s_nlist, s_nwide;
+    __shared__ uint32_t s_nchg, s_nlist, s_nair, s_nwide;
     __shared__ uint32_t s_red[3][kBlock / 64];
     __shared__ uint32_t s_wflags[2];  // per-wave flag bytes: [0] staging, [1] end of tile
     const uint64_t* tiles = B.active + shape * B.cap;
@@ -963,6 +963,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             const uint32_t zero = opaque_zero();
             s_nchg = zero;
             s_nlist = zero;
+            s_nair = zero;
             s_nwide = zero;
         }
         {
@@ -1056,10 +1057,15 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                     // positions past the grid's far side hold wrapped halo cells of a periodic partial tile
                     const bool inside = o.x0 + lx < P.nx && o.y0 + row % TY < P.ny && o.z0 + row / TY < P.nz;
                     const uint8_t ty = inside ? T.S.type[(lx + 1) + LX * rc] : kVoidLds;
-                    const bool cand = ty == kLava || (ty == kAir && ((near >> lx) & 1u));
-                    if (ty == kAir && !cand) tile_air = 1;
-                    const uint32_t slot = wave_append(cand, &s_nlist);
-                    if (cand) T.list[slot] = static_cast<uint16_t>(lx + 64 * row);
+                    // LAVA candidates fill the list from the front, AIR ones from the back: each wave of the
+                    // candidate loop then runs one rule, not both
+                    const bool cl = ty == kLava;
+                    const bool ca = ty == kAir && ((near >> lx) & 1u);
+                    if (ty == kAir && !ca) tile_air = 1;
+                    const uint32_t sl = wave_append(cl, &s_nlist);
+                    const uint32_t sa = wave_append(ca, &s_nair);
+                    if (cl) T.list[sl] = static_cast<uint16_t>(lx + 64 * row);
+                    if (ca) T.list[kTileCells - 1 - sa] = static_cast<uint16_t>(lx + 64 * row);
                 }
             }
 #pragma unroll
@@ -1088,15 +1094,17 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             }
         }
         __syncthreads();  // list and lava data of the tile are in LDS
-        const uint32_t nlist = s_nlist;
+        // candidate slots: LAVA cells at [0, nlava), AIR cells from the next wave boundary on (list order reversed)
+        const uint32_t nlava = s_nlist, abase = (nlava + 63u) & ~63u, nslots = abase + s_nair;
         // directions in which the tile holds lava on its boundary after the step (mark_halo): cells that stay or
         // become LAVA, and the cells left to ca_wide (either outcome)
         const int xm = min(kTileX - 1, P.nx - 1 - o.x0), ym = min(TY - 1, P.ny - 1 - o.y0),
                   zm = min(TZ - 1, P.nz - 1 - o.z0);
         uint32_t bnd = 0;
-        for (uint32_t j0 = 0; j0 < nlist; j0 += kBlock) {
+        for (uint32_t j0 = 0; j0 < nslots; j0 += kBlock) {
             const uint32_t j = j0 + tid;
-            const int e = j < nlist ? T.list[j] : 0;
+            const bool live = j < nlava || (j >= abase && j < nslots);
+            const int e = live ? T.list[j < nlava ? j : kTileCells - 1 - (j - abase)] : 0;
             const int lx = e & 63, row = e >> 6;
             const int ly = row % TY, lz = row / TY;
             const int x = o.x0 + lx, y = o.y0 + ly, z = o.z0 + lz;
@@ -1104,7 +1112,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             bool want = false, wide = false, stay = false, newborn = false;
             Change ch;
             uint32_t i = 0;
-            if (j < nlist) {
+            if (live) {
                 const uint32_t col = static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y);
                 i = col + plane * static_cast<uint32_t>(z);
                 const uint8_t ty = T.S.type[c];
