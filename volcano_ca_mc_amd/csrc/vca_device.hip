@@ -246,6 +246,40 @@ struct VoteFirst {
     }
 };
 
+// CandidateVote for candidates carrying at most two flowIds (where two flows meet): most votes wins; a tie between
+// the two picks by the hash among them in ascending flowId order (engine/src/ca/CaRules.hpp:127-183); the
+// winner's temperature is the largest of its candidates.  A third flowId sets overflow (ca_wide decides).
+struct VoteTwo {
+    int n = 0, n2 = 0;     // candidates of id / of id2
+    bool overflow = false;
+    int32_t id = 0, id2 = 0;
+    float mt = 0.0f, mt2 = 0.0f;
+    __device__ void add_if(bool ok, int32_t f, float t) {
+        const bool first = ok && n == 0;
+        const bool one = ok && !first && f == id;
+        const bool rest = ok && !first && !one;
+        const bool first2 = rest && n2 == 0;
+        const bool two = rest && !first2 && f == id2;
+        overflow = overflow || (rest && !first2 && !two);
+        id = first ? f : id;
+        mt = first ? t : ((one && mt < t) ? t : mt);  // std::max(a, b) == (a < b ? b : a)
+        n += (first || one) ? 1 : 0;
+        id2 = first2 ? f : id2;
+        mt2 = first2 ? t : ((two && mt2 < t) ? t : mt2);
+        n2 += (first2 || two) ? 1 : 0;
+    }
+    __device__ void add(int32_t f, float t) { add_if(true, f, t); }
+    __device__ void winner(uint64_t h_index, int32_t& flow, float& temp) const {
+        bool take2 = n2 > n;
+        if (n2 > 0 && n2 == n) {
+            const uint32_t pick = mod_small(splitmix64(h_index ^ kSaltTieBreak), 2u);
+            take2 = (pick == 1u) == (id2 > id);  // pick 0: the smaller flowId
+        }
+        flow = take2 ? id2 : id;
+        temp = take2 ? mt2 : mt;
+    }
+};
+
 // Vote over any number of distinct flowIds (private memory; the cells VoteFirst left to ca_wide).
 struct VoteWide {
     int n = 0;
@@ -1175,8 +1209,13 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                             gather_lds(P, D, T, c, lz, z, col, plane, spill_c, has_spill, bits, v);
                             has = v.n > 0;
                             if (v.overflow) {
-                                wide = true;  // several flowIds: ca_wide decides this cell
-                                has = false;
+                                // a second flowId (flows meet): the gather again with two slots; a third one
+                                // leaves the cell to ca_wide
+                                VoteTwo v2;
+                                gather_lds(P, D, T, c, lz, z, col, plane, spill_c, has_spill, bits, v2);
+                                has = !v2.overflow;
+                                wide = v2.overflow;
+                                if (has) v2.winner(h_index, wf, wt);
                             } else if (has) {
                                 v.winner(wf, wt);
                             }
