@@ -1496,11 +1496,37 @@ __device__ inline void mc_trial(const McParams& P, S* spin, uint32_t g, uint64_t
     int32_t prop = 1 + static_cast<int32_t>(qm < 65536u ? mod_small(ho, qm) : ho % qm);
     if (prop >= cur) ++prop;
     const uint32_t mask = P.nb.mask[var];
+    // every neighbour read issued before the first is used (no branch around a load: a position outside the mask
+    // or past an absorbing edge reads the cell itself and is not counted), each into a 32-bit register so that no
+    // result is packed -- and waited for -- early
+    int wx[3], wy[3], wz[3];
+    bool ox[3], oy[3], oz[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        wx[d] = x + d - 1;
+        ox[d] = wrap_axis(wx[d], P.nx, P.per[0]);
+        wy[d] = y + d - 1;
+        oy[d] = wrap_axis(wy[d], P.ny, P.per[1]);
+        wz[d] = z + d - 1;
+        oz[d] = wrap_axis(wz[d], P.nz, P.per[2]);
+    }
+    uint32_t sv[27];
+    bool use[27];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+        if (k == 13) continue;
+        const int dx = k % 3, dy = (k / 3) % 3, dz = k / 9;
+        use[k] = ((mask >> k) & 1u) && ox[dx] && oy[dy] && oz[dz];
+        const uint32_t q = static_cast<uint32_t>(wx[dx]) +
+                           static_cast<uint32_t>(P.nx) * (static_cast<uint32_t>(wy[dy]) +
+                                                          static_cast<uint32_t>(P.ny) * static_cast<uint32_t>(wz[dz]));
+        sv[k] = static_cast<uint32_t>(spin[use[k] ? q : g]);
+    }
     int in_dom = 0, same_a = 0, same_b = 0;
 #pragma unroll
     for (int k = 0; k < 27; ++k) {
-        if (k == 13 || !((mask >> k) & 1u)) continue;
-        const int32_t sj = nb_spin(P, spin, x, y, z, k);
+        if (k == 13) continue;
+        const int32_t sj = use[k] ? static_cast<int32_t>(static_cast<S>(sv[k])) : 0;
         in_dom += sj != 0;
         same_a += sj == cur;
         same_b += sj == prop;
