@@ -70,7 +70,8 @@ __device__ inline uint64_t readlane64(uint64_t v, int lane) {
     return static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
 }
 
-// Wave-aggregated slot reservation: every lane of the wave must call this (pred may differ per lane).
+// Wave-aggregated slot reservation: every lane of the wave must call this (pred may differ per lane).  The
+// leader's reserved base reaches the other lanes by v_readlane (no second LDS round trip).
 __device__ inline uint32_t wave_append(bool pred, uint32_t* counter) {
     const unsigned long long mask = __ballot(pred);
     const int lane = static_cast<int>(threadIdx.x & 63u);
@@ -78,9 +79,33 @@ __device__ inline uint32_t wave_append(bool pred, uint32_t* counter) {
     if (mask != 0) {
         const int leader = __ffsll(static_cast<long long>(mask)) - 1;
         if (lane == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
-        base = __shfl(base, leader);
+        base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(base), leader));
     }
     return base + static_cast<uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL)));
+}
+
+// K appends of one wave (slot[k] for pred[k]) with one atomic: all 64 lanes must be active.
+template <int K>
+__device__ inline void wave_append_k(const bool (&pred)[K], uint32_t* counter, uint32_t (&slot)[K]) {
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    unsigned long long m[K];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        m[k] = __ballot(pred[k]);
+        tot += static_cast<uint32_t>(__popcll(m[k]));
+    }
+    uint32_t base = 0;
+    if (tot != 0) {
+        if (lane == 0) base = atomicAdd(counter, tot);
+        base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(base), 0));
+    }
+    const unsigned long long below = (1ULL << lane) - 1ULL;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        slot[k] = base + static_cast<uint32_t>(__popcll(m[k] & below));
+        base += static_cast<uint32_t>(__popcll(m[k]));
+    }
 }
 
 // Reduces N per-thread values over the block (a sum, or a max where bit i of max_mask is set) and folds the
@@ -797,16 +822,20 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
             // the tile's lava cells into a list (tile-local index lx + 64 * row)
             const int lx = static_cast<int>(threadIdx.x & 63u);
             const int w = static_cast<int>(threadIdx.x >> 6);
+            bool lava[4];
+            uint32_t slot[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int row = w + 4 * k;
                 const int rc = (row % TY + 1) + H::LY * (row / TY + 1);
                 // positions past the grid's far side hold wrapped halo cells of a periodic partial tile
                 const bool inside = o.x0 + lx < P.nx && o.y0 + row % TY < P.ny && o.z0 + row / TY < P.nz;
-                const bool lava = inside && ((S.lava[rc] >> lx) & 1u);
-                const uint32_t slot = wave_append(lava, &s_n);
-                if (lava) s_list[slot] = static_cast<uint16_t>(lx + 64 * row);
+                lava[k] = inside && ((S.lava[rc] >> lx) & 1u);
             }
+            wave_append_k(lava, &s_n, slot);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (lava[k]) s_list[slot[k]] = static_cast<uint16_t>(lx + 64 * (w + 4 * k));
         }
         __syncthreads();
         const uint32_t nl = s_n;
@@ -1083,6 +1112,10 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                     near_j = u | (u << 1) | (u >> 1) | static_cast<uint64_t>(el) | (static_cast<uint64_t>(er) << 63);
                     if (!P.classic && P.ponding) near_j |= T.S.lava[rc - LY];
                 }
+                // LAVA candidates fill the list from the front, AIR ones from the back: each wave of the
+                // candidate loop then runs one rule, not both
+                bool cl[4], ca[4];
+                uint32_t sl[4], sa[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const int row = w + 4 * k;
@@ -1091,15 +1124,17 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                     // positions past the grid's far side hold wrapped halo cells of a periodic partial tile
                     const bool inside = o.x0 + lx < P.nx && o.y0 + row % TY < P.ny && o.z0 + row / TY < P.nz;
                     const uint8_t ty = inside ? T.S.type[(lx + 1) + LX * rc] : kVoidLds;
-                    // LAVA candidates fill the list from the front, AIR ones from the back: each wave of the
-                    // candidate loop then runs one rule, not both
-                    const bool cl = ty == kLava;
-                    const bool ca = ty == kAir && ((near >> lx) & 1u);
-                    if (ty == kAir && !ca) tile_air = 1;
-                    const uint32_t sl = wave_append(cl, &s_nlist);
-                    const uint32_t sa = wave_append(ca, &s_nair);
-                    if (cl) T.list[sl] = static_cast<uint16_t>(lx + 64 * row);
-                    if (ca) T.list[kTileCells - 1 - sa] = static_cast<uint16_t>(lx + 64 * row);
+                    cl[k] = ty == kLava;
+                    ca[k] = ty == kAir && ((near >> lx) & 1u);
+                    if (ty == kAir && !ca[k]) tile_air = 1;
+                }
+                wave_append_k(cl, &s_nlist, sl);
+                wave_append_k(ca, &s_nair, sa);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint16_t cell = static_cast<uint16_t>(lx + 64 * (w + 4 * k));
+                    if (cl[k]) T.list[sl[k]] = cell;
+                    if (ca[k]) T.list[kTileCells - 1 - sa[k]] = cell;
                 }
             }
 #pragma unroll
