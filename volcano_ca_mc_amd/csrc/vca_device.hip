@@ -40,10 +40,31 @@ __device__ inline bool wrap_axis(int& v, int n, int per) {
     return true;
 }
 
+// Wave reductions by DPP row shifts (lane i takes lane i - 1, - 2, - 4, - 8 of its 16-lane row, no LDS round
+// trip) and one v_readlane per row; every lane of the wave must be active.  The result is wave-uniform.
+template <int CTRL>
+__device__ inline uint32_t dpp_row_shr(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ inline void dpp_add64(uint32_t& lo, uint32_t& hi) {
+    const uint32_t l2 = dpp_row_shr<CTRL>(lo), h2 = dpp_row_shr<CTRL>(hi);
+    const uint32_t nl = lo + l2;
+    hi = hi + h2 + (nl < lo ? 1u : 0u);
+    lo = nl;
+}
 __device__ inline unsigned long long wave_sum(unsigned long long v) {
+    uint32_t lo = static_cast<uint32_t>(v), hi = static_cast<uint32_t>(v >> 32);
+    dpp_add64<0x111>(lo, hi);  // row_shr:1
+    dpp_add64<0x112>(lo, hi);  // row_shr:2
+    dpp_add64<0x114>(lo, hi);  // row_shr:4
+    dpp_add64<0x118>(lo, hi);  // row_shr:8
+    unsigned long long t = 0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    for (int r = 15; r < 64; r += 16)
+        t += static_cast<unsigned long long>(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(lo), r))) |
+             (static_cast<unsigned long long>(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hi), r))) << 32);
+    return t;
 }
 
 // threadIdx.x as a value the compiler cannot hoist out of a loop: per-lane index arithmetic derived from it is
@@ -203,9 +224,14 @@ __device__ inline void mark_halo(const CaParams& P, const CaDev& D, int tx, int 
     D.tile_halo[static_cast<size_t>(n) * kHaloBytes + static_cast<uint32_t>(26 - q)] = 1;
 }
 __device__ inline unsigned wave_or(unsigned v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v |= static_cast<unsigned>(__shfl_xor(static_cast<int>(v), o));
-    return v;
+    v |= dpp_row_shr<0x111>(v);
+    v |= dpp_row_shr<0x112>(v);
+    v |= dpp_row_shr<0x114>(v);
+    v |= dpp_row_shr<0x118>(v);
+    return static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 15) |
+                                 __builtin_amdgcn_readlane(static_cast<int>(v), 31) |
+                                 __builtin_amdgcn_readlane(static_cast<int>(v), 47) |
+                                 __builtin_amdgcn_readlane(static_cast<int>(v), 63));
 }
 
 // Cell k (0..3) of thread threadIdx.x inside the tile.
