@@ -56,7 +56,7 @@ struct NbTable {
 constexpr int kTileX = 64;
 constexpr int kTileCells = 1024;
 constexpr int kBlock = 256;
-constexpr int kWideBlocks = 64;  // grid of ca_wide
+constexpr int kWideBlocks = 512;  // grid of ca_wide (blocks past the wide-tile count exit at once)
 
 struct CaParams {
     int32_t nx, ny, nz;
