@@ -283,7 +283,7 @@ struct vca_ca {
     CaDev D{};
     int shape = 0;
     DevBuf arena;  // every region below: one allocation and one free per simulation
-    Span type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, tile_near, tile_halo, changes, tile_nchg,
+    Span type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, tile_near, tile_halo, tile_tchg, changes, tile_nchg,
         tile_nwide, part, ctrl, hist_a, hist_c;
     uint64_t n = 0;
     std::unique_ptr<Batch> own;   // batch of one: the solo path (made on the first solo enqueue)
@@ -729,7 +729,7 @@ namespace {
 struct CaPlan {
     Layout L;
     size_t hcap = 1;
-    size_t type = 0, flow = 0, temp = 0, solid = 0, arrival = 0, ctrl = 0, tile_near = 0, tile_halo = 0, part = 0,
+    size_t type = 0, flow = 0, temp = 0, solid = 0, arrival = 0, ctrl = 0, tile_near = 0, tile_halo = 0, tile_tchg = 0, part = 0,
            hist_a = 0, hist_c = 0;
     size_t zero_bytes = 0;
     size_t descent = 0, vent_cell = 0, tile_flags = 0, changes = 0, tile_nchg = 0, tile_nwide = 0, spill = 0;
@@ -748,6 +748,7 @@ CaPlan ca_plan(uint64_t n, uint32_t ntiles, size_t cols, bool lava, size_t vent_
     c.ctrl = L.add(sizeof(CaCtrl));
     c.tile_near = L.add(ntiles);
     c.tile_halo = L.add(static_cast<size_t>(ntiles) * kHaloBytes);
+    c.tile_tchg = L.add(static_cast<size_t>(ntiles) * 4);
     c.part = L.add(static_cast<size_t>(kPartRows) * sizeof(CaPart));
     c.hist_a = L.add(c.hcap * 8);
     c.hist_c = L.add(c.hcap * 8);
@@ -920,6 +921,7 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         region(ca->ctrl, plan.ctrl, sizeof(CaCtrl));
         region(ca->tile_near, plan.tile_near, P.ntiles);
         region(ca->tile_halo, plan.tile_halo, static_cast<size_t>(P.ntiles) * kHaloBytes);
+        region(ca->tile_tchg, plan.tile_tchg, static_cast<size_t>(P.ntiles) * 4);
         region(ca->part, plan.part, static_cast<size_t>(kPartRows) * sizeof(CaPart));
         region(ca->hist_a, plan.hist_a, plan.hcap * 8);
         region(ca->hist_c, plan.hist_c, plan.hcap * 8);
@@ -947,6 +949,7 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         D.tile_flags = ca->tile_flags.as<uint8_t>();
         D.tile_near = ca->tile_near.as<uint8_t>();
         D.tile_halo = ca->tile_halo.as<uint8_t>();
+        D.tile_tchg = ca->tile_tchg.as<uint32_t>();
         D.changes = ca->changes.as<Change>();
         D.tile_nchg = ca->tile_nchg.as<uint32_t>();
         D.tile_nwide = ca->tile_nwide.as<uint32_t>();

@@ -837,7 +837,27 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
         const bool has_spill = D.spill != nullptr;
         const uint32_t side = P.nb.side_mask;
         const TileCoord o = tile_origin(P, e & kTileMask);
-        __syncthreads();  // the previous tile's LDS reads are done
+        {
+            // D(n) of the tile's lava cells reads the types of the tile, its 8 horizontal neighbours and the tiles
+            // under them down to kMaxDrop + 1 cells below the tile's lowest layer (support, side cells, their drop
+            // chains; spill levels do not change).  When none of those tiles changed a cell's type in the previous
+            // step, the values written then still hold.  (The barrier also orders the previous tile's LDS reads.)
+            constexpr int NB = (kMaxDrop + 1 + TZ - 1) / TZ;  // tile layers below
+            constexpr int NR = 9 * (NB + 1);
+            const int tx = o.x0 / kTileX, ty = o.y0 / TY, tz = o.z0 / TZ;
+            const uint32_t now = static_cast<uint32_t>(D.ctrl->steps);
+            bool dirty = false;
+            for (int q = static_cast<int>(threadIdx.x); q < NR; q += kBlock) {
+                int qx = tx + q % 3 - 1, qy = ty + (q / 3) % 3 - 1;
+                const int qz = tz - q / 9;
+                if (qz >= 0 && wrap_axis(qx, P.ntx, P.per[0]) && wrap_axis(qy, P.nty, P.per[1]))
+                    dirty = dirty || D.tile_tchg[static_cast<uint32_t>(qx) +
+                                                 static_cast<uint32_t>(P.ntx) *
+                                                     (static_cast<uint32_t>(qy) +
+                                                      static_cast<uint32_t>(P.nty) * static_cast<uint32_t>(qz))] == now;
+            }
+            if (!__syncthreads_or(dirty)) continue;
+        }
         int own_air = 0;
         const uint32_t row_base = halo_row_lanes<TY, TZ>(P, o);
         const int gx = halo_x(P, o.x0 + static_cast<int>(threadIdx.x & 63u));
@@ -1040,6 +1060,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             P.classic ? P.nb.any_mask
                       : ((P.nb.any_mask & ~0x1FFu) | (P.ponding ? (P.nb.any_mask & (1u << 4)) : 0u));
         uint32_t w_lava = 0, w_born = 0;  // wave-uniform counts: lava cells after the step, born cells
+        bool w_tchg = false;              // wave-uniform: a cell changed type
         const TileCoord o = tile_origin(P, tile);
         __syncthreads();  // the previous tile's LDS reads are done
         int own_air = 0;
@@ -1204,7 +1225,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             const int ly = row % TY, lz = row / TY;
             const int x = o.x0 + lx, y = o.y0 + ly, z = o.z0 + lz;
             const int c = (lx + 1) + LX * ((ly + 1) + LY * (lz + 1));
-            bool want = false, wide = false, stay = false, newborn = false;
+            bool want = false, wide = false, stay = false, newborn = false, solid = false;
             Change ch;
             uint32_t i = 0;
             if (live) {
@@ -1226,6 +1247,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                             tc = tc - P.cool_amb;
                         if (tc < P.solidus) {
                             want = true;
+                            solid = true;
                             ch = Change{i, f, tc, stamp, kBasalt, kKindSolidify};
                         } else {
                             stay = true;
@@ -1297,6 +1319,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             }
             w_lava += static_cast<uint32_t>(__popcll(__ballot(stay || newborn)));
             w_born += static_cast<uint32_t>(__popcll(__ballot(newborn)));
+            w_tchg = w_tchg || __ballot(newborn || solid) != 0;
             if (stay || newborn || wide) bnd |= boundary_dirs(lx, ly, lz, xm, ym, zm);
             Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
             const uint32_t slot = wave_append(want, &s_nchg);
@@ -1311,8 +1334,9 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         // tile flags, record counts and the tile's counters (row tile % kPartRows of its simulation); every
         // record is one changed cell
         {
+            // bit 2: a cell of the tile changed type (born or solidified) -- tile_tchg, for the next step's D(n)
             const uint32_t f = (__ballot(tile_lava) ? static_cast<uint32_t>(kFlagLava) : 0u) |
-                               (__ballot(tile_air) ? static_cast<uint32_t>(kFlagAir) : 0u);
+                               (__ballot(tile_air) ? static_cast<uint32_t>(kFlagAir) : 0u) | (w_tchg ? 4u : 0u);
             bnd = wave_or(bnd);
             if ((tid & 63u) == 0) {
                 s_red[0][tid >> 6] = w_lava;
@@ -1331,7 +1355,9 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         }
         if (tid == 0) {
             const uint32_t wf = s_wflags[1];
-            D.tile_flags[tile] = static_cast<uint8_t>((wf | wf >> 8 | wf >> 16 | wf >> 24) & 0xFFu);
+            const uint32_t fl = (wf | wf >> 8 | wf >> 16 | wf >> 24) & 0xFFu;
+            D.tile_flags[tile] = static_cast<uint8_t>(fl & (kFlagLava | kFlagAir));
+            if (fl & 4u) D.tile_tchg[tile] = static_cast<uint32_t>(t) + 1u;
             const uint32_t nchg = s_nchg;
             D.tile_nchg[tile] = nchg;
             D.tile_nwide[tile] = s_nwide;
@@ -1394,6 +1420,7 @@ __global__ void __launch_bounds__(kBlock) ca_wide_kernel(BatchDev B) {
             if (!P.classic && wt < P.solidus) continue;
             // slots below the tile's other records: records + born <= candidates <= kTileCells
             rec[atomicAdd(&D.tile_nchg[tile], 1u)] = Change{i, wf, P.classic ? 0.0f : wt, stamp, kLava, kKindBorn};
+            D.tile_tchg[tile] = static_cast<uint32_t>(t) + 1u;
             ++born;
         }
         if (born) {

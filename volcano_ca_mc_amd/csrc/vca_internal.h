@@ -136,6 +136,7 @@ struct CaDev {
     uint8_t* tile_near;           // 1: one of the 26 neighbour tiles held lava when its last step ended (set by
                                   // that tile's rules, read and cleared by ca_front in lava mode; kept in classic)
     uint8_t* tile_halo;           // kHaloBytes per tile: byte q = 1 when the neighbour tile in direction q holds lava
+    uint32_t* tile_tchg;          // per tile: 1 + the last step in which a cell of the tile changed type (0: never)
                                   // in its cells next to this tile (same writers / reader / clearing as tile_near)
     Change* changes;              // kTileCells record slots per tile: tile t owns slots t*kTileCells ..
     uint32_t* tile_nchg;          // records of each visited tile this step (from slot 0 up)
