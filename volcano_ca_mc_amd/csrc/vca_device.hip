@@ -76,6 +76,12 @@ __device__ inline unsigned opaque_tid() {
     asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane));
     return (static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x))) & ~63u) | lane;
 }
+// The lane index alone (no thread index needed, so nothing is kept live for it).
+__device__ inline unsigned opaque_lane() {
+    unsigned lane;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane));
+    return lane;
+}
 // 0 that the compiler cannot hoist out of a loop (a hoisted constant held in a vector register across the loop
 // gets spilled).  One v_mov in an asm block.
 __device__ inline uint32_t opaque_zero() {
@@ -91,24 +97,29 @@ __device__ inline uint64_t readlane64(uint64_t v, int lane) {
     return static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
 }
 
+// Set bits of m below the calling lane (v_mbcnt: no lane mask is built, so none is kept in registers).
+__device__ inline uint32_t mbcnt64(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
 // Wave-aggregated slot reservation: every lane of the wave must call this (pred may differ per lane).  The
 // leader's reserved base reaches the other lanes by v_readlane (no second LDS round trip).
 __device__ inline uint32_t wave_append(bool pred, uint32_t* counter) {
     const unsigned long long mask = __ballot(pred);
-    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const uint32_t below = mbcnt64(mask);
     uint32_t base = 0;
     if (mask != 0) {
         const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-        if (lane == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
+        if (pred && below == 0) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
         base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(base), leader));
     }
-    return base + static_cast<uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL)));
+    return base + below;
 }
 
 // K appends of one wave (slot[k] for pred[k]) with one atomic: all 64 lanes must be active.
 template <int K>
 __device__ inline void wave_append_k(const bool (&pred)[K], uint32_t* counter, uint32_t (&slot)[K]) {
-    const int lane = static_cast<int>(threadIdx.x & 63u);
     unsigned long long m[K];
     uint32_t tot = 0;
 #pragma unroll
@@ -118,13 +129,12 @@ __device__ inline void wave_append_k(const bool (&pred)[K], uint32_t* counter, u
     }
     uint32_t base = 0;
     if (tot != 0) {
-        if (lane == 0) base = atomicAdd(counter, tot);
+        if (opaque_lane() == 0) base = atomicAdd(counter, tot);
         base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(base), 0));
     }
-    const unsigned long long below = (1ULL << lane) - 1ULL;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        slot[k] = base + static_cast<uint32_t>(__popcll(m[k] & below));
+        slot[k] = base + mbcnt64(m[k]);
         base += static_cast<uint32_t>(__popcll(m[k]));
     }
 }
@@ -712,7 +722,7 @@ struct HaloTypes {
 // v_readlane (scalar row index) instead of each wave redoing the row arithmetic in scalar code.
 template <int TY, int TZ>
 __device__ inline uint32_t halo_row_lanes(const CaParams& P, const TileCoord& o) {
-    const int lane = static_cast<int>(opaque_tid() & 63u);
+    const int lane = static_cast<int>(opaque_lane());
     return lane < Halo<TY, TZ>::R ? halo_row<TY, TZ>(P, o, lane) : kNoCell;
 }
 __device__ inline uint32_t row_of(uint32_t row_base, int r) {
@@ -721,11 +731,12 @@ __device__ inline uint32_t row_of(uint32_t row_base, int r) {
 
 template <int TY, int TZ, bool OWN = false>
 __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileCoord& o, HaloTypes<TY, TZ>& S,
-                                  bool with_spill, int32_t* s_spill, int& own_air, uint32_t row_base, int gx) {
+                                  bool with_spill, int32_t* s_spill, int& own_air, uint32_t row_base, int gx,
+                                  unsigned tid) {
     using H = Halo<TY, TZ>;
     constexpr int NS = OWN ? kTileX * TY : H::LXY;     // spill columns staged
     constexpr int KS = (NS + kBlock - 1) / kBlock;     // per thread
-    const int t = static_cast<int>(opaque_tid());
+    const int t = static_cast<int>(tid);
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int lx = t & 63;
     uint32_t v[H::KR];  // 32-bit so that no load result is packed (and waited for) before the stores
@@ -741,7 +752,7 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
     uint32_t e = kVoidLds;
     {
         // thread t < 2R: edge column (t & 1) of row t >> 1 (rows 0..31 in wave 0, 32..35 in wave 1)
-        const uint32_t base = static_cast<uint32_t>(__shfl(static_cast<int>(row_base), (t >> 1) & 63));
+        const uint32_t base = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(((t >> 1) & 63) << 2, static_cast<int>(row_base)));
         if (t < 2 * H::R) {
             const int ex = halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1);
             if (base != kNoCell && ex >= 0) e = D.type[base + static_cast<uint32_t>(ex)];
@@ -861,7 +872,7 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
         int own_air = 0;
         const uint32_t row_base = halo_row_lanes<TY, TZ>(P, o);
         const int gx = halo_x(P, o.x0 + static_cast<int>(threadIdx.x & 63u));
-        stage_rows<TY, TZ>(P, D, o, S, has_spill, s_spill, own_air, row_base, gx);
+        stage_rows<TY, TZ>(P, D, o, S, has_spill, s_spill, own_air, row_base, gx, opaque_tid());
         if (threadIdx.x == 0) s_n = 0;
         __syncthreads();
         {
@@ -1040,8 +1051,11 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
     // the first entry is read together with the list length (blockIdx.x < gridDim.x <= cap: in bounds)
     uint64_t next = tiles[blockIdx.x];
     const uint32_t nact = B.ctrl->nactive[shape];
+    // the wave's first thread, once (a scalar): inside the loop the thread index is rebuilt from it and the lane
+    // (see opaque_tid), so no vector register holds threadIdx.x across the loop
+    const unsigned wbase = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x))) & ~63u;
     for (uint32_t s = blockIdx.x; s < nact; s += gridDim.x) {
-        const unsigned tid = opaque_tid();  // see opaque_tid
+        const unsigned tid = wbase | opaque_lane();
         // uniform: one entry per block; low word slot << kTileBits | tile, high word the simulation's step
         const uint32_t e = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(next));
         const int32_t t = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(next >> 32));
@@ -1068,7 +1082,8 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         // tile, -1 for no cell)
         const uint32_t row_base = halo_row_lanes<TY, TZ>(P, o);
         const int gx = halo_x(P, o.x0 + static_cast<int>(tid & 63u));
-        const bool saw_lava = stage_rows<TY, TZ, true>(P, D, o, T.S, has_spill, T.spill, own_air, row_base, gx);
+        const bool saw_lava =
+            stage_rows<TY, TZ, true>(P, D, o, T.S, has_spill, T.spill, own_air, row_base, gx, tid);
         if (tid == 0) {
             const uint32_t zero = opaque_zero();
             s_nchg = zero;
@@ -1125,7 +1140,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             float etp = 0.0f;
             int32_t eds = -1;
             const bool elava = t < 2 * H::R && T.S.edge[t];
-            const uint32_t ebase = static_cast<uint32_t>(__shfl(static_cast<int>(row_base), (t >> 1) & 63));
+            const uint32_t ebase = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(((t >> 1) & 63) << 2, static_cast<int>(row_base)));
             if (elava) {
                 const uint32_t g = ebase + static_cast<uint32_t>(halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1));
                 efl = D.flow[g];
