@@ -1613,6 +1613,13 @@ __device__ inline void mc_trial(const McParams& P, S* spin, uint32_t g, uint64_t
         wz[d] = z + d - 1;
         oz[d] = wrap_axis(wz[d], P.nz, P.per[2]);
     }
+    // row and plane offsets of the three wrapped y and z values: a neighbour's index is then two adds
+    uint32_t ry[3], pz[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        ry[d] = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(wy[d]);
+        pz[d] = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny) * static_cast<uint32_t>(wz[d]);
+    }
     uint32_t sv[27];
     bool use[27];
 #pragma unroll
@@ -1620,9 +1627,7 @@ __device__ inline void mc_trial(const McParams& P, S* spin, uint32_t g, uint64_t
         if (k == 13) continue;
         const int dx = k % 3, dy = (k / 3) % 3, dz = k / 9;
         use[k] = ((mask >> k) & 1u) && ox[dx] && oy[dy] && oz[dz];
-        const uint32_t q = static_cast<uint32_t>(wx[dx]) +
-                           static_cast<uint32_t>(P.nx) * (static_cast<uint32_t>(wy[dy]) +
-                                                          static_cast<uint32_t>(P.ny) * static_cast<uint32_t>(wz[dz]));
+        const uint32_t q = static_cast<uint32_t>(wx[dx]) + ry[dy] + pz[dz];
         sv[k] = static_cast<uint32_t>(spin[use[k] ? q : g]);
     }
     int in_dom = 0, same_a = 0, same_b = 0;
