@@ -4,7 +4,6 @@
 #include <cmath>
 #include <filesystem>
 #include <fstream>
-#include <queue>
 #include <random>
 
 namespace eng {
@@ -131,16 +130,29 @@ Heightmap load_heightmap_csv(const std::string& path, const Dims& d) {
 // neighbour is flooded to max(own height, level of the column it was reached from).  The spill surface
 // is unique, so the visiting order of equal levels does not matter.
 Heightmap fill_depressions(const Heightmap& hm, bool px, bool py) {
+    // Levels are integers and a column is pushed at a level no lower than the one being drained: a bucket queue
+    // (one LIFO list per level, drained in increasing level), O(n + levels) instead of a heap's O(n log n).
     const int nx = hm.nx, ny = hm.ny;
     const size_t n = hm.height.size();
     Heightmap out = hm;
+    if (n == 0) return out;
+    const auto [mn, mx] = std::minmax_element(hm.height.begin(), hm.height.end());
+    const int lo = *mn;
+    const size_t levels = static_cast<size_t>(static_cast<int64_t>(*mx) - lo) + 1;
     std::vector<uint8_t> closed(n, 0);
-    using Item = std::pair<int, size_t>;
-    std::priority_queue<Item, std::vector<Item>, std::greater<Item>> open;
+    std::vector<int64_t> head(levels, -1);
+    std::vector<int64_t> next(n, -1);
+    size_t cur = levels;  // lowest level holding columns
+    auto push = [&](size_t i, int level) {
+        const size_t l = static_cast<size_t>(level - lo);
+        next[i] = head[l];
+        head[l] = static_cast<int64_t>(i);
+        cur = std::min(cur, l);
+    };
     auto seed = [&](size_t i) {
         if (!closed[i]) {
             closed[i] = 1;
-            open.push({hm.height[i], i});
+            push(i, hm.height[i]);
         }
     };
     for (int y = 0; y < ny; ++y)
@@ -149,31 +161,33 @@ Heightmap fill_depressions(const Heightmap& hm, bool px, bool py) {
             const bool ey = !py && ny > 1 && (y == 0 || y == ny - 1);
             if (ex || ey) seed(static_cast<size_t>(x) + static_cast<size_t>(nx) * y);
         }
-    if (open.empty() && n > 0) {
-        const int lo = *std::min_element(hm.height.begin(), hm.height.end());
+    if (cur == levels) {
         for (size_t i = 0; i < n; ++i)
             if (hm.height[i] == lo) seed(i);
     }
     const int dx[4] = {-1, 1, 0, 0}, dy[4] = {0, 0, -1, 1};
-    while (!open.empty()) {
-        const auto [level, i] = open.top();
-        open.pop();
-        const int x = static_cast<int>(i % static_cast<size_t>(nx)), y = static_cast<int>(i / static_cast<size_t>(nx));
-        for (int k = 0; k < 4; ++k) {
-            int sx = x + dx[k], sy = y + dy[k];
-            if (sx < 0 || sx >= nx) {
-                if (!px || nx == 1) continue;
-                sx = (sx + nx) % nx;
+    for (; cur < levels; ++cur) {
+        while (head[cur] >= 0) {
+            const size_t i = static_cast<size_t>(head[cur]);
+            head[cur] = next[i];
+            const int level = lo + static_cast<int>(cur);
+            const int x = static_cast<int>(i % static_cast<size_t>(nx)), y = static_cast<int>(i / static_cast<size_t>(nx));
+            for (int k = 0; k < 4; ++k) {
+                int sx = x + dx[k], sy = y + dy[k];
+                if (sx < 0 || sx >= nx) {
+                    if (!px || nx == 1) continue;
+                    sx = (sx + nx) % nx;
+                }
+                if (sy < 0 || sy >= ny) {
+                    if (!py || ny == 1) continue;
+                    sy = (sy + ny) % ny;
+                }
+                const size_t j = static_cast<size_t>(sx) + static_cast<size_t>(nx) * sy;
+                if (closed[j]) continue;
+                closed[j] = 1;
+                out.height[j] = std::max(hm.height[j], level);
+                push(j, out.height[j]);  // at a level >= cur: drained in this pass or a later one
             }
-            if (sy < 0 || sy >= ny) {
-                if (!py || ny == 1) continue;
-                sy = (sy + ny) % ny;
-            }
-            const size_t j = static_cast<size_t>(sx) + static_cast<size_t>(nx) * sy;
-            if (closed[j]) continue;
-            closed[j] = 1;
-            out.height[j] = std::max(hm.height[j], level);
-            open.push({out.height[j], j});
         }
     }
     return out;
