@@ -50,21 +50,18 @@ double smooth_noise(uint64_t seed, int x, int y) {
 }
 double cone_height(const Config& c, double r) { return c.terrain.peak * std::max(0.0, 1.0 - r / c.terrain.base); }
 
-std::vector<std::string> split_fields(const std::string& line) {
-    std::vector<std::string> out;
-    std::string cur;
-    for (char ch : line) {
+// Fields of a CSV line: runs of characters between separators (',' ';' space tab CR), as [begin, end) offsets.
+void split_fields(const std::string& line, std::vector<std::pair<size_t, size_t>>& out) {
+    out.clear();
+    size_t b = 0;
+    const size_t n = line.size();
+    for (size_t i = 0; i <= n; ++i) {
+        const char ch = i < n ? line[i] : ',';
         if (ch == ',' || ch == ';' || ch == ' ' || ch == '\t' || ch == '\r') {
-            if (!cur.empty()) {
-                out.push_back(cur);
-                cur.clear();
-            }
-        } else {
-            cur += ch;
+            if (i > b) out.emplace_back(b, i);
+            b = i + 1;
         }
     }
-    if (!cur.empty()) out.push_back(cur);
-    return out;
 }
 
 }  // namespace
@@ -101,21 +98,26 @@ Heightmap load_heightmap_csv(const std::string& path, const Dims& d) {
     hm.ny = d.ny;
     hm.height.reserve(static_cast<size_t>(d.nx) * d.ny);
     std::string line;
+    std::vector<std::pair<size_t, size_t>> f;
     int line_no = 0, row = 0;
     while (std::getline(in, line)) {
         ++line_no;
-        const std::vector<std::string> f = split_fields(line);
-        if (f.empty() || f[0][0] == '#') continue;
+        split_fields(line, f);
+        if (f.empty() || line[f[0].first] == '#') continue;
         if (row >= d.ny) throw ConfigError("Mapa wysokości " + path + " ma więcej wierszy danych niż ny = " + std::to_string(d.ny));
         if (static_cast<int>(f.size()) != d.nx)
             throw ConfigError("Mapa wysokości " + path + ", wiersz " + std::to_string(line_no) + ": " +
                               std::to_string(f.size()) + " wartości, oczekiwano nx = " + std::to_string(d.nx));
         for (size_t col = 0; col < f.size(); ++col) {
+            // parsed in place: a separator ends any number, so the field must be consumed up to its end exactly
+            const char* b = line.c_str() + f[col].first;
+            const char* e = line.c_str() + f[col].second;
             char* end = nullptr;
-            const double h = std::strtod(f[col].c_str(), &end);
-            if (end == f[col].c_str() || *end != '\0' || !std::isfinite(h) || h < 0.0)
+            const double h = std::strtod(b, &end);
+            if (end != e || !std::isfinite(h) || h < 0.0)
                 throw ConfigError("Mapa wysokości " + path + ", wiersz " + std::to_string(line_no) + ", kolumna " +
-                                  std::to_string(col + 1) + ": oczekiwano nieujemnej liczby, jest \"" + f[col] + "\"");
+                                  std::to_string(col + 1) + ": oczekiwano nieujemnej liczby, jest \"" +
+                                  std::string(b, e) + "\"");
             hm.height.push_back(std::min(static_cast<int>(std::ceil(h)), d.nz));
         }
         ++row;
