@@ -506,6 +506,10 @@ __global__ void ca_init_stop_kernel(CaParams P, CaDev D) {
 // Tile flags and the initial LAVA / AIR counts (countInitialState, LavaCA.hpp:1252-1280).
 __global__ void __launch_bounds__(kBlock) ca_init_flags_kernel(CaParams P, CaDev D) {
     __shared__ uint32_t s_bnd[kBlock / 64];
+    __shared__ unsigned long long s_tot[2][kBlock / 64];
+    // the block's lava and AIR counts, added to the totals once at the end (one atomic per block and counter:
+    // per-wave atomics on the same two addresses serialised, 2.5 ms for a 512 x 512 x 256 grid)
+    unsigned long long tl = 0, ta = 0;
     for (uint32_t t = blockIdx.x; t < P.ntiles; t += gridDim.x) {
         const TileCoord o = tile_origin(P, t);
         const int xm = min(kTileX - 1, P.nx - 1 - o.x0), ym = min(P.ty - 1, P.ny - 1 - o.y0),
@@ -527,12 +531,8 @@ __global__ void __launch_bounds__(kBlock) ca_init_flags_kernel(CaParams P, CaDev
         }
         bnd = wave_or(bnd);
         if ((threadIdx.x & 63u) == 0) s_bnd[threadIdx.x >> 6] = bnd;
-        const unsigned long long wl = wave_sum(static_cast<unsigned long long>(lava));
-        const unsigned long long wa = wave_sum(static_cast<unsigned long long>(air));
-        if ((threadIdx.x & 63u) == 0) {
-            if (wl) atomicAdd(&D.ctrl->lava_total, wl);
-            if (wa) atomicAdd(&D.ctrl->air_total, wa);
-        }
+        tl += wave_sum(static_cast<unsigned long long>(lava));
+        ta += wave_sum(static_cast<unsigned long long>(air));
         const int any_lava = __syncthreads_or(lava);
         const int any_air = __syncthreads_or(air);
         if (threadIdx.x == 0) D.tile_flags[t] = static_cast<uint8_t>((any_lava ? kFlagLava : 0) | (any_air ? kFlagAir : 0));
@@ -544,6 +544,20 @@ __global__ void __launch_bounds__(kBlock) ca_init_flags_kernel(CaParams P, CaDev
             mark_halo(P, D, tx, ty, tz, static_cast<int>(threadIdx.x), all);
         }
         __syncthreads();  // s_bnd is rewritten by the next tile
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        s_tot[0][threadIdx.x >> 6] = tl;
+        s_tot[1][threadIdx.x >> 6] = ta;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long l = 0, a = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            l += s_tot[0][w];
+            a += s_tot[1][w];
+        }
+        if (l) atomicAdd(&D.ctrl->lava_total, l);
+        if (a) atomicAdd(&D.ctrl->air_total, a);
     }
 }
 
@@ -1890,7 +1904,7 @@ void ca_launch_init_stop(const CaParams& P, const CaDev& D, hipStream_t s) {
 }
 
 void ca_launch_init_flags(const CaParams& P, const CaDev& D, hipStream_t s) {
-    const unsigned g = P.ntiles < 16384 ? P.ntiles : 16384;
+    const unsigned g = P.ntiles < 2048 ? P.ntiles : 2048;
     hipLaunchKernelGGL(ca_init_flags_kernel, dim3(g ? g : 1), dim3(kBlock), 0, s, P, D);
 }
 
