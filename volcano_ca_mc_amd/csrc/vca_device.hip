@@ -1804,16 +1804,24 @@ __global__ void __launch_bounds__(kBlock) mc_grain_union_kernel(McParams P, cons
     cell_xyz(P, g, x, y, z);
     const S s = spin[g];
     const uint32_t mask = P.nb.conn_mask & ~((1u << 14) - 1u);
+    // the 13 forward neighbours' spins read together (no branch around a load: a position outside the mask or
+    // past an absorbing edge reads the cell itself and is not used), then the unions
+    uint32_t nq[13], sv[13];
+    bool use[13];
 #pragma unroll
-    for (int k = 14; k < 27; ++k) {
-        if (!((mask >> k) & 1u)) continue;
+    for (int j2 = 0; j2 < 13; ++j2) {
+        const int k = 14 + j2;
         int qx = x + (k % 3 - 1), qy = y + ((k / 3) % 3 - 1), qz = z + (k / 9 - 1);
-        if (!wrap_axis(qx, P.nx, P.per[0]) || !wrap_axis(qy, P.ny, P.per[1]) || !wrap_axis(qz, P.nz, P.per[2]))
-            continue;
-        const uint32_t q = static_cast<uint32_t>(qx) +
-                           static_cast<uint32_t>(P.nx) * (static_cast<uint32_t>(qy) + static_cast<uint32_t>(P.ny) * qz);
-        if (spin[q] == s) uf_unite(parent, g, q);
+        use[j2] = ((mask >> k) & 1u) && wrap_axis(qx, P.nx, P.per[0]) && wrap_axis(qy, P.ny, P.per[1]) &&
+                  wrap_axis(qz, P.nz, P.per[2]);
+        nq[j2] = use[j2] ? static_cast<uint32_t>(qx) + static_cast<uint32_t>(P.nx) *
+                                                         (static_cast<uint32_t>(qy) + static_cast<uint32_t>(P.ny) * qz)
+                         : g;
+        sv[j2] = static_cast<uint32_t>(spin[nq[j2]]);
     }
+#pragma unroll
+    for (int j2 = 0; j2 < 13; ++j2)
+        if (use[j2] && static_cast<S>(sv[j2]) == s) uf_unite(parent, g, nq[j2]);
 }
 
 __global__ void __launch_bounds__(kBlock) mc_grain_flatten_kernel(const uint32_t* list, uint32_t count,
