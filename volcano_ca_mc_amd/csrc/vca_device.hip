@@ -1582,14 +1582,6 @@ __global__ void __launch_bounds__(kBlock) ca_owner_kernel(uint32_t* owner, uint3
 
 namespace {
 
-template <typename S>
-__device__ inline int32_t nb_spin(const McParams& P, const S* spin, int x, int y, int z, int k) {
-    int qx = x + (k % 3 - 1), qy = y + ((k / 3) % 3 - 1), qz = z + (k / 9 - 1);
-    if (!wrap_axis(qx, P.nx, P.per[0]) || !wrap_axis(qy, P.ny, P.per[1]) || !wrap_axis(qz, P.nz, P.per[2])) return 0;
-    return static_cast<int32_t>(
-        spin[static_cast<uint32_t>(qx) + static_cast<uint32_t>(P.nx) *
-                                             (static_cast<uint32_t>(qy) + static_cast<uint32_t>(P.ny) * qz)]);
-}
 
 __device__ inline void cell_xyz(const McParams& P, uint32_t g, int& x, int& y, int& z) {
     x = static_cast<int>(g % static_cast<uint32_t>(P.nx));
@@ -1598,24 +1590,12 @@ __device__ inline void cell_xyz(const McParams& P, uint32_t g, int& x, int& y, i
     z = static_cast<int>(r / static_cast<uint32_t>(P.ny));
 }
 
-// One Metropolis trial (PottsMC.cpp:403-443): h = hashIndex(prefix, g) already computed, mob_thr is the
-// exact 53-bit threshold ceil(m * 2^53) of the cell (skip iff (r >> 11) >= thr), accept is the table
-// T[before][after] (kDownhill: dE <= 0, else accept iff (r >> 11) < T).
+// Spins of the 26 neighbours of cell g = (x, y, z) in one round trip: every read is issued before the first is
+// used (no branch around a load: a position outside the mask or past an absorbing edge reads the cell itself
+// and has use[k] false), each into a 32-bit register so that no result is packed -- and waited for -- early.
 template <typename S>
-__device__ inline void mc_trial(const McParams& P, S* spin, uint32_t g, uint64_t h, int var, const uint64_t* accept,
-                                unsigned long long& att, unsigned long long& acc, unsigned long long& up) {
-    ++att;
-    int x, y, z;
-    cell_xyz(P, g, x, y, z);
-    const int32_t cur = static_cast<int32_t>(spin[g]);
-    const uint64_t ho = splitmix64(h ^ kSaltMcOrient);
-    const uint32_t qm = static_cast<uint32_t>(P.Q - 1);
-    int32_t prop = 1 + static_cast<int32_t>(qm < 65536u ? mod_small(ho, qm) : ho % qm);
-    if (prop >= cur) ++prop;
-    const uint32_t mask = P.nb.mask[var];
-    // every neighbour read issued before the first is used (no branch around a load: a position outside the mask
-    // or past an absorbing edge reads the cell itself and is not counted), each into a 32-bit register so that no
-    // result is packed -- and waited for -- early
+__device__ inline void nb_spins(const McParams& P, const S* spin, uint32_t g, int x, int y, int z, uint32_t mask,
+                                uint32_t (&sv)[27], bool (&use)[27]) {
     int wx[3], wy[3], wz[3];
     bool ox[3], oy[3], oz[3];
 #pragma unroll
@@ -1634,16 +1614,38 @@ __device__ inline void mc_trial(const McParams& P, S* spin, uint32_t g, uint64_t
         ry[d] = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(wy[d]);
         pz[d] = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny) * static_cast<uint32_t>(wz[d]);
     }
-    uint32_t sv[27];
-    bool use[27];
 #pragma unroll
     for (int k = 0; k < 27; ++k) {
-        if (k == 13) continue;
+        if (k == 13) {
+            use[k] = false;
+            sv[k] = 0;
+            continue;
+        }
         const int dx = k % 3, dy = (k / 3) % 3, dz = k / 9;
         use[k] = ((mask >> k) & 1u) && ox[dx] && oy[dy] && oz[dz];
         const uint32_t q = static_cast<uint32_t>(wx[dx]) + ry[dy] + pz[dz];
         sv[k] = static_cast<uint32_t>(spin[use[k] ? q : g]);
     }
+}
+
+// One Metropolis trial (PottsMC.cpp:403-443): h = hashIndex(prefix, g) already computed, mob_thr is the
+// exact 53-bit threshold ceil(m * 2^53) of the cell (skip iff (r >> 11) >= thr), accept is the table
+// T[before][after] (kDownhill: dE <= 0, else accept iff (r >> 11) < T).
+template <typename S>
+__device__ inline void mc_trial(const McParams& P, S* spin, uint32_t g, uint64_t h, int var, const uint64_t* accept,
+                                unsigned long long& att, unsigned long long& acc, unsigned long long& up) {
+    ++att;
+    int x, y, z;
+    cell_xyz(P, g, x, y, z);
+    const int32_t cur = static_cast<int32_t>(spin[g]);
+    const uint64_t ho = splitmix64(h ^ kSaltMcOrient);
+    const uint32_t qm = static_cast<uint32_t>(P.Q - 1);
+    int32_t prop = 1 + static_cast<int32_t>(qm < 65536u ? mod_small(ho, qm) : ho % qm);
+    if (prop >= cur) ++prop;
+    const uint32_t mask = P.nb.mask[var];
+    uint32_t sv[27];
+    bool use[27];
+    nb_spins(P, spin, g, x, y, z, mask, sv, use);
     int in_dom = 0, same_a = 0, same_b = 0;
 #pragma unroll
     for (int k = 0; k < 27; ++k) {
@@ -1739,11 +1741,13 @@ __global__ void __launch_bounds__(kBlock) mc_energy_kernel(McParams P, const S* 
             mask = nb_mask(P.nb, mod_small(splitmix64(splitmix64(prefix ^ g) ^ kSaltNeighborhood),
                                            static_cast<uint32_t>(P.nb.variants)));
         }
+        uint32_t sv[27];
+        bool use[27];
+        nb_spins(P, spin, g, x, y, z, mask, sv, use);
         int in_dom = 0, same = 0;
 #pragma unroll
         for (int k = 0; k < 27; ++k) {
-            if (k == 13 || !((mask >> k) & 1u)) continue;
-            const int32_t sj = nb_spin(P, spin, x, y, z, k);
+            const int32_t sj = use[k] ? static_cast<int32_t>(static_cast<S>(sv[k])) : 0;
             in_dom += sj != 0;
             same += sj == s;
         }
