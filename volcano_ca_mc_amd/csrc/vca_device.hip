@@ -1051,7 +1051,11 @@ __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY
 // 80 registers; the others five at most, 96 registers) -- more tiles in flight hide the staging latency better
 // than the few registers the limit costs.
 template <int TY, int TZ, class F>
-constexpr int kRulesWaves = (TY == 4 && TZ == 4 && sizeof(F) == 2) ? 6 : 5;
+constexpr int kRulesTilesPerCu = static_cast<int>(163840 / (sizeof(RulesTile<TY, TZ, F>) + 512));  // 160 KB of LDS
+template <int TY, int TZ, class F>
+constexpr int kRulesWaves = (TY == 4 && TZ == 4 && sizeof(F) == 2)
+                                ? 6
+                                : (kRulesTilesPerCu<TY, TZ, F> < 5 ? kRulesTilesPerCu<TY, TZ, F> : 5);
 
 template <int TY, int TZ, class F>
 __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_kernel(BatchDev B, int shape) {
@@ -1945,14 +1949,14 @@ void ca_launch_init_flags(const CaParams& P, const CaDev& D, hipStream_t s) {
 void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, bool flow16, int grid_tiles,
                     hipStream_t s, hipEvent_t* ev) {
     const unsigned gt = static_cast<unsigned>(grid_tiles);
-    if (ev) hipEventRecord(ev[0], s);
+    if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL(ca_front_kernel, dim3(grid_for((B.ntiles + kFrontPer - 1) / kFrontPer)), dim3(kBlock), 0, s,
                        B, shapes);
-    if (ev) hipEventRecord(ev[1], s);
+    if (ev) (void)hipEventRecord(ev[1], s);
     if (any_lava_mode)
         for (int k = 0; k < kShapes; ++k)
             if ((shapes >> k) & 1u) VCA_SHAPE_DISPATCH(k, ca_descent_kernel, dim3(gt), dim3(kBlock), 0, s, B);
-    if (ev) hipEventRecord(ev[2], s);
+    if (ev) (void)hipEventRecord(ev[2], s);
     for (int k = 0; k < kShapes; ++k)
         if ((shapes >> k) & 1u) {
             if (flow16)
@@ -1960,13 +1964,13 @@ void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, bool
             else
                 VCA_RULES_DISPATCH(k, int32_t, dim3(gt), dim3(kBlock), 0, s, B);
         }
-    if (ev) hipEventRecord(ev[3], s);
+    if (ev) (void)hipEventRecord(ev[3], s);
     hipLaunchKernelGGL(ca_wide_kernel, dim3(kWideBlocks), dim3(kBlock), 0, s, B);
-    if (ev) hipEventRecord(ev[4], s);
+    if (ev) (void)hipEventRecord(ev[4], s);
     hipLaunchKernelGGL(ca_apply_kernel, dim3((gt + 3) / 4), dim3(kBlock), 0, s, B);
-    if (ev) hipEventRecord(ev[5], s);
+    if (ev) (void)hipEventRecord(ev[5], s);
     hipLaunchKernelGGL(ca_finish_kernel, dim3(B.nslots), dim3(kBlock), 0, s, B);
-    if (ev) hipEventRecord(ev[6], s);
+    if (ev) (void)hipEventRecord(ev[6], s);
 }
 
 void ca_launch_status(const BatchDev& B, int32_t* out, hipStream_t s) {
