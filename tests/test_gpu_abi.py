@@ -258,3 +258,22 @@ def test_flow_stats_on_device_match_host_sums(ctx, name):
         expect = np.zeros(cap, np.uint8)
         expect[far] = 1
         assert np.array_equal(seen, expect), (name, f)
+
+
+@pytest.mark.parametrize("name", _lava_cases()[:4])
+def test_flow_stats_with_many_flows(ctx, name):
+    """More than 256 flowIds (sums gathered per cell, not per block): the real vents plus vents of no cells."""
+    d, cfg = _inputs(name)
+    print(f"case {name}: dims {cfg['space']}", flush=True)
+    hm = O.terrain_for(cfg, d)
+    per = cfg["boundary"] == "periodic"
+    spill = O.fill_depressions(hm, per, per)
+    vents = O.vents_for(cfg, hm)
+    gpu_ca = _gpu_ca(ctx, cfg, hm, spill, vents)
+    gpu_ca.run()
+    st = gpu_ca.result()
+    vl = vents.tolist() + [[k % 7, 0, 0, len(vents) + 1 + k] for k in range(300)]
+    la, ba, d2 = gpu_ca.flow_stats(vl)
+    hla, hba, hd2, _, _, _ = _flow_sums_numpy(st, gpu_ca.dims, vl)
+    assert np.array_equal(la, hla) and np.array_equal(ba, hba) and np.array_equal(d2, hd2), name
+    assert la[1:len(vents) + 1].sum() + ba[1:len(vents) + 1].sum() > 0, name

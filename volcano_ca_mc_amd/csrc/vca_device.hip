@@ -2062,58 +2062,95 @@ void mc_launch_labels(const McParams& P, const void* spin, int bytes, const uint
 // After the CA: flow sums of the final state and the MC domain lists, on the device
 // ------------------------------------------------------------------------------------------------
 
+// Calls fn(i, t) for every LAVA / BASALT cell i (type t) of the n cells of `type` (16-byte aligned): each lane
+// reads 16 cells with one load and skips a word of four without a domain byte in a few ALU operations, so a
+// pass over the grid keeps many loads in flight instead of waiting on one byte per lane.
+template <typename Fn>
+__device__ inline void for_domain_cells(uint64_t n, const uint8_t* type, Fn&& fn) {
+    const uint64_t chunks = (n + 15) / 16;
+    for (uint64_t q = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; q < chunks;
+         q += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        const uint64_t i0 = q * 16;
+        uint32_t w[4];
+        if (i0 + 16 <= n) {
+            const uint4 v = reinterpret_cast<const uint4*>(type)[q];
+            w[0] = v.x;
+            w[1] = v.y;
+            w[2] = v.z;
+            w[3] = v.w;
+        } else {  // the last, partial chunk: missing cells read as AIR
+            uint64_t lo = 0, hi = 0;
+            for (uint64_t i = i0; i < n; ++i) {
+                const uint32_t sft = static_cast<uint32_t>(i - i0);
+                const uint64_t t = type[i];
+                if (sft < 8)
+                    lo |= t << (8 * sft);
+                else
+                    hi |= t << (8 * (sft - 8));
+            }
+            w[0] = static_cast<uint32_t>(lo);
+            w[1] = static_cast<uint32_t>(lo >> 32);
+            w[2] = static_cast<uint32_t>(hi);
+            w[3] = static_cast<uint32_t>(hi >> 32);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            // a byte of y is zero iff that cell is LAVA (2) or BASALT (3); the zero-byte test is exact
+            const uint32_t y = (w[k] ^ 0x02020202u) & 0xFEFEFEFEu;
+            if (((y - 0x01010101u) & ~y & 0x80808080u) == 0) continue;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t t = (w[k] >> (8 * b)) & 0xFFu;
+                if ((t & 0xFEu) == 2u) fn(i0 + 4 * k + b, t);
+            }
+        }
+    }
+}
+
 // flowSums (Simulation.cpp:408-466): per flowId f in 1..nflows the LAVA and BASALT cells and the largest squared
-// horizontal distance to the flow's vent.  One block per grid row (y, z); a wave folds its lanes flow by flow
-// (lanes of one wave nearly always share one flow), so each (wave, flow) pair costs one atomic per counter.
+// horizontal distance to the flow's vent.  With at most kFlowLds flows a block gathers its sums in LDS and adds
+// them to the totals once (per-wave atomics on the few total addresses serialised: 2.1 ms on a 512x512x256 grid);
+// with more flows each cell goes to the totals directly, spread over many addresses.
+constexpr int kFlowLds = 256;
+
 __global__ void __launch_bounds__(kBlock) ca_flow_sums_kernel(CaParams P, CaDev D, const int32_t* vent_xy,
                                                                int32_t nflows, unsigned long long* lava,
                                                                unsigned long long* basalt,
                                                                unsigned long long* maxd2) {
-    const uint32_t rows = static_cast<uint32_t>(P.ny) * static_cast<uint32_t>(P.nz);
-    const int lane = static_cast<int>(threadIdx.x & 63u);
-    for (uint32_t r = blockIdx.x; r < rows; r += gridDim.x) {
-        const int y = static_cast<int>(r % static_cast<uint32_t>(P.ny));
-        const uint32_t row = r * static_cast<uint32_t>(P.nx);
-        for (int x0 = 0; x0 < P.nx; x0 += kBlock) {
-            const int x = x0 + static_cast<int>(threadIdx.x);
-            bool valid = false, is_lava = false;
-            int32_t f = 0;
-            unsigned long long d2 = 0;
-            if (x < P.nx) {
-                const uint8_t t = D.type[row + static_cast<uint32_t>(x)];
-                if (t == kLava || t == kBasalt) {
-                    f = D.flow[row + static_cast<uint32_t>(x)];
-                    valid = f >= 1 && f <= nflows;
-                    is_lava = t == kLava;
-                    if (valid) {
-                        const long long dx = x - vent_xy[2 * (f - 1)], dy = y - vent_xy[2 * (f - 1) + 1];
-                        d2 = static_cast<unsigned long long>(dx * dx + dy * dy);
-                    }
-                }
-            }
-            unsigned long long pend = __ballot(valid);
-            while (pend) {
-                const int leader = __ffsll(static_cast<long long>(pend)) - 1;
-                const int32_t fl = __shfl(f, leader);
-                const bool mine = valid && f == fl;
-                const unsigned long long m = __ballot(mine);
-                const unsigned long long ml = __ballot(mine && is_lava);
-                unsigned long long mx = mine ? d2 : 0ULL;
-#pragma unroll
-                for (int o = 32; o > 0; o >>= 1) {
-                    const unsigned long long v = __shfl_xor(mx, o);
-                    mx = v > mx ? v : mx;
-                }
-                if (lane == leader) {
-                    const unsigned long long nl = static_cast<unsigned long long>(__popcll(ml));
-                    const unsigned long long nb = static_cast<unsigned long long>(__popcll(m)) - nl;
-                    if (nl) atomicAdd(&lava[fl], nl);
-                    if (nb) atomicAdd(&basalt[fl], nb);
-                    atomicMax(&maxd2[fl], mx);
-                }
-                pend &= ~m;
-            }
+    __shared__ uint32_t s_cnt[2][kFlowLds + 1];  // LAVA, BASALT
+    __shared__ unsigned long long s_d2[kFlowLds + 1];
+    const bool in_lds = nflows <= kFlowLds;
+    if (in_lds) {
+        for (int f = static_cast<int>(threadIdx.x); f <= nflows; f += kBlock) {
+            s_cnt[0][f] = 0;
+            s_cnt[1][f] = 0;
+            s_d2[f] = 0;
         }
+        __syncthreads();
+    }
+    const uint64_t nx = static_cast<uint64_t>(P.nx), ny = static_cast<uint64_t>(P.ny);
+    for_domain_cells(nx * ny * static_cast<uint64_t>(P.nz), D.type, [&](uint64_t i, uint32_t t) {
+        const int32_t f = D.flow[i];
+        if (f < 1 || f > nflows) return;
+        const uint64_t r = i / nx;
+        const long long dx = static_cast<long long>(i - r * nx) - vent_xy[2 * (f - 1)];
+        const long long dy = static_cast<long long>(r % ny) - vent_xy[2 * (f - 1) + 1];
+        const unsigned long long d2 = static_cast<unsigned long long>(dx * dx + dy * dy);
+        if (in_lds) {
+            atomicAdd(&s_cnt[t == kLava ? 0 : 1][f], 1u);
+            atomicMax(&s_d2[f], d2);
+        } else {
+            atomicAdd(t == kLava ? &lava[f] : &basalt[f], 1ULL);
+            atomicMax(&maxd2[f], d2);
+        }
+    });
+    if (!in_lds) return;
+    __syncthreads();
+    for (int f = 1 + static_cast<int>(threadIdx.x); f <= nflows; f += kBlock) {
+        const uint32_t nl = s_cnt[0][f], nb = s_cnt[1][f];
+        if (nl) atomicAdd(&lava[f], static_cast<unsigned long long>(nl));
+        if (nb) atomicAdd(&basalt[f], static_cast<unsigned long long>(nb));
+        if (nl | nb) atomicMax(&maxd2[f], s_d2[f]);
     }
 }
 
@@ -2143,22 +2180,40 @@ __global__ void __launch_bounds__(kBlock) ca_flow_points_kernel(CaParams P, CaDe
 // the mobility of every cell (0 outside the domain).  Same double arithmetic as the host formula.
 __device__ inline bool mc_domain_cell(uint8_t t) { return t == kBasalt || t == kLava; }
 
-__global__ void __launch_bounds__(kBlock) mc_cool_end_kernel(uint64_t n, const uint8_t* type, const uint16_t* solid,
-                                                              const uint16_t* arrival, uint32_t* red) {
-    uint32_t e = 0;
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; i < n;
-         i += static_cast<uint64_t>(gridDim.x) * kBlock)
-        if (mc_domain_cell(type[i])) {
-            const uint32_t a = solid[i], b = arrival[i];
-            e = a > e ? a : e;
-            e = b > e ? b : e;
-        }
+// Block-wide min and max of one value per thread (every thread of the block calls; the results are valid in
+// thread 0).
+__device__ inline void block_min_max(uint32_t& lo, uint32_t& hi) {
+    __shared__ uint32_t s_lo[kBlock / 64], s_hi[kBlock / 64];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        const uint32_t v = __shfl_xor(e, o);
-        e = v > e ? v : e;
+        const uint32_t a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
     }
-    if ((threadIdx.x & 63u) == 0 && e) atomicMax(&red[0], e);
+    if ((threadIdx.x & 63u) == 0) {
+        s_lo[threadIdx.x >> 6] = lo;
+        s_hi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; ++k) {
+            lo = s_lo[k] < lo ? s_lo[k] : lo;
+            hi = s_hi[k] > hi ? s_hi[k] : hi;
+        }
+    }
+}
+
+// One atomic per block on each of the shared results (per-wave atomics on one address serialised).
+__global__ void __launch_bounds__(kBlock) mc_cool_end_kernel(uint64_t n, const uint8_t* type, const uint16_t* solid,
+                                                              const uint16_t* arrival, uint32_t* red) {
+    uint32_t lo = 0, e = 0;
+    for_domain_cells(n, type, [&](uint64_t i, uint32_t) {
+        const uint32_t a = solid[i], b = arrival[i];
+        e = a > e ? a : e;
+        e = b > e ? b : e;
+    });
+    block_min_max(lo, e);
+    if (threadIdx.x == 0 && e) atomicMax(&red[0], e);
 }
 
 __device__ inline uint32_t mc_cool_time(uint8_t t, uint32_t solid, uint32_t arrival, uint32_t end) {
@@ -2170,21 +2225,13 @@ __global__ void __launch_bounds__(kBlock) mc_cool_range_kernel(uint64_t n, const
                                                                 const uint16_t* arrival, uint32_t* red) {
     const uint32_t end = red[0];
     uint32_t lo = 0xFFFFFFFFu, hi = 0;
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; i < n;
-         i += static_cast<uint64_t>(gridDim.x) * kBlock) {
-        const uint8_t t = type[i];
-        if (!mc_domain_cell(t)) continue;
-        const uint32_t tm = mc_cool_time(t, solid[i], arrival[i], end);
+    for_domain_cells(n, type, [&](uint64_t i, uint32_t t) {
+        const uint32_t tm = mc_cool_time(static_cast<uint8_t>(t), solid[i], arrival[i], end);
         lo = tm < lo ? tm : lo;
         hi = tm > hi ? tm : hi;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint32_t a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
-        lo = a < lo ? a : lo;
-        hi = b > hi ? b : hi;
-    }
-    if ((threadIdx.x & 63u) == 0) {
+    });
+    block_min_max(lo, hi);
+    if (threadIdx.x == 0) {
         if (lo != 0xFFFFFFFFu) atomicMin(&red[1], lo);
         if (hi) atomicMax(&red[2], hi);
     }
@@ -2354,8 +2401,9 @@ __global__ void __launch_bounds__(kBlock) mc_list_scatter_kernel(McListSrc L, co
 void ca_launch_flow_sums(const CaParams& P, const CaDev& D, const int32_t* vent_xy, int32_t nflows,
                          unsigned long long* lava, unsigned long long* basalt, unsigned long long* maxd2,
                          hipStream_t s) {
-    const uint64_t rows = static_cast<uint64_t>(P.ny) * P.nz;
-    hipLaunchKernelGGL(ca_flow_sums_kernel, dim3(blocks_for(rows, 16384)), dim3(kBlock), 0, s, P, D, vent_xy, nflows,
+    const uint64_t n = static_cast<uint64_t>(P.nx) * P.ny * P.nz;
+    hipLaunchKernelGGL(ca_flow_sums_kernel, dim3(grid_for((n + 15) / 16, 2048)), dim3(kBlock), 0, s, P, D, vent_xy,
+                       nflows,
                        lava, basalt, maxd2);
 }
 
@@ -2368,7 +2416,7 @@ void ca_launch_flow_points(const CaParams& P, const CaDev& D, int32_t flow, int3
 
 void mc_launch_cooling(uint64_t n, const uint8_t* type, const uint16_t* solid, const uint16_t* arrival,
                        uint32_t* red, double mobility_min, float* mob, hipStream_t s) {
-    const unsigned g = grid_for(n, 8192);
+    const unsigned g = grid_for((n + 15) / 16, 2048);
     hipLaunchKernelGGL(mc_cool_end_kernel, dim3(g), dim3(kBlock), 0, s, n, type, solid, arrival, red);
     hipLaunchKernelGGL(mc_cool_range_kernel, dim3(g), dim3(kBlock), 0, s, n, type, solid, arrival, red);
     hipLaunchKernelGGL(mc_cool_mobility_kernel, dim3(grid_for(n, 65536)), dim3(kBlock), 0, s, n, type, solid, arrival,
