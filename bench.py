@@ -16,7 +16,10 @@ Printed: one JSON line.  value = CA cell-updates/s = cells x executed steps / ca
 roofline: the dominant kernel's algorithmic bytes per launch (26 B per processed cell-step for the CA
 kernels) / its mean launch time from HIP events recorded on the engine's stream during the timed runs (on every
 10th CA step: events on every launch would add their own cost to caSim);
-traffic = measured HBM bytes per launch of that kernel from the committed rocprofv3 counter summary.
+traffic = measured HBM bytes per launch of that kernel from the committed rocprofv3 counter summary;
+valu_issue_frac / salu_issue_frac = its VALU / SALU wave-instructions per launch (committed rocprofv3 counter
+summary) / the live mean launch time / the issue peak (256 CUs x 2.4 GHz).  mc_roofline: the same for mc_color,
+at 2 * colours + 4 B per spin attempt.
 cpu_baseline: the reference engine (oracle/_ref/engine, built from the reference sources) on the host
 cores, on a bounded sample (the full CA phase, the first 20 sweeps) of the same workload.
 """
@@ -39,6 +42,9 @@ REF_ENGINE = os.path.join(ROOT, "oracle", "_ref", "engine")
 HBM_PEAK_GBS = 8000.0
 TIMING_SAMPLE = 10  # kernel events on every 10th CA step and MC sweep
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
+PROFILE_ISSUE = os.path.join(ROOT, "profiles", "valu.json")
+# wave-instruction issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 VALU instruction
+ISSUE_PEAK_PER_S = 256 * 2.4e9
 
 
 def log(msg: str) -> None:
@@ -256,6 +262,32 @@ def main() -> int:
         if os.path.exists(PROFILE_TRAFFIC):
             with open(PROFILE_TRAFFIC) as f:
                 traffic = json.load(f).get(dom_k)
+        issue = {}
+        if os.path.exists(PROFILE_ISSUE):
+            with open(PROFILE_ISSUE) as f:
+                issue = json.load(f)
+
+        def issue_fracs(k: str, ms: float) -> dict:
+            e = issue.get(k)
+            if not e:
+                return {"valu_issue_frac": None, "salu_issue_frac": None}
+            return {"valu_issue_frac": round(e["valu"] / (ms / 1000.0) / ISSUE_PEAK_PER_S, 4),
+                    "salu_issue_frac": round(e["salu"] / (ms / 1000.0) / ISSUE_PEAK_PER_S, 4)}
+
+        mc_roofline = None
+        if "mc_color" in ktot and "mc_color" in bytes_per_launch:
+            mc_launch_ms = ktot["mc_color"][0] / ktot["mc_color"][1]
+            mc_ach = bytes_per_launch["mc_color"] / (mc_launch_ms / 1000.0) / 1e9
+            mc_traffic = None
+            if os.path.exists(PROFILE_TRAFFIC):
+                with open(PROFILE_TRAFFIC) as f:
+                    mc_traffic = json.load(f).get("mc_color")
+            mc_roofline = {"bound": "latency", "kernel": "mc_color", "kernel_mean_ms": round(mc_launch_ms, 5),
+                           "achieved": round(mc_ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(mc_ach / HBM_PEAK_GBS, 4), "traffic": mc_traffic,
+                           "bytes_per_unit": "2 * colours + 4 B per spin attempt",
+                           "algorithmic_bytes_per_launch": round(bytes_per_launch["mc_color"]),
+                           **issue_fracs("mc_color", mc_launch_ms)}
         w = runs[0].get("detail", {}).get("caWork", {})
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_k,
@@ -266,6 +298,7 @@ def main() -> int:
                     "staged_cells_per_launch": round(w.get("tilesStaged", 0) * w.get("tileCells", 0) /
                                                      max(1, steps)),
                     "minimal_bytes_per_launch": round(minimal_bytes(runs[0])),
+                    **issue_fracs(dom_k, mean_ms),
                     "kernel_share": {k: round(v[0] / sum(x[0] for x in ktot.values()), 3) for k, v in ktot.items()}}
         cpu = None if a.no_cpu else cpu_baseline(cfg_path, cells, tmp)
     result = {
@@ -289,6 +322,7 @@ def main() -> int:
                                f"Potts Q64 kT0.5 200 sweeps (checkerboard)", "cells": cells, "ca_steps": steps,
                    "mc_domain_cells": dom, "mc_sweeps": mcs, "parallelism": "single GPU"},
         "roofline": roofline,
+        "mc_roofline": mc_roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result))
