@@ -743,6 +743,19 @@ __device__ inline uint32_t row_of(uint32_t row_base, int r) {
     return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(row_base), r));
 }
 
+// Bit r set: halo row r (ly = r % LY, lz = r / LY) is one of the tile's own TY x TZ rows.
+template <int TY, int TZ>
+constexpr uint64_t own_row_mask() {
+    using H = Halo<TY, TZ>;
+    static_assert(H::R <= 64, "one mask bit per halo row");
+    uint64_t m = 0;
+    for (int r = 0; r < H::R; ++r) {
+        const int ly = r % H::LY, lz = r / H::LY;
+        if (ly >= 1 && ly <= TY && lz >= 1 && lz <= TZ) m |= 1ull << r;
+    }
+    return m;
+}
+
 template <int TY, int TZ, bool OWN = false>
 __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileCoord& o, HaloTypes<TY, TZ>& S,
                                   bool with_spill, int32_t* s_spill, int& own_air, uint32_t row_base, int gx,
@@ -752,6 +765,7 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
     constexpr int KS = (NS + kBlock - 1) / kBlock;     // per thread
     const int t = static_cast<int>(tid);
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    __builtin_assume(w >= 0 && w < kBlock / 64);
     const int lx = t & 63;
     uint32_t v[H::KR];  // 32-bit so that no load result is packed (and waited for) before the stores
 #pragma unroll
@@ -792,7 +806,10 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
             }
         }
     }
-    bool lava = false;
+    // row tests from one scalar mask (bit 4k: this wave's row k is an own row), lava rows OR-ed as masks
+    const uint64_t wown = own_row_mask<TY, TZ>() >> w;
+    const bool in_x = o.x0 + lx < P.nx;
+    uint64_t any_lava = 0;
 #pragma unroll
     for (int k = 0; k < H::KR; ++k) {
         const int r = w + 4 * k;
@@ -800,11 +817,11 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
             S.type[lx + 1 + H::LX * r] = static_cast<uint8_t>(v[k]);
             const uint64_t m = __ballot(v[k] == kLava);
             if (lx == 0) S.lava[r] = m;
-            lava = lava || m != 0;
-            const int ly = r % H::LY, lz = r / H::LY;
-            if (ly >= 1 && ly <= TY && lz >= 1 && lz <= TZ && o.x0 + lx < P.nx && v[k] == kAir) own_air = 1;
+            any_lava |= m;
+            if (((wown >> (4 * k)) & 1u) && in_x && v[k] == kAir) own_air = 1;
         }
     }
+    bool lava = any_lava != 0;
     if (t < 2 * H::R) {
         S.type[((t & 1) ? H::LX - 1 : 0) + H::LX * (t >> 1)] = static_cast<uint8_t>(e);
         S.edge[t] = e == kLava ? 1 : 0;
