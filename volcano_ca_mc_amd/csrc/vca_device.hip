@@ -290,14 +290,15 @@ struct VoteFirst {
     int n = 0;             // candidates counted (all of flowId id)
     bool overflow = false;
     int32_t id = 0;
-    float mt = 0.0f;
-    // add(f, t) when ok, nothing otherwise
+    float mt = -__builtin_inff();  // read only when n > 0: the first candidate's t replaces it (t is finite)
+    // add(f, t) when ok, nothing otherwise.  The first candidate needs no case of its own: with n == 0 it
+    // matches, takes id = f, and mt = t since -inf < t.
     __device__ void add_if(bool ok, int32_t f, float t) {
-        const bool first = ok && n == 0;
-        const bool same = ok && (first || f == id);
-        overflow = overflow || (ok && !same);
-        id = first ? f : id;
-        mt = first ? t : ((same && mt < t) ? t : mt);  // std::max(a, b) == (a < b ? b : a)
+        const bool match = n == 0 || f == id;
+        const bool same = ok && match;
+        overflow = overflow || (ok && !match);
+        id = same ? f : id;
+        mt = (same && mt < t) ? t : mt;  // std::max(a, b) == (a < b ? b : a)
         n += same ? 1 : 0;
     }
     __device__ void add(int32_t f, float t) { add_if(true, f, t); }
@@ -314,20 +315,19 @@ struct VoteTwo {
     int n = 0, n2 = 0;     // candidates of id / of id2
     bool overflow = false;
     int32_t id = 0, id2 = 0;
-    float mt = 0.0f, mt2 = 0.0f;
+    float mt = -__builtin_inff(), mt2 = -__builtin_inff();  // as in VoteFirst: an empty slot matches any f
     __device__ void add_if(bool ok, int32_t f, float t) {
-        const bool first = ok && n == 0;
-        const bool one = ok && !first && f == id;
-        const bool rest = ok && !first && !one;
-        const bool first2 = rest && n2 == 0;
-        const bool two = rest && !first2 && f == id2;
-        overflow = overflow || (rest && !first2 && !two);
-        id = first ? f : id;
-        mt = first ? t : ((one && mt < t) ? t : mt);  // std::max(a, b) == (a < b ? b : a)
-        n += (first || one) ? 1 : 0;
-        id2 = first2 ? f : id2;
-        mt2 = first2 ? t : ((two && mt2 < t) ? t : mt2);
-        n2 += (first2 || two) ? 1 : 0;
+        const bool m1 = n == 0 || f == id;
+        const bool m2 = n2 == 0 || f == id2;
+        const bool one = ok && m1;
+        const bool two = ok && !m1 && m2;
+        overflow = overflow || (ok && !m1 && !m2);
+        id = one ? f : id;
+        mt = (one && mt < t) ? t : mt;  // std::max(a, b) == (a < b ? b : a)
+        n += one ? 1 : 0;
+        id2 = two ? f : id2;
+        mt2 = (two && mt2 < t) ? t : mt2;
+        n2 += two ? 1 : 0;
     }
     __device__ void add(int32_t f, float t) { add_if(true, f, t); }
     __device__ void winner(uint64_t h_index, int32_t& flow, float& temp) const {
