@@ -1,27 +1,30 @@
 #!/usr/bin/env python3
-"""Benchmark of the GPU engine on the 3D cone workload (one MI355X).
+"""Benchmark of the GPU engine (one MI355X).
 
-Workload: 512 x 512 x 256 voxels, radial cone 220 -> 20 at r = 250 plus 4-octave
-value noise (amplitude 12), three craters within r = 30 of the summit, Moore neighbourhood, absorbing
-boundaries, up to 1000 lava-CA steps, then 200 checkerboard Potts sweeps (Q 64, kT 0.5) on the basalt.
-Synthetic, seeded terrain (volcano_ca_mc_amd/workloads.py).
+Default workload: the 3D caldera, the largest single-GPU configuration -- 1024 x 1024 x 384 voxels, a double
+cone with an inner ring depression (depth 60, radius 150) plus 6-octave value noise (amplitude 20), eight
+craters on the ring, random pentagonal neighbourhood drawn by the hash per cell, periodic x/y and absorbing z,
+up to 2000 lava-CA steps, then 100 checkerboard Potts sweeps (Q 128, kT 0.5) on the basalt.  Synthetic,
+seeded terrain (volcano_ca_mc_amd/workloads.py).
 
-One bench "step" = one complete job of that workload through the drop-in `engine` binary (C++ host +
-HIP kernels): the CA runs to its stop rule, then the MC sweeps.  The timed regions are the engine's
-caSim (all CA steps, state resident in HBM) and mcSim (all sweeps); terrain, allocation, MC set-up,
-statistics and file output are outside them (output.format = none).
+One bench "step" = one complete job of that workload through the drop-in `engine` binary (C++ host + HIP
+kernels): the CA runs to its stop rule, then the MC sweeps.  The timed regions are the engine's caSim (all CA
+steps, state resident in HBM) and mcSim (all sweeps); terrain, allocation, MC set-up, statistics and file
+output are outside them (output.format = none).
 
-Printed: one JSON line.  value = CA cell-updates/s = cells x executed steps / caSim (the reference's
-"ns per cell and step" unit); mc_value = MC spin-attempts/s = domain cells x sweeps / mcSim.
-roofline: the dominant kernel's algorithmic bytes per launch (26 B per processed cell-step for the CA
-kernels) / its mean launch time from HIP events recorded on the engine's stream during the timed runs (on every
-10th CA step: events on every launch would add their own cost to caSim);
-traffic = measured HBM bytes per launch of that kernel from the committed rocprofv3 counter summary;
-valu_issue_frac / salu_issue_frac = its VALU / SALU wave-instructions per launch (committed rocprofv3 counter
-summary) / the live mean launch time / the issue peak (256 CUs x 2.4 GHz).  mc_roofline: the same for mc_color,
-at 2 * colours + 4 B per spin attempt.
-cpu_baseline: the reference engine (oracle/_ref/engine, built from the reference sources) on the host
-cores, on a bounded sample (the full CA phase, the first 20 sweeps) of the same workload.
+Printed: one JSON line.
+  value      CA cell-updates/s = cells x executed steps / caSim (the reference's "ns per cell and step" unit)
+  mc_value   MC spin-attempts/s = domain cells x sweeps / mcSim
+  roofline   the dominant CA kernel (ca_rules): 26 B per cell it stages (13 B of state read + 13 B written) x
+             the cells of the tiles it stages per launch, over its mean launch time from HIP events recorded by
+             the kernel's own dispatch (every 10th step); traffic = HBM bytes per launch measured by rocprofv3
+             counters (profiles/traffic.json).  step_roofline: 26 B x the cells of every visited tile over the
+             whole step time (caSim / steps).
+  mc_roofline the colour kernel (mc_color): 2 * colours + 4 B per spin attempt; a launch handles one colour.
+  cpu_baseline the reference engine (oracle/_ref/engine, built from the reference sources) on every host core
+             the process may use, on a bounded sample of the same workload.
+  extra      the cone workload line, the 256-job batched mix (against the same jobs one at a time) and the
+             serving stream (Poisson arrivals; and the loop's capacity with every request offered at once).
 """
 from __future__ import annotations
 
@@ -45,6 +48,9 @@ PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
 PROFILE_ISSUE = os.path.join(ROOT, "profiles", "valu.json")
 # wave-instruction issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 VALU instruction
 ISSUE_PEAK_PER_S = 256 * 2.4e9
+CA_BYTES_PER_CELL = 26
+# CPU sample of each workload: the first CA steps and Potts sweeps the reference engine runs
+CPU_SAMPLE = {"caldera": (300, 5), "cone": (1000, 20), "cross": (400, 100)}
 
 
 def log(msg: str) -> None:
@@ -60,57 +66,147 @@ def run_engine(exe: str, cfg_path: str, out: str, extra, timeout: int = 1800) ->
         raise RuntimeError(f"{exe} failed ({r.returncode}): {r.stderr[-2000:]}")
     with open(os.path.join(out, "times.json")) as f:
         times = json.load(f)
+    with open(os.path.join(out, "stats.json"), encoding="utf-8") as f:
+        times["_stats"] = json.load(f)
     times["_wall_s"] = wall
     return times
 
 
-# Algorithmic bytes of one launch of each kernel, from the engine's work counters.  The CA figure is 26 B per
-# processed cell-step (13 B of state read + 13 B written: u8 type, i32 flowId, f32 temperature, u16 solid and
-# arrival steps), processed = visited tiles x 1024 cells; neighbour reads count once.  The MC figure is
-# 2 * colours + 4 B per trial (spin read once per colour pass, class read, spin written).
-CA_BYTES_PER_CELL = 26
+def physical_cores() -> int | None:
+    """Physical cores of the host (distinct (package, core) pairs in /proc/cpuinfo)."""
+    try:
+        pairs, phys, core = set(), None, None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("physical id"):
+                    phys = line.split(":")[1].strip()
+                elif line.startswith("core id"):
+                    core = line.split(":")[1].strip()
+                elif not line.strip():
+                    if core is not None:
+                        pairs.add((phys, core))
+                    phys = core = None
+        return len(pairs) or None
+    except OSError:
+        return None
 
 
-def kernel_bytes(times: dict) -> dict:
-    d = times.get("detail", {})
-    w = d.get("caWork", {})
-    k = d.get("kernels", {})
-    out = {}
-    steps = max(1, times.get("steps", 1))
-    visited_cells = w.get("tilesVisited", 0) * w.get("tileCells", 0)
-    if "ca_rules" in k:
-        out["ca_rules"] = CA_BYTES_PER_CELL * visited_cells / steps
-    if "mc_color" in k:
-        colours = max(1, k["mc_color"]["launches"] // max(1, times.get("mcs", 1)))
-        dom = times.get("domainCells", 0)
-        out["mc_color"] = dom * (2 * colours + 4) * times.get("mcs", 0) / max(1, k["mc_color"]["launches"])
-    return out
-
-
-def minimal_bytes(times: dict) -> float:
-    """Lower bound of what one ca_rules launch must move in this in-place design: the type byte of every
-    visited cell, flowId + temperature of every lava cell, one 16 B record per change."""
-    w = times.get("detail", {}).get("caWork", {})
-    steps = max(1, times.get("steps", 1))
-    return (w.get("tilesVisited", 0) * w.get("tileCells", 0) + 8 * w.get("lavaCellSteps", 0)
-            + 16 * w.get("changes", 0)) / steps
-
-
-def cpu_baseline(cfg_path: str, cells: int, tmp: str) -> dict | None:
+def cpu_baseline(kind: str, cfg_path: str, cells: int, tmp: str) -> dict | None:
     if not os.path.exists(REF_ENGINE):
         return None
-    threads = min(16, os.cpu_count() or 1)
+    threads = len(os.sched_getaffinity(0))
+    steps, sweeps = CPU_SAMPLE[kind]
     out = os.path.join(tmp, "cpu")
-    log(f"cpu baseline: reference engine, {threads} threads, full CA phase + first 20 sweeps")
-    t = run_engine(REF_ENGINE, cfg_path, out, ["--threads", str(threads), "--set", "mc.mcs=20",
-                                               "--set", "output.format=none"])
+    log(f"cpu baseline: reference engine, {threads} threads, first {steps} CA steps + {sweeps} sweeps")
+    t = run_engine(REF_ENGINE, cfg_path, out, ["--threads", str(threads), "--set", f"ca.maxSteps={steps}",
+                                               "--set", f"mc.mcs={sweeps}", "--set", "output.format=none",
+                                               "--set", "parallel.affinity=none"], timeout=3600)
     ca_rate = cells * t["steps"] / (t["caSim"] / 1000.0)
     mc_rate = t["domainCells"] * t["mcs"] / (t["mcSim"] / 1000.0) if t.get("mcSim") else None
-    return {"value": ca_rate, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
-            "mc_value": mc_rate, "mc_unit": "spin-attempts/s",
-            "sample": f"reference engine (oracle/_ref, OpenMP) on the same workload: all {t['steps']} CA steps and the "
-                      f"first {t['mcs']} Potts sweeps ({t['domainCells']} domain cells); caSim {t['caSim']:.0f} ms, "
-                      f"mcSim {t['mcSim']:.0f} ms"}
+    return {"value": ca_rate, "unit": "cell-updates/s", "cores": threads, "physical_cores": physical_cores(),
+            "kind": "reference", "mc_value": mc_rate, "mc_unit": "spin-attempts/s",
+            "sample": f"reference engine (oracle/_ref, OpenMP, {threads} threads = every core this process may use) "
+                      f"on the same workload cut to its first {t['steps']} CA steps and {t['mcs']} Potts sweeps "
+                      f"({t['domainCells']} domain cells at that point); caSim {t['caSim']:.0f} ms, mcSim "
+                      f"{t['mcSim']:.0f} ms.  Early steps have a smaller lava front than the whole run, so the "
+                      f"sampled rate overstates the CPU's full-run rate"}
+
+
+def _issue(k: str, ms: float) -> dict:
+    if not os.path.exists(PROFILE_ISSUE):
+        return {"valu_issue_frac": None, "salu_issue_frac": None}
+    with open(PROFILE_ISSUE) as f:
+        e = json.load(f).get(k)
+    if not e:
+        return {"valu_issue_frac": None, "salu_issue_frac": None}
+    return {"valu_issue_frac": round(e["valu"] / (ms / 1000.0) / ISSUE_PEAK_PER_S, 4),
+            "salu_issue_frac": round(e["salu"] / (ms / 1000.0) / ISSUE_PEAK_PER_S, 4)}
+
+
+def _traffic(workload: str, k: str):
+    if not os.path.exists(PROFILE_TRAFFIC):
+        return None
+    with open(PROFILE_TRAFFIC) as f:
+        t = json.load(f)
+    return t.get(workload, {}).get(k)
+
+
+def workload_line(kind: str, a, with_cpu: bool) -> dict:
+    """K timed jobs (after W warm-up jobs) of one scenario through the engine."""
+    from volcano_ca_mc_amd import workloads as W
+    wl = {"caldera": W.caldera_3d, "cone": W.cone_3d, "cross": W.cross_section_2d}[kind]()
+    nx, ny, nz = wl.dims
+    cells = nx * ny * nz
+    with tempfile.TemporaryDirectory(prefix="vca_bench_") as tmp:
+        cfg_path = wl.write(tmp)
+        # kernel times from HIP events on every TIMING_SAMPLE-th CA step / MC sweep of the timed jobs: timing every
+        # launch would add its own events' cost to the measured caSim
+        extra = ["--timing-sample", str(TIMING_SAMPLE), "--set", "output.format=none"]
+        runs = []
+        for i in range(a.warmup + a.steps):
+            t = run_engine(ENGINE, cfg_path, os.path.join(tmp, f"run{i}"), extra)
+            tag = "warmup" if i < a.warmup else "timed"
+            log(f"{kind} {tag} job {i}: steps {t['steps']} caSim {t['caSim']:.1f} ms mcSim {t['mcSim']:.1f} ms "
+                f"total {t['total']:.1f} ms")
+            if i >= a.warmup:
+                runs.append(t)
+        cpu = cpu_baseline(kind, cfg_path, cells, tmp) if with_cpu else None
+    ca_ms = statistics.median(t["caSim"] for t in runs)
+    mc_ms = statistics.median(t["mcSim"] for t in runs)
+    r0 = runs[0]
+    steps, dom, mcs = r0["steps"], r0["domainCells"], r0["mcs"]
+    colours = r0["_stats"].get("mc", {}).get("colors", 1)
+    lat = sorted(t["total"] for t in runs)
+    p99 = lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))]
+    # kernel means over the sampled launches of every timed job
+    ktot = {}
+    for t in runs:
+        for name, v in t.get("detail", {}).get("kernels", {}).items():
+            e = ktot.setdefault(name, [0.0, 0])
+            e[0] += v["ms"]
+            e[1] += v["launches"]
+    mean = {k: v[0] / v[1] for k, v in ktot.items() if v[1]}
+    w = r0.get("detail", {}).get("caWork", {})
+    tile_cells = w.get("tileCells", 1024)
+    staged = w.get("tilesStaged", 0) * tile_cells / max(1, steps)      # cells ca_rules stages per launch
+    visited = w.get("tilesVisited", 0) * tile_cells / max(1, steps)    # cells of every visited tile per step
+    step_ms = ca_ms / max(1, steps)
+    rules_bytes = CA_BYTES_PER_CELL * staged
+    rules_ach = rules_bytes / (mean["ca_rules"] / 1000.0) / 1e9
+    step_ach = CA_BYTES_PER_CELL * visited / (step_ms / 1000.0) / 1e9
+    step_kernels = [k for k in mean if k.startswith("ca_") and k != "ca_step"]
+    roofline = {"bound": "hbm", "achieved": round(rules_ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(rules_ach / HBM_PEAK_GBS, 4), "traffic": _traffic(kind, "ca_rules"),
+                "kernel": "ca_rules", "kernel_mean_ms": round(mean["ca_rules"], 5),
+                "timed_every_nth_step": TIMING_SAMPLE, "bytes_per_unit": f"{CA_BYTES_PER_CELL} B per staged cell",
+                "staged_cells_per_launch": round(staged), "algorithmic_bytes_per_launch": round(rules_bytes),
+                **_issue("ca_rules", mean["ca_rules"]),
+                "kernel_mean_ms_all": {k: round(mean[k], 5) for k in step_kernels},
+                "launches_per_step": len(step_kernels)}
+    step_roofline = {"achieved": round(step_ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(step_ach / HBM_PEAK_GBS, 4), "step_ms": round(step_ms, 5),
+                     "visited_cells_per_step": round(visited),
+                     "bytes_per_unit": f"{CA_BYTES_PER_CELL} B per cell of every visited tile, whole step"}
+    mc_roofline = None
+    if "mc_color" in mean and dom:
+        # one launch = one colour pass over domain / colours cells; 2 * colours + 4 B per trial
+        mc_bytes = dom * (2 * colours + 4) / colours
+        mc_ach = mc_bytes / (mean["mc_color"] / 1000.0) / 1e9
+        mc_roofline = {"bound": "latency", "kernel": "mc_color", "kernel_mean_ms": round(mean["mc_color"], 5),
+                       "achieved": round(mc_ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(mc_ach / HBM_PEAK_GBS, 4), "traffic": _traffic(kind, "mc_color"),
+                       "colours": colours, "bytes_per_unit": "2 * colours + 4 B per spin attempt",
+                       "algorithmic_bytes_per_launch": round(mc_bytes), **_issue("mc_color", mean["mc_color"])}
+    desc = {"caldera": f"caldera_3d {nx}x{ny}x{nz} random pentagonal, periodic x/y, 8 craters, maxSteps 2000, "
+                       f"Potts Q128 kT0.5 100 sweeps (checkerboard)",
+            "cone": f"cone_3d {nx}x{ny}x{nz} moore absorbing, 3 craters, maxSteps 1000, Potts Q64 kT0.5 200 sweeps",
+            "cross": f"cross_section_2d {nx}x{ny}x{nz} von Neumann absorbing, 1 crater, maxSteps 400, Potts Q64 kT0.5 "
+                     f"100 sweeps"}[kind]
+    return {"value": cells * steps / (ca_ms / 1000.0), "mc_value": dom * mcs / (mc_ms / 1000.0),
+            "p50_job_ms": statistics.median(lat), "p99_job_ms": p99, "ca_ms": ca_ms, "mc_ms": mc_ms,
+            "config": {"workload": desc, "cells": cells, "ca_steps": steps, "mc_domain_cells": dom, "mc_sweeps": mcs,
+                       "mc_colours": colours, "parallelism": "single GPU"},
+            "roofline": roofline, "step_roofline": step_roofline, "mc_roofline": mc_roofline, "cpu_baseline": cpu}
 
 
 def _write_jobs(workloads, folder: str) -> None:
@@ -144,11 +240,11 @@ def _same_stats(a: str, b: str, names) -> bool:
     return True
 
 
-def bench_batched(a) -> dict:
-    """256 uneven jobs: the batched pipeline against the same jobs one at a time (--max-jobs 1), same process
-    and engine; stats.json of every job must be equal between the two."""
+def bench_batched(jobs_n: int) -> dict:
+    """The uneven mix: the batched pipeline against the same jobs one at a time (--max-jobs 1), same engine;
+    stats.json of every job must be equal between the two."""
     from volcano_ca_mc_amd import workloads as W
-    jobs = W.batched_mix(jobs=a.jobs)
+    jobs = W.batched_mix(jobs=jobs_n)
     with tempfile.TemporaryDirectory(prefix="vca_batched_") as tmp:
         folder = os.path.join(tmp, "jobs")
         _write_jobs(jobs, folder)
@@ -159,171 +255,113 @@ def bench_batched(a) -> dict:
         log(f"batched: {many['makespanS']:.2f} s, {many['jobsPerS']:.2f} jobs/s")
         same = _same_stats(os.path.join(tmp, "one"), os.path.join(tmp, "many"), [w.name for w in jobs])
     return {"metric": "jobs/s (batched uneven simulations, bit-exact)", "value": many["jobsPerS"], "unit": "jobs/s",
-            "n_gpus": 1, "higher_is_better": True, "vs_one_at_a_time": many["jobsPerS"] / max(1e-9, one["jobsPerS"]),
+            "vs_one_at_a_time": many["jobsPerS"] / max(1e-9, one["jobsPerS"]),
             "one_at_a_time_jobs_per_s": one["jobsPerS"], "makespan_s": many["makespanS"],
             "one_at_a_time_makespan_s": one["makespanS"], "cell_updates_per_s": many["cellUpdatesPerS"],
             "p50_job_ms": many["p50LatencyMs"], "p99_job_ms": many["p99LatencyMs"], "identical_stats": same,
-            "gpu_thread_ms": many.get("gpuThread"), "one_at_a_time_gpu_thread_ms": one.get("gpuThread"),
-            "dtype": "u8+i32+f32", "data": "synthetic (seeded terrains)",
-            "config": {"workload": f"{len(jobs)} jobs from one seed: 2D edges 128-2048, 3D edges 64-256, 200-800 steps, "
-                                   "50 Potts sweeps Q64", "parallelism": "single GPU"}}
+            "gpu_thread_ms": many.get("gpuThread"),
+            "workload": f"{len(jobs)} jobs from one seed: 2D edges 128-2048, 3D edges 64-256, 200-800 steps, "
+                        "50 Potts sweeps Q64"}
 
 
-def bench_serving(a) -> dict:
+def bench_serving(rate: float, duration: float, scale: float, capacity_jobs: int) -> dict:
+    """The serving mix as a Poisson stream (latency at the offered rate), then the first capacity_jobs requests
+    of the same stream all offered at t = 0 (the loop's capacity)."""
     from volcano_ca_mc_amd import workloads as W
-    arr = W.serving_stream(rate=a.rate, duration=a.duration, scale=a.scale)
+    arr = W.serving_stream(rate=rate, duration=duration, scale=scale)
+    out = {"workload": f"Poisson {rate}/s for {duration} s: 70% 2D 512x256 x300 steps, 25% 3D 256^3 x600, "
+                       f"5% 768x768x256 x1200, 50-200 sweeps; grid scale {scale}", "offered_rate": rate}
     with tempfile.TemporaryDirectory(prefix="vca_serve_") as tmp:
         folder = os.path.join(tmp, "jobs")
         _write_jobs([x.workload for x in arr], folder)
-        with open(os.path.join(tmp, "jobs.jsonl"), "w") as f:
-            for x in arr:
-                f.write(json.dumps({"at": x.t, "config": os.path.join(folder, x.workload.name + ".json"),
-                                    "name": x.workload.name}) + "\n")
-        log(f"{len(arr)} requests over {a.duration} s written")
-        s = _pipeline(["--serve", os.path.join(tmp, "jobs.jsonl")], os.path.join(tmp, "out"))
-    return {"metric": "served jobs/s with p50/p99 latency (bit-exact)", "value": s["jobsPerS"], "unit": "jobs/s",
-            "n_gpus": 1, "higher_is_better": True, "p50_job_ms": s["p50LatencyMs"], "p99_job_ms": s["p99LatencyMs"],
-            "offered_rate": a.rate, "makespan_s": s["makespanS"], "cell_updates_per_s": s["cellUpdatesPerS"],
-            "spin_attempts_per_s": s["spinAttemptsPerS"], "jobs": s["jobs"], "ok": s["ok"], "dtype": "u8+i32+f32",
-            "gpu_thread_ms": s.get("gpuThread"),
-            "data": "synthetic (seeded terrains)",
-            "config": {"workload": f"Poisson {a.rate}/s for {a.duration} s: 70% 2D 512x256 x300 steps, 25% 3D 256^3 x600, "
-                                   f"5% 768x768x256 x1200, 50-200 sweeps; grid scale {a.scale}",
-                       "parallelism": "single GPU"}}
+
+        def stream(path, items, at0):
+            with open(path, "w") as f:
+                for x in items:
+                    f.write(json.dumps({"at": 0.0 if at0 else x.t,
+                                        "config": os.path.join(folder, x.workload.name + ".json"),
+                                        "name": x.workload.name}) + "\n")
+            return path
+
+        log(f"{len(arr)} requests over {duration} s written")
+        s = _pipeline(["--serve", stream(os.path.join(tmp, "jobs.jsonl"), arr, False)], os.path.join(tmp, "out"))
+        out.update({"jobs_per_s": s["jobsPerS"], "p50_job_ms": s["p50LatencyMs"], "p99_job_ms": s["p99LatencyMs"],
+                    "jobs": s["jobs"], "ok": s["ok"], "makespan_s": s["makespanS"],
+                    "cell_updates_per_s": s["cellUpdatesPerS"], "spin_attempts_per_s": s["spinAttemptsPerS"]})
+        cap = arr[:capacity_jobs]
+        c = _pipeline(["--serve", stream(os.path.join(tmp, "cap.jsonl"), cap, True)], os.path.join(tmp, "cap"))
+        out["capacity"] = {"jobs": c["jobs"], "ok": c["ok"], "jobs_per_s": c["jobsPerS"],
+                           "makespan_s": c["makespanS"], "cell_updates_per_s": c["cellUpdatesPerS"],
+                           "note": "the same requests all offered at t = 0"}
+    return out
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["cone", "batched", "serving"], default="cone",
-                    help="cone: the 3D cone job (default line); batched: 256 uneven jobs; serving: Poisson stream")
+    ap.add_argument("--workload", choices=["caldera", "cone", "cross", "batched", "serving"], default="caldera",
+                    help="caldera (default line), cone, cross: one scenario; batched: 256 uneven jobs; "
+                         "serving: Poisson stream")
     ap.add_argument("--jobs", type=int, default=256)
     ap.add_argument("--rate", type=float, default=20.0)
-    ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--duration", type=float, default=30.0)
+    ap.add_argument("--capacity-jobs", type=int, default=300)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3, help="timed jobs")
+    ap.add_argument("--steps", type=int, default=5, help="timed jobs")
     ap.add_argument("--warmup", type=int, default=1, help="untimed jobs")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1.0 = the benchmark workload)")
+    ap.add_argument("--no-extra", action="store_true", help="only the main workload line")
+    ap.add_argument("--scale", type=float, default=1.0, help="grid scale of the serving mix (1.0 = the benchmark)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
+        print(json.dumps({"error": "this engine runs on one GPU; runs on more than one GPU are not part of it",
                           "n_gpus": a.gpus}))
         return 2
     if not os.path.exists(ENGINE):
         raise SystemExit(f"{ENGINE} missing: run `python -m volcano_ca_mc_amd.build` first")
     if a.workload == "batched":
-        print(json.dumps(bench_batched(a)))
+        print(json.dumps({"n_gpus": 1, **bench_batched(a.jobs)}))
         return 0
     if a.workload == "serving":
-        print(json.dumps(bench_serving(a)))
+        s = bench_serving(a.rate, a.duration, a.scale, a.capacity_jobs)
+        print(json.dumps({"metric": "served jobs/s with p50/p99 latency (bit-exact)", "value": s["jobs_per_s"],
+                          "unit": "jobs/s", "n_gpus": 1, **s}))
         return 0
-    from volcano_ca_mc_amd import workloads as W
-    n = max(16, int(round(512 * a.scale)))
-    wl = W.cone_3d(n=n, nz=max(16, int(round(256 * a.scale))))
-    nx, ny, nz = wl.dims
-    cells = nx * ny * nz
-    with tempfile.TemporaryDirectory(prefix="vca_bench_") as tmp:
-        cfg_path = wl.write(tmp)
-        # kernel times from HIP events on every TIMING_SAMPLE-th CA step / MC sweep of the timed jobs: timing every
-        # launch would add its own events' cost to the measured caSim
-        extra = ["--timing-sample", str(TIMING_SAMPLE), "--set", "output.format=none"]
-        runs = []
-        for i in range(a.warmup + a.steps):
-            t = run_engine(ENGINE, cfg_path, os.path.join(tmp, f"run{i}"), extra)
-            tag = "warmup" if i < a.warmup else "timed"
-            log(f"{tag} job {i}: steps {t['steps']} caSim {t['caSim']:.1f} ms mcSim {t['mcSim']:.1f} ms "
-                f"total {t['total']:.1f} ms")
-            if i >= a.warmup:
-                runs.append(t)
-        ca_ms = [t["caSim"] for t in runs]
-        mc_ms = [t["mcSim"] for t in runs]
-        steps = runs[0]["steps"]
-        dom = runs[0]["domainCells"]
-        mcs = runs[0]["mcs"]
-        ca_rate = cells * steps / (statistics.median(ca_ms) / 1000.0)
-        mc_rate = dom * mcs / (statistics.median(mc_ms) / 1000.0)
-        lat = sorted(t["total"] for t in runs)
-        p50 = statistics.median(lat)
-        p99 = lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))]
-        # roofline of the dominant kernel (largest summed device time across the timed jobs)
-        ktot = {}
-        for t in runs:
-            for name, v in t.get("detail", {}).get("kernels", {}).items():
-                if name in ("ca_step", "mc_sweep"):
-                    continue
-                e = ktot.setdefault(name, [0.0, 0])
-                e[0] += v["ms"]
-                e[1] += v["launches"]
-        bytes_per_launch = kernel_bytes(runs[0])
-        dom_k = max((k for k in ktot if k in bytes_per_launch), key=lambda k: ktot[k][0])
-        mean_ms = ktot[dom_k][0] / ktot[dom_k][1]
-        achieved = bytes_per_launch[dom_k] / (mean_ms / 1000.0) / 1e9
-        traffic = None
-        if os.path.exists(PROFILE_TRAFFIC):
-            with open(PROFILE_TRAFFIC) as f:
-                traffic = json.load(f).get(dom_k)
-        issue = {}
-        if os.path.exists(PROFILE_ISSUE):
-            with open(PROFILE_ISSUE) as f:
-                issue = json.load(f)
-
-        def issue_fracs(k: str, ms: float) -> dict:
-            e = issue.get(k)
-            if not e:
-                return {"valu_issue_frac": None, "salu_issue_frac": None}
-            return {"valu_issue_frac": round(e["valu"] / (ms / 1000.0) / ISSUE_PEAK_PER_S, 4),
-                    "salu_issue_frac": round(e["salu"] / (ms / 1000.0) / ISSUE_PEAK_PER_S, 4)}
-
-        mc_roofline = None
-        if "mc_color" in ktot and "mc_color" in bytes_per_launch:
-            mc_launch_ms = ktot["mc_color"][0] / ktot["mc_color"][1]
-            mc_ach = bytes_per_launch["mc_color"] / (mc_launch_ms / 1000.0) / 1e9
-            mc_traffic = None
-            if os.path.exists(PROFILE_TRAFFIC):
-                with open(PROFILE_TRAFFIC) as f:
-                    mc_traffic = json.load(f).get("mc_color")
-            mc_roofline = {"bound": "latency", "kernel": "mc_color", "kernel_mean_ms": round(mc_launch_ms, 5),
-                           "achieved": round(mc_ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(mc_ach / HBM_PEAK_GBS, 4), "traffic": mc_traffic,
-                           "bytes_per_unit": "2 * colours + 4 B per spin attempt",
-                           "algorithmic_bytes_per_launch": round(bytes_per_launch["mc_color"]),
-                           **issue_fracs("mc_color", mc_launch_ms)}
-        w = runs[0].get("detail", {}).get("caWork", {})
-        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_k,
-                    "kernel_mean_ms": round(mean_ms, 5), "timed_every_nth_step": TIMING_SAMPLE, "algorithmic_bytes_per_launch": round(bytes_per_launch[dom_k]),
-                    "bytes_per_unit": f"{CA_BYTES_PER_CELL} B per processed cell-step" if dom_k == "ca_rules" else None,
-                    "processed_cells_per_launch": round(w.get("tilesVisited", 0) * w.get("tileCells", 0) /
-                                                        max(1, steps)),
-                    "staged_cells_per_launch": round(w.get("tilesStaged", 0) * w.get("tileCells", 0) /
-                                                     max(1, steps)),
-                    "minimal_bytes_per_launch": round(minimal_bytes(runs[0])),
-                    **issue_fracs(dom_k, mean_ms),
-                    "kernel_share": {k: round(v[0] / sum(x[0] for x in ktot.values()), 3) for k, v in ktot.items()}}
-        cpu = None if a.no_cpu else cpu_baseline(cfg_path, cells, tmp)
+    line = workload_line(a.workload, a, not a.no_cpu)
+    extra = {}
+    if not a.no_extra:
+        if a.workload != "cone":
+            log("extra: cone workload")
+            sub = argparse.Namespace(steps=3, warmup=1)
+            c = workload_line("cone", sub, False)
+            extra["cone"] = {k: c[k] for k in ("value", "mc_value", "p50_job_ms", "p99_job_ms", "config", "roofline",
+                                               "step_roofline", "mc_roofline")}
+        log("extra: batched mix")
+        extra["batched"] = bench_batched(a.jobs)
+        log("extra: serving stream")
+        extra["serving"] = bench_serving(a.rate, a.duration, 1.0, a.capacity_jobs)
     result = {
         "metric": "CA cell-updates/s + MC spin-attempts/s (bit-exact); p50/p99 job latency",
-        "value": ca_rate,
+        "value": line["value"],
         "unit": "cell-updates/s",
-        "mc_value": mc_rate,
+        "mc_value": line["mc_value"],
         "mc_unit": "spin-attempts/s",
-        "p50_job_ms": p50,
-        "p99_job_ms": p99,
+        "p50_job_ms": line["p50_job_ms"],
+        "p99_job_ms": line["p99_job_ms"],
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": statistics.median(ca_ms) + statistics.median(mc_ms),
+        "ms_per_step": line["ca_ms"] + line["mc_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8+i32+f32",
         "data": "synthetic (seeded terrain)",
-        "config": {"workload": f"cone_3d {nx}x{ny}x{nz} moore absorbing, 3 craters, maxSteps 1000, "
-                               f"Potts Q64 kT0.5 200 sweeps (checkerboard)", "cells": cells, "ca_steps": steps,
-                   "mc_domain_cells": dom, "mc_sweeps": mcs, "parallelism": "single GPU"},
-        "roofline": roofline,
-        "mc_roofline": mc_roofline,
-        "cpu_baseline": cpu,
+        "config": line["config"],
+        "roofline": line["roofline"],
+        "step_roofline": line["step_roofline"],
+        "mc_roofline": line["mc_roofline"],
+        "cpu_baseline": line["cpu_baseline"],
+        "extra": extra,
     }
     print(json.dumps(result))
     return 0

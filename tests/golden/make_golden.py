@@ -120,6 +120,22 @@ def case_matrix():
                                    "boundary": "periodic", "rngSeed": 21,
                                    "mc": dict(mcfull, Q=70000, mcs=10, kT=2.0, J=0.7, update="checkerboard")},
                                   None)
+    # the NoChange stop rule (LavaCA.hpp:1222-1226) with the reference's own parameters
+    # (engine/tests/test_front.cpp:159-178): lava that does not cool in the air stops spreading after the eruption,
+    # and a step without changes ends a run with a deterministic neighbourhood -- but not with a random one
+    nochange2d = {"space": {"nx": 120, "nz": 70}, "terrain": {"peakHeight": 45, "baseRadius": 55},
+                  "ca": {"coolingAmbient": 0, "coolingLateral": 3, "coolingFall": 1},
+                  "vents": {"eruptionSteps": 40},
+                  "mc": {"enabled": True, "Q": 10, "mcs": 6, "statsEvery": 3, "kT": 0.5, "update": "checkerboard"},
+                  "output": {"format": "bin"}}
+    for nb in ("moore", "von_neumann"):
+        cases[f"lava2d_nochange_{nb}"] = (_deep(nochange2d, {"neighborhood": nb}), None)
+    cases["lava2d_nochange_pentagonal_maxsteps"] = (_deep(nochange2d, {"neighborhood": "pentagonal",
+                                                                       "ca": {"maxSteps": 400}}), None)
+    cases["lava3d_nochange_moore"] = (lava3d("moore", "absorbing", ca={"coolingAmbient": 0, "maxSteps": 2000},
+                                             vents={"eruptionSteps": 30}, mc={"mcs": 4}), None)
+    cases["lava3d_nochange_vn_periodic"] = (lava3d("von_neumann", "periodic", ca={"coolingAmbient": 0, "maxSteps": 2000},
+                                                   vents={"eruptionSteps": 30}, mc={"mcs": 4}), None)
     # scaled-down versions of the benchmark scenarios (CSV terrain, explicit crater columns)
     w = W.cross_section_2d(seed=5, nx=256, nz=128, steps=200, mcs=20)
     w.config["mc"]["statsEvery"] = 5
@@ -144,13 +160,21 @@ def run_ref(cfg_path: str, out: str, cwd: str):
         raise RuntimeError(f"reference failed on {cfg_path}: {r.stderr}")
 
 
-def make_cases():
-    if os.path.isdir(CASES):
-        shutil.rmtree(CASES)
-    os.makedirs(CASES)
+def make_cases(only=None):
+    """Writes every case (only: the named cases, kept beside the others in the manifest)."""
     manifest = {}
+    if only:
+        with open(os.path.join(HERE, "manifest.json")) as f:
+            manifest = json.load(f)
+    elif os.path.isdir(CASES):
+        shutil.rmtree(CASES)
+    os.makedirs(CASES, exist_ok=True)
     for name, (cfg, wl) in case_matrix().items():
+        if only and name not in only:
+            continue
         d = os.path.join(CASES, name)
+        if os.path.isdir(d):
+            shutil.rmtree(d)
         os.makedirs(d)
         if wl is not None:
             wl.name = "config"
@@ -214,11 +238,12 @@ def make_scenario(name: str, threads: int = 8):
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--cases", action="store_true")
+    ap.add_argument("--only", nargs="*", default=None, help="with --cases: these cases only")
     ap.add_argument("--scenarios", nargs="*", default=[])
     a = ap.parse_args()
     if not os.path.exists(REF):
         sys.exit("oracle/_ref/engine is missing: run `make -C oracle ref` where /root/reference exists")
     if a.cases:
-        make_cases()
+        make_cases(a.only)
     for s in a.scenarios:
         make_scenario(s)

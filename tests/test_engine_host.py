@@ -62,3 +62,48 @@ def test_engine_spill_levels_match_oracle(fill_exe):
     for (h, px, py), row in zip(maps, rows):
         got = np.array(row.split(), dtype=np.int32).reshape(h.shape)
         assert np.array_equal(got, O.fill_depressions(h, bool(px), bool(py))), (h.shape, px, py)
+
+
+SHA_HARNESS = r"""
+#include <cstdio>
+#include <vector>
+#include "sha256.hpp"
+// stdin: records "<n> <chunk>" then n raw bytes; stdout: one hex digest per record (the bytes fed in chunks)
+int main() {
+    unsigned long n, chunk;
+    while (std::scanf("%lu %lu", &n, &chunk) == 2) {
+        std::getchar();
+        std::vector<unsigned char> b(n);
+        if (n) std::fread(b.data(), 1, n, stdin);
+        eng::Sha256 h;
+        for (unsigned long at = 0; at < n; at += chunk) h.update(b.data() + at, n - at < chunk ? n - at : chunk);
+        std::printf("%s %llu\n", h.hex().c_str(), (unsigned long long)h.bytes());
+    }
+}
+"""
+
+
+def test_engine_sha256_matches_hashlib(tmp_path):
+    """The digest of --digests mode (engine/sha256.cpp, FIPS 180-4) equals Python's hashlib on lengths around every
+    padding boundary, with the bytes fed in chunks of different sizes."""
+    import hashlib
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "sha.cpp"
+    src.write_text(SHA_HARNESS)
+    exe = tmp_path / "sha"
+    subprocess.run(["g++", "-std=c++17", "-O1", f"-I{ENGINE_SRC}", str(src), os.path.join(ENGINE_SRC, "sha256.cpp"),
+                    "-o", str(exe)], check=True)
+    rng = np.random.default_rng(5)
+    blobs, payload = [], b""
+    for n in list(range(0, 130)) + [447, 448, 511, 512, 1000, 65536, 300001]:
+        for chunk in (1, 7, 64, 1 << 20):
+            if chunk == 1 and n > 1000:
+                continue
+            b = rng.integers(0, 256, size=n, dtype=np.uint8).tobytes()
+            blobs.append(b)
+            payload += f"{n} {chunk}\n".encode() + b
+    out = subprocess.run([str(exe)], input=payload, capture_output=True, check=True).stdout.decode().split("\n")
+    for b, line in zip(blobs, out):
+        assert line == f"{hashlib.sha256(b).hexdigest()} {len(b)}"
+    assert hashlib.sha256(b"abc").hexdigest().startswith("ba7816bf")  # FIPS 180-4 example B.1

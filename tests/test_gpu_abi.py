@@ -88,10 +88,12 @@ def _multi_vent_cases():
 
 
 @pytest.mark.parametrize("name", _multi_vent_cases())
-def test_flow_ids_beyond_16_bits(ctx, name):
-    """Vent flowIds above 32767 (the rules kernel then holds flowIds in 32 bits of LDS, not 16): the run equals
-    the run with the case's own ids (checked against the oracle above), every flowId moved by the same offset --
-    the only order the rules put on flowIds, the vote's tie-break, is kept by the shift."""
+@pytest.mark.parametrize("ids", ["shifted", "permuted"])
+def test_flow_ids_beyond_16_bits(ctx, name, ids):
+    """Vent flowIds above 32767 (the rules kernel then holds flowIds in 32 bits of LDS, not 16), run by the HIP path
+    and by the oracle with the same ids: every state array, the step count, the stop reason and both histories must
+    be identical.  "shifted" moves every id by 40000 (the ids keep their order); "permuted" gives the vents large ids
+    in a different order than their list order, which changes which flow a tie between two flows picks."""
     d, cfg = _inputs(name)
     hm = spill = None
     if cfg["ca"]["mode"] == "lava":
@@ -99,22 +101,26 @@ def test_flow_ids_beyond_16_bits(ctx, name):
         per = cfg["boundary"] == "periodic"
         spill = O.fill_depressions(hm, per, per)
     vents = O.vents_for(cfg, hm)
-    shift = 40000
     moved = vents.copy()
-    moved[:, 3] += shift
-    assert vents[:, 3].min() >= 1 and moved[:, 3].min() > 32767
-    print(f"case {name}: dims {cfg['space']}, {len(vents)} vents", flush=True)
-    base = _gpu_ca(ctx, cfg, hm, spill, vents)
-    big = _gpu_ca(ctx, cfg, hm, spill, moved)
-    assert base.run() == big.run()
-    a, b = base.result(), big.result()
-    for k in ("type", "solid", "arrival"):
-        assert np.array_equal(a[k], b[k]), k
-    assert np.array_equal(a["temp"].view(np.uint32), b["temp"].view(np.uint32))
-    assert np.array_equal(np.where(a["flow"] != 0, a["flow"] + shift, 0), b["flow"])
-    assert base.histories() == big.histories()
-    base.close()
-    big.close()
+    if ids == "shifted":
+        moved[:, 3] += 40000
+    else:
+        big = [2000000 - 37 * k * k - 1000 * (k % 3) for k in range(len(vents))]
+        moved[:, 3] = np.array(big, dtype=moved.dtype)[::-1]
+    assert moved[:, 3].min() > 32767 and len(set(moved[:, 3].tolist())) == len(vents)
+    print(f"case {name}: dims {cfg['space']}, flowIds {moved[:, 3].tolist()}", flush=True)
+    orc = O.OracleCA(O.ca_params(cfg), hm, spill, moved)
+    orc.run()
+    ref = orc.state()
+    gpu = _gpu_ca(ctx, cfg, hm, spill, moved)
+    assert gpu.run() == (orc.steps, orc.stop_reason)
+    got = gpu.result()
+    for k in ("type", "flow", "solid", "arrival"):
+        assert np.array_equal(got[k], ref[k]), k
+    assert np.array_equal(got["temp"].view(np.uint32), ref["temp"].view(np.uint32))
+    assert gpu.histories() == orc.histories()
+    assert set(np.unique(got["flow"]).tolist()) <= set(moved[:, 3].tolist()) | {0}
+    gpu.close()
 
 
 def _mc_cases():

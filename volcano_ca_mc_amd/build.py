@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 LIB_SOURCES = ["vca_device.hip", "vca_api.hip"]
-ENGINE_SOURCES = ["engine/json.cpp", "engine/config.cpp", "engine/terrain.cpp", "engine/writer.cpp",
+ENGINE_SOURCES = ["engine/sha256.cpp", "engine/json.cpp", "engine/config.cpp", "engine/terrain.cpp", "engine/writer.cpp",
                   "engine/simulation.cpp", "engine/batch.cpp", "engine/pipeline.cpp", "engine/main.cpp"]
 
 

@@ -218,6 +218,7 @@ int run_batch(const CliOptions& opt) {
     }
     if (opt.timing_detail) extra.push_back("--timing-detail");
     if (opt.thread_stats) extra.push_back("--thread-stats");
+    if (opt.digests) extra.push_back("--digests");
     Progress progress(opt.progress);
     const fs::path summary = out / "batch_summary.csv";
     std::cout << "=== Seria (--batch): " << items.size() << " configów z " << folder.generic_string() << " -> "

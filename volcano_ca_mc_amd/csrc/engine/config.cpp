@@ -75,7 +75,8 @@ E get_enum(const Json& j, const std::string& path, std::initializer_list<std::pa
 std::vector<std::pair<int, int>> get_columns(const Json& j, const std::string& path, const Dims& d) {
     const Json& list = field(j, path);
     if (!list.is_array())
-        throw ConfigError("Parametr " + q(path) + " musi być listą par [x, y], podano: " + list.dump());
+        throw ConfigError("Parametr " + q(path) + " musi być listą par [x, y], np. [[80, 75], [82, 70]], podano: " +
+                          list.dump());
     std::vector<std::pair<int, int>> out;
     for (size_t i = 0; i < list.size(); ++i) {
         const Json& p = list[i];
@@ -87,11 +88,14 @@ std::vector<std::pair<int, int>> get_columns(const Json& j, const std::string& p
             throw ConfigError("Parametr " + q(where) + " musi być parą liczb całkowitych [x, y], podano: " + p.dump());
         const long long x = p[0].get_int(), y = p[1].get_int();
         if (x < 0 || x >= d.nx || y < 0 || y >= d.ny)
-            throw ConfigError("Parametr " + q(where) + " = " + p.dump() + " leży poza siatką " + std::to_string(d.nx) +
-                              " x " + std::to_string(d.ny));
+            throw ConfigError("Parametr " + q(where) + " = " + p.dump() + " leży poza siatką: x musi być z zakresu 0.." +
+                              std::to_string(d.nx - 1) + ", y z zakresu 0.." + std::to_string(d.ny - 1) +
+                              " (space.nx = " + std::to_string(d.nx) + ", space.ny = " + std::to_string(d.ny) + ")");
         for (const auto& c : out)
             if (c.first == x && c.second == y)
-                throw ConfigError("Parametr " + q(where) + " = " + p.dump() + " powtarza kolumnę z wcześniejszego elementu");
+                throw ConfigError("Parametr " + q(where) + " = " + p.dump() +
+                                  " powtarza kolumnę z wcześniejszego elementu listy; w jednej kolumnie może być "
+                                  "najwyżej jeden krater");
         out.emplace_back(static_cast<int>(x), static_cast<int>(y));
     }
     return out;
@@ -252,9 +256,12 @@ Config Config::from_json(const Json& j) {
     c.ca.ponding = get_bool(j, "ca.ponding");
     c.ca.front = get_enum<bool>(j, "ca.sweep", {{"front", true}, {"full", false}});
     if (runs_ca && !c.ca.classic && c.space.is2d() && c.space.ny != 1)
-        throw ConfigError("W trybie lava przestrzeń 2D to przekrój pionowy x–z: wymagane ny = 1");
+        throw ConfigError("W trybie lava przestrzeń 2D to przekrój pionowy x–z: wymagane ny = 1 (podano nx = " +
+                          std::to_string(c.space.nx) + ", ny = " + std::to_string(c.space.ny) + ", nz = " +
+                          std::to_string(c.space.nz) + ")");
     if (c.neighborhood == Neighborhood::Hexagonal && !c.space.is2d())
-        throw ConfigError("Sąsiedztwo „hexagonal” jest zdefiniowane tylko w 2D");
+        throw ConfigError("Sąsiedztwo „hexagonal” jest zdefiniowane tylko w 2D; dla przestrzeni 3D "
+                          "wybierz von_neumann, moore lub pentagonal");
 
     c.terrain.type = get_enum<TerrainType>(j, "terrain.type", {{"cone", TerrainType::Cone}, {"file", TerrainType::File}});
     c.terrain.peak = get_number(j, "terrain.peakHeight", 0.0, 1e6);
@@ -277,26 +284,36 @@ Config Config::from_json(const Json& j) {
     const size_t listed = get_columns(j, "vents.positions", c.space).size();
     const bool lava_ca = runs_ca && !c.ca.classic;
     if (lava_ca && c.vents.placement == VentPlacement::Positions) {
-        if (listed == 0) throw ConfigError("vents.placement = \"positions\" wymaga listy kraterów w „vents.positions”");
+        if (listed == 0)
+            throw ConfigError("vents.placement = \"positions\" wymaga listy kraterów w „vents.positions”, "
+                              "np. [[80, 75]] (w GUI: kliknięcie na mapie wysokości)");
         if (static_cast<size_t>(c.vents.count) != listed)
             throw ConfigError("vents.count (" + std::to_string(c.vents.count) +
-                              ") musi być równe liczbie kraterów w „vents.positions” (" + std::to_string(listed) + ")");
+                              ") musi być równe liczbie kraterów w „vents.positions” (" + std::to_string(listed) +
+                              "), bo przy placement = \"positions\" każdy krater ma podane położenie");
     } else if (lava_ca && listed > 0) {
-        c.warnings.push_back("vents.positions jest ignorowane, bo vents.placement nie jest \"positions\"");
+        c.warnings.push_back("vents.positions jest ignorowane, bo vents.placement = \"" +
+                             std::string(c.vents.placement == VentPlacement::Crater ? "crater" : "random") +
+                             "\" (położenia losowane); kratery w zadanych miejscach: placement = \"positions\"");
     } else if (runs_ca && c.ca.classic && c.vents.placement == VentPlacement::Positions) {
-        c.warnings.push_back("w trybie classic zarodki są losowane w całej przestrzeni; vents.positions jest ignorowane");
+        c.warnings.push_back("w trybie classic zarodki są losowane w całej przestrzeni, więc vents.placement = "
+                             "\"positions\" i vents.positions są ignorowane");
     }
     c.vents.eruption_steps = get_int(j, "vents.eruptionSteps", 0, 65535);
     c.vents.erupt_temp = get_number(j, "vents.eruptTemp", -1e6, 1e6);
     if (runs_ca && !c.ca.classic && c.vents.erupt_temp < c.ca.solidus)
-        throw ConfigError("vents.eruptTemp musi być nie mniejsze niż ca.solidusTemp");
+        throw ConfigError("vents.eruptTemp musi być nie mniejsze niż ca.solidusTemp, inaczej lawa "
+                          "zastygłaby od razu w kraterze");
     if (runs_ca && c.ca.classic && static_cast<uint64_t>(c.vents.count) > c.space.cells())
-        throw ConfigError("vents.count nie może przekraczać liczby komórek");
+        throw ConfigError("vents.count (" + std::to_string(c.vents.count) + ") nie może przekraczać liczby komórek (" +
+                          std::to_string(c.space.cells()) + ")");
 
     if (c.mc.enabled && !c.mc.full && c.ca.classic)
-        throw ConfigError("mc.domain = \"basalt\" wymaga ca.mode = \"lava\"; klasyczny rozrost MC to mc.domain = \"full\"");
+        throw ConfigError("mc.domain = \"basalt\" wymaga ca.mode = \"lava\" (w trybie classic nie ma bazaltu); "
+                          "klasyczny rozrost ziaren MC to mc.domain = \"full\"");
     if (c.mc.full_only() && c.mc.mobility_from_cooling)
-        c.warnings.push_back("mc.mobilityFromCooling dotyczy tylko mc.domain = \"basalt\"; w trybie full jest ignorowane");
+        c.warnings.push_back("mc.mobilityFromCooling dotyczy tylko mc.domain = \"basalt\" (czas stygnięcia lawy z CA); "
+                             "w trybie full jest ignorowane");
 
     c.output.format = get_enum<OutputFormat>(j, "output.format",
                                              {{"txt", OutputFormat::Txt},
@@ -312,6 +329,10 @@ Config Config::from_json(const Json& j) {
     get_enum<int>(j, "parallel.schedule", {{"static", 0}, {"dynamic", 1}, {"guided", 2}});
     c.parallel.chunk = get_int(j, "parallel.chunk", 0, 1000000);
     c.parallel.affinity = get_enum<std::string>(j, "parallel.affinity", {{"none", "none"}, {"pcores", "pcores"}});
+    // the sequential MC update runs as one lane on the device as well; the warning text is the reference's
+    if (c.mc.enabled && !c.mc.checkerboard && c.parallel.threads != 1)
+        c.warnings.push_back("mc.update = \"random\" jest z natury sekwencyjne (każda próba widzi wynik poprzednich), "
+                             "więc pętla MC idzie w jednym wątku; równoległe MC: mc.update = \"checkerboard\"");
     get_enum<bool>(j, "parallel.priority", {{"normal", false}, {"high", true}});
     get_enum<bool>(j, "mpi.exchange", {{"always", false}, {"changed", true}});
     get_enum<bool>(j, "mpi.pack", {{"manual", false}, {"datatype", true}});

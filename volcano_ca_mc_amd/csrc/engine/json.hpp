@@ -23,6 +23,12 @@ class JsonError : public std::runtime_error {
 public:
     using std::runtime_error::runtime_error;
 };
+// A number literal too large for a double (not a syntax error: callers that treat syntax errors specially let it
+// through, as the reference's JSON library does with its out_of_range error).
+class JsonRangeError : public std::runtime_error {
+public:
+    using std::runtime_error::runtime_error;
+};
 
 class Json {
 public:

@@ -35,6 +35,8 @@ std::string usage() {
            "  --graph-steps N    długość grafu HIP kroków wsadowych (domyślnie 16, 0 = bez grafów)\n"
            "  --max-jobs N       najwyżej N zadań naraz w potoku wsadowym (1 = po jednym; domyślnie bez limitu)\n"
            "  --timing-sample N  jak --timing-detail, ale czasy kerneli z co N-tego kroku CA i przemiatania MC\n"
+           "  --digests          zamiast plików wyników (ca_result.*, mc_result.*, migawki) zapisuje <plik>.sha256:\n"
+           "                     skrót SHA-256 i rozmiar tych samych bajtów (porównania przebiegów większych niż dysk)\n"
            "  --autotune N       przed przebiegiem CA: N kroków próbnych na każdy kształt kafla i tryb frontu,\n"
            "                     przebieg z najszybszym (wyniki bez zmian; wybór w times.json)\n"
            "  --progress         linie PROGRESS ca/mc/save/batch na stdout (dla GUI)\n"
@@ -100,6 +102,7 @@ CliOptions parse_cli(int argc, const char* const* argv) {
         else if (a == "--high-priority") opt.overrides.push_back("parallel.priority=high");
         else if (a == "--benchmark") opt.benchmark = true;
         else if (a == "--timing-detail") opt.timing_detail = true;
+        else if (a == "--digests") opt.digests = true;
         else if (a == "--timing-sample") {
             const std::string v = value(i, a);
             try {
@@ -148,6 +151,7 @@ int main(int argc, char** argv) {
             return 0;
         }
         if (opt.validate) return run_validate(opt, std::cout);
+        if (opt.digests) set_result_digests(true);
         if (!opt.batch_dir.empty()) return run_batch(opt);
         if (!opt.serve_path.empty()) return run_serve(opt);
         return run_from_cli(opt, since_start);

@@ -90,8 +90,9 @@ Json Timing::to_json() const {
 }
 
 void Progress::report(const char* stage, long long done, long long total, bool force, const char* key, uint64_t value) {
-    if (!on_ || (!force && !first_ && since_.ms() < 500.0)) return;
-    first_ = false;
+    // the clock runs from construction, like the reference's reporter: an unforced line needs 500 ms since the
+    // previous line (or since the start)
+    if (!on_ || (!force && since_.ms() < 500.0)) return;
     std::cout << "PROGRESS " << stage << ' ' << done << '/' << total;
     if (key) std::cout << ' ' << key << '=' << value;
     std::cout << '\n' << std::flush;
@@ -462,7 +463,8 @@ RunSummary run_simulation(const Config& cfg, const std::string& out_dir, Timing&
             fs::create_directories(snap_dir);
             write_snapshot(0);
         }
-        report(true);
+        // the step loop with snapshots reports only after steps (Simulation.cpp:226-240): no line for step 0
+        if (every <= 0) report(true);
         while (stop == VCA_RUNNING) {
             int chunk = progress.enabled() ? 64 : cfg.ca.max_steps;
             if (every > 0) chunk = std::min(chunk, every - steps % every);
@@ -998,7 +1000,8 @@ int run_validate(const CliOptions& opt, std::ostream& out) {
         for (const std::string& w : cfg.warnings) warnings.push_back(Json(w));
         if (!opt.mc_from.empty()) {
             if (cfg.mc.full)
-                errors.push_back(Json("--mc-from krystalizuje wynik CA z pliku, więc wymaga mc.domain = \"basalt\""));
+                errors.push_back(Json("--mc-from krystalizuje wynik CA z pliku, więc wymaga mc.domain = \"basalt\" "
+                                      "(podano \"full\")"));
             if (!cfg.mc.enabled)
                 warnings.push_back(Json("mc.enabled = false, a w trybie --mc-from Monte Carlo jest uruchamiane zawsze"));
             const fs::path dir(opt.mc_from);

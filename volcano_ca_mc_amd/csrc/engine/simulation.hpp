@@ -28,6 +28,7 @@ struct CliOptions {
     int steps_per_graph = 16;         // --graph-steps: hipGraph length of the batched CA steps (0 = direct)
     int max_jobs = 0;                 // --max-jobs: jobs admitted at once in the batched pipeline (0 = no limit)
     int autotune_steps = 0;           // --autotune: probe steps per launch configuration before the CA (0 = off)
+    bool digests = false;             // --digests: result files replaced by <file>.sha256 (digest and size)
     std::string serve_path;           // --serve <jobs.jsonl>
     int repeats = 3;
     std::string config_path, out_dir, mc_from, batch_dir;
@@ -77,7 +78,6 @@ public:
 private:
     bool on_;
     Clock since_;
-    bool first_ = true;
 };
 
 // RAII owner of a libvca context on device 0.

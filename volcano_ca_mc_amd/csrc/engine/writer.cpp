@@ -6,8 +6,13 @@
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
+#include <atomic>
+
+#include "sha256.hpp"
 
 namespace eng {
+
+std::atomic<bool> g_digests{false};
 
 namespace {
 
@@ -17,20 +22,50 @@ std::ofstream open_out(const std::string& path, bool binary) {
     return out;
 }
 
-template <class T>
-void put(std::ofstream& out, const std::vector<T>& v) {
-    out.write(reinterpret_cast<const char*>(v.data()), static_cast<std::streamsize>(v.size() * sizeof(T)));
-}
-
 void finish(std::ofstream& out, const std::string& path) {
     out.flush();
     if (!out) throw std::runtime_error("Błąd zapisu pliku: " + path);
 }
 
+// One result file: written to disk, or -- with result digests on -- only hashed, its SHA-256 and size going to
+// <path>.sha256 ("<hex> <bytes>\n") instead.  Either way it receives exactly the bytes of the file.
+class ResultOut {
+public:
+    ResultOut(const std::string& path, bool binary) : path_(path), digest_(g_digests.load()) {
+        if (!digest_) out_ = open_out(path, binary);
+    }
+    void write(const char* p, size_t n) {
+        if (digest_)
+            sha_.update(p, n);
+        else
+            out_.write(p, static_cast<std::streamsize>(n));
+    }
+    template <class T>
+    void put(const std::vector<T>& v) {
+        write(reinterpret_cast<const char*>(v.data()), v.size() * sizeof(T));
+    }
+    void close() {
+        if (!digest_) {
+            finish(out_, path_);
+            return;
+        }
+        std::ofstream d = open_out(path_ + ".sha256", false);
+        const uint64_t n = sha_.bytes();
+        d << sha_.hex() << ' ' << n << '\n';
+        finish(d, path_ + ".sha256");
+    }
+
+private:
+    std::string path_;
+    bool digest_;
+    std::ofstream out_;
+    Sha256 sha_;
+};
+
 // Buffered text writer: appends into a ~1 MB buffer and flushes it in one write.
 class Text {
 public:
-    explicit Text(std::ofstream& out) : out_(out) { buf_.reserve((1 << 20) + 512); }
+    explicit Text(ResultOut& out) : out_(out) { buf_.reserve((1 << 20) + 512); }
     ~Text() { flush(); }
     void add(const char* s, size_t n) {
         buf_.append(s, n);
@@ -38,35 +73,37 @@ public:
     }
     void add(const std::string& s) { add(s.data(), s.size()); }
     void flush() {
-        if (!buf_.empty()) out_.write(buf_.data(), static_cast<std::streamsize>(buf_.size()));
+        if (!buf_.empty()) out_.write(buf_.data(), buf_.size());
         buf_.clear();
     }
 
 private:
-    std::ofstream& out_;
+    ResultOut& out_;
     std::string buf_;
 };
 
 }  // namespace
 
+void set_result_digests(bool on) { g_digests.store(on); }
+
 void write_ca_bin(const std::string& path, const CaState& s) {
-    std::ofstream out = open_out(path, true);
+    ResultOut out(path, true);
     const char magic[4] = {'V', 'O', 'L', 'C'};
     const uint32_t version = 2;
     const int32_t dims[3] = {s.dims.nx, s.dims.ny, s.dims.nz};
     out.write(magic, 4);
     out.write(reinterpret_cast<const char*>(&version), 4);
     out.write(reinterpret_cast<const char*>(dims), 12);
-    put(out, s.type);
-    put(out, s.flow);
-    put(out, s.temp);
-    put(out, s.solid);
-    put(out, s.arrival);
-    finish(out, path);
+    out.put(s.type);
+    out.put(s.flow);
+    out.put(s.temp);
+    out.put(s.solid);
+    out.put(s.arrival);
+    out.close();
 }
 
 void write_ca_txt(const std::string& path, const CaState& s) {
-    std::ofstream out = open_out(path, true);
+    ResultOut out(path, true);
     {
         Text t(out);
         const Dims& d = s.dims;
@@ -83,21 +120,21 @@ void write_ca_txt(const std::string& path, const CaState& s) {
                     t.add(line, static_cast<size_t>(n));
                 }
     }
-    finish(out, path);
+    out.close();
 }
 
 void write_crystal_bin(const std::string& path, const Dims& d, const std::vector<int32_t>& crystal) {
-    std::ofstream out = open_out(path, true);
+    ResultOut out(path, true);
     const char magic[4] = {'V', 'M', 'C', '1'};
     const int32_t dims[3] = {d.nx, d.ny, d.nz};
     out.write(magic, 4);
     out.write(reinterpret_cast<const char*>(dims), 12);
-    put(out, crystal);
-    finish(out, path);
+    out.put(crystal);
+    out.close();
 }
 
 void write_crystal_txt(const std::string& path, const Dims& d, const std::vector<int32_t>& crystal) {
-    std::ofstream out = open_out(path, true);
+    ResultOut out(path, true);
     {
         Text t(out);
         char line[96];
@@ -111,7 +148,7 @@ void write_crystal_txt(const std::string& path, const Dims& d, const std::vector
                     t.add(line, static_cast<size_t>(n));
                 }
     }
-    finish(out, path);
+    out.close();
 }
 
 void write_mc_stats_csv(const std::string& path, const std::vector<McRow>& rows) {

@@ -40,6 +40,10 @@ struct McRow {
     double mean_grain_size = 0.0, energy = 0.0, accept_rate = 0.0;
 };
 
+// Result digests (engine --digests): the writers below record each result file's SHA-256 and size in <path>.sha256
+// instead of writing the file.  Process-wide; set before any job starts.
+void set_result_digests(bool on);
+
 void write_ca_bin(const std::string& path, const CaState& s);
 void write_ca_txt(const std::string& path, const CaState& s);
 void write_crystal_bin(const std::string& path, const Dims& d, const std::vector<int32_t>& crystal);

@@ -1161,9 +1161,12 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                 fl[k] = 0;
                 tp[k] = 0.0f;
                 ds[k] = -1;
-                if (r < H::R && ((T.S.lava[r] >> lx) & 1u)) {
+                // the staging wrote LAVA only where a cell exists; the load is guarded as well, so that no mask bit
+                // can address memory outside the grid (kNoCell row, gx == -1)
+                const uint32_t base = r < H::R ? row_of(row_base, r) : kNoCell;
+                if (base != kNoCell && gx >= 0 && ((T.S.lava[r] >> lx) & 1u)) {
                     lava |= 1u << k;
-                    const uint32_t g = row_of(row_base, r) + static_cast<uint32_t>(gx);
+                    const uint32_t g = base + static_cast<uint32_t>(gx);
                     fl[k] = D.flow[g];
                     if (!P.classic) {
                         tp[k] = D.temp[g];
@@ -1174,10 +1177,11 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             int32_t efl = 0;
             float etp = 0.0f;
             int32_t eds = -1;
-            const bool elava = t < 2 * H::R && T.S.edge[t];
             const uint32_t ebase = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(((t >> 1) & 63) << 2, static_cast<int>(row_base)));
+            const int ex = halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1);
+            const bool elava = t < 2 * H::R && ebase != kNoCell && ex >= 0 && T.S.edge[t];
             if (elava) {
-                const uint32_t g = ebase + static_cast<uint32_t>(halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1));
+                const uint32_t g = ebase + static_cast<uint32_t>(ex);
                 efl = D.flow[g];
                 if (!P.classic) {
                     etp = D.temp[g];
