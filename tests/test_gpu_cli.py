@@ -31,7 +31,7 @@ def _unzip(p):
         return gzip.decompress(f.read())
 
 
-def _check_outputs(expected, got):
+def _check_outputs(expected, got, ca_folder=None):
     for fn in sorted(os.listdir(expected)):
         p = os.path.join(expected, fn)
         if fn.endswith(".gz"):
@@ -41,7 +41,12 @@ def _check_outputs(expected, got):
         elif fn == "stats.json":
             s = json.loads((got / fn).read_text(encoding="utf-8"))
             s.pop("parallel", None)
-            assert s == json.load(open(p, encoding="utf-8")), fn
+            want = json.load(open(p, encoding="utf-8"))
+            if "source" in want:
+                # --mc-from records the absolute path of the CA folder it read: the run's own folder here
+                assert s["source"]["folder"] == os.path.abspath(ca_folder), s["source"]
+                s["source"]["folder"] = want["source"]["folder"]
+            assert s == want, fn
         elif fn in ("snapshots", "mc_snapshots"):
             want = sorted(os.listdir(p))
             assert sorted(x + ".gz" for x in os.listdir(got / fn)) == want, fn
@@ -88,7 +93,7 @@ def test_run_matches_reference(tmp_path, name):
     r = subprocess.run([ENGINE, "--config", "config.json", "--out", str(out), *args], cwd=work, capture_output=True,
                        text=True, timeout=300)
     assert r.returncode == 0, r.stderr
-    _check_outputs(os.path.join(d, "expected"), out)
+    _check_outputs(os.path.join(d, "expected"), out, str(ca))
     p = os.path.join(d, "expected", "progress.txt")
     if os.path.exists(p):
         ref = [ln for ln in open(p).read().splitlines() if ln]

@@ -53,6 +53,7 @@ struct Job {
     double latency_ms = 0.0;
     double done_s = 0.0;
     std::string error;
+    int exit_code = 0;    // what `engine --config` would have exited with (1 config, 2 arguments, 3 other)
 };
 
 uint64_t mc_footprint(const Config& cfg, const Dims& d) {
@@ -195,7 +196,7 @@ private:
                     mc_busy_[static_cast<size_t>(j.mc_stream)] = 0;
                     used_ -= j.fp_mc;
                 }
-                fail(j, e.what());
+                fail(j, e);
                 continue;
             }
             switch (j.stage) {
@@ -246,7 +247,22 @@ private:
         used_ -= j.fp_ca;
     }
 
+    // The job's error as a child `engine --config` reports it (its last "Błąd..." line and its exit code, which
+    // engine/src/app/Batch.cpp:100-114 copies into batch_summary.csv).
+    void fail(Job& j, const std::exception& e) {
+        if (dynamic_cast<const ConfigError*>(&e)) {
+            j.exit_code = 1;
+            fail(j, std::string("Błąd konfiguracji: ") + e.what());
+        } else if (dynamic_cast<const CliError*>(&e)) {
+            j.exit_code = 2;
+            fail(j, std::string("Błąd argumentów: ") + e.what());
+        } else {
+            j.exit_code = 3;
+            fail(j, std::string("Błąd: ") + e.what());
+        }
+    }
     void fail(Job& j, const std::string& what) {
+        if (j.exit_code == 0) j.exit_code = 3;
         j.error = what;
         if (j.ca) {
             if (j.slot >= 0) vca_batch_remove(batch_, j.ca);
@@ -294,7 +310,7 @@ private:
             try {
                 if (opt_.autotune_steps > 0) tuned(j);
             } catch (const std::exception& e) {
-                fail(j, e.what());
+                fail(j, e);
                 continue;
             }
             used_ += j.fp_ca;
@@ -369,7 +385,7 @@ private:
                     if (j.cfg.mc.enabled) j.prep = mc_prepare_from_ca(j.cfg, j.ca);
                 });
             } catch (const std::exception& e) {
-                fail(j, e.what());
+                fail(j, e);
             }
         }
         return true;
@@ -534,7 +550,7 @@ int run_batched(const CliOptions& opt) {
     for (const auto& j : jobs) {
         BatchOutcome o;
         o.item = BatchItem{j->name, j->config, j->out};
-        o.exit_code = j->stage == Stage::Done ? 0 : 3;
+        o.exit_code = j->stage == Stage::Done ? 0 : j->exit_code;
         o.error = j->error;
         o.wall_ms = j->latency_ms;
         ok += o.ok();
