@@ -283,8 +283,8 @@ struct vca_ca {
     CaDev D{};
     int shape = 0;
     DevBuf arena;  // every region below: one allocation and one free per simulation
-    Span type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, tile_near, tile_halo, tile_tchg, changes, tile_nchg,
-        tile_nwide, part, ctrl, hist_a, hist_c;
+    Span type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, tile_near, tile_halo, tile_tchg, listed, changes,
+        tile_nchg, tile_nwide, part, ctrl, hist_a, hist_c;
     uint64_t n = 0;
     std::unique_ptr<Batch> own;   // batch of one: the solo path (made on the first solo enqueue)
     bool flow16 = true;           // every flowId the simulation can hold (its vents') fits int16_t
@@ -370,7 +370,7 @@ struct Batch {
     vca_ctx* ctx = nullptr;
     int max_slots = 0;
     uint32_t cap = 0;                             // tiles
-    DevBuf owner, tile_base, active, lava, wide, ctrl, status;
+    DevBuf owner, tile_base, active, wide, ctrl, status;
     uint32_t sim_base = 0;
     bool has_table = false;
     std::vector<CaSim> host_sims;                 // what the table entries hold (the copies read from here)
@@ -383,6 +383,10 @@ struct Batch {
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     int capture_failures = 0;
+    int par = 0;                  // parity of the next step (the list copy it reads)
+    bool fin_prev = false;        // the last step's finish is still to be done (never true between enqueues)
+    SlotMask purge{};             // slots whose simulation left while the pending list copy may hold its entries
+    bool purge_any = false;
 
     void init(vca_ctx* c, int slots, uint32_t tiles) {
         ctx = c;
@@ -396,13 +400,14 @@ struct Batch {
         check(hipGetLastError(), "simulation table");
         owner.alloc(static_cast<size_t>(cap) * 4);
         tile_base.alloc(static_cast<size_t>(slots) * 4);
-        active.alloc(static_cast<size_t>(cap) * kShapes * 8);
-        lava.alloc(static_cast<size_t>(cap) * kShapes * 4);
+        active.alloc(static_cast<size_t>(cap) * kShapes * 2 * 8);
         wide.alloc(static_cast<size_t>(cap) * 4);
         ctrl.alloc(sizeof(BatchCtrl));
         check(hipMemsetAsync(owner.p, 0xFF, owner.bytes, s), "memset");  // kNoSlot
         check(hipMemsetAsync(ctrl.p, 0, sizeof(BatchCtrl), s), "memset");
         members.assign(static_cast<size_t>(slots), nullptr);
+        par = 0;
+        fin_prev = false;
         free_ranges.clear();
         hwm = 0;
         B.sim_base = sim_base;
@@ -412,7 +417,6 @@ struct Batch {
         B.nslots = 0;
         B.cap = cap;
         B.active = active.as<uint64_t>();
-        B.lava = lava.as<uint32_t>();
         B.wide = wide.as<uint32_t>();
         B.ctrl = ctrl.as<BatchCtrl>();
         grid_tiles = static_cast<int>(std::min<uint32_t>(cap, 8192u));
@@ -445,6 +449,12 @@ struct Batch {
     bool any_lava_mode() const {
         for (const vca_ca* c : members)
             if (c && !c->P.classic) return true;
+        return false;
+    }
+    // some member needs the scan of ca_front (classic mode, or every tile visited)
+    bool any_scan() const {
+        for (const vca_ca* c : members)
+            if (c && (c->P.classic || c->P.dense)) return true;
         return false;
     }
     bool flow16() const {
@@ -513,9 +523,15 @@ struct Batch {
         sim.D = ca->D;
         sim.running = 1;
         sim.shape = ca->shape;
+        sim.push = !ca->P.classic && !ca->P.dense;
+        // entries of simulations that left go before the new one is listed (the pending copy never holds more
+        // entries than the batch has tiles)
+        flush_purge();
         ca_write_sim(sim_base + static_cast<uint32_t>(slot), &sim, s);
         check(hipMemcpyAsync(tile_base.as<uint32_t>() + slot, &base, 4, hipMemcpyHostToDevice, s), "batch add");
         ca_launch_owner(owner.as<uint32_t>(), base, ca->P.ntiles, static_cast<uint32_t>(slot), s);
+        // a lava-front simulation's first tiles (later steps list their own)
+        if (sim.push) ca_launch_seed(B, static_cast<uint32_t>(slot), ca->P.ntiles, par, s);
         check(hipGetLastError(), "batch add");
         members[static_cast<size_t>(slot)] = ca;
         bases.resize(static_cast<size_t>(max_slots));
@@ -532,49 +548,99 @@ struct Batch {
         check(hipGetLastError(), "batch remove");
         give_range(base, ca->P.ntiles);
         members[static_cast<size_t>(slot)] = nullptr;
+        // the pending list copy may hold entries the simulation pushed for a step that has not run
+        purge.bits[static_cast<uint32_t>(slot) >> 5] |= 1u << (static_cast<uint32_t>(slot) & 31u);
+        purge_any = true;
         refresh();
     }
     std::vector<uint32_t> bases;
 
+    void flush_purge() {
+        if (!purge_any) return;
+        ca_launch_purge(B, purge, par, ctx->stream);
+        check(hipGetLastError(), "batch purge");
+        purge = SlotMask{};
+        purge_any = false;
+    }
+    StepLaunch step_config() const {
+        StepLaunch L{};
+        L.shapes = shapes();
+        L.lava_mode = any_lava_mode();
+        L.flow16 = flow16();
+        L.scan = any_scan();
+        L.par = par;
+        L.fin_prev = fin_prev;
+        L.grid_tiles = grid_tiles;
+        return L;
+    }
+    // One step on stream s (no timing); advances the parity and the pending finish.
+    void launch_step(hipStream_t s, hipEvent_t* ev, bool* rec) {
+        const StepLaunch L = step_config();
+        ca_launch_step(B, L, s, ev, rec);
+        par ^= 1;
+        fin_prev = !L.scan;  // with the scan, ca_finish ran after the step
+    }
+    // The pending finish of the last step, if any (every enqueue ends with it: the host reads the stop flags next).
+    void close_steps(hipStream_t s, hipEvent_t* ev) {
+        if (fin_prev) ca_launch_finish(B, s, ev);
+        fin_prev = false;
+    }
     void launch_direct(int steps) {
         vca_ctx* c = ctx;
-        const uint32_t sh = shapes();
-        const bool lm = any_lava_mode(), f16 = flow16();
+        static const char* kNames[kCaKernels] = {"ca_front", "ca_descent", "ca_rules", "ca_wide", "ca_apply", "ca_finish"};
         for (int k = 0; k < steps; ++k) {
             if (!c->time_ca_step()) {
-                ca_launch_step(B, sh, lm, f16, grid_tiles, c->stream, nullptr);
+                launch_step(c->stream, nullptr, nullptr);
                 continue;
             }
-            hipEvent_t ev[7];
-            size_t idx[7];
-            for (int e = 0; e < 7; ++e) {
+            hipEvent_t ev[2 * kCaKernels];
+            size_t idx[2 * kCaKernels];
+            for (int e = 0; e < 2 * kCaKernels; ++e) {
                 ev[e] = c->take();
                 idx[e] = c->used - 1;
             }
-            ca_launch_step(B, sh, lm, f16, grid_tiles, c->stream, ev);
-            c->mark("ca_front", idx[0], idx[1]);
-            if (lm) c->mark("ca_descent", idx[1], idx[2]);
-            c->mark("ca_rules", idx[2], idx[3]);
-            c->mark("ca_wide", idx[3], idx[4]);
-            c->mark("ca_apply", idx[4], idx[5]);
-            c->mark("ca_finish", idx[5], idx[6]);
-            c->mark("ca_step", idx[0], idx[6]);
-            if (c->used > 7 * 256) c->drain();  // bound the pool on long enqueues
+            bool rec[kCaKernels] = {};
+            launch_step(c->stream, ev, rec);
+            int first = -1, last = -1;
+            for (int q = 0; q < kCaKernels; ++q) {
+                if (!rec[q]) continue;
+                c->mark(kNames[q], idx[2 * q], idx[2 * q + 1]);
+                if (first < 0) first = q;
+                last = q;
+            }
+            if (first >= 0) c->mark("ca_step", idx[2 * first], idx[2 * last + 1]);
+            if (c->used > 2 * kCaKernels * 256) c->drain();  // bound the pool on long enqueues
+        }
+        if (fin_prev && c->timing) {
+            hipEvent_t ev[2] = {c->take(), c->take()};
+            const size_t a = c->used - 2;
+            close_steps(c->stream, ev);
+            c->mark("ca_finish", a, a + 1);
+        } else {
+            close_steps(c->stream, nullptr);
         }
         check(hipGetLastError(), "ca launch");
     }
-    // Captures steps_per_graph steps of the current membership (no memset or host work inside).
+    // Captures steps_per_graph steps (an even count: the graph starts and ends on parity 0) and the closing finish
+    // of the current membership (no memset or host work inside).
     bool capture() {
         hipStream_t s = ctx->stream;
         if (s == nullptr) return false;  // the legacy stream cannot be captured
         hipGraph_t g = nullptr;
+        const int par0 = par;
+        const bool fin0 = fin_prev;
+        par = 0;
+        fin_prev = false;
         if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) {
             (void)hipGetLastError();
+            par = par0;
+            fin_prev = fin0;
             return false;
         }
-        const uint32_t sh = shapes();
-        const bool lm = any_lava_mode(), f16 = flow16();
-        for (int k = 0; k < steps_per_graph; ++k) ca_launch_step(B, sh, lm, f16, grid_tiles, s, nullptr);
+        for (int k = 0; k < steps_per_graph; ++k) launch_step(s, nullptr, nullptr);
+        close_steps(s, nullptr);
+        par = par0;
+        fin_prev = fin0;
         // a capture invalidated by the runtime leaves nothing enqueued: the caller launches the steps directly
         if (hipStreamEndCapture(s, &g) != hipSuccess || g == nullptr) {
             (void)hipGetLastError();
@@ -594,10 +660,16 @@ struct Batch {
     }
     void enqueue(int steps) {
         if (steps <= 0 || count() == 0) return;
+        flush_purge();
         if (steps_per_graph > 0 && !ctx->timing) {
             if (!graph && !capture()) ++capture_failures;
             if (capture_failures >= 3) steps_per_graph = 0;  // this process cannot capture: direct launches
-            while (graph && steps >= graph_steps) {
+            // a graph starts on parity 0
+            if (graph && steps >= graph_steps + 1 && par != 0) {
+                launch_direct(1);
+                --steps;
+            }
+            while (graph && par == 0 && steps >= graph_steps) {
                 check(hipGraphLaunch(graph, ctx->stream), "hipGraphLaunch");
                 steps -= graph_steps;
             }
@@ -729,8 +801,8 @@ namespace {
 struct CaPlan {
     Layout L;
     size_t hcap = 1;
-    size_t type = 0, flow = 0, temp = 0, solid = 0, arrival = 0, ctrl = 0, tile_near = 0, tile_halo = 0, tile_tchg = 0, part = 0,
-           hist_a = 0, hist_c = 0;
+    size_t type = 0, flow = 0, temp = 0, solid = 0, arrival = 0, ctrl = 0, tile_near = 0, tile_halo = 0, tile_tchg = 0, listed = 0,
+           part = 0, hist_a = 0, hist_c = 0;
     size_t zero_bytes = 0;
     size_t descent = 0, vent_cell = 0, tile_flags = 0, changes = 0, tile_nchg = 0, tile_nwide = 0, spill = 0;
     size_t staging = 0, st_height = 0, st_cells = 0, st_flows = 0, st_temps = 0, st_bytes = 0;
@@ -749,6 +821,7 @@ CaPlan ca_plan(uint64_t n, uint32_t ntiles, size_t cols, bool lava, size_t vent_
     c.tile_near = L.add(ntiles);
     c.tile_halo = L.add(static_cast<size_t>(ntiles) * kHaloBytes);
     c.tile_tchg = L.add(static_cast<size_t>(ntiles) * 4);
+    c.listed = L.add(static_cast<size_t>(ntiles) * 4);
     c.part = L.add(static_cast<size_t>(kPartRows) * sizeof(CaPart));
     c.hist_a = L.add(c.hcap * 8);
     c.hist_c = L.add(c.hcap * 8);
@@ -922,6 +995,7 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         region(ca->tile_near, plan.tile_near, P.ntiles);
         region(ca->tile_halo, plan.tile_halo, static_cast<size_t>(P.ntiles) * kHaloBytes);
         region(ca->tile_tchg, plan.tile_tchg, static_cast<size_t>(P.ntiles) * 4);
+        region(ca->listed, plan.listed, static_cast<size_t>(P.ntiles) * 4);
         region(ca->part, plan.part, static_cast<size_t>(kPartRows) * sizeof(CaPart));
         region(ca->hist_a, plan.hist_a, plan.hcap * 8);
         region(ca->hist_c, plan.hist_c, plan.hcap * 8);
@@ -950,6 +1024,7 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         D.tile_near = ca->tile_near.as<uint8_t>();
         D.tile_halo = ca->tile_halo.as<uint8_t>();
         D.tile_tchg = ca->tile_tchg.as<uint32_t>();
+        D.listed = ca->listed.as<uint32_t>();
         D.changes = ca->changes.as<Change>();
         D.tile_nchg = ca->tile_nchg.as<uint32_t>();
         D.tile_nwide = ca->tile_nwide.as<uint32_t>();
@@ -1286,7 +1361,7 @@ extern "C" int vca_ca_footprint(const vca_ca_desc* d, uint64_t* bytes) {
         // the simulation's allocation (vents: a few slots), and the batch-of-one lists a solo run adds
         const CaPlan plan = ca_plan(n, static_cast<uint32_t>(std::min<uint64_t>(tiles, kTileMask)), cols, d->classic == 0,
                                     64, 64, d->max_steps);
-        *bytes = plan.L.total + tiles * (kShapes * 12 + 8) + 4096;
+        *bytes = plan.L.total + tiles * (2 * kShapes * 8 + 8) + 4096;
     });
 }
 
@@ -1440,7 +1515,7 @@ extern "C" int vca_batch_set_graph(vca_batch* b, int32_t steps_per_graph) {
         require(b && steps_per_graph >= 0 && steps_per_graph <= 4096, "invalid argument");
         b->b.ctx->set_device();
         b->b.drop_graph();
-        b->b.steps_per_graph = steps_per_graph;
+        b->b.steps_per_graph = steps_per_graph + (steps_per_graph & 1);  // graphs start and end on parity 0
     });
 }
 

@@ -21,7 +21,10 @@
 // thresholds (no exp on the device).
 //
 // Temperatures use only IEEE float subtract / compare (built with -ffp-contract=off).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "vca_internal.h"
 
@@ -232,6 +235,32 @@ __device__ inline void mark_halo(const CaParams& P, const CaDev& D, int tx, int 
     const uint32_t n = static_cast<uint32_t>(qx) + static_cast<uint32_t>(P.ntx) *
                                                        (static_cast<uint32_t>(qy) + static_cast<uint32_t>(P.nty) * static_cast<uint32_t>(qz));
     D.tile_halo[static_cast<size_t>(n) * kHaloBytes + static_cast<uint32_t>(26 - q)] = 1;
+}
+// Lane q < 27 of wave 0 (all 64 lanes call): when bit q of dirs is set, lists tile q of the 3 x 3 x 3 block around tile
+// (tx, ty, tz) (q = 13: the tile itself) for the simulation's step `step` in list copy par -- once per step, by the
+// tile's stamp (listed): the first lane to stamp a tile appends it.  Periodic axes wrap, tiles past an absorbing edge
+// do not exist.  One list atomic per call.
+__device__ inline void push_next(const CaParams& P, const CaDev& D, const BatchDev& B, int tx, int ty, int tz, int q,
+                                 uint32_t dirs, uint32_t slot, uint32_t step, int par, int shape) {
+    bool add = false;
+    uint32_t n = 0;
+    if (q < 27 && ((dirs >> q) & 1u)) {
+        int qx = tx + q % 3 - 1, qy = ty + (q / 3) % 3 - 1, qz = tz + q / 9 - 1;
+        if (wrap_axis(qx, P.ntx, P.per[0]) && wrap_axis(qy, P.nty, P.per[1]) && wrap_axis(qz, P.ntz, P.per[2])) {
+            n = static_cast<uint32_t>(qx) +
+                static_cast<uint32_t>(P.ntx) * (static_cast<uint32_t>(qy) + static_cast<uint32_t>(P.nty) * static_cast<uint32_t>(qz));
+            add = atomicExch(&D.listed[n], step) != step;
+        }
+    }
+    const unsigned long long m = __ballot(add);
+    if (m == 0) return;
+    const int lane = static_cast<int>(opaque_lane());
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&B.ctrl->nactive[par][shape], static_cast<uint32_t>(__popcll(m)));
+    base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(base), 0));
+    if (add)
+        B.active[(par * kShapes + shape) * static_cast<size_t>(B.cap) + base + mbcnt64(m)] =
+            (static_cast<uint64_t>(step) << 32) | (slot << kTileBits) | n;
 }
 __device__ inline unsigned wave_or(unsigned v) {
     v |= dpp_row_shr<0x111>(v);
@@ -565,122 +594,164 @@ __global__ void __launch_bounds__(kBlock) ca_init_flags_kernel(CaParams P, CaDev
 // CA step kernels
 // ------------------------------------------------------------------------------------------------
 
-// tiles per thread of ca_front: one -- the scan is latency-bound, so more threads beat fewer list atomics (a
-// 512 x 512 x 256 grid of 65536 tiles: 8 per thread 19.5 us, 4 12.7, 2 9.3, 1 8.0 per step)
-constexpr int kFrontPer = 1;
-
-// Visited-tile lists of one step of every running simulation of the batch (kFrontPer global tiles per thread).  A tile
-// is visited when it holds lava (lava cools) or holds AIR next to a tile with lava: the reference's "may
-// change" set (CaRules.hpp:69-79, LavaCA.hpp:844-871) at tile granularity.  In lava mode the tiles holding
-// lava also go to a second list: the cells whose D(n) is needed.  One list per tile shape; a block counts its
-// entries by wave ballots and reserves them with one atomic per list.
-__global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B, uint32_t shapes) {
-    // lists 2*shape (visited) and 2*shape + 1 (lava); per wave entry counts, then each block's base from one atomic
-    // per list (a device-wide counter is the contended resource: one atomic per block, not per wave)
-    __shared__ uint32_t s_cnt[2 * kShapes][kBlock / 64];
-    __shared__ uint32_t s_base[2 * kShapes];
+// Staged-tile lists of one step for the simulations that need a scan (classic mode, or ca.sweep = full), one lane per
+// global tile.  A tile is visited when it holds lava (lava cools) or holds AIR next to a tile with lava: the
+// reference's "may change" set (CaRules.hpp:69-79, LavaCA.hpp:844-871) at tile granularity, every tile with
+// ca.sweep = full; a visited tile is staged when it or its 1-cell halo holds lava, the others settle here.  Lava-front
+// simulations are skipped: their rules kernel pushed this step's tiles already.  One list per tile shape (copy of
+// parity par); a block counts its entries by wave ballots and reserves them with one atomic per list (a device-wide
+// counter is the contended resource: one atomic per block, not per wave).  The scan is latency-bound, so one tile per
+// lane (a 512 x 512 x 256 grid of 65536 tiles: 8 per lane 19.5 us, 4 12.7, 2 9.3, 1 8.0 per step).
+__global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B, uint32_t shapes, int par) {
+    __shared__ uint32_t s_cnt[kShapes][kBlock / 64];
+    __shared__ uint32_t s_base[kShapes];
     const int w = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63u);
     const unsigned long long below = (1ULL << lane) - 1ULL;
-    uint64_t ent[kFrontPer];
-    uint32_t code[kFrontPer];  // bit 0 visited, bit 1 lava, bits 2.. shape
-#pragma unroll
-    for (int k = 0; k < kFrontPer; ++k) {
-        const uint32_t g = (blockIdx.x * kFrontPer + k) * kBlock + threadIdx.x;
-        const uint32_t slot = g < B.ntiles ? B.owner[g] : kNoSlot;
-        // a wave's 64 tiles nearly always belong to one simulation: then its parameters are read once, as scalars
-        const uint32_t s0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(slot)));
-        const bool uniform = __ballot(slot != s0) == 0;
-        ent[k] = 0;
-        code[k] = 0;
-        auto visit = [&](const CaSim& sim, uint32_t sl) -> bool {
-            if (!sim.running || sim.D.ctrl->stop) return false;
-            const CaParams& P = sim.P;
-            const uint32_t t = g - B.tile_base[sl];
-            const uint8_t f = sim.D.tile_flags[t];
-            const bool lava = !P.classic && (f & kFlagLava);
-            // a neighbour tile with lava marked this tile when its own step ended (mark_near); lava tiles are
-            // visited every step in lava mode, so their marks are renewed and this step's are cleared here
-            uint8_t* near = sim.D.tile_near;
-            const uint8_t nb = near[t];
-            if (nb && !P.classic) near[t] = 0;
-            const bool act =
-                P.dense || lava || (P.classic && (f & kFlagLava) && (f & kFlagAir)) || ((f & kFlagAir) && nb);
-            // a visited tile goes to the rules only when it or its 1-cell halo holds lava: its own flag, or a
-            // neighbour's halo byte (mark_halo); the bytes are cleared for the next step in lava mode.  Any other
-            // visited tile is settled here: none of its cells can change, its flags stay as they are.
-            bool stage = (f & kFlagLava) != 0;
-            if (act) {
-                uint4* hb = reinterpret_cast<uint4*>(sim.D.tile_halo + static_cast<size_t>(t) * kHaloBytes);
-                bool any = stage;  // a tile with lava is staged anyway: its bytes are only cleared
-                if (!stage) {
-                    const uint4 h0 = hb[0], h1 = hb[1];
-                    any = (h0.x | h0.y | h0.z | h0.w | h1.x | h1.y | h1.z | h1.w) != 0;
-                    stage = any;
-                }
-                if (any && !P.classic) {
-                    hb[0] = make_uint4(0, 0, 0, 0);
-                    hb[1] = make_uint4(0, 0, 0, 0);
-                }
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t slot = g < B.ntiles ? B.owner[g] : kNoSlot;
+    // a wave's 64 tiles nearly always belong to one simulation: then its parameters are read once, as scalars
+    const uint32_t s0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(slot)));
+    const bool uniform = __ballot(slot != s0) == 0;
+    uint64_t ent = 0;
+    int code = -1;  // shape of the list the tile goes to, -1: none
+    auto visit = [&](const CaSim& sim, uint32_t sl) -> bool {
+        if (!sim.running || sim.push || sim.D.ctrl->stop) return false;
+        const CaParams& P = sim.P;
+        const uint32_t t = g - B.tile_base[sl];
+        const uint8_t f = sim.D.tile_flags[t];
+        const bool lava = !P.classic && (f & kFlagLava);
+        // a neighbour tile with lava marked this tile when its own step ended (mark_near); lava tiles are visited
+        // every step in lava mode, so their marks are renewed and this step's are cleared here
+        uint8_t* near = sim.D.tile_near;
+        const uint8_t nb = near[t];
+        if (nb && !P.classic) near[t] = 0;
+        const bool act = P.dense || lava || (P.classic && (f & kFlagLava) && (f & kFlagAir)) || ((f & kFlagAir) && nb);
+        // a visited tile goes to the rules only when it or its 1-cell halo holds lava: its own flag, or a
+        // neighbour's halo byte (mark_halo); the bytes are cleared for the next step in lava mode.  Any other
+        // visited tile is settled here: none of its cells can change, its flags stay as they are.
+        bool stage = (f & kFlagLava) != 0;
+        if (act) {
+            uint4* hb = reinterpret_cast<uint4*>(sim.D.tile_halo + static_cast<size_t>(t) * kHaloBytes);
+            bool any = stage;  // a tile with lava is staged anyway: its bytes are only cleared
+            if (!stage) {
+                const uint4 h0 = hb[0], h1 = hb[1];
+                any = (h0.x | h0.y | h0.z | h0.w | h1.x | h1.y | h1.z | h1.w) != 0;
+                stage = any;
             }
-            ent[k] = (static_cast<uint64_t>(static_cast<uint32_t>(sim.D.ctrl->steps)) << 32) | (sl << kTileBits) | t;
-            code[k] = (act && stage ? 1u : 0u) | (lava ? 2u : 0u) | (static_cast<uint32_t>(sim.shape) << 2);
-            return act;
-        };
-        // visited tiles are counted here (staged or settled): one atomic per wave when the wave is one simulation's
-        if (uniform) {
-            if (s0 != kNoSlot) {
-                const CaSim& sim = g_sims[B.sim_base + s0];
-                const unsigned long long m = __ballot(visit(sim, s0));
-                if (m && lane == 0)
-                    atomicAdd(&sim.D.part[(g >> 6) % kPartRows].tiles, static_cast<unsigned long long>(__popcll(m)));
+            if (any && !P.classic) {
+                hb[0] = make_uint4(0, 0, 0, 0);
+                hb[1] = make_uint4(0, 0, 0, 0);
             }
-        } else if (slot != kNoSlot) {
-            const CaSim& sim = g_sims[B.sim_base + slot];
-            if (visit(sim, slot)) atomicAdd(&sim.D.part[(g - B.tile_base[slot]) % kPartRows].tiles, 1ULL);
         }
+        ent = (static_cast<uint64_t>(static_cast<uint32_t>(sim.D.ctrl->steps)) << 32) | (sl << kTileBits) | t;
+        code = act && stage ? sim.shape : -1;
+        return act;
+    };
+    // visited tiles are counted here (staged or settled): one atomic per wave when the wave is one simulation's
+    if (uniform) {
+        if (s0 != kNoSlot) {
+            const CaSim& sim = g_sims[B.sim_base + s0];
+            const unsigned long long m = __ballot(visit(sim, s0));
+            if (m && lane == 0)
+                atomicAdd(&sim.D.part[(g >> 6) % kPartRows].tiles, static_cast<unsigned long long>(__popcll(m)));
+        }
+    } else if (slot != kNoSlot) {
+        const CaSim& sim = g_sims[B.sim_base + slot];
+        if (visit(sim, slot)) atomicAdd(&sim.D.part[(g - B.tile_base[slot]) % kPartRows].tiles, 1ULL);
     }
 #pragma unroll
-    for (int l = 0; l < 2 * kShapes; ++l) {
-        if (!((shapes >> (l >> 1)) & 1u)) continue;
-        const uint32_t want = (static_cast<uint32_t>(l >> 1) << 2) | ((l & 1) ? 2u : 1u);
-        uint32_t n = 0;
-#pragma unroll
-        for (int k = 0; k < kFrontPer; ++k) {
-            const bool m = (code[k] >> 2) == (want >> 2) && (code[k] & want & 3u);
-            n += static_cast<uint32_t>(__popcll(__ballot(m)));
-        }
+    for (int l = 0; l < kShapes; ++l) {
+        if (!((shapes >> l) & 1u)) continue;
+        const uint32_t n = static_cast<uint32_t>(__popcll(__ballot(code == l)));
         if (lane == 0) s_cnt[l][w] = n;
     }
     __syncthreads();
-    if (threadIdx.x < 2 * kShapes) {
+    if (threadIdx.x < kShapes) {
         const int l = static_cast<int>(threadIdx.x);
         uint32_t tot = 0;
-        if ((shapes >> (l >> 1)) & 1u)
+        if ((shapes >> l) & 1u)
             for (int q = 0; q < kBlock / 64; ++q) tot += s_cnt[l][q];
-        uint32_t* counter = (l & 1) ? &B.ctrl->nlava[l >> 1] : &B.ctrl->nactive[l >> 1];
-        s_base[l] = tot ? atomicAdd(counter, tot) : 0u;
+        s_base[l] = tot ? atomicAdd(&B.ctrl->nactive[par][l], tot) : 0u;
     }
     __syncthreads();
 #pragma unroll
-    for (int l = 0; l < 2 * kShapes; ++l) {
-        if (!((shapes >> (l >> 1)) & 1u)) continue;
-        const uint32_t want = (static_cast<uint32_t>(l >> 1) << 2) | ((l & 1) ? 2u : 1u);
+    for (int l = 0; l < kShapes; ++l) {
+        if (!((shapes >> l) & 1u)) continue;
         uint32_t pos = s_base[l];
         for (int q = 0; q < w; ++q) pos += s_cnt[l][q];
-#pragma unroll
-        for (int k = 0; k < kFrontPer; ++k) {
-            const bool m = (code[k] >> 2) == (want >> 2) && (code[k] & want & 3u);
-            const unsigned long long mb = __ballot(m);
-            if (m) {
-                const uint32_t at = pos + static_cast<uint32_t>(__popcll(mb & below));
-                if (l & 1)
-                    B.lava[(l >> 1) * B.cap + at] = static_cast<uint32_t>(ent[k]);
-                else
-                    B.active[(l >> 1) * B.cap + at] = ent[k];
-            }
-            pos += static_cast<uint32_t>(__popcll(mb));
-        }
+        const unsigned long long mb = __ballot(code == l);
+        if (code == l) B.active[(par * kShapes + l) * static_cast<size_t>(B.cap) + pos + __popcll(mb & below)] = ent;
     }
+}
+
+// finishStep (LavaCA.hpp:1206-1229) + frontCounts (LavaCA.hpp:865-871) of the simulation in slot `slot`: sums and
+// clears its counter rows, writes the histories, advances the step count and applies the stop rule.  Every thread
+// of the block calls (block-uniform slot).
+__device__ void finish_slot(const BatchDev& B, uint32_t slot) {
+    __shared__ unsigned long long red[5][kBlock / 64];
+    const CaSim& sim = g_sims[B.sim_base + slot];
+    if (!sim.running) return;
+    const CaParams& P = sim.P;
+    const CaDev& D = sim.D;
+    CaCtrl* c = D.ctrl;
+    if (__builtin_amdgcn_readfirstlane(c->stop)) return;
+    unsigned long long la = 0, bo = 0, ch = 0, tl = 0, st = 0;
+    for (int r = static_cast<int>(threadIdx.x); r < kPartRows; r += kBlock) {
+        CaPart* row = &D.part[r];
+        la += row->lava_after;
+        bo += row->born;
+        ch += row->changed;
+        tl += row->tiles;
+        st += row->staged;
+        *row = CaPart{0, 0, 0, 0, 0};  // cleared for the next step
+    }
+    la = wave_sum(la);
+    bo = wave_sum(bo);
+    ch = wave_sum(ch);
+    tl = wave_sum(tl);
+    st = wave_sum(st);
+    const int w = static_cast<int>(threadIdx.x >> 6);
+    if ((threadIdx.x & 63u) == 0) {
+        red[0][w] = la;
+        red[1][w] = bo;
+        red[2][w] = ch;
+        red[3][w] = tl;
+        red[4][w] = st;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long lava_after = 0, born = 0, changed = 0, tiles = 0, staged = 0;
+        for (int k = 0; k < kBlock / 64; ++k) {
+            lava_after += red[0][k];
+            born += red[1][k];
+            changed += red[2][k];
+            tiles += red[3][k];
+            staged += red[4][k];
+        }
+        const int32_t t = c->steps;
+        const unsigned long long active = P.classic ? c->lava_total + born : lava_after;
+        const unsigned long long air = c->air_total - born;
+        D.hist_active[t] = active;
+        D.hist_changed[t] = changed;
+        c->steps = t + 1;
+        c->lava_total = active;
+        c->air_total = air;
+        int32_t stop = 0;
+        if (!P.classic && active == 0)
+            stop = 1;  // NoLava
+        else if (P.classic && air == 0)
+            stop = 2;  // NoAir
+        else if (changed == 0 && P.nb.variants == 1)
+            stop = 3;  // NoChange
+        else if (t + 1 >= P.max_steps)
+            stop = 4;  // MaxSteps
+        c->stop = stop;
+        // lava-front simulations stage exactly the tiles they visit (no scan)
+        c->tiles_visited += sim.push ? staged : tiles;
+        c->tiles_staged += staged;
+        c->changes_total += changed;  // every change is one record
+    }
+    __syncthreads();  // red is reused by the caller's next block-wide step
 }
 
 // A tile of 64 x TY x TZ cells and its 1-cell halo, as staged in LDS by the descent and rules kernels:
@@ -861,35 +932,52 @@ __device__ int drop_depth_lds(const CaDev& D, const uint8_t* s_type, int c, int 
 // D(n) of every lava cell of the tiles holding lava (LavaCA.hpp:421-429, 877-888), from the LDS-staged tile:
 // hasSupport of the cell, then the deepest dropDepth over its AIR side neighbours (union of the variants).
 template <int TY, int TZ>
-__global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shape) {
+// The first kernel of a step: it clears the counts of the list copy the previous step read (and the wide-tile count),
+// does the previous step's finish when fin != 0 (blocks 0 .. nslots-1, one slot each), then computes D(n) of the
+// staged tiles of parity par that hold lava.
+__global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shape, int par, int fin) {
     using H = Halo<TY, TZ>;
     __shared__ HaloTypes<TY, TZ> S;
     __shared__ int32_t s_spill[H::LXY];
     __shared__ uint16_t s_list[kTileCells];
     __shared__ uint32_t s_n;
     const uint8_t* s_type = S.type;
-    const uint32_t n = B.ctrl->nlava[shape];
-    const uint32_t* tiles = B.lava + shape * B.cap;
+    if (blockIdx.x == 0 && threadIdx.x < kShapes) {
+        // every kernel of the previous step is done with its list copy; the rules kernel of this step pushes the next
+        // step's tiles there
+        B.ctrl->nactive[par ^ 1][threadIdx.x] = 0u;
+        if (threadIdx.x == 0) B.ctrl->nwide = 0u;
+    }
+    if (fin && blockIdx.x < B.nslots) finish_slot(B, blockIdx.x);
+    const uint32_t n = B.ctrl->nactive[par][shape];
+    const uint64_t* tiles = B.active + (par * kShapes + shape) * static_cast<size_t>(B.cap);
     for (uint32_t s = blockIdx.x; s < n; s += gridDim.x) {
-        const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[s]);  // uniform: one entry per block
+        // uniform: one entry per block; low word slot << kTileBits | tile, high word the simulation's step
+        const uint64_t ent = tiles[s];
+        const uint32_t e = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ent));
+        const uint32_t now = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ent >> 32));
         const CaSim& sim = g_sims[B.sim_base + (e >> kTileBits)];
         const CaParams& P = sim.P;
         const CaDev& D = sim.D;
         const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
         const bool has_spill = D.spill != nullptr;
         const uint32_t side = P.nb.side_mask;
-        const TileCoord o = tile_origin(P, e & kTileMask);
+        const uint32_t tile = e & kTileMask;
+        const TileCoord o = tile_origin(P, tile);
         {
             // D(n) of the tile's lava cells reads the types of the tile, its 8 horizontal neighbours and the tiles
             // under them down to kMaxDrop + 1 cells below the tile's lowest layer (support, side cells, their drop
             // chains; spill levels do not change).  When none of those tiles changed a cell's type in the previous
-            // step, the values written then still hold.  (The barrier also orders the previous tile's LDS reads.)
+            // step, the values written then still hold.  Only tiles holding lava have D(n) to compute, and entries of
+            // a slot that left the batch or of a simulation that stopped are skipped.  (The barrier also orders the
+            // previous tile's LDS reads: every tile passes it.)
             constexpr int NB = (kMaxDrop + 1 + TZ - 1) / TZ;  // tile layers below
             constexpr int NR = 9 * (NB + 1);
             const int tx = o.x0 / kTileX, ty = o.y0 / TY, tz = o.z0 / TZ;
-            const uint32_t now = static_cast<uint32_t>(D.ctrl->steps);
+            const bool go = sim.running && !P.classic && !__builtin_amdgcn_readfirstlane(D.ctrl->stop) &&
+                            (__builtin_amdgcn_readfirstlane(D.tile_flags[tile]) & kFlagLava);
             bool dirty = false;
-            for (int q = static_cast<int>(threadIdx.x); q < NR; q += kBlock) {
+            for (int q = static_cast<int>(threadIdx.x); go && q < NR; q += kBlock) {
                 int qx = tx + q % 3 - 1, qy = ty + (q / 3) % 3 - 1;
                 const int qz = tz - q / 9;
                 if (qz >= 0 && wrap_axis(qx, P.ntx, P.per[0]) && wrap_axis(qy, P.nty, P.per[1]))
@@ -1075,17 +1163,17 @@ constexpr int kRulesWaves = (TY == 4 && TZ == 4 && sizeof(F) == 2)
                                 : (kRulesTilesPerCu<TY, TZ, F> < 5 ? kRulesTilesPerCu<TY, TZ, F> : 5);
 
 template <int TY, int TZ, class F>
-__global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_kernel(BatchDev B, int shape) {
+__global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_kernel(BatchDev B, int shape, int par) {
     using H = Halo<TY, TZ>;
     constexpr int LX = H::LX, LY = H::LY;
     __shared__ RulesTile<TY, TZ, F> T;
     __shared__ uint32_t s_nchg, s_nlist, s_nair, s_nwide;
     __shared__ uint32_t s_red[3][kBlock / 64];
     __shared__ uint32_t s_wflags[2];  // per-wave flag bytes: [0] staging, [1] end of tile
-    const uint64_t* tiles = B.active + shape * B.cap;
+    const uint64_t* tiles = B.active + (par * kShapes + shape) * static_cast<size_t>(B.cap);
     // the first entry is read together with the list length (blockIdx.x < gridDim.x <= cap: in bounds)
     uint64_t next = tiles[blockIdx.x];
-    const uint32_t nact = B.ctrl->nactive[shape];
+    const uint32_t nact = B.ctrl->nactive[par][shape];
     // the wave's first thread, once (a scalar): inside the loop the thread index is rebuilt from it and the lane
     // (see opaque_tid), so no vector register holds threadIdx.x across the loop
     const unsigned wbase = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x))) & ~63u;
@@ -1112,6 +1200,8 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         bool w_tchg = false;              // wave-uniform: a cell changed type
         const TileCoord o = tile_origin(P, tile);
         __syncthreads();  // the previous tile's LDS reads are done
+        // an entry of a slot that left the batch, or of a simulation that stopped after the entry was pushed
+        if (!sim.running || __builtin_amdgcn_readfirstlane(D.ctrl->stop)) continue;
         int own_air = 0;
         // lane j: base of halo row j; gx: this lane's column (x == nx is the wrapped x = 0 of a periodic partial
         // tile, -1 for no cell)
@@ -1400,12 +1490,20 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             }
         }
         __syncthreads();
-        // a tile holding lava after the step marks its neighbours for the next step's front: any lava (visit), and
-        // lava next to each neighbour (stage)
+        // a tile holding lava after the step lists the next step's tiles (lava front: itself and the neighbours it
+        // holds lava next to) or marks its neighbours for the next step's scan (any lava: visit; lava next to each
+        // neighbour: stage)
         if ((s_wflags[1] & 0x01010101u) != 0) {
             const int tx = o.x0 / kTileX, tyy = o.y0 / TY, tz = o.z0 / TZ;
-            mark_near(P, D, tx, tyy, tz, static_cast<int>(tid));
-            mark_halo(P, D, tx, tyy, tz, static_cast<int>(tid), s_red[2][0] | s_red[2][1] | s_red[2][2] | s_red[2][3]);
+            const uint32_t bnd_all = s_red[2][0] | s_red[2][1] | s_red[2][2] | s_red[2][3];
+            if (sim.push) {
+                if (tid < 64)
+                    push_next(P, D, B, tx, tyy, tz, static_cast<int>(tid), bnd_all | (1u << 13), e >> kTileBits,
+                              static_cast<uint32_t>(t) + 1u, par ^ 1, shape);
+            } else {
+                mark_near(P, D, tx, tyy, tz, static_cast<int>(tid));
+                mark_halo(P, D, tx, tyy, tz, static_cast<int>(tid), bnd_all);
+            }
         }
         if (tid == 0) {
             const uint32_t wf = s_wflags[1];
@@ -1487,17 +1585,20 @@ __global__ void __launch_bounds__(kBlock) ca_wide_kernel(BatchDev B) {
 }
 
 // Scatters each visited tile's change records into the state (LavaCA.hpp:1052-1066).
-__global__ void __launch_bounds__(kBlock) ca_apply_kernel(BatchDev B) {
+__global__ void __launch_bounds__(kBlock) ca_apply_kernel(BatchDev B, int par) {
     // one wave per staged tile
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const uint32_t waves = gridDim.x * (kBlock / 64);
 #pragma unroll 1
     for (int k = 0; k < kShapes; ++k) {
-        const uint32_t nact = B.ctrl->nactive[k];
-        const uint64_t* tiles = B.active + k * B.cap;
+        const uint32_t nact = B.ctrl->nactive[par][k];
+        const uint64_t* tiles = B.active + (par * kShapes + k) * static_cast<size_t>(B.cap);
         for (uint32_t s = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); s < nact; s += waves) {
             const uint32_t e = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(tiles[s])));
-            const CaDev& D = g_sims[B.sim_base + (e >> kTileBits)].D;
+            const CaSim& sim = g_sims[B.sim_base + (e >> kTileBits)];
+            // entries the rules kernel skipped (slot left the batch, simulation stopped) have no records of this step
+            if (!sim.running || __builtin_amdgcn_readfirstlane(sim.D.ctrl->stop)) continue;
+            const CaDev& D = sim.D;
             const uint32_t tile = e & kTileMask;
             const uint32_t n = D.tile_nchg[tile];
             const Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
@@ -1516,77 +1617,86 @@ __global__ void __launch_bounds__(kBlock) ca_apply_kernel(BatchDev B) {
     }
 }
 
-// finishStep (LavaCA.hpp:1206-1229) + frontCounts (LavaCA.hpp:865-871), one block per slot.  Block 0 also
-// clears the batch's list counters for the next step (every kernel of this step has finished with them).
-__global__ void __launch_bounds__(kBlock) ca_finish_kernel(BatchDev B) {
-    __shared__ unsigned long long red[5][kBlock / 64];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        for (int k = 0; k < kShapes; ++k) {
-            B.ctrl->nactive[k] = 0;
-            B.ctrl->nlava[k] = 0;
-        }
-        B.ctrl->nwide = 0;
-    }
-    const CaSim& sim = g_sims[B.sim_base + blockIdx.x];
-    if (!sim.running) return;
+// finishStep of every slot of the batch, one block per slot (after a step whose finish the next step's first kernel
+// does not do: every step of a batch that needs ca_front, the last step of a run of steps otherwise).
+__global__ void __launch_bounds__(kBlock) ca_finish_kernel(BatchDev B) { finish_slot(B, blockIdx.x); }
+
+// Initial list of a lava-front simulation that joins a batch: every tile that holds lava or has a neighbour tile with
+// lava (a superset of the tiles whose 1-cell halo holds lava: the extra tiles settle in the rules kernel), for the
+// simulation's current step, into list copy par.  One lane per tile, one list atomic per block.
+__global__ void __launch_bounds__(kBlock) ca_seed_kernel(BatchDev B, uint32_t slot, int par) {
+    __shared__ uint32_t s_cnt[kBlock / 64];
+    __shared__ uint32_t s_base;
+    const CaSim& sim = g_sims[B.sim_base + slot];
+    if (!sim.running || !sim.push) return;
     const CaParams& P = sim.P;
     const CaDev& D = sim.D;
-    CaCtrl* c = D.ctrl;
-    if (c->stop) return;
-    unsigned long long la = 0, bo = 0, ch = 0, tl = 0, st = 0;
-    for (int r = static_cast<int>(threadIdx.x); r < kPartRows; r += kBlock) {
-        CaPart* row = &D.part[r];
-        la += row->lava_after;
-        bo += row->born;
-        ch += row->changed;
-        tl += row->tiles;
-        st += row->staged;
-        *row = CaPart{0, 0, 0, 0, 0};  // cleared for the next step
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    bool want = false;
+    if (t < P.ntiles && !D.ctrl->stop) {
+        const uint32_t r = div_magic(t, P.div_ntx_m, P.div_ntx_p);
+        const int tx = static_cast<int>(t - r * static_cast<uint32_t>(P.ntx));
+        const uint32_t tz_ = div_magic(r, P.div_nty_m, P.div_nty_p);
+        const int ty = static_cast<int>(r - tz_ * static_cast<uint32_t>(P.nty)), tz = static_cast<int>(tz_);
+        for (int q = 0; q < 27 && !want; ++q) {
+            int qx = tx + q % 3 - 1, qy = ty + (q / 3) % 3 - 1, qz = tz + q / 9 - 1;
+            if (!wrap_axis(qx, P.ntx, P.per[0]) || !wrap_axis(qy, P.nty, P.per[1]) || !wrap_axis(qz, P.ntz, P.per[2]))
+                continue;
+            want = (D.tile_flags[static_cast<uint32_t>(qx) + static_cast<uint32_t>(P.ntx) *
+                                                             (static_cast<uint32_t>(qy) + static_cast<uint32_t>(P.nty) *
+                                                                                              static_cast<uint32_t>(qz))] &
+                    kFlagLava) != 0;
+        }
     }
-    la = wave_sum(la);
-    bo = wave_sum(bo);
-    ch = wave_sum(ch);
-    tl = wave_sum(tl);
-    st = wave_sum(st);
+    const unsigned long long m = __ballot(want);
     const int w = static_cast<int>(threadIdx.x >> 6);
-    if ((threadIdx.x & 63u) == 0) {
-        red[0][w] = la;
-        red[1][w] = bo;
-        red[2][w] = ch;
-        red[3][w] = tl;
-        red[4][w] = st;
+    if ((threadIdx.x & 63u) == 0) s_cnt[w] = static_cast<uint32_t>(__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int k = 0; k < kBlock / 64; ++k) tot += s_cnt[k];
+        s_base = tot ? atomicAdd(&B.ctrl->nactive[par][sim.shape], tot) : 0u;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    unsigned long long lava_after = 0, born = 0, changed = 0, tiles = 0, staged = 0;
-    for (int k = 0; k < kBlock / 64; ++k) {
-        lava_after += red[0][k];
-        born += red[1][k];
-        changed += red[2][k];
-        tiles += red[3][k];
-        staged += red[4][k];
+    if (want) {
+        uint32_t pos = s_base + mbcnt64(m);
+        for (int k = 0; k < w; ++k) pos += s_cnt[k];
+        B.active[(par * kShapes + sim.shape) * static_cast<size_t>(B.cap) + pos] =
+            (static_cast<uint64_t>(static_cast<uint32_t>(D.ctrl->steps)) << 32) | (slot << kTileBits) | t;
     }
-    const int32_t t = c->steps;
-    const unsigned long long active = P.classic ? c->lava_total + born : lava_after;
-    const unsigned long long air = c->air_total - born;
-    D.hist_active[t] = active;
-    D.hist_changed[t] = changed;
-    c->steps = t + 1;
-    c->lava_total = active;
-    c->air_total = air;
-    int32_t stop = 0;
-    if (!P.classic && active == 0)
-        stop = 1;  // NoLava
-    else if (P.classic && air == 0)
-        stop = 2;  // NoAir
-    else if (changed == 0 && P.nb.variants == 1)
-        stop = 3;  // NoChange
-    else if (t + 1 >= P.max_steps)
-        stop = 4;  // MaxSteps
-    c->stop = stop;
-    c->tiles_visited += tiles;
-    c->tiles_staged += staged;
-    c->changes_total += changed;  // every change is one record
+}
+
+// Removes the entries of the slots in `slots` from list copy par (one block; in place, order kept otherwise).
+__global__ void __launch_bounds__(1024) ca_purge_kernel(BatchDev B, SlotMask slots, int par) {
+    __shared__ uint32_t s_cnt[1024 / 64];
+    for (int k = 0; k < kShapes; ++k) {
+        const uint32_t n = B.ctrl->nactive[par][k];
+        uint64_t* L = B.active + (par * kShapes + k) * static_cast<size_t>(B.cap);
+        uint32_t out = 0;
+        for (uint32_t base = 0; base < n; base += 1024) {
+            const uint32_t i = base + threadIdx.x;
+            uint64_t v = 0;
+            bool keep = false;
+            if (i < n) {
+                v = L[i];
+                const uint32_t sl = static_cast<uint32_t>(v) >> kTileBits;  // < kMaxSlots
+                keep = !((slots.bits[sl >> 5] >> (sl & 31u)) & 1u);
+            }
+            const unsigned long long m = __ballot(keep);
+            const int w = static_cast<int>(threadIdx.x >> 6);
+            if ((threadIdx.x & 63u) == 0) s_cnt[w] = static_cast<uint32_t>(__popcll(m));
+            __syncthreads();  // also: the chunk's entries are read before any is written (positions < base + 1024)
+            uint32_t pos = out, tot = 0;
+            for (int q = 0; q < 1024 / 64; ++q) {
+                pos += q < w ? s_cnt[q] : 0u;
+                tot += s_cnt[q];
+            }
+            if (keep) L[pos + mbcnt64(m)] = v;
+            out += tot;
+            __syncthreads();  // s_cnt is rewritten by the next chunk
+        }
+        if (threadIdx.x == 0) B.ctrl->nactive[par][k] = out;
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) ca_status_kernel(BatchDev B, int32_t* out) {
@@ -1945,53 +2055,95 @@ void ca_launch_init_flags(const CaParams& P, const CaDev& D, hipStream_t s) {
     hipLaunchKernelGGL(ca_init_flags_kernel, dim3(g ? g : 1), dim3(kBlock), 0, s, P, D);
 }
 
-#define VCA_SHAPE_DISPATCH(k, KERNEL, ...)                                                              \
-    do {                                                                                               \
-        switch (k) {                                                                                   \
-            case 0: hipLaunchKernelGGL((KERNEL<4, 4>), __VA_ARGS__, 0); break;                         \
-            case 1: hipLaunchKernelGGL((KERNEL<2, 8>), __VA_ARGS__, 1); break;                         \
-            case 2: hipLaunchKernelGGL((KERNEL<8, 2>), __VA_ARGS__, 2); break;                         \
-            case 3: hipLaunchKernelGGL((KERNEL<1, 16>), __VA_ARGS__, 3); break;                        \
-            default: hipLaunchKernelGGL((KERNEL<16, 1>), __VA_ARGS__, 4); break;                       \
-        }                                                                                              \
-    } while (0)
+// A launch whose own dispatch records `start` / `stop` when they are given (kernel time without the launch gaps).
+template <typename Kernel, typename... Args>
+void launch(Kernel kernel, dim3 grid, hipStream_t s, hipEvent_t start, hipEvent_t stop, Args... args) {
+    if (!start && !stop)
+        hipLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, s, args...);
+    else
+        hipExtLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, s, start, stop, 0, args...);
+}
 
-#define VCA_RULES_DISPATCH(k, F, ...)                                                                  \
-    do {                                                                                               \
-        switch (k) {                                                                                   \
-            case 0: hipLaunchKernelGGL((ca_rules_kernel<4, 4, F>), __VA_ARGS__, 0); break;            \
-            case 1: hipLaunchKernelGGL((ca_rules_kernel<2, 8, F>), __VA_ARGS__, 1); break;            \
-            case 2: hipLaunchKernelGGL((ca_rules_kernel<8, 2, F>), __VA_ARGS__, 2); break;            \
-            case 3: hipLaunchKernelGGL((ca_rules_kernel<1, 16, F>), __VA_ARGS__, 3); break;           \
-            default: hipLaunchKernelGGL((ca_rules_kernel<16, 1, F>), __VA_ARGS__, 4); break;          \
-        }                                                                                              \
-    } while (0)
+// Events of kernel k of a step: ev[2k] recorded at the start of its first launch, ev[2k + 1] at the end of its last
+// (a kernel launched once per tile shape); rec[k] tells that the pair was recorded.
+struct StepEvents {
+    hipEvent_t* ev;
+    bool* rec;
+    hipEvent_t start(int k, bool first) const {
+        if (!ev || !first) return nullptr;
+        rec[k] = true;
+        return ev[2 * k];
+    }
+    hipEvent_t stop(int k, bool last) const { return ev && last ? ev[2 * k + 1] : nullptr; }
+};
 
-void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, bool flow16, int grid_tiles,
-                    hipStream_t s, hipEvent_t* ev) {
-    const unsigned gt = static_cast<unsigned>(grid_tiles);
-    if (ev) (void)hipEventRecord(ev[0], s);
-    hipLaunchKernelGGL(ca_front_kernel, dim3(grid_for((B.ntiles + kFrontPer - 1) / kFrontPer)), dim3(kBlock), 0, s,
-                       B, shapes);
-    if (ev) (void)hipEventRecord(ev[1], s);
-    if (any_lava_mode)
-        for (int k = 0; k < kShapes; ++k)
-            if ((shapes >> k) & 1u) VCA_SHAPE_DISPATCH(k, ca_descent_kernel, dim3(gt), dim3(kBlock), 0, s, B);
-    if (ev) (void)hipEventRecord(ev[2], s);
+template <int TY, int TZ>
+void launch_descent(int shape, dim3 g, hipStream_t s, hipEvent_t a, hipEvent_t b, const BatchDev& B, int par, int fin) {
+    launch(ca_descent_kernel<TY, TZ>, g, s, a, b, B, shape, par, fin);
+}
+template <int TY, int TZ, class F>
+void launch_rules(int shape, dim3 g, hipStream_t s, hipEvent_t a, hipEvent_t b, const BatchDev& B, int par) {
+    launch(ca_rules_kernel<TY, TZ, F>, g, s, a, b, B, shape, par);
+}
+void descent_shape(int k, dim3 g, hipStream_t s, hipEvent_t a, hipEvent_t b, const BatchDev& B, int par, int fin) {
+    switch (k) {
+        case 0: launch_descent<4, 4>(0, g, s, a, b, B, par, fin); break;
+        case 1: launch_descent<2, 8>(1, g, s, a, b, B, par, fin); break;
+        case 2: launch_descent<8, 2>(2, g, s, a, b, B, par, fin); break;
+        case 3: launch_descent<1, 16>(3, g, s, a, b, B, par, fin); break;
+        default: launch_descent<16, 1>(4, g, s, a, b, B, par, fin); break;
+    }
+}
+template <class F>
+void rules_shape(int k, dim3 g, hipStream_t s, hipEvent_t a, hipEvent_t b, const BatchDev& B, int par) {
+    switch (k) {
+        case 0: launch_rules<4, 4, F>(0, g, s, a, b, B, par); break;
+        case 1: launch_rules<2, 8, F>(1, g, s, a, b, B, par); break;
+        case 2: launch_rules<8, 2, F>(2, g, s, a, b, B, par); break;
+        case 3: launch_rules<1, 16, F>(3, g, s, a, b, B, par); break;
+        default: launch_rules<16, 1, F>(4, g, s, a, b, B, par); break;
+    }
+}
+
+// One step.  Kernel order: [ca_front] ca_descent (+ the previous step's finish when L.fin_prev) ca_rules ca_wide
+// ca_apply [ca_finish]; ca_front and ca_finish only when a member needs the scan.  Every kernel of a step reads the
+// list copy L.par.  Without lava mode there is no D(n) to compute, but ca_descent still runs (it clears the previous
+// copy's counts and does the pending finish).
+void ca_launch_step(const BatchDev& B, const StepLaunch& L, hipStream_t s, hipEvent_t* ev, bool* rec) {
+    const StepEvents E{ev, rec};
+    const dim3 gt(static_cast<unsigned>(L.grid_tiles));
+    if (L.scan) launch(ca_front_kernel, dim3(grid_for(B.ntiles)), s, E.start(0, true), E.stop(0, true), B, L.shapes, L.par);
+    int shapes[kShapes], ns = 0;
     for (int k = 0; k < kShapes; ++k)
-        if ((shapes >> k) & 1u) {
-            if (flow16)
-                VCA_RULES_DISPATCH(k, int16_t, dim3(gt), dim3(kBlock), 0, s, B);
-            else
-                VCA_RULES_DISPATCH(k, int32_t, dim3(gt), dim3(kBlock), 0, s, B);
-        }
-    if (ev) (void)hipEventRecord(ev[3], s);
-    hipLaunchKernelGGL(ca_wide_kernel, dim3(kWideBlocks), dim3(kBlock), 0, s, B);
-    if (ev) (void)hipEventRecord(ev[4], s);
-    hipLaunchKernelGGL(ca_apply_kernel, dim3((gt + 3) / 4), dim3(kBlock), 0, s, B);
-    if (ev) (void)hipEventRecord(ev[5], s);
-    hipLaunchKernelGGL(ca_finish_kernel, dim3(B.nslots), dim3(kBlock), 0, s, B);
-    if (ev) (void)hipEventRecord(ev[6], s);
+        if ((L.shapes >> k) & 1u) shapes[ns++] = k;
+    // the first descent launch clears the counts and does the finish (blocks 0 .. nslots - 1): it needs that many
+    const dim3 g0(std::max<unsigned>(gt.x, B.nslots));
+    for (int i = 0; i < ns; ++i) {
+        const bool lead = i == 0;
+        descent_shape(shapes[i], lead ? g0 : gt, s, E.start(1, lead), E.stop(1, i == ns - 1), B, L.par,
+                      lead && L.fin_prev ? 1 : 0);
+    }
+    for (int i = 0; i < ns; ++i) {
+        if (L.flow16)
+            rules_shape<int16_t>(shapes[i], gt, s, E.start(2, i == 0), E.stop(2, i == ns - 1), B, L.par);
+        else
+            rules_shape<int32_t>(shapes[i], gt, s, E.start(2, i == 0), E.stop(2, i == ns - 1), B, L.par);
+    }
+    launch(ca_wide_kernel, dim3(kWideBlocks), s, E.start(3, true), E.stop(3, true), B);
+    launch(ca_apply_kernel, dim3((gt.x + 3) / 4), s, E.start(4, true), E.stop(4, true), B, L.par);
+    if (L.scan) launch(ca_finish_kernel, dim3(B.nslots), s, E.start(5, true), E.stop(5, true), B);
+}
+
+void ca_launch_finish(const BatchDev& B, hipStream_t s, hipEvent_t* ev) {
+    launch(ca_finish_kernel, dim3(B.nslots), s, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr, B);
+}
+
+void ca_launch_seed(const BatchDev& B, uint32_t slot, uint32_t ntiles, int par, hipStream_t s) {
+    hipLaunchKernelGGL(ca_seed_kernel, dim3(grid_for(ntiles)), dim3(kBlock), 0, s, B, slot, par);
+}
+
+void ca_launch_purge(const BatchDev& B, const SlotMask& slots, int par, hipStream_t s) {
+    hipLaunchKernelGGL(ca_purge_kernel, dim3(1), dim3(1024), 0, s, B, slots, par);
 }
 
 void ca_launch_status(const BatchDev& B, int32_t* out, hipStream_t s) {

@@ -137,7 +137,8 @@ struct CaDev {
                                   // that tile's rules, read and cleared by ca_front in lava mode; kept in classic)
     uint8_t* tile_halo;           // kHaloBytes per tile: byte q = 1 when the neighbour tile in direction q holds lava
     uint32_t* tile_tchg;          // per tile: 1 + the last step in which a cell of the tile changed type (0: never)
-                                  // in its cells next to this tile (same writers / reader / clearing as tile_near)
+    uint32_t* listed;             // per tile: the step whose tile list holds the tile (lava front: the rules kernel
+                                  // pushes the next step's tiles; the stamp lists each tile once)
     Change* changes;              // kTileCells record slots per tile: tile t owns slots t*kTileCells ..
     uint32_t* tile_nchg;          // records of each visited tile this step (from slot 0 up)
     uint32_t* tile_nwide;         // cells left to ca_wide (kKindWide records from slot kTileCells-1 down)
@@ -160,6 +161,8 @@ struct CaSim {
     CaDev D;
     int32_t running;              // 1 while the simulation takes part in the batch's steps
     int32_t shape;
+    int32_t push;                 // lava front (lava mode, ca.sweep = front): the rules kernel lists the next step's
+                                  // tiles; otherwise ca_front scans the tiles every step
 };
 
 // A batch: simulations in slots, their tiles in one global tile range.  Work-list entries are
@@ -170,9 +173,11 @@ constexpr uint32_t kTileMask = (1u << kTileBits) - 1u;
 constexpr int kMaxSlots = 1024;
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
+// Tile lists come in two copies: a step reads the copy of its parity and the rules kernel pushes the next step's
+// tiles into the other one (lava front); the first kernel of a step clears the counts of the copy the previous step
+// read.
 struct BatchCtrl {
-    uint32_t nactive[kShapes];    // visited tiles per shape
-    uint32_t nlava[kShapes];      // tiles holding lava per shape (lava mode)
+    uint32_t nactive[2][kShapes]; // staged tiles per step parity and shape
     uint32_t nwide;               // tiles with cells for ca_wide
     uint32_t reserved[5];
 };
@@ -188,8 +193,7 @@ struct BatchDev {
     uint32_t ntiles;              // global tiles in use (high-water mark)
     uint32_t nslots;
     uint32_t cap;                 // capacity of each list
-    uint64_t* active;             // kShapes lists of cap entries: step << 32 | entry
-    uint32_t* lava;               // kShapes lists of cap entries
+    uint64_t* active;             // 2 x kShapes lists of cap entries (parity-major): step << 32 | entry
     uint32_t* wide;               // cap entries
     BatchCtrl* ctrl;
 };
@@ -224,11 +228,34 @@ void ca_launch_seeds(const CaParams& P, const CaDev& D, const uint32_t* cells, c
                      const float* temps, int n, hipStream_t s);
 void ca_launch_init_flags(const CaParams& P, const CaDev& D, hipStream_t s);
 void ca_launch_init_stop(const CaParams& P, const CaDev& D, hipStream_t s);
-// One step of every running simulation of the batch: front lists, D(n), rules, wide votes, apply, finish.
-// shapes = bit mask of the tile shapes present; flow16 = every flowId of the batch fits int16_t (the rules kernel
-// then holds them in 16 bits of LDS); ev != nullptr records 7 events around the kernel groups.
-void ca_launch_step(const BatchDev& B, uint32_t shapes, bool any_lava_mode, bool flow16, int grid_tiles,
-                    hipStream_t s, hipEvent_t* ev);
+// One step of every running simulation of the batch.
+struct StepLaunch {
+    uint32_t shapes;              // bit mask of the tile shapes present
+    bool lava_mode;               // some member runs in lava mode (D(n) is needed)
+    bool flow16;                  // every flowId of the batch fits int16_t (the rules kernel holds them in 16 bits)
+    bool scan;                    // some member needs ca_front (classic mode or ca.sweep = full): then ca_finish
+                                  // runs after every step; otherwise the previous step's finish runs inside the
+                                  // first kernel of the next step
+    int par;                      // parity of the step (which list copy it reads)
+    bool fin_prev;                // the previous step's finish is still to be done (by this step's first kernel)
+    int grid_tiles;
+};
+// Kernel timing: kCaKernels kernels in launch order; ev[2k], ev[2k + 1] (when non-null) are recorded by kernel k's
+// own dispatch (start and end); a kernel not launched in this step leaves its pair unrecorded (flag false).
+constexpr int kCaKernels = 6;     // front, descent, rules, wide, apply, finish
+void ca_launch_step(const BatchDev& B, const StepLaunch& L, hipStream_t s, hipEvent_t* ev, bool* recorded);
+// The finish of the last step enqueued (after a run of steps whose finish is done by the next step's first kernel);
+// ev (nullable): the start / end events of the launch.
+void ca_launch_finish(const BatchDev& B, hipStream_t s, hipEvent_t* ev);
+// Lists the tiles of slot's simulation that the next step stages (lava front: tiles with lava or with lava in their
+// 1-cell halo), into the copy of parity par.
+void ca_launch_seed(const BatchDev& B, uint32_t slot, uint32_t ntiles, int par, hipStream_t s);
+// Removes the entries of the slots in `slots` from the copy of parity par (their simulations left the batch; entries
+// they pushed for the next step would otherwise stay until a step reads and skips them).
+struct SlotMask {
+    uint32_t bits[kMaxSlots / 32];
+};
+void ca_launch_purge(const BatchDev& B, const SlotMask& slots, int par, hipStream_t s);
 // owner[base .. base + n) = slot
 void ca_launch_owner(uint32_t* owner, uint32_t base, uint32_t n, uint32_t slot, hipStream_t s);
 // out[2*s] = steps, out[2*s+1] = stop (-1 when slot s is not running) for s < nslots.
