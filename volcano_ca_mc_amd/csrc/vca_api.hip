@@ -283,7 +283,7 @@ struct vca_ca {
     CaDev D{};
     int shape = 0;
     DevBuf arena;  // every region below: one allocation and one free per simulation
-    Span type, flow, temp, solid, arrival, descent, spill, vent_cell, tile_flags, tile_near, tile_halo, tile_tchg, listed, changes,
+    Span type, flow, temp, temp2, solid, arrival, descent, spill, vent_cell, tile_flags, tile_near, tile_halo, tile_tchg, listed, changes,
         tile_nchg, tile_nwide, part, ctrl, hist_a, hist_c;
     uint64_t n = 0;
     std::unique_ptr<Batch> own;   // batch of one: the solo path (made on the first solo enqueue)
@@ -291,6 +291,8 @@ struct vca_ca {
     hipStream_t side = nullptr;   // during vca_ca_collect: the pooled stream its reads go on
     Batch* member = nullptr;      // the shared batch this simulation is in
     int slot = -1;
+    // the temperature copy holding the state after `steps` steps
+    const void* temp_after(int32_t steps) const { return (steps & 1) ? temp2.p : temp.p; }
 };
 
 // Entries of the per-device simulation table (kSimTable) handed out to batches as contiguous ranges.  A range
@@ -801,7 +803,7 @@ namespace {
 struct CaPlan {
     Layout L;
     size_t hcap = 1;
-    size_t type = 0, flow = 0, temp = 0, solid = 0, arrival = 0, ctrl = 0, tile_near = 0, tile_halo = 0, tile_tchg = 0, listed = 0,
+    size_t type = 0, flow = 0, temp = 0, temp2 = 0, solid = 0, arrival = 0, ctrl = 0, tile_near = 0, tile_halo = 0, tile_tchg = 0, listed = 0,
            part = 0, hist_a = 0, hist_c = 0;
     size_t zero_bytes = 0;
     size_t descent = 0, vent_cell = 0, tile_flags = 0, changes = 0, tile_nchg = 0, tile_nwide = 0, spill = 0;
@@ -815,6 +817,7 @@ CaPlan ca_plan(uint64_t n, uint32_t ntiles, size_t cols, bool lava, size_t vent_
     c.type = L.add(n);
     c.flow = L.add(n * 4);
     c.temp = L.add(n * 4);
+    c.temp2 = L.add(n * 4);
     c.solid = L.add(n * 2);
     c.arrival = L.add(n * 2);
     c.ctrl = L.add(sizeof(CaCtrl));
@@ -989,6 +992,7 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         region(ca->type, plan.type, n);
         region(ca->flow, plan.flow, n * 4);
         region(ca->temp, plan.temp, n * 4);
+        region(ca->temp2, plan.temp2, n * 4);
         region(ca->solid, plan.solid, n * 2);
         region(ca->arrival, plan.arrival, n * 2);
         region(ca->ctrl, plan.ctrl, sizeof(CaCtrl));
@@ -1015,7 +1019,8 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         CaDev& D = ca->D;
         D.type = ca->type.as<uint8_t>();
         D.flow = ca->flow.as<int32_t>();
-        D.temp = ca->temp.as<float>();
+        D.temp[0] = ca->temp.as<float>();
+        D.temp[1] = ca->temp2.as<float>();
         D.solid = ca->solid.as<uint16_t>();
         D.arrival = ca->arrival.as<uint16_t>();
         D.descent = ca->descent.as<int8_t>();
@@ -1175,7 +1180,7 @@ extern "C" int vca_ca_collect(vca_ca* ca, vca_ca_result* r) {
         const uint64_t n = ca->n;
         staged_download(r->type, ca->type.p, n, ca->side);
         staged_download(r->flow, ca->flow.p, n * 4, ca->side);
-        staged_download(r->temp, ca->temp.p, n * 4, ca->side);
+        staged_download(r->temp, ca->temp_after(c.steps), n * 4, ca->side);
         staged_download(r->solid, ca->solid.p, n * 2, ca->side);
         staged_download(r->arrival, ca->arrival.p, n * 2, ca->side);
         check(hipStreamSynchronize(ca->side), "collect");
@@ -1324,11 +1329,13 @@ extern "C" int vca_ca_download(vca_ca* ca, uint8_t* type, int32_t* flow, float* 
     return guarded([&] {
         require(ca, "ca == NULL");
         ca->ctx->set_device();
-        check(hipStreamSynchronize(ca->ctx->stream), "sync");
+        int32_t steps = 0;
+        ca_status_sync(ca, &steps, nullptr);
         const uint64_t n = ca->n;
         if (type) check(hipMemcpyAsync(type, ca->type.p, n, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
         if (flow) check(hipMemcpyAsync(flow, ca->flow.p, n * 4, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
-        if (temp) check(hipMemcpyAsync(temp, ca->temp.p, n * 4, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
+        if (temp)
+            check(hipMemcpyAsync(temp, ca->temp_after(steps), n * 4, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
         if (solid) check(hipMemcpyAsync(solid, ca->solid.p, n * 2, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
         if (arrival) check(hipMemcpyAsync(arrival, ca->arrival.p, n * 2, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
         check(hipStreamSynchronize(ca->ctx->stream), "download");

@@ -411,8 +411,8 @@ struct VoteWide {
 
 // tryPonding (LavaCA.hpp:443-461)
 template <class V>
-__device__ void try_ponding(const CaParams& P, const CaDev& D, int x, int y, int z, uint32_t col, uint32_t plane,
-                            V& v) {
+__device__ void try_ponding(const CaParams& P, const CaDev& D, const float* temp, int x, int y, int z, uint32_t col,
+                            uint32_t plane, V& v) {
     if (z > D.spill[col] || z == 0) return;
     const uint32_t b = col + plane * static_cast<uint32_t>(z - 1);
     if (D.type[b] != kLava) return;
@@ -427,12 +427,13 @@ __device__ void try_ponding(const CaParams& P, const CaDev& D, int x, int y, int
                                  plane * static_cast<uint32_t>(z - 1)];
         if (t == kAir) return;
     }
-    v.add(D.flow[b], D.temp[b] - P.cool_lat);
+    v.add(D.flow[b], temp[b] - P.cool_lat);
 }
 
 // gatherLavaCandidates (LavaCA.hpp:465-519) for the AIR cell (x, y, z).
 template <class V>
-__device__ void gather_lava(const CaParams& P, const CaDev& D, int x, int y, int z, uint32_t mask, V& v) {
+__device__ void gather_lava(const CaParams& P, const CaDev& D, const float* temp, int x, int y, int z, uint32_t mask,
+                            V& v) {
     const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
     const uint32_t col = static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y);
     int cdrop = -100;
@@ -441,7 +442,7 @@ __device__ void gather_lava(const CaParams& P, const CaDev& D, int x, int y, int
         if (k == 13 || !((mask >> k) & 1u)) continue;
         const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
         if (dz < 0) {
-            if (dx == 0 && dy == 0 && P.ponding) try_ponding(P, D, x, y, z, col, plane, v);
+            if (dx == 0 && dy == 0 && P.ponding) try_ponding(P, D, temp, x, y, z, col, plane, v);
             continue;
         }
         int qx = x + dx, qy = y + dy, qz = z + dz;
@@ -465,7 +466,7 @@ __device__ void gather_lava(const CaParams& P, const CaDev& D, int x, int y, int
                 cooling = P.cool_lat;
             }
         }
-        v.add(D.flow[ni], D.temp[ni] - cooling);
+        v.add(D.flow[ni], temp[ni] - cooling);
     }
 }
 
@@ -511,7 +512,7 @@ __global__ void ca_seeds_kernel(CaDev D, const uint32_t* cells, const int32_t* f
         const uint32_t i = cells[k];
         D.type[i] = kLava;
         D.flow[i] = flows[k];
-        D.temp[i] = temps[k];
+        D.temp[0][i] = temps[k];  // the state before step 0
         D.solid[i] = 0;
         D.arrival[i] = 0;
     }
@@ -1168,7 +1169,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
     constexpr int LX = H::LX, LY = H::LY;
     __shared__ RulesTile<TY, TZ, F> T;
     __shared__ uint32_t s_nchg, s_nlist, s_nair, s_nwide;
-    __shared__ uint32_t s_red[3][kBlock / 64];
+    __shared__ uint32_t s_red[4][kBlock / 64];
     __shared__ uint32_t s_wflags[2];  // per-wave flag bytes: [0] staging, [1] end of tile
     const uint64_t* tiles = B.active + (par * kShapes + shape) * static_cast<size_t>(B.cap);
     // the first entry is read together with the list length (blockIdx.x < gridDim.x <= cap: in bounds)
@@ -1197,7 +1198,10 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             P.classic ? P.nb.any_mask
                       : ((P.nb.any_mask & ~0x1FFu) | (P.ponding ? (P.nb.any_mask & (1u << 4)) : 0u));
         uint32_t w_lava = 0, w_born = 0;  // wave-uniform counts: lava cells after the step, born cells
+        uint32_t w_temp = 0;              // wave-uniform count: lava cells whose temperature changed (no record)
         bool w_tchg = false;              // wave-uniform: a cell changed type
+        const float* tcur = D.temp[t & 1];
+        float* tnext = D.temp[(t + 1) & 1];
         const TileCoord o = tile_origin(P, tile);
         __syncthreads();  // the previous tile's LDS reads are done
         // an entry of a slot that left the batch, or of a simulation that stopped after the entry was pushed
@@ -1259,7 +1263,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                     const uint32_t g = base + static_cast<uint32_t>(gx);
                     fl[k] = D.flow[g];
                     if (!P.classic) {
-                        tp[k] = D.temp[g];
+                        tp[k] = tcur[g];
                         ds[k] = D.descent[g];
                     }
                 }
@@ -1274,7 +1278,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                 const uint32_t g = ebase + static_cast<uint32_t>(ex);
                 efl = D.flow[g];
                 if (!P.classic) {
-                    etp = D.temp[g];
+                    etp = tcur[g];
                     eds = D.descent[g];
                 }
             }
@@ -1369,7 +1373,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             const int ly = row % TY, lz = row / TY;
             const int x = o.x0 + lx, y = o.y0 + ly, z = o.z0 + lz;
             const int c = (lx + 1) + LX * ((ly + 1) + LY * (lz + 1));
-            bool want = false, wide = false, stay = false, newborn = false, solid = false;
+            bool want = false, wide = false, stay = false, newborn = false, solid = false, cooled = false;
             Change ch;
             uint32_t i = 0;
             if (live) {
@@ -1389,6 +1393,9 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                             tc = P.erupt_temp;
                         else
                             tc = tc - P.cool_amb;
+                        // the next step's temperature, written whether or not it changed (the other copy holds the
+                        // value of two steps ago)
+                        tnext[i] = tc;
                         if (tc < P.solidus) {
                             want = true;
                             solid = true;
@@ -1396,10 +1403,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                         } else {
                             stay = true;
                             tile_lava = 1;
-                            if (tc != before) {
-                                want = true;
-                                ch = Change{i, f, tc, 0, kLava, kKindTemp};
-                            }
+                            cooled = tc != before;
                         }
                     }
                 } else if (ty == kAir) {
@@ -1454,6 +1458,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                     } else if (has && (P.classic || wt >= P.solidus)) {
                         want = true;
                         ch = Change{i, wf, P.classic ? 0.0f : wt, stamp, kLava, kKindBorn};
+                        if (!P.classic) tnext[i] = wt;  // read by the next step as the new lava cell's temperature
                         newborn = true;
                         tile_lava = 1;
                     } else {
@@ -1463,6 +1468,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             }
             w_lava += static_cast<uint32_t>(__popcll(__ballot(stay || newborn)));
             w_born += static_cast<uint32_t>(__popcll(__ballot(newborn)));
+            w_temp += static_cast<uint32_t>(__popcll(__ballot(cooled)));
             w_tchg = w_tchg || __ballot(newborn || solid) != 0;
             if (stay || newborn || wide) bnd |= boundary_dirs(lx, ly, lz, xm, ym, zm);
             Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
@@ -1486,6 +1492,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                 s_red[0][tid >> 6] = w_lava;
                 s_red[1][tid >> 6] = w_born;
                 s_red[2][tid >> 6] = bnd;
+                s_red[3][tid >> 6] = w_temp;
                 reinterpret_cast<uint8_t*>(&s_wflags[1])[tid >> 6] = static_cast<uint8_t>(f);
             }
         }
@@ -1514,15 +1521,17 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             D.tile_nchg[tile] = nchg;
             D.tile_nwide[tile] = s_nwide;
             if (s_nwide) B.wide[atomicAdd(&B.ctrl->nwide, 1u)] = e;
-            uint32_t r0 = 0, r1 = 0;
+            uint32_t r0 = 0, r1 = 0, r3 = 0;
             for (int w = 0; w < kBlock / 64; ++w) {
                 r0 += s_red[0][w];
                 r1 += s_red[1][w];
+                r3 += s_red[3][w];
             }
             CaPart* row = &D.part[tile % kPartRows];
             if (r0) atomicAdd(&row->lava_after, static_cast<unsigned long long>(r0));
             if (r1) atomicAdd(&row->born, static_cast<unsigned long long>(r1));
-            if (nchg) atomicAdd(&row->changed, static_cast<unsigned long long>(nchg));
+            // changed cells: every record (born, solidified) and every lava cell whose temperature changed
+            if (nchg + r3) atomicAdd(&row->changed, static_cast<unsigned long long>(nchg + r3));
             atomicAdd(&row->staged, 1ULL);
         }
     }
@@ -1564,7 +1573,7 @@ __global__ void __launch_bounds__(kBlock) ca_wide_kernel(BatchDev B) {
             if (P.classic)
                 gather_classic(P, D, x, y, z, mask, v);
             else
-                gather_lava(P, D, x, y, z, mask, v);
+                gather_lava(P, D, D.temp[t & 1], x, y, z, mask, v);
             if (v.n == 0) continue;
             int32_t wf = 0;
             float wt = 0.0f;
@@ -1572,6 +1581,7 @@ __global__ void __launch_bounds__(kBlock) ca_wide_kernel(BatchDev B) {
             if (!P.classic && wt < P.solidus) continue;
             // slots below the tile's other records: records + born <= candidates <= kTileCells
             rec[atomicAdd(&D.tile_nchg[tile], 1u)] = Change{i, wf, P.classic ? 0.0f : wt, stamp, kLava, kKindBorn};
+            if (!P.classic) D.temp[(t + 1) & 1][i] = wt;  // the new lava cell's temperature in the next step
             D.tile_tchg[tile] = static_cast<uint32_t>(t) + 1u;
             ++born;
         }
@@ -1594,7 +1604,9 @@ __global__ void __launch_bounds__(kBlock) ca_apply_kernel(BatchDev B, int par) {
         const uint32_t nact = B.ctrl->nactive[par][k];
         const uint64_t* tiles = B.active + (par * kShapes + k) * static_cast<size_t>(B.cap);
         for (uint32_t s = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); s < nact; s += waves) {
-            const uint32_t e = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(tiles[s])));
+            const uint64_t ent = tiles[s];
+            const uint32_t e = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(ent))));
+            const uint32_t tstep = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(ent >> 32))));
             const CaSim& sim = g_sims[B.sim_base + (e >> kTileBits)];
             // entries the rules kernel skipped (slot left the batch, simulation stopped) have no records of this step
             if (!sim.running || __builtin_amdgcn_readfirstlane(sim.D.ctrl->stop)) continue;
@@ -1602,15 +1614,18 @@ __global__ void __launch_bounds__(kBlock) ca_apply_kernel(BatchDev B, int par) {
             const uint32_t tile = e & kTileMask;
             const uint32_t n = D.tile_nchg[tile];
             const Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
+            // the step's own temperature copy: a solidified cell keeps its last temperature in both copies (the
+            // rules kernel wrote the other one); a born cell's temperature is in the next step's copy already
+            float* tcur = D.temp[tstep & 1u];
             for (uint32_t j = static_cast<uint32_t>(lane); j < n; j += 64) {
                 const Change c = rec[j];
                 D.type[c.idx] = c.type;
-                D.temp[c.idx] = c.temp;
                 if (c.kind == kKindBorn) {
                     D.flow[c.idx] = c.flow;
                     D.arrival[c.idx] = c.stamp;
-                } else if (c.kind == kKindSolidify) {
+                } else {
                     D.solid[c.idx] = c.stamp;
+                    tcur[c.idx] = c.temp;
                 }
             }
         }

@@ -96,9 +96,9 @@ struct Change {
     float temp;
     uint16_t stamp;
     uint8_t type;
-    uint8_t kind;                 // kTemp, kSolidify, kBorn
+    uint8_t kind;                 // kSolidify, kBorn (temperature changes of lava cells are written directly)
 };
-constexpr uint8_t kKindTemp = 0, kKindSolidify = 1, kKindBorn = 2, kKindWide = 3;
+constexpr uint8_t kKindSolidify = 1, kKindBorn = 2, kKindWide = 3;
 
 struct CaCtrl {
     int32_t steps;
@@ -126,7 +126,11 @@ constexpr int kMcSlots = 16;
 struct CaDev {
     uint8_t* type;
     int32_t* flow;
-    float* temp;
+    // temperatures at the start of a step, by the step's parity: step t reads temp[t & 1] and writes every lava cell's
+    // new temperature to temp[(t + 1) & 1] (no record, no scatter); a cell that solidifies in step t gets its last
+    // temperature into temp[t & 1] as well (by ca_apply, after every read of the step), so both copies hold the
+    // temperatures of every cell that is not lava, and temp[steps & 1] is the state after the last step
+    float* temp[2];
     uint16_t* solid;
     uint16_t* arrival;
     int8_t* descent;
