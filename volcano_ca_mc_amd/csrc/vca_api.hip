@@ -284,7 +284,7 @@ struct vca_ca {
     int shape = 0;
     DevBuf arena;  // every region below: one allocation and one free per simulation
     Span type, flow, temp, temp2, solid, arrival, descent, spill, vent_cell, tile_flags, tile_near, tile_halo, tile_tchg, listed, changes,
-        tile_nchg, tile_nwide, part, ctrl, hist_a, hist_c;
+        tile_nchg, part, ctrl, hist_a, hist_c;
     uint64_t n = 0;
     std::unique_ptr<Batch> own;   // batch of one: the solo path (made on the first solo enqueue)
     bool flow16 = true;           // every flowId the simulation can hold (its vents') fits int16_t
@@ -372,7 +372,7 @@ struct Batch {
     vca_ctx* ctx = nullptr;
     int max_slots = 0;
     uint32_t cap = 0;                             // tiles
-    DevBuf owner, tile_base, active, wide, ctrl, status;
+    DevBuf owner, tile_base, active, ctrl, status;
     uint32_t sim_base = 0;
     bool has_table = false;
     std::vector<CaSim> host_sims;                 // what the table entries hold (the copies read from here)
@@ -403,7 +403,6 @@ struct Batch {
         owner.alloc(static_cast<size_t>(cap) * 4);
         tile_base.alloc(static_cast<size_t>(slots) * 4);
         active.alloc(static_cast<size_t>(cap) * kShapes * 2 * 8);
-        wide.alloc(static_cast<size_t>(cap) * 4);
         ctrl.alloc(sizeof(BatchCtrl));
         check(hipMemsetAsync(owner.p, 0xFF, owner.bytes, s), "memset");  // kNoSlot
         check(hipMemsetAsync(ctrl.p, 0, sizeof(BatchCtrl), s), "memset");
@@ -419,7 +418,6 @@ struct Batch {
         B.nslots = 0;
         B.cap = cap;
         B.active = active.as<uint64_t>();
-        B.wide = wide.as<uint32_t>();
         B.ctrl = ctrl.as<BatchCtrl>();
         grid_tiles = static_cast<int>(std::min<uint32_t>(cap, 8192u));
     }
@@ -589,7 +587,7 @@ struct Batch {
     }
     void launch_direct(int steps) {
         vca_ctx* c = ctx;
-        static const char* kNames[kCaKernels] = {"ca_front", "ca_descent", "ca_rules", "ca_wide", "ca_apply", "ca_finish"};
+        static const char* kNames[kCaKernels] = {"ca_front", "ca_descent", "ca_rules", "ca_apply", "ca_finish"};
         for (int k = 0; k < steps; ++k) {
             if (!c->time_ca_step()) {
                 launch_step(c->stream, nullptr, nullptr);
@@ -806,7 +804,7 @@ struct CaPlan {
     size_t type = 0, flow = 0, temp = 0, temp2 = 0, solid = 0, arrival = 0, ctrl = 0, tile_near = 0, tile_halo = 0, tile_tchg = 0, listed = 0,
            part = 0, hist_a = 0, hist_c = 0;
     size_t zero_bytes = 0;
-    size_t descent = 0, vent_cell = 0, tile_flags = 0, changes = 0, tile_nchg = 0, tile_nwide = 0, spill = 0;
+    size_t descent = 0, vent_cell = 0, tile_flags = 0, changes = 0, tile_nchg = 0, spill = 0;
     size_t staging = 0, st_height = 0, st_cells = 0, st_flows = 0, st_temps = 0, st_bytes = 0;
 };
 
@@ -835,7 +833,6 @@ CaPlan ca_plan(uint64_t n, uint32_t ntiles, size_t cols, bool lava, size_t vent_
     const size_t records = static_cast<size_t>(ntiles) * kTileCells * sizeof(Change);
     c.changes = L.add(records);
     c.tile_nchg = L.add(static_cast<size_t>(ntiles) * 4);
-    c.tile_nwide = L.add(static_cast<size_t>(ntiles) * 4);
     if (lava) c.spill = L.add(cols * 4);
     Layout st;
     c.st_height = st.add(lava ? cols * 4 : 0);
@@ -1008,7 +1005,6 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         region(ca->tile_flags, plan.tile_flags, P.ntiles);
         region(ca->changes, plan.changes, static_cast<size_t>(P.ntiles) * kTileCells * sizeof(Change));
         region(ca->tile_nchg, plan.tile_nchg, static_cast<size_t>(P.ntiles) * 4);
-        region(ca->tile_nwide, plan.tile_nwide, static_cast<size_t>(P.ntiles) * 4);
         if (!classic) region(ca->spill, plan.spill, cols * 4);
         // state, counters, neighbour marks, partials and histories start at zero; descent at "none"
         check(hipMemsetAsync(base, 0, plan.zero_bytes, s), "memset");
@@ -1032,7 +1028,6 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         D.listed = ca->listed.as<uint32_t>();
         D.changes = ca->changes.as<Change>();
         D.tile_nchg = ca->tile_nchg.as<uint32_t>();
-        D.tile_nwide = ca->tile_nwide.as<uint32_t>();
         D.part = ca->part.as<CaPart>();
         D.ctrl = ca->ctrl.as<CaCtrl>();
         D.hist_active = ca->hist_a.as<unsigned long long>();
@@ -1368,7 +1363,7 @@ extern "C" int vca_ca_footprint(const vca_ca_desc* d, uint64_t* bytes) {
         // the simulation's allocation (vents: a few slots), and the batch-of-one lists a solo run adds
         const CaPlan plan = ca_plan(n, static_cast<uint32_t>(std::min<uint64_t>(tiles, kTileMask)), cols, d->classic == 0,
                                     64, 64, d->max_steps);
-        *bytes = plan.L.total + tiles * (2 * kShapes * 8 + 8) + 4096;
+        *bytes = plan.L.total + tiles * (2 * kShapes * 8 + 4) + 4096;
     });
 }
 

@@ -1,20 +1,22 @@
 // HIP kernels of the lava CA step and the Potts/Metropolis sweep, written for gfx950 (CDNA4).
 //
-// CA step (one synchronous step of engine/src/ca/LavaCA.hpp) = six stream-ordered launches:
-//   ca_front   : active tile list — a 64x4x4 (3D) / 64x1x16 (x-z) / 64x16x1 (x-y) tile is visited when it
-//                holds lava, or holds AIR next to a tile with lava (the reference's "may change" set,
-//                engine/src/ca/CaRules.hpp:69-79, at tile granularity) — plus the list of tiles holding lava.
-//                Wave ballot + one atomic per wave appends the tile ids.
-//   ca_descent : D(n) of every lava cell (LavaCA.hpp:421-429, 877-888), one block per lava tile staged in LDS.
-//   ca_rules   : one block per visited tile staged in LDS with its halo; the rules of every cell read the
-//                start-of-step state only and write the cell's new state as a record into the tile's own
-//                slot range of the change buffer (LDS counter, no global atomics), count lava/born/changed,
-//                and set the tile's lava/air flags from the new state (LavaCA.hpp:963-1048, 1109-1202).
-//   ca_wide    : the few AIR cells whose candidates carry several flowIds (full vote).
-//   ca_apply   : scatters the records of every visited tile into the state (LavaCA.hpp:1052-1066).
-//   ca_finish  : counter rows -> histories, stop rule (LavaCA.hpp:1206-1229).
-// Every kernel first reads the device stop flag and returns when a stop rule already fired, so the host
-// can enqueue many steps without waiting.
+// CA step (one synchronous step of engine/src/ca/LavaCA.hpp) = three stream-ordered launches for lava-front
+// simulations:
+//   ca_descent : clears the counts of the previous step's tile list, does the previous step's finish (step counters
+//                -> histories, stop rule, LavaCA.hpp:1206-1229), and computes D(n) of every lava cell
+//                (LavaCA.hpp:421-429, 877-888) of the staged tiles whose D(n) can have changed, one block per tile
+//                staged in LDS.
+//   ca_rules   : one block per staged tile (tile + 1-cell halo in LDS); the rules of every cell read the
+//                start-of-step state only; lava cells write their next temperature to the step's other temperature
+//                copy, type changes go as records into the tile's own slot range of the change buffer (LDS counter,
+//                no global atomics); counters, tile flags, and the next step's tile list (the tile itself and the
+//                neighbours it holds lava next to: the reference's "may change" set, CaRules.hpp:69-79, at tile
+//                granularity) (LavaCA.hpp:963-1048, 1109-1202).
+//   ca_apply   : scatters the records of every staged tile into the state (LavaCA.hpp:1052-1066).
+// The finish of the last step of a run of steps is one more launch (ca_finish).  Simulations that need every tile
+// visited (ca.sweep = full) or classic mode add ca_front (a scan of every tile) before and ca_finish after each step.
+// Every kernel skips the entries of a simulation whose stop rule fired, so the host can enqueue many steps without
+// waiting.
 //
 // MC checkerboard sweep (engine/src/mc/PottsMC.cpp:393-514): one launch per colour over the colour's list
 // of domain cells; Metropolis and mobility decisions are exact integer compares against host-built 53-bit
@@ -286,23 +288,6 @@ __device__ inline uint32_t col_index(const CaParams& P, int x, int y) {
     return static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y);
 }
 
-// hasSupport (LavaCA.hpp:390-399); col = x + nx*y of an interior column.
-__device__ inline bool has_support(const CaDev& D, uint32_t col, uint32_t plane, int z) {
-    if (z == 0) return true;
-    const uint8_t b = D.type[col + plane * static_cast<uint32_t>(z - 1)];
-    if (b == kRock || b == kBasalt) return true;
-    return b == kLava && D.spill != nullptr && z - 1 < D.spill[col];
-}
-
-// dropDepth (LavaCA.hpp:404-414)
-__device__ int drop_depth(const CaDev& D, uint32_t col, uint32_t plane, int z) {
-    for (int d = 0; d <= kMaxDrop && z - d >= 0; ++d) {
-        if (d > 0 && D.type[col + plane * static_cast<uint32_t>(z - d)] != kAir) return -1;
-        if (has_support(D, col, plane, z - d)) return d;
-    }
-    return kMaxDrop;
-}
-
 // h % m for a small m >= 1 (m < 2^16) without the 64-bit division routine: 2^32 = q*m + r32 gives
 // h = hi * 2^32 + lo == (hi mod m) * r32 + lo mod m  (mod m).
 __device__ inline uint32_t mod_small(uint64_t h, uint32_t m) {
@@ -313,8 +298,7 @@ __device__ inline uint32_t mod_small(uint64_t h, uint32_t m) {
 
 // CandidateVote (engine/src/ca/CaRules.hpp:127-183) in registers for the common case, a cell whose candidates
 // all carry one flowId: the winner is that flowId with the largest candidate temperature (most votes, and no tie
-// to break).  A second flowId among the candidates sets overflow: such cells (where flows meet) are decided by
-// ca_wide.
+// to break).  A second flowId among the candidates sets overflow: such cells (where flows meet) are gathered again.
 struct VoteFirst {
     int n = 0;             // candidates counted (all of flowId id)
     bool overflow = false;
@@ -322,7 +306,7 @@ struct VoteFirst {
     float mt = -__builtin_inff();  // read only when n > 0: the first candidate's t replaces it (t is finite)
     // add(f, t) when ok, nothing otherwise.  The first candidate needs no case of its own: with n == 0 it
     // matches, takes id = f, and mt = t since -inf < t.
-    __device__ void add_if(bool ok, int32_t f, float t) {
+    __device__ void add_if(bool ok, int32_t f, float t, int /*k*/) {
         const bool match = n == 0 || f == id;
         const bool same = ok && match;
         overflow = overflow || (ok && !match);
@@ -330,7 +314,6 @@ struct VoteFirst {
         mt = (same && mt < t) ? t : mt;  // std::max(a, b) == (a < b ? b : a)
         n += same ? 1 : 0;
     }
-    __device__ void add(int32_t f, float t) { add_if(true, f, t); }
     __device__ void winner(int32_t& flow, float& temp) const {
         flow = id;
         temp = mt;
@@ -339,13 +322,13 @@ struct VoteFirst {
 
 // CandidateVote for candidates carrying at most two flowIds (where two flows meet): most votes wins; a tie between
 // the two picks by the hash among them in ascending flowId order (engine/src/ca/CaRules.hpp:127-183); the
-// winner's temperature is the largest of its candidates.  A third flowId sets overflow (ca_wide decides).
+// winner's temperature is the largest of its candidates.  A third flowId sets overflow (the full vote decides).
 struct VoteTwo {
     int n = 0, n2 = 0;     // candidates of id / of id2
     bool overflow = false;
     int32_t id = 0, id2 = 0;
     float mt = -__builtin_inff(), mt2 = -__builtin_inff();  // as in VoteFirst: an empty slot matches any f
-    __device__ void add_if(bool ok, int32_t f, float t) {
+    __device__ void add_if(bool ok, int32_t f, float t, int /*k*/) {
         const bool m1 = n == 0 || f == id;
         const bool m2 = n2 == 0 || f == id2;
         const bool one = ok && m1;
@@ -358,7 +341,6 @@ struct VoteTwo {
         mt2 = (two && mt2 < t) ? t : mt2;
         n2 += two ? 1 : 0;
     }
-    __device__ void add(int32_t f, float t) { add_if(true, f, t); }
     __device__ void winner(uint64_t h_index, int32_t& flow, float& temp) const {
         bool take2 = n2 > n;
         if (n2 > 0 && n2 == n) {
@@ -370,121 +352,12 @@ struct VoteTwo {
     }
 };
 
-// Vote over any number of distinct flowIds (private memory; the cells VoteFirst left to ca_wide).
-struct VoteWide {
-    int n = 0;
-    bool overflow = false;
-    int32_t id[32];
-    int cnt[32];
-    float mt[32];
-    __device__ void add(int32_t f, float t) {
-        for (int s = 0; s < n; ++s) {
-            if (id[s] == f) {
-                ++cnt[s];
-                if (mt[s] < t) mt[s] = t;
-                return;
-            }
-        }
-        id[n] = f;
-        cnt[n] = 1;
-        mt[n] = t;
-        ++n;
-    }
-    __device__ void winner(uint64_t h_index, int32_t& flow, float& temp) const {
-        int best = 0;
-        for (int s = 0; s < n; ++s) best = cnt[s] > best ? cnt[s] : best;
-        int k = 0;
-        for (int s = 0; s < n; ++s) k += cnt[s] == best ? 1 : 0;
-        int pick = 0;
-        if (n > 1 && k > 1) pick = static_cast<int>(mod_small(splitmix64(h_index ^ kSaltTieBreak), static_cast<uint32_t>(k)));
-        for (int s = 0; s < n; ++s) {
-            if (cnt[s] != best) continue;
-            int rank = 0;
-            for (int q = 0; q < n; ++q) rank += (cnt[q] == best && id[q] < id[s]) ? 1 : 0;
-            if (rank == pick) {
-                flow = id[s];
-                temp = mt[s];
-            }
-        }
-    }
+// The accepted candidate positions of a cell (bit k: the neighbour at position k is a candidate), for the full vote
+// of vote_positions.
+struct VoteMask {
+    uint32_t acc = 0;
+    __device__ void add_if(bool ok, int32_t, float, int k) { acc |= (ok ? 1u : 0u) << k; }
 };
-
-// tryPonding (LavaCA.hpp:443-461)
-template <class V>
-__device__ void try_ponding(const CaParams& P, const CaDev& D, const float* temp, int x, int y, int z, uint32_t col,
-                            uint32_t plane, V& v) {
-    if (z > D.spill[col] || z == 0) return;
-    const uint32_t b = col + plane * static_cast<uint32_t>(z - 1);
-    if (D.type[b] != kLava) return;
-    if (!has_support(D, col, plane, z - 1)) return;
-    const int hx[4] = {-1, 1, 0, 0};
-    const int hy[4] = {0, 0, -1, 1};
-    const int nh = P.ny > 1 ? 4 : 2;
-    for (int k = 0; k < nh; ++k) {
-        int qx = x + hx[k], qy = y + hy[k];
-        if (!wrap_axis(qx, P.nx, P.per[0]) || !wrap_axis(qy, P.ny, P.per[1])) return;  // VOID
-        const uint8_t t = D.type[static_cast<uint32_t>(qx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(qy) +
-                                 plane * static_cast<uint32_t>(z - 1)];
-        if (t == kAir) return;
-    }
-    v.add(D.flow[b], temp[b] - P.cool_lat);
-}
-
-// gatherLavaCandidates (LavaCA.hpp:465-519) for the AIR cell (x, y, z).
-template <class V>
-__device__ void gather_lava(const CaParams& P, const CaDev& D, const float* temp, int x, int y, int z, uint32_t mask,
-                            V& v) {
-    const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
-    const uint32_t col = static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y);
-    int cdrop = -100;
-#pragma unroll
-    for (int k = 0; k < 27; ++k) {
-        if (k == 13 || !((mask >> k) & 1u)) continue;
-        const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
-        if (dz < 0) {
-            if (dx == 0 && dy == 0 && P.ponding) try_ponding(P, D, temp, x, y, z, col, plane, v);
-            continue;
-        }
-        int qx = x + dx, qy = y + dy, qz = z + dz;
-        if (!wrap_axis(qx, P.nx, P.per[0]) || !wrap_axis(qy, P.ny, P.per[1]) || !wrap_axis(qz, P.nz, P.per[2]))
-            continue;
-        const uint32_t ni = static_cast<uint32_t>(qx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(qy) +
-                            plane * static_cast<uint32_t>(qz);
-        if (D.type[ni] != kLava) continue;
-        float cooling;
-        if (dz > 0 && dx == 0 && dy == 0) {
-            cooling = P.cool_fall;
-        } else {
-            const int d = D.descent[ni];
-            if (d < 0) continue;
-            if (cdrop == -100) cdrop = drop_depth(D, col, plane, z);
-            if (dz > 0) {
-                if (cdrop != 0 || d != 1) continue;
-                cooling = P.cool_fall;
-            } else {
-                if (cdrop != d || (d == 1 && P.nb.diagonal_falls)) continue;
-                cooling = P.cool_lat;
-            }
-        }
-        v.add(D.flow[ni], temp[ni] - cooling);
-    }
-}
-
-// Classic grain growth: every lava (grain) neighbour votes (LavaCA.hpp:1165-1202).
-template <class V>
-__device__ void gather_classic(const CaParams& P, const CaDev& D, int x, int y, int z, uint32_t mask, V& v) {
-    const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
-#pragma unroll
-    for (int k = 0; k < 27; ++k) {
-        if (k == 13 || !((mask >> k) & 1u)) continue;
-        int qx = x + (k % 3 - 1), qy = y + ((k / 3) % 3 - 1), qz = z + (k / 9 - 1);
-        if (!wrap_axis(qx, P.nx, P.per[0]) || !wrap_axis(qy, P.ny, P.per[1]) || !wrap_axis(qz, P.nz, P.per[2]))
-            continue;
-        const uint32_t ni = static_cast<uint32_t>(qx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(qy) +
-                            plane * static_cast<uint32_t>(qz);
-        if (D.type[ni] == kLava) v.add(D.flow[ni], 0.0f);
-    }
-}
 
 }  // namespace
 
@@ -933,7 +806,7 @@ __device__ int drop_depth_lds(const CaDev& D, const uint8_t* s_type, int c, int 
 // D(n) of every lava cell of the tiles holding lava (LavaCA.hpp:421-429, 877-888), from the LDS-staged tile:
 // hasSupport of the cell, then the deepest dropDepth over its AIR side neighbours (union of the variants).
 template <int TY, int TZ>
-// The first kernel of a step: it clears the counts of the list copy the previous step read (and the wide-tile count),
+// The first kernel of a step: it clears the counts of the list copy the previous step read,
 // does the previous step's finish when fin != 0 (blocks 0 .. nslots-1, one slot each), then computes D(n) of the
 // staged tiles of parity par that hold lava.
 __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shape, int par, int fin) {
@@ -947,7 +820,6 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
         // every kernel of the previous step is done with its list copy; the rules kernel of this step pushes the next
         // step's tiles there
         B.ctrl->nactive[par ^ 1][threadIdx.x] = 0u;
-        if (threadIdx.x == 0) B.ctrl->nwide = 0u;
     }
     if (fin && blockIdx.x < B.nslots) finish_slot(B, blockIdx.x);
     const uint32_t n = B.ctrl->nactive[par][shape];
@@ -1096,8 +968,8 @@ __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY
             const int k2 = two ? __builtin_ctz(bits) : k1;
             bits &= bits - 1;
             const int32_t f1 = T.flow[c + koff<LX, LXY>(k1)], f2 = T.flow[c + koff<LX, LXY>(k2)];
-            v.add(f1, 0.0f);
-            v.add_if(two, f2, 0.0f);
+            v.add_if(true, f1, 0.0f, k1);
+            v.add_if(two, f2, 0.0f, k2);
         }
         return;
     }
@@ -1120,7 +992,7 @@ __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY
             }
             ok = closed;
         }
-        v.add_if(ok, T.flow[n], T.temp[n] - P.cool_lat);
+        v.add_if(ok, T.flow[n], T.temp[n] - P.cool_lat, 4);
     }
     // dropDepth of the cell, once, when a side or diagonal-up neighbour may need it (pure function of the cell)
     const int cdrop = (bits & ~(1u << 22)) ? drop_depth_lds<TY, TZ>(D, T.S.type, c, lz, z, col, plane, spill_c, has_spill)
@@ -1143,16 +1015,69 @@ __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY
         const int d1 = T.desc[n1], d2 = T.desc[n2];
         const int32_t f1 = T.flow[n1], f2 = T.flow[n2];
         const float t1 = T.temp[n1], t2 = T.temp[n2];
-        v.add_if(accept(k1, d1), f1, t1 - (k1 >= 18 ? P.cool_fall : P.cool_lat));
-        v.add_if(two && accept(k2, d2), f2, t2 - (k2 >= 18 ? P.cool_fall : P.cool_lat));
+        v.add_if(accept(k1, d1), f1, t1 - (k1 >= 18 ? P.cool_fall : P.cool_lat), k1);
+        v.add_if(two && accept(k2, d2), f2, t2 - (k2 >= 18 ? P.cool_fall : P.cool_lat), k2);
     }
+}
+
+// CandidateVote (engine/src/ca/CaRules.hpp:127-183) over any number of flowIds, for a cell whose candidates carry three
+// or more (rare: where three flows meet).  acc = the accepted candidate positions; the candidate at position k carries
+// the flowId and temperature of the LDS position c + koff(k), minus the cooling of its move (falls from above:
+// coolingFall; side, diagonal-down and ponding moves: coolingLateral; classic mode: temperature 0).  Most candidates
+// wins; a tie picks by the hash among the tied flowIds in ascending order; the temperature is the largest of the
+// winner's candidates.  Loops over the positions instead of per-flowId tables (no private arrays).
+template <int TY, int TZ, class F>
+__device__ void vote_positions(const CaParams& P, const RulesTile<TY, TZ, F>& T, int c, uint32_t acc, uint64_t h_index,
+                               int32_t& wf, float& wt) {
+    constexpr int LX = Halo<TY, TZ>::LX, LXY = Halo<TY, TZ>::LXY;
+    auto flow_at = [&](int k) { return static_cast<int32_t>(T.flow[c + koff<LX, LXY>(k)]); };
+    auto count_of = [&](int32_t f) {
+        int n = 0;
+        for (uint32_t m = acc; m; m &= m - 1) n += flow_at(__builtin_ctz(m)) == f ? 1 : 0;
+        return n;
+    };
+    // a position is a flowId's first when no lower accepted position carries the same flowId
+    auto first_of_flow = [&](int k) {
+        const int32_t f = flow_at(k);
+        for (uint32_t m = acc & ((1u << k) - 1u); m; m &= m - 1)
+            if (flow_at(__builtin_ctz(m)) == f) return false;
+        return true;
+    };
+    int best = 0;
+    for (uint32_t m = acc; m; m &= m - 1) {
+        const int n = count_of(flow_at(__builtin_ctz(m)));
+        best = n > best ? n : best;
+    }
+    uint32_t tied = 0;  // first positions of the flowIds with the most candidates
+    for (uint32_t m = acc; m; m &= m - 1) {
+        const int k = __builtin_ctz(m);
+        if (first_of_flow(k) && count_of(flow_at(k)) == best) tied |= 1u << k;
+    }
+    const uint32_t ntied = static_cast<uint32_t>(__builtin_popcount(tied));
+    const uint32_t pick = ntied > 1 ? mod_small(splitmix64(h_index ^ kSaltTieBreak), ntied) : 0u;
+    int32_t win = 0;
+    for (uint32_t m = tied; m; m &= m - 1) {
+        const int32_t f = flow_at(__builtin_ctz(m));
+        uint32_t rank = 0;  // tied flowIds below f
+        for (uint32_t q = tied; q; q &= q - 1) rank += flow_at(__builtin_ctz(q)) < f ? 1u : 0u;
+        if (rank == pick) win = f;
+    }
+    float mt = -__builtin_inff();
+    for (uint32_t m = acc; m; m &= m - 1) {
+        const int k = __builtin_ctz(m);
+        if (flow_at(k) != win) continue;
+        const float t = P.classic ? 0.0f : T.temp[c + koff<LX, LXY>(k)] - (k >= 18 ? P.cool_fall : P.cool_lat);
+        mt = mt < t ? t : mt;  // std::max in the order of the positions
+    }
+    wf = win;
+    wt = mt;
 }
 
 // Rules of every cell of the visited tiles (LavaCA.hpp:963-1048, 1109-1202).  The tile and a 1-cell halo
 // are staged in LDS first (types for every position, then flowId / temperature / D(n) for LAVA positions,
 // all independent coalesced loads), so an AIR cell folds its 26 neighbour types into a bitmask without
-// branches and gathers its candidates from LDS.  Cells whose candidates carry more than one flowId are
-// handed to ca_wide.
+// branches and gathers its candidates from LDS.  Cells whose candidates carry more than one flowId are gathered
+// again with two vote slots, three or more flowIds with the full vote (vote_positions).
 // Register budget: as many waves per SIMD as tiles per CU fit in LDS (the 4 x 4 tile with 16-bit flowIds: six,
 // 80 registers; the others five at most, 96 registers) -- more tiles in flight hide the staging latency better
 // than the few registers the limit costs.
@@ -1168,7 +1093,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
     using H = Halo<TY, TZ>;
     constexpr int LX = H::LX, LY = H::LY;
     __shared__ RulesTile<TY, TZ, F> T;
-    __shared__ uint32_t s_nchg, s_nlist, s_nair, s_nwide;
+    __shared__ uint32_t s_nchg, s_nlist, s_nair;
     __shared__ uint32_t s_red[4][kBlock / 64];
     __shared__ uint32_t s_wflags[2];  // per-wave flag bytes: [0] staging, [1] end of tile
     const uint64_t* tiles = B.active + (par * kShapes + shape) * static_cast<size_t>(B.cap);
@@ -1218,7 +1143,6 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             s_nchg = zero;
             s_nlist = zero;
             s_nair = zero;
-            s_nwide = zero;
         }
         {
             // block OR of saw_lava (bit 0) and own_air (bit 1): one flag byte per wave
@@ -1234,7 +1158,6 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             if (tid == 0) {
                 D.tile_flags[tile] = static_cast<uint8_t>(any_air ? kFlagAir : 0);
                 D.tile_nchg[tile] = 0;
-                D.tile_nwide[tile] = 0;
                 atomicAdd(&D.part[tile % kPartRows].staged, 1ULL);
             }
             continue;
@@ -1361,7 +1284,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         // candidate slots: LAVA cells at [0, nlava), AIR cells from the next wave boundary on (list order reversed)
         const uint32_t nlava = s_nlist, abase = (nlava + 63u) & ~63u, nslots = abase + s_nair;
         // directions in which the tile holds lava on its boundary after the step (mark_halo): cells that stay or
-        // become LAVA, and the cells left to ca_wide (either outcome)
+        // become LAVA
         const int xm = min(kTileX - 1, P.nx - 1 - o.x0), ym = min(TY - 1, P.ny - 1 - o.y0),
                   zm = min(TZ - 1, P.nz - 1 - o.z0);
         uint32_t bnd = 0;
@@ -1373,7 +1296,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             const int ly = row % TY, lz = row / TY;
             const int x = o.x0 + lx, y = o.y0 + ly, z = o.z0 + lz;
             const int c = (lx + 1) + LX * ((ly + 1) + LY * (lz + 1));
-            bool want = false, wide = false, stay = false, newborn = false, solid = false, cooled = false;
+            bool want = false, stay = false, newborn = false, solid = false, cooled = false;
             Change ch;
             uint32_t i = 0;
             if (live) {
@@ -1440,22 +1363,23 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                             gather_lds(P, D, T, c, lz, z, col, plane, spill_c, has_spill, bits, v);
                             has = v.n > 0;
                             if (v.overflow) {
-                                // a second flowId (flows meet): the gather again with two slots; a third one
-                                // leaves the cell to ca_wide
+                                // a second flowId (flows meet): the gather again with two slots; a third one: the
+                                // full vote over the accepted positions (rare: where three flows meet)
                                 VoteTwo v2;
                                 gather_lds(P, D, T, c, lz, z, col, plane, spill_c, has_spill, bits, v2);
-                                has = !v2.overflow;
-                                wide = v2.overflow;
-                                if (has) v2.winner(h_index, wf, wt);
+                                if (!v2.overflow) {
+                                    v2.winner(h_index, wf, wt);
+                                } else {
+                                    VoteMask vm;
+                                    gather_lds(P, D, T, c, lz, z, col, plane, spill_c, has_spill, bits, vm);
+                                    vote_positions(P, T, c, vm.acc, h_index, wf, wt);
+                                }
                             } else if (has) {
                                 v.winner(wf, wt);
                             }
                         }
                     }
-                    if (wide) {
-                        tile_lava = 1;  // either outcome: both flags keep the next front a superset
-                        tile_air = 1;
-                    } else if (has && (P.classic || wt >= P.solidus)) {
+                    if (has && (P.classic || wt >= P.solidus)) {
                         want = true;
                         ch = Change{i, wf, P.classic ? 0.0f : wt, stamp, kLava, kKindBorn};
                         if (!P.classic) tnext[i] = wt;  // read by the next step as the new lava cell's temperature
@@ -1470,16 +1394,10 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             w_born += static_cast<uint32_t>(__popcll(__ballot(newborn)));
             w_temp += static_cast<uint32_t>(__popcll(__ballot(cooled)));
             w_tchg = w_tchg || __ballot(newborn || solid) != 0;
-            if (stay || newborn || wide) bnd |= boundary_dirs(lx, ly, lz, xm, ym, zm);
+            if (stay || newborn) bnd |= boundary_dirs(lx, ly, lz, xm, ym, zm);
             Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
             const uint32_t slot = wave_append(want, &s_nchg);
             if (want) rec[slot] = ch;
-            if (__ballot(wide)) {
-                // records fill the tile's slots from 0 up, cells for ca_wide from kTileCells - 1 down; both
-                // together are at most the tile's candidates, so they never meet
-                const uint32_t wslot = wave_append(wide, &s_nwide);
-                if (wide) rec[kTileCells - 1 - wslot] = Change{i, 0, 0.0f, 0, 0, kKindWide};
-            }
         }
         // tile flags, record counts and the tile's counters (row tile % kPartRows of its simulation); every
         // record is one changed cell
@@ -1519,8 +1437,6 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             if (fl & 4u) D.tile_tchg[tile] = static_cast<uint32_t>(t) + 1u;
             const uint32_t nchg = s_nchg;
             D.tile_nchg[tile] = nchg;
-            D.tile_nwide[tile] = s_nwide;
-            if (s_nwide) B.wide[atomicAdd(&B.ctrl->nwide, 1u)] = e;
             uint32_t r0 = 0, r1 = 0, r3 = 0;
             for (int w = 0; w < kBlock / 64; ++w) {
                 r0 += s_red[0][w];
@@ -1533,63 +1449,6 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             // changed cells: every record (born, solidified) and every lava cell whose temperature changed
             if (nchg + r3) atomicAdd(&row->changed, static_cast<unsigned long long>(nchg + r3));
             atomicAdd(&row->staged, 1ULL);
-        }
-    }
-}
-
-// AIR cells whose candidates carry more than one flowId (where flows meet): one block per tile that has such
-// cells; the full vote over private memory, then the record and counters the rules kernel left to this
-// kernel.  Reads the start-of-step state only, so it runs between the rules and the apply launches.
-__global__ void __launch_bounds__(kBlock) ca_wide_kernel(BatchDev B) {
-    __shared__ uint32_t s_cell[kTileCells];
-    const uint32_t n = B.ctrl->nwide;
-    for (uint32_t s = blockIdx.x; s < n; s += gridDim.x) {
-        const uint32_t e = __builtin_amdgcn_readfirstlane(B.wide[s]);
-        const CaSim& sim = g_sims[B.sim_base + (e >> kTileBits)];
-        const CaParams& P = sim.P;
-        const CaDev& D = sim.D;
-        const uint32_t tile = e & kTileMask;
-        Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
-        const uint32_t nw = D.tile_nwide[tile];
-        __syncthreads();  // the previous tile's cells are read
-        for (uint32_t j = threadIdx.x; j < nw; j += kBlock) s_cell[j] = rec[kTileCells - 1 - j].idx;
-        __syncthreads();  // born records may now reuse those slots
-        const int32_t t = D.ctrl->steps;
-        const uint64_t prefix = hash_prefix(P.seed, static_cast<uint64_t>(t));
-        const uint16_t stamp = static_cast<uint16_t>(t < 65535 ? t : 65535);
-        unsigned long long born = 0;
-        for (uint32_t j = threadIdx.x; j < nw; j += kBlock) {
-            const uint32_t i = s_cell[j];
-            const int x = static_cast<int>(i % static_cast<uint32_t>(P.nx));
-            const uint32_t r = i / static_cast<uint32_t>(P.nx);
-            const int y = static_cast<int>(r % static_cast<uint32_t>(P.ny));
-            const int z = static_cast<int>(r / static_cast<uint32_t>(P.ny));
-            const uint64_t h_index = splitmix64(prefix ^ i);
-            uint32_t mask = P.nb.mask[0];
-            if (P.nb.variants > 1)
-                mask = nb_mask(P.nb, mod_small(splitmix64(h_index ^ kSaltNeighborhood),
-                                               static_cast<uint32_t>(P.nb.variants)));
-            VoteWide v;
-            if (P.classic)
-                gather_classic(P, D, x, y, z, mask, v);
-            else
-                gather_lava(P, D, D.temp[t & 1], x, y, z, mask, v);
-            if (v.n == 0) continue;
-            int32_t wf = 0;
-            float wt = 0.0f;
-            v.winner(h_index, wf, wt);
-            if (!P.classic && wt < P.solidus) continue;
-            // slots below the tile's other records: records + born <= candidates <= kTileCells
-            rec[atomicAdd(&D.tile_nchg[tile], 1u)] = Change{i, wf, P.classic ? 0.0f : wt, stamp, kLava, kKindBorn};
-            if (!P.classic) D.temp[(t + 1) & 1][i] = wt;  // the new lava cell's temperature in the next step
-            D.tile_tchg[tile] = static_cast<uint32_t>(t) + 1u;
-            ++born;
-        }
-        if (born) {
-            CaPart* row = &D.part[tile % kPartRows];
-            atomicAdd(&row->lava_after, born);
-            atomicAdd(&row->born, born);
-            atomicAdd(&row->changed, born);
         }
     }
 }
@@ -2120,8 +1979,8 @@ void rules_shape(int k, dim3 g, hipStream_t s, hipEvent_t a, hipEvent_t b, const
     }
 }
 
-// One step.  Kernel order: [ca_front] ca_descent (+ the previous step's finish when L.fin_prev) ca_rules ca_wide
-// ca_apply [ca_finish]; ca_front and ca_finish only when a member needs the scan.  Every kernel of a step reads the
+// One step.  Kernel order: [ca_front] ca_descent (+ the previous step's finish when L.fin_prev) ca_rules ca_apply
+// [ca_finish]; ca_front and ca_finish only when a member needs the scan.  Every kernel of a step reads the
 // list copy L.par.  Without lava mode there is no D(n) to compute, but ca_descent still runs (it clears the previous
 // copy's counts and does the pending finish).
 void ca_launch_step(const BatchDev& B, const StepLaunch& L, hipStream_t s, hipEvent_t* ev, bool* rec) {
@@ -2144,9 +2003,8 @@ void ca_launch_step(const BatchDev& B, const StepLaunch& L, hipStream_t s, hipEv
         else
             rules_shape<int32_t>(shapes[i], gt, s, E.start(2, i == 0), E.stop(2, i == ns - 1), B, L.par);
     }
-    launch(ca_wide_kernel, dim3(kWideBlocks), s, E.start(3, true), E.stop(3, true), B);
-    launch(ca_apply_kernel, dim3((gt.x + 3) / 4), s, E.start(4, true), E.stop(4, true), B, L.par);
-    if (L.scan) launch(ca_finish_kernel, dim3(B.nslots), s, E.start(5, true), E.stop(5, true), B);
+    launch(ca_apply_kernel, dim3((gt.x + 3) / 4), s, E.start(3, true), E.stop(3, true), B, L.par);
+    if (L.scan) launch(ca_finish_kernel, dim3(B.nslots), s, E.start(4, true), E.stop(4, true), B);
 }
 
 void ca_launch_finish(const BatchDev& B, hipStream_t s, hipEvent_t* ev) {

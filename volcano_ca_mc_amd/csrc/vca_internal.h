@@ -56,7 +56,6 @@ struct NbTable {
 constexpr int kTileX = 64;
 constexpr int kTileCells = 1024;
 constexpr int kBlock = 256;
-constexpr int kWideBlocks = 512;  // grid of ca_wide (blocks past the wide-tile count exit at once)
 
 struct CaParams {
     int32_t nx, ny, nz;
@@ -98,7 +97,7 @@ struct Change {
     uint8_t type;
     uint8_t kind;                 // kSolidify, kBorn (temperature changes of lava cells are written directly)
 };
-constexpr uint8_t kKindSolidify = 1, kKindBorn = 2, kKindWide = 3;
+constexpr uint8_t kKindSolidify = 1, kKindBorn = 2;
 
 struct CaCtrl {
     int32_t steps;
@@ -145,7 +144,6 @@ struct CaDev {
                                   // pushes the next step's tiles; the stamp lists each tile once)
     Change* changes;              // kTileCells record slots per tile: tile t owns slots t*kTileCells ..
     uint32_t* tile_nchg;          // records of each visited tile this step (from slot 0 up)
-    uint32_t* tile_nwide;         // cells left to ca_wide (kKindWide records from slot kTileCells-1 down)
     CaPart* part;                 // kPartRows rows
     CaCtrl* ctrl;
     unsigned long long* hist_active;
@@ -182,8 +180,7 @@ constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 // read.
 struct BatchCtrl {
     uint32_t nactive[2][kShapes]; // staged tiles per step parity and shape
-    uint32_t nwide;               // tiles with cells for ca_wide
-    uint32_t reserved[5];
+    uint32_t reserved[6];
 };
 
 // Simulations of every batch live in one constant-memory table on each device (read by the step kernels through
@@ -198,7 +195,6 @@ struct BatchDev {
     uint32_t nslots;
     uint32_t cap;                 // capacity of each list
     uint64_t* active;             // 2 x kShapes lists of cap entries (parity-major): step << 32 | entry
-    uint32_t* wide;               // cap entries
     BatchCtrl* ctrl;
 };
 
@@ -246,7 +242,7 @@ struct StepLaunch {
 };
 // Kernel timing: kCaKernels kernels in launch order; ev[2k], ev[2k + 1] (when non-null) are recorded by kernel k's
 // own dispatch (start and end); a kernel not launched in this step leaves its pair unrecorded (flag false).
-constexpr int kCaKernels = 6;     // front, descent, rules, wide, apply, finish
+constexpr int kCaKernels = 5;     // front, descent, rules, apply, finish
 void ca_launch_step(const BatchDev& B, const StepLaunch& L, hipStream_t s, hipEvent_t* ev, bool* recorded);
 // The finish of the last step enqueued (after a run of steps whose finish is done by the next step's first kernel);
 // ev (nullable): the start / end events of the launch.
