@@ -806,25 +806,29 @@ __device__ int drop_depth_lds(const CaDev& D, const uint8_t* s_type, int c, int 
 // D(n) of every lava cell of the tiles holding lava (LavaCA.hpp:421-429, 877-888), from the LDS-staged tile:
 // hasSupport of the cell, then the deepest dropDepth over its AIR side neighbours (union of the variants).
 template <int TY, int TZ>
-// The first kernel of a step: it clears the counts of the list copy the previous step read,
-// does the previous step's finish when fin != 0 (blocks 0 .. nslots-1, one slot each), then computes D(n) of the
-// staged tiles of parity par that hold lava.
-__global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shape, int par, int fin) {
+// The first kernel of a step: it clears the counts of the list copy the previous step read, does the previous step's
+// finish in its first `fin_blocks` blocks (one slot each; 0 when the finish is not pending) and computes D(n) of the
+// staged tiles of parity par that hold lava in the blocks after them, alongside the finish.
+__global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shape, int par, uint32_t fin_blocks) {
     using H = Halo<TY, TZ>;
     __shared__ HaloTypes<TY, TZ> S;
     __shared__ int32_t s_spill[H::LXY];
     __shared__ uint16_t s_list[kTileCells];
     __shared__ uint32_t s_n;
     const uint8_t* s_type = S.type;
-    if (blockIdx.x == 0 && threadIdx.x < kShapes) {
+    if (blockIdx.x == fin_blocks && threadIdx.x < kShapes) {
         // every kernel of the previous step is done with its list copy; the rules kernel of this step pushes the next
         // step's tiles there
         B.ctrl->nactive[par ^ 1][threadIdx.x] = 0u;
     }
-    if (fin && blockIdx.x < B.nslots) finish_slot(B, blockIdx.x);
+    if (blockIdx.x < fin_blocks) {
+        finish_slot(B, blockIdx.x);
+        return;
+    }
+    const uint32_t tile_blocks = gridDim.x - fin_blocks;
     const uint32_t n = B.ctrl->nactive[par][shape];
     const uint64_t* tiles = B.active + (par * kShapes + shape) * static_cast<size_t>(B.cap);
-    for (uint32_t s = blockIdx.x; s < n; s += gridDim.x) {
+    for (uint32_t s = blockIdx.x - fin_blocks; s < n; s += tile_blocks) {
         // uniform: one entry per block; low word slot << kTileBits | tile, high word the simulation's step
         const uint64_t ent = tiles[s];
         const uint32_t e = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ent));
@@ -1129,8 +1133,13 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         float* tnext = D.temp[(t + 1) & 1];
         const TileCoord o = tile_origin(P, tile);
         __syncthreads();  // the previous tile's LDS reads are done
-        // an entry of a slot that left the batch, or of a simulation that stopped after the entry was pushed
-        if (!sim.running || __builtin_amdgcn_readfirstlane(D.ctrl->stop)) continue;
+        // an entry of a slot that left the batch, or of a simulation that stopped after the entry was pushed (its
+        // tile gets no records this step: ca_apply then only checks the slot)
+        if (!sim.running) continue;
+        if (__builtin_amdgcn_readfirstlane(D.ctrl->stop)) {
+            if (tid == 0) D.tile_nchg[tile] = 0u;
+            continue;
+        }
         int own_air = 0;
         // lane j: base of halo row j; gx: this lane's column (x == nx is the wrapped x = 0 of a periodic partial
         // tile, -1 for no cell)
@@ -1467,8 +1476,9 @@ __global__ void __launch_bounds__(kBlock) ca_apply_kernel(BatchDev B, int par) {
             const uint32_t e = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(ent))));
             const uint32_t tstep = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(ent >> 32))));
             const CaSim& sim = g_sims[B.sim_base + (e >> kTileBits)];
-            // entries the rules kernel skipped (slot left the batch, simulation stopped) have no records of this step
-            if (!sim.running || __builtin_amdgcn_readfirstlane(sim.D.ctrl->stop)) continue;
+            // entries of a slot that left the batch: the rules kernel skipped them (a stopped simulation's tiles have
+            // their record counts zeroed by the rules kernel)
+            if (!sim.running) continue;
             const CaDev& D = sim.D;
             const uint32_t tile = e & kTileMask;
             const uint32_t n = D.tile_nchg[tile];
@@ -1952,20 +1962,22 @@ struct StepEvents {
 };
 
 template <int TY, int TZ>
-void launch_descent(int shape, dim3 g, hipStream_t s, hipEvent_t a, hipEvent_t b, const BatchDev& B, int par, int fin) {
-    launch(ca_descent_kernel<TY, TZ>, g, s, a, b, B, shape, par, fin);
+void launch_descent(int shape, dim3 g, hipStream_t s, hipEvent_t a, hipEvent_t b, const BatchDev& B, int par,
+                    uint32_t fin_blocks) {
+    launch(ca_descent_kernel<TY, TZ>, g, s, a, b, B, shape, par, fin_blocks);
 }
 template <int TY, int TZ, class F>
 void launch_rules(int shape, dim3 g, hipStream_t s, hipEvent_t a, hipEvent_t b, const BatchDev& B, int par) {
     launch(ca_rules_kernel<TY, TZ, F>, g, s, a, b, B, shape, par);
 }
-void descent_shape(int k, dim3 g, hipStream_t s, hipEvent_t a, hipEvent_t b, const BatchDev& B, int par, int fin) {
+void descent_shape(int k, dim3 g, hipStream_t s, hipEvent_t a, hipEvent_t b, const BatchDev& B, int par,
+                   uint32_t fin_blocks) {
     switch (k) {
-        case 0: launch_descent<4, 4>(0, g, s, a, b, B, par, fin); break;
-        case 1: launch_descent<2, 8>(1, g, s, a, b, B, par, fin); break;
-        case 2: launch_descent<8, 2>(2, g, s, a, b, B, par, fin); break;
-        case 3: launch_descent<1, 16>(3, g, s, a, b, B, par, fin); break;
-        default: launch_descent<16, 1>(4, g, s, a, b, B, par, fin); break;
+        case 0: launch_descent<4, 4>(0, g, s, a, b, B, par, fin_blocks); break;
+        case 1: launch_descent<2, 8>(1, g, s, a, b, B, par, fin_blocks); break;
+        case 2: launch_descent<8, 2>(2, g, s, a, b, B, par, fin_blocks); break;
+        case 3: launch_descent<1, 16>(3, g, s, a, b, B, par, fin_blocks); break;
+        default: launch_descent<16, 1>(4, g, s, a, b, B, par, fin_blocks); break;
     }
 }
 template <class F>
@@ -1990,12 +2002,11 @@ void ca_launch_step(const BatchDev& B, const StepLaunch& L, hipStream_t s, hipEv
     int shapes[kShapes], ns = 0;
     for (int k = 0; k < kShapes; ++k)
         if ((L.shapes >> k) & 1u) shapes[ns++] = k;
-    // the first descent launch clears the counts and does the finish (blocks 0 .. nslots - 1): it needs that many
-    const dim3 g0(std::max<unsigned>(gt.x, B.nslots));
+    // the first descent launch clears the counts and, with nslots blocks before the tile blocks, does the finish
     for (int i = 0; i < ns; ++i) {
-        const bool lead = i == 0;
-        descent_shape(shapes[i], lead ? g0 : gt, s, E.start(1, lead), E.stop(1, i == ns - 1), B, L.par,
-                      lead && L.fin_prev ? 1 : 0);
+        const uint32_t fin_blocks = i == 0 && L.fin_prev ? B.nslots : 0u;
+        descent_shape(shapes[i], dim3(gt.x + fin_blocks), s, E.start(1, i == 0), E.stop(1, i == ns - 1), B, L.par,
+                      fin_blocks);
     }
     for (int i = 0; i < ns; ++i) {
         if (L.flow16)
