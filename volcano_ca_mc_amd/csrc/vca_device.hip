@@ -1100,6 +1100,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
     __shared__ uint32_t s_nchg, s_nlist, s_nair;
     __shared__ uint32_t s_red[4][kBlock / 64];
     __shared__ uint32_t s_wflags[2];  // per-wave flag bytes: [0] staging, [1] end of tile
+    __shared__ uint32_t s_bnd0[kBlock / 64];  // per wave: directions with LAVA on the tile's boundary at the step's start
     const uint64_t* tiles = B.active + (par * kShapes + shape) * static_cast<size_t>(B.cap);
     // the first entry is read together with the list length (blockIdx.x < gridDim.x <= cap: in bounds)
     uint64_t next = tiles[blockIdx.x];
@@ -1241,6 +1242,9 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                 }
                 // LAVA candidates fill the list from the front, AIR ones from the back: each wave of the
                 // candidate loop then runs one rule, not both
+                const int xm0 = min(kTileX - 1, P.nx - 1 - o.x0), ym0 = min(TY - 1, P.ny - 1 - o.y0),
+                          zm0 = min(TZ - 1, P.nz - 1 - o.z0);
+                uint32_t bnd0 = 0;  // directions in which the tile holds LAVA on its boundary at the step's start
                 bool cl[4], ca[4];
                 uint32_t sl[4], sa[4];
 #pragma unroll
@@ -1254,7 +1258,10 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                     cl[k] = ty == kLava;
                     ca[k] = ty == kAir && ((near >> lx) & 1u);
                     if (ty == kAir && !ca[k]) tile_air = 1;
+                    if (cl[k]) bnd0 |= boundary_dirs(lx, row % TY, row / TY, xm0, ym0, zm0);
                 }
+                bnd0 = wave_or(bnd0);
+                if (lx == 0) s_bnd0[w] = bnd0;
                 wave_append_k(cl, &s_nlist, sl);
                 wave_append_k(ca, &s_nair, sa);
 #pragma unroll
@@ -1292,6 +1299,16 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         __syncthreads();  // list and lava data of the tile are in LDS
         // candidate slots: LAVA cells at [0, nlava), AIR cells from the next wave boundary on (list order reversed)
         const uint32_t nlava = s_nlist, abase = (nlava + 63u) & ~63u, nslots = abase + s_nair;
+        // Lava front: the next step's tiles next to the lava the tile holds at the step's start (and the tile itself)
+        // are listed now, by the last wave (the one with the fewest candidates), so that the list atomics overlap the
+        // candidate loop; the end of the tile lists only what the step's new lava adds (a tile listed but left with
+        // no lava in its halo settles in the next step).
+        const uint32_t dirs0 = (s_bnd0[0] | s_bnd0[1] | s_bnd0[2] | s_bnd0[3]) | (nlava ? (1u << 13) : 0u);
+        if (sim.push && dirs0 && (tid >> 6) == kBlock / 64 - 1) {
+            const int tx = o.x0 / kTileX, tyy = o.y0 / TY, tz = o.z0 / TZ;
+            push_next(P, D, B, tx, tyy, tz, static_cast<int>(tid & 63u), dirs0, e >> kTileBits, static_cast<uint32_t>(t) + 1u,
+                      par ^ 1, shape);
+        }
         // directions in which the tile holds lava on its boundary after the step (mark_halo): cells that stay or
         // become LAVA
         const int xm = min(kTileX - 1, P.nx - 1 - o.x0), ym = min(TY - 1, P.ny - 1 - o.y0),
@@ -1431,8 +1448,9 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             const int tx = o.x0 / kTileX, tyy = o.y0 / TY, tz = o.z0 / TZ;
             const uint32_t bnd_all = s_red[2][0] | s_red[2][1] | s_red[2][2] | s_red[2][3];
             if (sim.push) {
-                if (tid < 64)
-                    push_next(P, D, B, tx, tyy, tz, static_cast<int>(tid), bnd_all | (1u << 13), e >> kTileBits,
+                const uint32_t late = (bnd_all | (1u << 13)) & ~dirs0;
+                if (late && tid < 64)
+                    push_next(P, D, B, tx, tyy, tz, static_cast<int>(tid), late, e >> kTileBits,
                               static_cast<uint32_t>(t) + 1u, par ^ 1, shape);
             } else {
                 mark_near(P, D, tx, tyy, tz, static_cast<int>(tid));
