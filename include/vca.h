@@ -95,6 +95,15 @@ int vca_ca_destroy(vca_ca* ca);
 /* LavaCA::step x n (LavaCA.hpp:230-256): enqueues up to max_new_steps synchronous steps; steps after a
  * stop rule fired are no-ops on the device.  Blocks until done, returns the step count and stop reason. */
 int vca_ca_run(vca_ca* ca, int32_t max_new_steps, int32_t* steps_done, int32_t* stop_reason);
+/* vca_ca_run without any device allocation.  No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid it).  The simulation (created
+ * beforehand by vca_ca_create) is advanced exactly as vca_ca_run does; the batch-of-one state vca_ca_run allocates on
+ * first use (list counts, tile owners, the two copies of the tile lists) lives instead in a static device array of
+ * 16 MiB per device, set up by vca_ctx_create: grids of at most 199,716 tiles of 1024 cells, about 204 M cells
+ * (VCA_ERR_ARG beyond, with the limit in the message).  Calls on one device are serialised by a lock; the array keeps the last call's
+ * contents and is re-initialised by every call.  VCA_ERR_STATE when the simulation is in a batch or was advanced by
+ * vca_ca_run / vca_ca_enqueue before (their private batch holds its lists). */
+int vca_ca_run_noalloc(vca_ca* ca, int32_t max_new_steps, int32_t* steps_done, int32_t* stop_reason);
 /* Enqueue only (no host wait); pair with vca_ca_status. */
 int vca_ca_enqueue(vca_ca* ca, int32_t new_steps);
 /* LavaCA::steps / stopReason (LavaCA.hpp:262-264); synchronises the context stream. */

@@ -1,0 +1,72 @@
+"""vca_ca_run_noalloc (no device allocation: its tile lists in the library's static device array) against vca_ca_run,
+through the C ABI: two calls in a row, the second with a smaller grid, each bit-exact against vca_ca_run on a second
+handle with the same inputs (every state array, the step count, the stop reason, both histories); the state errors and
+the size limit."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from volcano_ca_mc_amd import native
+    c = native.Context(0)
+    yield c
+    c.close()
+
+
+def _make(ctx, name):
+    from volcano_ca_mc_amd import native as N
+    d = os.path.join(GOLD, "cases", name)
+    with open(os.path.join(d, "config.json")) as f:
+        cfg = O.merged(json.load(f))
+    hm = O.terrain_for(cfg, d)
+    per = cfg["boundary"] == "periodic"
+    spill = O.fill_depressions(hm, per, per)
+    vents = O.vents_for(cfg, hm)
+    p = O.ca_params(cfg)
+    params = N.CaParams(seed=p.seed, max_steps=p.max_steps, eruption_steps=p.eruption_steps, erupt_temp=p.erupt_temp,
+                        solidus_temp=p.solidus_temp, cooling_ambient=p.cooling_ambient, cooling_lateral=p.cooling_lateral,
+                        cooling_fall=p.cooling_fall, ponding=bool(p.ponding))
+    s = cfg["space"]
+    return lambda: N.LavaCA(ctx, (s["nx"], s["ny"], s["nz"]), params, cfg["neighborhood"], per, hm, spill,
+                            vents.tolist())
+
+
+@pytest.mark.parametrize("names", [("scen_caldera_3d", "lava2d_moore_absorbing")])
+def test_noalloc_run_equals_run(ctx, names):
+    for name in names:  # two calls in a row; the second grid is smaller
+        print(f"vca_ca_run_noalloc case {name}", flush=True)
+        make = _make(ctx, name)
+        ref, got = make(), make()
+        assert ref.run() == got.run(noalloc=True)
+        a, b = ref.result(), got.result()
+        for k in ("type", "flow", "solid", "arrival"):
+            assert np.array_equal(a[k], b[k]), (name, k)
+        assert np.array_equal(a["temp"].view(np.uint32), b["temp"].view(np.uint32)), name
+        assert ref.histories() == got.histories(), name
+        ref.close()
+        got.close()
+
+
+def test_noalloc_state_errors_and_size_limit(ctx):
+    from volcano_ca_mc_amd import native as N
+    make = _make(ctx, "lava2d_von_neumann_absorbing")
+    ca = make()
+    ca.run(5)  # advanced by vca_ca_run: its private batch holds the lists
+    with pytest.raises(RuntimeError, match="libvca error -4"):
+        ca.run(5, noalloc=True)
+    ca.close()
+    # a grid of 200,704 tiles (classic x-y plane 16384 x 12544, tiles of 64 x 16): beyond the array's 199,716
+    big = N.LavaCA(ctx, (16384, 12544, 1), N.CaParams(seed=1, classic=True, max_steps=3), "moore", False, None, None,
+                   [[5, 5, 0, 1]])
+    with pytest.raises(RuntimeError, match="at most 199716 tiles"):
+        big.run(3, noalloc=True)
+    big.close()

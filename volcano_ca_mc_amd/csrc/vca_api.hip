@@ -257,6 +257,10 @@ struct vca_ctx {
 
 struct Batch;
 
+namespace {
+void noalloc_setup(int device);  // vca_ca_run_noalloc's per-device set-up (below)
+}  // namespace
+
 // A region of a simulation's single device allocation.
 struct Span {
     void* p = nullptr;
@@ -389,6 +393,7 @@ struct Batch {
     bool fin_prev = false;        // the last step's finish is still to be done (never true between enqueues)
     SlotMask purge{};             // slots whose simulation left while the pending list copy may hold its entries
     bool purge_any = false;
+    bool external = false;        // lists and counts in caller-provided device memory (vca_ca_run_noalloc)
 
     void init(vca_ctx* c, int slots, uint32_t tiles) {
         ctx = c;
@@ -420,6 +425,54 @@ struct Batch {
         B.active = active.as<uint64_t>();
         B.ctrl = ctrl.as<BatchCtrl>();
         grid_tiles = static_cast<int>(std::min<uint32_t>(cap, 8192u));
+    }
+    // A batch of one slot over device memory the library does not allocate: counts, tile owners and the two list copies
+    // carved out of `base` (bytes), for as many tiles as fit; the simulation table entry `entry` is the caller's.
+    // Nothing is allocated on the device; the host vectors get their final sizes here.
+    void init_external(int device, char* base, size_t bytes, uint32_t entry) {
+        external = true;
+        max_slots = 1;
+        sim_base = entry;
+        has_table = false;
+        Layout L;
+        const size_t o_ctrl = L.add(sizeof(BatchCtrl)), o_base = L.add(4), fixed = L.total;
+        // per tile: its owner (4 B) and its entries in both copies of every shape's list (2 x kShapes x 8 B), each
+        // region 256-byte aligned
+        cap = static_cast<uint32_t>(std::min<size_t>((bytes - fixed - 512) / (4 + 2 * kShapes * 8), kTileMask));
+        const size_t o_owner = L.add(static_cast<size_t>(cap) * 4);
+        const size_t o_active = L.add(static_cast<size_t>(cap) * kShapes * 2 * 8);
+        if (L.total > bytes) throw VcaError(VCA_ERR_ARG, "scratch layout exceeds its array");
+        host_sims.assign(1, CaSim{});
+        members.assign(1, nullptr);
+        bases.assign(1, 0);
+        free_ranges.clear();
+        free_ranges.reserve(4);
+        hwm = 0;
+        par = 0;
+        fin_prev = false;
+        B.sim_base = sim_base;
+        B.owner = reinterpret_cast<uint32_t*>(base + o_owner);
+        B.tile_base = reinterpret_cast<uint32_t*>(base + o_base);
+        B.ntiles = 0;
+        B.nslots = 1;
+        B.cap = cap;
+        B.active = reinterpret_cast<uint64_t*>(base + o_active);
+        B.ctrl = reinterpret_cast<BatchCtrl*>(base + o_ctrl);
+        grid_tiles = static_cast<int>(std::min<uint32_t>(cap, 8192u));
+        (void)device;
+    }
+    // Ready for the next simulation (external batch): counts cleared by a kernel, host state reset.
+    void reset_external(vca_ctx* c) {
+        ctx = c;
+        fill_u32(reinterpret_cast<uint32_t*>(B.ctrl), 0u, sizeof(BatchCtrl) / 4, ctx->stream);
+        members[0] = nullptr;
+        free_ranges.clear();
+        hwm = 0;
+        par = 0;
+        fin_prev = false;
+        purge = SlotMask{};
+        purge_any = false;
+        steps_per_graph = 0;
     }
     ~Batch() {
         drop_graph();
@@ -528,8 +581,8 @@ struct Batch {
         // entries than the batch has tiles)
         flush_purge();
         ca_write_sim(sim_base + static_cast<uint32_t>(slot), &sim, s);
-        check(hipMemcpyAsync(tile_base.as<uint32_t>() + slot, &base, 4, hipMemcpyHostToDevice, s), "batch add");
-        ca_launch_owner(owner.as<uint32_t>(), base, ca->P.ntiles, static_cast<uint32_t>(slot), s);
+        fill_u32(B.tile_base + slot, base, 1, s);
+        ca_launch_owner(B.owner, base, ca->P.ntiles, static_cast<uint32_t>(slot), s);
         // a lava-front simulation's first tiles (later steps list their own)
         if (sim.push) ca_launch_seed(B, static_cast<uint32_t>(slot), ca->P.ntiles, par, s);
         check(hipGetLastError(), "batch add");
@@ -544,7 +597,7 @@ struct Batch {
         host_sims[static_cast<size_t>(slot)].running = 0;
         ca_write_sim(sim_base + static_cast<uint32_t>(slot), &host_sims[static_cast<size_t>(slot)], s);
         const uint32_t base = bases[static_cast<size_t>(slot)];
-        ca_launch_owner(owner.as<uint32_t>(), base, ca->P.ntiles, kNoSlot, s);
+        ca_launch_owner(B.owner, base, ca->P.ntiles, kNoSlot, s);
         check(hipGetLastError(), "batch remove");
         give_range(base, ca->P.ntiles);
         members[static_cast<size_t>(slot)] = nullptr;
@@ -718,6 +771,8 @@ extern "C" int vca_ctx_create(int device, void* stream, vca_ctx** out) {
         check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
         if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
             throw VcaError(VCA_ERR_DEVICE, std::string("libvca is built for gfx950, device is ") + prop.gcnArchName);
+        check(hipSetDevice(device), "hipSetDevice");
+        noalloc_setup(device);
         auto* c = new vca_ctx();
         c->device = device;
         c->set_device();
@@ -1432,6 +1487,94 @@ extern "C" int vca_ca_tiles(vca_ca* ca, uint64_t* tiles) {
     return guarded([&] {
         require(ca && tiles, "NULL argument");
         *tiles = ca->P.ntiles;
+    });
+}
+
+// ===================================================================================================
+// vca_ca_run without device allocation
+// ===================================================================================================
+// The device memory of vca_ca_run_noalloc (its batch of one: list counts, tile owners, both copies of the tile lists)
+// is this array, one copy per device, held until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress gives for the current device (kept per device by noalloc_setup).
+constexpr size_t kScratchBytes = static_cast<size_t>(16) << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+struct NoAlloc {
+    void* ptr = nullptr;          // this device's copy of `scratch`
+    Batch* batch = nullptr;       // the batch of one over it (never freed)
+    std::mutex mu;                // one call at a time per device (the calls share the array)
+};
+std::mutex g_noalloc_mu;
+NoAlloc* g_noalloc[kMaxDevices] = {};
+
+// Once per device, with that device current (vca_ctx_create): the array's address on the device, a simulation table
+// entry of its own and the host side of its batch -- everything vca_ca_run_noalloc would otherwise have to allocate.
+void noalloc_setup(int device) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (device < 0 || device >= kMaxDevices || g_noalloc[device]) return;
+    void* p = nullptr;
+    check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    auto* na = new NoAlloc();
+    na->ptr = p;
+    na->batch = new Batch();
+    na->batch->init_external(device, static_cast<char*>(p), kScratchBytes, SimTable::take(device, 1));
+    g_noalloc[device] = na;
+}
+}  // namespace
+
+extern "C" int vca_ca_run_noalloc(vca_ca* ca, int32_t max_new_steps, int32_t* steps_done, int32_t* stop_reason) {
+    return guarded([&] {
+        require(ca, "ca == NULL");
+        require(max_new_steps >= 0, "max_new_steps < 0");
+        if (ca->member) throw VcaError(VCA_ERR_STATE, "the simulation is in a batch: advance it with vca_batch_*");
+        if (ca->own)
+            throw VcaError(VCA_ERR_STATE, "the simulation was advanced by vca_ca_run / vca_ca_enqueue, whose private batch "
+                                          "holds its tile lists: continue with those");
+        vca_ctx* ctx = ca->ctx;
+        ctx->set_device();
+        NoAlloc* na = ctx->device < kMaxDevices ? g_noalloc[ctx->device] : nullptr;
+        if (!na) throw VcaError(VCA_ERR_STATE, "no static device array set up for this device");
+        Batch& b = *na->batch;
+        if (ca->P.ntiles > b.cap)
+            throw VcaError(VCA_ERR_ARG, "vca_ca_run_noalloc: the grid has " + std::to_string(ca->P.ntiles) +
+                                            " tiles of 1024 cells; the static device array (" +
+                                            std::to_string(kScratchBytes >> 20) + " MiB) holds the lists of at most " +
+                                            std::to_string(b.cap) + " tiles");
+        std::lock_guard<std::mutex> lock(na->mu);
+        // timed launches take events from the context's pool, which may create events: not here
+        const bool timing = ctx->timing;
+        ctx->timing = false;
+        auto leave = [&] {
+            if (b.members[0]) {
+                b.host_sims[0].running = 0;
+                ca_write_sim(b.sim_base, &b.host_sims[0], ctx->stream);
+                b.members[0] = nullptr;
+            }
+            (void)hipStreamSynchronize(ctx->stream);
+            ctx->timing = timing;
+        };
+        try {
+            b.reset_external(ctx);
+            b.attach(ca);
+            int32_t steps = 0, stop = 0;
+            ca_status_sync(ca, &steps, &stop);
+            int remaining = max_new_steps;
+            while (remaining > 0 && stop == VCA_RUNNING) {
+                const int chunk = std::min(remaining, 128);
+                b.enqueue(chunk);
+                check(hipGetLastError(), "ca launch");
+                remaining -= chunk;
+                ca_status_sync(ca, &steps, &stop);
+            }
+            leave();
+            if (steps_done) *steps_done = steps;
+            if (stop_reason) *stop_reason = stop;
+        } catch (...) {
+            leave();
+            throw;
+        }
     });
 }
 

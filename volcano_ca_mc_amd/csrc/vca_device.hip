@@ -2057,6 +2057,17 @@ void ca_write_sim(uint32_t index, const CaSim* sim, hipStream_t s) {
                                  hipMemcpyHostToDevice, s);
 }
 
+__global__ void __launch_bounds__(kBlock) fill_u32_kernel(uint32_t* p, uint32_t v, uint64_t n) {
+    for (uint64_t j = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; j < n;
+         j += static_cast<uint64_t>(gridDim.x) * kBlock)
+        p[j] = v;
+}
+
+void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n, 4096)), dim3(kBlock), 0, s, p, v, n);
+}
+
 void ca_launch_owner(uint32_t* owner, uint32_t base, uint32_t n, uint32_t slot, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(ca_owner_kernel, dim3(grid_for(n, 4096)), dim3(kBlock), 0, s, owner, base, n, slot);

@@ -256,6 +256,8 @@ struct SlotMask {
     uint32_t bits[kMaxSlots / 32];
 };
 void ca_launch_purge(const BatchDev& B, const SlotMask& slots, int par, hipStream_t s);
+// p[0 .. n) = v (a kernel, not a memset: usable where the caller may capture the stream)
+void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 // owner[base .. base + n) = slot
 void ca_launch_owner(uint32_t* owner, uint32_t base, uint32_t n, uint32_t slot, hipStream_t s);
 // out[2*s] = steps, out[2*s+1] = stop (-1 when slot s is not running) for s < nslots.

@@ -82,6 +82,7 @@ def lib() -> C.CDLL:
         "vca_ctx_set_timing": [vp, i32], "vca_ctx_kernel_time": [vp, C.c_char_p, C.POINTER(KernelTime)],
         "vca_ca_create": [vp, C.POINTER(CaDesc), vp, vp, vp, i32, C.POINTER(vp)], "vca_ca_destroy": [vp],
         "vca_ca_run": [vp, i32, C.POINTER(i32), C.POINTER(i32)], "vca_ca_enqueue": [vp, i32],
+        "vca_ca_run_noalloc": [vp, i32, C.POINTER(i32), C.POINTER(i32)],
         "vca_ca_status": [vp, C.POINTER(i32), C.POINTER(i32)],
         "vca_ca_histories": [vp, vp, vp, i32, C.POINTER(i32)], "vca_ca_download": [vp, vp, vp, vp, vp, vp],
         "vca_ca_work": [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(i32)],
@@ -105,7 +106,7 @@ def lib() -> C.CDLL:
 
 EXPORTED = ("vca_version", "vca_last_error", "vca_ctx_create", "vca_ctx_destroy", "vca_ctx_synchronize",
             "vca_device_memory", "vca_ctx_set_timing", "vca_ctx_kernel_time", "vca_ca_create", "vca_ca_destroy",
-            "vca_ca_run", "vca_ca_enqueue", "vca_ca_status", "vca_ca_histories", "vca_ca_download", "vca_ca_work",
+            "vca_ca_run", "vca_ca_run_noalloc", "vca_ca_enqueue", "vca_ca_status", "vca_ca_histories", "vca_ca_download", "vca_ca_work",
             "vca_ca_footprint", "vca_ca_tiles", "vca_batch_create", "vca_batch_destroy", "vca_batch_add",
             "vca_batch_remove", "vca_batch_set_graph", "vca_batch_enqueue", "vca_batch_run", "vca_batch_info",
             "vca_batch_status", "vca_ca_collect", "vca_ca_flow_stats", "vca_ca_flow_points", "vca_ca_autotune",
@@ -233,9 +234,12 @@ class LavaCA:
         except Exception:
             pass
 
-    def run(self, max_new_steps: int = 1 << 30):
+    def run(self, max_new_steps: int = 1 << 30, noalloc: bool = False):
+        """LavaCA::run: steps until a stop rule fires or max_new_steps are done.  noalloc: through vca_ca_run_noalloc
+        (no device allocation; its lists in the library's static device array)."""
         steps, stop = C.c_int32(), C.c_int32()
-        _check(lib().vca_ca_run(self.h, min(max_new_steps, 2 ** 31 - 1), C.byref(steps), C.byref(stop)))
+        fn = lib().vca_ca_run_noalloc if noalloc else lib().vca_ca_run
+        _check(fn(self.h, min(max_new_steps, 2 ** 31 - 1), C.byref(steps), C.byref(stop)))
         return int(steps.value), int(stop.value)
 
     def step(self) -> bool:
