@@ -20,7 +20,11 @@ Printed: one JSON line.
              the kernel's own dispatch (every 10th step); traffic = HBM bytes per launch measured by rocprofv3
              counters (profiles/traffic.json).  step_roofline: 26 B x the cells of every visited tile over the
              whole step time (caSim / steps).
-  mc_roofline the colour kernel (mc_color): 2 * colours + 4 B per spin attempt; a launch handles one colour.
+  mc_roofline the MC sweep kernel: per sweep, every domain cell's list entry (4 B), mobility threshold (8 B, when
+             cooling mobility drives the trial skip) and spin written (spin bytes), and the whole domain's spins read
+             once per colour pass (colours x spin bytes) -- dom x (4 + 8 + spin_bytes x (colours + 1)) B per sweep;
+             over the sweep time of the persistent kernel (mc_sweeps: all sweeps of a statistics interval in one
+             launch) or of the per-colour launches (mc_color x colours).
   cpu_baseline the reference engine (oracle/_ref/engine, built from the reference sources) on every host core
              the process may use, on a bounded sample of the same workload.
   extra      the cone workload line, the 256-job batched mix (against the same jobs one at a time) and the
@@ -112,11 +116,11 @@ def cpu_baseline(kind: str, cfg_path: str, cells: int, tmp: str) -> dict | None:
                       f"sampled rate overstates the CPU's full-run rate"}
 
 
-def _issue(k: str, ms: float) -> dict:
+def _issue(workload: str, k: str, ms: float) -> dict:
     if not os.path.exists(PROFILE_ISSUE):
         return {"valu_issue_frac": None, "salu_issue_frac": None}
     with open(PROFILE_ISSUE) as f:
-        e = json.load(f).get(k)
+        e = json.load(f).get(workload, {}).get(k)
     if not e:
         return {"valu_issue_frac": None, "salu_issue_frac": None}
     return {"valu_issue_frac": round(e["valu"] / (ms / 1000.0) / ISSUE_PEAK_PER_S, 4),
@@ -180,23 +184,39 @@ def workload_line(kind: str, a, with_cpu: bool) -> dict:
                 "kernel": "ca_rules", "kernel_mean_ms": round(mean["ca_rules"], 5),
                 "timed_every_nth_step": TIMING_SAMPLE, "bytes_per_unit": f"{CA_BYTES_PER_CELL} B per staged cell",
                 "staged_cells_per_launch": round(staged), "algorithmic_bytes_per_launch": round(rules_bytes),
-                **_issue("ca_rules", mean["ca_rules"]),
+                **_issue(kind, "ca_rules", mean["ca_rules"]),
                 "kernel_mean_ms_all": {k: round(mean[k], 5) for k in step_kernels},
-                "launches_per_step": len(step_kernels)}
+                "launches_per_step": round(sum(ktot[k][1] for k in step_kernels) / max(1, ktot["ca_rules"][1]), 3)}
     step_roofline = {"achieved": round(step_ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(step_ach / HBM_PEAK_GBS, 4), "step_ms": round(step_ms, 5),
                      "visited_cells_per_step": round(visited),
                      "bytes_per_unit": f"{CA_BYTES_PER_CELL} B per cell of every visited tile, whole step"}
     mc_roofline = None
-    if "mc_color" in mean and dom:
-        # one launch = one colour pass over domain / colours cells; 2 * colours + 4 B per trial
-        mc_bytes = dom * (2 * colours + 4) / colours
-        mc_ach = mc_bytes / (mean["mc_color"] / 1000.0) / 1e9
-        mc_roofline = {"bound": "latency", "kernel": "mc_color", "kernel_mean_ms": round(mean["mc_color"], 5),
+    mst = r0["_stats"].get("mc", {})
+    if dom and ("mc_sweeps" in mean or "mc_color" in mean):
+        q = int(mst.get("Q", 2))
+        spin_bytes = 1 if q <= 255 else (2 if q <= 65535 else 4)
+        dyn = mst.get("mobility") == "cooling"
+        sweep_bytes = dom * (4 + (8 if dyn else 0) + spin_bytes * (colours + 1))
+        if "mc_sweeps" in mean:
+            kname = "mc_sweeps"
+            launches_per_job = ktot[kname][1] / len(runs)
+            sweeps_per_launch = mcs / launches_per_job
+            sweep_ms = mean[kname] / sweeps_per_launch
+        else:
+            kname = "mc_color"
+            sweeps_per_launch = 1.0 / colours
+            sweep_ms = mean[kname] * colours
+        mc_bytes = sweep_bytes * sweeps_per_launch
+        mc_ach = mc_bytes / (mean[kname] / 1000.0) / 1e9
+        mc_roofline = {"bound": "latency", "kernel": kname, "kernel_mean_ms": round(mean[kname], 5),
+                       "sweeps_per_launch": round(sweeps_per_launch, 4), "sweep_ms": round(sweep_ms, 5),
                        "achieved": round(mc_ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": round(mc_ach / HBM_PEAK_GBS, 4), "traffic": _traffic(kind, "mc_color"),
-                       "colours": colours, "bytes_per_unit": "2 * colours + 4 B per spin attempt",
-                       "algorithmic_bytes_per_launch": round(mc_bytes), **_issue("mc_color", mean["mc_color"])}
+                       "frac": round(mc_ach / HBM_PEAK_GBS, 4), "traffic": _traffic(kind, kname),
+                       "colours": colours, "spin_bytes": spin_bytes, "mobility_skip": dyn,
+                       "bytes_per_unit": f"{4 + (8 if dyn else 0) + spin_bytes * (colours + 1)} B per domain cell "
+                                         f"and sweep",
+                       "algorithmic_bytes_per_launch": round(mc_bytes), **_issue(kind, kname, mean[kname])}
     desc = {"caldera": f"caldera_3d {nx}x{ny}x{nz} random pentagonal, periodic x/y, 8 craters, maxSteps 2000, "
                        f"Potts Q128 kT0.5 100 sweeps (checkerboard)",
             "cone": f"cone_3d {nx}x{ny}x{nz} moore absorbing, 3 craters, maxSteps 1000, Potts Q64 kT0.5 200 sweeps",

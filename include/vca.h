@@ -228,8 +228,18 @@ int vca_mc_create(vca_ctx* ctx, const vca_mc_desc* desc, const uint8_t* domain, 
 int vca_mc_create_from_ca(vca_ctx* ctx, const vca_mc_desc* desc, vca_ca* ca, int32_t use_mobility,
                           double mobility_min, vca_mc** out);
 int vca_mc_destroy(vca_mc* mc);
-/* PottsMC::sweep x n (PottsMC.cpp:516-533); counts are cumulative since creation. */
+/* PottsMC::sweep x n (PottsMC.cpp:516-533); counts are cumulative since creation.  Checkerboard sweeps run one
+ * launch per colour pass unless vca_mc_set_sweep_mode asked for the persistent kernel. */
 int vca_mc_sweep(vca_mc* mc, int32_t n, vca_mc_counts* counts);
+/* How vca_mc_sweep launches checkerboard sweeps (no counterpart in the reference; for tests and tuning):
+ * VCA_MC_SWEEP_AUTO (default: one launch per colour pass), VCA_MC_SWEEP_PER_COLOUR (the same) or
+ * VCA_MC_SWEEP_PERSISTENT (all n sweeps of a call in ONE launch of at most one block per compute unit, a device-wide
+ * barrier between colour passes; slower than a launch per pass on a whole MI355X, whose eight XCDs each have their
+ * own L2).  Same results in every mode.  VCA_ERR_STATE for the random update. */
+#define VCA_MC_SWEEP_AUTO 0
+#define VCA_MC_SWEEP_PER_COLOUR 1
+#define VCA_MC_SWEEP_PERSISTENT 2
+int vca_mc_set_sweep_mode(vca_mc* mc, int32_t mode);
 int vca_mc_counts_get(vca_mc* mc, vca_mc_counts* counts);
 /* PottsMC::measure (PottsMC.cpp:535-553) on the device.  classes (2 entries: high, low) are filled
  * when the handle has mobility data and classes != NULL; high_cells gets the number of domain cells

@@ -182,6 +182,63 @@ def test_mc_matches_oracle(ctx, name):
         assert O.mc_result_bin(dims, gpu.crystals()) == blob
 
 
+@pytest.mark.parametrize("name", ["lava3d_moore_periodic", "scen_caldera_3d", "mcfull3d_moore_odd", "mcfull2d_vn_odd",
+                                  "lava2d_von_neumann_absorbing"])
+@pytest.mark.parametrize("mode", [1, 2])
+def test_mc_sweep_modes_match_oracle(ctx, name, mode):
+    """One launch per colour pass (mode 1) and one persistent launch per call (mode 2, a device-wide barrier between
+    passes), called for 1, 2, 3 ... sweeps at a time, over the whole space (up to 27 colours on odd periodic axes):
+    spins and counters equal the oracle's after every call."""
+    import ctypes as Cc
+    from volcano_ca_mc_amd import native as N
+    d, cfg = _inputs(name)
+    s = cfg["space"]
+    dims = (s["nx"], s["ny"], s["nz"])
+    mc = cfg["mc"]
+    periodic = cfg["boundary"] == "periodic"
+    per = [periodic and dims[0] > 1, periodic and dims[1] > 1, periodic and dims[2] > 1]
+    print(f"case {name} mode {mode}", flush=True)
+    params = O.McParams(*dims, O.NB_KINDS[cfg["neighborhood"]], (Cc.c_int * 3)(*map(int, per)), int(cfg["rngSeed"]),
+                        int(mc["Q"]), float(mc["J"]), float(mc["kT"]), 1)
+    orc = O.OracleMC(params, None, None)
+    gpu = N.PottsMC(ctx, dims, cfg["neighborhood"], per, int(cfg["rngSeed"]), int(mc["Q"]), float(mc["J"]),
+                    float(mc["kT"]), True, None, None, mobility_dynamics=False)
+    gpu.set_sweep_mode(mode)
+    done, k = 0, 1
+    while done < 12:
+        for _ in range(k):
+            orc.sweep()
+        gpu.sweep(k)
+        done += k
+        k += 1
+        assert gpu.counters() == orc.counters(), done
+        assert np.array_equal(gpu.crystals(), orc.spins()), done
+    gpu.close()
+
+
+def test_mc_persistent_many_trials_per_thread(ctx):
+    """A domain whose colours hold several cells per thread of the persistent grid (full 256 x 256 x 64 space, eight
+    colours of 524,288 cells): the persistent launch equals the per-colour launches, including the mobility skip."""
+    from volcano_ca_mc_amd import native as N
+    dims = (256, 256, 64)
+    n = dims[0] * dims[1] * dims[2]
+    rng = np.random.default_rng(5)
+    mob = rng.random(n, dtype=np.float32)
+    out = []
+    for mode in (1, 2):
+        print(f"persistent check mode {mode}", flush=True)
+        m = N.PottsMC(ctx, dims, "pentagonal", [True, True, False], 77, 128, 1.0, 0.5, True, None, mob,
+                      mobility_dynamics=True)
+        m.set_sweep_mode(mode)
+        m.sweep(3)
+        m.sweep(4)
+        out.append((m.counters(), m.crystals(), m.measure()))
+        m.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]
+
+
 def test_mc_from_ca_matches_host_domain(ctx):
     """vca_mc_create_from_ca derives the basalt domain and cooling mobility from the device CA state."""
     from volcano_ca_mc_amd import native as N

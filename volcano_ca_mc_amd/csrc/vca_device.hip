@@ -1665,24 +1665,27 @@ __device__ inline void nb_spins(const McParams& P, const S* spin, uint32_t g, in
     }
 }
 
-// One Metropolis trial (PottsMC.cpp:403-443): h = hashIndex(prefix, g) already computed, mob_thr is the
-// exact 53-bit threshold ceil(m * 2^53) of the cell (skip iff (r >> 11) >= thr), accept is the table
-// T[before][after] (kDownhill: dE <= 0, else accept iff (r >> 11) < T).
+// The spins a Metropolis trial of cell g reads (its own and those of neighbourhood variant `var`), in one round trip.
 template <typename S>
-__device__ inline void mc_trial(const McParams& P, S* spin, uint32_t g, uint64_t h, int var, const uint64_t* accept,
-                                unsigned long long& att, unsigned long long& acc, unsigned long long& up) {
-    ++att;
+__device__ inline void mc_gather(const McParams& P, const S* spin, uint32_t g, int var, int32_t& cur,
+                                 uint32_t (&sv)[27], bool (&use)[27]) {
     int x, y, z;
     cell_xyz(P, g, x, y, z);
-    const int32_t cur = static_cast<int32_t>(spin[g]);
+    cur = static_cast<int32_t>(spin[g]);
+    nb_spins(P, spin, g, x, y, z, P.nb.mask[var], sv, use);
+}
+
+// One Metropolis trial (PottsMC.cpp:403-443) on the gathered spins: h = hashIndex(prefix, g), accept is the table
+// T[before][after] (kDownhill: dE <= 0, else accept iff (r >> 11) < T).
+template <typename S>
+__device__ inline void mc_decide(const McParams& P, S* spin, uint32_t g, uint64_t h, int32_t cur,
+                                 const uint32_t (&sv)[27], const bool (&use)[27], const uint64_t* accept,
+                                 unsigned long long& att, unsigned long long& acc, unsigned long long& up) {
+    ++att;
     const uint64_t ho = splitmix64(h ^ kSaltMcOrient);
     const uint32_t qm = static_cast<uint32_t>(P.Q - 1);
     int32_t prop = 1 + static_cast<int32_t>(qm < 65536u ? mod_small(ho, qm) : ho % qm);
     if (prop >= cur) ++prop;
-    const uint32_t mask = P.nb.mask[var];
-    uint32_t sv[27];
-    bool use[27];
-    nb_spins(P, spin, g, x, y, z, mask, sv, use);
     int in_dom = 0, same_a = 0, same_b = 0;
 #pragma unroll
     for (int k = 0; k < 27; ++k) {
@@ -1706,22 +1709,144 @@ __device__ inline void mc_trial(const McParams& P, S* spin, uint32_t g, uint64_t
     }
 }
 
+template <typename S>
+__device__ inline void mc_trial(const McParams& P, S* spin, uint32_t g, uint64_t h, int var, const uint64_t* accept,
+                                unsigned long long& att, unsigned long long& acc, unsigned long long& up) {
+    int32_t cur;
+    uint32_t sv[27];
+    bool use[27];
+    mc_gather(P, spin, g, var, cur, sv, use);
+    mc_decide(P, spin, g, h, cur, sv, use, accept, att, acc, up);
+}
+
 }  // namespace
 
+namespace {
+
+// What a trial needs before it reads spins: the cell (list entry j), its hash, whether the mobility skip lets it
+// try, and its neighbourhood variant.  Reads only the list and the thresholds, which no pass writes.
+struct McPrep {
+    uint32_t g;
+    int32_t var;
+    uint64_t h;
+    bool go;
+};
+
+__device__ inline McPrep mc_prep(const McParams& P, const uint32_t* list, const uint64_t* mob_thr, uint32_t j,
+                                 uint32_t hi, uint64_t prefix) {
+    McPrep t{0u, 0, 0ull, false};
+    if (j < hi) {
+        t.g = list[j];
+        t.h = splitmix64(prefix ^ t.g);
+        t.go = mob_thr == nullptr || (splitmix64(t.h ^ kSaltMcMobility) >> 11) < mob_thr[j];
+        t.var = P.nb.variants > 1 ? static_cast<int>(mod_small(splitmix64(t.h ^ kSaltNeighborhood),
+                                                               static_cast<uint32_t>(P.nb.variants)))
+                                  : 0;
+    }
+    return t;
+}
+
+}  // namespace
+
+// One colour pass (PottsMC.cpp:450-514): one trial per list entry.
 template <typename S>
 __global__ void __launch_bounds__(kBlock) mc_color_kernel(McParams P, S* spin, const uint32_t* list,
                                                           const uint64_t* mob_thr, uint32_t count, uint64_t prefix,
                                                           const uint64_t* accept, unsigned long long* counters) {
     unsigned long long c[3] = {0, 0, 0};  // attempts, accepted, uphill
     for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < count; j += gridDim.x * kBlock) {
-        const uint32_t g = list[j];
-        const uint64_t h = splitmix64(prefix ^ g);
-        if (mob_thr == nullptr || (splitmix64(h ^ kSaltMcMobility) >> 11) < mob_thr[j]) {
-            const int var = P.nb.variants > 1
-                                ? static_cast<int>(mod_small(splitmix64(h ^ kSaltNeighborhood),
-                                                             static_cast<uint32_t>(P.nb.variants)))
-                                : 0;
-            mc_trial(P, spin, g, h, var, accept, c[0], c[1], c[2]);
+        const McPrep t = mc_prep(P, list, mob_thr, j, count, prefix);
+        if (t.go) mc_trial(P, spin, t.g, t.h, t.var, accept, c[0], c[1], c[2]);
+    }
+    block_fold(c, 0u, counters + (blockIdx.x % kMcRows) * kMcCounters);
+}
+
+namespace {
+
+// Device-wide barrier of the persistent sweep kernel.  Blocks arrive in eight groups (blockIdx % 8, the XCD a block
+// is dispatched to); the last arrival of a group arrives at the top counter, and the last top arrival publishes the
+// generation `target` (barrier k of a launch: the generation read at the launch's start + k).  Every counter is reset
+// by its last arrival before the generation moves on, so the state is ready for the next barrier and the next launch.
+// Every write of the grid before the barrier is visible to every block after it: each arrival releases (one L2
+// write-back), the chain of last arrivals acquires and releases, and a waiter polls the generation with relaxed
+// loads (which read past the XCD's L2) and acquires once when it arrived -- an acquire per poll would invalidate the
+// L2 under the blocks still working.  A wait ends after `limit` ticks of the constant-rate clock or when another
+// block gave up: it then sets `abort` and returns false, and the caller stops -- so the kernel ends even if its
+// blocks were not all resident (the host reports the flag as an error).
+__device__ bool grid_barrier(McGridBar* bar, uint32_t target, uint64_t limit) {
+    __shared__ int s_ok;
+    __syncthreads();  // the block's writes are done before thread 0 releases them
+    if (threadIdx.x == 0) {
+        const uint32_t G = gridDim.x, grp = blockIdx.x & 7u, groups = G < 8u ? G : 8u;
+        const uint32_t members = (G - grp + 7u) >> 3;
+        if (__hip_atomic_fetch_add(&bar->group[grp][0], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) ==
+            members - 1u) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __hip_atomic_store(&bar->group[grp][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_fetch_add(&bar->top[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == groups - 1u) {
+                __hip_atomic_store(&bar->top[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&bar->gen[0], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        int ok = 1;
+        const uint64_t t0 = wall_clock64();
+        while (__hip_atomic_load(&bar->gen[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != target) {
+            if (__hip_atomic_load(&bar->abort[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+                wall_clock64() - t0 > limit) {
+                __hip_atomic_store(&bar->abort[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// This block's share of colour pass `col`: list positions [lo, hi).
+__device__ inline void mc_share(const McColours& C, int col, uint32_t chunk, uint32_t& lo, uint32_t& hi) {
+    const uint32_t b = C.start[col], n = C.start[col + 1] - b;
+    lo = b + static_cast<uint32_t>(static_cast<uint64_t>(n) * chunk / gridDim.x);
+    hi = b + static_cast<uint32_t>(static_cast<uint64_t>(n) * (chunk + 1u) / gridDim.x);
+}
+
+}  // namespace
+
+// nsweeps checkerboard sweeps in one launch (PottsMC.cpp:450-533): the colour passes of every sweep in order, a
+// device-wide barrier between passes.  The grid is at most one block per CU (4 waves of fewer than 128 VGPRs, so four
+// such blocks fit on a CU): every block is resident together, also beside the three other MC streams of the batched
+// pipeline.  Block b takes
+// the same share of every colour's list -- chunk order groups the blocks of one XCD on one stretch of the domain.
+// The acceptance table sits in LDS.  The hashes of a trial depend only on (sweep, cell): the trials are those of
+// one launch per colour.
+template <typename S>
+__global__ void __launch_bounds__(kBlock) mc_sweeps_kernel(McParams P, S* spin, const uint32_t* list,
+                                                           const uint64_t* mob_thr, McColours C, uint64_t sweep0,
+                                                           int32_t nsweeps, const uint64_t* accept,
+                                                           unsigned long long* counters, McGridBar* bar, uint64_t limit) {
+    __shared__ uint64_t s_acc[27 * 27];
+    for (int k = threadIdx.x; k < 27 * 27; k += kBlock) s_acc[k] = accept[k];
+    __syncthreads();
+    const uint32_t G = gridDim.x, grp = blockIdx.x & 7u;
+    uint32_t chunk = blockIdx.x >> 3;  // this block's share: groups in order, blocks of a group consecutive
+    for (uint32_t q = 0; q < grp; ++q) chunk += (G - q + 7u) >> 3;
+    // the generation before the first barrier (no barrier of this launch can complete before every block read it)
+    uint32_t gen = __hip_atomic_load(&bar->gen[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long c[3] = {0, 0, 0};
+    bool live = true;
+    for (int32_t sw = 0; sw < nsweeps && live; ++sw) {
+        const uint64_t prefix = hash_prefix(P.seed, sweep0 + static_cast<uint64_t>(sw));
+        for (int32_t col = 0; col < C.count && live; ++col) {
+            uint32_t lo, hi;
+            mc_share(C, col, chunk, lo, hi);
+            for (uint32_t j = lo + threadIdx.x; j < hi; j += kBlock) {
+                const McPrep t = mc_prep(P, list, mob_thr, j, hi, prefix);
+                if (t.go) mc_trial(P, spin, t.g, t.h, t.var, s_acc, c[0], c[1], c[2]);
+            }
+            if (sw + 1 < nsweeps || col + 1 < C.count) live = grid_barrier(bar, ++gen, limit);
         }
     }
     block_fold(c, 0u, counters + (blockIdx.x % kMcRows) * kMcCounters);
@@ -2094,6 +2219,15 @@ void mc_launch_color(const McParams& P, void* spin, int bytes, const uint32_t* l
     VCA_SPIN_DISPATCH(bytes, hipLaunchKernelGGL(mc_color_kernel<S>, dim3(grid_for(count, 4096u)), dim3(kBlock), 0, s,
                                                 P, static_cast<S*>(spin), list, mob_thr, count, prefix, accept,
                                                 counters));
+}
+
+void mc_launch_sweeps(const McParams& P, void* spin, int bytes, const uint32_t* list, const uint64_t* mob_thr,
+                      const McColours& C, uint64_t sweep0, int32_t nsweeps, const uint64_t* accept,
+                      unsigned long long* counters, McGridBar* bar, uint64_t limit, unsigned grid, hipStream_t s) {
+    if (nsweeps <= 0 || grid == 0) return;
+    VCA_SPIN_DISPATCH(bytes, hipLaunchKernelGGL(mc_sweeps_kernel<S>, dim3(grid), dim3(kBlock), 0, s, P,
+                                                static_cast<S*>(spin), list, mob_thr, C, sweep0, nsweeps, accept,
+                                                counters, bar, limit));
 }
 
 void mc_launch_random(const McParams& P, void* spin, int bytes, const uint32_t* list, const uint64_t* mob_thr,

@@ -208,6 +208,21 @@ struct McParams {
 
 constexpr uint64_t kDownhill = ~0ULL;  // accept-table entry for dE <= 0
 
+// Colour passes of a checkerboard sweep: colour c is list[start[c] .. start[c + 1]) (at most 3 x 3 x 3 colours).
+struct McColours {
+    uint32_t start[28];
+    int32_t count;
+};
+
+// Barrier state of the persistent sweep kernel (one per MC simulation, zero at creation): arrival counters of the
+// eight block groups, of the groups, the generation and the give-up flag, each on a 256-byte line of its own.
+struct McGridBar {
+    uint32_t group[8][64];
+    uint32_t top[64];
+    uint32_t gen[64];
+    uint32_t abort[64];
+};
+
 // Source of the MC domain lists built on the device: mode 0 every cell, 1 dom[i] != 0, 2 dom = CA cell types
 // (LAVA or BASALT).  ncol colours (1 = one list in index order); parity colouring or index[] over the axis
 // classes as in buildColors (PottsMC.cpp:145-224).
@@ -287,6 +302,11 @@ void mc_launch_list_scatter(const McListSrc& L, const uint32_t* off, const uint3
 void mc_launch_color(const McParams& P, void* spin, int spin_bytes, const uint32_t* list, const uint64_t* mob_thr,
                      uint32_t count, uint64_t prefix, const uint64_t* accept, unsigned long long* counters,
                      hipStream_t s);
+// nsweeps checkerboard sweeps in one launch of `grid` blocks (at most one per CU), a device-wide barrier between
+// colour passes; `limit` bounds each barrier wait in ticks of the device's constant-rate clock.
+void mc_launch_sweeps(const McParams& P, void* spin, int spin_bytes, const uint32_t* list, const uint64_t* mob_thr,
+                      const McColours& C, uint64_t sweep0, int32_t nsweeps, const uint64_t* accept,
+                      unsigned long long* counters, McGridBar* bar, uint64_t limit, unsigned grid, hipStream_t s);
 void mc_launch_random(const McParams& P, void* spin, int spin_bytes, const uint32_t* list, const uint64_t* mob_thr,
                       uint32_t count, uint64_t prefix, const uint64_t* accept, unsigned long long* counters,
                       hipStream_t s);

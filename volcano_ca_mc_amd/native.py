@@ -90,7 +90,7 @@ def lib() -> C.CDLL:
         "vca_ca_autotune": [vp, C.POINTER(CaDesc), vp, vp, vp, i32, i32, C.POINTER(TuneResult)],
         "vca_mc_create": [vp, C.POINTER(McDesc), vp, vp, i32, C.POINTER(vp)],
         "vca_mc_create_from_ca": [vp, C.POINTER(McDesc), vp, i32, C.c_double, C.POINTER(vp)],
-        "vca_mc_destroy": [vp], "vca_mc_sweep": [vp, i32, C.POINTER(McCounts)],
+        "vca_mc_destroy": [vp], "vca_mc_sweep": [vp, i32, C.POINTER(McCounts)], "vca_mc_set_sweep_mode": [vp, i32],
         "vca_mc_counts_get": [vp, C.POINTER(McCounts)],
         "vca_mc_measure": [vp, C.POINTER(McStats), C.POINTER(McClass), C.POINTER(u64)],
         "vca_mc_download": [vp, vp], "vca_mc_download_labels": [vp, vp],
@@ -110,7 +110,8 @@ EXPORTED = ("vca_version", "vca_last_error", "vca_ctx_create", "vca_ctx_destroy"
             "vca_ca_footprint", "vca_ca_tiles", "vca_batch_create", "vca_batch_destroy", "vca_batch_add",
             "vca_batch_remove", "vca_batch_set_graph", "vca_batch_enqueue", "vca_batch_run", "vca_batch_info",
             "vca_batch_status", "vca_ca_collect", "vca_ca_flow_stats", "vca_ca_flow_points", "vca_ca_autotune",
-            "vca_mc_create", "vca_mc_create_from_ca", "vca_mc_destroy", "vca_mc_sweep", "vca_mc_counts_get",
+            "vca_mc_create", "vca_mc_create_from_ca", "vca_mc_destroy", "vca_mc_sweep", "vca_mc_set_sweep_mode",
+            "vca_mc_counts_get",
             "vca_mc_measure", "vca_mc_download", "vca_mc_download_labels", "vca_mc_info",
             "vca_mc_download_mobility")
 
@@ -391,6 +392,14 @@ class PottsMC:
             self.close()
         except Exception:
             pass
+
+    # vca_mc_set_sweep_mode values
+    SWEEP_AUTO, SWEEP_PER_COLOUR, SWEEP_PERSISTENT = 0, 1, 2
+
+    def set_sweep_mode(self, mode: int) -> None:
+        """How checkerboard sweeps are launched (same results): auto, one launch per colour pass, or one persistent
+        launch per call."""
+        _check(lib().vca_mc_set_sweep_mode(self.h, mode))
 
     def sweep(self, n: int = 1):
         c = McCounts()
