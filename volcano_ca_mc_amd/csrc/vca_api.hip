@@ -259,6 +259,15 @@ struct vca_ctx {
 
 struct Batch;
 
+// Blocks of the tile kernels' launches (each block strides over the step's tile list; the device's list length is not
+// known to the host): 32 per CU, at most one per tile the batch can hold.  Measured on MI355X: blocks beyond a small
+// front cost little (caldera caSim 44.2 ms at 32 per CU, 43.6 at 16, 45.0 at 8), while fewer blocks lose the dynamic
+// balance of a large front (cone 86 ms at 32 per CU, 91 at 16, 98 at 6, 118 at 4).
+inline int tile_grid(const vca_ctx* c, uint32_t cap) {
+    const uint64_t g = static_cast<uint64_t>(std::max(1, c->cus)) * 32u;
+    return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(cap, g)));
+}
+
 namespace {
 void noalloc_setup(int device);  // vca_ca_run_noalloc's per-device set-up (below)
 }  // namespace
@@ -426,7 +435,7 @@ struct Batch {
         B.cap = cap;
         B.active = active.as<uint64_t>();
         B.ctrl = ctrl.as<BatchCtrl>();
-        grid_tiles = static_cast<int>(std::min<uint32_t>(cap, 8192u));
+        grid_tiles = tile_grid(c, cap);
     }
     // A batch of one slot over device memory the library does not allocate: counts, tile owners and the two list copies
     // carved out of `base` (bytes), for as many tiles as fit; the simulation table entry `entry` is the caller's.
@@ -460,12 +469,13 @@ struct Batch {
         B.cap = cap;
         B.active = reinterpret_cast<uint64_t*>(base + o_active);
         B.ctrl = reinterpret_cast<BatchCtrl*>(base + o_ctrl);
-        grid_tiles = static_cast<int>(std::min<uint32_t>(cap, 8192u));
+        grid_tiles = 0;  // set by reset_external (the context's device)
         (void)device;
     }
     // Ready for the next simulation (external batch): counts cleared by a kernel, host state reset.
     void reset_external(vca_ctx* c) {
         ctx = c;
+        grid_tiles = tile_grid(c, cap);
         fill_u32(reinterpret_cast<uint32_t*>(B.ctrl), 0u, sizeof(BatchCtrl) / 4, ctx->stream);
         members[0] = nullptr;
         free_ranges.clear();
