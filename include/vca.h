@@ -263,8 +263,8 @@ typedef struct {
  * regular sample of the run at a fraction of the events' own cost), 0 turns timing off.  Events come from a pool
  * and are read back at the synchronisation points the calls already have (no extra host sync). */
 int vca_ctx_set_timing(vca_ctx* ctx, int32_t on);
-/* name: "ca_front", "ca_descent", "ca_rules", "ca_apply", "ca_finish", "ca_step" (whole step),
- * "mc_color" (one colour launch), "mc_sweep" (whole sweep) */
+/* name: "ca_front", "ca_descent", "ca_rules", "ca_finish", "ca_step" (whole step), "mc_color" (one colour launch),
+ * "mc_sweep" (whole sweep of colour launches), "mc_sweeps" (one persistent sweep launch) */
 int vca_ctx_kernel_time(vca_ctx* ctx, const char* name, vca_kernel_time* out);
 
 #ifdef __cplusplus

@@ -874,7 +874,7 @@ int run_from_cli(const CliOptions& opt, const Clock& since_start) {
         d["unit"] = "ms";
         d["sampleEvery"] = std::max(1, opt.timing_every);  // launches of every N-th CA step / MC sweep timed
         Json k = Json::object();
-        for (const char* name : {"ca_front", "ca_descent", "ca_rules", "ca_apply", "ca_finish", "ca_step",
+        for (const char* name : {"ca_front", "ca_descent", "ca_rules", "ca_finish", "ca_step",
                                   "mc_color", "mc_sweep", "mc_sweeps"}) {
             vca_kernel_time kt{};
             vca_check(vca_ctx_kernel_time(gpu.ctx(), name, &kt), "GPU");

@@ -298,15 +298,16 @@ struct vca_ca {
     CaDev D{};
     int shape = 0;
     DevBuf arena;  // every region below: one allocation and one free per simulation
-    Span type, flow, temp, temp2, solid, arrival, descent, spill, vent_cell, tile_flags, tile_near, tile_halo, tile_tchg, listed, changes,
-        tile_nchg, part, ctrl, hist_a, hist_c;
+    Span type, type2, flow, temp, temp2, solid, arrival, descent, spill, vent_cell, tile_flags, tile_near, tile_halo,
+        tile_tchg, listed, part, ctrl, hist_a, hist_c;
     uint64_t n = 0;
     std::unique_ptr<Batch> own;   // batch of one: the solo path (made on the first solo enqueue)
     bool flow16 = true;           // every flowId the simulation can hold (its vents') fits int16_t
     hipStream_t side = nullptr;   // during vca_ca_collect: the pooled stream its reads go on
     Batch* member = nullptr;      // the shared batch this simulation is in
     int slot = -1;
-    // the temperature copy holding the state after `steps` steps
+    // the type and temperature copies holding the state after `steps` steps
+    const void* type_after(int32_t steps) const { return (steps & 1) ? type2.p : type.p; }
     const void* temp_after(int32_t steps) const { return (steps & 1) ? temp2.p : temp.p; }
 };
 
@@ -652,7 +653,7 @@ struct Batch {
     }
     void launch_direct(int steps) {
         vca_ctx* c = ctx;
-        static const char* kNames[kCaKernels] = {"ca_front", "ca_descent", "ca_rules", "ca_apply", "ca_finish"};
+        static const char* kNames[kCaKernels] = {"ca_front", "ca_descent", "ca_rules", "ca_finish"};
         for (int k = 0; k < steps; ++k) {
             if (!c->time_ca_step()) {
                 launch_step(c->stream, nullptr, nullptr);
@@ -869,14 +870,15 @@ extern "C" int vca_ctx_kernel_time(vca_ctx* ctx, const char* name, vca_kernel_ti
 namespace {
 // Regions of a simulation's single allocation (creation and the footprint estimate use the same plan).  The
 // regions cleared at creation come first, so that one memset covers them.  Creation inputs (rock heights, seed
-// cells) are staged in the step records, which no kernel reads before the first step.
+// cells) are staged in the D(n) array when they fit (it is set to "none" after them), else in a region of their
+// own.
 struct CaPlan {
     Layout L;
     size_t hcap = 1;
-    size_t type = 0, flow = 0, temp = 0, temp2 = 0, solid = 0, arrival = 0, ctrl = 0, tile_near = 0, tile_halo = 0, tile_tchg = 0, listed = 0,
-           part = 0, hist_a = 0, hist_c = 0;
+    size_t type = 0, type2 = 0, flow = 0, temp = 0, temp2 = 0, solid = 0, arrival = 0, ctrl = 0, tile_near = 0,
+           tile_halo = 0, tile_tchg = 0, listed = 0, part = 0, hist_a = 0, hist_c = 0;
     size_t zero_bytes = 0;
-    size_t descent = 0, vent_cell = 0, tile_flags = 0, changes = 0, tile_nchg = 0, spill = 0;
+    size_t descent = 0, vent_cell = 0, tile_flags = 0, spill = 0;
     size_t staging = 0, st_height = 0, st_cells = 0, st_flows = 0, st_temps = 0, st_bytes = 0;
 };
 
@@ -885,6 +887,7 @@ CaPlan ca_plan(uint64_t n, uint32_t ntiles, size_t cols, bool lava, size_t vent_
     Layout& L = c.L;
     c.hcap = static_cast<size_t>(std::max(1, max_steps));
     c.type = L.add(n);
+    c.type2 = L.add(n);
     c.flow = L.add(n * 4);
     c.temp = L.add(n * 4);
     c.temp2 = L.add(n * 4);
@@ -902,9 +905,6 @@ CaPlan ca_plan(uint64_t n, uint32_t ntiles, size_t cols, bool lava, size_t vent_
     c.descent = L.add(n);
     c.vent_cell = L.add(vent_slots * 4);
     c.tile_flags = L.add(ntiles);
-    const size_t records = static_cast<size_t>(ntiles) * kTileCells * sizeof(Change);
-    c.changes = L.add(records);
-    c.tile_nchg = L.add(static_cast<size_t>(ntiles) * 4);
     if (lava) c.spill = L.add(cols * 4);
     Layout st;
     c.st_height = st.add(lava ? cols * 4 : 0);
@@ -912,7 +912,7 @@ CaPlan ca_plan(uint64_t n, uint32_t ntiles, size_t cols, bool lava, size_t vent_
     c.st_flows = st.add(nvents * 4);
     c.st_temps = st.add(nvents * 4);
     c.st_bytes = st.total;
-    c.staging = st.total <= records ? c.changes : L.add(st.total);
+    c.staging = st.total <= n ? c.descent : L.add(st.total);
     return c;
 }
 }  // namespace
@@ -1059,6 +1059,7 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
             sp.bytes = bytes;
         };
         region(ca->type, plan.type, n);
+        region(ca->type2, plan.type2, n);
         region(ca->flow, plan.flow, n * 4);
         region(ca->temp, plan.temp, n * 4);
         region(ca->temp2, plan.temp2, n * 4);
@@ -1075,17 +1076,15 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         region(ca->descent, plan.descent, n);
         region(ca->vent_cell, plan.vent_cell, vent_table.size() * 4);
         region(ca->tile_flags, plan.tile_flags, P.ntiles);
-        region(ca->changes, plan.changes, static_cast<size_t>(P.ntiles) * kTileCells * sizeof(Change));
-        region(ca->tile_nchg, plan.tile_nchg, static_cast<size_t>(P.ntiles) * 4);
         if (!classic) region(ca->spill, plan.spill, cols * 4);
         // state, counters, neighbour marks, partials and histories start at zero; descent at "none"
         check(hipMemsetAsync(base, 0, plan.zero_bytes, s), "memset");
-        check(hipMemsetAsync(ca->descent.p, 0xFF, n, s), "memset");
         check(hipMemcpyAsync(ca->vent_cell.p, vent_table.data(), vent_table.size() * 4, hipMemcpyHostToDevice, s),
               "copy vents");
 
         CaDev& D = ca->D;
-        D.type = ca->type.as<uint8_t>();
+        D.type[0] = ca->type.as<uint8_t>();
+        D.type[1] = ca->type2.as<uint8_t>();
         D.flow = ca->flow.as<int32_t>();
         D.temp[0] = ca->temp.as<float>();
         D.temp[1] = ca->temp2.as<float>();
@@ -1098,8 +1097,6 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         D.tile_halo = ca->tile_halo.as<uint8_t>();
         D.tile_tchg = ca->tile_tchg.as<uint32_t>();
         D.listed = ca->listed.as<uint32_t>();
-        D.changes = ca->changes.as<Change>();
-        D.tile_nchg = ca->tile_nchg.as<uint32_t>();
         D.part = ca->part.as<CaPart>();
         D.ctrl = ca->ctrl.as<CaCtrl>();
         D.hist_active = ca->hist_a.as<unsigned long long>();
@@ -1126,6 +1123,8 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
                             reinterpret_cast<const float*>(staging + plan.st_temps), nvents, s);
             check(hipGetLastError(), "seeds");
         }
+        // the staged inputs are read: D(n) starts at "none"
+        check(hipMemsetAsync(ca->descent.p, 0xFF, n, s), "memset");
         // countInitialState (LavaCA.hpp:1252-1280) and the initial stop rule, on the device
         ca_launch_init_flags(P, D, s);
         ca_launch_init_stop(P, D, s);
@@ -1245,8 +1244,9 @@ extern "C" int vca_ca_collect(vca_ca* ca, vca_ca_result* r) {
         if (m > 0 && r->active) staged_download(r->active, ca->hist_a.p, m * 8, ca->side);
         if (m > 0 && r->changed) staged_download(r->changed, ca->hist_c.p, m * 8, ca->side);
         const uint64_t n = ca->n;
-        staged_download(r->type, ca->type.p, n, ca->side);
+        staged_download(r->type, ca->type_after(c.steps), n, ca->side);
         staged_download(r->flow, ca->flow.p, n * 4, ca->side);
+        if (r->temp) ca_launch_final_temps(ca->D, n, c.steps & 1, ca->side);
         staged_download(r->temp, ca->temp_after(c.steps), n * 4, ca->side);
         staged_download(r->solid, ca->solid.p, n * 2, ca->side);
         staged_download(r->arrival, ca->arrival.p, n * 2, ca->side);
@@ -1256,12 +1256,14 @@ extern "C" int vca_ca_collect(vca_ca* ca, vca_ca_result* r) {
 }
 
 namespace {
-void require_stopped(vca_ca* ca, hipStream_t s) {
+// Returns the step count (its parity selects the type and temperature copies holding the state).
+int32_t require_stopped(vca_ca* ca, hipStream_t s) {
     if (ca->member) throw VcaError(VCA_ERR_STATE, "the simulation is in a batch: remove it first");
     CaCtrl c;
     check(hipMemcpyAsync(&c, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, s), "read ctrl");
     check(hipStreamSynchronize(s), "read ctrl");
     if (c.stop == VCA_RUNNING) throw VcaError(VCA_ERR_STATE, "the simulation is still running");
+    return c.steps;
 }
 }  // namespace
 
@@ -1272,7 +1274,7 @@ extern "C" int vca_ca_flow_stats(vca_ca* ca, const vca_vent* vents, int32_t nflo
         require(nflows == 0 || (vents && lava && basalt && maxd2), "NULL argument");
         ca->ctx->set_device();
         SideStream side(ca->ctx);
-        require_stopped(ca, side.s);
+        const int par = require_stopped(ca, side.s) & 1;
         const size_t m = static_cast<size_t>(nflows) + 1;
         std::vector<int32_t> xy(2 * m, 0);
         for (int32_t f = 0; f < nflows; ++f) {
@@ -1286,7 +1288,7 @@ extern "C" int vca_ca_flow_stats(vca_ca* ca, const vca_vent* vents, int32_t nflo
         check(hipMemcpyAsync(dxy.p, xy.data(), xy.size() * 4, hipMemcpyHostToDevice, side.s), "copy vents");
         check(hipMemsetAsync(out.p, 0, 3 * m * 8, side.s), "memset");
         unsigned long long* o = out.as<unsigned long long>();
-        if (nflows > 0) ca_launch_flow_sums(ca->P, ca->D, dxy.as<int32_t>(), nflows, o, o + m, o + 2 * m, side.s);
+        if (nflows > 0) ca_launch_flow_sums(ca->P, ca->D, par, dxy.as<int32_t>(), nflows, o, o + m, o + 2 * m, side.s);
         check(hipGetLastError(), "flow sums");
         std::vector<uint64_t> h(3 * m);
         check(hipMemcpyAsync(h.data(), out.p, 3 * m * 8, hipMemcpyDeviceToHost, side.s), "read flow sums");
@@ -1305,12 +1307,12 @@ extern "C" int vca_ca_flow_points(vca_ca* ca, const vca_vent* vent, int32_t flow
         require(ca && vent && seen && cap > 0, "invalid argument");
         ca->ctx->set_device();
         SideStream side(ca->ctx);
-        require_stopped(ca, side.s);
+        const int par = require_stopped(ca, side.s) & 1;
         AllocStream alloc_on(side.s);
         DevBuf buf;
         buf.alloc(static_cast<size_t>(cap) * 4);
         check(hipMemsetAsync(buf.p, 0, static_cast<size_t>(cap) * 4, side.s), "memset");
-        ca_launch_flow_points(ca->P, ca->D, flow, vent->x, vent->y, d2, buf.as<uint32_t>(), static_cast<uint32_t>(cap),
+        ca_launch_flow_points(ca->P, ca->D, par, flow, vent->x, vent->y, d2, buf.as<uint32_t>(), static_cast<uint32_t>(cap),
                               side.s);
         check(hipGetLastError(), "flow points");
         std::vector<uint32_t> h(static_cast<size_t>(cap));
@@ -1399,10 +1401,12 @@ extern "C" int vca_ca_download(vca_ca* ca, uint8_t* type, int32_t* flow, float* 
         int32_t steps = 0;
         ca_status_sync(ca, &steps, nullptr);
         const uint64_t n = ca->n;
-        if (type) check(hipMemcpyAsync(type, ca->type.p, n, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
+        if (type) check(hipMemcpyAsync(type, ca->type_after(steps), n, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
         if (flow) check(hipMemcpyAsync(flow, ca->flow.p, n * 4, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
-        if (temp)
+        if (temp) {
+            ca_launch_final_temps(ca->D, n, steps & 1, ca->ctx->stream);
             check(hipMemcpyAsync(temp, ca->temp_after(steps), n * 4, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
+        }
         if (solid) check(hipMemcpyAsync(solid, ca->solid.p, n * 2, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
         if (arrival) check(hipMemcpyAsync(arrival, ca->arrival.p, n * 2, hipMemcpyDeviceToHost, ca->ctx->stream), "download");
         check(hipStreamSynchronize(ca->ctx->stream), "download");
@@ -1926,10 +1930,12 @@ extern "C" int vca_mc_create_from_ca(vca_ctx* ctx, const vca_mc_desc* d, vca_ca*
         hipStream_t s = ctx->stream;
         // a stopped simulation's state is final (later steps are no-ops), so ctx may be another context whose
         // stream is not ordered after the CA's
+        int32_t steps = 0;
         {
             SideStream side(ca->ctx);
-            require_stopped(ca, side.s);
+            steps = require_stopped(ca, side.s);
         }
+        const uint8_t* types = static_cast<const uint8_t*>(ca->type_after(steps));
         // CoolingMobility (CoolingMobility.cpp:8-129) from the CA result on the device
         mc->has_mob_cells = true;
         mc->mob_cells.alloc(mc->n * 4);
@@ -1937,12 +1943,12 @@ extern "C" int vca_mc_create_from_ca(vca_ctx* ctx, const vca_mc_desc* d, vca_ca*
         red.alloc(16);
         const uint32_t init[4] = {0, 0xFFFFFFFFu, 0, 0};
         check(hipMemcpyAsync(red.p, init, sizeof init, hipMemcpyHostToDevice, s), "copy");
-        mc_launch_cooling(mc->n, ca->type.as<uint8_t>(), ca->solid.as<uint16_t>(), ca->arrival.as<uint16_t>(),
+        mc_launch_cooling(mc->n, types, ca->solid.as<uint16_t>(), ca->arrival.as<uint16_t>(),
                           red.as<uint32_t>(), mobility_min, mc->mob_cells.as<float>(), s);
         check(hipGetLastError(), "cooling mobility");
         McListSrc L{};
         L.mode = 2;
-        L.dom = ca->type.as<uint8_t>();
+        L.dom = types;
         *out = mc_build(ctx, d, L, std::move(mc), use_mobility != 0).release();
     });
 }

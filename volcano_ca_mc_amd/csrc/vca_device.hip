@@ -1,18 +1,18 @@
 // HIP kernels of the lava CA step and the Potts/Metropolis sweep, written for gfx950 (CDNA4).
 //
-// CA step (one synchronous step of engine/src/ca/LavaCA.hpp) = three stream-ordered launches for lava-front
+// CA step (one synchronous step of engine/src/ca/LavaCA.hpp) = two stream-ordered launches for lava-front
 // simulations:
 //   ca_descent : clears the counts of the previous step's tile list, does the previous step's finish (step counters
 //                -> histories, stop rule, LavaCA.hpp:1206-1229), and computes D(n) of every lava cell
 //                (LavaCA.hpp:421-429, 877-888) of the staged tiles whose D(n) can have changed, one block per tile
 //                staged in LDS.
 //   ca_rules   : one block per staged tile (tile + 1-cell halo in LDS); the rules of every cell read the
-//                start-of-step state only; lava cells write their next temperature to the step's other temperature
-//                copy, type changes go as records into the tile's own slot range of the change buffer (LDS counter,
-//                no global atomics); counters, tile flags, and the next step's tile list (the tile itself and the
-//                neighbours it holds lava next to: the reference's "may change" set, CaRules.hpp:69-79, at tile
-//                granularity) (LavaCA.hpp:963-1048, 1109-1202).
-//   ca_apply   : scatters the records of every staged tile into the state (LavaCA.hpp:1052-1066).
+//                start-of-step state only (type and temperature copies of the step's parity); the tile's next state
+//                goes into the other copies (types that differ, every lava cell's temperature), flowIds and step
+//                stamps of changed cells in place -- so no scatter pass follows (the reference applies its change
+//                list after the rules, LavaCA.hpp:1052-1066); counters, tile flags, and the next step's tile list
+//                (the tile itself and the neighbours it holds lava next to: the reference's "may change" set,
+//                CaRules.hpp:69-79, at tile granularity) (LavaCA.hpp:963-1048, 1109-1202).
 // The finish of the last step of a run of steps is one more launch (ca_finish).  Simulations that need every tile
 // visited (ca.sweep = full) or classic mode add ca_front (a scan of every tile) before and ca_finish after each step.
 // Every kernel skips the entries of a simulation whose stop rule fired, so the host can enqueue many steps without
@@ -374,7 +374,9 @@ __global__ void __launch_bounds__(kBlock) ca_terrain_kernel(CaParams P, CaDev D,
         const int z = static_cast<int>(i / plane);
         int h = height[col];
         h = h < P.nz ? h : P.nz;
-        D.type[i] = z < h ? kRock : kAir;
+        const uint8_t t = z < h ? kRock : kAir;
+        D.type[0][i] = t;
+        D.type[1][i] = t;
     }
 }
 
@@ -383,7 +385,8 @@ __global__ void ca_seeds_kernel(CaDev D, const uint32_t* cells, const int32_t* f
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     for (int k = 0; k < n; ++k) {
         const uint32_t i = cells[k];
-        D.type[i] = kLava;
+        D.type[0][i] = kLava;
+        D.type[1][i] = kLava;
         D.flow[i] = flows[k];
         D.temp[0][i] = temps[k];  // the state before step 0
         D.solid[i] = 0;
@@ -424,9 +427,9 @@ __global__ void __launch_bounds__(kBlock) ca_init_flags_kernel(CaParams P, CaDev
             int x, y, z;
             tile_cell(P, o, k, x, y, z);
             if (x < P.nx && y < P.ny && z < P.nz) {
-                const uint8_t ty = D.type[static_cast<uint32_t>(x) +
-                                          static_cast<uint32_t>(P.nx) * (static_cast<uint32_t>(y) +
-                                                                         static_cast<uint32_t>(P.ny) * z)];
+                const uint8_t ty = D.type[0][static_cast<uint32_t>(x) +
+                                             static_cast<uint32_t>(P.nx) * (static_cast<uint32_t>(y) +
+                                                                            static_cast<uint32_t>(P.ny) * z)];
                 lava += ty == kLava;
                 air += ty == kAir;
                 if (ty == kLava) bnd |= boundary_dirs(x - o.x0, y - o.y0, z - o.z0, xm, ym, zm);
@@ -499,11 +502,16 @@ __global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B, uint32_t s
         uint8_t* near = sim.D.tile_near;
         const uint8_t nb = near[t];
         if (nb && !P.classic) near[t] = 0;
-        const bool act = P.dense || lava || (P.classic && (f & kFlagLava) && (f & kFlagAir)) || ((f & kFlagAir) && nb);
-        // a visited tile goes to the rules only when it or its 1-cell halo holds lava: its own flag, or a
-        // neighbour's halo byte (mark_halo); the bytes are cleared for the next step in lava mode.  Any other
-        // visited tile is settled here: none of its cells can change, its flags stay as they are.
-        bool stage = (f & kFlagLava) != 0;
+        // a tile that changed a cell's type in the previous step brings the other type copy up to date in this one
+        const int32_t now = sim.D.ctrl->steps;
+        const bool changed = now > 0 && sim.D.tile_tchg[t] == static_cast<uint32_t>(now);
+        const bool act = P.dense || lava || changed || (P.classic && (f & kFlagLava) && (f & kFlagAir)) ||
+                         ((f & kFlagAir) && nb);
+        // a visited tile goes to the rules only when it or its 1-cell halo holds lava (its own flag, or a
+        // neighbour's halo byte, mark_halo; the bytes are cleared for the next step in lava mode) or it changed in
+        // the previous step.  Any other visited tile is settled here: none of its cells can change, its flags stay
+        // as they are.
+        bool stage = (f & kFlagLava) != 0 || changed;
         if (act) {
             uint4* hb = reinterpret_cast<uint4*>(sim.D.tile_halo + static_cast<size_t>(t) * kHaloBytes);
             bool any = stage;  // a tile with lava is staged anyway: its bytes are only cleared
@@ -517,7 +525,7 @@ __global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B, uint32_t s
                 hb[1] = make_uint4(0, 0, 0, 0);
             }
         }
-        ent = (static_cast<uint64_t>(static_cast<uint32_t>(sim.D.ctrl->steps)) << 32) | (sl << kTileBits) | t;
+        ent = (static_cast<uint64_t>(static_cast<uint32_t>(now)) << 32) | (sl << kTileBits) | t;
         code = act && stage ? sim.shape : -1;
         return act;
     };
@@ -701,8 +709,20 @@ constexpr uint64_t own_row_mask() {
     return m;
 }
 
-template <int TY, int TZ, bool OWN = false>
-__device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileCoord& o, HaloTypes<TY, TZ>& S,
+// What a step writes into the other type copy for the own cells of a tile, as two bit planes of one 64-bit mask per
+// tile row (row = ly + TY * lz, bit lx) in 32-bit words w[plane][2 * row + (lx >> 5)]: plane 0 = the cell is
+// written, plane 1 = with BASALT (else LAVA).
+template <int TY, int TZ>
+struct NextTypes {
+    uint32_t w[2][2 * TY * TZ];
+};
+
+// With SYNC, N gets the previous step's changes that the other type copy `other` does not hold yet: when `changed`
+// (the tile changed a cell's type in the previous step) the tile's own cells are read from that copy as well, else
+// the copies agree on them.
+template <int TY, int TZ, bool OWN = false, bool SYNC = false>
+__device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const uint8_t* type, const uint8_t* other,
+                                  bool changed, NextTypes<TY, TZ>* N, const TileCoord& o, HaloTypes<TY, TZ>& S,
                                   bool with_spill, int32_t* s_spill, int& own_air, uint32_t row_base, int gx,
                                   unsigned tid) {
     using H = Halo<TY, TZ>;
@@ -712,14 +732,18 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     __builtin_assume(w >= 0 && w < kBlock / 64);
     const int lx = t & 63;
+    // row tests from one scalar mask (bit 4k: this wave's row k is an own row)
+    const uint64_t wown = own_row_mask<TY, TZ>() >> w;
     uint32_t v[H::KR];  // 32-bit so that no load result is packed (and waited for) before the stores
+    uint32_t ov[H::KR];
 #pragma unroll
     for (int k = 0; k < H::KR; ++k) {
         const int r = w + 4 * k;
         v[k] = kVoidLds;
+        ov[k] = kVoidLds;
         if (r < H::R) {
             const uint32_t base = row_of(row_base, r);
-            if (base != kNoCell && gx >= 0) v[k] = D.type[base + static_cast<uint32_t>(gx)];
+            if (base != kNoCell && gx >= 0) v[k] = type[base + static_cast<uint32_t>(gx)];
         }
     }
     uint32_t e = kVoidLds;
@@ -728,7 +752,7 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
         const uint32_t base = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(((t >> 1) & 63) << 2, static_cast<int>(row_base)));
         if (t < 2 * H::R) {
             const int ex = halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1);
-            if (base != kNoCell && ex >= 0) e = D.type[base + static_cast<uint32_t>(ex)];
+            if (base != kNoCell && ex >= 0) e = type[base + static_cast<uint32_t>(ex)];
         }
     }
     int32_t sp[KS];
@@ -751,8 +775,17 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
             }
         }
     }
-    // row tests from one scalar mask (bit 4k: this wave's row k is an own row), lava rows OR-ed as masks
-    const uint64_t wown = own_row_mask<TY, TZ>() >> w;
+    if (SYNC && changed) {
+#pragma unroll
+        for (int k = 0; k < H::KR; ++k) {
+            const int r = w + 4 * k;
+            if (r < H::R && ((wown >> (4 * k)) & 1u)) {
+                const uint32_t base = row_of(row_base, r);
+                if (base != kNoCell && gx >= 0) ov[k] = other[base + static_cast<uint32_t>(gx)];
+            }
+        }
+    }
+    // lava rows OR-ed as masks
     const bool in_x = o.x0 + lx < P.nx;
     uint64_t any_lava = 0;
 #pragma unroll
@@ -764,6 +797,24 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
             if (lx == 0) S.lava[r] = m;
             any_lava |= m;
             if (((wown >> (4 * k)) & 1u) && in_x && v[k] == kAir) own_air = 1;
+            if (SYNC && ((wown >> (4 * k)) & 1u)) {
+                // own row: ly = r % LY - 1, lz = r / LY - 1; rows past the grid's far side hold wrapped halo cells
+                const int ly = r % H::LY - 1, lz = r / H::LY - 1;
+                const bool inside = in_x && o.y0 + ly < P.ny && o.z0 + lz < P.nz;
+                uint64_t m0 = 0, m1 = 0;
+                if (changed) {
+                    const bool differ = inside && v[k] != ov[k];  // born (LAVA) or solidified (BASALT) in step t - 1
+                    m0 = __ballot(differ);
+                    m1 = __ballot(differ && v[k] == kBasalt);
+                }
+                if (lx == 0) {
+                    const int row = ly + TY * lz;
+                    N->w[0][2 * row] = static_cast<uint32_t>(m0);
+                    N->w[0][2 * row + 1] = static_cast<uint32_t>(m0 >> 32);
+                    N->w[1][2 * row] = static_cast<uint32_t>(m1);
+                    N->w[1][2 * row + 1] = static_cast<uint32_t>(m1 >> 32);
+                }
+            }
         }
     }
     bool lava = any_lava != 0;
@@ -786,7 +837,7 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const TileC
 // column col (global) with spill height spill_c.  Layers held in LDS are read there, deeper ones from global
 // memory.
 template <int TY, int TZ>
-__device__ int drop_depth_lds(const CaDev& D, const uint8_t* s_type, int c, int lz, int z, uint32_t col,
+__device__ int drop_depth_lds(const uint8_t* type, const uint8_t* s_type, int c, int lz, int z, uint32_t col,
                               uint32_t plane, int spill_c, bool has_spill) {
     constexpr int LXY = Halo<TY, TZ>::LXY;
     int zc = z;
@@ -795,7 +846,7 @@ __device__ int drop_depth_lds(const CaDev& D, const uint8_t* s_type, int c, int 
         // type(zc) is AIR here: the cell itself when d == 0, checked one iteration earlier otherwise
         if (zc == 0) return d;  // hasSupport: the floor
         const int l = lz - d;   // halo layer of zc - 1
-        const uint8_t b = l >= 0 ? s_type[c - (d + 1) * LXY] : D.type[col + plane * static_cast<uint32_t>(zc - 1)];
+        const uint8_t b = l >= 0 ? s_type[c - (d + 1) * LXY] : type[col + plane * static_cast<uint32_t>(zc - 1)];
         if (b == kRock || b == kBasalt) return d;
         if (b == kLava && has_spill && zc - 1 < spill_c) return d;
         if (b != kAir) return d + 1 <= kMaxDrop ? -1 : kMaxDrop;
@@ -868,7 +919,9 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
         int own_air = 0;
         const uint32_t row_base = halo_row_lanes<TY, TZ>(P, o);
         const int gx = halo_x(P, o.x0 + static_cast<int>(threadIdx.x & 63u));
-        stage_rows<TY, TZ>(P, D, o, S, has_spill, s_spill, own_air, row_base, gx, opaque_tid());
+        const uint8_t* type = D.type[now & 1u];
+        stage_rows<TY, TZ>(P, D, type, nullptr, false, static_cast<NextTypes<TY, TZ>*>(nullptr), o, S, has_spill,
+                           s_spill, own_air, row_base, gx, opaque_tid());
         if (threadIdx.x == 0) s_n = 0;
         __syncthreads();
         {
@@ -917,7 +970,7 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
                     wrap_axis(qx, P.nx, P.per[0]);
                     wrap_axis(qy, P.ny, P.per[1]);
                     const uint32_t qc = static_cast<uint32_t>(qx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(qy);
-                    const int dd = drop_depth_lds<TY, TZ>(D, s_type, nb, lz, z, qc, plane,
+                    const int dd = drop_depth_lds<TY, TZ>(type, s_type, nb, lz, z, qc, plane,
                                                           has_spill ? s_spill[sc + dx + H::LX * dy] : 0, has_spill);
                     d = dd > d ? dd : d;
                 }
@@ -941,7 +994,40 @@ struct RulesTile {
     int32_t spill[kTileX * TY];  // spill heights of the tile's own columns (lx + 64 * ly)
     // LAVA bits of halo row r over all LX = 66 positions (bit p = halo column p), as three 32-bit parts + 0
     uint32_t row66[H::R][4];
+    // what the step writes into the other type copy (see NextTypes): the previous step's changes (staging), then this
+    // step's on top
+    NextTypes<TY, TZ> next;
 };
+
+// This step's change of own cell e (tile-local lx + 64 * row) into N (born: LAVA, else BASALT; a cell that
+// solidifies may have been born in the previous step: BASALT wins).  LDS atomics: other cells share the words.
+template <int TY, int TZ>
+__device__ inline void next_type(NextTypes<TY, TZ>& N, int e, bool born) {
+    const int q = 2 * (e >> 6) + ((e >> 5) & 1);
+    const uint32_t bit = 1u << (e & 31);
+    atomicOr(&N.w[0][q], bit);
+    if (!born) atomicOr(&N.w[1][q], bit);
+}
+
+// Writes the tile's cells marked in N into the next step's type copy: after them the copy holds the tile's types
+// after the step.  Thread tid takes cells tid + 256 k (row w + 4k, column lx), the cells it built candidates for.
+template <int TY, int TZ>
+__device__ inline void write_types(const CaParams& P, const TileCoord& o, const NextTypes<TY, TZ>& N, uint8_t* tyn,
+                                   unsigned tid) {
+    const int lx = static_cast<int>(tid & 63u), w = static_cast<int>(tid >> 6);
+#pragma unroll
+    for (int k = 0; k < kTileCells / kBlock; ++k) {
+        const int row = w + 4 * k;
+        const int q = 2 * row + (lx >> 5);
+        if ((N.w[0][q] >> (lx & 31)) & 1u) {
+            const uint32_t i = static_cast<uint32_t>(o.x0 + lx) +
+                               static_cast<uint32_t>(P.nx) * (static_cast<uint32_t>(o.y0 + row % TY) +
+                                                              static_cast<uint32_t>(P.ny) *
+                                                                  static_cast<uint32_t>(o.z0 + row / TY));
+            tyn[i] = ((N.w[1][q] >> (lx & 31)) & 1u) ? kBasalt : kLava;
+        }
+    }
+}
 
 // gatherLavaCandidates (LavaCA.hpp:465-519) / the classic vote (LavaCA.hpp:1165-1202) for the AIR cell at LDS
 // position c; bits = the cell's neighbour positions (k as in vca_internal.h) that hold LAVA and that the rules
@@ -958,7 +1044,7 @@ __device__ inline int koff(int k) {
 }
 
 template <int TY, int TZ, class F, class V>
-__device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY, TZ, F>& T, int c, int lz, int z,
+__device__ void gather_lds(const CaParams& P, const uint8_t* type, const RulesTile<TY, TZ, F>& T, int c, int lz, int z,
                            uint32_t col, uint32_t plane, int spill_c, bool has_spill, uint32_t bits, V& v) {
     constexpr int LX = Halo<TY, TZ>::LX, LXY = Halo<TY, TZ>::LXY;
     // positions are taken two at a time so that their LDS reads are in flight together (the vote does not depend
@@ -984,7 +1070,7 @@ __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY
         const int n = c - LXY;
         bool ok = !(z > spill_c || z == 0);
         if (ok && z - 1 > 0) {  // hasSupport(z - 1)
-            const uint8_t b2 = lz >= 1 ? T.S.type[c - 2 * LXY] : D.type[col + plane * static_cast<uint32_t>(z - 2)];
+            const uint8_t b2 = lz >= 1 ? T.S.type[c - 2 * LXY] : type[col + plane * static_cast<uint32_t>(z - 2)];
             ok = b2 == kRock || b2 == kBasalt || (b2 == kLava && has_spill && z - 2 < spill_c);
         }
         if (ok) {
@@ -999,7 +1085,7 @@ __device__ void gather_lds(const CaParams& P, const CaDev& D, const RulesTile<TY
         v.add_if(ok, T.flow[n], T.temp[n] - P.cool_lat, 4);
     }
     // dropDepth of the cell, once, when a side or diagonal-up neighbour may need it (pure function of the cell)
-    const int cdrop = (bits & ~(1u << 22)) ? drop_depth_lds<TY, TZ>(D, T.S.type, c, lz, z, col, plane, spill_c, has_spill)
+    const int cdrop = (bits & ~(1u << 22)) ? drop_depth_lds<TY, TZ>(type, T.S.type, c, lz, z, col, plane, spill_c, has_spill)
                                            : -100;
     const bool diag = P.nb.diagonal_falls != 0;
     // k >= 18: dz > 0 (k == 22 directly above, a free fall); else dz == 0
@@ -1097,8 +1183,8 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
     using H = Halo<TY, TZ>;
     constexpr int LX = H::LX, LY = H::LY;
     __shared__ RulesTile<TY, TZ, F> T;
-    __shared__ uint32_t s_nchg, s_nlist, s_nair;
-    __shared__ uint32_t s_red[4][kBlock / 64];
+    __shared__ uint32_t s_nlist, s_nair;
+    __shared__ uint32_t s_red[5][kBlock / 64];
     __shared__ uint32_t s_wflags[2];  // per-wave flag bytes: [0] staging, [1] end of tile
     __shared__ uint32_t s_bnd0[kBlock / 64];  // per wave: directions with LAVA on the tile's boundary at the step's start
     const uint64_t* tiles = B.active + (par * kShapes + shape) * static_cast<size_t>(B.cap);
@@ -1128,29 +1214,30 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             P.classic ? P.nb.any_mask
                       : ((P.nb.any_mask & ~0x1FFu) | (P.ponding ? (P.nb.any_mask & (1u << 4)) : 0u));
         uint32_t w_lava = 0, w_born = 0;  // wave-uniform counts: lava cells after the step, born cells
+        uint32_t w_chg = 0;               // wave-uniform count: cells that changed type
         uint32_t w_temp = 0;              // wave-uniform count: lava cells whose temperature changed (no record)
         bool w_tchg = false;              // wave-uniform: a cell changed type
         const float* tcur = D.temp[t & 1];
         float* tnext = D.temp[(t + 1) & 1];
+        const uint8_t* tyc = D.type[t & 1];
+        uint8_t* tyn = D.type[(t + 1) & 1];
         const TileCoord o = tile_origin(P, tile);
         __syncthreads();  // the previous tile's LDS reads are done
         // an entry of a slot that left the batch, or of a simulation that stopped after the entry was pushed (its
-        // tile gets no records this step: ca_apply then only checks the slot)
+        // state is final: the step is not run)
         if (!sim.running) continue;
-        if (__builtin_amdgcn_readfirstlane(D.ctrl->stop)) {
-            if (tid == 0) D.tile_nchg[tile] = 0u;
-            continue;
-        }
+        if (__builtin_amdgcn_readfirstlane(D.ctrl->stop)) continue;
         int own_air = 0;
         // lane j: base of halo row j; gx: this lane's column (x == nx is the wrapped x = 0 of a periodic partial
         // tile, -1 for no cell)
         const uint32_t row_base = halo_row_lanes<TY, TZ>(P, o);
         const int gx = halo_x(P, o.x0 + static_cast<int>(tid & 63u));
-        const bool saw_lava =
-            stage_rows<TY, TZ, true>(P, D, o, T.S, has_spill, T.spill, own_air, row_base, gx, tid);
+        // the tile changed a cell's type in the previous step: the other type copy is behind on it
+        const bool changed = t > 0 && __builtin_amdgcn_readfirstlane(D.tile_tchg[tile]) == static_cast<uint32_t>(t);
+        const bool saw_lava = stage_rows<TY, TZ, true, true>(P, D, tyc, tyn, changed, &T.next, o, T.S, has_spill,
+                                                             T.spill, own_air, row_base, gx, tid);
         if (tid == 0) {
             const uint32_t zero = opaque_zero();
-            s_nchg = zero;
             s_nlist = zero;
             s_nair = zero;
         }
@@ -1163,11 +1250,12 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         __syncthreads();
         const uint32_t stage_flags = s_wflags[0];
         if (!(stage_flags & 0x01010101u)) {
-            // no lava in the tile or its halo: no cell changes; the tile keeps AIR iff it has AIR
+            // no lava in the tile or its halo: no cell changes; the tile keeps AIR iff it has AIR, and the next type
+            // copy takes the previous step's changes
             const int any_air = (stage_flags & 0x02020202u) != 0;
+            if (changed) write_types<TY, TZ>(P, o, T.next, tyn, tid);
             if (tid == 0) {
                 D.tile_flags[tile] = static_cast<uint8_t>(any_air ? kFlagAir : 0);
-                D.tile_nchg[tile] = 0;
                 atomicAdd(&D.part[tile % kPartRows].staged, 1ULL);
             }
             continue;
@@ -1323,7 +1411,6 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             const int x = o.x0 + lx, y = o.y0 + ly, z = o.z0 + lz;
             const int c = (lx + 1) + LX * ((ly + 1) + LY * (lz + 1));
             bool want = false, stay = false, newborn = false, solid = false, cooled = false;
-            Change ch;
             uint32_t i = 0;
             if (live) {
                 const uint32_t col = static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y);
@@ -1348,7 +1435,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                         if (tc < P.solidus) {
                             want = true;
                             solid = true;
-                            ch = Change{i, f, tc, stamp, kBasalt, kKindSolidify};
+                            D.solid[i] = stamp;  // no rule reads it
                         } else {
                             stay = true;
                             tile_lava = 1;
@@ -1386,18 +1473,18 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                         if (bits) {
                             const int spill_c = has_spill ? T.spill[lx + kTileX * ly] : 0;
                             VoteFirst v;
-                            gather_lds(P, D, T, c, lz, z, col, plane, spill_c, has_spill, bits, v);
+                            gather_lds(P, tyc, T, c, lz, z, col, plane, spill_c, has_spill, bits, v);
                             has = v.n > 0;
                             if (v.overflow) {
                                 // a second flowId (flows meet): the gather again with two slots; a third one: the
                                 // full vote over the accepted positions (rare: where three flows meet)
                                 VoteTwo v2;
-                                gather_lds(P, D, T, c, lz, z, col, plane, spill_c, has_spill, bits, v2);
+                                gather_lds(P, tyc, T, c, lz, z, col, plane, spill_c, has_spill, bits, v2);
                                 if (!v2.overflow) {
                                     v2.winner(h_index, wf, wt);
                                 } else {
                                     VoteMask vm;
-                                    gather_lds(P, D, T, c, lz, z, col, plane, spill_c, has_spill, bits, vm);
+                                    gather_lds(P, tyc, T, c, lz, z, col, plane, spill_c, has_spill, bits, vm);
                                     vote_positions(P, T, c, vm.acc, h_index, wf, wt);
                                 }
                             } else if (has) {
@@ -1407,7 +1494,9 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                     }
                     if (has && (P.classic || wt >= P.solidus)) {
                         want = true;
-                        ch = Change{i, wf, P.classic ? 0.0f : wt, stamp, kLava, kKindBorn};
+                        // the cell is AIR at the step's start, so no rule of this step reads its flowId
+                        D.flow[i] = wf;
+                        D.arrival[i] = stamp;
                         if (!P.classic) tnext[i] = wt;  // read by the next step as the new lava cell's temperature
                         newborn = true;
                         tile_lava = 1;
@@ -1419,11 +1508,12 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             w_lava += static_cast<uint32_t>(__popcll(__ballot(stay || newborn)));
             w_born += static_cast<uint32_t>(__popcll(__ballot(newborn)));
             w_temp += static_cast<uint32_t>(__popcll(__ballot(cooled)));
+            w_chg += static_cast<uint32_t>(__popcll(__ballot(want)));
             w_tchg = w_tchg || __ballot(newborn || solid) != 0;
             if (stay || newborn) bnd |= boundary_dirs(lx, ly, lz, xm, ym, zm);
-            Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
-            const uint32_t slot = wave_append(want, &s_nchg);
-            if (want) rec[slot] = ch;
+            // the new type goes into the next type copy at the tile's end (a born cell was not a change of the
+            // previous step; a cell that solidifies may have been born then: the new type wins)
+            if (want) next_type(T.next, e, newborn);
         }
         // tile flags, record counts and the tile's counters (row tile % kPartRows of its simulation); every
         // record is one changed cell
@@ -1437,13 +1527,14 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                 s_red[1][tid >> 6] = w_born;
                 s_red[2][tid >> 6] = bnd;
                 s_red[3][tid >> 6] = w_temp;
+                s_red[4][tid >> 6] = w_chg;
                 reinterpret_cast<uint8_t*>(&s_wflags[1])[tid >> 6] = static_cast<uint8_t>(f);
             }
         }
         __syncthreads();
         // a tile holding lava after the step lists the next step's tiles (lava front: itself and the neighbours it
         // holds lava next to) or marks its neighbours for the next step's scan (any lava: visit; lava next to each
-        // neighbour: stage)
+        // neighbour: stage); a tile that changed a cell's type is listed too (it brings the other copy up to date)
         if ((s_wflags[1] & 0x01010101u) != 0) {
             const int tx = o.x0 / kTileX, tyy = o.y0 / TY, tz = o.z0 / TZ;
             const uint32_t bnd_all = s_red[2][0] | s_red[2][1] | s_red[2][2] | s_red[2][3];
@@ -1462,60 +1553,23 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             const uint32_t fl = (wf | wf >> 8 | wf >> 16 | wf >> 24) & 0xFFu;
             D.tile_flags[tile] = static_cast<uint8_t>(fl & (kFlagLava | kFlagAir));
             if (fl & 4u) D.tile_tchg[tile] = static_cast<uint32_t>(t) + 1u;
-            const uint32_t nchg = s_nchg;
-            D.tile_nchg[tile] = nchg;
-            uint32_t r0 = 0, r1 = 0, r3 = 0;
+            uint32_t r0 = 0, r1 = 0, r3 = 0, nchg = 0;
             for (int w = 0; w < kBlock / 64; ++w) {
                 r0 += s_red[0][w];
                 r1 += s_red[1][w];
                 r3 += s_red[3][w];
+                nchg += s_red[4][w];
             }
             CaPart* row = &D.part[tile % kPartRows];
             if (r0) atomicAdd(&row->lava_after, static_cast<unsigned long long>(r0));
             if (r1) atomicAdd(&row->born, static_cast<unsigned long long>(r1));
-            // changed cells: every record (born, solidified) and every lava cell whose temperature changed
+            // changed cells: every type change (born, solidified) and every lava cell whose temperature changed
             if (nchg + r3) atomicAdd(&row->changed, static_cast<unsigned long long>(nchg + r3));
             atomicAdd(&row->staged, 1ULL);
         }
-    }
-}
-
-// Scatters each visited tile's change records into the state (LavaCA.hpp:1052-1066).
-__global__ void __launch_bounds__(kBlock) ca_apply_kernel(BatchDev B, int par) {
-    // one wave per staged tile
-    const int lane = static_cast<int>(threadIdx.x & 63u);
-    const uint32_t waves = gridDim.x * (kBlock / 64);
-#pragma unroll 1
-    for (int k = 0; k < kShapes; ++k) {
-        const uint32_t nact = B.ctrl->nactive[par][k];
-        const uint64_t* tiles = B.active + (par * kShapes + k) * static_cast<size_t>(B.cap);
-        for (uint32_t s = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); s < nact; s += waves) {
-            const uint64_t ent = tiles[s];
-            const uint32_t e = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(ent))));
-            const uint32_t tstep = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(ent >> 32))));
-            const CaSim& sim = g_sims[B.sim_base + (e >> kTileBits)];
-            // entries of a slot that left the batch: the rules kernel skipped them (a stopped simulation's tiles have
-            // their record counts zeroed by the rules kernel)
-            if (!sim.running) continue;
-            const CaDev& D = sim.D;
-            const uint32_t tile = e & kTileMask;
-            const uint32_t n = D.tile_nchg[tile];
-            const Change* rec = D.changes + static_cast<size_t>(tile) * kTileCells;
-            // the step's own temperature copy: a solidified cell keeps its last temperature in both copies (the
-            // rules kernel wrote the other one); a born cell's temperature is in the next step's copy already
-            float* tcur = D.temp[tstep & 1u];
-            for (uint32_t j = static_cast<uint32_t>(lane); j < n; j += 64) {
-                const Change c = rec[j];
-                D.type[c.idx] = c.type;
-                if (c.kind == kKindBorn) {
-                    D.flow[c.idx] = c.flow;
-                    D.arrival[c.idx] = c.stamp;
-                } else {
-                    D.solid[c.idx] = c.stamp;
-                    tcur[c.idx] = c.temp;
-                }
-            }
-        }
+        // last: a wait for the list atomics' returns above would also wait for these stores
+        if (changed || (s_red[4][0] | s_red[4][1] | s_red[4][2] | s_red[4][3]))
+            write_types<TY, TZ>(P, o, T.next, tyn, tid);
     }
 }
 
@@ -1524,8 +1578,9 @@ __global__ void __launch_bounds__(kBlock) ca_apply_kernel(BatchDev B, int par) {
 __global__ void __launch_bounds__(kBlock) ca_finish_kernel(BatchDev B) { finish_slot(B, blockIdx.x); }
 
 // Initial list of a lava-front simulation that joins a batch: every tile that holds lava or has a neighbour tile with
-// lava (a superset of the tiles whose 1-cell halo holds lava: the extra tiles settle in the rules kernel), for the
-// simulation's current step, into list copy par.  One lane per tile, one list atomic per block.
+// lava (a superset of the tiles whose 1-cell halo holds lava: the extra tiles settle in the rules kernel), and every
+// tile that changed a cell's type in the last step, for the simulation's current step, into list copy par.  One lane
+// per tile, one list atomic per block.
 __global__ void __launch_bounds__(kBlock) ca_seed_kernel(BatchDev B, uint32_t slot, int par) {
     __shared__ uint32_t s_cnt[kBlock / 64];
     __shared__ uint32_t s_base;
@@ -1536,6 +1591,9 @@ __global__ void __launch_bounds__(kBlock) ca_seed_kernel(BatchDev B, uint32_t sl
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     bool want = false;
     if (t < P.ntiles && !D.ctrl->stop) {
+        // a tile that changed a cell's type in the last step brings the other type copy up to date in the next
+        const int32_t now = D.ctrl->steps;
+        want = now > 0 && D.tile_tchg[t] == static_cast<uint32_t>(now);
         const uint32_t r = div_magic(t, P.div_ntx_m, P.div_ntx_p);
         const int tx = static_cast<int>(t - r * static_cast<uint32_t>(P.ntx));
         const uint32_t tz_ = div_magic(r, P.div_nty_m, P.div_nty_p);
@@ -2134,8 +2192,8 @@ void rules_shape(int k, dim3 g, hipStream_t s, hipEvent_t a, hipEvent_t b, const
     }
 }
 
-// One step.  Kernel order: [ca_front] ca_descent (+ the previous step's finish when L.fin_prev) ca_rules ca_apply
-// [ca_finish]; ca_front and ca_finish only when a member needs the scan.  Every kernel of a step reads the
+// One step.  Kernel order: [ca_front] ca_descent (+ the previous step's finish when L.fin_prev) ca_rules [ca_finish];
+// ca_front and ca_finish only when a member needs the scan.  Every kernel of a step reads the
 // list copy L.par.  Without lava mode there is no D(n) to compute, but ca_descent still runs (it clears the previous
 // copy's counts and does the pending finish).
 void ca_launch_step(const BatchDev& B, const StepLaunch& L, hipStream_t s, hipEvent_t* ev, bool* rec) {
@@ -2157,8 +2215,7 @@ void ca_launch_step(const BatchDev& B, const StepLaunch& L, hipStream_t s, hipEv
         else
             rules_shape<int32_t>(shapes[i], gt, s, E.start(2, i == 0), E.stop(2, i == ns - 1), B, L.par);
     }
-    launch(ca_apply_kernel, dim3((gt.x + 3) / 4), s, E.start(3, true), E.stop(3, true), B, L.par);
-    if (L.scan) launch(ca_finish_kernel, dim3(B.nslots), s, E.start(4, true), E.stop(4, true), B);
+    if (L.scan) launch(ca_finish_kernel, dim3(B.nslots), s, E.start(3, true), E.stop(3, true), B);
 }
 
 void ca_launch_finish(const BatchDev& B, hipStream_t s, hipEvent_t* ev) {
@@ -2333,7 +2390,7 @@ __device__ inline void for_domain_cells(uint64_t n, const uint8_t* type, Fn&& fn
 // with more flows each cell goes to the totals directly, spread over many addresses.
 constexpr int kFlowLds = 256;
 
-__global__ void __launch_bounds__(kBlock) ca_flow_sums_kernel(CaParams P, CaDev D, const int32_t* vent_xy,
+__global__ void __launch_bounds__(kBlock) ca_flow_sums_kernel(CaParams P, CaDev D, int par, const int32_t* vent_xy,
                                                                int32_t nflows, unsigned long long* lava,
                                                                unsigned long long* basalt,
                                                                unsigned long long* maxd2) {
@@ -2349,7 +2406,7 @@ __global__ void __launch_bounds__(kBlock) ca_flow_sums_kernel(CaParams P, CaDev 
         __syncthreads();
     }
     const uint64_t nx = static_cast<uint64_t>(P.nx), ny = static_cast<uint64_t>(P.ny);
-    for_domain_cells(nx * ny * static_cast<uint64_t>(P.nz), D.type, [&](uint64_t i, uint32_t t) {
+    for_domain_cells(nx * ny * static_cast<uint64_t>(P.nz), D.type[par], [&](uint64_t i, uint32_t t) {
         const int32_t f = D.flow[i];
         if (f < 1 || f > nflows) return;
         const uint64_t r = i / nx;
@@ -2374,9 +2431,20 @@ __global__ void __launch_bounds__(kBlock) ca_flow_sums_kernel(CaParams P, CaDev 
     }
 }
 
+// Before a read of the temperatures after `steps` steps (copy par = steps & 1): every BASALT cell's last temperature
+// into that copy (the step that solidified the cell at step stamp s wrote it into copy (s + 1) & 1 only).
+__global__ void __launch_bounds__(kBlock) ca_final_temps_kernel(uint64_t n, const uint8_t* type, const uint16_t* solid,
+                                                                float* t0, float* t1, int par) {
+    float* dst = par ? t1 : t0;
+    const float* src = par ? t0 : t1;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * kBlock)
+        if (type[i] == kBasalt && ((solid[i] + 1) & 1) != par) dst[i] = src[i];
+}
+
 // Marks seen[|dx|] for the LAVA / BASALT cells of flow `flow` whose squared distance to (vx, vy) is d2: which
 // lattice points of that circle the flow reaches (the reach is the largest hypot among them).
-__global__ void __launch_bounds__(kBlock) ca_flow_points_kernel(CaParams P, CaDev D, int32_t flow, int32_t vx,
+__global__ void __launch_bounds__(kBlock) ca_flow_points_kernel(CaParams P, CaDev D, int par, int32_t flow, int32_t vx,
                                                                  int32_t vy, unsigned long long d2, uint32_t* seen,
                                                                  uint32_t cap) {
     const uint32_t rows = static_cast<uint32_t>(P.ny) * static_cast<uint32_t>(P.nz);
@@ -2384,7 +2452,7 @@ __global__ void __launch_bounds__(kBlock) ca_flow_points_kernel(CaParams P, CaDe
         const int y = static_cast<int>(r % static_cast<uint32_t>(P.ny));
         const uint32_t row = r * static_cast<uint32_t>(P.nx);
         for (int x = static_cast<int>(threadIdx.x); x < P.nx; x += kBlock) {
-            const uint8_t t = D.type[row + static_cast<uint32_t>(x)];
+            const uint8_t t = D.type[par][row + static_cast<uint32_t>(x)];
             if (t != kLava && t != kBasalt) continue;
             if (D.flow[row + static_cast<uint32_t>(x)] != flow) continue;
             const long long dx = x - vx, dy = y - vy;
@@ -2618,19 +2686,24 @@ __global__ void __launch_bounds__(kBlock) mc_list_scatter_kernel(McListSrc L, co
     }
 }
 
-void ca_launch_flow_sums(const CaParams& P, const CaDev& D, const int32_t* vent_xy, int32_t nflows,
+void ca_launch_flow_sums(const CaParams& P, const CaDev& D, int par, const int32_t* vent_xy, int32_t nflows,
                          unsigned long long* lava, unsigned long long* basalt, unsigned long long* maxd2,
                          hipStream_t s) {
     const uint64_t n = static_cast<uint64_t>(P.nx) * P.ny * P.nz;
-    hipLaunchKernelGGL(ca_flow_sums_kernel, dim3(grid_for((n + 15) / 16, 2048)), dim3(kBlock), 0, s, P, D, vent_xy,
+    hipLaunchKernelGGL(ca_flow_sums_kernel, dim3(grid_for((n + 15) / 16, 2048)), dim3(kBlock), 0, s, P, D, par, vent_xy,
                        nflows,
                        lava, basalt, maxd2);
 }
 
-void ca_launch_flow_points(const CaParams& P, const CaDev& D, int32_t flow, int32_t vx, int32_t vy,
+void ca_launch_final_temps(const CaDev& D, uint64_t n, int par, hipStream_t s) {
+    hipLaunchKernelGGL(ca_final_temps_kernel, dim3(grid_for(n, 8192)), dim3(kBlock), 0, s, n, D.type[par], D.solid,
+                       D.temp[0], D.temp[1], par);
+}
+
+void ca_launch_flow_points(const CaParams& P, const CaDev& D, int par, int32_t flow, int32_t vx, int32_t vy,
                            unsigned long long d2, uint32_t* seen, uint32_t cap, hipStream_t s) {
     const uint64_t rows = static_cast<uint64_t>(P.ny) * P.nz;
-    hipLaunchKernelGGL(ca_flow_points_kernel, dim3(blocks_for(rows, 16384)), dim3(kBlock), 0, s, P, D, flow, vx, vy,
+    hipLaunchKernelGGL(ca_flow_points_kernel, dim3(blocks_for(rows, 16384)), dim3(kBlock), 0, s, P, D, par, flow, vx, vy,
                        d2, seen, cap);
 }
 

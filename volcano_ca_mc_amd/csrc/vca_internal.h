@@ -88,17 +88,6 @@ __host__ __device__ inline uint32_t div_magic(uint32_t x, uint32_t m, int32_t p)
     return static_cast<uint32_t>((static_cast<uint64_t>(x) * m) >> p);
 }
 
-// One change of one cell in one step (applied after every rule of the step was evaluated).
-struct Change {
-    uint32_t idx;
-    int32_t flow;
-    float temp;
-    uint16_t stamp;
-    uint8_t type;
-    uint8_t kind;                 // kSolidify, kBorn (temperature changes of lava cells are written directly)
-};
-constexpr uint8_t kKindSolidify = 1, kKindBorn = 2;
-
 struct CaCtrl {
     int32_t steps;
     int32_t stop;
@@ -123,12 +112,18 @@ constexpr int kMcCounters = 4;
 constexpr int kMcSlots = 16;
 
 struct CaDev {
-    uint8_t* type;
+    // cell types at the start of a step, by the step's parity: step t reads type[t & 1] and writes the tile's cells
+    // that differ into type[(t + 1) & 1] -- its own changes, and the changes of step t - 1 that copy does not hold yet
+    // (every tile that changed a cell in step t - 1 is visited in step t); type[steps & 1] is the state after the
+    // last step
+    uint8_t* type[2];
+    // flowId, solidification and arrival steps: written in place by the step that changes the cell (a cell becomes
+    // LAVA only from AIR, and no rule reads the flowId of a cell that is not LAVA at the step's start)
     int32_t* flow;
     // temperatures at the start of a step, by the step's parity: step t reads temp[t & 1] and writes every lava cell's
-    // new temperature to temp[(t + 1) & 1] (no record, no scatter); a cell that solidifies in step t gets its last
-    // temperature into temp[t & 1] as well (by ca_apply, after every read of the step), so both copies hold the
-    // temperatures of every cell that is not lava, and temp[steps & 1] is the state after the last step
+    // new temperature (also the last one of a cell that solidifies) to temp[(t + 1) & 1].  temp[steps & 1] is the
+    // state after the last step except for BASALT cells, whose last temperature is in temp[(solid + 1) & 1] (no rule
+    // reads the temperature of a cell that is not LAVA); ca_final_temps copies those before a read of the state
     float* temp[2];
     uint16_t* solid;
     uint16_t* arrival;
@@ -142,8 +137,6 @@ struct CaDev {
     uint32_t* tile_tchg;          // per tile: 1 + the last step in which a cell of the tile changed type (0: never)
     uint32_t* listed;             // per tile: the step whose tile list holds the tile (lava front: the rules kernel
                                   // pushes the next step's tiles; the stamp lists each tile once)
-    Change* changes;              // kTileCells record slots per tile: tile t owns slots t*kTileCells ..
-    uint32_t* tile_nchg;          // records of each visited tile this step (from slot 0 up)
     CaPart* part;                 // kPartRows rows
     CaCtrl* ctrl;
     unsigned long long* hist_active;
@@ -257,7 +250,7 @@ struct StepLaunch {
 };
 // Kernel timing: kCaKernels kernels in launch order; ev[2k], ev[2k + 1] (when non-null) are recorded by kernel k's
 // own dispatch (start and end); a kernel not launched in this step leaves its pair unrecorded (flag false).
-constexpr int kCaKernels = 5;     // front, descent, rules, apply, finish
+constexpr int kCaKernels = 4;     // front, descent, rules, finish
 void ca_launch_step(const BatchDev& B, const StepLaunch& L, hipStream_t s, hipEvent_t* ev, bool* recorded);
 // The finish of the last step enqueued (after a run of steps whose finish is done by the next step's first kernel);
 // ev (nullable): the start / end events of the launch.
@@ -283,10 +276,13 @@ void ca_write_sim(uint32_t index, const CaSim* sim, hipStream_t s);
 
 // After the CA: per-flow LAVA / BASALT counts and largest squared vent distance (arrays of nflows + 1, zeroed by
 // the caller; vent_xy = x, y of flow f at 2*(f-1)); seen[|dx|] = 1 for the cells of one flow at squared distance d2.
-void ca_launch_flow_sums(const CaParams& P, const CaDev& D, const int32_t* vent_xy, int32_t nflows,
+// The last temperature of every BASALT cell into temperature copy par (= steps & 1) before that copy is read.
+void ca_launch_final_temps(const CaDev& D, uint64_t n, int par, hipStream_t s);
+// par: the type copy holding the state (the step count's parity)
+void ca_launch_flow_sums(const CaParams& P, const CaDev& D, int par, const int32_t* vent_xy, int32_t nflows,
                          unsigned long long* lava, unsigned long long* basalt, unsigned long long* maxd2,
                          hipStream_t s);
-void ca_launch_flow_points(const CaParams& P, const CaDev& D, int32_t flow, int32_t vx, int32_t vy,
+void ca_launch_flow_points(const CaParams& P, const CaDev& D, int par, int32_t flow, int32_t vx, int32_t vy,
                            unsigned long long d2, uint32_t* seen, uint32_t cap, hipStream_t s);
 // Cooling mobility of every cell from the CA result; red[3] = {0, 0xFFFFFFFF, 0} on entry (end, tmin, tmax).
 void mc_launch_cooling(uint64_t n, const uint8_t* type, const uint16_t* solid, const uint16_t* arrival,
