@@ -835,7 +835,7 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const uint8
 
 // dropDepth (LavaCA.hpp:404-414) of the AIR cell at halo position c, which sits in halo layer lz + 1 of the
 // column col (global) with spill height spill_c.  Layers held in LDS are read there, deeper ones from global
-// memory.
+// memory (one load per cell: reading several cells ahead measured slower, deep drops are rare).
 template <int TY, int TZ>
 __device__ int drop_depth_lds(const uint8_t* type, const uint8_t* s_type, int c, int lz, int z, uint32_t col,
                               uint32_t plane, int spill_c, bool has_spill) {
@@ -902,10 +902,11 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
             constexpr int NB = (kMaxDrop + 1 + TZ - 1) / TZ;  // tile layers below
             constexpr int NR = 9 * (NB + 1);
             const int tx = o.x0 / kTileX, ty = o.y0 / TY, tz = o.z0 / TZ;
+            // the stop flag, the tile's flags and the marks of the tiles below are read together
             const bool go = sim.running && !P.classic && !__builtin_amdgcn_readfirstlane(D.ctrl->stop) &&
                             (__builtin_amdgcn_readfirstlane(D.tile_flags[tile]) & kFlagLava);
             bool dirty = false;
-            for (int q = static_cast<int>(threadIdx.x); go && q < NR; q += kBlock) {
+            for (int q = static_cast<int>(threadIdx.x); sim.running && !P.classic && q < NR; q += kBlock) {
                 int qx = tx + q % 3 - 1, qy = ty + (q / 3) % 3 - 1;
                 const int qz = tz - q / 9;
                 if (qz >= 0 && wrap_axis(qx, P.ntx, P.per[0]) && wrap_axis(qy, P.nty, P.per[1]))
@@ -914,7 +915,7 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
                                                      (static_cast<uint32_t>(qy) +
                                                       static_cast<uint32_t>(P.nty) * static_cast<uint32_t>(qz))] == now;
             }
-            if (!__syncthreads_or(dirty)) continue;
+            if (!__syncthreads_or(go && dirty)) continue;
         }
         int own_air = 0;
         const uint32_t row_base = halo_row_lanes<TY, TZ>(P, o);
@@ -1226,7 +1227,9 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         // an entry of a slot that left the batch, or of a simulation that stopped after the entry was pushed (its
         // state is final: the step is not run)
         if (!sim.running) continue;
-        if (__builtin_amdgcn_readfirstlane(D.ctrl->stop)) continue;
+        // the stop flag is read together with the staging loads (staging writes LDS only); a stopped simulation's
+        // tile goes no further
+        const int32_t stop = D.ctrl->stop;
         int own_air = 0;
         // lane j: base of halo row j; gx: this lane's column (x == nx is the wrapped x = 0 of a periodic partial
         // tile, -1 for no cell)
@@ -1236,6 +1239,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         const bool changed = t > 0 && __builtin_amdgcn_readfirstlane(D.tile_tchg[tile]) == static_cast<uint32_t>(t);
         const bool saw_lava = stage_rows<TY, TZ, true, true>(P, D, tyc, tyn, changed, &T.next, o, T.S, has_spill,
                                                              T.spill, own_air, row_base, gx, tid);
+        if (__builtin_amdgcn_readfirstlane(stop)) continue;
         if (tid == 0) {
             const uint32_t zero = opaque_zero();
             s_nlist = zero;
