@@ -348,9 +348,16 @@ SCENARIOS = {
 
 
 def _main(argv):
-    """python -m volcano_ca_mc_amd.workloads write <scenario> <folder>: writes config JSON + CSV."""
-    if len(argv) != 3 or argv[0] != "write" or argv[1] not in SCENARIOS:
-        raise SystemExit("usage: python -m volcano_ca_mc_amd.workloads write {%s} <folder>" % "|".join(SCENARIOS))
+    """python -m volcano_ca_mc_amd.workloads write <scenario> <folder>: writes config JSON + CSV (scenario
+    batched_mix: the 256 jobs of the uneven mix, one config JSON + CSV each, for engine --batch <folder>)."""
+    if len(argv) != 3 or argv[0] != "write" or argv[1] not in list(SCENARIOS) + ["batched_mix"]:
+        raise SystemExit("usage: python -m volcano_ca_mc_amd.workloads write {%s|batched_mix} <folder>"
+                         % "|".join(SCENARIOS))
+    if argv[1] == "batched_mix":
+        for wl in batched_mix():
+            wl.write(argv[2])
+        print(argv[2])
+        return
     print(SCENARIOS[argv[1]]().write(argv[2]))
 
 
