@@ -636,7 +636,11 @@ struct Batch {
         L.scan = any_scan();
         L.par = par;
         L.fin_prev = fin_prev;
-        L.grid_tiles = grid_tiles;
+        uint64_t tiles[kShapes] = {};
+        for (const vca_ca* c : members)
+            if (c) tiles[c->shape] += c->P.ntiles;
+        for (int k = 0; k < kShapes; ++k)
+            L.grid[k] = static_cast<int>(std::min<uint64_t>(static_cast<uint64_t>(grid_tiles), tiles[k]));
         return L;
     }
     // One step on stream s (no timing); advances the parity and the pending finish.

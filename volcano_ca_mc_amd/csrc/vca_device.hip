@@ -2202,7 +2202,6 @@ void rules_shape(int k, dim3 g, hipStream_t s, hipEvent_t a, hipEvent_t b, const
 // copy's counts and does the pending finish).
 void ca_launch_step(const BatchDev& B, const StepLaunch& L, hipStream_t s, hipEvent_t* ev, bool* rec) {
     const StepEvents E{ev, rec};
-    const dim3 gt(static_cast<unsigned>(L.grid_tiles));
     if (L.scan) launch(ca_front_kernel, dim3(grid_for(B.ntiles)), s, E.start(0, true), E.stop(0, true), B, L.shapes, L.par);
     int shapes[kShapes], ns = 0;
     for (int k = 0; k < kShapes; ++k)
@@ -2210,10 +2209,11 @@ void ca_launch_step(const BatchDev& B, const StepLaunch& L, hipStream_t s, hipEv
     // the first descent launch clears the counts and, with nslots blocks before the tile blocks, does the finish
     for (int i = 0; i < ns; ++i) {
         const uint32_t fin_blocks = i == 0 && L.fin_prev ? B.nslots : 0u;
-        descent_shape(shapes[i], dim3(gt.x + fin_blocks), s, E.start(1, i == 0), E.stop(1, i == ns - 1), B, L.par,
-                      fin_blocks);
+        descent_shape(shapes[i], dim3(static_cast<unsigned>(L.grid[shapes[i]]) + fin_blocks), s, E.start(1, i == 0),
+                      E.stop(1, i == ns - 1), B, L.par, fin_blocks);
     }
     for (int i = 0; i < ns; ++i) {
+        const dim3 gt(static_cast<unsigned>(L.grid[shapes[i]]));
         if (L.flow16)
             rules_shape<int16_t>(shapes[i], gt, s, E.start(2, i == 0), E.stop(2, i == ns - 1), B, L.par);
         else

@@ -246,7 +246,8 @@ struct StepLaunch {
                                   // first kernel of the next step
     int par;                      // parity of the step (which list copy it reads)
     bool fin_prev;                // the previous step's finish is still to be done (by this step's first kernel)
-    int grid_tiles;
+    int grid[kShapes];            // blocks of the tile kernels per shape: at most the batch's grid, at most the tiles
+                                  // of the members with that shape (a list holds each tile at most once)
 };
 // Kernel timing: kCaKernels kernels in launch order; ev[2k], ev[2k + 1] (when non-null) are recorded by kernel k's
 // own dispatch (start and end); a kernel not launched in this step leaves its pair unrecorded (flag false).
