@@ -135,7 +135,7 @@ def _traffic(workload: str, k: str):
     return t.get(workload, {}).get(k)
 
 
-def workload_line(kind: str, a, with_cpu: bool) -> dict:
+def workload_line(kind: str, a, with_cpu: bool, engine_args=()) -> dict:
     """K timed jobs (after W warm-up jobs) of one scenario through the engine."""
     from volcano_ca_mc_amd import workloads as W
     wl = {"caldera": W.caldera_3d, "cone": W.cone_3d, "cross": W.cross_section_2d}[kind]()
@@ -145,7 +145,7 @@ def workload_line(kind: str, a, with_cpu: bool) -> dict:
         cfg_path = wl.write(tmp)
         # kernel times from HIP events on every TIMING_SAMPLE-th CA step / MC sweep of the timed jobs: timing every
         # launch would add its own events' cost to the measured caSim
-        extra = ["--timing-sample", str(TIMING_SAMPLE), "--set", "output.format=none"]
+        extra = ["--timing-sample", str(TIMING_SAMPLE), "--set", "output.format=none", *engine_args]
         runs = []
         for i in range(a.warmup + a.steps):
             t = run_engine(ENGINE, cfg_path, os.path.join(tmp, f"run{i}"), extra)
@@ -172,12 +172,15 @@ def workload_line(kind: str, a, with_cpu: bool) -> dict:
     mean = {k: v[0] / v[1] for k, v in ktot.items() if v[1]}
     w = r0.get("detail", {}).get("caWork", {})
     tile_cells = w.get("tileCells", 1024)
-    staged = w.get("tilesStaged", 0) * tile_cells / max(1, steps)      # cells ca_rules stages per launch
-    visited = w.get("tilesVisited", 0) * tile_cells / max(1, steps)    # cells of every visited tile per step
+    # steps the step kernels ran; the rest (a quiescent tail: lava only cooling) completed by the fast-forward
+    launched = w.get("stepsLaunched", steps)
+    staged = w.get("tilesStaged", 0) * tile_cells / max(1, launched)      # cells ca_rules stages per launch
+    visited = w.get("tilesVisited", 0) * tile_cells / max(1, launched)    # cells of every visited tile per step
     step_ms = ca_ms / max(1, steps)
     rules_bytes = CA_BYTES_PER_CELL * staged
     rules_ach = rules_bytes / (mean["ca_rules"] / 1000.0) / 1e9
-    step_ach = CA_BYTES_PER_CELL * visited / (step_ms / 1000.0) / 1e9
+    # the whole CA phase: 26 B per cell of every visited tile over caSim
+    step_ach = CA_BYTES_PER_CELL * w.get("tilesVisited", 0) * tile_cells / (ca_ms / 1000.0) / 1e9
     step_kernels = [k for k in mean if k.startswith("ca_") and k != "ca_step"]
     roofline = {"bound": "hbm", "achieved": round(rules_ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(rules_ach / HBM_PEAK_GBS, 4), "traffic": _traffic(kind, "ca_rules"),
@@ -189,8 +192,8 @@ def workload_line(kind: str, a, with_cpu: bool) -> dict:
                 "launches_per_step": round(sum(ktot[k][1] for k in step_kernels) / max(1, ktot["ca_rules"][1]), 3)}
     step_roofline = {"achieved": round(step_ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(step_ach / HBM_PEAK_GBS, 4), "step_ms": round(step_ms, 5),
-                     "visited_cells_per_step": round(visited),
-                     "bytes_per_unit": f"{CA_BYTES_PER_CELL} B per cell of every visited tile, whole step"}
+                     "visited_cells_per_launched_step": round(visited),
+                     "bytes_per_unit": f"{CA_BYTES_PER_CELL} B per cell of every visited tile, over the whole caSim"}
     mc_roofline = None
     mst = r0["_stats"].get("mc", {})
     if dom and ("mc_sweeps" in mean or "mc_color" in mean):
@@ -224,7 +227,9 @@ def workload_line(kind: str, a, with_cpu: bool) -> dict:
                      f"100 sweeps"}[kind]
     return {"value": cells * steps / (ca_ms / 1000.0), "mc_value": dom * mcs / (mc_ms / 1000.0),
             "p50_job_ms": statistics.median(lat), "p99_job_ms": p99, "ca_ms": ca_ms, "mc_ms": mc_ms,
-            "config": {"workload": desc, "cells": cells, "ca_steps": steps, "mc_domain_cells": dom, "mc_sweeps": mcs,
+            "config": {"workload": desc, "cells": cells, "ca_steps": steps, "ca_steps_launched": launched,
+                       "ca_steps_fast_forwarded": w.get("fastForwardSteps", 0),
+                       "ca_fast_forward_from": w.get("fastForwardFrom", -1), "mc_domain_cells": dom, "mc_sweeps": mcs,
                        "mc_colours": colours, "parallelism": "single GPU"},
             "roofline": roofline, "step_roofline": step_roofline, "mc_roofline": mc_roofline, "cpu_baseline": cpu}
 
@@ -349,6 +354,10 @@ def main() -> int:
     line = workload_line(a.workload, a, not a.no_cpu)
     extra = {}
     if not a.no_extra:
+        # the same workload with every CA step launched (no quiescent fast-forward): what the step kernels alone give
+        log(f"extra: {a.workload} with every step launched")
+        e = workload_line(a.workload, argparse.Namespace(steps=2, warmup=0), False, ["--no-fast-forward"])
+        extra["every_step_launched"] = {k: e[k] for k in ("value", "ca_ms", "config", "roofline", "step_roofline")}
         if a.workload != "cone":
             log("extra: cone workload")
             sub = argparse.Namespace(steps=3, warmup=1)

@@ -93,8 +93,18 @@ int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* desc, const int32_t* height, 
                   const vca_vent* vents, int32_t nvents, vca_ca** out);
 int vca_ca_destroy(vca_ca* ca);
 /* LavaCA::step x n (LavaCA.hpp:230-256): enqueues up to max_new_steps synchronous steps; steps after a
- * stop rule fired are no-ops on the device.  Blocks until done, returns the step count and stop reason. */
+ * stop rule fired are no-ops on the device.  Blocks until done, returns the step count and stop reason.
+ * When the run would reach the simulation's last step and the simulation has become quiescent (lava mode, the
+ * eruption over, and in the last step no AIR cell had LAVA at a position the rules read: from then on every step
+ * only cools the lava cells, LavaCA.hpp:1129-1147), the remaining steps are completed by the fast-forward: each lava
+ * cell's cooling replayed on its own, with the same float arithmetic, histories and stop rule (finishStep,
+ * LavaCA.hpp:1206-1229) -- the state, histories and step count are those of stepping on. */
 int vca_ca_run(vca_ca* ca, int32_t max_new_steps, int32_t* steps_done, int32_t* stop_reason);
+/* The fast-forward of vca_ca_run, vca_ca_run_noalloc and vca_batch_status / vca_batch_run (no counterpart in the
+ * reference, which steps on): on (1, the default) or off (0, every step is launched).  Results do not depend on it. */
+int vca_ca_set_fast_forward(vca_ca* ca, int32_t on);
+/* The step the fast-forward started from (-1 when it did not run) and the steps it completed. */
+int vca_ca_fast_forwarded(vca_ca* ca, int32_t* from_step, int32_t* steps);
 /* vca_ca_run without any device allocation.  No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that forbid it).  The simulation (created
  * beforehand by vca_ca_create) is advanced exactly as vca_ca_run does; the batch-of-one state vca_ca_run allocates on
@@ -131,6 +141,8 @@ typedef struct {
     uint64_t* changed;
     int32_t cap;
     uint64_t tiles_staged; /* visited tiles whose halo held lava (the rest settle without being read) */
+    int32_t ff_from;       /* the step the fast-forward started from, -1 when it did not run */
+    int32_t ff_steps;      /* steps completed by the fast-forward (steps - ff_steps were launched) */
 } vca_ca_result;
 int vca_ca_collect(vca_ca* ca, vca_ca_result* r);
 
@@ -185,7 +197,8 @@ int vca_batch_enqueue(vca_batch* b, int32_t steps);
 int vca_batch_run(vca_batch* b, int32_t max_steps, int32_t* running);
 int vca_batch_info(vca_batch* b, int32_t* members, uint64_t* tiles_used, int32_t* running);
 /* Step count and stop reason of every slot in one device read: out[2*s] = steps, out[2*s+1] = stop reason
- * (-1 for an empty slot); cap = entries of out / 2.  Synchronises the context stream. */
+ * (-1 for an empty slot); cap = entries of out / 2.  Synchronises the context stream.  A member that has become
+ * quiescent (see vca_ca_run) and has the fast-forward on is completed first: members of a batch run to their stop. */
 int vca_batch_status(vca_batch* b, int32_t* out, int32_t cap);
 
 /* ---- Potts MC: replaces PottsMC ------------------------------------------------------------ */

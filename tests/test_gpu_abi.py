@@ -68,15 +68,19 @@ def test_ca_matches_oracle_and_reference(ctx, name):
     orc = O.OracleCA(O.ca_params(cfg), hm, spill, vents)
     orc.run()
     ref = orc.state()
-    for dense in (False, True):
+    # front and dense mode with the quiescent fast-forward, front mode stepping every step
+    for dense, ff in ((False, True), (True, True), (False, False)):
         gpu = _gpu_ca(ctx, cfg, hm, spill, vents, dense=dense)
+        gpu.set_fast_forward(ff)
         steps, stop = gpu.run()
-        assert (steps, stop) == (orc.steps, orc.stop_reason)
+        assert (steps, stop) == (orc.steps, orc.stop_reason), (dense, ff)
         got = gpu.result()
         for k in ("type", "flow", "solid", "arrival"):
-            assert np.array_equal(got[k], ref[k]), k
-        assert np.array_equal(got["temp"].view(np.uint32), ref["temp"].view(np.uint32))
-        assert gpu.histories() == orc.histories()
+            assert np.array_equal(got[k], ref[k]), (k, dense, ff)
+        assert np.array_equal(got["temp"].view(np.uint32), ref["temp"].view(np.uint32)), (dense, ff)
+        assert gpu.histories() == orc.histories(), (dense, ff)
+        if not ff:
+            assert gpu.fast_forwarded() == (-1, 0)
         gpu.close()
     # and directly against the reference engine's file
     blob = gzip.decompress(open(os.path.join(d, "ca_result.bin.gz"), "rb").read())
@@ -340,3 +344,61 @@ def test_flow_stats_with_many_flows(ctx, name):
     hla, hba, hd2, _, _, _ = _flow_sums_numpy(st, gpu_ca.dims, vl)
     assert np.array_equal(la, hla) and np.array_equal(ba, hba) and np.array_equal(d2, hd2), name
     assert la[1:len(vents) + 1].sum() + ba[1:len(vents) + 1].sum() > 0, name
+
+
+def _lava_cases():
+    return [c for c in _ca_cases() if _inputs(c)[1]["ca"]["mode"] == "lava"]
+
+
+@pytest.mark.parametrize("name", _lava_cases())
+def test_fast_forward_from_any_step(ctx, name):
+    """The quiescent fast-forward started from different steps: a first run() stops short of the end (so it launches
+    every step), a second run() goes to the end and fast-forwards from wherever the first stopped, once the
+    simulation is quiescent.  The state, step count, stop reason and both histories must equal the oracle's, which
+    steps every step; the fast-forward must start at or after the end of the eruption."""
+    d, cfg = _inputs(name)
+    hm = O.terrain_for(cfg, d)
+    per = cfg["boundary"] == "periodic"
+    spill = O.fill_depressions(hm, per, per)
+    vents = O.vents_for(cfg, hm)
+    orc = O.OracleCA(O.ca_params(cfg), hm, spill, vents)
+    orc.run()
+    ref = orc.state()
+    eruption = int(cfg["vents"]["eruptionSteps"])
+    for first in sorted({1, orc.steps // 3, orc.steps // 2, max(1, orc.steps - 2)}):
+        print(f"case {name}: first run of {first} steps, then to the end", flush=True)
+        gpu = _gpu_ca(ctx, cfg, hm, spill, vents)
+        gpu.run(first)
+        assert gpu.fast_forwarded() == (-1, 0) or gpu.steps == orc.steps
+        assert gpu.run() == (orc.steps, orc.stop_reason)
+        got = gpu.result()
+        for k in ("type", "flow", "solid", "arrival"):
+            assert np.array_equal(got[k], ref[k]), (k, first)
+        assert np.array_equal(got["temp"].view(np.uint32), ref["temp"].view(np.uint32)), first
+        assert gpu.histories() == orc.histories(), first
+        start, n = gpu.fast_forwarded()
+        if n:
+            assert start >= max(first, eruption) and start + n == orc.steps, (start, n, first)
+        gpu.close()
+
+
+def test_fast_forward_runs_on_quiescent_tails(ctx):
+    """The fast-forward is taken where a run ends in a quiescent tail: every lava case whose last steps only cool the
+    lava (the oracle's histories: no cell born, i.e. the active count never rises, over the last steps) -- at least
+    the cooling tails of the NoLava cases -- reports fast-forwarded steps."""
+    taken = []
+    for name in _lava_cases():
+        d, cfg = _inputs(name)
+        hm = O.terrain_for(cfg, d)
+        per = cfg["boundary"] == "periodic"
+        spill = O.fill_depressions(hm, per, per)
+        vents = O.vents_for(cfg, hm)
+        gpu = _gpu_ca(ctx, cfg, hm, spill, vents)
+        steps, stop = gpu.run()
+        start, n = gpu.fast_forwarded()
+        gpu.close()
+        if n:
+            taken.append(name)
+            assert start + n == steps
+    print(f"fast-forwarded: {len(taken)} of {len(_lava_cases())}: {taken}", flush=True)
+    assert len(taken) >= 10

@@ -39,6 +39,8 @@ std::string usage() {
            "                     skrót SHA-256 i rozmiar tych samych bajtów (porównania przebiegów większych niż dysk)\n"
            "  --autotune N       przed przebiegiem CA: N kroków próbnych na każdy kształt kafla i tryb frontu,\n"
            "                     przebieg z najszybszym (wyniki bez zmian; wybór w times.json)\n"
+           "  --no-fast-forward  każdy krok CA uruchamiany na GPU, także po ustaniu przepływów (gdy lawa już tylko\n"
+           "                     stygnie); wyniki bez zmian\n"
            "  --progress         linie PROGRESS ca/mc/save/batch na stdout (dla GUI)\n"
            "  --validate         tylko walidacja configu; JSON {\"ok\", \"errors\", \"warnings\"} na stdout\n"
            "  --print-defaults   pełny domyślny config JSON\n"
@@ -103,6 +105,7 @@ CliOptions parse_cli(int argc, const char* const* argv) {
         else if (a == "--benchmark") opt.benchmark = true;
         else if (a == "--timing-detail") opt.timing_detail = true;
         else if (a == "--digests") opt.digests = true;
+        else if (a == "--no-fast-forward") opt.fast_forward = false;
         else if (a == "--timing-sample") {
             const std::string v = value(i, a);
             try {
@@ -152,6 +155,7 @@ int main(int argc, char** argv) {
         }
         if (opt.validate) return run_validate(opt, std::cout);
         if (opt.digests) set_result_digests(true);
+        set_fast_forward(opt.fast_forward);
         if (!opt.batch_dir.empty()) return run_batch(opt);
         if (!opt.serve_path.empty()) return run_serve(opt);
         return run_from_cli(opt, since_start);

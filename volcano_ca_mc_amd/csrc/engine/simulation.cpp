@@ -1,6 +1,7 @@
 #include "simulation.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <filesystem>
@@ -98,6 +99,12 @@ void Progress::report(const char* stage, long long done, long long total, bool f
     std::cout << '\n' << std::flush;
     since_.reset();
 }
+
+namespace {
+std::atomic<bool> g_fast_forward{true};
+}  // namespace
+
+void set_fast_forward(bool on) { g_fast_forward.store(on); }
 
 void vca_check(int rc, const char* what) {
     if (rc != VCA_OK) {
@@ -295,6 +302,7 @@ vca_ca* create_ca(const Config& cfg, const CaInputs& in, Gpu& gpu) {
                             in.spill.empty() ? nullptr : in.spill.height.data(), vv.data(), static_cast<int32_t>(vv.size()),
                             &h),
               "CA");
+    vca_check(vca_ca_set_fast_forward(h, g_fast_forward.load() ? 1 : 0), "CA");
     return h;
 }
 
@@ -326,6 +334,8 @@ void ca_collect(vca_ca* ca, const Config& cfg, RunSummary& s, bool with_state) {
     s.tiles_visited = r.tiles_visited;
     s.tiles_staged = r.tiles_staged;
     s.changes = r.changes;
+    s.ff_from = r.ff_from;
+    s.ff_steps = r.ff_steps;
     s.tile_cells = 1024;
 }
 
@@ -874,7 +884,7 @@ int run_from_cli(const CliOptions& opt, const Clock& since_start) {
         d["unit"] = "ms";
         d["sampleEvery"] = std::max(1, opt.timing_every);  // launches of every N-th CA step / MC sweep timed
         Json k = Json::object();
-        for (const char* name : {"ca_front", "ca_descent", "ca_rules", "ca_finish", "ca_step",
+        for (const char* name : {"ca_front", "ca_descent", "ca_rules", "ca_finish", "ca_step", "ca_ff",
                                   "mc_color", "mc_sweep", "mc_sweeps"}) {
             vca_kernel_time kt{};
             vca_check(vca_ctx_kernel_time(gpu.ctx(), name, &kt), "GPU");
@@ -891,6 +901,10 @@ int run_from_cli(const CliOptions& opt, const Clock& since_start) {
             w["tilesStaged"] = Json(static_cast<unsigned long long>(s->tiles_staged));
             w["tileCells"] = s->tile_cells;
             w["changes"] = Json(static_cast<unsigned long long>(s->changes));
+            // steps the step kernels ran (the rest completed by the quiescent fast-forward)
+            w["stepsLaunched"] = s->steps - s->ff_steps;
+            w["fastForwardFrom"] = s->ff_from;
+            w["fastForwardSteps"] = s->ff_steps;
             uint64_t lava = 0;
             for (uint64_t a : s->active) lava += a;
             w["lavaCellSteps"] = Json(static_cast<unsigned long long>(lava));

@@ -28,6 +28,7 @@ struct CliOptions {
     int steps_per_graph = 16;         // --graph-steps: hipGraph length of the batched CA steps (0 = direct)
     int max_jobs = 0;                 // --max-jobs: jobs admitted at once in the batched pipeline (0 = no limit)
     int autotune_steps = 0;           // --autotune: probe steps per launch configuration before the CA (0 = off)
+    bool fast_forward = true;         // --no-fast-forward: every CA step launched (vca_ca_set_fast_forward)
     bool digests = false;             // --digests: result files replaced by <file>.sha256 (digest and size)
     std::string serve_path;           // --serve <jobs.jsonl>
     int repeats = 3;
@@ -93,6 +94,8 @@ private:
     std::string name_;
 };
 void vca_check(int rc, const char* what);
+// The quiescent fast-forward of every simulation this process creates (on unless --no-fast-forward).
+void set_fast_forward(bool on);
 
 enum class Cooling { None, SolidOnly, Full };
 
@@ -115,6 +118,7 @@ struct RunSummary {
     std::vector<Vent> vents;
     std::vector<uint64_t> active, changed;
     uint64_t tiles_visited = 0, tiles_staged = 0, changes = 0;
+    int32_t ff_from = -1, ff_steps = 0;  // quiescent fast-forward: first step, steps completed
     int32_t tile_cells = 0;
     CaState result;               // the final state, when something on the host needs it (result files)
     // flowSums of the final state (per flowId, entry 0 unused), computed on the device

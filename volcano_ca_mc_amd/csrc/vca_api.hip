@@ -306,6 +306,9 @@ struct vca_ca {
     hipStream_t side = nullptr;   // during vca_ca_collect: the pooled stream its reads go on
     Batch* member = nullptr;      // the shared batch this simulation is in
     int slot = -1;
+    // once the simulation is quiescent, a run that goes to its end completes it by the fast-forward
+    // (vca_ca_set_fast_forward)
+    bool ff = true;
     // the type and temperature copies holding the state after `steps` steps
     const void* type_after(int32_t steps) const { return (steps & 1) ? type2.p : type.p; }
     const void* temp_after(int32_t steps) const { return (steps & 1) ? temp2.p : temp.p; }
@@ -1244,6 +1247,8 @@ extern "C" int vca_ca_collect(vca_ca* ca, vca_ca_result* r) {
         r->tiles_visited = c.tiles_visited;
         r->tiles_staged = c.tiles_staged;
         r->changes = c.changes_total;
+        r->ff_from = c.ff_steps > 0 ? c.ff_from : -1;
+        r->ff_steps = c.ff_steps;
         const size_t m = static_cast<size_t>(std::min(c.steps, std::max(0, r->cap)));
         if (m > 0 && r->active) staged_download(r->active, ca->hist_a.p, m * 8, ca->side);
         if (m > 0 && r->changed) staged_download(r->changed, ca->hist_c.p, m * 8, ca->side);
@@ -1326,13 +1331,38 @@ extern "C" int vca_ca_flow_points(vca_ca* ca, const vca_vent* vent, int32_t flow
     });
 }
 
-static void ca_status_sync(vca_ca* ca, int32_t* steps, int32_t* stop) {
+static void ca_status_sync(vca_ca* ca, int32_t* steps, int32_t* stop, int32_t* quiet = nullptr) {
     CaCtrl c;
     check(hipMemcpyAsync(&c, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, ca->ctx->stream), "read ctrl");
     check(hipStreamSynchronize(ca->ctx->stream), "ca step");
     ca->ctx->drain();
     if (steps) *steps = c.steps;
     if (stop) *stop = c.stop;
+    if (quiet) *quiet = c.quiet;
+}
+
+// The quiescent fast-forward of a running simulation at step `steps` on the context's stream (timed as "ca_ff" when
+// the context times kernels).  Afterwards the simulation has stopped, exactly as stepping on would have left it.
+static void ca_fast_forward(vca_ca* ca, int32_t steps) {
+    vca_ctx* c = ca->ctx;
+    if (c->timing) {
+        hipEvent_t ev[2] = {c->take(), c->take()};
+        const size_t a = c->used - 2;
+        ca_launch_fast_forward(ca->P, ca->D, steps, c->stream, ev);
+        c->mark("ca_ff", a, a + 1);
+    } else {
+        ca_launch_fast_forward(ca->P, ca->D, steps, c->stream, nullptr);
+    }
+    check(hipGetLastError(), "fast-forward");
+}
+
+// A run that may go on for `remaining` more steps: when the simulation is quiescent and the run would reach its
+// last step anyway, the fast-forward completes it (one read of the status afterwards).
+static void ca_maybe_fast_forward(vca_ca* ca, int64_t remaining, int32_t& steps, int32_t& stop, int32_t quiet) {
+    if (!ca->ff || stop != VCA_RUNNING || quiet <= 0 || steps < quiet) return;
+    if (remaining < static_cast<int64_t>(ca->P.max_steps) - steps) return;
+    ca_fast_forward(ca, steps);
+    ca_status_sync(ca, &steps, &stop);
 }
 
 extern "C" int vca_ca_enqueue(vca_ca* ca, int32_t new_steps) {
@@ -1356,18 +1386,39 @@ extern "C" int vca_ca_run(vca_ca* ca, int32_t max_new_steps, int32_t* steps_done
         require(ca, "ca == NULL");
         if (ca->member) throw VcaError(VCA_ERR_STATE, "the simulation is in a batch: advance it with vca_batch_*");
         ca->ctx->set_device();
-        int32_t steps = 0, stop = 0;
-        ca_status_sync(ca, &steps, &stop);
+        int32_t steps = 0, stop = 0, quiet = 0;
+        ca_status_sync(ca, &steps, &stop, &quiet);
         int remaining = max_new_steps;
+        ca_maybe_fast_forward(ca, remaining, steps, stop, quiet);
         // Chunks of device-side steps with one host check in between: steps after a stop are no-ops.
         while (remaining > 0 && stop == VCA_RUNNING) {
             const int chunk = std::min(remaining, 128);
             if (vca_ca_enqueue(ca, chunk) != VCA_OK) throw VcaError(VCA_ERR_HIP, g_last_error);
             remaining -= chunk;
-            ca_status_sync(ca, &steps, &stop);
+            ca_status_sync(ca, &steps, &stop, &quiet);
+            ca_maybe_fast_forward(ca, remaining, steps, stop, quiet);
         }
         if (steps_done) *steps_done = steps;
         if (stop_reason) *stop_reason = stop;
+    });
+}
+
+extern "C" int vca_ca_set_fast_forward(vca_ca* ca, int32_t on) {
+    return guarded([&] {
+        require(ca, "ca == NULL");
+        ca->ff = on != 0;
+    });
+}
+
+extern "C" int vca_ca_fast_forwarded(vca_ca* ca, int32_t* from_step, int32_t* steps) {
+    return guarded([&] {
+        require(ca, "ca == NULL");
+        ca->ctx->set_device();
+        CaCtrl c;
+        check(hipMemcpyAsync(&c, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, ca->ctx->stream), "read ctrl");
+        check(hipStreamSynchronize(ca->ctx->stream), "read ctrl");
+        if (from_step) *from_step = c.ff_steps > 0 ? c.ff_from : -1;
+        if (steps) *steps = c.ff_steps;
     });
 }
 
@@ -1583,15 +1634,17 @@ extern "C" int vca_ca_run_noalloc(vca_ca* ca, int32_t max_new_steps, int32_t* st
         try {
             b.reset_external(ctx);
             b.attach(ca);
-            int32_t steps = 0, stop = 0;
-            ca_status_sync(ca, &steps, &stop);
+            int32_t steps = 0, stop = 0, quiet = 0;
+            ca_status_sync(ca, &steps, &stop, &quiet);
             int remaining = max_new_steps;
+            ca_maybe_fast_forward(ca, remaining, steps, stop, quiet);
             while (remaining > 0 && stop == VCA_RUNNING) {
                 const int chunk = std::min(remaining, 128);
                 b.enqueue(chunk);
                 check(hipGetLastError(), "ca launch");
                 remaining -= chunk;
-                ca_status_sync(ca, &steps, &stop);
+                ca_status_sync(ca, &steps, &stop, &quiet);
+                ca_maybe_fast_forward(ca, remaining, steps, stop, quiet);
             }
             leave();
             if (steps_done) *steps_done = steps;
@@ -1653,18 +1706,31 @@ extern "C" int vca_batch_status(vca_batch* b, int32_t* out, int32_t cap) {
         Batch& bb = b->b;
         bb.ctx->set_device();
         AllocStream alloc_on(bb.ctx->stream);
-        if (!bb.status.p) bb.status.alloc(static_cast<size_t>(bb.max_slots) * 8);
-        ca_launch_status(bb.B, bb.status.as<int32_t>(), bb.ctx->stream);
-        check(hipGetLastError(), "batch status");
-        std::vector<int32_t> all(static_cast<size_t>(bb.B.nslots) * 2);
-        check(hipMemcpyAsync(all.data(), bb.status.p, all.size() * 4, hipMemcpyDeviceToHost, bb.ctx->stream),
-              "batch status");
-        check(hipStreamSynchronize(bb.ctx->stream), "batch status");
-        bb.ctx->drain();
+        if (!bb.status.p) bb.status.alloc(static_cast<size_t>(bb.max_slots) * 12);
+        std::vector<int32_t> all(static_cast<size_t>(bb.B.nslots) * 3);
+        auto read = [&] {
+            ca_launch_status(bb.B, bb.status.as<int32_t>(), bb.ctx->stream);
+            check(hipGetLastError(), "batch status");
+            check(hipMemcpyAsync(all.data(), bb.status.p, all.size() * 4, hipMemcpyDeviceToHost, bb.ctx->stream),
+                  "batch status");
+            check(hipStreamSynchronize(bb.ctx->stream), "batch status");
+            bb.ctx->drain();
+        };
+        read();
+        // members that are quiescent (and run to their end in the batch): completed by the fast-forward
+        bool ff = false;
+        for (uint32_t k = 0; k < bb.B.nslots; ++k) {
+            vca_ca* m = bb.members[k];
+            const int32_t steps = all[3 * k], stop = all[3 * k + 1], quiet = all[3 * k + 2];
+            if (!m || !m->ff || stop != VCA_RUNNING || quiet <= 0 || steps < quiet) continue;
+            ca_fast_forward(m, steps);
+            ff = true;
+        }
+        if (ff) read();
         for (int32_t k = 0; k < cap; ++k) {
             const bool in = static_cast<uint32_t>(k) < bb.B.nslots && bb.members[static_cast<size_t>(k)];
-            out[2 * k] = in ? all[2 * static_cast<size_t>(k)] : 0;
-            out[2 * k + 1] = in ? all[2 * static_cast<size_t>(k) + 1] : -1;
+            out[2 * k] = in ? all[3 * static_cast<size_t>(k)] : 0;
+            out[2 * k + 1] = in ? all[3 * static_cast<size_t>(k) + 1] : -1;
         }
     });
 }

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 
 #include "vca_internal.h"
 
@@ -570,14 +571,14 @@ __global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B, uint32_t s
 // clears its counter rows, writes the histories, advances the step count and applies the stop rule.  Every thread
 // of the block calls (block-uniform slot).
 __device__ void finish_slot(const BatchDev& B, uint32_t slot) {
-    __shared__ unsigned long long red[5][kBlock / 64];
+    __shared__ unsigned long long red[6][kBlock / 64];
     const CaSim& sim = g_sims[B.sim_base + slot];
     if (!sim.running) return;
     const CaParams& P = sim.P;
     const CaDev& D = sim.D;
     CaCtrl* c = D.ctrl;
     if (__builtin_amdgcn_readfirstlane(c->stop)) return;
-    unsigned long long la = 0, bo = 0, ch = 0, tl = 0, st = 0;
+    unsigned long long la = 0, bo = 0, ch = 0, tl = 0, st = 0, ac = 0;
     for (int r = static_cast<int>(threadIdx.x); r < kPartRows; r += kBlock) {
         CaPart* row = &D.part[r];
         la += row->lava_after;
@@ -585,13 +586,15 @@ __device__ void finish_slot(const BatchDev& B, uint32_t slot) {
         ch += row->changed;
         tl += row->tiles;
         st += row->staged;
-        *row = CaPart{0, 0, 0, 0, 0};  // cleared for the next step
+        ac += row->air_cand;
+        *row = CaPart{0, 0, 0, 0, 0, 0};  // cleared for the next step
     }
     la = wave_sum(la);
     bo = wave_sum(bo);
     ch = wave_sum(ch);
     tl = wave_sum(tl);
     st = wave_sum(st);
+    ac = wave_sum(ac);
     const int w = static_cast<int>(threadIdx.x >> 6);
     if ((threadIdx.x & 63u) == 0) {
         red[0][w] = la;
@@ -599,16 +602,18 @@ __device__ void finish_slot(const BatchDev& B, uint32_t slot) {
         red[2][w] = ch;
         red[3][w] = tl;
         red[4][w] = st;
+        red[5][w] = ac;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long lava_after = 0, born = 0, changed = 0, tiles = 0, staged = 0;
+        unsigned long long lava_after = 0, born = 0, changed = 0, tiles = 0, staged = 0, air_cand = 0;
         for (int k = 0; k < kBlock / 64; ++k) {
             lava_after += red[0][k];
             born += red[1][k];
             changed += red[2][k];
             tiles += red[3][k];
             staged += red[4][k];
+            air_cand += red[5][k];
         }
         const int32_t t = c->steps;
         const unsigned long long active = P.classic ? c->lava_total + born : lava_after;
@@ -628,6 +633,10 @@ __device__ void finish_slot(const BatchDev& B, uint32_t slot) {
         else if (t + 1 >= P.max_steps)
             stop = 4;  // MaxSteps
         c->stop = stop;
+        // Quiescence (lava mode): no AIR cell had LAVA at a position the rules read in this step, so none was born;
+        // LAVA only turns into BASALT and AIR stays AIR, so no AIR cell can get a candidate in any later step either.
+        // With the eruption over from step t + 1 on, every later step only cools the lava cells, each on its own.
+        if (!P.classic && air_cand == 0 && t + 1 >= P.eruption_steps && c->quiet == 0) c->quiet = t + 1;
         // lava-front simulations stage exactly the tiles they visit (no scan)
         c->tiles_visited += sim.push ? staged : tiles;
         c->tiles_staged += staged;
@@ -1565,6 +1574,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                 nchg += s_red[4][w];
             }
             CaPart* row = &D.part[tile % kPartRows];
+            if (s_nair) atomicAdd(&row->air_cand, static_cast<unsigned long long>(s_nair));
             if (r0) atomicAdd(&row->lava_after, static_cast<unsigned long long>(r0));
             if (r1) atomicAdd(&row->born, static_cast<unsigned long long>(r1));
             // changed cells: every type change (born, solidified) and every lava cell whose temperature changed
@@ -1666,13 +1676,192 @@ __global__ void __launch_bounds__(1024) ca_purge_kernel(BatchDev B, SlotMask slo
 __global__ void __launch_bounds__(kBlock) ca_status_kernel(BatchDev B, int32_t* out) {
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < B.nslots; k += gridDim.x * kBlock) {
         const CaSim& sim = g_sims[B.sim_base + k];
-        out[2 * k] = sim.running ? sim.D.ctrl->steps : 0;
-        out[2 * k + 1] = sim.running ? sim.D.ctrl->stop : -1;
+        out[3 * k] = sim.running ? sim.D.ctrl->steps : 0;
+        out[3 * k + 1] = sim.running ? sim.D.ctrl->stop : -1;
+        out[3 * k + 2] = sim.running ? sim.D.ctrl->quiet : 0;
     }
 }
 
 __global__ void __launch_bounds__(kBlock) ca_owner_kernel(uint32_t* owner, uint32_t base, uint32_t n, uint32_t slot) {
     for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) owner[base + j] = slot;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Quiescent fast-forward
+// ------------------------------------------------------------------------------------------------
+// From CaCtrl::quiet on, a lava-mode step changes nothing but the lava cells' temperatures (rule 2,
+// LavaCA.hpp:1129-1147 without the eruption branch): T -= coolingAmbient, and the cell becomes BASALT in the step in
+// which T drops below the solidus.  Cells do not interact, so the remaining steps are each cell's own sequence of
+// the same float subtractions; the histories follow from the step in which each cell solidifies or stops changing.
+
+// Cooling of one lava cell of temperature T over at most h steps: s = the step (from 0) in which it solidifies, h when
+// it does not; e = the number of leading steps in which it counts as changed (its temperature changed or it
+// solidified); T = its temperature after them (its last one when it solidifies).  A subtraction that leaves T as it
+// is leaves it so in every later step.
+__device__ inline void ff_cool(float& T, float cool, float solidus, int h, int& s, int& e) {
+    s = h;
+    e = h;
+    #pragma clang loop unroll(disable)
+    for (int k = 0; k < h; ++k) {
+        const float n = T - cool;
+        if (n < solidus) {
+            T = n;
+            s = k;
+            e = k + 1;
+            return;
+        }
+        if (n == T) {
+            e = k;
+            return;
+        }
+        T = n;
+    }
+}
+
+// The lava cells at the start of step t0, one wave per 64 tiles (the tiles flagged with lava; a lava cell's tile is
+// always flagged).  FINAL = false: difference counts over the steps t0 .. t0 + h - 1 into the history arrays (zeroed
+// by the caller): hist_active[t0 + s] += 1 for a cell leaving LAVA in step t0 + s, hist_changed[t0 + e] += 1 for a
+// cell that stops counting as changed after step t0 + e - 1.  FINAL = true: the state after the steps the scan kept
+// (h = CaCtrl::steps - t0) into both type and both temperature copies -- also the tiles that changed a type in step
+// t0 - 1, whose other type copy is behind -- and the tiles' lava flags.
+template <bool FINAL>
+__global__ void __launch_bounds__(kBlock) ca_ff_tiles_kernel(CaParams P, CaDev D, int32_t t0, int32_t h) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t first = blockIdx.x * kBlock + (threadIdx.x & ~63u);
+    const uint8_t* cur = D.type[t0 & 1];
+    const float* tcur = D.temp[t0 & 1];
+    if (FINAL) h = __builtin_amdgcn_readfirstlane(D.ctrl->steps) - t0;
+    bool want = false;
+    if (first + lane < P.ntiles) {
+        want = (D.tile_flags[first + lane] & kFlagLava) != 0;
+        if (FINAL && t0 > 0) want = want || D.tile_tchg[first + lane] == static_cast<uint32_t>(t0);
+    }
+    const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
+    for (uint64_t m = __ballot(want); m; m &= m - 1) {
+        const uint32_t tile = first + static_cast<uint32_t>(__builtin_ctzll(m));
+        const TileCoord o = tile_origin(P, tile);
+        const int x = o.x0 + static_cast<int>(lane);
+        bool left = false;
+        for (int row = 0; row < P.ty * P.tz; ++row) {
+            const int y = o.y0 + row % P.ty, z = o.z0 + row / P.ty;
+            if (x >= P.nx || y >= P.ny || z >= P.nz) continue;
+            const uint32_t i = static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y) +
+                               plane * static_cast<uint32_t>(z);
+            const uint8_t ty = cur[i];
+            if (ty != kLava) {
+                if (FINAL) D.type[(t0 + 1) & 1][i] = ty;
+                continue;
+            }
+            float T = tcur[i];
+            int s, e;
+            ff_cool(T, P.cool_amb, P.solidus, h, s, e);
+            if (!FINAL) {
+                if (s < h) atomicAdd(&D.hist_active[t0 + s], 1ULL);
+                if (e < h) atomicAdd(&D.hist_changed[t0 + e], 1ULL);
+            } else {
+                const bool solid = s < h;
+                const uint8_t nt = solid ? kBasalt : kLava;
+                D.type[0][i] = nt;
+                D.type[1][i] = nt;
+                D.temp[0][i] = T;
+                D.temp[1][i] = T;
+                if (solid) D.solid[i] = static_cast<uint16_t>(t0 + s);  // < maxSteps <= 65535
+                left = left || !solid;
+            }
+        }
+        if (FINAL) {
+            const bool any = __ballot(left) != 0;
+            if (lane == 0)
+                D.tile_flags[tile] = static_cast<uint8_t>((D.tile_flags[tile] & ~kFlagLava) | (any ? kFlagLava : 0));
+        }
+    }
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ inline unsigned long long wave_scan(unsigned long long v) {
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned long long u = __shfl_up(v, static_cast<unsigned>(d), 64);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
+
+// One block: the histories of steps t0 .. t0 + h - 1 from the difference counts (N = the lava cells at t0: active =
+// N - cells that left LAVA so far, changed = N - cells that stopped changing so far), the first step whose finish
+// stops the run (finishStep's order, LavaCA.hpp:1206-1229: NoLava, NoChange for a deterministic neighbourhood,
+// MaxSteps -- reached at the latest in step t0 + h - 1 = maxSteps - 1), and CaCtrl as that finish leaves it.
+__global__ void __launch_bounds__(kBlock) ca_ff_scan_kernel(CaParams P, CaDev D, int32_t t0, int32_t h) {
+    __shared__ unsigned long long s_a[kBlock / 64], s_c[kBlock / 64], s_sum[kBlock / 64];
+    __shared__ int s_first[kBlock / 64];
+    __shared__ int s_stop_k, s_reason;
+    __shared__ unsigned long long s_active;
+    CaCtrl* c = D.ctrl;
+    const unsigned long long N = c->lava_total;
+    const int w = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63u);
+    unsigned long long carry_a = 0, carry_c = 0, changed_sum = 0;
+    int stop_k = -1;
+    for (int base = 0; base < h && stop_k < 0; base += kBlock) {
+        const int k = base + static_cast<int>(threadIdx.x);
+        const unsigned long long a = k < h ? D.hist_active[t0 + k] : 0ULL;
+        const unsigned long long cc = k < h ? D.hist_changed[t0 + k] : 0ULL;
+        const unsigned long long wa = wave_scan(a), wc = wave_scan(cc);
+        if (lane == 63) {
+            s_a[w] = wa;
+            s_c[w] = wc;
+        }
+        __syncthreads();
+        unsigned long long pa = carry_a + wa, pc = carry_c + wc;
+        for (int q = 0; q < w; ++q) {
+            pa += s_a[q];
+            pc += s_c[q];
+        }
+        for (int q = 0; q < kBlock / 64; ++q) {
+            carry_a += s_a[q];
+            carry_c += s_c[q];
+        }
+        const unsigned long long active = N - pa, changed = N - pc;
+        int reason = 0;
+        if (k < h) {
+            if (active == 0)
+                reason = 1;  // NoLava
+            else if (changed == 0 && P.nb.variants == 1)
+                reason = 3;  // NoChange
+            else if (t0 + k + 1 >= P.max_steps)
+                reason = 4;  // MaxSteps
+        }
+        const uint64_t hit = __ballot(reason != 0);
+        if (lane == 0) s_first[w] = hit ? base + 64 * w + static_cast<int>(__builtin_ctzll(hit)) : INT_MAX;
+        __syncthreads();
+        int kmin = INT_MAX;
+        for (int q = 0; q < kBlock / 64; ++q) kmin = min(kmin, s_first[q]);
+        if (k < h && k <= kmin) {
+            D.hist_active[t0 + k] = active;
+            D.hist_changed[t0 + k] = changed;
+            changed_sum += changed;
+        }
+        if (k == kmin) {
+            s_stop_k = k;
+            s_reason = reason;
+            s_active = active;
+        }
+        if (kmin != INT_MAX) stop_k = kmin;
+        __syncthreads();  // s_a, s_c, s_first are rewritten by the next chunk
+    }
+    changed_sum = wave_sum(changed_sum);
+    if (lane == 0) s_sum[w] = changed_sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long total = 0;
+        for (int q = 0; q < kBlock / 64; ++q) total += s_sum[q];
+        c->steps = t0 + s_stop_k + 1;
+        c->stop = s_reason;
+        c->lava_total = s_active;
+        c->changes_total += total;
+        c->ff_from = t0;
+        c->ff_steps = s_stop_k + 1;
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -2257,6 +2446,19 @@ void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
 void ca_launch_owner(uint32_t* owner, uint32_t base, uint32_t n, uint32_t slot, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(ca_owner_kernel, dim3(grid_for(n, 4096)), dim3(kBlock), 0, s, owner, base, n, slot);
+}
+
+void ca_launch_fast_forward(const CaParams& P, const CaDev& D, int32_t t0, hipStream_t s, hipEvent_t* ev) {
+    const int32_t h = P.max_steps - t0;
+    if (h <= 0) return;
+    if (ev) (void)hipEventRecord(ev[0], s);
+    fill_u32(reinterpret_cast<uint32_t*>(D.hist_active + t0), 0u, 2ull * static_cast<uint64_t>(h), s);
+    fill_u32(reinterpret_cast<uint32_t*>(D.hist_changed + t0), 0u, 2ull * static_cast<uint64_t>(h), s);
+    const dim3 g((P.ntiles + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(ca_ff_tiles_kernel<false>, g, dim3(kBlock), 0, s, P, D, t0, h);
+    hipLaunchKernelGGL(ca_ff_scan_kernel, dim3(1), dim3(kBlock), 0, s, P, D, t0, h);
+    hipLaunchKernelGGL(ca_ff_tiles_kernel<true>, g, dim3(kBlock), 0, s, P, D, t0, h);
+    if (ev) (void)hipEventRecord(ev[1], s);
 }
 
 #define VCA_SPIN_DISPATCH(bytes, CALL)                 \

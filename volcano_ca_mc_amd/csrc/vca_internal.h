@@ -94,6 +94,12 @@ struct CaCtrl {
     unsigned long long lava_total, air_total;
     unsigned long long tiles_visited, changes_total;
     unsigned long long tiles_staged;
+    // first step from which the simulation is quiescent (no AIR cell can get LAVA again and the eruption is over:
+    // every later step only cools the lava cells), 0 while it is not; set by the finish of the step before it
+    int32_t quiet;
+    // the fast-forward: the step it started from and the steps it completed (0 when it did not run)
+    int32_t ff_from, ff_steps;
+    int32_t pad;
 };
 
 // Counters of one step spread over kPartRows rows (a tile's counts go to row tile % kPartRows: visited tiles
@@ -101,6 +107,7 @@ struct CaCtrl {
 // address every block updates.
 struct CaPart {
     unsigned long long lava_after, born, changed, tiles, staged;
+    unsigned long long air_cand;  // AIR cells with LAVA at a position the rules read (candidates of rule 1)
 };
 constexpr int kPartRows = 256;
 
@@ -269,8 +276,13 @@ void ca_launch_purge(const BatchDev& B, const SlotMask& slots, int par, hipStrea
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 // owner[base .. base + n) = slot
 void ca_launch_owner(uint32_t* owner, uint32_t base, uint32_t n, uint32_t slot, hipStream_t s);
-// out[2*s] = steps, out[2*s+1] = stop (-1 when slot s is not running) for s < nslots.
+// out[3*s] = steps, out[3*s+1] = stop (-1 when slot s is not running), out[3*s+2] = CaCtrl::quiet, for s < nslots.
 void ca_launch_status(const BatchDev& B, int32_t* out, hipStream_t s);
+// Quiescent fast-forward of one simulation from step t0 (>= CaCtrl::quiet, not stopped) to its stop: each lava
+// cell's cooling replayed on its own, the histories of the steps t0 .. stop, the final state in both type and
+// temperature copies, and the step count, stop reason and counters in CaCtrl -- what stepping on would give.
+// ev (nullable): start / end events of the whole sequence.
+void ca_launch_fast_forward(const CaParams& P, const CaDev& D, int32_t t0, hipStream_t s, hipEvent_t* ev);
 // Writes entry `index` of the device's simulation table (stream ordered; `sim` must stay valid until the
 // stream reaches the copy).
 void ca_write_sim(uint32_t index, const CaSim* sim, hipStream_t s);

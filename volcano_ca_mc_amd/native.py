@@ -84,6 +84,7 @@ def lib() -> C.CDLL:
         "vca_ca_run": [vp, i32, C.POINTER(i32), C.POINTER(i32)], "vca_ca_enqueue": [vp, i32],
         "vca_ca_run_noalloc": [vp, i32, C.POINTER(i32), C.POINTER(i32)],
         "vca_ca_status": [vp, C.POINTER(i32), C.POINTER(i32)],
+        "vca_ca_set_fast_forward": [vp, i32], "vca_ca_fast_forwarded": [vp, C.POINTER(i32), C.POINTER(i32)],
         "vca_ca_histories": [vp, vp, vp, i32, C.POINTER(i32)], "vca_ca_download": [vp, vp, vp, vp, vp, vp],
         "vca_ca_work": [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(i32)],
         "vca_ca_flow_stats": [vp, vp, i32, vp, vp, vp], "vca_ca_flow_points": [vp, vp, i32, u64, vp, i32],
@@ -107,6 +108,7 @@ def lib() -> C.CDLL:
 EXPORTED = ("vca_version", "vca_last_error", "vca_ctx_create", "vca_ctx_destroy", "vca_ctx_synchronize",
             "vca_device_memory", "vca_ctx_set_timing", "vca_ctx_kernel_time", "vca_ca_create", "vca_ca_destroy",
             "vca_ca_run", "vca_ca_run_noalloc", "vca_ca_enqueue", "vca_ca_status", "vca_ca_histories", "vca_ca_download", "vca_ca_work",
+            "vca_ca_set_fast_forward", "vca_ca_fast_forwarded",
             "vca_ca_footprint", "vca_ca_tiles", "vca_batch_create", "vca_batch_destroy", "vca_batch_add",
             "vca_batch_remove", "vca_batch_set_graph", "vca_batch_enqueue", "vca_batch_run", "vca_batch_info",
             "vca_batch_status", "vca_ca_collect", "vca_ca_flow_stats", "vca_ca_flow_points", "vca_ca_autotune",
@@ -245,6 +247,16 @@ class LavaCA:
 
     def step(self) -> bool:
         return self.run(1)[1] == 0
+
+    def set_fast_forward(self, on: bool) -> None:
+        """Quiescent fast-forward of run() on (the default) or off (every step launched); results are the same."""
+        _check(lib().vca_ca_set_fast_forward(self.h, 1 if on else 0))
+
+    def fast_forwarded(self):
+        """(step the fast-forward started from or -1, steps it completed)."""
+        f, n = C.c_int32(), C.c_int32()
+        _check(lib().vca_ca_fast_forwarded(self.h, C.byref(f), C.byref(n)))
+        return int(f.value), int(n.value)
 
     def enqueue(self, n: int):
         _check(lib().vca_ca_enqueue(self.h, n))
