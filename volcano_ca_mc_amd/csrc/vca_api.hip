@@ -1331,38 +1331,69 @@ extern "C" int vca_ca_flow_points(vca_ca* ca, const vca_vent* vent, int32_t flow
     });
 }
 
-static void ca_status_sync(vca_ca* ca, int32_t* steps, int32_t* stop, int32_t* quiet = nullptr) {
+// Status of a simulation on the context's stream: step count, stop reason, and when the fast-forward may run.
+struct CaStatus {
+    int32_t steps = 0, stop = 0, quiet = 0, qdead = 0;
+};
+static CaStatus ca_status_read(vca_ca* ca) {
     CaCtrl c;
     check(hipMemcpyAsync(&c, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, ca->ctx->stream), "read ctrl");
     check(hipStreamSynchronize(ca->ctx->stream), "ca step");
     ca->ctx->drain();
-    if (steps) *steps = c.steps;
-    if (stop) *stop = c.stop;
-    if (quiet) *quiet = c.quiet;
+    return CaStatus{c.steps, c.stop, c.quiet, c.qdead};
+}
+static void ca_status_sync(vca_ca* ca, int32_t* steps, int32_t* stop) {
+    const CaStatus st = ca_status_read(ca);
+    if (steps) *steps = st.steps;
+    if (stop) *stop = st.stop;
 }
 
-// The quiescent fast-forward of a running simulation at step `steps` on the context's stream (timed as "ca_ff" when
-// the context times kernels).  Afterwards the simulation has stopped, exactly as stepping on would have left it.
-static void ca_fast_forward(vca_ca* ca, int32_t steps) {
+// The fast-forward of a running simulation that is in batch B (slot, list copy par of its next step) from step
+// `steps` over at most h steps, on the context's stream (timed as "ca_ff" when the context times kernels): to its stop
+// when no AIR cell can get lava any more (quiet), or up to the first solidification while none can be born (qdead).
+// Returns false when neither holds.
+static bool ca_fast_forward(vca_ca* ca, const CaStatus& st, int64_t h, const BatchDev& B, uint32_t slot, int par) {
+    if (!ca->ff || st.stop != VCA_RUNNING) return false;
+    h = std::min<int64_t>(h, static_cast<int64_t>(ca->P.max_steps) - st.steps);
+    if (h <= 0) return false;
+    const bool quiet = st.quiet > 0 && st.steps >= st.quiet;
+    const bool dead = st.qdead > 0 && st.steps == st.qdead;
+    if (!quiet && !dead) return false;
     vca_ctx* c = ca->ctx;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    size_t a = 0;
     if (c->timing) {
-        hipEvent_t ev[2] = {c->take(), c->take()};
-        const size_t a = c->used - 2;
-        ca_launch_fast_forward(ca->P, ca->D, steps, c->stream, ev);
-        c->mark("ca_ff", a, a + 1);
-    } else {
-        ca_launch_fast_forward(ca->P, ca->D, steps, c->stream, nullptr);
+        ev[0] = c->take();
+        ev[1] = c->take();
+        a = c->used - 2;
     }
+    ca_launch_fast_forward(ca->P, ca->D, B, slot, ca->shape, par, st.steps, static_cast<int32_t>(h), !quiet, c->stream,
+                           c->timing ? ev : nullptr);
+    if (c->timing) c->mark("ca_ff", a, a + 1);
     check(hipGetLastError(), "fast-forward");
+    return true;
 }
 
-// A run that may go on for `remaining` more steps: when the simulation is quiescent and the run would reach its
-// last step anyway, the fast-forward completes it (one read of the status afterwards).
-static void ca_maybe_fast_forward(vca_ca* ca, int64_t remaining, int32_t& steps, int32_t& stop, int32_t quiet) {
-    if (!ca->ff || stop != VCA_RUNNING || quiet <= 0 || steps < quiet) return;
-    if (remaining < static_cast<int64_t>(ca->P.max_steps) - steps) return;
-    ca_fast_forward(ca, steps);
-    ca_status_sync(ca, &steps, &stop);
+// Steps a simulation of the batch of one `b` (slot 0) until it stops or `max_new_steps` more steps are done: chunks
+// of device-side steps with one host check in between (steps after a stop are no-ops), and the fast-forward where it
+// applies.  After a fast-forward the chunks start short again (the next one may apply soon).
+static CaStatus ca_run_steps(vca_ca* ca, Batch& b, int64_t max_new_steps) {
+    CaStatus st = ca_status_read(ca);
+    const int64_t target = static_cast<int64_t>(st.steps) + std::max<int64_t>(0, max_new_steps);
+    int chunk = 128;
+    while (st.stop == VCA_RUNNING && st.steps < target) {
+        if (ca_fast_forward(ca, st, target - st.steps, b.B, 0u, b.par)) {
+            st = ca_status_read(ca);
+            chunk = 1;
+            continue;
+        }
+        const int n = static_cast<int>(std::min<int64_t>(target - st.steps, chunk));
+        b.enqueue(n);
+        check(hipGetLastError(), "ca launch");
+        chunk = std::min(128, 2 * chunk);
+        st = ca_status_read(ca);
+    }
+    return st;
 }
 
 extern "C" int vca_ca_enqueue(vca_ca* ca, int32_t new_steps) {
@@ -1386,20 +1417,16 @@ extern "C" int vca_ca_run(vca_ca* ca, int32_t max_new_steps, int32_t* steps_done
         require(ca, "ca == NULL");
         if (ca->member) throw VcaError(VCA_ERR_STATE, "the simulation is in a batch: advance it with vca_batch_*");
         ca->ctx->set_device();
-        int32_t steps = 0, stop = 0, quiet = 0;
-        ca_status_sync(ca, &steps, &stop, &quiet);
-        int remaining = max_new_steps;
-        ca_maybe_fast_forward(ca, remaining, steps, stop, quiet);
-        // Chunks of device-side steps with one host check in between: steps after a stop are no-ops.
-        while (remaining > 0 && stop == VCA_RUNNING) {
-            const int chunk = std::min(remaining, 128);
-            if (vca_ca_enqueue(ca, chunk) != VCA_OK) throw VcaError(VCA_ERR_HIP, g_last_error);
-            remaining -= chunk;
-            ca_status_sync(ca, &steps, &stop, &quiet);
-            ca_maybe_fast_forward(ca, remaining, steps, stop, quiet);
+        if (!ca->own) {  // the private batch (members[0] is this simulation) is made on the first solo enqueue
+            AllocStream alloc_on(ca->ctx->stream);
+            auto own = std::make_unique<Batch>();
+            own->init(ca->ctx, 1, ca->P.ntiles);
+            own->attach(ca);
+            ca->own = std::move(own);
         }
-        if (steps_done) *steps_done = steps;
-        if (stop_reason) *stop_reason = stop;
+        const CaStatus st = ca_run_steps(ca, *ca->own, max_new_steps);
+        if (steps_done) *steps_done = st.steps;
+        if (stop_reason) *stop_reason = st.stop;
     });
 }
 
@@ -1634,21 +1661,10 @@ extern "C" int vca_ca_run_noalloc(vca_ca* ca, int32_t max_new_steps, int32_t* st
         try {
             b.reset_external(ctx);
             b.attach(ca);
-            int32_t steps = 0, stop = 0, quiet = 0;
-            ca_status_sync(ca, &steps, &stop, &quiet);
-            int remaining = max_new_steps;
-            ca_maybe_fast_forward(ca, remaining, steps, stop, quiet);
-            while (remaining > 0 && stop == VCA_RUNNING) {
-                const int chunk = std::min(remaining, 128);
-                b.enqueue(chunk);
-                check(hipGetLastError(), "ca launch");
-                remaining -= chunk;
-                ca_status_sync(ca, &steps, &stop, &quiet);
-                ca_maybe_fast_forward(ca, remaining, steps, stop, quiet);
-            }
+            const CaStatus st = ca_run_steps(ca, b, max_new_steps);
             leave();
-            if (steps_done) *steps_done = steps;
-            if (stop_reason) *stop_reason = stop;
+            if (steps_done) *steps_done = st.steps;
+            if (stop_reason) *stop_reason = st.stop;
         } catch (...) {
             leave();
             throw;
@@ -1706,8 +1722,8 @@ extern "C" int vca_batch_status(vca_batch* b, int32_t* out, int32_t cap) {
         Batch& bb = b->b;
         bb.ctx->set_device();
         AllocStream alloc_on(bb.ctx->stream);
-        if (!bb.status.p) bb.status.alloc(static_cast<size_t>(bb.max_slots) * 12);
-        std::vector<int32_t> all(static_cast<size_t>(bb.B.nslots) * 3);
+        if (!bb.status.p) bb.status.alloc(static_cast<size_t>(bb.max_slots) * 16);
+        std::vector<int32_t> all(static_cast<size_t>(bb.B.nslots) * 4);
         auto read = [&] {
             ca_launch_status(bb.B, bb.status.as<int32_t>(), bb.ctx->stream);
             check(hipGetLastError(), "batch status");
@@ -1717,20 +1733,18 @@ extern "C" int vca_batch_status(vca_batch* b, int32_t* out, int32_t cap) {
             bb.ctx->drain();
         };
         read();
-        // members that are quiescent (and run to their end in the batch): completed by the fast-forward
+        // members where no cell can be born any more (to their stop) or until a type changes: the fast-forward
         bool ff = false;
         for (uint32_t k = 0; k < bb.B.nslots; ++k) {
             vca_ca* m = bb.members[k];
-            const int32_t steps = all[3 * k], stop = all[3 * k + 1], quiet = all[3 * k + 2];
-            if (!m || !m->ff || stop != VCA_RUNNING || quiet <= 0 || steps < quiet) continue;
-            ca_fast_forward(m, steps);
-            ff = true;
+            const CaStatus st{all[4 * k], all[4 * k + 1], all[4 * k + 2], all[4 * k + 3]};
+            if (m && ca_fast_forward(m, st, m->P.max_steps, bb.B, k, bb.par)) ff = true;
         }
         if (ff) read();
         for (int32_t k = 0; k < cap; ++k) {
             const bool in = static_cast<uint32_t>(k) < bb.B.nslots && bb.members[static_cast<size_t>(k)];
-            out[2 * k] = in ? all[3 * static_cast<size_t>(k)] : 0;
-            out[2 * k + 1] = in ? all[3 * static_cast<size_t>(k) + 1] : -1;
+            out[2 * k] = in ? all[4 * static_cast<size_t>(k)] : 0;
+            out[2 * k + 1] = in ? all[4 * static_cast<size_t>(k) + 1] : -1;
         }
     });
 }

@@ -353,6 +353,14 @@ struct VoteTwo {
     }
 };
 
+// Whether any candidate the rules accept carries a candidate temperature at the solidus or above (gathered over the
+// union of the variants: a cell without one is not born in any step that keeps the types and lowers no bound).
+struct VoteAlive {
+    float solidus;
+    bool alive = false;
+    __device__ void add_if(bool ok, int32_t, float t, int) { alive = alive || (ok && t >= solidus); }
+};
+
 // The accepted candidate positions of a cell (bit k: the neighbour at position k is a candidate), for the full vote
 // of vote_positions.
 struct VoteMask {
@@ -571,50 +579,39 @@ __global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B, uint32_t s
 // clears its counter rows, writes the histories, advances the step count and applies the stop rule.  Every thread
 // of the block calls (block-uniform slot).
 __device__ void finish_slot(const BatchDev& B, uint32_t slot) {
-    __shared__ unsigned long long red[6][kBlock / 64];
+    __shared__ unsigned long long red[8][kBlock / 64];
     const CaSim& sim = g_sims[B.sim_base + slot];
     if (!sim.running) return;
     const CaParams& P = sim.P;
     const CaDev& D = sim.D;
     CaCtrl* c = D.ctrl;
     if (__builtin_amdgcn_readfirstlane(c->stop)) return;
-    unsigned long long la = 0, bo = 0, ch = 0, tl = 0, st = 0, ac = 0;
+    unsigned long long v[8] = {};
     for (int r = static_cast<int>(threadIdx.x); r < kPartRows; r += kBlock) {
         CaPart* row = &D.part[r];
-        la += row->lava_after;
-        bo += row->born;
-        ch += row->changed;
-        tl += row->tiles;
-        st += row->staged;
-        ac += row->air_cand;
-        *row = CaPart{0, 0, 0, 0, 0, 0};  // cleared for the next step
+        v[0] += row->lava_after;
+        v[1] += row->born;
+        v[2] += row->changed;
+        v[3] += row->tiles;
+        v[4] += row->staged;
+        v[5] += row->air_cand;
+        v[6] += row->air_alive;
+        v[7] += row->tchanged;
+        *row = CaPart{0, 0, 0, 0, 0, 0, 0, 0};  // cleared for the next step
     }
-    la = wave_sum(la);
-    bo = wave_sum(bo);
-    ch = wave_sum(ch);
-    tl = wave_sum(tl);
-    st = wave_sum(st);
-    ac = wave_sum(ac);
     const int w = static_cast<int>(threadIdx.x >> 6);
-    if ((threadIdx.x & 63u) == 0) {
-        red[0][w] = la;
-        red[1][w] = bo;
-        red[2][w] = ch;
-        red[3][w] = tl;
-        red[4][w] = st;
-        red[5][w] = ac;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        v[q] = wave_sum(v[q]);
+        if ((threadIdx.x & 63u) == 0) red[q][w] = v[q];
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long lava_after = 0, born = 0, changed = 0, tiles = 0, staged = 0, air_cand = 0;
-        for (int k = 0; k < kBlock / 64; ++k) {
-            lava_after += red[0][k];
-            born += red[1][k];
-            changed += red[2][k];
-            tiles += red[3][k];
-            staged += red[4][k];
-            air_cand += red[5][k];
-        }
+        unsigned long long sum[8] = {};
+        for (int k = 0; k < kBlock / 64; ++k)
+            for (int q = 0; q < 8; ++q) sum[q] += red[q][k];
+        const unsigned long long lava_after = sum[0], born = sum[1], changed = sum[2], tiles = sum[3],
+                                 staged = sum[4], air_cand = sum[5], air_alive = sum[6], tchanged = sum[7];
         const int32_t t = c->steps;
         const unsigned long long active = P.classic ? c->lava_total + born : lava_after;
         const unsigned long long air = c->air_total - born;
@@ -634,9 +631,14 @@ __device__ void finish_slot(const BatchDev& B, uint32_t slot) {
             stop = 4;  // MaxSteps
         c->stop = stop;
         // Quiescence (lava mode): no AIR cell had LAVA at a position the rules read in this step, so none was born;
-        // LAVA only turns into BASALT and AIR stays AIR, so no AIR cell can get a candidate in any later step either.
-        // With the eruption over from step t + 1 on, every later step only cools the lava cells, each on its own.
-        if (!P.classic && air_cand == 0 && t + 1 >= P.eruption_steps && c->quiet == 0) c->quiet = t + 1;
+        // LAVA only turns into BASALT and AIR stays AIR, so no AIR cell can get a candidate in any later step either:
+        // every later step only cools the lava cells (and keeps the vents at eruptTemp while they erupt), each on its
+        // own.
+        if (!P.classic && air_cand == 0 && c->quiet == 0) c->quiet = t + 1;
+        // No type changed in this step and no AIR candidate could be born under any variant at these or lower
+        // temperatures: the next steps keep the same types until a lava cell solidifies.
+        c->qdead = (!P.classic && born == 0 && tchanged == 0 && air_alive == 0) ? t + 1 : 0;
+        c->born_last = born != 0 ? 1 : 0;
         // lava-front simulations stage exactly the tiles they visit (no scan)
         c->tiles_visited += sim.push ? staged : tiles;
         c->tiles_staged += staged;
@@ -1194,7 +1196,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
     constexpr int LX = H::LX, LY = H::LY;
     __shared__ RulesTile<TY, TZ, F> T;
     __shared__ uint32_t s_nlist, s_nair;
-    __shared__ uint32_t s_red[5][kBlock / 64];
+    __shared__ uint32_t s_red[6][kBlock / 64];
     __shared__ uint32_t s_wflags[2];  // per-wave flag bytes: [0] staging, [1] end of tile
     __shared__ uint32_t s_bnd0[kBlock / 64];  // per wave: directions with LAVA on the tile's boundary at the step's start
     const uint64_t* tiles = B.active + (par * kShapes + shape) * static_cast<size_t>(B.cap);
@@ -1226,6 +1228,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         uint32_t w_lava = 0, w_born = 0;  // wave-uniform counts: lava cells after the step, born cells
         uint32_t w_chg = 0;               // wave-uniform count: cells that changed type
         uint32_t w_temp = 0;              // wave-uniform count: lava cells whose temperature changed (no record)
+        uint32_t w_alive = 0;             // wave-uniform count: AIR candidates not born that a later step could bear
         bool w_tchg = false;              // wave-uniform: a cell changed type
         const float* tcur = D.temp[t & 1];
         float* tnext = D.temp[(t + 1) & 1];
@@ -1239,6 +1242,8 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         // the stop flag is read together with the staging loads (staging writes LDS only); a stopped simulation's
         // tile goes no further
         const int32_t stop = D.ctrl->stop;
+        // cells were born in the last step: no quiescence to find in this one (every AIR candidate counts as alive)
+        const int32_t born_last = D.ctrl->born_last;
         int own_air = 0;
         // lane j: base of halo row j; gx: this lane's column (x == nx is the wrapped x = 0 of a periodic partial
         // tile, -1 for no cell)
@@ -1423,7 +1428,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             const int ly = row % TY, lz = row / TY;
             const int x = o.x0 + lx, y = o.y0 + ly, z = o.z0 + lz;
             const int c = (lx + 1) + LX * ((ly + 1) + LY * (lz + 1));
-            bool want = false, stay = false, newborn = false, solid = false, cooled = false;
+            bool want = false, stay = false, newborn = false, solid = false, cooled = false, alive = false;
             uint32_t i = 0;
             if (live) {
                 const uint32_t col = static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y);
@@ -1475,6 +1480,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                     int32_t wf = 0;
                     float wt = 0.0f;
                     bool has = false;
+                    const int spill_c = has_spill ? T.spill[lx + kTileX * ly] : 0;
                     if (lava_bits) {
                         // rule 1 (LavaCA.hpp:1115-1128) / classic (LavaCA.hpp:1175-1190)
                         const uint64_t h_index = splitmix64(prefix ^ i);
@@ -1484,7 +1490,6 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                                                            static_cast<uint32_t>(P.nb.variants)));
                         const uint32_t bits = lava_bits & mask;
                         if (bits) {
-                            const int spill_c = has_spill ? T.spill[lx + kTileX * ly] : 0;
                             VoteFirst v;
                             gather_lds(P, tyc, T, c, lz, z, col, plane, spill_c, has_spill, bits, v);
                             has = v.n > 0;
@@ -1515,12 +1520,25 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                         tile_lava = 1;
                     } else {
                         tile_air = 1;
+                        // could a later step bear it while no type changes (CaCtrl::qdead)?  Any source the rules
+                        // accept under any variant with a candidate temperature at the solidus or above; unchecked
+                        // (after a step with births) counts as yes
+                        if (lava_bits && !P.classic) {
+                            if (born_last) {
+                                alive = true;
+                            } else {
+                                VoteAlive va{P.solidus};
+                                gather_lds(P, tyc, T, c, lz, z, col, plane, spill_c, has_spill, lava_bits, va);
+                                alive = va.alive;
+                            }
+                        }
                     }
                 }
             }
             w_lava += static_cast<uint32_t>(__popcll(__ballot(stay || newborn)));
             w_born += static_cast<uint32_t>(__popcll(__ballot(newborn)));
             w_temp += static_cast<uint32_t>(__popcll(__ballot(cooled)));
+            w_alive += static_cast<uint32_t>(__popcll(__ballot(alive)));
             w_chg += static_cast<uint32_t>(__popcll(__ballot(want)));
             w_tchg = w_tchg || __ballot(newborn || solid) != 0;
             if (stay || newborn) bnd |= boundary_dirs(lx, ly, lz, xm, ym, zm);
@@ -1541,6 +1559,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                 s_red[2][tid >> 6] = bnd;
                 s_red[3][tid >> 6] = w_temp;
                 s_red[4][tid >> 6] = w_chg;
+                s_red[5][tid >> 6] = w_alive;
                 reinterpret_cast<uint8_t*>(&s_wflags[1])[tid >> 6] = static_cast<uint8_t>(f);
             }
         }
@@ -1566,15 +1585,18 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             const uint32_t fl = (wf | wf >> 8 | wf >> 16 | wf >> 24) & 0xFFu;
             D.tile_flags[tile] = static_cast<uint8_t>(fl & (kFlagLava | kFlagAir));
             if (fl & 4u) D.tile_tchg[tile] = static_cast<uint32_t>(t) + 1u;
-            uint32_t r0 = 0, r1 = 0, r3 = 0, nchg = 0;
+            uint32_t r0 = 0, r1 = 0, r3 = 0, nchg = 0, nalive = 0;
             for (int w = 0; w < kBlock / 64; ++w) {
                 r0 += s_red[0][w];
                 r1 += s_red[1][w];
                 r3 += s_red[3][w];
                 nchg += s_red[4][w];
+                nalive += s_red[5][w];
             }
             CaPart* row = &D.part[tile % kPartRows];
             if (s_nair) atomicAdd(&row->air_cand, static_cast<unsigned long long>(s_nair));
+            if (nalive) atomicAdd(&row->air_alive, static_cast<unsigned long long>(nalive));
+            if (nchg) atomicAdd(&row->tchanged, static_cast<unsigned long long>(nchg));
             if (r0) atomicAdd(&row->lava_after, static_cast<unsigned long long>(r0));
             if (r1) atomicAdd(&row->born, static_cast<unsigned long long>(r1));
             // changed cells: every type change (born, solidified) and every lava cell whose temperature changed
@@ -1676,9 +1698,10 @@ __global__ void __launch_bounds__(1024) ca_purge_kernel(BatchDev B, SlotMask slo
 __global__ void __launch_bounds__(kBlock) ca_status_kernel(BatchDev B, int32_t* out) {
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < B.nslots; k += gridDim.x * kBlock) {
         const CaSim& sim = g_sims[B.sim_base + k];
-        out[3 * k] = sim.running ? sim.D.ctrl->steps : 0;
-        out[3 * k + 1] = sim.running ? sim.D.ctrl->stop : -1;
-        out[3 * k + 2] = sim.running ? sim.D.ctrl->quiet : 0;
+        out[4 * k] = sim.running ? sim.D.ctrl->steps : 0;
+        out[4 * k + 1] = sim.running ? sim.D.ctrl->stop : -1;
+        out[4 * k + 2] = sim.running ? sim.D.ctrl->quiet : 0;
+        out[4 * k + 3] = sim.running ? sim.D.ctrl->qdead : 0;
     }
 }
 
@@ -1687,43 +1710,55 @@ __global__ void __launch_bounds__(kBlock) ca_owner_kernel(uint32_t* owner, uint3
 }
 
 // ------------------------------------------------------------------------------------------------
-// Quiescent fast-forward
+// Fast-forward over steps in which no cell can be born
 // ------------------------------------------------------------------------------------------------
-// From CaCtrl::quiet on, a lava-mode step changes nothing but the lava cells' temperatures (rule 2,
-// LavaCA.hpp:1129-1147 without the eruption branch): T -= coolingAmbient, and the cell becomes BASALT in the step in
-// which T drops below the solidus.  Cells do not interact, so the remaining steps are each cell's own sequence of
-// the same float subtractions; the histories follow from the step in which each cell solidifies or stops changing.
+// While no cell can be born, a lava-mode step changes nothing but the lava cells' temperatures and types (rule 2,
+// LavaCA.hpp:1129-1147): a vent cell of an erupting step is reset to eruptTemp, every other lava cell loses
+// coolingAmbient, and a cell whose new temperature is below the solidus becomes BASALT in that step.  Cells do not
+// interact, so those steps are each cell's own sequence of the same float operations; the histories follow from the
+// steps in which each cell solidifies or starts / stops changing.
 
-// Cooling of one lava cell of temperature T over at most h steps: s = the step (from 0) in which it solidifies, h when
-// it does not; e = the number of leading steps in which it counts as changed (its temperature changed or it
-// solidified); T = its temperature after them (its last one when it solidifies).  A subtraction that leaves T as it
-// is leaves it so in every later step.
-__device__ inline void ff_cool(float& T, float cool, float solidus, int h, int& s, int& e) {
-    s = h;
-    e = h;
+// Replays one lava cell (temperature T, vent: it is its flow's vent cell) from step t0 over at most h steps.  Returns
+// the step (from 0) in which it solidifies, h when it does not; T = its temperature after the steps replayed (its
+// last one when it solidifies).  flip(k, +1 / -1): from step t0 + k on the cell starts / stops counting as changed
+// (its temperature changed, or it solidified) -- called only for k < h.
+template <class F>
+__device__ inline int ff_replay(float& T, bool vent, int t0, int h, const CaParams& P, F&& flip) {
+    bool prev = false;
     #pragma clang loop unroll(disable)
     for (int k = 0; k < h; ++k) {
-        const float n = T - cool;
-        if (n < solidus) {
-            T = n;
-            s = k;
-            e = k + 1;
-            return;
-        }
-        if (n == T) {
-            e = k;
-            return;
+        const bool erupt = vent && t0 + k < P.eruption_steps;
+        const float n = erupt ? P.erupt_temp : T - P.cool_amb;
+        const bool sol = n < P.solidus;
+        const bool ch = sol || n != T;
+        if (ch != prev) {
+            flip(k, ch ? 1 : -1);
+            prev = ch;
         }
         T = n;
+        if (sol) {
+            if (k + 1 < h) flip(k + 1, -1);
+            return k;
+        }
+        if (!ch) {
+            // unchanged: it stays so until the eruption ends (a vent) or for good
+            if (!erupt) return h;
+            k = P.eruption_steps - t0 - 1;  // resumes at the first step after the eruption
+        }
     }
+    return h;
 }
 
-// The lava cells at the start of step t0, one wave per 64 tiles (the tiles flagged with lava; a lava cell's tile is
-// always flagged).  FINAL = false: difference counts over the steps t0 .. t0 + h - 1 into the history arrays (zeroed
-// by the caller): hist_active[t0 + s] += 1 for a cell leaving LAVA in step t0 + s, hist_changed[t0 + e] += 1 for a
-// cell that stops counting as changed after step t0 + e - 1.  FINAL = true: the state after the steps the scan kept
-// (h = CaCtrl::steps - t0) into both type and both temperature copies -- also the tiles that changed a type in step
-// t0 - 1, whose other type copy is behind -- and the tiles' lava flags.
+// Sets up a fast-forward: the first step with a solidification is not known yet.
+__global__ void ca_ff_init_kernel(CaDev D) { D.ctrl->ff_sol = INT_MAX; }
+
+// The lava cells at the start of step t0, one wave per 64 tiles (the tiles flagged with lava: a lava cell's tile always
+// is).  FINAL = false: difference counts over the steps t0 .. t0 + h - 1 into the history arrays (zeroed by the
+// caller) -- hist_active[t0 + s] += 1 for a cell that solidifies in step t0 + s, hist_changed[t0 + k] += +1 / -1
+// where a cell starts / stops counting as changed -- and the first step with a solidification (CaCtrl::ff_sol).
+// FINAL = true: the state after the steps the scan kept (h = CaCtrl::steps - t0) into both type and both temperature
+// copies (also in the tiles that changed a type in step t0 - 1, whose other type copy is behind), the tiles' lava
+// flags, and the step of a tile's last type change (tile_tchg) where a cell solidified in the last of those steps.
 template <bool FINAL>
 __global__ void __launch_bounds__(kBlock) ca_ff_tiles_kernel(CaParams P, CaDev D, int32_t t0, int32_t h) {
     const uint32_t lane = threadIdx.x & 63u;
@@ -1737,11 +1772,12 @@ __global__ void __launch_bounds__(kBlock) ca_ff_tiles_kernel(CaParams P, CaDev D
         if (FINAL && t0 > 0) want = want || D.tile_tchg[first + lane] == static_cast<uint32_t>(t0);
     }
     const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
+    int sol_min = INT_MAX;
     for (uint64_t m = __ballot(want); m; m &= m - 1) {
         const uint32_t tile = first + static_cast<uint32_t>(__builtin_ctzll(m));
         const TileCoord o = tile_origin(P, tile);
         const int x = o.x0 + static_cast<int>(lane);
-        bool left = false;
+        bool left = false, last = false;
         for (int row = 0; row < P.ty * P.tz; ++row) {
             const int y = o.y0 + row % P.ty, z = o.z0 + row / P.ty;
             if (x >= P.nx || y >= P.ny || z >= P.nz) continue;
@@ -1752,13 +1788,19 @@ __global__ void __launch_bounds__(kBlock) ca_ff_tiles_kernel(CaParams P, CaDev D
                 if (FINAL) D.type[(t0 + 1) & 1][i] = ty;
                 continue;
             }
+            const int32_t f = D.flow[i];
+            const bool vent = f > 0 && f < P.nvent_slots && D.vent_cell[f] == i;
             float T = tcur[i];
-            int s, e;
-            ff_cool(T, P.cool_amb, P.solidus, h, s, e);
             if (!FINAL) {
-                if (s < h) atomicAdd(&D.hist_active[t0 + s], 1ULL);
-                if (e < h) atomicAdd(&D.hist_changed[t0 + e], 1ULL);
+                const int s = ff_replay(T, vent, t0, h, P, [&](int k, int d) {
+                    atomicAdd(&D.hist_changed[t0 + k], static_cast<unsigned long long>(static_cast<long long>(d)));
+                });
+                if (s < h) {
+                    atomicAdd(&D.hist_active[t0 + s], 1ULL);
+                    sol_min = min(sol_min, s);
+                }
             } else {
+                const int s = ff_replay(T, vent, t0, h, P, [](int, int) {});
                 const bool solid = s < h;
                 const uint8_t nt = solid ? kBasalt : kLava;
                 D.type[0][i] = nt;
@@ -1767,13 +1809,21 @@ __global__ void __launch_bounds__(kBlock) ca_ff_tiles_kernel(CaParams P, CaDev D
                 D.temp[1][i] = T;
                 if (solid) D.solid[i] = static_cast<uint16_t>(t0 + s);  // < maxSteps <= 65535
                 left = left || !solid;
+                last = last || (solid && s == h - 1);
             }
         }
         if (FINAL) {
-            const bool any = __ballot(left) != 0;
-            if (lane == 0)
+            const bool any = __ballot(left) != 0, chg = __ballot(last) != 0;
+            if (lane == 0) {
                 D.tile_flags[tile] = static_cast<uint8_t>((D.tile_flags[tile] & ~kFlagLava) | (any ? kFlagLava : 0));
+                if (chg) D.tile_tchg[tile] = static_cast<uint32_t>(t0 + h);
+            }
         }
+    }
+    if (!FINAL) {
+        // the wave's smallest, one atomic per wave that saw a solidification
+        for (int d = 32; d >= 1; d >>= 1) sol_min = min(sol_min, __shfl_xor(sol_min, d, 64));
+        if (lane == 0 && sol_min != INT_MAX) atomicMin(&D.ctrl->ff_sol, sol_min);
     }
 }
 
@@ -1788,21 +1838,29 @@ __device__ inline unsigned long long wave_scan(unsigned long long v) {
     return v;
 }
 
-// One block: the histories of steps t0 .. t0 + h - 1 from the difference counts (N = the lava cells at t0: active =
-// N - cells that left LAVA so far, changed = N - cells that stopped changing so far), the first step whose finish
+// One block: the histories of the steps t0 .. t0 + he - 1 from the difference counts (N = the lava cells at t0:
+// active = N - cells solidified so far, changed = the running sum of the +1 / -1 flips), where he = h, or with
+// until_solid the steps up to and including the first with a solidification; the first of those steps whose finish
 // stops the run (finishStep's order, LavaCA.hpp:1206-1229: NoLava, NoChange for a deterministic neighbourhood,
-// MaxSteps -- reached at the latest in step t0 + h - 1 = maxSteps - 1), and CaCtrl as that finish leaves it.
-__global__ void __launch_bounds__(kBlock) ca_ff_scan_kernel(CaParams P, CaDev D, int32_t t0, int32_t h) {
+// MaxSteps); and CaCtrl as the finish of the last step kept leaves it (when no stop rule fires, the simulation
+// goes on from t0 + he with the next step counting as the first after a step without births).
+__global__ void __launch_bounds__(kBlock) ca_ff_scan_kernel(CaParams P, CaDev D, int32_t t0, int32_t h,
+                                                            int32_t until_solid) {
     __shared__ unsigned long long s_a[kBlock / 64], s_c[kBlock / 64], s_sum[kBlock / 64];
     __shared__ int s_first[kBlock / 64];
     __shared__ int s_stop_k, s_reason;
     __shared__ unsigned long long s_active;
     CaCtrl* c = D.ctrl;
     const unsigned long long N = c->lava_total;
+    if (until_solid && c->ff_sol < h) h = c->ff_sol + 1;
     const int w = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63u);
     unsigned long long carry_a = 0, carry_c = 0, changed_sum = 0;
-    int stop_k = -1;
-    for (int base = 0; base < h && stop_k < 0; base += kBlock) {
+    int done = 0;
+    if (threadIdx.x == 0) {
+        s_stop_k = h - 1;  // no stop rule fires: the last step kept
+        s_reason = 0;
+    }
+    for (int base = 0; base < h && !done; base += kBlock) {
         const int k = base + static_cast<int>(threadIdx.x);
         const unsigned long long a = k < h ? D.hist_active[t0 + k] : 0ULL;
         const unsigned long long cc = k < h ? D.hist_changed[t0 + k] : 0ULL;
@@ -1821,7 +1879,7 @@ __global__ void __launch_bounds__(kBlock) ca_ff_scan_kernel(CaParams P, CaDev D,
             carry_a += s_a[q];
             carry_c += s_c[q];
         }
-        const unsigned long long active = N - pa, changed = N - pc;
+        const unsigned long long active = N - pa, changed = pc;
         int reason = 0;
         if (k < h) {
             if (active == 0)
@@ -1844,9 +1902,9 @@ __global__ void __launch_bounds__(kBlock) ca_ff_scan_kernel(CaParams P, CaDev D,
         if (k == kmin) {
             s_stop_k = k;
             s_reason = reason;
-            s_active = active;
         }
-        if (kmin != INT_MAX) stop_k = kmin;
+        if (k == min(kmin, h - 1)) s_active = active;
+        done = kmin != INT_MAX;
         __syncthreads();  // s_a, s_c, s_first are rewritten by the next chunk
     }
     changed_sum = wave_sum(changed_sum);
@@ -1859,8 +1917,24 @@ __global__ void __launch_bounds__(kBlock) ca_ff_scan_kernel(CaParams P, CaDev D,
         c->stop = s_reason;
         c->lava_total = s_active;
         c->changes_total += total;
-        c->ff_from = t0;
-        c->ff_steps = s_stop_k + 1;
+        if (c->ff_steps == 0) c->ff_from = t0;
+        c->ff_steps += s_stop_k + 1;
+        c->born_last = 0;
+        c->qdead = 0;  // the types may have changed in the last step kept: the next finish decides again
+    }
+}
+
+// A simulation that goes on after a fast-forward: its entries of list copy par (pushed for step t0) move to the
+// step it now is at.
+__global__ void __launch_bounds__(kBlock) ca_ff_relabel_kernel(BatchDev B, CaDev D, uint32_t slot, int shape, int par) {
+    if (D.ctrl->stop) return;
+    const uint32_t now = static_cast<uint32_t>(D.ctrl->steps);
+    const uint32_t n = B.ctrl->nactive[par][shape];
+    uint64_t* list = B.active + (par * kShapes + shape) * static_cast<size_t>(B.cap);
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        const uint64_t e = list[j];
+        if (((static_cast<uint32_t>(e) >> kTileBits) & 0x3FFu) == slot)
+            list[j] = (static_cast<uint64_t>(now) << 32) | (e & 0xFFFFFFFFull);
     }
 }
 
@@ -2448,16 +2522,18 @@ void ca_launch_owner(uint32_t* owner, uint32_t base, uint32_t n, uint32_t slot, 
     hipLaunchKernelGGL(ca_owner_kernel, dim3(grid_for(n, 4096)), dim3(kBlock), 0, s, owner, base, n, slot);
 }
 
-void ca_launch_fast_forward(const CaParams& P, const CaDev& D, int32_t t0, hipStream_t s, hipEvent_t* ev) {
-    const int32_t h = P.max_steps - t0;
+void ca_launch_fast_forward(const CaParams& P, const CaDev& D, const BatchDev& B, uint32_t slot, int shape, int par,
+                            int32_t t0, int32_t h, bool until_solid, hipStream_t s, hipEvent_t* ev) {
     if (h <= 0) return;
     if (ev) (void)hipEventRecord(ev[0], s);
+    hipLaunchKernelGGL(ca_ff_init_kernel, dim3(1), dim3(1), 0, s, D);
     fill_u32(reinterpret_cast<uint32_t*>(D.hist_active + t0), 0u, 2ull * static_cast<uint64_t>(h), s);
     fill_u32(reinterpret_cast<uint32_t*>(D.hist_changed + t0), 0u, 2ull * static_cast<uint64_t>(h), s);
     const dim3 g((P.ntiles + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(ca_ff_tiles_kernel<false>, g, dim3(kBlock), 0, s, P, D, t0, h);
-    hipLaunchKernelGGL(ca_ff_scan_kernel, dim3(1), dim3(kBlock), 0, s, P, D, t0, h);
+    hipLaunchKernelGGL(ca_ff_scan_kernel, dim3(1), dim3(kBlock), 0, s, P, D, t0, h, until_solid ? 1 : 0);
     hipLaunchKernelGGL(ca_ff_tiles_kernel<true>, g, dim3(kBlock), 0, s, P, D, t0, h);
+    hipLaunchKernelGGL(ca_ff_relabel_kernel, dim3(grid_for(B.cap, 1024)), dim3(kBlock), 0, s, B, D, slot, shape, par);
     if (ev) (void)hipEventRecord(ev[1], s);
 }
 

@@ -94,12 +94,18 @@ struct CaCtrl {
     unsigned long long lava_total, air_total;
     unsigned long long tiles_visited, changes_total;
     unsigned long long tiles_staged;
-    // first step from which the simulation is quiescent (no AIR cell can get LAVA again and the eruption is over:
-    // every later step only cools the lava cells), 0 while it is not; set by the finish of the step before it
+    // first step from which no AIR cell has LAVA at a position the rules read (none can ever be born again: every
+    // later step only cools the lava cells, with the vents' eruption), 0 while there is one; set by the finish of
+    // the step before it
     int32_t quiet;
-    // the fast-forward: the step it started from and the steps it completed (0 when it did not run)
+    // = steps when the last step changed no cell's type and none of its AIR candidates had a source the rules accept
+    // (under any neighbourhood variant) with a candidate temperature at the solidus or above: until a cell's type
+    // changes, no cell can be born; 0 otherwise (set by every finish)
+    int32_t qdead;
+    int32_t born_last;            // cells born in the last step were counted (the rules kernel then skips the check)
+    // the fast-forward: the step it last started from (-1: never) and the steps it completed in all
     int32_t ff_from, ff_steps;
-    int32_t pad;
+    int32_t ff_sol;               // fast-forward: steps from its start to the first step in which a lava cell solidifies
 };
 
 // Counters of one step spread over kPartRows rows (a tile's counts go to row tile % kPartRows: visited tiles
@@ -108,6 +114,8 @@ struct CaCtrl {
 struct CaPart {
     unsigned long long lava_after, born, changed, tiles, staged;
     unsigned long long air_cand;  // AIR cells with LAVA at a position the rules read (candidates of rule 1)
+    unsigned long long air_alive; // of those, the ones not born that a later step could bear (see CaCtrl::qdead)
+    unsigned long long tchanged;  // cells whose type changed (born or solidified)
 };
 constexpr int kPartRows = 256;
 
@@ -276,13 +284,17 @@ void ca_launch_purge(const BatchDev& B, const SlotMask& slots, int par, hipStrea
 void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 // owner[base .. base + n) = slot
 void ca_launch_owner(uint32_t* owner, uint32_t base, uint32_t n, uint32_t slot, hipStream_t s);
-// out[3*s] = steps, out[3*s+1] = stop (-1 when slot s is not running), out[3*s+2] = CaCtrl::quiet, for s < nslots.
+// out[4*s] = steps, out[4*s+1] = stop (-1 when slot s is not running), out[4*s+2] = CaCtrl::quiet, out[4*s+3] =
+// CaCtrl::qdead, for s < nslots.
 void ca_launch_status(const BatchDev& B, int32_t* out, hipStream_t s);
-// Quiescent fast-forward of one simulation from step t0 (>= CaCtrl::quiet, not stopped) to its stop: each lava
-// cell's cooling replayed on its own, the histories of the steps t0 .. stop, the final state in both type and
-// temperature copies, and the step count, stop reason and counters in CaCtrl -- what stepping on would give.
-// ev (nullable): start / end events of the whole sequence.
-void ca_launch_fast_forward(const CaParams& P, const CaDev& D, int32_t t0, hipStream_t s, hipEvent_t* ev);
+// Fast-forward of one simulation from step t0 over at most h steps, while no cell can be born: each lava cell's
+// cooling (and its vent's eruption) replayed on its own, the histories of those steps, the state after them in both
+// type and both temperature copies, and CaCtrl as stepping on would leave it -- stopped when a stop rule fires within
+// them.  until_solid (CaCtrl::qdead): only up to and including the first step in which a lava cell solidifies (a
+// type change may let a cell be born afterwards); otherwise (CaCtrl::quiet) over all h.  When the simulation goes
+// on, its entries in list copy par of batch B (slot) are moved to the new step.  ev (nullable): start / end events.
+void ca_launch_fast_forward(const CaParams& P, const CaDev& D, const BatchDev& B, uint32_t slot, int shape, int par,
+                            int32_t t0, int32_t h, bool until_solid, hipStream_t s, hipEvent_t* ev);
 // Writes entry `index` of the device's simulation table (stream ordered; `sim` must stay valid until the
 // stream reaches the copy).
 void ca_write_sim(uint32_t index, const CaSim* sim, hipStream_t s);
