@@ -92,12 +92,15 @@ def build_calib() -> str:
 
 
 def build(engine: bool = True) -> None:
+    """The product: libvca.so and the engine (the counter-calibration program is measurement tooling, built by
+    build_calib on its own)."""
     build_lib()
     if engine:
         build_engine()
-    build_calib()
 
 
 if __name__ == "__main__":
     build(engine="--no-engine" not in sys.argv)
+    if "--calib" in sys.argv:
+        build_calib()
     print(LIB)
