@@ -240,10 +240,10 @@ def _write_jobs(workloads, folder: str) -> None:
         wl.write(folder)
 
 
-def _pipeline(args, out: str) -> dict:
+def _pipeline(args, out: str, engine_args=()) -> dict:
     t0 = time.time()
     # the engine's per-job progress lines go to our stderr (the run's log)
-    r = subprocess.run([ENGINE, *args, "--out", out, "--set", "output.format=none"], stdout=sys.stderr,
+    r = subprocess.run([ENGINE, *args, "--out", out, "--set", "output.format=none", *engine_args], stdout=sys.stderr,
                        stderr=sys.stderr, timeout=3600)
     if r.returncode != 0:
         raise RuntimeError(f"engine {' '.join(args)} failed ({r.returncode})")
@@ -265,7 +265,7 @@ def _same_stats(a: str, b: str, names) -> bool:
     return True
 
 
-def bench_batched(jobs_n: int) -> dict:
+def bench_batched(jobs_n: int, engine_args=()) -> dict:
     """The uneven mix: the batched pipeline against the same jobs one at a time (--max-jobs 1), same engine;
     stats.json of every job must be equal between the two."""
     from volcano_ca_mc_amd import workloads as W
@@ -274,9 +274,9 @@ def bench_batched(jobs_n: int) -> dict:
         folder = os.path.join(tmp, "jobs")
         _write_jobs(jobs, folder)
         log(f"{len(jobs)} jobs written")
-        one = _pipeline(["--batch", folder, "--gpu-batch", "--max-jobs", "1"], os.path.join(tmp, "one"))
+        one = _pipeline(["--batch", folder, "--gpu-batch", "--max-jobs", "1"], os.path.join(tmp, "one"), engine_args)
         log(f"one at a time: {one['makespanS']:.2f} s, {one['jobsPerS']:.2f} jobs/s")
-        many = _pipeline(["--batch", folder, "--gpu-batch"], os.path.join(tmp, "many"))
+        many = _pipeline(["--batch", folder, "--gpu-batch"], os.path.join(tmp, "many"), engine_args)
         log(f"batched: {many['makespanS']:.2f} s, {many['jobsPerS']:.2f} jobs/s")
         same = _same_stats(os.path.join(tmp, "one"), os.path.join(tmp, "many"), [w.name for w in jobs])
     return {"metric": "jobs/s (batched uneven simulations, bit-exact)", "value": many["jobsPerS"], "unit": "jobs/s",

@@ -306,9 +306,21 @@ struct vca_ca {
     hipStream_t side = nullptr;   // during vca_ca_collect: the pooled stream its reads go on
     Batch* member = nullptr;      // the shared batch this simulation is in
     int slot = -1;
-    // once the simulation is quiescent, a run that goes to its end completes it by the fast-forward
-    // (vca_ca_set_fast_forward)
+    // the fast-forward where it applies (vca_ca_set_fast_forward)
     bool ff = true;
+    // a fast-forward up to the next solidification costs a few launches and a host wait: after one that covered few
+    // steps, the next is not tried before step ff_hold (the hold doubles each time, and resets after a long one)
+    int32_t ff_hold = 0, ff_backoff = kFfBackoff;
+    static constexpr int32_t kFfBackoff = 64, kFfMaxBackoff = 8192, kFfWorthSteps = 32;
+    // after a fast-forward from `from` that left the simulation at `to` (running)
+    void ff_done(int32_t from, int32_t to) {
+        if (to - from >= kFfWorthSteps) {
+            ff_backoff = kFfBackoff;
+        } else {
+            ff_hold = to + ff_backoff;
+            ff_backoff = std::min(2 * ff_backoff, kFfMaxBackoff);
+        }
+    }
     // the type and temperature copies holding the state after `steps` steps
     const void* type_after(int32_t steps) const { return (steps & 1) ? type2.p : type.p; }
     const void* temp_after(int32_t steps) const { return (steps & 1) ? temp2.p : temp.p; }
@@ -387,6 +399,12 @@ private:
 };
 
 // Simulations advanced together (see vca_batch_* in vca.h).  The solo path of a vca_ca is a batch of one.
+// Entries each sub-list of a batch holds at most: a simulation of n tiles has at most ceil(n / kSub) tiles of one
+// residue, and the batch's simulations share at most `tiles` tiles.
+inline uint32_t sub_capacity(uint32_t tiles, int slots) {
+    return (tiles + kSub - 1) / kSub + static_cast<uint32_t>(slots) + 1u;
+}
+
 struct Batch {
     vca_ctx* ctx = nullptr;
     int max_slots = 0;
@@ -422,7 +440,8 @@ struct Batch {
         check(hipGetLastError(), "simulation table");
         owner.alloc(static_cast<size_t>(cap) * 4);
         tile_base.alloc(static_cast<size_t>(slots) * 4);
-        active.alloc(static_cast<size_t>(cap) * kShapes * 2 * 8);
+        const uint32_t subcap = sub_capacity(cap, slots);
+        active.alloc(static_cast<size_t>(subcap) * kSub * kShapes * 2 * 8);
         ctrl.alloc(sizeof(BatchCtrl));
         check(hipMemsetAsync(owner.p, 0xFF, owner.bytes, s), "memset");  // kNoSlot
         check(hipMemsetAsync(ctrl.p, 0, sizeof(BatchCtrl), s), "memset");
@@ -436,7 +455,7 @@ struct Batch {
         B.tile_base = tile_base.as<uint32_t>();
         B.ntiles = 0;
         B.nslots = 0;
-        B.cap = cap;
+        B.subcap = subcap;
         B.active = active.as<uint64_t>();
         B.ctrl = ctrl.as<BatchCtrl>();
         grid_tiles = tile_grid(c, cap);
@@ -451,11 +470,17 @@ struct Batch {
         has_table = false;
         Layout L;
         const size_t o_ctrl = L.add(sizeof(BatchCtrl)), o_base = L.add(4), fixed = L.total;
-        // per tile: its owner (4 B) and its entries in both copies of every shape's list (2 x kShapes x 8 B), each
-        // region 256-byte aligned
-        cap = static_cast<uint32_t>(std::min<size_t>((bytes - fixed - 512) / (4 + 2 * kShapes * 8), kTileMask));
+        // per tile: its owner (4 B) and its entries in both copies of every shape's sub-lists (2 x kShapes x 8 B, plus
+        // the sub-lists' rounding), each region 256-byte aligned: the most tiles that fit
+        auto need = [&](uint32_t c) {
+            return fixed + ((static_cast<size_t>(c) * 4 + 255) & ~static_cast<size_t>(255)) +
+                   static_cast<size_t>(sub_capacity(c, 1)) * kSub * kShapes * 2 * 8;
+        };
+        cap = static_cast<uint32_t>(std::min<size_t>((bytes - fixed) / (4 + 2 * kShapes * 8), kTileMask));
+        while (cap > 1 && need(cap) > bytes) --cap;
+        const uint32_t subcap = sub_capacity(cap, 1);
         const size_t o_owner = L.add(static_cast<size_t>(cap) * 4);
-        const size_t o_active = L.add(static_cast<size_t>(cap) * kShapes * 2 * 8);
+        const size_t o_active = L.add(static_cast<size_t>(subcap) * kSub * kShapes * 2 * 8);
         if (L.total > bytes) throw VcaError(VCA_ERR_ARG, "scratch layout exceeds its array");
         host_sims.assign(1, CaSim{});
         members.assign(1, nullptr);
@@ -470,7 +495,7 @@ struct Batch {
         B.tile_base = reinterpret_cast<uint32_t*>(base + o_base);
         B.ntiles = 0;
         B.nslots = 1;
-        B.cap = cap;
+        B.subcap = subcap;
         B.active = reinterpret_cast<uint64_t*>(base + o_active);
         B.ctrl = reinterpret_cast<BatchCtrl*>(base + o_ctrl);
         grid_tiles = 0;  // set by reset_external (the context's device)
@@ -642,8 +667,10 @@ struct Batch {
         uint64_t tiles[kShapes] = {};
         for (const vca_ca* c : members)
             if (c) tiles[c->shape] += c->P.ntiles;
+        // a multiple of kSub blocks (block b walks sub-list b % kSub)
         for (int k = 0; k < kShapes; ++k)
-            L.grid[k] = static_cast<int>(std::min<uint64_t>(static_cast<uint64_t>(grid_tiles), tiles[k]));
+            L.grid[k] = static_cast<int>(
+                (std::min<uint64_t>(static_cast<uint64_t>(grid_tiles), tiles[k]) + kSub - 1) / kSub * kSub);
         return L;
     }
     // One step on stream s (no timing); advances the parity and the pending finish.
@@ -1353,11 +1380,13 @@ static void ca_status_sync(vca_ca* ca, int32_t* steps, int32_t* stop) {
 // when no AIR cell can get lava any more (quiet), or up to the first solidification while none can be born (qdead).
 // Returns false when neither holds.
 static bool ca_fast_forward(vca_ca* ca, const CaStatus& st, int64_t h, const BatchDev& B, uint32_t slot, int par) {
-    if (!ca->ff || st.stop != VCA_RUNNING) return false;
+    // the fast-forward reads the simulation's tiles from its entries of the next step's list: lava-front simulations
+    // (ca.sweep = full lists its tiles only when the step starts)
+    if (!ca->ff || st.stop != VCA_RUNNING || ca->P.dense || ca->P.classic) return false;
     h = std::min<int64_t>(h, static_cast<int64_t>(ca->P.max_steps) - st.steps);
     if (h <= 0) return false;
     const bool quiet = st.quiet > 0 && st.steps >= st.quiet;
-    const bool dead = st.qdead > 0 && st.steps == st.qdead;
+    const bool dead = st.qdead > 0 && st.steps == st.qdead && st.steps >= ca->ff_hold;
     if (!quiet && !dead) return false;
     vca_ctx* c = ca->ctx;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -1383,8 +1412,11 @@ static CaStatus ca_run_steps(vca_ca* ca, Batch& b, int64_t max_new_steps) {
     int chunk = 128;
     while (st.stop == VCA_RUNNING && st.steps < target) {
         if (ca_fast_forward(ca, st, target - st.steps, b.B, 0u, b.par)) {
+            const int32_t from = st.steps;
             st = ca_status_read(ca);
-            chunk = 1;
+            ca->ff_done(from, st.steps);
+            // after a long one the next may apply right after the solidification that ended it
+            if (st.steps - from >= vca_ca::kFfWorthSteps) chunk = 1;
             continue;
         }
         const int n = static_cast<int>(std::min<int64_t>(target - st.steps, chunk));
@@ -1734,13 +1766,21 @@ extern "C" int vca_batch_status(vca_batch* b, int32_t* out, int32_t cap) {
         };
         read();
         // members where no cell can be born any more (to their stop) or until a type changes: the fast-forward
+        std::vector<int32_t> from(bb.B.nslots, -1);
         bool ff = false;
         for (uint32_t k = 0; k < bb.B.nslots; ++k) {
             vca_ca* m = bb.members[k];
             const CaStatus st{all[4 * k], all[4 * k + 1], all[4 * k + 2], all[4 * k + 3]};
-            if (m && ca_fast_forward(m, st, m->P.max_steps, bb.B, k, bb.par)) ff = true;
+            if (m && ca_fast_forward(m, st, m->P.max_steps, bb.B, k, bb.par)) {
+                from[k] = st.steps;
+                ff = true;
+            }
         }
-        if (ff) read();
+        if (ff) {
+            read();
+            for (uint32_t k = 0; k < bb.B.nslots; ++k)
+                if (from[k] >= 0 && bb.members[k]) bb.members[k]->ff_done(from[k], all[4 * k]);
+        }
         for (int32_t k = 0; k < cap; ++k) {
             const bool in = static_cast<uint32_t>(k) < bb.B.nslots && bb.members[static_cast<size_t>(k)];
             out[2 * k] = in ? all[4 * static_cast<size_t>(k)] : 0;

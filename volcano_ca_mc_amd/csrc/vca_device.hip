@@ -109,6 +109,49 @@ __device__ inline uint32_t mbcnt64(unsigned long long m) {
                                       __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
 }
 
+// Sub-list of tile t (its index within its simulation): t plus its higher 5-bit groups, modulo kSub.  Every aligned run
+// of kSub consecutive tiles covers each sub-list once (the added groups are constant over it), so a simulation of n
+// tiles holds at most ceil(n / kSub) entries of a sub-list; and since every bit of t takes part, a front that covers
+// only some tile columns still spreads over all sub-lists (the consumers walk them in an interleaved order).
+__device__ inline uint32_t sub_of(uint32_t t) {
+    return (t + (t >> 5) + (t >> 10) + (t >> 15) + (t >> 20)) & (kSub - 1);
+}
+
+// Sub-list j of tile list (par, shape) and its count (each count on a 64-byte line of its own).
+__device__ inline uint32_t* list_count(const BatchDev& B, int par, int shape, int j) {
+    return &B.ctrl->nactive[par][shape][j][0];
+}
+__device__ inline uint64_t* sub_list(const BatchDev& B, int par, int shape, int j) {
+    return B.active + static_cast<size_t>((par * kShapes + shape) * kSub + j) * B.subcap;
+}
+
+// Appends ent of every lane with add (all 64 lanes call) to sub-list (par, shape, sub_of(tile)): the lanes are
+// grouped by sub-list, the lowest lane of each group reserves the group's slots, all groups with one atomic
+// instruction (its lanes address different counts), and each lane reads its group's base from the leader.
+__device__ inline void wave_list_append(const BatchDev& B, int par, int shape, bool add, uint32_t tile, uint64_t ent) {
+    const unsigned long long m = __ballot(add);
+    if (m == 0) return;
+    const int lane = static_cast<int>(opaque_lane());
+    const uint32_t j = sub_of(tile);
+    int lead = lane;
+    uint32_t size = 0, off = 0;
+    for (unsigned long long rest = m; rest;) {
+        const int l0 = static_cast<int>(__builtin_ctzll(rest));
+        const uint32_t j0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(j), l0));
+        const unsigned long long grp = __ballot(add && j == j0);
+        if (add && j == j0) {
+            lead = l0;
+            off = mbcnt64(grp);
+        }
+        if (lane == l0) size = static_cast<uint32_t>(__popcll(grp));
+        rest &= ~grp;
+    }
+    uint32_t base = 0;
+    if (add && lead == lane) base = atomicAdd(list_count(B, par, shape, static_cast<int>(j)), size);
+    base = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(lead << 2, static_cast<int>(base)));
+    if (add) sub_list(B, par, shape, static_cast<int>(j))[base + off] = ent;
+}
+
 // Wave-aggregated slot reservation: every lane of the wave must call this (pred may differ per lane).  The
 // leader's reserved base reaches the other lanes by v_readlane (no second LDS round trip).
 __device__ inline uint32_t wave_append(bool pred, uint32_t* counter) {
@@ -242,7 +285,7 @@ __device__ inline void mark_halo(const CaParams& P, const CaDev& D, int tx, int 
 // Lane q < 27 of wave 0 (all 64 lanes call): when bit q of dirs is set, lists tile q of the 3 x 3 x 3 block around tile
 // (tx, ty, tz) (q = 13: the tile itself) for the simulation's step `step` in list copy par -- once per step, by the
 // tile's stamp (listed): the first lane to stamp a tile appends it.  Periodic axes wrap, tiles past an absorbing edge
-// do not exist.  One list atomic per call.
+// do not exist.  One list atomic instruction per call.
 __device__ inline void push_next(const CaParams& P, const CaDev& D, const BatchDev& B, int tx, int ty, int tz, int q,
                                  uint32_t dirs, uint32_t slot, uint32_t step, int par, int shape) {
     bool add = false;
@@ -255,15 +298,7 @@ __device__ inline void push_next(const CaParams& P, const CaDev& D, const BatchD
             add = atomicExch(&D.listed[n], step) != step;
         }
     }
-    const unsigned long long m = __ballot(add);
-    if (m == 0) return;
-    const int lane = static_cast<int>(opaque_lane());
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&B.ctrl->nactive[par][shape], static_cast<uint32_t>(__popcll(m)));
-    base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(base), 0));
-    if (add)
-        B.active[(par * kShapes + shape) * static_cast<size_t>(B.cap) + base + mbcnt64(m)] =
-            (static_cast<uint64_t>(step) << 32) | (slot << kTileBits) | n;
+    wave_list_append(B, par, shape, add, n, (static_cast<uint64_t>(step) << 32) | (slot << kTileBits) | n);
 }
 __device__ inline unsigned wave_or(unsigned v) {
     v |= dpp_row_shr<0x111>(v);
@@ -488,11 +523,27 @@ __global__ void __launch_bounds__(kBlock) ca_init_flags_kernel(CaParams P, CaDev
 // parity par); a block counts its entries by wave ballots and reserves them with one atomic per list (a device-wide
 // counter is the contended resource: one atomic per block, not per wave).  The scan is latency-bound, so one tile per
 // lane (a 512 x 512 x 256 grid of 65536 tiles: 8 per lane 19.5 us, 4 12.7, 2 9.3, 1 8.0 per step).
+// Appends ent of every thread with code >= 0 to sub-list (par, code, sub_of(tile)), all threads of the block
+// calling: positions within the block from LDS counts, one device atomic per non-empty sub-list of the block.
+__device__ inline void block_list_append(const BatchDev& B, int par, int code, uint32_t tile, uint64_t ent) {
+    __shared__ uint32_t s_cnt[kShapes * kSub], s_base[kShapes * kSub];
+    static_assert(kShapes * kSub <= kBlock, "one thread per sub-list");
+    if (threadIdx.x < kShapes * kSub) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const int key = code >= 0 ? code * kSub + static_cast<int>(sub_of(tile)) : 0;
+    uint32_t pos = 0;
+    if (code >= 0) pos = atomicAdd(&s_cnt[key], 1u);
+    __syncthreads();
+    if (threadIdx.x < kShapes * kSub) {
+        const uint32_t c = s_cnt[threadIdx.x];
+        s_base[threadIdx.x] = c ? atomicAdd(list_count(B, par, threadIdx.x / kSub, threadIdx.x % kSub), c) : 0u;
+    }
+    __syncthreads();
+    if (code >= 0) sub_list(B, par, code, key % kSub)[s_base[key] + pos] = ent;
+}
+
 __global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B, uint32_t shapes, int par) {
-    __shared__ uint32_t s_cnt[kShapes][kBlock / 64];
-    __shared__ uint32_t s_base[kShapes];
-    const int w = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63u);
-    const unsigned long long below = (1ULL << lane) - 1ULL;
+    const int lane = static_cast<int>(threadIdx.x & 63u);
     const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t slot = g < B.ntiles ? B.owner[g] : kNoSlot;
     // a wave's 64 tiles nearly always belong to one simulation: then its parameters are read once, as scalars
@@ -550,29 +601,8 @@ __global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B, uint32_t s
         const CaSim& sim = g_sims[B.sim_base + slot];
         if (visit(sim, slot)) atomicAdd(&sim.D.part[(g - B.tile_base[slot]) % kPartRows].tiles, 1ULL);
     }
-#pragma unroll
-    for (int l = 0; l < kShapes; ++l) {
-        if (!((shapes >> l) & 1u)) continue;
-        const uint32_t n = static_cast<uint32_t>(__popcll(__ballot(code == l)));
-        if (lane == 0) s_cnt[l][w] = n;
-    }
-    __syncthreads();
-    if (threadIdx.x < kShapes) {
-        const int l = static_cast<int>(threadIdx.x);
-        uint32_t tot = 0;
-        if ((shapes >> l) & 1u)
-            for (int q = 0; q < kBlock / 64; ++q) tot += s_cnt[l][q];
-        s_base[l] = tot ? atomicAdd(&B.ctrl->nactive[par][l], tot) : 0u;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int l = 0; l < kShapes; ++l) {
-        if (!((shapes >> l) & 1u)) continue;
-        uint32_t pos = s_base[l];
-        for (int q = 0; q < w; ++q) pos += s_cnt[l][q];
-        const unsigned long long mb = __ballot(code == l);
-        if (code == l) B.active[(par * kShapes + l) * static_cast<size_t>(B.cap) + pos + __popcll(mb & below)] = ent;
-    }
+    (void)shapes;
+    block_list_append(B, par, code, static_cast<uint32_t>(ent) & kTileMask, ent);
 }
 
 // finishStep (LavaCA.hpp:1206-1229) + frontCounts (LavaCA.hpp:865-871) of the simulation in slot `slot`: sums and
@@ -878,19 +908,20 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
     __shared__ uint16_t s_list[kTileCells];
     __shared__ uint32_t s_n;
     const uint8_t* s_type = S.type;
-    if (blockIdx.x == fin_blocks && threadIdx.x < kShapes) {
+    if (blockIdx.x == fin_blocks && threadIdx.x < kShapes * kSub) {
         // every kernel of the previous step is done with its list copy; the rules kernel of this step pushes the next
         // step's tiles there
-        B.ctrl->nactive[par ^ 1][threadIdx.x] = 0u;
+        *list_count(B, par ^ 1, threadIdx.x / kSub, threadIdx.x % kSub) = 0u;
     }
     if (blockIdx.x < fin_blocks) {
         finish_slot(B, blockIdx.x);
         return;
     }
-    const uint32_t tile_blocks = gridDim.x - fin_blocks;
-    const uint32_t n = B.ctrl->nactive[par][shape];
-    const uint64_t* tiles = B.active + (par * kShapes + shape) * static_cast<size_t>(B.cap);
-    for (uint32_t s = blockIdx.x - fin_blocks; s < n; s += tile_blocks) {
+    // tile block b walks sub-list b % kSub (the tile blocks are a multiple of kSub)
+    const uint32_t tb = blockIdx.x - fin_blocks, stride = (gridDim.x - fin_blocks) / kSub;
+    const uint32_t n = *list_count(B, par, shape, static_cast<int>(tb % kSub));
+    const uint64_t* tiles = sub_list(B, par, shape, static_cast<int>(tb % kSub));
+    for (uint32_t s = tb / kSub; s < n; s += stride) {
         // uniform: one entry per block; low word slot << kTileBits | tile, high word the simulation's step
         const uint64_t ent = tiles[s];
         const uint32_t e = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ent));
@@ -1199,19 +1230,21 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
     __shared__ uint32_t s_red[6][kBlock / 64];
     __shared__ uint32_t s_wflags[2];  // per-wave flag bytes: [0] staging, [1] end of tile
     __shared__ uint32_t s_bnd0[kBlock / 64];  // per wave: directions with LAVA on the tile's boundary at the step's start
-    const uint64_t* tiles = B.active + (par * kShapes + shape) * static_cast<size_t>(B.cap);
-    // the first entry is read together with the list length (blockIdx.x < gridDim.x <= cap: in bounds)
-    uint64_t next = tiles[blockIdx.x];
-    const uint32_t nact = B.ctrl->nactive[par][shape];
+    // block b walks sub-list b % kSub (the grid is a multiple of kSub); its first entry is read together with the
+    // sub-list's length
+    const uint32_t stride = gridDim.x / kSub, s0 = blockIdx.x / kSub;
+    const uint64_t* tiles = sub_list(B, par, shape, static_cast<int>(blockIdx.x % kSub));
+    uint64_t next = s0 < B.subcap ? tiles[s0] : 0ull;
+    const uint32_t nact = *list_count(B, par, shape, static_cast<int>(blockIdx.x % kSub));
     // the wave's first thread, once (a scalar): inside the loop the thread index is rebuilt from it and the lane
     // (see opaque_tid), so no vector register holds threadIdx.x across the loop
     const unsigned wbase = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x))) & ~63u;
-    for (uint32_t s = blockIdx.x; s < nact; s += gridDim.x) {
+    for (uint32_t s = s0; s < nact; s += stride) {
         const unsigned tid = wbase | opaque_lane();
         // uniform: one entry per block; low word slot << kTileBits | tile, high word the simulation's step
         const uint32_t e = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(next));
         const int32_t t = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(next >> 32));
-        if (s + gridDim.x < nact) next = tiles[s + gridDim.x];
+        if (s + stride < nact) next = tiles[s + stride];
         const CaSim& sim = g_sims[B.sim_base + (e >> kTileBits)];
         const CaParams& P = sim.P;
         const CaDev& D = sim.D;
@@ -1616,10 +1649,8 @@ __global__ void __launch_bounds__(kBlock) ca_finish_kernel(BatchDev B) { finish_
 // Initial list of a lava-front simulation that joins a batch: every tile that holds lava or has a neighbour tile with
 // lava (a superset of the tiles whose 1-cell halo holds lava: the extra tiles settle in the rules kernel), and every
 // tile that changed a cell's type in the last step, for the simulation's current step, into list copy par.  One lane
-// per tile, one list atomic per block.
+// per tile, one list atomic per sub-list a block appends to.
 __global__ void __launch_bounds__(kBlock) ca_seed_kernel(BatchDev B, uint32_t slot, int par) {
-    __shared__ uint32_t s_cnt[kBlock / 64];
-    __shared__ uint32_t s_base;
     const CaSim& sim = g_sims[B.sim_base + slot];
     if (!sim.running || !sim.push) return;
     const CaParams& P = sim.P;
@@ -1644,30 +1675,17 @@ __global__ void __launch_bounds__(kBlock) ca_seed_kernel(BatchDev B, uint32_t sl
                     kFlagLava) != 0;
         }
     }
-    const unsigned long long m = __ballot(want);
-    const int w = static_cast<int>(threadIdx.x >> 6);
-    if ((threadIdx.x & 63u) == 0) s_cnt[w] = static_cast<uint32_t>(__popcll(m));
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t tot = 0;
-        for (int k = 0; k < kBlock / 64; ++k) tot += s_cnt[k];
-        s_base = tot ? atomicAdd(&B.ctrl->nactive[par][sim.shape], tot) : 0u;
-    }
-    __syncthreads();
-    if (want) {
-        uint32_t pos = s_base + mbcnt64(m);
-        for (int k = 0; k < w; ++k) pos += s_cnt[k];
-        B.active[(par * kShapes + sim.shape) * static_cast<size_t>(B.cap) + pos] =
-            (static_cast<uint64_t>(static_cast<uint32_t>(D.ctrl->steps)) << 32) | (slot << kTileBits) | t;
-    }
+    block_list_append(B, par, want ? sim.shape : -1, t,
+                      (static_cast<uint64_t>(static_cast<uint32_t>(D.ctrl->steps)) << 32) | (slot << kTileBits) | t);
 }
 
-// Removes the entries of the slots in `slots` from list copy par (one block; in place, order kept otherwise).
+// Removes the entries of the slots in `slots` from list copy par, every sub-list (one block; in place, order kept
+// otherwise).
 __global__ void __launch_bounds__(1024) ca_purge_kernel(BatchDev B, SlotMask slots, int par) {
     __shared__ uint32_t s_cnt[1024 / 64];
-    for (int k = 0; k < kShapes; ++k) {
-        const uint32_t n = B.ctrl->nactive[par][k];
-        uint64_t* L = B.active + (par * kShapes + k) * static_cast<size_t>(B.cap);
+    for (int k = 0; k < kShapes * kSub; ++k) {
+        const uint32_t n = *list_count(B, par, k / kSub, k % kSub);
+        uint64_t* L = sub_list(B, par, k / kSub, k % kSub);
         uint32_t out = 0;
         for (uint32_t base = 0; base < n; base += 1024) {
             const uint32_t i = base + threadIdx.x;
@@ -1691,7 +1709,8 @@ __global__ void __launch_bounds__(1024) ca_purge_kernel(BatchDev B, SlotMask slo
             out += tot;
             __syncthreads();  // s_cnt is rewritten by the next chunk
         }
-        if (threadIdx.x == 0) B.ctrl->nactive[par][k] = out;
+        if (threadIdx.x == 0) *list_count(B, par, k / kSub, k % kSub) = out;
+        __syncthreads();  // the count is read at the next list's start
     }
 }
 
@@ -1749,58 +1768,86 @@ __device__ inline int ff_replay(float& T, bool vent, int t0, int h, const CaPara
     return h;
 }
 
+// hist[key] += val for every lane with has (all 64 lanes call), one device atomic per distinct key of the wave: many
+// cells share a step (every lava cell starts changing in the first one), and atomics on one 64-byte line serialise.
+__device__ inline void wave_hist_add(unsigned long long* hist, bool has, int key, long long val) {
+    for (unsigned long long m = __ballot(has); m;) {
+        const int l0 = static_cast<int>(__builtin_ctzll(m));
+        const int k0 = __builtin_amdgcn_readlane(key, l0);
+        const bool in = has && key == k0;
+        m &= ~__ballot(in);
+        const unsigned long long sum = wave_sum(in ? static_cast<unsigned long long>(val) : 0ull);
+        if (static_cast<int>(opaque_lane()) == l0 && sum != 0) atomicAdd(&hist[k0], sum);
+    }
+}
+
 // Sets up a fast-forward: the first step with a solidification is not known yet.
 __global__ void ca_ff_init_kernel(CaDev D) { D.ctrl->ff_sol = INT_MAX; }
 
-// The lava cells at the start of step t0, one wave per 64 tiles (the tiles flagged with lava: a lava cell's tile always
-// is).  FINAL = false: difference counts over the steps t0 .. t0 + h - 1 into the history arrays (zeroed by the
-// caller) -- hist_active[t0 + s] += 1 for a cell that solidifies in step t0 + s, hist_changed[t0 + k] += +1 / -1
-// where a cell starts / stops counting as changed -- and the first step with a solidification (CaCtrl::ff_sol).
-// FINAL = true: the state after the steps the scan kept (h = CaCtrl::steps - t0) into both type and both temperature
-// copies (also in the tiles that changed a type in step t0 - 1, whose other type copy is behind), the tiles' lava
-// flags, and the step of a tile's last type change (tile_tchg) where a cell solidified in the last of those steps.
+// The lava cells at the start of step t0, in the tiles of the simulation's entries of list copy par (every tile that
+// holds lava is listed, and so is every tile that changed a type in step t0 - 1), one block per tile and a lane per
+// cell, blocks walking the sub-lists as the step kernels do.  FINAL = false: difference counts over the steps t0 ..
+// t0 + h - 1 into the history arrays (zeroed by the caller) -- hist_active[t0 + s] += 1 for a cell that solidifies in
+// step t0 + s, hist_changed[t0 + k] += +1 / -1 where a cell starts / stops counting as changed -- and the first step
+// with a solidification (CaCtrl::ff_sol).  FINAL = true: the state after the steps the scan kept (h = CaCtrl::steps -
+// t0) into both type and both temperature copies (the other type copy is behind in the tiles that changed a type in
+// step t0 - 1), the tiles' lava flags, and the step of a tile's last type change (tile_tchg) where a cell solidified in
+// the last of those steps.
 template <bool FINAL>
-__global__ void __launch_bounds__(kBlock) ca_ff_tiles_kernel(CaParams P, CaDev D, int32_t t0, int32_t h) {
+__global__ void __launch_bounds__(kBlock) ca_ff_tiles_kernel(BatchDev B, uint32_t slot, int shape, int par, CaParams P,
+                                                             CaDev D, int32_t t0, int32_t h) {
+    __shared__ int s_flags;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t first = blockIdx.x * kBlock + (threadIdx.x & ~63u);
+    const int w = static_cast<int>(threadIdx.x >> 6);
     const uint8_t* cur = D.type[t0 & 1];
     const float* tcur = D.temp[t0 & 1];
     if (FINAL) h = __builtin_amdgcn_readfirstlane(D.ctrl->steps) - t0;
-    bool want = false;
-    if (first + lane < P.ntiles) {
-        want = (D.tile_flags[first + lane] & kFlagLava) != 0;
-        if (FINAL && t0 > 0) want = want || D.tile_tchg[first + lane] == static_cast<uint32_t>(t0);
-    }
     const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
+    const uint32_t stride = gridDim.x / kSub;
+    const uint32_t n = *list_count(B, par, shape, static_cast<int>(blockIdx.x % kSub));
+    const uint64_t* list = sub_list(B, par, shape, static_cast<int>(blockIdx.x % kSub));
     int sol_min = INT_MAX;
-    for (uint64_t m = __ballot(want); m; m &= m - 1) {
-        const uint32_t tile = first + static_cast<uint32_t>(__builtin_ctzll(m));
+    for (uint32_t e = blockIdx.x / kSub; e < n; e += stride) {
+        const uint32_t ent = static_cast<uint32_t>(list[e]);
+        if ((ent >> kTileBits) != slot) continue;  // block-uniform
+        const uint32_t tile = ent & kTileMask;
         const TileCoord o = tile_origin(P, tile);
         const int x = o.x0 + static_cast<int>(lane);
         bool left = false, last = false;
-        for (int row = 0; row < P.ty * P.tz; ++row) {
+#pragma unroll
+        for (int k = 0; k < kTileCells / kBlock; ++k) {  // wave-uniform: row w + 4 k
+            const int row = w + 4 * k;
             const int y = o.y0 + row % P.ty, z = o.z0 + row / P.ty;
-            if (x >= P.nx || y >= P.ny || z >= P.nz) continue;
+            const bool in = x < P.nx && y < P.ny && z < P.nz;
             const uint32_t i = static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y) +
                                plane * static_cast<uint32_t>(z);
-            const uint8_t ty = cur[i];
-            if (ty != kLava) {
-                if (FINAL) D.type[(t0 + 1) & 1][i] = ty;
-                continue;
-            }
-            const int32_t f = D.flow[i];
-            const bool vent = f > 0 && f < P.nvent_slots && D.vent_cell[f] == i;
-            float T = tcur[i];
-            if (!FINAL) {
-                const int s = ff_replay(T, vent, t0, h, P, [&](int k, int d) {
-                    atomicAdd(&D.hist_changed[t0 + k], static_cast<unsigned long long>(static_cast<long long>(d)));
+            const uint8_t ty = in ? cur[i] : kVoidLds;
+            const bool lava = ty == kLava;
+            if (FINAL && in && !lava) D.type[(t0 + 1) & 1][i] = ty;
+            int s = h;
+            float T = 0.0f;
+            int fk[4] = {0, 0, 0, 0}, fd[4] = {0, 0, 0, 0}, nf = 0;
+            if (lava) {
+                const int32_t f = D.flow[i];
+                const bool vent = f > 0 && f < P.nvent_slots && D.vent_cell[f] == i;
+                T = tcur[i];
+                // a cell starts / stops counting as changed at most four times (a vent: its first reset, the
+                // eruption's constant temperature, the cooling after it, its end)
+                s = ff_replay(T, vent, t0, h, P, [&](int kk, int d) {
+                    if (!FINAL && nf < 4) {
+                        fk[nf] = kk;
+                        fd[nf] = d;
+                    }
+                    ++nf;
                 });
-                if (s < h) {
-                    atomicAdd(&D.hist_active[t0 + s], 1ULL);
-                    sol_min = min(sol_min, s);
-                }
-            } else {
-                const int s = ff_replay(T, vent, t0, h, P, [](int, int) {});
+            }
+            if (!FINAL) {
+                // added wave by wave: many cells share a step
+#pragma unroll
+                for (int q = 0; q < 4; ++q) wave_hist_add(D.hist_changed + t0, lava && q < nf, fk[q], fd[q]);
+                wave_hist_add(D.hist_active + t0, lava && s < h, s, 1);
+                if (lava && s < h) sol_min = min(sol_min, s);
+            } else if (lava) {
                 const bool solid = s < h;
                 const uint8_t nt = solid ? kBasalt : kLava;
                 D.type[0][i] = nt;
@@ -1813,11 +1860,18 @@ __global__ void __launch_bounds__(kBlock) ca_ff_tiles_kernel(CaParams P, CaDev D
             }
         }
         if (FINAL) {
-            const bool any = __ballot(left) != 0, chg = __ballot(last) != 0;
-            if (lane == 0) {
-                D.tile_flags[tile] = static_cast<uint8_t>((D.tile_flags[tile] & ~kFlagLava) | (any ? kFlagLava : 0));
-                if (chg) D.tile_tchg[tile] = static_cast<uint32_t>(t0 + h);
+            if (threadIdx.x == 0) s_flags = 0;
+            __syncthreads();
+            const uint32_t f = (__ballot(left) ? 1u : 0u) | (__ballot(last) ? 2u : 0u);
+            if (lane == 0 && f) atomicOr(&s_flags, static_cast<int>(f));
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const int fl = s_flags;
+                D.tile_flags[tile] =
+                    static_cast<uint8_t>((D.tile_flags[tile] & ~kFlagLava) | ((fl & 1) ? kFlagLava : 0));
+                if (fl & 2) D.tile_tchg[tile] = static_cast<uint32_t>(t0 + h);
             }
+            __syncthreads();  // s_flags is reset for the next tile
         }
     }
     if (!FINAL) {
@@ -1929,8 +1983,8 @@ __global__ void __launch_bounds__(kBlock) ca_ff_scan_kernel(CaParams P, CaDev D,
 __global__ void __launch_bounds__(kBlock) ca_ff_relabel_kernel(BatchDev B, CaDev D, uint32_t slot, int shape, int par) {
     if (D.ctrl->stop) return;
     const uint32_t now = static_cast<uint32_t>(D.ctrl->steps);
-    const uint32_t n = B.ctrl->nactive[par][shape];
-    uint64_t* list = B.active + (par * kShapes + shape) * static_cast<size_t>(B.cap);
+    const uint32_t n = *list_count(B, par, shape, static_cast<int>(blockIdx.y));  // one sub-list per grid row
+    uint64_t* list = sub_list(B, par, shape, static_cast<int>(blockIdx.y));
     for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
         const uint64_t e = list[j];
         if (((static_cast<uint32_t>(e) >> kTileBits) & 0x3FFu) == slot)
@@ -2529,11 +2583,13 @@ void ca_launch_fast_forward(const CaParams& P, const CaDev& D, const BatchDev& B
     hipLaunchKernelGGL(ca_ff_init_kernel, dim3(1), dim3(1), 0, s, D);
     fill_u32(reinterpret_cast<uint32_t*>(D.hist_active + t0), 0u, 2ull * static_cast<uint64_t>(h), s);
     fill_u32(reinterpret_cast<uint32_t*>(D.hist_changed + t0), 0u, 2ull * static_cast<uint64_t>(h), s);
-    const dim3 g((P.ntiles + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(ca_ff_tiles_kernel<false>, g, dim3(kBlock), 0, s, P, D, t0, h);
+    // the step kernels' grid for the simulation's tiles: a multiple of kSub blocks, at most 32 per CU-sized share
+    const dim3 g(static_cast<unsigned>(std::min<uint64_t>((static_cast<uint64_t>(P.ntiles) + kSub - 1) / kSub, 256u) * kSub));
+    hipLaunchKernelGGL(ca_ff_tiles_kernel<false>, g, dim3(kBlock), 0, s, B, slot, shape, par, P, D, t0, h);
     hipLaunchKernelGGL(ca_ff_scan_kernel, dim3(1), dim3(kBlock), 0, s, P, D, t0, h, until_solid ? 1 : 0);
-    hipLaunchKernelGGL(ca_ff_tiles_kernel<true>, g, dim3(kBlock), 0, s, P, D, t0, h);
-    hipLaunchKernelGGL(ca_ff_relabel_kernel, dim3(grid_for(B.cap, 1024)), dim3(kBlock), 0, s, B, D, slot, shape, par);
+    hipLaunchKernelGGL(ca_ff_tiles_kernel<true>, g, dim3(kBlock), 0, s, B, slot, shape, par, P, D, t0, h);
+    hipLaunchKernelGGL(ca_ff_relabel_kernel, dim3(grid_for(B.subcap, 64), kSub), dim3(kBlock), 0, s, B, D, slot, shape,
+                       par);
     if (ev) (void)hipEventRecord(ev[1], s);
 }
 
