@@ -1389,6 +1389,17 @@ static bool ca_fast_forward(vca_ca* ca, const CaStatus& st, int64_t h, const Bat
     const bool dead = st.qdead > 0 && st.steps == st.qdead && st.steps >= ca->ff_hold;
     if (!quiet && !dead) return false;
     vca_ctx* c = ca->ctx;
+    if (!quiet) {
+        // up to the first solidification only: skipped (and held off) when that is estimated within a few steps
+        ca_launch_ff_estimate(ca->P, ca->D, B, slot, ca->shape, par, st.steps, c->stream);
+        CaCtrl cc;
+        check(hipMemcpyAsync(&cc, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, c->stream), "read ctrl");
+        check(hipStreamSynchronize(c->stream), "fast-forward estimate");
+        if (cc.ff_sol < vca_ca::kFfWorthSteps) {
+            ca->ff_done(st.steps, st.steps);
+            return false;
+        }
+    }
     hipEvent_t ev[2] = {nullptr, nullptr};
     size_t a = 0;
     if (c->timing) {

@@ -1927,6 +1927,44 @@ __global__ void __launch_bounds__(kBlock) ca_ff_tiles_kernel(BatchDev B, uint32_
     }
 }
 
+// Estimate of the steps from t0 to the first solidification (CaCtrl::ff_sol, set to INT_MAX by ca_ff_init first): per
+// lava cell (temperature - solidus) / coolingAmbient, a vent while it erupts from the eruption's end on -- no replay,
+// so that a fast-forward that would end within a few steps can be skipped cheaply.
+__global__ void __launch_bounds__(kBlock) ca_ff_estimate_kernel(BatchDev B, uint32_t slot, int shape, int par, CaParams P,
+                                                                CaDev D, int32_t t0) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const int w = static_cast<int>(threadIdx.x >> 6);
+    const uint8_t* cur = D.type[t0 & 1];
+    const float* tcur = D.temp[t0 & 1];
+    const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
+    const uint32_t stride = gridDim.x / kSub;
+    const uint32_t n = *list_count(B, par, shape, static_cast<int>(blockIdx.x % kSub));
+    const uint64_t* list = sub_list(B, par, shape, static_cast<int>(blockIdx.x % kSub));
+    const float rc = P.cool_amb > 0.0f ? 1.0f / P.cool_amb : 0.0f;
+    float best = 1e9f;
+    for (uint32_t e = blockIdx.x / kSub; e < n; e += stride) {
+        const uint32_t ent = static_cast<uint32_t>(list[e]);
+        if ((ent >> kTileBits) != slot) continue;
+        const TileCoord o = tile_origin(P, ent & kTileMask);
+        const int x = o.x0 + static_cast<int>(lane);
+        for (int k = 0; k < kTileCells / kBlock; ++k) {
+            const int row = w + 4 * k;
+            const int y = o.y0 + row % P.ty, z = o.z0 + row / P.ty;
+            if (x >= P.nx || y >= P.ny || z >= P.nz) continue;
+            const uint32_t i = static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y) +
+                               plane * static_cast<uint32_t>(z);
+            if (cur[i] != kLava) continue;
+            const int32_t f = D.flow[i];
+            const bool vent = t0 < P.eruption_steps && f > 0 && f < P.nvent_slots && D.vent_cell[f] == i;
+            const float est = vent ? static_cast<float>(P.eruption_steps - t0) + (P.erupt_temp - P.solidus) * rc
+                                   : (tcur[i] - P.solidus) * rc;
+            best = rc > 0.0f ? fminf(best, est) : best;
+        }
+    }
+    for (int d = 32; d >= 1; d >>= 1) best = fminf(best, __shfl_xor(best, d, 64));
+    if (lane == 0 && best < 1e9f) atomicMin(&D.ctrl->ff_sol, static_cast<int>(fmaxf(0.0f, best)));
+}
+
 // Inclusive prefix sum over the 64 lanes of a wave.
 __device__ inline unsigned long long wave_scan(unsigned long long v) {
     const int lane = static_cast<int>(threadIdx.x & 63u);
@@ -2620,6 +2658,13 @@ void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
 void ca_launch_owner(uint32_t* owner, uint32_t base, uint32_t n, uint32_t slot, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(ca_owner_kernel, dim3(grid_for(n, 4096)), dim3(kBlock), 0, s, owner, base, n, slot);
+}
+
+void ca_launch_ff_estimate(const CaParams& P, const CaDev& D, const BatchDev& B, uint32_t slot, int shape, int par,
+                           int32_t t0, hipStream_t s) {
+    hipLaunchKernelGGL(ca_ff_init_kernel, dim3(1), dim3(1), 0, s, D);
+    const dim3 g(static_cast<unsigned>(std::min<uint64_t>((static_cast<uint64_t>(P.ntiles) + kSub - 1) / kSub, 256u) * kSub));
+    hipLaunchKernelGGL(ca_ff_estimate_kernel, g, dim3(kBlock), 0, s, B, slot, shape, par, P, D, t0);
 }
 
 void ca_launch_fast_forward(const CaParams& P, const CaDev& D, const BatchDev& B, uint32_t slot, int shape, int par,

@@ -298,6 +298,10 @@ void ca_launch_status(const BatchDev& B, int32_t* out, hipStream_t s);
 // on, its entries in list copy par of batch B (slot) are moved to the new step.  ev (nullable): start / end events.
 void ca_launch_fast_forward(const CaParams& P, const CaDev& D, const BatchDev& B, uint32_t slot, int shape, int par,
                             int32_t t0, int32_t h, bool until_solid, hipStream_t s, hipEvent_t* ev);
+// Estimate of the steps from t0 to the first solidification into CaCtrl::ff_sol (no replay): whether a fast-forward
+// up to it is worth its launches.
+void ca_launch_ff_estimate(const CaParams& P, const CaDev& D, const BatchDev& B, uint32_t slot, int shape, int par,
+                           int32_t t0, hipStream_t s);
 // Writes entry `index` of the device's simulation table (stream ordered; `sim` must stay valid until the
 // stream reaches the copy).
 void ca_write_sim(uint32_t index, const CaSim* sim, hipStream_t s);
