@@ -109,7 +109,7 @@ int vca_ca_fast_forwarded(vca_ca* ca, int32_t* from_step, int32_t* steps);
  * memory (code under HIP graph capture, tight serving loops, harnesses that forbid it).  The simulation (created
  * beforehand by vca_ca_create) is advanced exactly as vca_ca_run does; the batch-of-one state vca_ca_run allocates on
  * first use (list counts, tile owners, the two copies of the tile lists) lives instead in a static device array of
- * 16 MiB per device, set up by vca_ctx_create: grids of at most 199,392 tiles of 1024 cells, about 204 M cells
+ * 16 MiB per device, set up by vca_ctx_create: grids of at most 199,713 tiles of 1024 cells, about 204 M cells
  * (VCA_ERR_ARG beyond, with the limit in the message).  Calls on one device are serialised by a lock; the array keeps the last call's
  * contents and is re-initialised by every call.  VCA_ERR_STATE when the simulation is in a batch or was advanced by
  * vca_ca_run / vca_ca_enqueue before (their private batch holds its lists). */

@@ -64,9 +64,9 @@ def test_noalloc_state_errors_and_size_limit(ctx):
     with pytest.raises(RuntimeError, match="libvca error -4"):
         ca.run(5, noalloc=True)
     ca.close()
-    # a grid of 200,704 tiles (classic x-y plane 16384 x 12544, tiles of 64 x 16): beyond the array's 199,392
+    # a grid of 200,704 tiles (classic x-y plane 16384 x 12544, tiles of 64 x 16): beyond the array's 199,713
     big = N.LavaCA(ctx, (16384, 12544, 1), N.CaParams(seed=1, classic=True, max_steps=3), "moore", False, None, None,
                    [[5, 5, 0, 1]])
-    with pytest.raises(RuntimeError, match="at most 199392 tiles"):
+    with pytest.raises(RuntimeError, match="at most 199713 tiles"):
         big.run(3, noalloc=True)
     big.close()

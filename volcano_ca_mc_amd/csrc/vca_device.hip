@@ -109,10 +109,11 @@ __device__ inline uint32_t mbcnt64(unsigned long long m) {
                                       __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
 }
 
-// Sub-list of tile t (its index within its simulation): t plus its higher 5-bit groups, modulo kSub.  Every aligned run
-// of kSub consecutive tiles covers each sub-list once (the added groups are constant over it), so a simulation of n
-// tiles holds at most ceil(n / kSub) entries of a sub-list; and since every bit of t takes part, a front that covers
-// only some tile columns still spreads over all sub-lists (the consumers walk them in an interleaved order).
+// Sub-list of tile t (its index within its simulation; with kSub = 1 there is one): t plus its higher 5-bit groups,
+// modulo kSub (a power of two up to 32).  Every aligned run of kSub consecutive tiles covers each sub-list once (the
+// added groups are constant over it), so a simulation of n tiles holds at most ceil(n / kSub) entries of a sub-list;
+// and since every bit of t takes part, a front that covers only some tile columns still spreads over all sub-lists
+// (the consumers walk them in an interleaved order).
 __device__ inline uint32_t sub_of(uint32_t t) {
     return (t + (t >> 5) + (t >> 10) + (t >> 15) + (t >> 20)) & (kSub - 1);
 }
@@ -726,20 +727,6 @@ struct HaloTypes {
 // lx + 64 * ly; 0 where a column holds no cells).  Every load of a thread
 // is issued before its first LDS store, so a tile costs one memory latency.  Returns whether this thread saw
 // lava; own_air |= this thread saw AIR among the tile's own cells.
-// A spill level as the rules kernel keeps it: int8_t relative to the tile's lowest layer z0, clamped to [-24, 24].  The
-// rules compare spill levels only with z' in [z0 - 19, z0 + 16] (a cell of the tile, the two layers under it for
-// ponding, the drop chains down to 18 layers under it): a level above z0 + 24 or below z0 - 24 compares with every such
-// z' as its clamped value does.  int32_t: the level itself.
-template <class SP>
-__device__ inline SP spill_store(int32_t v, int z0) {
-    if constexpr (sizeof(SP) == 1) {
-        const int r = v - z0;
-        return static_cast<SP>(r < -24 ? -24 : (r > 24 ? 24 : r));
-    } else {
-        return static_cast<SP>(v);
-    }
-}
-
 // Halo row bases of the tile at o, one per lane: lane j < R holds halo_row(j).  Rows are then read with
 // v_readlane (scalar row index) instead of each wave redoing the row arithmetic in scalar code.
 template <int TY, int TZ>
@@ -775,10 +762,10 @@ struct NextTypes {
 // With SYNC, N gets the previous step's changes that the other type copy `other` does not hold yet: when `changed`
 // (the tile changed a cell's type in the previous step) the tile's own cells are read from that copy as well, else
 // the copies agree on them.
-template <int TY, int TZ, bool OWN = false, bool SYNC = false, class SP = int32_t>
+template <int TY, int TZ, bool OWN = false, bool SYNC = false>
 __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const uint8_t* type, const uint8_t* other,
                                   bool changed, NextTypes<TY, TZ>* N, const TileCoord& o, HaloTypes<TY, TZ>& S,
-                                  bool with_spill, SP* s_spill, int& own_air, uint32_t row_base, int gx,
+                                  bool with_spill, int32_t* s_spill, int& own_air, uint32_t row_base, int gx,
                                   unsigned tid) {
     using H = Halo<TY, TZ>;
     constexpr int NS = OWN ? kTileX * TY : H::LXY;     // spill columns staged
@@ -882,7 +869,7 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const uint8
 #pragma unroll
         for (int k = 0; k < KS; ++k) {
             const int q = t + k * kBlock;
-            if (q < NS) s_spill[q] = spill_store<SP>(sp[k], o.z0);
+            if (q < NS) s_spill[q] = sp[k];
         }
     }
     return lava;
@@ -1048,7 +1035,7 @@ struct RulesTile {
     int8_t desc[H::N];
     F flow[H::N];
     float temp[H::N];
-    int8_t spill[kTileX * TY];   // spill heights of the tile's own columns (lx + 64 * ly), relative (spill_store)
+    int32_t spill[kTileX * TY];  // spill heights of the tile's own columns (lx + 64 * ly)
     // LAVA bits of halo row r over all LX = 66 positions (bit p = halo column p), as three 32-bit parts + 0
     uint32_t row66[H::R][4];
     // what the step writes into the other type copy (see NextTypes): the previous step's changes (staging), then this
@@ -1225,13 +1212,15 @@ __device__ void vote_positions(const CaParams& P, const RulesTile<TY, TZ, F>& T,
 // all independent coalesced loads), so an AIR cell folds its 26 neighbour types into a bitmask without
 // branches and gathers its candidates from LDS.  Cells whose candidates carry more than one flowId are gathered
 // again with two vote slots, three or more flowIds with the full vote (vote_positions).
-// Register budget: as many waves per SIMD as tiles per CU fit in LDS, at most seven (the 4 x 4 tile with 16-bit
-// flowIds and 8-bit relative spill levels: 22.6 KB, seven tiles per CU at 72 registers) -- more tiles in flight hide
-// the staging latency better than the few registers the limit costs.
+// Register budget: as many waves per SIMD as tiles per CU fit in LDS (the 4 x 4 tile with 16-bit flowIds: six,
+// 80 registers; the others five at most, 96 registers) -- more tiles in flight hide the staging latency better
+// than the few registers the limit costs.
 template <int TY, int TZ, class F>
 constexpr int kRulesTilesPerCu = static_cast<int>(163840 / (sizeof(RulesTile<TY, TZ, F>) + 512));  // 160 KB of LDS
 template <int TY, int TZ, class F>
-constexpr int kRulesWaves = kRulesTilesPerCu<TY, TZ, F> < 7 ? kRulesTilesPerCu<TY, TZ, F> : 7;
+constexpr int kRulesWaves = (TY == 4 && TZ == 4 && sizeof(F) == 2)
+                                ? 6
+                                : (kRulesTilesPerCu<TY, TZ, F> < 5 ? kRulesTilesPerCu<TY, TZ, F> : 5);
 
 template <int TY, int TZ, class F>
 __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_kernel(BatchDev B, int shape, int par) {
@@ -1296,34 +1285,8 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         const int gx = halo_x(P, o.x0 + static_cast<int>(tid & 63u));
         // the tile changed a cell's type in the previous step: the other type copy is behind on it
         const bool changed = t > 0 && __builtin_amdgcn_readfirstlane(D.tile_tchg[tile]) == static_cast<uint32_t>(t);
-        // the tile's own temperatures and, while the vents erupt, flowIds, read with the types (cell tid + 256 k of
-        // the tile: row w + 4 k, column lx): the loop cools the tile's lava cells from them, and a tile without AIR
-        // candidates reads nothing more
-        const int ow = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
-        float otp[4];
-        int32_t ofl[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int row = ow + 4 * k;
-            const uint32_t base = row_of(row_base, (row % TY + 1) + LY * (row / TY + 1));
-            otp[k] = 0.0f;
-            ofl[k] = 0;
-            if (!P.classic && base != kNoCell && gx >= 0) {
-                otp[k] = tcur[base + static_cast<uint32_t>(gx)];
-                if (erupting) ofl[k] = D.flow[base + static_cast<uint32_t>(gx)];
-            }
-        }
-        const bool saw_lava = stage_rows<TY, TZ, true, true, int8_t>(P, D, tyc, tyn, changed, &T.next, o, T.S,
-                                                                     has_spill, T.spill, own_air, row_base, gx, tid);
-        if (!P.classic) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int row = ow + 4 * k;
-                const int q = static_cast<int>(tid & 63u) + 1 + LX * ((row % TY + 1) + LY * (row / TY + 1));
-                T.temp[q] = otp[k];
-                if (erupting) T.flow[q] = static_cast<F>(ofl[k]);
-            }
-        }
+        const bool saw_lava = stage_rows<TY, TZ, true, true>(P, D, tyc, tyn, changed, &T.next, o, T.S, has_spill,
+                                                             T.spill, own_air, row_base, gx, tid);
         if (__builtin_amdgcn_readfirstlane(stop)) continue;
         if (tid == 0) {
             const uint32_t zero = opaque_zero();
@@ -1351,14 +1314,52 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         }
         int tile_lava = 0, tile_air = 0;
         {
-            // the candidate lists and the row masks of the LAVA positions, from the staged types
+            // flowId / temperature / D(n) of the LAVA positions, all loads in flight before the stores
             const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
             const int lx = static_cast<int>(tid & 63u);
             const int t = static_cast<int>(tid);
+            int32_t fl[H::KR];
+            float tp[H::KR];
+            int32_t ds[H::KR];  // 32-bit: see stage_rows
+            uint32_t lava = 0;
+#pragma unroll
+            for (int k = 0; k < H::KR; ++k) {
+                const int r = w + 4 * k;
+                fl[k] = 0;
+                tp[k] = 0.0f;
+                ds[k] = -1;
+                // the staging wrote LAVA only where a cell exists; the load is guarded as well, so that no mask bit
+                // can address memory outside the grid (kNoCell row, gx == -1)
+                const uint32_t base = r < H::R ? row_of(row_base, r) : kNoCell;
+                if (base != kNoCell && gx >= 0 && ((T.S.lava[r] >> lx) & 1u)) {
+                    lava |= 1u << k;
+                    const uint32_t g = base + static_cast<uint32_t>(gx);
+                    fl[k] = D.flow[g];
+                    if (!P.classic) {
+                        tp[k] = tcur[g];
+                        ds[k] = D.descent[g];
+                    }
+                }
+            }
+            int32_t efl = 0;
+            float etp = 0.0f;
+            int32_t eds = -1;
+            const uint32_t ebase = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(((t >> 1) & 63) << 2, static_cast<int>(row_base)));
+            const int ex = halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1);
+            const bool elava = t < 2 * H::R && ebase != kNoCell && ex >= 0 && T.S.edge[t];
+            if (elava) {
+                const uint32_t g = ebase + static_cast<uint32_t>(ex);
+                efl = D.flow[g];
+                if (!P.classic) {
+                    etp = tcur[g];
+                    eds = D.descent[g];
+                }
+            }
             {
-                // candidate cells: LAVA, or AIR next to a LAVA position the rules read -- the 3 x 3 halo rows around
-                // the cell's row dilated by one in x in classic mode; in lava mode the rows of the cell's layer and the
-                // one above (dz >= 0) plus, with ponding, the cell directly below.  Every other cell keeps its state.
+                // candidate cells (while the lava loads above are in flight): LAVA, or AIR next to a LAVA position
+                // the rules read -- the 3 x 3 halo rows around the cell's row dilated by one in x in classic mode;
+                // in lava mode the rows of the cell's layer and the one above (dz >= 0) plus, with ponding, the cell
+                // directly below.  Every other cell keeps its state.
                 // lane j < 16 folds the halo rows around the tile's row j; each wave then reads its four rows' masks
                 // from those lanes
                 uint64_t near_j = 0;
@@ -1410,61 +1411,6 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                     if (ca[k]) T.list[kTileCells - 1 - sa[k]] = cell;
                 }
             }
-            if (t < H::R) {
-                const uint64_t m = T.S.lava[t];
-                const uint64_t lo = (m << 1) | static_cast<uint64_t>(T.S.edge[2 * t]);
-                const uint32_t hi = static_cast<uint32_t>(m >> 63) | (static_cast<uint32_t>(T.S.edge[2 * t + 1]) << 1);
-                T.row66[t][0] = static_cast<uint32_t>(lo);
-                T.row66[t][1] = static_cast<uint32_t>(lo >> 32);
-                T.row66[t][2] = hi;
-                T.row66[t][3] = 0;
-            }
-        }
-        __syncthreads();  // the candidate lists are in LDS
-        if (s_nair != 0) {
-            // flowId / temperature / D(n) of the LAVA positions (the sources the AIR candidates gather from), all
-            // loads in flight before the stores; without AIR candidates only the tile's own lava temperatures
-            // (and flowIds while the vents erupt) are read, and those were staged with the types
-            const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
-            const int lx = static_cast<int>(tid & 63u);
-            const int t = static_cast<int>(tid);
-            int32_t fl[H::KR];
-            float tp[H::KR];
-            int32_t ds[H::KR];  // 32-bit: see stage_rows
-            uint32_t lava = 0;
-#pragma unroll
-            for (int k = 0; k < H::KR; ++k) {
-                const int r = w + 4 * k;
-                fl[k] = 0;
-                tp[k] = 0.0f;
-                ds[k] = -1;
-                // the staging wrote LAVA only where a cell exists; the load is guarded as well, so that no mask bit
-                // can address memory outside the grid (kNoCell row, gx == -1)
-                const uint32_t base = r < H::R ? row_of(row_base, r) : kNoCell;
-                if (base != kNoCell && gx >= 0 && ((T.S.lava[r] >> lx) & 1u)) {
-                    lava |= 1u << k;
-                    const uint32_t g = base + static_cast<uint32_t>(gx);
-                    fl[k] = D.flow[g];
-                    if (!P.classic) {
-                        tp[k] = tcur[g];
-                        ds[k] = D.descent[g];
-                    }
-                }
-            }
-            int32_t efl = 0;
-            float etp = 0.0f;
-            int32_t eds = -1;
-            const uint32_t ebase = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(((t >> 1) & 63) << 2, static_cast<int>(row_base)));
-            const int ex = halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1);
-            const bool elava = t < 2 * H::R && ebase != kNoCell && ex >= 0 && T.S.edge[t];
-            if (elava) {
-                const uint32_t g = ebase + static_cast<uint32_t>(ex);
-                efl = D.flow[g];
-                if (!P.classic) {
-                    etp = tcur[g];
-                    eds = D.descent[g];
-                }
-            }
 #pragma unroll
             for (int k = 0; k < H::KR; ++k) {
                 if ((lava >> k) & 1u) {
@@ -1480,8 +1426,17 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                 T.temp[q] = etp;
                 T.desc[q] = static_cast<int8_t>(eds);
             }
-            __syncthreads();  // the lava data of the halo is in LDS
+            if (t < H::R) {
+                const uint64_t m = T.S.lava[t];
+                const uint64_t lo = (m << 1) | static_cast<uint64_t>(T.S.edge[2 * t]);
+                const uint32_t hi = static_cast<uint32_t>(m >> 63) | (static_cast<uint32_t>(T.S.edge[2 * t + 1]) << 1);
+                T.row66[t][0] = static_cast<uint32_t>(lo);
+                T.row66[t][1] = static_cast<uint32_t>(lo >> 32);
+                T.row66[t][2] = hi;
+                T.row66[t][3] = 0;
+            }
         }
+        __syncthreads();  // list and lava data of the tile are in LDS
         // candidate slots: LAVA cells at [0, nlava), AIR cells from the next wave boundary on (list order reversed)
         const uint32_t nlava = s_nlist, abase = (nlava + 63u) & ~63u, nslots = abase + s_nair;
         // Lava front: the next step's tiles next to the lava the tile holds at the step's start (and the tile itself)
@@ -1559,7 +1514,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                     int32_t wf = 0;
                     float wt = 0.0f;
                     bool has = false;
-                    const int spill_c = has_spill ? o.z0 + T.spill[lx + kTileX * ly] : 0;
+                    const int spill_c = has_spill ? T.spill[lx + kTileX * ly] : 0;
                     if (lava_bits) {
                         // rule 1 (LavaCA.hpp:1115-1128) / classic (LavaCA.hpp:1175-1190)
                         const uint64_t h_index = splitmix64(prefix ^ i);
@@ -1947,17 +1902,30 @@ __global__ void __launch_bounds__(kBlock) ca_ff_estimate_kernel(BatchDev B, uint
         if ((ent >> kTileBits) != slot) continue;
         const TileCoord o = tile_origin(P, ent & kTileMask);
         const int x = o.x0 + static_cast<int>(lane);
-        for (int k = 0; k < kTileCells / kBlock; ++k) {
+        // the four cells' type, flowId and temperature read together, then the vent test
+        constexpr int K = kTileCells / kBlock;
+        uint32_t ci[K];
+        uint32_t ty[K];
+        int32_t fl[K];
+        float tp[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
             const int row = w + 4 * k;
             const int y = o.y0 + row % P.ty, z = o.z0 + row / P.ty;
-            if (x >= P.nx || y >= P.ny || z >= P.nz) continue;
-            const uint32_t i = static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y) +
-                               plane * static_cast<uint32_t>(z);
-            if (cur[i] != kLava) continue;
-            const int32_t f = D.flow[i];
-            const bool vent = t0 < P.eruption_steps && f > 0 && f < P.nvent_slots && D.vent_cell[f] == i;
+            const bool in = x < P.nx && y < P.ny && z < P.nz;
+            ci[k] = static_cast<uint32_t>(x) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(y) +
+                    plane * static_cast<uint32_t>(z);
+            ty[k] = in ? cur[ci[k]] : kVoidLds;
+            fl[k] = in ? D.flow[ci[k]] : 0;
+            tp[k] = in ? tcur[ci[k]] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (ty[k] != kLava) continue;
+            const int32_t f = fl[k];
+            const bool vent = t0 < P.eruption_steps && f > 0 && f < P.nvent_slots && D.vent_cell[f] == ci[k];
             const float est = vent ? static_cast<float>(P.eruption_steps - t0) + (P.erupt_temp - P.solidus) * rc
-                                   : (tcur[i] - P.solidus) * rc;
+                                   : (tp[k] - P.solidus) * rc;
             best = rc > 0.0f ? fminf(best, est) : best;
         }
     }

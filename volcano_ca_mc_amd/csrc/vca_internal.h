@@ -185,10 +185,12 @@ constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
 // Tile lists come in two copies: a step reads the copy of its parity and the rules kernel pushes the next step's
 // tiles into the other one (lava front); the first kernel of a step clears the counts of the copy the previous step
-// read.  Each list (parity, shape) is kSub sub-lists: a tile goes to the sub-list its index within its simulation
-// selects (sub_of in vca_device.hip), and each sub-list's count sits on a 64-byte line of its own -- device atomics on
-// one 64-byte line serialise (about 12 ns each on MI355X), and thousands of tiles append per step.
-constexpr int kSub = 32;
+// read.  Each list (parity, shape) can be split into kSub sub-lists: a tile goes to the sub-list its index within its
+// simulation selects (sub_of in vca_device.hip), each sub-list's count on a 64-byte line of its own -- device atomics
+// on one 64-byte line serialise (about 12 ns each on MI355X, measured), and thousands of tiles append per step.
+// Measured with 32 sub-lists: no gain on the 1024 x 1024 x 384 caldera, 3 % slower on the 512 x 512 x 256 cone (the
+// producers' grouping by sub-list costs more scalar issue than the serialised appends), so one list is used.
+constexpr int kSub = 1;
 struct BatchCtrl {
     uint32_t nactive[2][kShapes][kSub][16];  // staged tiles per step parity, shape and sub-list (count at [0])
 };
