@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "vca_internal.h"
 
@@ -94,6 +95,17 @@ __device__ inline uint32_t opaque_zero() {
     uint32_t z;
     asm volatile("v_mov_b32 %0, 0" : "=v"(z));
     return z;
+}
+
+// p (a global-memory pointer), held in scalar registers from here on: the compiler may not sink the load of a
+// pointer into each of a row loop's branches (a scalar load and its wait per row).  An empty asm block; the result
+// is typed as global memory again, so that its loads stay global (not flat) loads.
+template <class T>
+using global_ptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ inline global_ptr<T> pinned(T* p) {
+    asm volatile("" : "+s"(p));
+    return (global_ptr<T>)p;
 }
 
 // v of lane `lane` (uniform) into scalar registers.
@@ -722,34 +734,29 @@ struct HaloTypes {
     uint8_t edge[2 * Halo<TY, TZ>::R];
 };
 
-// Stages the types of the tile at o and its halo, row by row, with the row lava masks, and (with_spill)
-// the spill heights of the LX x LY halo columns, or with OWN of the tile's own 64 x TY columns only (index
-// lx + 64 * ly; 0 where a column holds no cells).  Every load of a thread
-// is issued before its first LDS store, so a tile costs one memory latency.  Returns whether this thread saw
-// lava; own_air |= this thread saw AIR among the tile's own cells.
-// Halo row bases of the tile at o, one per lane: lane j < R holds halo_row(j).  Rows are then read with
-// v_readlane (scalar row index) instead of each wave redoing the row arithmetic in scalar code.
+// Halo rows of the tile at o, one per lane: lane j < R holds the global index of x = 0 in halo row j, or 0 when the
+// row holds no cells (so that every address formed from a row base lies inside the grid); bit j of `ok` is set when
+// it holds cells.  Rows are then read with v_readlane (scalar row index) instead of each wave redoing the row
+// arithmetic in scalar code.
+struct HaloRows {
+    uint32_t base;
+    uint64_t ok;
+};
 template <int TY, int TZ>
-__device__ inline uint32_t halo_row_lanes(const CaParams& P, const TileCoord& o) {
+__device__ inline HaloRows halo_rows(const CaParams& P, const TileCoord& o) {
     const int lane = static_cast<int>(opaque_lane());
-    return lane < Halo<TY, TZ>::R ? halo_row<TY, TZ>(P, o, lane) : kNoCell;
+    const uint32_t b = lane < Halo<TY, TZ>::R ? halo_row<TY, TZ>(P, o, lane) : kNoCell;
+    return {b != kNoCell ? b : 0u, static_cast<uint64_t>(__ballot(b != kNoCell))};
 }
 __device__ inline uint32_t row_of(uint32_t row_base, int r) {
     return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(row_base), r));
 }
-
-// Bit r set: halo row r (ly = r % LY, lz = r / LY) is one of the tile's own TY x TZ rows.
-template <int TY, int TZ>
-constexpr uint64_t own_row_mask() {
-    using H = Halo<TY, TZ>;
-    static_assert(H::R <= 64, "one mask bit per halo row");
-    uint64_t m = 0;
-    for (int r = 0; r < H::R; ++r) {
-        const int ly = r % H::LY, lz = r / H::LY;
-        if (ly >= 1 && ly <= TY && lz >= 1 && lz <= TZ) m |= 1ull << r;
-    }
-    return m;
-}
+// Lane `lane` of a wave-uniform 64-bit value (ballot result) as this lane's bool: the mask becomes the exec mask
+// of a branch or the lane mask of a select directly.
+__device__ inline bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+// v with the wave-uniform `bits` in lane k (k a compile-time constant in the callers' unrolled loops; lane = this
+// thread's lane): one compare and one select.
+__device__ inline uint32_t put_lane(uint32_t v, uint32_t bits, int k, int lane) { return lane == k ? bits : v; }
 
 // What a step writes into the other type copy for the own cells of a tile, as two bit planes of one 64-bit mask per
 // tile row (row = ly + TY * lz, bit lx) in 32-bit words w[plane][2 * row + (lx >> 5)]: plane 0 = the cell is
@@ -759,14 +766,23 @@ struct NextTypes {
     uint32_t w[2][2 * TY * TZ];
 };
 
+// Stages the types of the tile at o and its halo, row by row, with the row lava masks, and (with_spill)
+// the spill heights of the LX x LY halo columns, or with OWN of the tile's own 64 x TY columns only (index
+// lx + 64 * ly; 0 where a column holds no cells).  Every load of a thread is issued before its first LDS store, so a
+// tile costs one memory latency.  Returns whether this thread saw lava; own_air |= this thread saw AIR among the
+// tile's own cells; lane k of lava_lo / lava_hi gets the low / high word of the lava mask of the wave's row k (halo
+// row w + 4k of wave w).
+// The wave's row tests are three scalar masks built once by its lanes (lane k: row k), and every row is loaded
+// without a guard from a valid address (a row without cells reads row base 0; a position without a cell is voided by
+// a select), so no row costs exec-mask changes.
 // With SYNC, N gets the previous step's changes that the other type copy `other` does not hold yet: when `changed`
 // (the tile changed a cell's type in the previous step) the tile's own cells are read from that copy as well, else
 // the copies agree on them.
 template <int TY, int TZ, bool OWN = false, bool SYNC = false>
 __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const uint8_t* type, const uint8_t* other,
                                   bool changed, NextTypes<TY, TZ>* N, const TileCoord& o, HaloTypes<TY, TZ>& S,
-                                  bool with_spill, int32_t* s_spill, int& own_air, uint32_t row_base, int gx,
-                                  unsigned tid) {
+                                  bool with_spill, int32_t* s_spill, int& own_air, const HaloRows& rows, int gx,
+                                  unsigned tid, uint32_t& lava_lo, uint32_t& lava_hi) {
     using H = Halo<TY, TZ>;
     constexpr int NS = OWN ? kTileX * TY : H::LXY;     // spill columns staged
     constexpr int KS = (NS + kBlock - 1) / kBlock;     // per thread
@@ -774,27 +790,35 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const uint8
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     __builtin_assume(w >= 0 && w < kBlock / 64);
     const int lx = t & 63;
-    // row tests from one scalar mask (bit 4k: this wave's row k is an own row)
-    const uint64_t wown = own_row_mask<TY, TZ>() >> w;
+    // lane k < KR describes the wave's row k: whether it is a halo row, holds cells, is one of the tile's own rows,
+    // and (SYNC, after a step that changed a type) is an own row inside the grid in y and z; its own-row index
+    const int rk = w + 4 * lx;
+    const bool lrow = lx < H::KR && rk < H::R;
+    const int oly = rk % H::LY - 1, olz = rk / H::LY - 1;
+    const bool lown = lrow && oly >= 0 && oly < TY && olz >= 0 && olz < TZ;
+    const uint32_t okk = static_cast<uint32_t>(__ballot(lrow && ((rows.ok >> (rk & 63)) & 1u)));
+    const uint32_t ownk = static_cast<uint32_t>(__ballot(lown));
+    const uint32_t insk =
+        (SYNC && changed) ? static_cast<uint32_t>(__ballot(lown && o.y0 + oly < P.ny && o.z0 + olz < P.nz)) : 0u;
+    const bool xin = gx >= 0;
+    const uint32_t gxc = xin ? static_cast<uint32_t>(gx) : 0u;
     uint32_t v[H::KR];  // 32-bit so that no load result is packed (and waited for) before the stores
     uint32_t ov[H::KR];
 #pragma unroll
     for (int k = 0; k < H::KR; ++k) {
-        const int r = w + 4 * k;
         v[k] = kVoidLds;
         ov[k] = kVoidLds;
-        if (r < H::R) {
-            const uint32_t base = row_of(row_base, r);
-            if (base != kNoCell && gx >= 0) v[k] = type[base + static_cast<uint32_t>(gx)];
-        }
+        // lanes R .. 63 of rows.base hold 0 as well: a row index past R still reads a valid address
+        if (4 * k + 3 < H::R || w + 4 * k < H::R) v[k] = type[row_of(rows.base, w + 4 * k) + gxc];
     }
     uint32_t e = kVoidLds;
     {
         // thread t < 2R: edge column (t & 1) of row t >> 1 (rows 0..31 in wave 0, 32..35 in wave 1)
-        const uint32_t base = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(((t >> 1) & 63) << 2, static_cast<int>(row_base)));
+        const int er = (t >> 1) & 63;
+        const uint32_t base = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(er << 2, static_cast<int>(rows.base)));
         if (t < 2 * H::R) {
             const int ex = halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1);
-            if (base != kNoCell && ex >= 0) e = type[base + static_cast<uint32_t>(ex)];
+            if (((rows.ok >> er) & 1u) && ex >= 0) e = type[base + static_cast<uint32_t>(ex)];
         }
     }
     int32_t sp[KS];
@@ -817,48 +841,52 @@ __device__ inline bool stage_rows(const CaParams& P, const CaDev& D, const uint8
             }
         }
     }
-    if (SYNC && changed) {
+    if (SYNC && insk) {
 #pragma unroll
-        for (int k = 0; k < H::KR; ++k) {
-            const int r = w + 4 * k;
-            if (r < H::R && ((wown >> (4 * k)) & 1u)) {
-                const uint32_t base = row_of(row_base, r);
-                if (base != kNoCell && gx >= 0) ov[k] = other[base + static_cast<uint32_t>(gx)];
-            }
-        }
+        for (int k = 0; k < H::KR; ++k)
+            if ((insk >> k) & 1u) ov[k] = other[row_of(rows.base, w + 4 * k) + gxc];
     }
-    // lava rows OR-ed as masks
-    const bool in_x = o.x0 + lx < P.nx;
-    uint64_t any_lava = 0;
+    // lava rows OR-ed as masks; each row's masks go into lane k of the wave's mask registers
+    const uint64_t xm = __ballot(xin);
+    const uint64_t inx = __ballot(o.x0 + lx < P.nx);
+    uint64_t any_lava = 0, air = 0;
+    uint32_t lv_lo = 0, lv_hi = 0, n0_lo = 0, n0_hi = 0, n1_lo = 0, n1_hi = 0;
 #pragma unroll
     for (int k = 0; k < H::KR; ++k) {
         const int r = w + 4 * k;
-        if (r < H::R) {
-            S.type[lx + 1 + H::LX * r] = static_cast<uint8_t>(v[k]);
-            const uint64_t m = __ballot(v[k] == kLava);
-            if (lx == 0) S.lava[r] = m;
+        if (4 * k + 3 < H::R || r < H::R) {
+            const uint32_t val = lane_bit(((okk >> k) & 1u) ? xm : 0ull) ? v[k] : kVoidLds;
+            S.type[lx + 1 + H::LX * r] = static_cast<uint8_t>(val);
+            const uint64_t m = __ballot(val == kLava);
+            lv_lo = put_lane(lv_lo, static_cast<uint32_t>(m), k, lx);
+            lv_hi = put_lane(lv_hi, static_cast<uint32_t>(m >> 32), k, lx);
             any_lava |= m;
-            if (((wown >> (4 * k)) & 1u) && in_x && v[k] == kAir) own_air = 1;
-            if (SYNC && ((wown >> (4 * k)) & 1u)) {
-                // own row: ly = r % LY - 1, lz = r / LY - 1; rows past the grid's far side hold wrapped halo cells
-                const int ly = r % H::LY - 1, lz = r / H::LY - 1;
-                const bool inside = in_x && o.y0 + ly < P.ny && o.z0 + lz < P.nz;
-                uint64_t m0 = 0, m1 = 0;
-                if (changed) {
-                    const bool differ = inside && v[k] != ov[k];  // born (LAVA) or solidified (BASALT) in step t - 1
-                    m0 = __ballot(differ);
-                    m1 = __ballot(differ && v[k] == kBasalt);
-                }
-                if (lx == 0) {
-                    const int row = ly + TY * lz;
-                    N->w[0][2 * row] = static_cast<uint32_t>(m0);
-                    N->w[0][2 * row + 1] = static_cast<uint32_t>(m0 >> 32);
-                    N->w[1][2 * row] = static_cast<uint32_t>(m1);
-                    N->w[1][2 * row + 1] = static_cast<uint32_t>(m1 >> 32);
+            if ((ownk >> k) & 1u) {
+                air |= __ballot(val == kAir) & inx;
+                if (SYNC && ((insk >> k) & 1u)) {
+                    // own row inside the grid: the cells born (LAVA) or solidified (BASALT) in step t - 1; rows past
+                    // the grid's far side hold wrapped halo cells
+                    const uint64_t d = __ballot(val != ov[k]) & inx;
+                    const uint64_t b = d & __ballot(val == kBasalt);
+                    n0_lo = put_lane(n0_lo, static_cast<uint32_t>(d), k, lx);
+                    n0_hi = put_lane(n0_hi, static_cast<uint32_t>(d >> 32), k, lx);
+                    n1_lo = put_lane(n1_lo, static_cast<uint32_t>(b), k, lx);
+                    n1_hi = put_lane(n1_hi, static_cast<uint32_t>(b >> 32), k, lx);
                 }
             }
         }
     }
+    if (lrow) S.lava[rk] = (static_cast<uint64_t>(lv_hi) << 32) | lv_lo;
+    if (SYNC && lown) {
+        const int row = oly + TY * olz;
+        N->w[0][2 * row] = n0_lo;
+        N->w[0][2 * row + 1] = n0_hi;
+        N->w[1][2 * row] = n1_lo;
+        N->w[1][2 * row + 1] = n1_hi;
+    }
+    if (air) own_air = 1;
+    lava_lo = lv_lo;
+    lava_hi = lv_hi;
     bool lava = any_lava != 0;
     if (t < 2 * H::R) {
         S.type[((t & 1) ? H::LX - 1 : 0) + H::LX * (t >> 1)] = static_cast<uint8_t>(e);
@@ -961,11 +989,12 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
             if (!__syncthreads_or(go && dirty)) continue;
         }
         int own_air = 0;
-        const uint32_t row_base = halo_row_lanes<TY, TZ>(P, o);
+        const HaloRows rows = halo_rows<TY, TZ>(P, o);
         const int gx = halo_x(P, o.x0 + static_cast<int>(threadIdx.x & 63u));
         const uint8_t* type = D.type[now & 1u];
+        uint32_t lava_lo, lava_hi;
         stage_rows<TY, TZ>(P, D, type, nullptr, false, static_cast<NextTypes<TY, TZ>*>(nullptr), o, S, has_spill,
-                           s_spill, own_air, row_base, gx, opaque_tid());
+                           s_spill, own_air, rows, gx, opaque_tid(), lava_lo, lava_hi);
         if (threadIdx.x == 0) s_n = 0;
         __syncthreads();
         {
@@ -1281,12 +1310,13 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         int own_air = 0;
         // lane j: base of halo row j; gx: this lane's column (x == nx is the wrapped x = 0 of a periodic partial
         // tile, -1 for no cell)
-        const uint32_t row_base = halo_row_lanes<TY, TZ>(P, o);
+        const HaloRows rows = halo_rows<TY, TZ>(P, o);
         const int gx = halo_x(P, o.x0 + static_cast<int>(tid & 63u));
         // the tile changed a cell's type in the previous step: the other type copy is behind on it
         const bool changed = t > 0 && __builtin_amdgcn_readfirstlane(D.tile_tchg[tile]) == static_cast<uint32_t>(t);
+        uint32_t lava_lo, lava_hi;  // lane k: lava mask of the wave's halo row k
         const bool saw_lava = stage_rows<TY, TZ, true, true>(P, D, tyc, tyn, changed, &T.next, o, T.S, has_spill,
-                                                             T.spill, own_air, row_base, gx, tid);
+                                                             T.spill, own_air, rows, gx, tid, lava_lo, lava_hi);
         if (__builtin_amdgcn_readfirstlane(stop)) continue;
         if (tid == 0) {
             const uint32_t zero = opaque_zero();
@@ -1314,39 +1344,50 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         }
         int tile_lava = 0, tile_air = 0;
         {
-            // flowId / temperature / D(n) of the LAVA positions, all loads in flight before the stores
+            // flowId / temperature / D(n) of the LAVA positions, all loads in flight before the stores; each row's
+            // lava mask (from the staging, in lane k) is the exec mask of its loads, and every row base is a valid
+            // index, so no mask bit can address memory outside the grid
             const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
             const int lx = static_cast<int>(tid & 63u);
             const int t = static_cast<int>(tid);
+            const uint32_t gxc = gx >= 0 ? static_cast<uint32_t>(gx) : 0u;
             int32_t fl[H::KR];
             float tp[H::KR];
             int32_t ds[H::KR];  // 32-bit: see stage_rows
-            uint32_t lava = 0;
+            const global_ptr<const int32_t> flow_g = pinned(static_cast<const int32_t*>(D.flow));
+            const global_ptr<const float> temp_g = pinned(tcur);
+            const global_ptr<const int8_t> desc_g = pinned(static_cast<const int8_t*>(D.descent));
+            auto load_rows = [&](auto lava_mode) {
 #pragma unroll
-            for (int k = 0; k < H::KR; ++k) {
-                const int r = w + 4 * k;
-                fl[k] = 0;
-                tp[k] = 0.0f;
-                ds[k] = -1;
-                // the staging wrote LAVA only where a cell exists; the load is guarded as well, so that no mask bit
-                // can address memory outside the grid (kNoCell row, gx == -1)
-                const uint32_t base = r < H::R ? row_of(row_base, r) : kNoCell;
-                if (base != kNoCell && gx >= 0 && ((T.S.lava[r] >> lx) & 1u)) {
-                    lava |= 1u << k;
-                    const uint32_t g = base + static_cast<uint32_t>(gx);
-                    fl[k] = D.flow[g];
-                    if (!P.classic) {
-                        tp[k] = tcur[g];
-                        ds[k] = D.descent[g];
+                for (int k = 0; k < H::KR; ++k) {
+                    fl[k] = 0;
+                    tp[k] = 0.0f;
+                    ds[k] = -1;
+                    if (4 * k + 3 < H::R || w + 4 * k < H::R) {
+                        const uint64_t m = static_cast<uint64_t>(row_of(lava_lo, k)) |
+                                           (static_cast<uint64_t>(row_of(lava_hi, k)) << 32);
+                        if (lane_bit(m)) {
+                            const uint32_t g = row_of(rows.base, w + 4 * k) + gxc;
+                            fl[k] = flow_g[g];
+                            if (lava_mode) {
+                                tp[k] = temp_g[g];
+                                ds[k] = desc_g[g];
+                            }
+                        }
                     }
                 }
-            }
+            };
+            if (P.classic)
+                load_rows(std::false_type{});
+            else
+                load_rows(std::true_type{});
             int32_t efl = 0;
             float etp = 0.0f;
             int32_t eds = -1;
-            const uint32_t ebase = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(((t >> 1) & 63) << 2, static_cast<int>(row_base)));
+            const int er = (t >> 1) & 63;
+            const uint32_t ebase = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(er << 2, static_cast<int>(rows.base)));
             const int ex = halo_x(P, (t & 1) ? o.x0 + kTileX : o.x0 - 1);
-            const bool elava = t < 2 * H::R && ebase != kNoCell && ex >= 0 && T.S.edge[t];
+            const bool elava = t < 2 * H::R && ((rows.ok >> er) & 1u) && ex >= 0 && T.S.edge[t];
             if (elava) {
                 const uint32_t g = ebase + static_cast<uint32_t>(ex);
                 efl = D.flow[g];
@@ -1387,14 +1428,18 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                 uint32_t bnd0 = 0;  // directions in which the tile holds LAVA on its boundary at the step's start
                 bool cl[4], ca[4];
                 uint32_t sl[4], sa[4];
+                // positions past the grid's far side hold wrapped halo cells of a periodic partial tile: bit k of cin
+                // (from lane k) = the wave's own row k lies inside the grid in y and z
+                const uint64_t inx = __ballot(o.x0 + lx < P.nx);
+                const int crow = w + 4 * lx;
+                const uint32_t cin =
+                    static_cast<uint32_t>(__ballot(lx < 4 && o.y0 + crow % TY < P.ny && o.z0 + crow / TY < P.nz));
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const int row = w + 4 * k;
                     const int rc = (row % TY + 1) + LY * (row / TY + 1);
                     const uint64_t near = readlane64(near_j, row);
-                    // positions past the grid's far side hold wrapped halo cells of a periodic partial tile
-                    const bool inside = o.x0 + lx < P.nx && o.y0 + row % TY < P.ny && o.z0 + row / TY < P.nz;
-                    const uint8_t ty = inside ? T.S.type[(lx + 1) + LX * rc] : kVoidLds;
+                    const uint8_t ty = lane_bit(((cin >> k) & 1u) ? inx : 0ull) ? T.S.type[(lx + 1) + LX * rc] : kVoidLds;
                     cl[k] = ty == kLava;
                     ca[k] = ty == kAir && ((near >> lx) & 1u);
                     if (ty == kAir && !ca[k]) tile_air = 1;
@@ -1411,9 +1456,10 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                     if (ca[k]) T.list[kTileCells - 1 - sa[k]] = cell;
                 }
             }
+            // every position of the rows: the rules read these arrays at LAVA positions only
 #pragma unroll
             for (int k = 0; k < H::KR; ++k) {
-                if ((lava >> k) & 1u) {
+                if (4 * k + 3 < H::R || w + 4 * k < H::R) {
                     const int q = lx + 1 + H::LX * (w + 4 * k);
                     T.flow[q] = static_cast<F>(fl[k]);
                     T.temp[q] = tp[k];
