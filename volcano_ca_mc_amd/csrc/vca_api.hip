@@ -299,7 +299,7 @@ struct vca_ca {
     int shape = 0;
     DevBuf arena;  // every region below: one allocation and one free per simulation
     Span type, type2, flow, temp, temp2, solid, arrival, descent, spill, vent_cell, tile_flags, tile_near, tile_halo,
-        tile_tchg, listed, part, ctrl, hist_a, hist_c;
+        tile_tchg, listed, dirtied, part, ctrl, hist_a, hist_c;
     uint64_t n = 0;
     std::unique_ptr<Batch> own;   // batch of one: the solo path (made on the first solo enqueue)
     bool flow16 = true;           // every flowId the simulation can hold (its vents') fits int16_t
@@ -409,7 +409,7 @@ struct Batch {
     vca_ctx* ctx = nullptr;
     int max_slots = 0;
     uint32_t cap = 0;                             // tiles
-    DevBuf owner, tile_base, active, ctrl, status;
+    DevBuf owner, tile_base, active, desc, ctrl, status;
     uint32_t sim_base = 0;
     bool has_table = false;
     std::vector<CaSim> host_sims;                 // what the table entries hold (the copies read from here)
@@ -427,6 +427,10 @@ struct Batch {
     SlotMask purge{};             // slots whose simulation left while the pending list copy may hold its entries
     bool purge_any = false;
     bool external = false;        // lists and counts in caller-provided device memory (vca_ca_run_noalloc)
+    // the descent lists of the next step are complete: every member's last step ran its rules kernel in this batch
+    // (not after a member joined or was fast-forwarded)
+    bool desc_ok = false;
+    bool desc_ready() const { return desc_ok || B.desc_list == nullptr; }
 
     void init(vca_ctx* c, int slots, uint32_t tiles) {
         ctx = c;
@@ -442,6 +446,9 @@ struct Batch {
         tile_base.alloc(static_cast<size_t>(slots) * 4);
         const uint32_t subcap = sub_capacity(cap, slots);
         active.alloc(static_cast<size_t>(subcap) * kSub * kShapes * 2 * 8);
+        // descent lists and, after them, their counts
+        const size_t desc_bytes = (static_cast<size_t>(subcap) * kShapes * 2 * 8 + 255) & ~static_cast<size_t>(255);
+        desc.alloc(desc_bytes + 2 * kShapes * 16 * 4);
         ctrl.alloc(sizeof(BatchCtrl));
         check(hipMemsetAsync(owner.p, 0xFF, owner.bytes, s), "memset");  // kNoSlot
         check(hipMemsetAsync(ctrl.p, 0, sizeof(BatchCtrl), s), "memset");
@@ -457,7 +464,11 @@ struct Batch {
         B.nslots = 0;
         B.subcap = subcap;
         B.active = active.as<uint64_t>();
+        B.desc_list = desc.as<uint64_t>();
+        B.desc_count = reinterpret_cast<uint32_t*>(static_cast<char*>(desc.p) + desc_bytes);
+        check(hipMemsetAsync(B.desc_count, 0, 2 * kShapes * 16 * 4, s), "memset");
         B.ctrl = ctrl.as<BatchCtrl>();
+        desc_ok = false;
         grid_tiles = tile_grid(c, cap);
     }
     // A batch of one slot over device memory the library does not allocate: counts, tile owners and the two list copies
@@ -497,6 +508,8 @@ struct Batch {
         B.nslots = 1;
         B.subcap = subcap;
         B.active = reinterpret_cast<uint64_t*>(base + o_active);
+        B.desc_list = nullptr;  // no descent lists: the descent checks every listed tile
+        B.desc_count = nullptr;
         B.ctrl = reinterpret_cast<BatchCtrl*>(base + o_ctrl);
         grid_tiles = 0;  // set by reset_external (the context's device)
         (void)device;
@@ -627,6 +640,7 @@ struct Batch {
         // a lava-front simulation's first tiles (later steps list their own)
         if (sim.push) ca_launch_seed(B, static_cast<uint32_t>(slot), ca->P.ntiles, par, s);
         check(hipGetLastError(), "batch add");
+        desc_ok = false;  // the new member's tiles are in no descent list
         members[static_cast<size_t>(slot)] = ca;
         bases.resize(static_cast<size_t>(max_slots));
         bases[static_cast<size_t>(slot)] = base;
@@ -664,6 +678,7 @@ struct Batch {
         L.scan = any_scan();
         L.par = par;
         L.fin_prev = fin_prev;
+        L.desc_list = desc_ok && B.desc_list != nullptr;
         uint64_t tiles[kShapes] = {};
         for (const vca_ca* c : members)
             if (c) tiles[c->shape] += c->P.ntiles;
@@ -678,6 +693,7 @@ struct Batch {
         const StepLaunch L = step_config();
         ca_launch_step(B, L, s, ev, rec);
         par ^= 1;
+        desc_ok = true;  // this step's rules kernel listed the next step's descent tiles
         fin_prev = !L.scan;  // with the scan, ca_finish ran after the step
     }
     // The pending finish of the last step, if any (every enqueue ends with it: the host reads the stop flags next).
@@ -728,19 +744,23 @@ struct Batch {
         if (s == nullptr) return false;  // the legacy stream cannot be captured
         hipGraph_t g = nullptr;
         const int par0 = par;
-        const bool fin0 = fin_prev;
+        const bool fin0 = fin_prev, desc0 = desc_ok;
+        // a graph starts on parity 0, after a step that filled the descent lists (see enqueue)
         par = 0;
         fin_prev = false;
+        desc_ok = true;
         if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) {
             (void)hipGetLastError();
             par = par0;
             fin_prev = fin0;
+            desc_ok = desc0;
             return false;
         }
         for (int k = 0; k < steps_per_graph; ++k) launch_step(s, nullptr, nullptr);
         close_steps(s, nullptr);
         par = par0;
         fin_prev = fin0;
+        desc_ok = desc0;
         // a capture invalidated by the runtime leaves nothing enqueued: the caller launches the steps directly
         if (hipStreamEndCapture(s, &g) != hipSuccess || g == nullptr) {
             (void)hipGetLastError();
@@ -764,14 +784,15 @@ struct Batch {
         if (steps_per_graph > 0 && !ctx->timing) {
             if (!graph && !capture()) ++capture_failures;
             if (capture_failures >= 3) steps_per_graph = 0;  // this process cannot capture: direct launches
-            // a graph starts on parity 0
-            if (graph && steps >= graph_steps + 1 && par != 0) {
+            // a graph starts on parity 0, after a step whose rules kernel filled the descent lists
+            while (graph && steps >= graph_steps + 1 && (par != 0 || !desc_ready())) {
                 launch_direct(1);
                 --steps;
             }
-            while (graph && par == 0 && steps >= graph_steps) {
+            while (graph && par == 0 && desc_ready() && steps >= graph_steps) {
                 check(hipGraphLaunch(graph, ctx->stream), "hipGraphLaunch");
                 steps -= graph_steps;
+                desc_ok = true;
             }
         }
         launch_direct(steps);
@@ -910,7 +931,7 @@ struct CaPlan {
     Layout L;
     size_t hcap = 1;
     size_t type = 0, type2 = 0, flow = 0, temp = 0, temp2 = 0, solid = 0, arrival = 0, ctrl = 0, tile_near = 0,
-           tile_halo = 0, tile_tchg = 0, listed = 0, part = 0, hist_a = 0, hist_c = 0;
+           tile_halo = 0, tile_tchg = 0, listed = 0, dirtied = 0, part = 0, hist_a = 0, hist_c = 0;
     size_t zero_bytes = 0;
     size_t descent = 0, vent_cell = 0, tile_flags = 0, spill = 0;
     size_t staging = 0, st_height = 0, st_cells = 0, st_flows = 0, st_temps = 0, st_bytes = 0;
@@ -932,6 +953,7 @@ CaPlan ca_plan(uint64_t n, uint32_t ntiles, size_t cols, bool lava, size_t vent_
     c.tile_halo = L.add(static_cast<size_t>(ntiles) * kHaloBytes);
     c.tile_tchg = L.add(static_cast<size_t>(ntiles) * 4);
     c.listed = L.add(static_cast<size_t>(ntiles) * 4);
+    c.dirtied = L.add(static_cast<size_t>(ntiles) * 4);
     c.part = L.add(static_cast<size_t>(kPartRows) * sizeof(CaPart));
     c.hist_a = L.add(c.hcap * 8);
     c.hist_c = L.add(c.hcap * 8);
@@ -1104,6 +1126,7 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         region(ca->tile_halo, plan.tile_halo, static_cast<size_t>(P.ntiles) * kHaloBytes);
         region(ca->tile_tchg, plan.tile_tchg, static_cast<size_t>(P.ntiles) * 4);
         region(ca->listed, plan.listed, static_cast<size_t>(P.ntiles) * 4);
+        region(ca->dirtied, plan.dirtied, static_cast<size_t>(P.ntiles) * 4);
         region(ca->part, plan.part, static_cast<size_t>(kPartRows) * sizeof(CaPart));
         region(ca->hist_a, plan.hist_a, plan.hcap * 8);
         region(ca->hist_c, plan.hist_c, plan.hcap * 8);
@@ -1131,6 +1154,7 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         D.tile_halo = ca->tile_halo.as<uint8_t>();
         D.tile_tchg = ca->tile_tchg.as<uint32_t>();
         D.listed = ca->listed.as<uint32_t>();
+        D.dirtied = ca->dirtied.as<uint32_t>();
         D.part = ca->part.as<CaPart>();
         D.ctrl = ca->ctrl.as<CaCtrl>();
         D.hist_active = ca->hist_a.as<unsigned long long>();
@@ -1423,6 +1447,7 @@ static CaStatus ca_run_steps(vca_ca* ca, Batch& b, int64_t max_new_steps) {
     int chunk = 128;
     while (st.stop == VCA_RUNNING && st.steps < target) {
         if (ca_fast_forward(ca, st, target - st.steps, b.B, 0u, b.par)) {
+            b.desc_ok = false;  // the fast-forward changed types no descent list holds
             const int32_t from = st.steps;
             st = ca_status_read(ca);
             ca->ff_done(from, st.steps);
@@ -1564,7 +1589,7 @@ extern "C" int vca_ca_footprint(const vca_ca_desc* d, uint64_t* bytes) {
         // the simulation's allocation (vents: a few slots), and the batch-of-one lists a solo run adds
         const CaPlan plan = ca_plan(n, static_cast<uint32_t>(std::min<uint64_t>(tiles, kTileMask)), cols, d->classic == 0,
                                     64, 64, d->max_steps);
-        *bytes = plan.L.total + tiles * (2 * kShapes * 8 + 4) + 4096;
+        *bytes = plan.L.total + tiles * (4 * kShapes * 8 + 4) + 4096;
     });
 }
 
@@ -1755,6 +1780,9 @@ extern "C" int vca_batch_add(vca_batch* b, vca_ca* ca, int32_t* slot) {
         b->b.ctx->set_device();
         ca->slot = b->b.attach(ca);
         ca->member = &b->b;
+        // the private batch of earlier solo steps holds lists for a step the shared batch moves past: a later solo run
+        // makes a new one
+        ca->own.reset();
         if (slot) *slot = ca->slot;
     });
 }
@@ -1785,6 +1813,7 @@ extern "C" int vca_batch_status(vca_batch* b, int32_t* out, int32_t cap) {
             if (m && ca_fast_forward(m, st, m->P.max_steps, bb.B, k, bb.par)) {
                 from[k] = st.steps;
                 ff = true;
+                bb.desc_ok = false;  // the fast-forward changed types no descent list holds
             }
         }
         if (ff) {

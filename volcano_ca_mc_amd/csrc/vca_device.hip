@@ -134,6 +134,10 @@ __device__ inline uint32_t sub_of(uint32_t t) {
 __device__ inline uint32_t* list_count(const BatchDev& B, int par, int shape, int j) {
     return &B.ctrl->nactive[par][shape][j][0];
 }
+// Count of descent list (par, shape) (each on a 64-byte line of its own).
+__device__ inline uint32_t* desc_count(const BatchDev& B, int par, int shape) {
+    return B.desc_count + (par * kShapes + shape) * 16;
+}
 __device__ inline uint64_t* sub_list(const BatchDev& B, int par, int shape, int j) {
     return B.active + static_cast<size_t>((par * kShapes + shape) * kSub + j) * B.subcap;
 }
@@ -313,6 +317,37 @@ __device__ inline void push_next(const CaParams& P, const CaDev& D, const BatchD
     }
     wave_list_append(B, par, shape, add, n, (static_cast<uint64_t>(step) << 32) | (slot << kTileBits) | n);
 }
+// Tile layers below a tile that D(n) of its cells reads: drops reach kMaxDrop + 1 cells below its lowest layer.
+template <int TZ>
+constexpr int kDescentLayers = (kMaxDrop + 1 + TZ - 1) / TZ;
+
+// After a step in which the tile (tx, ty, tz) changed a cell's type: every tile whose D(n) reads it -- the 3 x 3
+// columns around it, from its layer up kDescentLayers tile layers -- that can hold lava after the step into the
+// descent list of the next step (list copy par, entries for `step`), each once (the `dirtied` stamp).  A tile holds
+// lava after the step only if it held lava before it or had a cell born in it (then it changed a type and lists
+// itself), so its lava flag, read before or after its own rules of this step wrote it, selects every such tile.
+// Thread q of the block takes tile q of that set (all threads call: one list atomic per wave).
+template <int TZ>
+__device__ inline void push_descent(const CaParams& P, const CaDev& D, const BatchDev& B, int tx, int ty, int tz, int q,
+                                    uint32_t slot, uint32_t step, int par, int shape) {
+    static_assert(9 * (kDescentLayers<TZ> + 1) <= kBlock, "one thread per tile of the set");
+    bool add = false;
+    uint32_t n = 0;
+    if (q < 9 * (kDescentLayers<TZ> + 1)) {
+        int qx = tx + q % 3 - 1, qy = ty + (q / 3) % 3 - 1;
+        const int qz = tz + q / 9;
+        if (qz < P.ntz && wrap_axis(qx, P.ntx, P.per[0]) && wrap_axis(qy, P.nty, P.per[1])) {
+            n = static_cast<uint32_t>(qx) +
+                static_cast<uint32_t>(P.ntx) * (static_cast<uint32_t>(qy) + static_cast<uint32_t>(P.nty) * static_cast<uint32_t>(qz));
+            add = (q == 4 || (D.tile_flags[n] & kFlagLava)) && atomicExch(&D.dirtied[n], step) != step;
+        }
+    }
+    const uint32_t at = wave_append(add, desc_count(B, par, shape));
+    if (add)
+        B.desc_list[static_cast<size_t>(par * kShapes + shape) * B.subcap + at] =
+            (static_cast<uint64_t>(step) << 32) | (slot << kTileBits) | n;
+}
+
 __device__ inline unsigned wave_or(unsigned v) {
     v |= dpp_row_shr<0x111>(v);
     v |= dpp_row_shr<0x112>(v);
@@ -930,7 +965,10 @@ template <int TY, int TZ>
 // The first kernel of a step: it clears the counts of the list copy the previous step read, does the previous step's
 // finish in its first `fin_blocks` blocks (one slot each; 0 when the finish is not pending) and computes D(n) of the
 // staged tiles of parity par that hold lava in the blocks after them, alongside the finish.
-__global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shape, int par, uint32_t fin_blocks) {
+// With desc, the tile blocks walk the step's descent list (the tiles whose D(n) may have changed: each only checks
+// that it holds lava), else every listed tile with the check of the tiles its D(n) reads.
+__global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shape, int par, uint32_t fin_blocks,
+                                                            int desc) {
     using H = Halo<TY, TZ>;
     __shared__ HaloTypes<TY, TZ> S;
     __shared__ int32_t s_spill[H::LXY];
@@ -941,16 +979,19 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
         // every kernel of the previous step is done with its list copy; the rules kernel of this step pushes the next
         // step's tiles there
         *list_count(B, par ^ 1, threadIdx.x / kSub, threadIdx.x % kSub) = 0u;
+        if (B.desc_list && threadIdx.x < kShapes) *desc_count(B, par ^ 1, static_cast<int>(threadIdx.x)) = 0u;
     }
     if (blockIdx.x < fin_blocks) {
         finish_slot(B, blockIdx.x);
         return;
     }
-    // tile block b walks sub-list b % kSub (the tile blocks are a multiple of kSub)
-    const uint32_t tb = blockIdx.x - fin_blocks, stride = (gridDim.x - fin_blocks) / kSub;
-    const uint32_t n = *list_count(B, par, shape, static_cast<int>(tb % kSub));
-    const uint64_t* tiles = sub_list(B, par, shape, static_cast<int>(tb % kSub));
-    for (uint32_t s = tb / kSub; s < n; s += stride) {
+    // tile block b walks sub-list b % kSub (the tile blocks are a multiple of kSub), or with desc the descent list
+    const uint32_t tb = blockIdx.x - fin_blocks;
+    const uint32_t stride = desc ? gridDim.x - fin_blocks : (gridDim.x - fin_blocks) / kSub;
+    const uint32_t n = desc ? *desc_count(B, par, shape) : *list_count(B, par, shape, static_cast<int>(tb % kSub));
+    const uint64_t* tiles = desc ? B.desc_list + static_cast<size_t>(par * kShapes + shape) * B.subcap
+                                 : sub_list(B, par, shape, static_cast<int>(tb % kSub));
+    for (uint32_t s = desc ? tb : tb / kSub; s < n; s += stride) {
         // uniform: one entry per block; low word slot << kTileBits | tile, high word the simulation's step
         const uint64_t ent = tiles[s];
         const uint32_t e = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ent));
@@ -970,14 +1011,13 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
             // step, the values written then still hold.  Only tiles holding lava have D(n) to compute, and entries of
             // a slot that left the batch or of a simulation that stopped are skipped.  (The barrier also orders the
             // previous tile's LDS reads: every tile passes it.)
-            constexpr int NB = (kMaxDrop + 1 + TZ - 1) / TZ;  // tile layers below
-            constexpr int NR = 9 * (NB + 1);
+            constexpr int NR = 9 * (kDescentLayers<TZ> + 1);
             const int tx = o.x0 / kTileX, ty = o.y0 / TY, tz = o.z0 / TZ;
             // the stop flag, the tile's flags and the marks of the tiles below are read together
             const bool go = sim.running && !P.classic && !__builtin_amdgcn_readfirstlane(D.ctrl->stop) &&
                             (__builtin_amdgcn_readfirstlane(D.tile_flags[tile]) & kFlagLava);
-            bool dirty = false;
-            for (int q = static_cast<int>(threadIdx.x); sim.running && !P.classic && q < NR; q += kBlock) {
+            bool dirty = desc != 0;  // a descent-list entry is one by construction
+            for (int q = static_cast<int>(threadIdx.x); !desc && sim.running && !P.classic && q < NR; q += kBlock) {
                 int qx = tx + q % 3 - 1, qy = ty + (q / 3) % 3 - 1;
                 const int qz = tz - q / 9;
                 if (qz >= 0 && wrap_axis(qx, P.ntx, P.per[0]) && wrap_axis(qy, P.nty, P.per[1]))
@@ -1647,6 +1687,10 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         // a tile holding lava after the step lists the next step's tiles (lava front: itself and the neighbours it
         // holds lava next to) or marks its neighbours for the next step's scan (any lava: visit; lava next to each
         // neighbour: stage); a tile that changed a cell's type is listed too (it brings the other copy up to date)
+        // a tile that changed a cell's type lists the tiles whose D(n) reads it for the next step's descent
+        if (B.desc_list && !P.classic && (s_wflags[1] & 0x04040404u))
+            push_descent<TZ>(P, D, B, o.x0 / kTileX, o.y0 / TY, o.z0 / TZ, static_cast<int>(tid), e >> kTileBits,
+                             static_cast<uint32_t>(t) + 1u, par ^ 1, shape);
         if ((s_wflags[1] & 0x01010101u) != 0) {
             const int tx = o.x0 / kTileX, tyy = o.y0 / TY, tz = o.z0 / TZ;
             const uint32_t bnd_all = s_red[2][0] | s_red[2][1] | s_red[2][2] | s_red[2][3];
@@ -2583,21 +2627,21 @@ struct StepEvents {
 
 template <int TY, int TZ>
 void launch_descent(int shape, dim3 g, hipStream_t s, hipEvent_t a, hipEvent_t b, const BatchDev& B, int par,
-                    uint32_t fin_blocks) {
-    launch(ca_descent_kernel<TY, TZ>, g, s, a, b, B, shape, par, fin_blocks);
+                    uint32_t fin_blocks, int desc) {
+    launch(ca_descent_kernel<TY, TZ>, g, s, a, b, B, shape, par, fin_blocks, desc);
 }
 template <int TY, int TZ, class F>
 void launch_rules(int shape, dim3 g, hipStream_t s, hipEvent_t a, hipEvent_t b, const BatchDev& B, int par) {
     launch(ca_rules_kernel<TY, TZ, F>, g, s, a, b, B, shape, par);
 }
 void descent_shape(int k, dim3 g, hipStream_t s, hipEvent_t a, hipEvent_t b, const BatchDev& B, int par,
-                   uint32_t fin_blocks) {
+                   uint32_t fin_blocks, int desc) {
     switch (k) {
-        case 0: launch_descent<4, 4>(0, g, s, a, b, B, par, fin_blocks); break;
-        case 1: launch_descent<2, 8>(1, g, s, a, b, B, par, fin_blocks); break;
-        case 2: launch_descent<8, 2>(2, g, s, a, b, B, par, fin_blocks); break;
-        case 3: launch_descent<1, 16>(3, g, s, a, b, B, par, fin_blocks); break;
-        default: launch_descent<16, 1>(4, g, s, a, b, B, par, fin_blocks); break;
+        case 0: launch_descent<4, 4>(0, g, s, a, b, B, par, fin_blocks, desc); break;
+        case 1: launch_descent<2, 8>(1, g, s, a, b, B, par, fin_blocks, desc); break;
+        case 2: launch_descent<8, 2>(2, g, s, a, b, B, par, fin_blocks, desc); break;
+        case 3: launch_descent<1, 16>(3, g, s, a, b, B, par, fin_blocks, desc); break;
+        default: launch_descent<16, 1>(4, g, s, a, b, B, par, fin_blocks, desc); break;
     }
 }
 template <class F>
@@ -2625,7 +2669,7 @@ void ca_launch_step(const BatchDev& B, const StepLaunch& L, hipStream_t s, hipEv
     for (int i = 0; i < ns; ++i) {
         const uint32_t fin_blocks = i == 0 && L.fin_prev ? B.nslots : 0u;
         descent_shape(shapes[i], dim3(static_cast<unsigned>(L.grid[shapes[i]]) + fin_blocks), s, E.start(1, i == 0),
-                      E.stop(1, i == ns - 1), B, L.par, fin_blocks);
+                      E.stop(1, i == ns - 1), B, L.par, fin_blocks, L.desc_list ? 1 : 0);
     }
     for (int i = 0; i < ns; ++i) {
         const dim3 gt(static_cast<unsigned>(L.grid[shapes[i]]));

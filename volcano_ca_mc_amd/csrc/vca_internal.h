@@ -152,6 +152,7 @@ struct CaDev {
     uint32_t* tile_tchg;          // per tile: 1 + the last step in which a cell of the tile changed type (0: never)
     uint32_t* listed;             // per tile: the step whose tile list holds the tile (lava front: the rules kernel
                                   // pushes the next step's tiles; the stamp lists each tile once)
+    uint32_t* dirtied;            // per tile: the step whose descent list holds the tile (stamp, as `listed`)
     CaPart* part;                 // kPartRows rows
     CaCtrl* ctrl;
     unsigned long long* hist_active;
@@ -208,6 +209,12 @@ struct BatchDev {
     uint32_t subcap;              // capacity of each sub-list: a simulation of n tiles holds at most ceil(n / kSub) of
                                   // a sub-list's entries, so ceil(tiles / kSub) + slots covers every batch
     uint64_t* active;             // 2 x kShapes x kSub sub-lists of subcap entries (parity-major): step << 32 | entry
+    // 2 x kShapes descent lists of subcap entries (parity-major, entries as in `active`): the tiles whose D(n) may have
+    // changed -- every tile whose 3 x 3 columns down to the layers its drops reach hold a tile that changed a cell's
+    // type in the previous step, pushed by that step's rules kernel; nullptr when the batch keeps none (the descent
+    // then checks every listed tile)
+    uint64_t* desc_list;
+    uint32_t* desc_count;         // [2][kShapes][16]: entries of descent list (parity, shape) at [par][shape][0]
     BatchCtrl* ctrl;
 };
 
@@ -266,6 +273,8 @@ struct StepLaunch {
                                   // first kernel of the next step
     int par;                      // parity of the step (which list copy it reads)
     bool fin_prev;                // the previous step's finish is still to be done (by this step's first kernel)
+    bool desc_list;               // the descent walks the descent lists (every member's previous step ran its rules
+                                  // kernel in this batch), else every listed tile with its own check
     int grid[kShapes];            // blocks of the tile kernels per shape: at most the batch's grid, at most the tiles
                                   // of the members with that shape (a list holds each tile at most once)
 };
