@@ -1202,10 +1202,11 @@ __device__ void gather_lds(const CaParams& P, const uint8_t* type, const RulesTi
                                            : -100;
     const bool diag = P.nb.diagonal_falls != 0;
     // k >= 18: dz > 0 (k == 22 directly above, a free fall); else dz == 0
+    // (bitwise, not short-circuit: selects instead of exec-mask branches)
     auto accept = [&](int k, int d) {
-        const bool up = k >= 18;
-        const bool ok = d >= 0 && (up ? (cdrop == 0 && d == 1) : (cdrop == d && !(d == 1 && diag)));
-        return k == 22 || ok;
+        const bool up_ok = (cdrop == 0) & (d == 1);
+        const bool side_ok = (cdrop == d) & !((d == 1) & diag);
+        return (k == 22) | ((d >= 0) & (k >= 18 ? up_ok : side_ok));
     };
     #pragma clang loop unroll(disable)
     while (bits) {
@@ -1388,6 +1389,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             // lava mask (from the staging, in lane k) is the exec mask of its loads, and every row base is a valid
             // index, so no mask bit can address memory outside the grid
             const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
+            __builtin_assume(w >= 0 && w < kBlock / 64);
             const int lx = static_cast<int>(tid & 63u);
             const int t = static_cast<int>(tid);
             const uint32_t gxc = gx >= 0 ? static_cast<uint32_t>(gx) : 0u;
@@ -1684,13 +1686,13 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
             }
         }
         __syncthreads();
-        // a tile holding lava after the step lists the next step's tiles (lava front: itself and the neighbours it
-        // holds lava next to) or marks its neighbours for the next step's scan (any lava: visit; lava next to each
-        // neighbour: stage); a tile that changed a cell's type is listed too (it brings the other copy up to date)
         // a tile that changed a cell's type lists the tiles whose D(n) reads it for the next step's descent
         if (B.desc_list && !P.classic && (s_wflags[1] & 0x04040404u))
             push_descent<TZ>(P, D, B, o.x0 / kTileX, o.y0 / TY, o.z0 / TZ, static_cast<int>(tid), e >> kTileBits,
                              static_cast<uint32_t>(t) + 1u, par ^ 1, shape);
+        // a tile holding lava after the step lists the next step's tiles (lava front: itself and the neighbours it
+        // holds lava next to) or marks its neighbours for the next step's scan (any lava: visit; lava next to each
+        // neighbour: stage); a tile that changed a cell's type is listed too (it brings the other copy up to date)
         if ((s_wflags[1] & 0x01010101u) != 0) {
             const int tx = o.x0 / kTileX, tyy = o.y0 / TY, tz = o.z0 / TZ;
             const uint32_t bnd_all = s_red[2][0] | s_red[2][1] | s_red[2][2] | s_red[2][3];
