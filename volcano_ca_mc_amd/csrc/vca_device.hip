@@ -1004,20 +1004,29 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
         const uint32_t side = P.nb.side_mask;
         const uint32_t tile = e & kTileMask;
         const TileCoord o = tile_origin(P, tile);
-        {
-            // D(n) of the tile's lava cells reads the types of the tile, its 8 horizontal neighbours and the tiles
-            // under them down to kMaxDrop + 1 cells below the tile's lowest layer (support, side cells, their drop
-            // chains; spill levels do not change).  When none of those tiles changed a cell's type in the previous
-            // step, the values written then still hold.  Only tiles holding lava have D(n) to compute, and entries of
-            // a slot that left the batch or of a simulation that stopped are skipped.  (The barrier also orders the
-            // previous tile's LDS reads: every tile passes it.)
+        // D(n) of the tile's lava cells reads the types of the tile, its 8 horizontal neighbours and the tiles under
+        // them down to kMaxDrop + 1 cells below the tile's lowest layer (support, side cells, their drop chains; spill
+        // levels do not change).  When none of those tiles changed a cell's type in the previous step, the values
+        // written then still hold.  Only tiles holding lava have D(n) to compute, and entries of a slot that left the
+        // batch or of a simulation that stopped are skipped.
+        uint32_t stop = 0, flags = kFlagLava;
+        if (desc) {
+            // a descent-list entry needs D(n) unless its simulation stopped or the tile holds no lava (rare: the list
+            // takes lava tiles only): both flags are read together with the staging loads, which are safe for any
+            // running simulation
+            __syncthreads();  // the previous tile's LDS reads are done
+            if (!sim.running || P.classic) continue;
+            stop = D.ctrl->stop;
+            flags = D.tile_flags[tile];
+        } else {
             constexpr int NR = 9 * (kDescentLayers<TZ> + 1);
             const int tx = o.x0 / kTileX, ty = o.y0 / TY, tz = o.z0 / TZ;
-            // the stop flag, the tile's flags and the marks of the tiles below are read together
+            // the stop flag, the tile's flags and the marks of the tiles below are read together (the barrier also
+            // orders the previous tile's LDS reads: every tile passes it)
             const bool go = sim.running && !P.classic && !__builtin_amdgcn_readfirstlane(D.ctrl->stop) &&
                             (__builtin_amdgcn_readfirstlane(D.tile_flags[tile]) & kFlagLava);
-            bool dirty = desc != 0;  // a descent-list entry is one by construction
-            for (int q = static_cast<int>(threadIdx.x); !desc && sim.running && !P.classic && q < NR; q += kBlock) {
+            bool dirty = false;
+            for (int q = static_cast<int>(threadIdx.x); sim.running && !P.classic && q < NR; q += kBlock) {
                 int qx = tx + q % 3 - 1, qy = ty + (q / 3) % 3 - 1;
                 const int qz = tz - q / 9;
                 if (qz >= 0 && wrap_axis(qx, P.ntx, P.per[0]) && wrap_axis(qy, P.nty, P.per[1]))
@@ -1035,6 +1044,7 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
         uint32_t lava_lo, lava_hi;
         stage_rows<TY, TZ>(P, D, type, nullptr, false, static_cast<NextTypes<TY, TZ>*>(nullptr), o, S, has_spill,
                            s_spill, own_air, rows, gx, opaque_tid(), lava_lo, lava_hi);
+        if (__builtin_amdgcn_readfirstlane(stop) || !(__builtin_amdgcn_readfirstlane(flags) & kFlagLava)) continue;
         if (threadIdx.x == 0) s_n = 0;
         __syncthreads();
         {
