@@ -973,7 +973,9 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
     __shared__ HaloTypes<TY, TZ> S;
     __shared__ int32_t s_spill[H::LXY];
     __shared__ uint16_t s_list[kTileCells];
-    __shared__ uint32_t s_n;
+    __shared__ uint32_t s_deep[kTileCells];  // lava cells with support and AIR side neighbours: cell | mask << 16
+    __shared__ int32_t s_dd[kTileCells];     // their D(n), folded over the neighbours
+    __shared__ uint32_t s_n, s_nd;
     const uint8_t* s_type = S.type;
     if (blockIdx.x == fin_blocks && threadIdx.x < kShapes * kSub) {
         // every kernel of the previous step is done with its list copy; the rules kernel of this step pushes the next
@@ -1045,7 +1047,10 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
         stage_rows<TY, TZ>(P, D, type, nullptr, false, static_cast<NextTypes<TY, TZ>*>(nullptr), o, S, has_spill,
                            s_spill, own_air, rows, gx, opaque_tid(), lava_lo, lava_hi);
         if (__builtin_amdgcn_readfirstlane(stop) || !(__builtin_amdgcn_readfirstlane(flags) & kFlagLava)) continue;
-        if (threadIdx.x == 0) s_n = 0;
+        if (threadIdx.x == 0) {
+            s_n = 0;
+            s_nd = 0;
+        }
         __syncthreads();
         {
             // the tile's lava cells into a list (tile-local index lx + 64 * row)
@@ -1068,38 +1073,72 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
         }
         __syncthreads();
         const uint32_t nl = s_n;
-        for (uint32_t j = threadIdx.x; j < nl; j += kBlock) {
-            const int e = s_list[j];
+        // D(n) = -1 without support, else the deepest dropDepth over the AIR side neighbours (0 without one).  Cells
+        // with such neighbours go into a second list, whose (cell, neighbour) pairs are then spread over the threads
+        // (one drop chain each instead of up to eight in a row) and folded with LDS maxima.
+        auto cell_index = [&](int e) {
+            const int lx = e & 63, row = e >> 6;
+            return col_index(P, o.x0 + lx, o.y0 + row % TY) + plane * static_cast<uint32_t>(o.z0 + row / TY);
+        };
+        for (uint32_t j0 = 0; j0 < nl; j0 += kBlock) {
+            const uint32_t j = j0 + threadIdx.x;
+            uint32_t m = 0;
+            bool deep = false;
+            if (j < nl) {
+                const int e = s_list[j];
+                const int lx = e & 63, row = e >> 6;
+                const int ly = row % TY, lz = row / TY;
+                const int z = o.z0 + lz;
+                const int c = (lx + 1) + H::LX * ((ly + 1) + H::LY * (lz + 1));
+                bool sup = z == 0;  // hasSupport (LavaCA.hpp:390-399)
+                if (!sup) {
+                    const uint8_t b = s_type[c - H::LXY];
+                    sup = b == kRock || b == kBasalt || (b == kLava && has_spill && z - 1 < s_spill[(lx + 1) + H::LX * (ly + 1)]);
+                }
+                if (sup) {
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {  // the dz == 0 positions q = 9 .. 17 but 13
+                        const int q = 9 + i + (i >= 4 ? 1 : 0);
+                        const int nb = c + (q % 3 - 1) + H::LX * ((q / 3) % 3 - 1);
+                        // VOID (no cell) is not AIR
+                        if (((side >> q) & 1u) && s_type[nb] == kAir) m |= 1u << i;
+                    }
+                }
+                deep = m != 0;
+                if (!deep) D.descent[cell_index(e)] = static_cast<int8_t>(sup ? 0 : -1);
+                m = static_cast<uint32_t>(e) | (m << 16);
+            }
+            const uint32_t at = wave_append(deep, &s_nd);
+            if (deep) {
+                s_deep[at] = m;
+                s_dd[at] = 0;
+            }
+        }
+        __syncthreads();
+        const uint32_t nd = s_nd;
+        for (uint32_t p = threadIdx.x; p < 8 * nd; p += kBlock) {
+            const uint32_t v = s_deep[p >> 3];
+            const int i = static_cast<int>(p & 7u);
+            if (!((v >> (16 + i)) & 1u)) continue;
+            const int e = static_cast<int>(v & 0xFFFFu);
             const int lx = e & 63, row = e >> 6;
             const int ly = row % TY, lz = row / TY;
             const int x = o.x0 + lx, y = o.y0 + ly, z = o.z0 + lz;
             const int c = (lx + 1) + H::LX * ((ly + 1) + H::LY * (lz + 1));
             const int sc = (lx + 1) + H::LX * (ly + 1);
-            bool sup = z == 0;  // hasSupport (LavaCA.hpp:390-399)
-            if (!sup) {
-                const uint8_t b = s_type[c - H::LXY];
-                sup = b == kRock || b == kBasalt || (b == kLava && has_spill && z - 1 < s_spill[sc]);
-            }
-            int d = -1;
-            if (sup) {
-                d = 0;
-#pragma unroll
-                for (int q = 9; q < 18; ++q) {  // dz == 0 positions
-                    if (q == 13 || !((side >> q) & 1u)) continue;
-                    const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1;
-                    const int nb = c + dx + H::LX * dy;
-                    if (s_type[nb] != kAir) continue;  // VOID (no cell) is not AIR
-                    int qx = x + dx, qy = y + dy;
-                    wrap_axis(qx, P.nx, P.per[0]);
-                    wrap_axis(qy, P.ny, P.per[1]);
-                    const uint32_t qc = static_cast<uint32_t>(qx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(qy);
-                    const int dd = drop_depth_lds<TY, TZ>(type, s_type, nb, lz, z, qc, plane,
-                                                          has_spill ? s_spill[sc + dx + H::LX * dy] : 0, has_spill);
-                    d = dd > d ? dd : d;
-                }
-            }
-            D.descent[col_index(P, x, y) + plane * static_cast<uint32_t>(z)] = static_cast<int8_t>(d);
+            const int q = 9 + i + (i >= 4 ? 1 : 0);
+            const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1;
+            int qx = x + dx, qy = y + dy;
+            wrap_axis(qx, P.nx, P.per[0]);
+            wrap_axis(qy, P.ny, P.per[1]);
+            const uint32_t qc = static_cast<uint32_t>(qx) + static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(qy);
+            const int dd = drop_depth_lds<TY, TZ>(type, s_type, c + dx + H::LX * dy, lz, z, qc, plane,
+                                                  has_spill ? s_spill[sc + dx + H::LX * dy] : 0, has_spill);
+            if (dd > 0) atomicMax(&s_dd[p >> 3], dd);
         }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < nd; j += kBlock)
+            D.descent[cell_index(static_cast<int>(s_deep[j] & 0xFFFFu))] = static_cast<int8_t>(s_dd[j]);
     }
 }
 
