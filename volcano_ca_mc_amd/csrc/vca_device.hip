@@ -993,9 +993,13 @@ __global__ void __launch_bounds__(kBlock) ca_descent_kernel(BatchDev B, int shap
     const uint32_t n = desc ? *desc_count(B, par, shape) : *list_count(B, par, shape, static_cast<int>(tb % kSub));
     const uint64_t* tiles = desc ? B.desc_list + static_cast<size_t>(par * kShapes + shape) * B.subcap
                                  : sub_list(B, par, shape, static_cast<int>(tb % kSub));
-    for (uint32_t s = desc ? tb : tb / kSub; s < n; s += stride) {
+    // the block's first entry is read together with the list's length, each later one an iteration ahead
+    const uint32_t s0 = desc ? tb : tb / kSub;
+    uint64_t next = s0 < B.subcap ? tiles[s0] : 0ull;
+    for (uint32_t s = s0; s < n; s += stride) {
         // uniform: one entry per block; low word slot << kTileBits | tile, high word the simulation's step
-        const uint64_t ent = tiles[s];
+        const uint64_t ent = next;
+        if (s + stride < n) next = tiles[s + stride];
         const uint32_t e = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ent));
         const uint32_t now = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ent >> 32));
         const CaSim& sim = g_sims[B.sim_base + (e >> kTileBits)];
