@@ -299,7 +299,7 @@ struct vca_ca {
     int shape = 0;
     DevBuf arena;  // every region below: one allocation and one free per simulation
     Span type, type2, flow, temp, temp2, solid, arrival, descent, spill, vent_cell, tile_flags, tile_near, tile_halo,
-        tile_tchg, listed, dirtied, part, ctrl, hist_a, hist_c;
+        tile_tchg, listed, dirtied, tile_vent, part, ctrl, hist_a, hist_c;
     uint64_t n = 0;
     std::unique_ptr<Batch> own;   // batch of one: the solo path (made on the first solo enqueue)
     bool flow16 = true;           // every flowId the simulation can hold (its vents') fits int16_t
@@ -931,7 +931,7 @@ struct CaPlan {
     Layout L;
     size_t hcap = 1;
     size_t type = 0, type2 = 0, flow = 0, temp = 0, temp2 = 0, solid = 0, arrival = 0, ctrl = 0, tile_near = 0,
-           tile_halo = 0, tile_tchg = 0, listed = 0, dirtied = 0, part = 0, hist_a = 0, hist_c = 0;
+           tile_halo = 0, tile_tchg = 0, listed = 0, dirtied = 0, tile_vent = 0, part = 0, hist_a = 0, hist_c = 0;
     size_t zero_bytes = 0;
     size_t descent = 0, vent_cell = 0, tile_flags = 0, spill = 0;
     size_t staging = 0, st_height = 0, st_cells = 0, st_flows = 0, st_temps = 0, st_bytes = 0;
@@ -954,6 +954,7 @@ CaPlan ca_plan(uint64_t n, uint32_t ntiles, size_t cols, bool lava, size_t vent_
     c.tile_tchg = L.add(static_cast<size_t>(ntiles) * 4);
     c.listed = L.add(static_cast<size_t>(ntiles) * 4);
     c.dirtied = L.add(static_cast<size_t>(ntiles) * 4);
+    c.tile_vent = L.add(ntiles);
     c.part = L.add(static_cast<size_t>(kPartRows) * sizeof(CaPart));
     c.hist_a = L.add(c.hcap * 8);
     c.hist_c = L.add(c.hcap * 8);
@@ -1127,6 +1128,7 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         region(ca->tile_tchg, plan.tile_tchg, static_cast<size_t>(P.ntiles) * 4);
         region(ca->listed, plan.listed, static_cast<size_t>(P.ntiles) * 4);
         region(ca->dirtied, plan.dirtied, static_cast<size_t>(P.ntiles) * 4);
+        region(ca->tile_vent, plan.tile_vent, P.ntiles);
         region(ca->part, plan.part, static_cast<size_t>(kPartRows) * sizeof(CaPart));
         region(ca->hist_a, plan.hist_a, plan.hcap * 8);
         region(ca->hist_c, plan.hist_c, plan.hcap * 8);
@@ -1155,11 +1157,13 @@ extern "C" int vca_ca_create(vca_ctx* ctx, const vca_ca_desc* d, const int32_t* 
         D.tile_tchg = ca->tile_tchg.as<uint32_t>();
         D.listed = ca->listed.as<uint32_t>();
         D.dirtied = ca->dirtied.as<uint32_t>();
+        D.tile_vent = ca->tile_vent.as<uint8_t>();
         D.part = ca->part.as<CaPart>();
         D.ctrl = ca->ctrl.as<CaCtrl>();
         D.hist_active = ca->hist_a.as<unsigned long long>();
         D.hist_changed = ca->hist_c.as<unsigned long long>();
         D.spill = nullptr;
+        ca_launch_vent_tiles(P, D, s);  // after the vent table's upload, on the same stream
         char* staging = base + plan.staging;
         if (!classic) {
             D.spill = ca->spill.as<int32_t>();

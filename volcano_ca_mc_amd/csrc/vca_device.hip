@@ -472,6 +472,19 @@ __global__ void __launch_bounds__(kBlock) ca_terrain_kernel(CaParams P, CaDev D,
     }
 }
 
+// Tiles that hold the vent cell of some flowId (vent_cell[f]): the rules test a lava cell against its flow's vent only
+// there.
+__global__ void __launch_bounds__(kBlock) ca_vent_tiles_kernel(CaParams P, CaDev D) {
+    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= static_cast<uint32_t>(P.nvent_slots)) return;
+    const uint32_t i = D.vent_cell[f];
+    if (i == 0xFFFFFFFFu) return;
+    const uint32_t nx = static_cast<uint32_t>(P.nx), plane = nx * static_cast<uint32_t>(P.ny);
+    const uint32_t z = i / plane, y = (i - z * plane) / nx, x = i - z * plane - y * nx;
+    D.tile_vent[x / kTileX + static_cast<uint32_t>(P.ntx) * (y / static_cast<uint32_t>(P.ty) +
+                                                             static_cast<uint32_t>(P.nty) * (z / static_cast<uint32_t>(P.tz)))] = 1;
+}
+
 __global__ void ca_seeds_kernel(CaDev D, const uint32_t* cells, const int32_t* flows, const float* temps, int n) {
     // Sequential in list order, like placeSeeds (LavaCA.hpp:215-227): a later seed in the same cell wins.
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
@@ -1408,6 +1421,8 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         const int gx = halo_x(P, o.x0 + static_cast<int>(tid & 63u));
         // the tile changed a cell's type in the previous step: the other type copy is behind on it
         const bool changed = t > 0 && __builtin_amdgcn_readfirstlane(D.tile_tchg[tile]) == static_cast<uint32_t>(t);
+        // a lava cell of the tile may be its flow's vent (the eruption rule's test needs the vent table)
+        const bool vent_tile = erupting && __builtin_amdgcn_readfirstlane(D.tile_vent[tile]) != 0;
         uint32_t lava_lo, lava_hi;  // lane k: lava mask of the wave's halo row k
         const bool saw_lava = stage_rows<TY, TZ, true, true>(P, D, tyc, tyn, changed, &T.next, o, T.S, has_spill,
                                                              T.spill, own_air, rows, gx, tid, lava_lo, lava_hi);
@@ -1618,7 +1633,7 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                         const int32_t f = T.flow[c];
                         float tc = T.temp[c];
                         const float before = tc;
-                        if (erupting && f > 0 && f < P.nvent_slots && D.vent_cell[f] == i)
+                        if (vent_tile && f > 0 && f < P.nvent_slots && D.vent_cell[f] == i)
                             tc = P.erupt_temp;
                         else
                             tc = tc - P.cool_amb;
@@ -2642,6 +2657,10 @@ inline unsigned blocks_for(uint64_t items, unsigned cap) {
 void ca_launch_terrain(const CaParams& P, const CaDev& D, const int32_t* height, hipStream_t s) {
     const uint64_t n = static_cast<uint64_t>(P.nx) * P.ny * P.nz;
     hipLaunchKernelGGL(ca_terrain_kernel, dim3(grid_for(n, 16384)), dim3(kBlock), 0, s, P, D, height);
+}
+
+void ca_launch_vent_tiles(const CaParams& P, const CaDev& D, hipStream_t s) {
+    hipLaunchKernelGGL(ca_vent_tiles_kernel, dim3(grid_for(static_cast<uint64_t>(P.nvent_slots))), dim3(kBlock), 0, s, P, D);
 }
 
 void ca_launch_seeds(const CaParams&, const CaDev& D, const uint32_t* cells, const int32_t* flows, const float* temps,

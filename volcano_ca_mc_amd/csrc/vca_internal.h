@@ -153,6 +153,8 @@ struct CaDev {
     uint32_t* listed;             // per tile: the step whose tile list holds the tile (lava front: the rules kernel
                                   // pushes the next step's tiles; the stamp lists each tile once)
     uint32_t* dirtied;            // per tile: the step whose descent list holds the tile (stamp, as `listed`)
+    uint8_t* tile_vent;           // per tile: 1 when one of its cells is the vent cell of some flowId (the only tiles
+                                  // whose lava cells the eruption rule has to test)
     CaPart* part;                 // kPartRows rows
     CaCtrl* ctrl;
     unsigned long long* hist_active;
@@ -259,6 +261,7 @@ struct McListSrc {
 
 // ---- launchers (vca_device.hip) ----
 void ca_launch_terrain(const CaParams& P, const CaDev& D, const int32_t* height, hipStream_t s);
+void ca_launch_vent_tiles(const CaParams& P, const CaDev& D, hipStream_t s);
 void ca_launch_seeds(const CaParams& P, const CaDev& D, const uint32_t* cells, const int32_t* flows,
                      const float* temps, int n, hipStream_t s);
 void ca_launch_init_flags(const CaParams& P, const CaDev& D, hipStream_t s);
