@@ -512,6 +512,7 @@ __global__ void ca_init_stop_kernel(CaParams P, CaDev D) {
     else if (P.max_steps <= 0)
         stop = 4;  // MaxSteps
     c->stop = stop;
+    c->prefix = hash_prefix(P.seed, 0);
 }
 
 // Tile flags and the initial LAVA / AIR counts (countInitialState, LavaCA.hpp:1252-1280).
@@ -669,7 +670,7 @@ __global__ void __launch_bounds__(kBlock) ca_front_kernel(BatchDev B, uint32_t s
 // finishStep (LavaCA.hpp:1206-1229) + frontCounts (LavaCA.hpp:865-871) of the simulation in slot `slot`: sums and
 // clears its counter rows, writes the histories, advances the step count and applies the stop rule.  Every thread
 // of the block calls (block-uniform slot).
-__device__ void finish_slot(const BatchDev& B, uint32_t slot) {
+__device__ __forceinline__ void finish_slot(const BatchDev& B, uint32_t slot) {
     __shared__ unsigned long long red[8][kBlock / 64];
     const CaSim& sim = g_sims[B.sim_base + slot];
     if (!sim.running) return;
@@ -677,6 +678,9 @@ __device__ void finish_slot(const BatchDev& B, uint32_t slot) {
     const CaDev& D = sim.D;
     CaCtrl* c = D.ctrl;
     if (__builtin_amdgcn_readfirstlane(c->stop)) return;
+    // the step and the next step's hash prefix as wave-uniform (scalar) values
+    const int32_t t = __builtin_amdgcn_readfirstlane(c->steps);
+    const uint64_t next_prefix = hash_prefix(P.seed, static_cast<uint64_t>(t) + 1u);
     unsigned long long v[8] = {};
     for (int r = static_cast<int>(threadIdx.x); r < kPartRows; r += kBlock) {
         CaPart* row = &D.part[r];
@@ -703,12 +707,12 @@ __device__ void finish_slot(const BatchDev& B, uint32_t slot) {
             for (int q = 0; q < 8; ++q) sum[q] += red[q][k];
         const unsigned long long lava_after = sum[0], born = sum[1], changed = sum[2], tiles = sum[3],
                                  staged = sum[4], air_cand = sum[5], air_alive = sum[6], tchanged = sum[7];
-        const int32_t t = c->steps;
         const unsigned long long active = P.classic ? c->lava_total + born : lava_after;
         const unsigned long long air = c->air_total - born;
         D.hist_active[t] = active;
         D.hist_changed[t] = changed;
         c->steps = t + 1;
+        c->prefix = next_prefix;
         c->lava_total = active;
         c->air_total = air;
         int32_t stop = 0;
@@ -1386,7 +1390,9 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         const CaParams& P = sim.P;
         const CaDev& D = sim.D;
         const uint32_t tile = e & kTileMask;
-        const uint64_t prefix = hash_prefix(P.seed, static_cast<uint64_t>(t));
+        // hashPrefix(seed, t) (Rng.hpp:66-72), kept in the simulation's control block by the kernel that advanced it
+        // to step t; read with the stop flag
+        const uint64_t prefix = D.ctrl->prefix;
         const bool erupting = t < P.eruption_steps;
         const uint16_t stamp = static_cast<uint16_t>(t < 65535 ? t : 65535);
         const uint32_t plane = static_cast<uint32_t>(P.nx) * static_cast<uint32_t>(P.ny);
@@ -2180,6 +2186,7 @@ __global__ void __launch_bounds__(kBlock) ca_ff_scan_kernel(CaParams P, CaDev D,
         unsigned long long total = 0;
         for (int q = 0; q < kBlock / 64; ++q) total += s_sum[q];
         c->steps = t0 + s_stop_k + 1;
+        c->prefix = hash_prefix(P.seed, static_cast<uint64_t>(t0 + s_stop_k + 1));
         c->stop = s_reason;
         c->lava_total = s_active;
         c->changes_total += total;

@@ -106,6 +106,8 @@ struct CaCtrl {
     // the fast-forward: the step it last started from (-1: never) and the steps it completed in all
     int32_t ff_from, ff_steps;
     int32_t ff_sol;               // fast-forward: steps from its start to the first step in which a lava cell solidifies
+    uint64_t prefix;              // hashPrefix(seed, steps): the hash prefix of the next step (kept with `steps` by
+                                  // every kernel that advances it, so that the rules kernel loads it)
 };
 
 // Counters of one step spread over kPartRows rows (a tile's counts go to row tile % kPartRows: visited tiles
