@@ -189,6 +189,8 @@ NbTable make_nb(int kind, int nx, int ny, int nz) {
     for (int k = 0; k < 9; ++k)
         if (k != 4 && ((t.any_mask >> k) & 1u)) t.diagonal_falls = 1;  // dz == -1 with dx|dy != 0
     t.conn_mask = variant_masks(kind == VCA_VON_NEUMANN ? VCA_VON_NEUMANN : VCA_MOORE, nx, ny, nz)[0];
+    t.var_r32 = static_cast<uint32_t>((uint64_t{1} << 32) % static_cast<uint64_t>(t.variants));
+    t.var_inv = 1.0 / static_cast<double>(t.variants);
     return t;
 }
 

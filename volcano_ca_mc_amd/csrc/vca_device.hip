@@ -379,6 +379,20 @@ __device__ inline uint32_t mod_small(uint64_t h, uint32_t m) {
     const uint32_t a = static_cast<uint32_t>(h >> 32) % m, b = static_cast<uint32_t>(h) % m;
     return (a * r32 + b) % m;
 }
+// The same value for m in [1, 2^16) from its reciprocal inv = 1 / m (double) and r32 = 2^32 mod m (both set by the
+// host): a 32-bit quotient estimate trunc(x * inv) is exact or one too small (its relative error is below 2^-52, so
+// it is off by less than 2^-20, and a quotient's fraction is 0 or at least 1 / m), fixed by one subtraction -- three
+// multiplications instead of four integer divisions.
+__device__ inline uint32_t mod32_inv(uint32_t x, uint32_t m, double inv) {
+    const uint32_t q = static_cast<uint32_t>(static_cast<double>(x) * inv);
+    const uint32_t r = x - q * m;
+    return r >= m ? r - m : r;
+}
+__device__ inline uint32_t mod_small_inv(uint64_t h, uint32_t m, uint32_t r32, double inv) {
+    const uint32_t a = mod32_inv(static_cast<uint32_t>(h >> 32), m, inv);
+    const uint32_t b = mod32_inv(static_cast<uint32_t>(h), m, inv);
+    return mod32_inv(a * r32 + b, m, inv);  // a * r32 + b < m * m < 2^32
+}
 
 // CandidateVote (engine/src/ca/CaRules.hpp:127-183) in registers for the common case, a cell whose candidates
 // all carry one flowId: the winner is that flowId with the largest candidate temperature (most votes, and no tie
@@ -1682,8 +1696,9 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
                         const uint64_t h_index = splitmix64(prefix ^ i);
                         uint32_t mask = P.nb.mask[0];
                         if (P.nb.variants > 1)
-                            mask = nb_mask(P.nb, mod_small(splitmix64(h_index ^ kSaltNeighborhood),
-                                                           static_cast<uint32_t>(P.nb.variants)));
+                            mask = nb_mask(P.nb, mod_small_inv(splitmix64(h_index ^ kSaltNeighborhood),
+                                                               static_cast<uint32_t>(P.nb.variants), P.nb.var_r32,
+                                                               P.nb.var_inv));
                         const uint32_t bits = lava_bits & mask;
                         if (bits) {
                             VoteFirst v;

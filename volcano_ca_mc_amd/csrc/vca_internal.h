@@ -50,6 +50,8 @@ struct NbTable {
     uint32_t side_mask;           // union over variants of dz == 0 positions (D(n), LavaCA.hpp:102-112)
     int32_t diagonal_falls;       // some variant has dz == -1 with dx|dy != 0
     uint32_t conn_mask;           // grain connectivity: Moore, or von Neumann when the kind is von Neumann
+    uint32_t var_r32;             // 2^32 mod variants and 1 / variants: the CA rules draw a cell's variant without
+    double var_inv;               // integer division (mod_small_inv)
 };
 
 // Tile = 64 x TY x TZ cells (TY*TZ = 16): one 256-thread workgroup, 4 cells per thread.
