@@ -135,8 +135,9 @@ def _traffic(workload: str, k: str):
     return t.get(workload, {}).get(k)
 
 
-def workload_line(kind: str, a, with_cpu: bool, engine_args=()) -> dict:
-    """K timed jobs (after W warm-up jobs) of one scenario through the engine."""
+def workload_line(kind: str, a, with_cpu: bool, engine_args=(), profiled: bool = True) -> dict:
+    """K timed jobs (after W warm-up jobs) of one scenario through the engine.  profiled: the committed counter
+    summaries (profiles/*.json) describe this run's launches, so its issue fractions and traffic come from them."""
     from volcano_ca_mc_amd import workloads as W
     wl = {"caldera": W.caldera_3d, "cone": W.cone_3d, "cross": W.cross_section_2d}[kind]()
     nx, ny, nz = wl.dims
@@ -183,11 +184,12 @@ def workload_line(kind: str, a, with_cpu: bool, engine_args=()) -> dict:
     step_ach = CA_BYTES_PER_CELL * w.get("tilesVisited", 0) * tile_cells / (ca_ms / 1000.0) / 1e9
     step_kernels = [k for k in mean if k.startswith("ca_") and k != "ca_step"]
     roofline = {"bound": "hbm", "achieved": round(rules_ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(rules_ach / HBM_PEAK_GBS, 4), "traffic": _traffic(kind, "ca_rules"),
+                "frac": round(rules_ach / HBM_PEAK_GBS, 4), "traffic": _traffic(kind, "ca_rules") if profiled else None,
                 "kernel": "ca_rules", "kernel_mean_ms": round(mean["ca_rules"], 5),
                 "timed_every_nth_step": TIMING_SAMPLE, "bytes_per_unit": f"{CA_BYTES_PER_CELL} B per staged cell",
                 "staged_cells_per_launch": round(staged), "algorithmic_bytes_per_launch": round(rules_bytes),
-                **_issue(kind, "ca_rules", mean["ca_rules"]),
+                **(_issue(kind, "ca_rules", mean["ca_rules"]) if profiled
+                   else {"valu_issue_frac": None, "salu_issue_frac": None}),
                 "kernel_mean_ms_all": {k: round(mean[k], 5) for k in step_kernels},
                 "launches_per_step": round(sum(ktot[k][1] for k in step_kernels) / max(1, ktot["ca_rules"][1]), 3)}
     step_roofline = {"achieved": round(step_ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -356,7 +358,9 @@ def main() -> int:
     if not a.no_extra:
         # the same workload with every CA step launched (no quiescent fast-forward): what the step kernels alone give
         log(f"extra: {a.workload} with every step launched")
-        e = workload_line(a.workload, argparse.Namespace(steps=2, warmup=0), False, ["--no-fast-forward"])
+        # (no counter summary describes its launches: no issue fractions or measured traffic)
+        e = workload_line(a.workload, argparse.Namespace(steps=2, warmup=0), False, ["--no-fast-forward"],
+                          profiled=False)
         extra["every_step_launched"] = {k: e[k] for k in ("value", "ca_ms", "config", "roofline", "step_roofline")}
         if a.workload != "cone":
             log("extra: cone workload")
