@@ -310,6 +310,7 @@ struct vca_ca {
     int slot = -1;
     // the fast-forward where it applies (vca_ca_set_fast_forward)
     bool ff = true;
+    bool hold = false;  // CaCtrl::hold as last set (ca_set_hold)
     // a fast-forward up to the next solidification costs a few launches and a host wait: after one that covered few
     // steps, the next is not tried before step ff_hold (the hold doubles each time, and resets after a long one)
     int32_t ff_hold = 0, ff_backoff = kFfBackoff;
@@ -1298,7 +1299,8 @@ extern "C" int vca_ca_collect(vca_ca* ca, vca_ca_result* r) {
         CaCtrl c;
         check(hipMemcpyAsync(&c, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, ca->side), "read ctrl");
         check(hipStreamSynchronize(ca->side), "read ctrl");
-        if (c.stop == VCA_RUNNING) throw VcaError(VCA_ERR_STATE, "the simulation is still running");
+        if (c.stop == VCA_RUNNING || c.stop == kStopHeld)
+            throw VcaError(VCA_ERR_STATE, "the simulation is still running");
         r->steps = c.steps;
         r->stop_reason = c.stop;
         r->tiles_visited = c.tiles_visited;
@@ -1328,7 +1330,8 @@ int32_t require_stopped(vca_ca* ca, hipStream_t s) {
     CaCtrl c;
     check(hipMemcpyAsync(&c, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, s), "read ctrl");
     check(hipStreamSynchronize(s), "read ctrl");
-    if (c.stop == VCA_RUNNING) throw VcaError(VCA_ERR_STATE, "the simulation is still running");
+    if (c.stop == VCA_RUNNING || c.stop == kStopHeld)
+        throw VcaError(VCA_ERR_STATE, "the simulation is still running");
     return c.steps;
 }
 }  // namespace
@@ -1389,15 +1392,25 @@ extern "C" int vca_ca_flow_points(vca_ca* ca, const vca_vent* vent, int32_t flow
 }
 
 // Status of a simulation on the context's stream: step count, stop reason, and when the fast-forward may run.
+// A held simulation (CaCtrl::hold) reads as running.
 struct CaStatus {
     int32_t steps = 0, stop = 0, quiet = 0, qdead = 0;
+    bool held = false;
 };
 static CaStatus ca_status_read(vca_ca* ca) {
     CaCtrl c;
     check(hipMemcpyAsync(&c, ca->ctrl.p, sizeof(CaCtrl), hipMemcpyDeviceToHost, ca->ctx->stream), "read ctrl");
     check(hipStreamSynchronize(ca->ctx->stream), "ca step");
     ca->ctx->drain();
-    return CaStatus{c.steps, c.stop, c.quiet, c.qdead};
+    const bool held = c.stop == kStopHeld;
+    return CaStatus{c.steps, held ? VCA_RUNNING : c.stop, c.quiet, c.qdead, held};
+}
+// Switches the simulation's hold (CaCtrl::hold) on or off, in stream order; off releases a held simulation.
+static void ca_set_hold(vca_ca* ca, bool on) {
+    if (ca->hold == on) return;
+    ca_launch_hold(ca->ctrl.as<CaCtrl>(), on ? 1 : 0, ca->ctx->stream);
+    check(hipGetLastError(), "hold");
+    ca->hold = on;
 }
 static void ca_status_sync(vca_ca* ca, int32_t* steps, int32_t* stop) {
     const CaStatus st = ca_status_read(ca);
@@ -1447,7 +1460,10 @@ static bool ca_fast_forward(vca_ca* ca, const CaStatus& st, int64_t h, const Bat
 // Steps a simulation of the batch of one `b` (slot 0) until it stops or `max_new_steps` more steps are done: chunks
 // of device-side steps with one host check in between (steps after a stop are no-ops), and the fast-forward where it
 // applies.  After a fast-forward the chunks start short again (the next one may apply soon).
+// With the fast-forward on, a lava-front simulation runs with its hold on: the step that finds it quiet holds it, so
+// the rest of the chunk's launches are no-ops and the fast-forward starts from that step.
 static CaStatus ca_run_steps(vca_ca* ca, Batch& b, int64_t max_new_steps) {
+    ca_set_hold(ca, ca->ff && !ca->P.classic && !ca->P.dense);
     CaStatus st = ca_status_read(ca);
     const int64_t target = static_cast<int64_t>(st.steps) + std::max<int64_t>(0, max_new_steps);
     int chunk = 128;
@@ -1461,12 +1477,16 @@ static CaStatus ca_run_steps(vca_ca* ca, Batch& b, int64_t max_new_steps) {
             if (st.steps - from >= vca_ca::kFfWorthSteps) chunk = 1;
             continue;
         }
+        // (a held simulation is quiet, which admits the fast-forward: released here only should that not apply)
+        if (st.held) ca_set_hold(ca, false);
         const int n = static_cast<int>(std::min<int64_t>(target - st.steps, chunk));
         b.enqueue(n);
         check(hipGetLastError(), "ca launch");
         chunk = std::min(128, 2 * chunk);
         st = ca_status_read(ca);
     }
+    // held at the last step asked for: released, so that any later stepping goes on from it
+    if (st.held) ca_set_hold(ca, false);
     return st;
 }
 
@@ -1482,6 +1502,7 @@ extern "C" int vca_ca_enqueue(vca_ca* ca, int32_t new_steps) {
             own->attach(ca);
             ca->own = std::move(own);
         }
+        ca_set_hold(ca, false);  // no host fast-forward follows these steps
         ca->own->enqueue(new_steps);
     });
 }
@@ -1784,6 +1805,7 @@ extern "C" int vca_batch_add(vca_batch* b, vca_ca* ca, int32_t* slot) {
         require(ca->ctx == b->b.ctx, "the simulation and the batch use different contexts");
         if (ca->member) throw VcaError(VCA_ERR_STATE, "the simulation is already in a batch");
         b->b.ctx->set_device();
+        ca_set_hold(ca, false);  // batch members step without the hold
         ca->slot = b->b.attach(ca);
         ca->member = &b->b;
         // the private batch of earlier solo steps holds lists for a step the shared batch moves past: a later solo run

@@ -738,6 +738,8 @@ __device__ __forceinline__ void finish_slot(const BatchDev& B, uint32_t slot) {
             stop = 3;  // NoChange
         else if (t + 1 >= P.max_steps)
             stop = 4;  // MaxSteps
+        // quiet (below) with the host's hold on: the simulation waits for the host's fast-forward
+        if (stop == 0 && sim.push && air_cand == 0 && c->hold) stop = kStopHeld;
         c->stop = stop;
         // Quiescence (lava mode): no AIR cell had LAVA at a position the rules read in this step, so none was born;
         // LAVA only turns into BASALT and AIR stays AIR, so no AIR cell can get a candidate in any later step either:
@@ -1446,7 +1448,12 @@ __global__ void __launch_bounds__(kBlock, (kRulesWaves<TY, TZ, F>)) ca_rules_ker
         uint32_t lava_lo, lava_hi;  // lane k: lava mask of the wave's halo row k
         const bool saw_lava = stage_rows<TY, TZ, true, true>(P, D, tyc, tyn, changed, &T.next, o, T.S, has_spill,
                                                              T.spill, own_air, rows, gx, tid, lava_lo, lava_hi);
-        if (__builtin_amdgcn_readfirstlane(stop)) continue;
+        if (__builtin_amdgcn_readfirstlane(stop)) {
+            // a held simulation keeps its tiles listed for the step it is at (the fast-forward reads them there)
+            if (__builtin_amdgcn_readfirstlane(stop) == kStopHeld && tid < 64u)
+                wave_list_append(B, par ^ 1, shape, (tid & 63u) == 0u, tile, (static_cast<uint64_t>(t) << 32) | e);
+            continue;
+        }
         if (tid == 0) {
             const uint32_t zero = opaque_zero();
             s_nlist = zero;
@@ -1900,7 +1907,8 @@ __global__ void __launch_bounds__(kBlock) ca_status_kernel(BatchDev B, int32_t* 
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < B.nslots; k += gridDim.x * kBlock) {
         const CaSim& sim = g_sims[B.sim_base + k];
         out[4 * k] = sim.running ? sim.D.ctrl->steps : 0;
-        out[4 * k + 1] = sim.running ? sim.D.ctrl->stop : -1;
+        const int32_t stop = sim.running ? sim.D.ctrl->stop : -1;
+        out[4 * k + 1] = stop == kStopHeld ? 0 : stop;
         out[4 * k + 2] = sim.running ? sim.D.ctrl->quiet : 0;
         out[4 * k + 3] = sim.running ? sim.D.ctrl->qdead : 0;
     }
@@ -2688,6 +2696,16 @@ void ca_launch_vent_tiles(const CaParams& P, const CaDev& D, hipStream_t s) {
 void ca_launch_seeds(const CaParams&, const CaDev& D, const uint32_t* cells, const int32_t* flows, const float* temps,
                      int n, hipStream_t s) {
     hipLaunchKernelGGL(ca_seeds_kernel, dim3(1), dim3(64), 0, s, D, cells, flows, temps, n);
+}
+
+// The host's hold switch (CaCtrl::hold); switching it off releases a held simulation (it steps on).
+__global__ void ca_hold_kernel(CaCtrl* c, int hold) {
+    if (threadIdx.x != 0) return;
+    c->hold = hold;
+    if (!hold && c->stop == kStopHeld) c->stop = 0;
+}
+void ca_launch_hold(CaCtrl* c, int hold, hipStream_t s) {
+    hipLaunchKernelGGL(ca_hold_kernel, dim3(1), dim3(64), 0, s, c, hold);
 }
 
 void ca_launch_init_stop(const CaParams& P, const CaDev& D, hipStream_t s) {

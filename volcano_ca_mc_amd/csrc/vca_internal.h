@@ -110,7 +110,13 @@ struct CaCtrl {
     int32_t ff_sol;               // fast-forward: steps from its start to the first step in which a lava cell solidifies
     uint64_t prefix;              // hashPrefix(seed, steps): the hash prefix of the next step (kept with `steps` by
                                   // every kernel that advances it, so that the rules kernel loads it)
+    // set by the host while it steps a lava-front simulation with the fast-forward on: the finish of a step after
+    // which no AIR cell has a candidate (quiet) holds the simulation (stop = kStopHeld) instead of letting the
+    // remaining launched steps run; a held simulation's steps are no-ops that keep its tiles listed, until the host's
+    // fast-forward (which the quiet condition admits) takes over from the held step
+    int32_t hold;
 };
+constexpr int32_t kStopHeld = 8;  // CaCtrl::stop of a held simulation (running for every reader outside the kernels)
 
 // Counters of one step spread over kPartRows rows (a tile's counts go to row tile % kPartRows: visited tiles
 // from ca_front, the rest from the rules kernel; ca_finish sums and clears them), so that no counter is one
@@ -269,6 +275,7 @@ void ca_launch_vent_tiles(const CaParams& P, const CaDev& D, hipStream_t s);
 void ca_launch_seeds(const CaParams& P, const CaDev& D, const uint32_t* cells, const int32_t* flows,
                      const float* temps, int n, hipStream_t s);
 void ca_launch_init_flags(const CaParams& P, const CaDev& D, hipStream_t s);
+void ca_launch_hold(CaCtrl* c, int hold, hipStream_t s);
 void ca_launch_init_stop(const CaParams& P, const CaDev& D, hipStream_t s);
 // One step of every running simulation of the batch.
 struct StepLaunch {
