@@ -1405,6 +1405,9 @@ static CaStatus ca_status_read(vca_ca* ca) {
     const bool held = c.stop == kStopHeld;
     return CaStatus{c.steps, held ? VCA_RUNNING : c.stop, c.quiet, c.qdead, held};
 }
+// Whether the simulation runs with its hold on where a host fast-forward follows its steps (lava front, the
+// fast-forward on).
+static bool ca_holds(const vca_ca* ca) { return ca->ff && !ca->P.classic && !ca->P.dense; }
 // Switches the simulation's hold (CaCtrl::hold) on or off, in stream order; off releases a held simulation.
 static void ca_set_hold(vca_ca* ca, bool on) {
     if (ca->hold == on) return;
@@ -1463,7 +1466,7 @@ static bool ca_fast_forward(vca_ca* ca, const CaStatus& st, int64_t h, const Bat
 // With the fast-forward on, a lava-front simulation runs with its hold on: the step that finds it quiet holds it, so
 // the rest of the chunk's launches are no-ops and the fast-forward starts from that step.
 static CaStatus ca_run_steps(vca_ca* ca, Batch& b, int64_t max_new_steps) {
-    ca_set_hold(ca, ca->ff && !ca->P.classic && !ca->P.dense);
+    ca_set_hold(ca, ca_holds(ca));
     CaStatus st = ca_status_read(ca);
     const int64_t target = static_cast<int64_t>(st.steps) + std::max<int64_t>(0, max_new_steps);
     int chunk = 128;
@@ -1529,6 +1532,9 @@ extern "C" int vca_ca_set_fast_forward(vca_ca* ca, int32_t on) {
     return guarded([&] {
         require(ca, "ca == NULL");
         ca->ff = on != 0;
+        // a batch member holds while the fast-forward is on; off releases it (nothing would fast-forward it)
+        ca->ctx->set_device();
+        ca_set_hold(ca, ca->member != nullptr && ca_holds(ca));
     });
 }
 
@@ -1805,7 +1811,8 @@ extern "C" int vca_batch_add(vca_batch* b, vca_ca* ca, int32_t* slot) {
         require(ca->ctx == b->b.ctx, "the simulation and the batch use different contexts");
         if (ca->member) throw VcaError(VCA_ERR_STATE, "the simulation is already in a batch");
         b->b.ctx->set_device();
-        ca_set_hold(ca, false);  // batch members step without the hold
+        // the batch's status read fast-forwards a held member (from the step that found it quiet)
+        ca_set_hold(ca, ca_holds(ca));
         ca->slot = b->b.attach(ca);
         ca->member = &b->b;
         // the private batch of earlier solo steps holds lists for a step the shared batch moves past: a later solo run
