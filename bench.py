@@ -175,7 +175,12 @@ def workload_line(kind: str, a, with_cpu: bool, engine_args=(), profiled: bool =
     tile_cells = w.get("tileCells", 1024)
     # steps the step kernels ran; the rest (a quiescent tail: lava only cooling) completed by the fast-forward
     launched = w.get("stepsLaunched", steps)
-    staged = w.get("tilesStaged", 0) * tile_cells / max(1, launched)      # cells ca_rules stages per launch
+    # ca_rules launches per job: the launches after the step that held a quiet simulation (until the host's status
+    # read) are no-ops and are timed like the others, so the staged cells are spread over every launch; one launch in
+    # TIMING_SAMPLE is timed (the first included), which fixes the count to within TIMING_SAMPLE / 2
+    rules_timed = ktot.get("ca_rules", [0.0, 0])[1] / max(1, len(runs))
+    rules_launches = max(launched, round(rules_timed * TIMING_SAMPLE - TIMING_SAMPLE / 2))
+    staged = w.get("tilesStaged", 0) * tile_cells / max(1, rules_launches)  # cells ca_rules stages per launch
     visited = w.get("tilesVisited", 0) * tile_cells / max(1, launched)    # cells of every visited tile per step
     step_ms = ca_ms / max(1, steps)
     rules_bytes = CA_BYTES_PER_CELL * staged
@@ -188,6 +193,7 @@ def workload_line(kind: str, a, with_cpu: bool, engine_args=(), profiled: bool =
                 "kernel": "ca_rules", "kernel_mean_ms": round(mean["ca_rules"], 5),
                 "timed_every_nth_step": TIMING_SAMPLE, "bytes_per_unit": f"{CA_BYTES_PER_CELL} B per staged cell",
                 "staged_cells_per_launch": round(staged), "algorithmic_bytes_per_launch": round(rules_bytes),
+                "launches": rules_launches,
                 **(_issue(kind, "ca_rules", mean["ca_rules"]) if profiled
                    else {"valu_issue_frac": None, "salu_issue_frac": None}),
                 "kernel_mean_ms_all": {k: round(mean[k], 5) for k in step_kernels},
