@@ -389,6 +389,33 @@ def test_fast_forward_from_any_step(ctx, name):
         gpu.close()
 
 
+@pytest.mark.parametrize("name", _lava_cases())
+def test_hold_with_single_steps(ctx, name):
+    """Single steps that alternate between run(1) (the hold on: a step that finds the simulation quiet holds it, and
+    the next run fast-forwards from it) and enqueue(1) (the hold off: a held simulation is released and steps on),
+    to the stop: state arrays (float32 bits), step count, stop reason and both histories identical to the oracle's
+    whole run."""
+    d, cfg = _inputs(name)
+    hm = O.terrain_for(cfg, d)
+    per = cfg["boundary"] == "periodic"
+    spill = O.fill_depressions(hm, per, per)
+    vents = O.vents_for(cfg, hm)
+    full = O.OracleCA(O.ca_params(cfg), hm, spill, vents)
+    full.run()
+    print(f"case {name}: {full.steps} steps, run(1) and enqueue(1) in turn", flush=True)
+    gpu = _gpu_ca(ctx, cfg, hm, spill, vents)
+    k = 0
+    while gpu.status()[1] == 0:
+        assert k <= 2 * full.steps + 2, "no progress"
+        if k % 2 == 0:
+            gpu.run(1)
+        else:
+            gpu.enqueue(1)
+        k += 1
+    _same_state(gpu, full, "single steps")
+    gpu.close()
+
+
 def test_fast_forward_runs_on_quiescent_tails(ctx):
     """The fast-forward is taken: runs that end in steps where no cell can be born (the cooling tails of the NoLava
     cases, erupting vents whose lava cannot flow) report fast-forwarded steps."""
